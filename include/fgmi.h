@@ -1,0 +1,195 @@
+/*
+ * fgmi.h -- C-ABI of the MI355X (gfx950) LLG time-step engine.
+ *
+ * This is the drop-in boundary for FeeLLGood's per-timestep hot path. Every entry point is
+ * extern "C", takes plain pointers and sizes, returns an int status and never exits or throws.
+ * The context owns all device memory; host arrays passed in are copied.
+ *
+ * Reference interfaces replaced (FeeLLGood, C++ classes called from the time loop):
+ *   fg_create             Mesh::mesh + LinAlgebra ctor + scal_fmm::fmm ctor
+ *                         (src/mesh.h:38-135, src/linear_algebra.h:40-64, src/fmm_demag.h:64-90)
+ *   fg_set_state          Mesh::mesh::init_distrib / readSol (src/mesh.h:221-277, src/read.cpp:199-249)
+ *   fg_get_state          node getters used by savesol (src/save.cpp:164-186)
+ *   fg_base_projection    LinAlgebra::base_projection (src/linear_algebra.h:105,
+ *                         src/linear_algebra.cpp:3-11); the random angle is passed in explicitly
+ *   fg_prepare_elements   LinAlgebra::prepareElements(const Vector3d&, const timing&)
+ *                         (src/linear_algebra.h:75-76, src/linear_algebra.cpp:26-52)
+ *   fg_solve              bool LinAlgebra::solve(const timing&) + get_v_max
+ *                         (src/linear_algebra.h:94,101-102, src/solver.cpp:6-90)
+ *   fg_calc_demag         scal_fmm::fmm::calc_demag (src/fmm_demag.h:96-101)
+ *   fg_energy             Fem::energy (src/fem.h:152, src/energy.cpp:5-68)
+ *   fg_evolution          Fem::evolution (src/fem.h:159-163)
+ *   fg_max_angle          Mesh::mesh::max_angle (src/mesh.h:295-306)
+ *   fg_average            Mesh::mesh::avg (src/mesh.cpp:89-106)
+ *
+ * Threading: one context per host thread; calls on one context are serialised on the
+ * context's HIP stream.
+ */
+#ifndef FGMI_H
+#define FGMI_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes (int return value of every entry point) ---- */
+#define FG_OK 0
+#define FG_SOLVE_FAILED 1     /* fg_solve: the Krylov solve did not converge (reference returns true) */
+#define FG_ERR_ARG (-1)       /* invalid argument */
+#define FG_ERR_DEVICE (-2)    /* HIP runtime error */
+#define FG_ERR_MESH (-3)      /* degenerate element or inconsistent mesh */
+#define FG_ERR_STATE (-4)     /* call order violated */
+#define FG_ERR_RANGE (-5)     /* size beyond a documented limit */
+
+/* ---- Krylov status, same values as algebra::algoStatus (src/algebra/iter.h:24-30) ---- */
+#define FG_ITER_UNDEFINED (-1)
+#define FG_ITER_CONVERGED 0
+#define FG_ITER_OVERFLOW 1
+#define FG_ITER_CANNOT_CONVERGE 2
+
+/* ---- which node state a call addresses (Nodes::step, src/node.h:25-29) ---- */
+#define FG_CURRENT 0
+#define FG_NEXT 1
+#define FG_BOTH 2
+
+/* ---- energy slots (ENERGY_TYPE, src/fem.h:30-36) ---- */
+#define FG_E_EXCHANGE 0
+#define FG_E_ANISOTROPY 1
+#define FG_E_DEMAG 2
+#define FG_E_ZEEMAN 3
+
+/* ---- demag methods ---- */
+#define FG_DEMAG_DIRECT 0     /* exact O(N*S) summation */
+#define FG_DEMAG_FMM 1        /* octree fast multipole method */
+
+/* ---- precision modes ---- */
+#define FG_PREC_FP64 0        /* plain FP64 everywhere */
+#define FG_PREC_MIXED 1       /* FP32 inner Krylov + FP64 refinement, FP32 near-field P2P */
+
+/* Material of a volume region (Tetra::prm, src/tetra.h:226-252). Ms <= 0: non-magnetic. */
+typedef struct fg_volume_region {
+    double alpha;      /* alpha_LLG, dimensionless */
+    double Ae;         /* exchange stiffness [J/m] */
+    double Ms;         /* saturation magnetisation [A/m] */
+    double K;          /* uniaxial anisotropy [J/m^3] */
+    double uk[3];      /* uniaxial axis (unit) */
+    double K3;         /* cubic anisotropy [J/m^3] */
+    double ex[3];      /* cubic axes (unit) */
+    double ey[3];
+    double ez[3];
+} fg_volume_region;
+
+/* Material of a surface region (Triangle::prm, src/triangle.h:69-101). */
+typedef struct fg_surface_region {
+    double Ks;                 /* surface (Neel) anisotropy [J/m^2] */
+    double uk[3];              /* surface anisotropy axis (unit) */
+    int32_t suppress_charges;  /* nonzero: the triangles of this region carry no charges */
+    int32_t reserved;
+} fg_surface_region;
+
+/*
+ * A preprocessed mesh, as Mesh::mesh holds it after its constructor: node positions in metres,
+ * nodes sorted along the longest axis, tetrahedra with positive orientation, every boundary and
+ * interface triangle present with its signed dMs (src/mesh.cpp:121-245,334-367).
+ */
+typedef struct fg_mesh {
+    int32_t n_nodes;
+    int32_t n_tets;
+    int32_t n_tris;
+    int32_t n_vol_regions;
+    int32_t n_surf_regions;
+    int32_t reserved;
+    const double *pos;          /* [3*n_nodes] x,y,z per node */
+    const int32_t *tet_nodes;   /* [4*n_tets] 0-based node indices */
+    const int32_t *tet_region;  /* [n_tets] index into vol */
+    const int32_t *tri_nodes;   /* [3*n_tris] */
+    const int32_t *tri_region;  /* [n_tris] index into surf */
+    const double *tri_dMs;      /* [n_tris] signed Ms jump, dMs*n = Ms*n_outward */
+    const fg_volume_region *vol;
+    const fg_surface_region *surf;
+} fg_mesh;
+
+/* Solver and demag options. Zero-initialise, then call fg_default_options. */
+typedef struct fg_options {
+    double tol;            /* finite_element_solver.tolerance (default 1e-6) */
+    int32_t max_iter;      /* finite_element_solver.max(iter) (default 700) */
+    int32_t demag_method;  /* FG_DEMAG_DIRECT or FG_DEMAG_FMM */
+    int32_t fmm_order;     /* expansion degree p of the FMM (default 8) */
+    int32_t fmm_leaf;      /* target number of sources per leaf (default 128; 0 = autotune) */
+    int32_t precision;     /* FG_PREC_FP64 or FG_PREC_MIXED */
+    int32_t device;        /* HIP device ordinal */
+    int32_t profile;       /* nonzero: time every kernel with HIP events */
+    int32_t reserved;
+} fg_options;
+
+/* Result of fg_solve (iteration<T> state, src/algebra/iter.h:36-179, and LinAlgebra::v_max). */
+typedef struct fg_solve_info {
+    int32_t status;        /* FG_ITER_* */
+    int32_t iterations;    /* BiCGSTAB iterations (nit) */
+    double residual;       /* last monitored residual norm */
+    double rhs_norm;       /* ||b|| */
+    double v_max;          /* gamma0 * max_i |(x_2i, x_2i+1)| [1/s] */
+    int32_t refinements;   /* FP64 refinement sweeps (mixed precision), else 0 */
+    int32_t inner_iterations; /* FP32 inner iterations (mixed precision), else 0 */
+} fg_solve_info;
+
+typedef struct fg_ctx fg_ctx;
+
+void fg_default_options(fg_options *opt);
+
+/* Build the device plan (matrix pattern, gather lists, geometry, octree) and upload the mesh. */
+int fg_create(const fg_mesh *mesh, const fg_options *opt, fg_ctx **out);
+int fg_destroy(fg_ctx *ctx);
+
+/* Copy a node state in (which = FG_CURRENT, FG_NEXT or FG_BOTH). NULL pointers write zeros.
+ * u, v: [3*n_nodes]; phi, phiv: [n_nodes]. */
+int fg_set_state(fg_ctx *ctx, int which, const double *u, const double *v, const double *phi,
+                 const double *phiv);
+/* Copy a node state out (which = FG_CURRENT or FG_NEXT). NULL pointers are skipped. */
+int fg_get_state(fg_ctx *ctx, int which, double *u, double *v, double *phi, double *phiv);
+
+/* Tangent basis of every node from u CURRENT, rotated by the angle r (the reference draws
+ * r = M_2_PI * U(0,1) from std::mt19937(rand())). */
+int fg_base_projection(fg_ctx *ctx, double r);
+/* Element pass for a uniform applied field Hext [A/m] (B/mu0); dt and the timing prefactor. */
+int fg_prepare_elements(fg_ctx *ctx, const double Hext[3], double dt, double prefactor);
+/* Assemble, build the initial guess, run BiCGSTAB, update the NEXT state. Returns FG_OK,
+ * FG_SOLVE_FAILED (status ITER_OVERFLOW / CANNOT_CONVERGE / residual above tol) or an error. */
+int fg_solve(fg_ctx *ctx, double dt, fg_solve_info *info);
+/* Demag potentials phi (from u NEXT) and phi_v (from v NEXT), written to the NEXT state. */
+int fg_calc_demag(fg_ctx *ctx);
+/* Energies of the NEXT state for a uniform applied field Hext [A/m]; E[4] in joules. */
+int fg_energy(fg_ctx *ctx, const double Hext[3], double E[4]);
+/* NEXT -> CURRENT. */
+int fg_evolution(fg_ctx *ctx);
+/* acos(min over magnetic edges of u_i . u_j) on the NEXT state. */
+int fg_max_angle(fg_ctx *ctx, double *angle);
+/* Volume average of u NEXT (which = 0) or v NEXT (which = 1) over the magnetic tetrahedra. */
+int fg_average(fg_ctx *ctx, int which, double out[3]);
+
+/* ---- introspection used by the parity tests and the benchmark ---- */
+/* Basis vectors [3*n_nodes] each. */
+int fg_get_basis(fg_ctx *ctx, double *ep, double *eq);
+/* Number of rows (2*n_nodes) and stored entries of the assembled matrix (4*(N + 2E)). */
+int fg_system_size(fg_ctx *ctx, int64_t *rows, int64_t *nnz);
+/* Materialise the assembled (unmasked) matrix in CSR with sorted columns and the masked RHS, as
+ * solver<2>::K and L_rhs hold them after assembly (src/solver.cpp:11-48). */
+int fg_get_system(fg_ctx *ctx, int64_t *rowptr, int32_t *col, double *val, double *rhs);
+/* The last solution vector x (= v/gamma0 in the tangent basis), [2*n_nodes]. */
+int fg_get_solution(fg_ctx *ctx, double *x);
+/* Device-side per-kernel timing (opt.profile != 0): name, launches and summed milliseconds of
+ * kernel k. Returns FG_ERR_RANGE past the last kernel. */
+int fg_kernel_stats(fg_ctx *ctx, int k, char *name, int name_len, int64_t *launches, double *ms);
+int fg_reset_kernel_stats(fg_ctx *ctx);
+/* Wait for all work queued on the context's stream. */
+int fg_synchronize(fg_ctx *ctx);
+/* Last error message of this context (or of fg_create when ctx is NULL). */
+const char *fg_last_error(fg_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FGMI_H */

@@ -1,0 +1,235 @@
+"""Python handle on the FP64 CPU oracle (oracle/_build/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and bench.py's CPU
+baseline leg, and there only as the checker. The product never imports it.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+from feellgood_amd import native as nv
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "_build", "liboracle.so")
+
+_d = C.POINTER(C.c_double)
+_i32 = C.POINTER(C.c_int32)
+_i64 = C.POINTER(C.c_int64)
+
+
+class SolveInfo(C.Structure):
+    _fields_ = [("status", C.c_int32), ("iterations", C.c_int32), ("residual", C.c_double),
+                ("rhs_norm", C.c_double), ("v_max", C.c_double)]
+
+
+class RunSettings(C.Structure):
+    _fields_ = [("final_time", C.c_double), ("evol_time_step", C.c_double), ("dt_min", C.c_double),
+                ("dt_max", C.c_double), ("du_max", C.c_double), ("tol", C.c_double),
+                ("max_iter", C.c_int32), ("seed", C.c_uint32), ("Bext", C.c_double * 3),
+                ("max_rows", C.c_int32), ("verbose", C.c_int32)]
+
+
+SYMBOLS = {
+    "orc_create": (C.c_void_p, [C.POINTER(nv.Mesh), C.c_double, C.c_int]),
+    "orc_destroy": (None, [C.c_void_p]),
+    "orc_last_error": (C.c_char_p, []),
+    "orc_n_edges": (C.c_int, [C.c_void_p]),
+    "orc_n_mag_tris": (C.c_int, [C.c_void_p]),
+    "orc_set_state": (None, [C.c_void_p, C.c_int, _d, _d, _d, _d]),
+    "orc_get_state": (None, [C.c_void_p, C.c_int, _d, _d, _d, _d]),
+    "orc_base_projection": (None, [C.c_void_p, C.c_double]),
+    "orc_get_basis": (None, [C.c_void_p, _d, _d]),
+    "orc_prepare_elements": (None, [C.c_void_p, _d, C.c_double, C.c_double]),
+    "orc_solve": (C.c_int, [C.c_void_p, C.c_double, C.POINTER(SolveInfo)]),
+    "orc_bicg_dir": (C.c_int, [C.c_int64, _i64, _i32, _d, _d, _d, C.c_int, _i32, C.c_double, C.c_int,
+                               C.POINTER(SolveInfo)]),
+    "orc_system_size": (None, [C.c_void_p, _i64, _i64]),
+    "orc_get_system": (None, [C.c_void_p, _i64, _i32, _d, _d]),
+    "orc_get_solution": (None, [C.c_void_p, _d]),
+    "orc_get_tet_element": (None, [C.c_void_p, C.c_int, _d, _d]),
+    "orc_calc_demag": (None, [C.c_void_p]),
+    "orc_n_sources": (C.c_int64, [C.c_void_p]),
+    "orc_get_sources": (None, [C.c_void_p, _d, _d, _d]),
+    "orc_get_corrections": (None, [C.c_void_p, _d, _d]),
+    "orc_energy": (None, [C.c_void_p, _d, _d]),
+    "orc_evolution": (None, [C.c_void_p]),
+    "orc_max_angle": (C.c_double, [C.c_void_p]),
+    "orc_average": (None, [C.c_void_p, C.c_int, _d]),
+    "orc_tet_geometry": (C.c_int, [_d, _i32, _d, _d]),
+    "orc_alpha_eff": (None, [C.c_double, C.c_double, _d, C.c_int, _d]),
+    "orc_tet_lumping": (None, [_d, _d, _d, _d, C.c_double, _d]),
+    "orc_tet_integrales": (None, [_d, _d, _d, _d, _d, _d, _d, _d, C.POINTER(nv.VolumeRegion), _d,
+                                  C.c_double, C.c_double, _d, _d]),
+    "orc_tri_potential": (C.c_double, [_d, _d, C.c_double, C.c_int]),
+    "orc_set_basis": (None, [_d, C.c_double, _d, _d]),
+    "orc_mt_uniform": (None, [C.c_uint32, C.c_int, _d]),
+    "orc_basis_angles": (None, [C.c_uint32, C.c_int, _d]),
+    "orc_run": (C.c_int, [C.POINTER(nv.Mesh), _d, C.POINTER(RunSettings), _d, _i32]),
+}
+
+_lib = None
+
+
+def build():
+    """Compile the oracle with its Makefile (g++, OpenMP)."""
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            build()
+        l = C.CDLL(LIB)
+        for name, (res, args) in SYMBOLS.items():
+            fn = getattr(l, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = l
+    return _lib
+
+
+def d(a):
+    return nv.dptr(a)
+
+
+def mt_uniform(seed, n):
+    out = np.zeros(n)
+    lib().orc_mt_uniform(seed, n, d(out))
+    return out
+
+
+def basis_angles(seed, n):
+    out = np.zeros(n)
+    lib().orc_basis_angles(seed, n, d(out))
+    return out
+
+
+def bicg_dir(rowptr, col, val, rhs, x0, mask=(), tol=1e-6, max_iter=700):
+    """algebra::bicg_dir on a CSR matrix; returns (x, SolveInfo)."""
+    rowptr = np.ascontiguousarray(rowptr, dtype=np.int64)
+    col = np.ascontiguousarray(col, dtype=np.int32)
+    val = np.ascontiguousarray(val, dtype=np.float64)
+    rhs = np.ascontiguousarray(rhs, dtype=np.float64)
+    x = np.array(x0, dtype=np.float64)
+    m = np.ascontiguousarray(mask, dtype=np.int32)
+    info = SolveInfo()
+    lib().orc_bicg_dir(len(rhs), rowptr.ctypes.data_as(_i64), col.ctypes.data_as(_i32), d(val), d(rhs),
+                       d(x), len(m), m.ctypes.data_as(_i32), tol, max_iter, C.byref(info))
+    return x, info
+
+
+class Problem:
+    """The oracle's whole-problem state, mirroring the fg_* calls of include/fgmi.h."""
+
+    def __init__(self, mesh, tol=1e-6, max_iter=700):
+        self.N = mesh.n_nodes
+        self._mesh = mesh
+        h = lib().orc_create(C.byref(mesh), tol, max_iter)
+        if not h:
+            raise RuntimeError(lib().orc_last_error().decode())
+        self._h = C.c_void_p(h)
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().orc_destroy(self._h)
+            self._h = None
+
+    def set_state(self, which, u=None, v=None, phi=None, phiv=None):
+        c = lambda a: None if a is None else np.ascontiguousarray(a, dtype=np.float64).reshape(-1)
+        u, v, phi, phiv = c(u), c(v), c(phi), c(phiv)
+        lib().orc_set_state(self._h, which, d(u), d(v), d(phi), d(phiv))
+
+    def get_state(self, which):
+        u = np.zeros(3 * self.N); v = np.zeros(3 * self.N)
+        phi = np.zeros(self.N); phiv = np.zeros(self.N)
+        lib().orc_get_state(self._h, which, d(u), d(v), d(phi), d(phiv))
+        return u.reshape(-1, 3), v.reshape(-1, 3), phi, phiv
+
+    def base_projection(self, r):
+        lib().orc_base_projection(self._h, r)
+
+    def basis(self):
+        ep = np.zeros(3 * self.N); eq = np.zeros(3 * self.N)
+        lib().orc_get_basis(self._h, d(ep), d(eq))
+        return ep.reshape(-1, 3), eq.reshape(-1, 3)
+
+    def prepare_elements(self, Hext, dt, prefactor):
+        H = np.ascontiguousarray(Hext, dtype=np.float64)
+        lib().orc_prepare_elements(self._h, d(H), dt, prefactor)
+
+    def solve(self, dt):
+        info = SolveInfo()
+        rc = lib().orc_solve(self._h, dt, C.byref(info))
+        return rc, info
+
+    def system(self):
+        rows, nnz = C.c_int64(), C.c_int64()
+        lib().orc_system_size(self._h, C.byref(rows), C.byref(nnz))
+        rp = np.zeros(rows.value + 1, np.int64); col = np.zeros(nnz.value, np.int32)
+        val = np.zeros(nnz.value); rhs = np.zeros(rows.value)
+        lib().orc_get_system(self._h, rp.ctypes.data_as(_i64), col.ctypes.data_as(_i32), d(val), d(rhs))
+        return rp, col, val, rhs
+
+    def solution(self):
+        x = np.zeros(2 * self.N)
+        lib().orc_get_solution(self._h, d(x))
+        return x
+
+    def tet_element(self, k):
+        Kp = np.zeros(64); Lp = np.zeros(8)
+        lib().orc_get_tet_element(self._h, k, d(Kp), d(Lp))
+        return Kp.reshape(8, 8), Lp
+
+    def calc_demag(self):
+        lib().orc_calc_demag(self._h)
+
+    def sources(self):
+        n = lib().orc_n_sources(self._h)
+        xyz = np.zeros(3 * n); qu = np.zeros(n); qv = np.zeros(n)
+        lib().orc_get_sources(self._h, d(xyz), d(qu), d(qv))
+        return xyz.reshape(-1, 3), qu, qv
+
+    def corrections(self):
+        cu = np.zeros(self.N); cv = np.zeros(self.N)
+        lib().orc_get_corrections(self._h, d(cu), d(cv))
+        return cu, cv
+
+    def energy(self, Hext):
+        H = np.ascontiguousarray(Hext, dtype=np.float64)
+        E = np.zeros(4)
+        lib().orc_energy(self._h, d(H), d(E))
+        return E
+
+    def evolution(self):
+        lib().orc_evolution(self._h)
+
+    def max_angle(self):
+        return lib().orc_max_angle(self._h)
+
+    def average(self, which=0):
+        out = np.zeros(3)
+        lib().orc_average(self._h, which, d(out))
+        return out
+
+    @property
+    def n_edges(self):
+        return lib().orc_n_edges(self._h)
+
+    @property
+    def n_mag_tris(self):
+        return lib().orc_n_mag_tris(self._h)
+
+
+def run(mesh, u0, final_time, evol_time_step, dt_min, dt_max, du_max, Bext, seed, tol=1e-6,
+        max_iter=700, max_rows=100000, verbose=False):
+    """The reference time loop with direct-sum demag; returns (status, evol rows)."""
+    s = RunSettings(final_time, evol_time_step, dt_min, dt_max, du_max, tol, max_iter, seed,
+                    (C.c_double * 3)(*Bext), max_rows, 1 if verbose else 0)
+    evol = np.zeros(12 * max_rows)
+    n = C.c_int32()
+    u0 = np.ascontiguousarray(u0, dtype=np.float64).reshape(-1)
+    st = lib().orc_run(C.byref(mesh), d(u0), C.byref(s), d(evol), C.byref(n))
+    return st, evol[:12 * n.value].reshape(-1, 12)

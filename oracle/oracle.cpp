@@ -1,0 +1,1259 @@
+/*
+ * oracle.cpp -- FP64 CPU restatement of FeeLLGood's per-timestep LLG path.
+ *
+ * TEST INFRASTRUCTURE ONLY (see oracle.h). Written from the reference's mathematics; each
+ * function cites the reference file:line it restates. Arithmetic follows the reference's order
+ * where that is observable (element matrices through P*AE*P^T, sequential Krylov vector ops,
+ * row-ordered SpMV), so parity with the product is a matter of rounding, not of algorithm.
+ *
+ * Parity pinning: every function here is checked by tests/test_oracle.py against known answers
+ * from the reference's unit tests (inputs regenerated with std::mt19937(5489)), closed forms and
+ * invariants, and end to end against ci-tests/full_test.py's golden values.
+ */
+#include "oracle.h"
+
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <limits>
+#include <random>
+#include <string>
+#include <utility>
+#include <vector>
+
+namespace {
+
+/* ---------------- constants (src/config.h.in:31-32,57-68) ---------------- */
+const double MU0 = 1.25663706127e-6;
+const double GAMMA0 = 1.76085962784e11 * MU0;
+const double THETA = 0.5;
+const double EPSILON_TET = 1e-40;
+
+/* Gauss tables, default build NPI = 5 / 4 (src/tetra.h:190-220, src/triangle.h:42-64) */
+const int TN = 4, TNPI = 5;
+const double TA = 1. / 4., TB = 1. / 6., TC = 1. / 2., TD = -2. / 15., TE = 3. / 40.;
+const double tu[TNPI] = {TA, TB, TB, TB, TC};
+const double tv[TNPI] = {TA, TB, TB, TC, TB};
+const double tw[TNPI] = {TA, TB, TC, TB, TB};
+const double tpds[TNPI] = {TD, TE, TE, TE, TE};
+double ta[TN][TNPI];
+
+const int FN = 3, FNPI = 4;
+const double fu[FNPI] = {1 / 3., 1 / 5., 3 / 5., 1 / 5.};
+const double fv[FNPI] = {1 / 3., 1 / 5., 1 / 5., 3 / 5.};
+const double fpds[FNPI] = {-27 / 96., 25 / 96., 25 / 96., 25 / 96.};
+double fa[FN][FNPI];
+
+struct TablesInit {
+    TablesInit()
+    {
+        for (int j = 0; j < TNPI; j++) {
+            ta[0][j] = 1. - tu[j] - tv[j] - tw[j];
+            ta[1][j] = tu[j];
+            ta[2][j] = tv[j];
+            ta[3][j] = tw[j];
+        }
+        for (int j = 0; j < FNPI; j++) {
+            fa[0][j] = 1. - fu[j] - fv[j];
+            fa[1][j] = fu[j];
+            fa[2][j] = fv[j];
+        }
+    }
+} tables_init;
+
+thread_local std::string g_err;
+
+struct V3 {
+    double x, y, z;
+};
+inline V3 v3(const double *p) { return {p[0], p[1], p[2]}; }
+inline V3 operator+(V3 a, V3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+inline V3 operator-(V3 a, V3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+inline V3 operator*(double s, V3 a) { return {s * a.x, s * a.y, s * a.z}; }
+inline double dot(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+inline V3 cross(V3 a, V3 b) { return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x}; }
+inline double norm(V3 a) { return std::sqrt(dot(a, a)); }
+inline double comp(V3 a, int k) { return k == 0 ? a.x : (k == 1 ? a.y : a.z); }
+inline V3 normalized(V3 a)
+{
+    double n2 = dot(a, a);
+    if (n2 > 0) {
+        double n = std::sqrt(n2);
+        return {a.x / n, a.y / n, a.z / n};
+    }
+    return a;
+}
+inline double sq(double x) { return x * x; }
+
+/* ---------------- node state (src/node.h:47-122) ---------------- */
+struct DataNode {
+    V3 u{0, 0, 0}, v{0, 0, 0};
+    double phi = 0, phiv = 0;
+};
+struct Node {
+    V3 p{0, 0, 0}, ep{0, 0, 0}, eq{0, 0, 0};
+    DataNode d[2];
+};
+
+/* Node::setBasis (src/node.h:73-102) */
+void set_basis(const V3 &u, double r, V3 &ep, V3 &eq)
+{
+    double au[3] = {std::fabs(u.x), std::fabs(u.y), std::fabs(u.z)};
+    int minIdx = 0;
+    for (int k = 1; k < 3; k++)
+        if (au[k] < au[minIdx]) minIdx = k;
+    V3 e{0, 0, 0};
+    if (minIdx == 0) e.x = 1; else if (minIdx == 1) e.y = 1; else e.z = 1;
+    e = e - dot(e, u) * u;
+    e = normalized(e);
+    V3 q = cross(u, e);
+    V3 new_ep = std::cos(r) * e - std::sin(r) * q;
+    eq = std::sin(r) * e + std::cos(r) * q;
+    ep = new_ep;
+}
+
+/* ---------------- tetrahedron (src/tetra.h:280-303, src/tetra.cpp:393-424) ---------------- */
+struct TetGeo {
+    int ind[4];
+    int reg;
+    double da[4][3];
+    double w[TNPI];
+};
+
+/* returns -1 when singular */
+int tet_geometry(const V3 P[4], int ind[4], double da[4][3], double w[TNPI], V3 *Pout)
+{
+    V3 p[4] = {P[0], P[1], P[2], P[3]};
+    V3 p0p1 = p[1] - p[0], p0p2 = p[2] - p[0], p0p3 = p[3] - p[0];
+    double mixed = dot(p0p1, cross(p0p2, p0p3));
+    if (std::fabs(mixed) < EPSILON_TET) return -1;
+    if (mixed < 0.0) {
+        std::swap(ind[2], ind[3]);
+        std::swap(p[2], p[3]);
+    }
+    p0p1 = p[1] - p[0];
+    p0p2 = p[2] - p[0];
+    p0p3 = p[3] - p[0];
+    double J[3][3] = {{p0p1.x, p0p2.x, p0p3.x}, {p0p1.y, p0p2.y, p0p3.y}, {p0p1.z, p0p2.z, p0p3.z}};
+    /* Matrix3d::determinant: expansion along the first row */
+    double detJ = J[0][0] * (J[1][1] * J[2][2] - J[1][2] * J[2][1]) -
+                  J[0][1] * (J[1][0] * J[2][2] - J[1][2] * J[2][0]) +
+                  J[0][2] * (J[1][0] * J[2][1] - J[1][1] * J[2][0]);
+    /* Matrix3d::inverse: adjugate times 1/det, det from the first-column cofactors */
+    auto cof = [&J](int i, int j) {
+        int i1 = (i + 1) % 3, i2 = (i + 2) % 3, j1 = (j + 1) % 3, j2 = (j + 2) % 3;
+        return J[i1][j1] * J[i2][j2] - J[i1][j2] * J[i2][j1];
+    };
+    double invdet = 1.0 / (cof(0, 0) * J[0][0] + cof(1, 0) * J[1][0] + cof(2, 0) * J[2][0]);
+    double Ji[3][3];
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) Ji[j][i] = cof(i, j) * invdet;
+    const double dadu[4][3] = {{-1., -1., -1.}, {1., 0., 0.}, {0., 1., 0.}, {0., 0., 1.}};
+    for (int i = 0; i < 4; i++)
+        for (int k = 0; k < 3; k++) {
+            double s = 0;
+            for (int l = 0; l < 3; l++) s += dadu[i][l] * Ji[l][k];
+            da[i][k] = s;
+        }
+    for (int j = 0; j < TNPI; j++) w[j] = detJ * tpds[j];
+    if (Pout)
+        for (int i = 0; i < 4; i++) Pout[i] = p[i];
+    return 0;
+}
+
+/* Tetra::calc_alpha_eff (src/tetra.cpp:47-75) */
+double alpha_eff(double dt, double alpha, double h)
+{
+    double reduced_dt = GAMMA0 * dt;
+    const double r = 0.1;
+    const double M = 2. * alpha * r / reduced_dt;
+    if (h > 0.) {
+        if (h > M) return alpha + reduced_dt / 2. * M;
+        return alpha + reduced_dt / 2. * h;
+    }
+    if (h < -M) return alpha / (1. + reduced_dt / (2. * alpha) * M);
+    return alpha / (1. - reduced_dt / (2. * alpha) * h);
+}
+
+/* Tet::lumping (src/tetra.cpp:108-131) with calcDiagBlock / calcOffDiagBlock (:133-148) */
+void tet_lumping(const double da[4][3], const double w[TNPI], const V3 u_nod[4],
+                 const double aeff[TNPI], double prefactor, double AE[12][12])
+{
+    double wsum = 0;
+    for (int j = 0; j < TNPI; j++) wsum += w[j];
+    double contrib[4], a_w[4];
+    for (int i = 0; i < 4; i++) {
+        double c = 0, m = 0;
+        for (int j = 0; j < TNPI; j++) {
+            c += ta[i][j] * (w[j] * aeff[j]);
+            m += ta[i][j] * w[j];
+        }
+        contrib[i] = c;
+        a_w[i] = m;
+    }
+    double blk[4][4];
+    for (int i = 0; i < 4; i++)
+        for (int j = 0; j < 4; j++) {
+            double s = da[i][0] * da[j][0] + da[i][1] * da[j][1] + da[i][2] * da[j][2];
+            blk[i][j] = s * (prefactor * wsum);
+        }
+    for (int i = 0; i < 4; i++) blk[i][i] += contrib[i];
+    for (int b = 0; b < 3; b++)
+        for (int i = 0; i < 4; i++)
+            for (int j = 0; j < 4; j++) AE[4 * b + i][4 * b + j] += blk[i][j];
+    for (int i = 0; i < 4; i++) {
+        double X = a_w[i] * u_nod[i].x, Y = a_w[i] * u_nod[i].y, Z = a_w[i] * u_nod[i].z;
+        AE[4 + i][8 + i] -= X;
+        AE[8 + i][4 + i] += X;
+        AE[0 + i][8 + i] += Y;
+        AE[8 + i][0 + i] -= Y;
+        AE[0 + i][4 + i] -= Z;
+        AE[4 + i][0 + i] += Z;
+    }
+}
+
+/* Tet::integrales (src/tetra.cpp:210-307) for a uniform Hext, no spin transfer, no drift. */
+void tet_integrales(const double da[4][3], const double w[TNPI], const V3 u[4], const V3 v[4],
+                    const double phi[4], const double phiv[4], const V3 ep[4], const V3 eq[4],
+                    const fg_volume_region &prm, const V3 &Hext, double dt, double prefactor,
+                    double Kp[8][8], double Lp[8])
+{
+    const double alpha = prm.alpha;
+    const double Ms = prm.Ms;
+    const double Abis = 2.0 * prm.Ae / (MU0 * Ms);
+    const double s_dt = THETA * dt * GAMMA0;
+
+    /* interpolation (src/tetra.h:323-358) */
+    V3 U[TNPI], V[TNPI], dU[3];
+    for (int g = 0; g < TNPI; g++) {
+        V3 su{0, 0, 0}, sv{0, 0, 0};
+        for (int i = 0; i < 4; i++) {
+            su = su + ta[i][g] * u[i];
+            sv = sv + ta[i][g] * v[i];
+        }
+        U[g] = su;
+        V[g] = sv;
+    }
+    for (int k = 0; k < 3; k++) {
+        V3 s{0, 0, 0};
+        for (int i = 0; i < 4; i++) s = s + da[i][k] * u[i];
+        dU[k] = s;
+    }
+    V3 Hd{0, 0, 0}, Hv{0, 0, 0};
+    for (int i = 0; i < 4; i++) {
+        V3 dai{da[i][0], da[i][1], da[i][2]};
+        Hd = Hd - phi[i] * dai;
+        Hv = Hv - phiv[i] * dai;
+    }
+    double uHeff[TNPI];
+    double exch = -Abis * (dot(dU[0], dU[0]) + dot(dU[1], dU[1]) + dot(dU[2], dU[2]));
+    V3 Han[TNPI];
+    for (int g = 0; g < TNPI; g++) {
+        uHeff[g] = exch;
+        Han[g] = {0, 0, 0};
+    }
+    if (prm.K != 0) { /* calc_aniso_uniax (src/tetra.cpp:171-181), called with s_dt/gamma0 */
+        double Kbis = 2.0 * prm.K / (MU0 * Ms);
+        V3 uk = v3(prm.uk);
+        double sdt = s_dt / GAMMA0;
+        for (int g = 0; g < TNPI; g++) {
+            Han[g] = Han[g] + (Kbis * dot(uk, U[g] + sdt * V[g])) * uk;
+            uHeff[g] += Kbis * sq(dot(U[g], uk));
+        }
+    }
+    if (prm.K3 != 0) { /* calc_aniso_cub (src/tetra.cpp:183-208) */
+        double K3bis = 2.0 * prm.K3 / (MU0 * Ms);
+        V3 ex = v3(prm.ex), ey = v3(prm.ey), ez = v3(prm.ez);
+        double sdt = s_dt / GAMMA0;
+        for (int g = 0; g < TNPI; g++) {
+            V3 uk_u{dot(ex, U[g]), dot(ey, U[g]), dot(ez, U[g])};
+            V3 uk_v{dot(ex, V[g]), dot(ey, V[g]), dot(ez, V[g])};
+            V3 uuu{uk_u.x * (1.0 - uk_u.x * uk_u.x), uk_u.y * (1.0 - uk_u.y * uk_u.y),
+                   uk_u.z * (1.0 - uk_u.z * uk_u.z)};
+            V3 tmp{uk_v.x * ex.x, uk_v.y * ex.y, uk_v.z * ex.z};
+            V3 one3{1 - 3 * uk_u.x * uk_u.x, 1 - 3 * uk_u.y * uk_u.y, 1 - 3 * uk_u.z * uk_u.z};
+            V3 t2{tmp.x * one3.x, tmp.y * one3.y, tmp.z * one3.z};
+            V3 h = uuu.x * ex + uuu.y * ey + uuu.z * ez + sdt * t2;
+            Han[g] = Han[g] + (-K3bis) * h;
+            uHeff[g] += -K3bis * dot(uk_u, uuu);
+        }
+    }
+    V3 Heff = Hd + Hext; /* Hst = 0 */
+    for (int g = 0; g < TNPI; g++) uHeff[g] += dot(U[g], Heff);
+    double aeff[TNPI];
+    for (int g = 0; g < TNPI; g++) aeff[g] = alpha_eff(dt, alpha, uHeff[g]);
+
+    double AE[12][12];
+    std::memset(AE, 0, sizeof(AE));
+    tet_lumping(da, w, u, aeff, prefactor * s_dt * Abis, AE);
+
+    /* P (src/element.h:81-96) and Perm (src/tetra.cpp:270-271) */
+    double P[8][12];
+    std::memset(P, 0, sizeof(P));
+    for (int i = 0; i < 4; i++) {
+        P[i][i] = ep[i].x;
+        P[i][4 + i] = ep[i].y;
+        P[i][8 + i] = ep[i].z;
+        P[4 + i][i] = eq[i].x;
+        P[4 + i][4 + i] = eq[i].y;
+        P[4 + i][8 + i] = eq[i].z;
+    }
+    double PAE[8][12];
+    for (int r = 0; r < 8; r++)
+        for (int c = 0; c < 12; c++) {
+            double s = 0;
+            for (int k = 0; k < 12; k++) s += P[r][k] * AE[k][c];
+            PAE[r][c] = s;
+        }
+    for (int r = 0; r < 8; r++) {
+        int rp = (r + 4) % 8; /* Perm swaps the ep and eq halves */
+        for (int c = 0; c < 8; c++) {
+            double s = 0;
+            for (int k = 0; k < 12; k++) s += PAE[r][k] * P[c][k];
+            Kp[rp][c] = s;
+        }
+    }
+
+    /* BE (src/tetra.cpp:277-303) */
+    V3 BE[4] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+    const double sdt_g = s_dt / GAMMA0;
+    for (int g = 0; g < TNPI; g++) {
+        V3 H = (Hd + Hext) + (Han[g] + sdt_g * Hv);
+        const double wg = w[g];
+        for (int i = 0; i < 4; i++) {
+            const double ai_w = wg * ta[i][g];
+            V3 ex = da[i][0] * dU[0] + da[i][1] * dU[1] + da[i][2] * dU[2];
+            BE[i] = BE[i] - (wg * Abis) * ex;
+            BE[i] = BE[i] + ai_w * H;
+        }
+    }
+    /* Lp = Perm * P * vec(BE) (src/tetra.cpp:306) */
+    for (int i = 0; i < 4; i++) {
+        Lp[4 + i] = ep[i].x * BE[i].x + ep[i].y * BE[i].y + ep[i].z * BE[i].z;
+        Lp[i] = eq[i].x * BE[i].x + eq[i].y * BE[i].y + eq[i].z * BE[i].z;
+    }
+}
+
+/* ---------------- triangle (src/triangle.h:113-127, src/triangle.cpp) ---------------- */
+struct TriGeo {
+    int ind[3];
+    int reg;
+    double surf;
+    V3 n;
+    double w[FNPI];
+    double dMs;
+};
+
+/* Tri::potential (src/triangle.cpp:87-125) */
+double tri_potential(const V3 P[3], const double s[3], double surf, double dMs, int i)
+{
+    int ii = (i + 1) % 3, iii = (i + 2) % 3;
+    V3 p1p2 = P[ii] - P[i];
+    V3 p1p3 = P[iii] - P[i];
+    auto f = [](double x) { return std::sqrt(1.0 + x * x); };
+    double b = norm(p1p2);
+    double t = dot(p1p2, p1p3) / b;
+    double _2s = 2. * surf;
+    double h = _2s / b;
+    double c = (t - b) / h;
+    double r = h * f(t / h);
+    double log_1 = std::log((c * t + h + f(c) * r) / (b * (c + f(c))));
+    double xi = b * log_1 / f(c);
+    double s1 = s[i], s2 = s[ii], s3 = s[iii];
+    double pot = xi * s1 + ((xi * (h + c * t) - b * (r - b)) * s2 + b * (r - b - c * xi) * s3) * b /
+                              (_2s * (1 + c * c));
+    return 0.5 * dMs * pot;
+}
+
+} // namespace
+
+/* ====================================================================================== */
+struct orc_problem {
+    int N = 0;
+    std::vector<Node> node;
+    std::vector<TetGeo> tet;
+    std::vector<TriGeo> tri;
+    std::vector<fg_volume_region> vol;
+    std::vector<fg_surface_region> surf;
+    std::vector<std::pair<int, int>> edges;
+    std::vector<char> magNode;
+    std::vector<int> magTet, magTri;
+    std::vector<int> lvd;
+    /* element matrices (src/element.h:61-65) */
+    std::vector<double> Kp;  /* 64 per tet */
+    std::vector<double> Lp;  /* 8 per tet */
+    std::vector<double> LpT; /* 6 per tri */
+    /* sparse matrix rows (src/algebra/sparseMat.h:59-75) */
+    std::vector<int64_t> rowptr;
+    std::vector<int32_t> col;
+    std::vector<double> val;
+    std::vector<double> L;
+    std::vector<double> X;
+    double v_max = 0;
+    double tol = 1e-6;
+    int max_iter = 700;
+    /* demag bookkeeping */
+    std::vector<double> src_xyz;
+    std::vector<double> src_qu, src_qv;
+    std::vector<double> corr_u, corr_v;
+    double totalMagVol = 0;
+
+    bool isMagTet(const TetGeo &t) const { return vol[t.reg].Ms > 0; }
+    double &K(int i, int j)
+    {
+        auto b = col.begin() + rowptr[i], e = col.begin() + rowptr[i + 1];
+        auto it = std::lower_bound(b, e, j);
+        return val[it - col.begin()];
+    }
+};
+
+extern "C" {
+
+const char *orc_last_error(void) { return g_err.c_str(); }
+
+orc_problem *orc_create(const fg_mesh *m, double tol, int max_iter)
+{
+    if (!m || m->n_nodes <= 0 || m->n_tets <= 0) {
+        g_err = "empty mesh";
+        return nullptr;
+    }
+    auto *p = new orc_problem();
+    p->N = m->n_nodes;
+    p->tol = tol;
+    p->max_iter = max_iter;
+    p->vol.assign(m->vol, m->vol + m->n_vol_regions);
+    if (m->n_surf_regions > 0) p->surf.assign(m->surf, m->surf + m->n_surf_regions);
+    p->node.resize(p->N);
+    for (int i = 0; i < p->N; i++) p->node[i].p = v3(m->pos + 3 * i);
+    p->tet.resize(m->n_tets);
+    for (int k = 0; k < m->n_tets; k++) {
+        TetGeo &t = p->tet[k];
+        for (int i = 0; i < 4; i++) t.ind[i] = m->tet_nodes[4 * k + i];
+        t.reg = m->tet_region[k];
+        V3 P[4];
+        for (int i = 0; i < 4; i++) P[i] = p->node[t.ind[i]].p;
+        if (tet_geometry(P, t.ind, t.da, t.w, nullptr) != 0) {
+            g_err = "singular tetrahedron " + std::to_string(k);
+            delete p;
+            return nullptr;
+        }
+    }
+    p->tri.resize(m->n_tris);
+    for (int k = 0; k < m->n_tris; k++) {
+        TriGeo &f = p->tri[k];
+        for (int i = 0; i < 3; i++) f.ind[i] = m->tri_nodes[3 * k + i];
+        f.reg = m->tri_region[k];
+        f.dMs = m->tri_dMs[k];
+        V3 nv = cross(p->node[f.ind[1]].p - p->node[f.ind[0]].p, p->node[f.ind[2]].p - p->node[f.ind[0]].p);
+        f.surf = 0.5 * norm(nv);
+        f.n = normalized(nv);
+        for (int j = 0; j < FNPI; j++) f.w[j] = 2.0 * f.surf * fpds[j];
+    }
+    /* edges (src/mesh.h:100-114) */
+    p->edges.reserve(6 * p->tet.size());
+    for (const TetGeo &t : p->tet)
+        for (int i = 0; i < 3; i++)
+            for (int j = i + 1; j < 4; j++) p->edges.push_back(std::minmax(t.ind[i], t.ind[j]));
+    std::sort(p->edges.begin(), p->edges.end());
+    p->edges.erase(std::unique(p->edges.begin(), p->edges.end()), p->edges.end());
+    /* magnetic sets (src/mesh.h:115-131) */
+    p->magNode.assign(p->N, 0);
+    for (size_t k = 0; k < p->tet.size(); k++)
+        if (p->isMagTet(p->tet[k])) {
+            p->magTet.push_back((int)k);
+            for (int i = 0; i < 4; i++) p->magNode[p->tet[k].ind[i]] = 1;
+            double vol = 0;
+            for (int j = 0; j < TNPI; j++) vol += p->tet[k].w[j];
+            p->totalMagVol += vol;
+        }
+    for (size_t k = 0; k < p->tri.size(); k++) {
+        const TriGeo &f = p->tri[k];
+        bool mag = p->magNode[f.ind[0]] && p->magNode[f.ind[1]] && p->magNode[f.ind[2]];
+        bool supp = (!p->surf.empty()) && p->surf[f.reg].suppress_charges;
+        if (mag && !supp) p->magTri.push_back((int)k);
+    }
+    /* lvd (src/linear_algebra.h:55-63) */
+    for (int i = 0; i < p->N; i++)
+        if (!p->magNode[i]) {
+            p->lvd.push_back(2 * i);
+            p->lvd.push_back(2 * i + 1);
+        }
+    /* matrix shape (src/solver.h:285-314) */
+    std::vector<std::vector<int>> shape(2 * p->N);
+    auto add_block = [&](int i, int j) {
+        for (int k = 0; k < 2; k++)
+            for (int l = 0; l < 2; l++) shape[2 * i + k].push_back(2 * j + l);
+    };
+    for (int i = 0; i < p->N; i++) add_block(i, i);
+    for (auto &e : p->edges)
+        if (p->magNode[e.first] && p->magNode[e.second]) {
+            add_block(e.first, e.second);
+            add_block(e.second, e.first);
+        }
+    p->rowptr.assign(2 * p->N + 1, 0);
+    for (int r = 0; r < 2 * p->N; r++) {
+        auto &s = shape[r];
+        std::sort(s.begin(), s.end());
+        s.erase(std::unique(s.begin(), s.end()), s.end());
+        p->rowptr[r + 1] = p->rowptr[r] + (int64_t)s.size();
+    }
+    p->col.resize(p->rowptr.back());
+    for (int r = 0; r < 2 * p->N; r++) std::copy(shape[r].begin(), shape[r].end(), p->col.begin() + p->rowptr[r]);
+    p->val.assign(p->col.size(), 0.0);
+    p->L.assign(2 * p->N, 0.0);
+    p->X.assign(2 * p->N, 0.0);
+    p->Kp.assign(64 * p->tet.size(), 0.0);
+    p->Lp.assign(8 * p->tet.size(), 0.0);
+    p->LpT.assign(6 * p->tri.size(), 0.0);
+    p->corr_u.assign(p->N, 0.0);
+    p->corr_v.assign(p->N, 0.0);
+    return p;
+}
+
+void orc_destroy(orc_problem *p) { delete p; }
+int orc_n_edges(const orc_problem *p) { return (int)p->edges.size(); }
+int orc_n_mag_tris(const orc_problem *p) { return (int)p->magTri.size(); }
+
+void orc_set_state(orc_problem *p, int which, const double *u, const double *v, const double *phi,
+                   const double *phiv)
+{
+    for (int s = 0; s < 2; s++) {
+        if (which != FG_BOTH && which != s) continue;
+        for (int i = 0; i < p->N; i++) {
+            DataNode &d = p->node[i].d[s];
+            d.u = u ? v3(u + 3 * i) : V3{0, 0, 0};
+            d.v = v ? v3(v + 3 * i) : V3{0, 0, 0};
+            d.phi = phi ? phi[i] : 0.0;
+            d.phiv = phiv ? phiv[i] : 0.0;
+        }
+    }
+}
+
+void orc_get_state(const orc_problem *p, int which, double *u, double *v, double *phi, double *phiv)
+{
+    int s = which == FG_NEXT ? 1 : 0;
+    for (int i = 0; i < p->N; i++) {
+        const DataNode &d = p->node[i].d[s];
+        if (u) { u[3 * i] = d.u.x; u[3 * i + 1] = d.u.y; u[3 * i + 2] = d.u.z; }
+        if (v) { v[3 * i] = d.v.x; v[3 * i + 1] = d.v.y; v[3 * i + 2] = d.v.z; }
+        if (phi) phi[i] = d.phi;
+        if (phiv) phiv[i] = d.phiv;
+    }
+}
+
+/* LinAlgebra::base_projection (src/linear_algebra.cpp:3-11) with r given; mesh::setBasis
+ * (src/mesh.h:178-182) */
+void orc_base_projection(orc_problem *p, double r)
+{
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < p->N; i++) set_basis(p->node[i].d[0].u, r, p->node[i].ep, p->node[i].eq);
+}
+
+void orc_get_basis(const orc_problem *p, double *ep, double *eq)
+{
+    for (int i = 0; i < p->N; i++) {
+        const Node &n = p->node[i];
+        ep[3 * i] = n.ep.x; ep[3 * i + 1] = n.ep.y; ep[3 * i + 2] = n.ep.z;
+        eq[3 * i] = n.eq.x; eq[3 * i + 1] = n.eq.y; eq[3 * i + 2] = n.eq.z;
+    }
+}
+
+/* LinAlgebra::prepareElements (src/linear_algebra.cpp:26-52) */
+void orc_prepare_elements(orc_problem *p, const double Hext_[3], double dt, double prefactor)
+{
+    V3 Hext = v3(Hext_);
+    const int T = (int)p->tet.size();
+#pragma omp parallel for schedule(static)
+    for (int k = 0; k < T; k++) {
+        const TetGeo &t = p->tet[k];
+        if (!p->isMagTet(t)) continue;
+        V3 u[4], v[4], ep[4], eq[4];
+        double phi[4], phiv[4];
+        for (int i = 0; i < 4; i++) {
+            const Node &n = p->node[t.ind[i]];
+            u[i] = n.d[0].u;
+            v[i] = n.d[0].v;
+            phi[i] = n.d[0].phi;
+            phiv[i] = n.d[0].phiv;
+            ep[i] = n.ep;
+            eq[i] = n.eq;
+        }
+        double Kp[8][8], Lp[8];
+        tet_integrales(t.da, t.w, u, v, phi, phiv, ep, eq, p->vol[t.reg], Hext, dt, prefactor, Kp, Lp);
+        std::memcpy(&p->Kp[64 * k], Kp, sizeof(Kp));
+        std::memcpy(&p->Lp[8 * k], Lp, sizeof(Lp));
+    }
+    /* Tri::integrales (src/triangle.cpp:6-36) for magnetic triangles with Ks != 0 */
+    for (size_t k = 0; k < p->tri.size(); k++) {
+        const TriGeo &f = p->tri[k];
+        bool mag = p->magNode[f.ind[0]] && p->magNode[f.ind[1]] && p->magNode[f.ind[2]];
+        if (!mag || p->surf.empty() || p->surf[f.reg].Ks == 0) continue;
+        const fg_surface_region &sp = p->surf[f.reg];
+        double Kbis = 2.0 * sp.Ks / f.dMs;
+        V3 uk = v3(sp.uk);
+        V3 BE[3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+        for (int g = 0; g < FNPI; g++) {
+            V3 U{0, 0, 0};
+            for (int i = 0; i < 3; i++) U = U + fa[i][g] * p->node[f.ind[i]].d[0].u;
+            double pre = f.w[g] * Kbis * dot(uk, U);
+            for (int i = 0; i < 3; i++) BE[i] = BE[i] + (pre * fa[i][g]) * uk;
+        }
+        for (int i = 0; i < 3; i++) {
+            const Node &n = p->node[f.ind[i]];
+            p->LpT[6 * k + 3 + i] = dot(n.ep, BE[i]);
+            p->LpT[6 * k + i] = dot(n.eq, BE[i]);
+        }
+    }
+}
+
+} // extern "C"
+
+namespace {
+/* algebra:: vector helpers (src/algebra/algebra.h:34-65, src/algebra/algebraCore.h:8-17),
+ * sequential like the reference */
+double vdot(const std::vector<double> &X, const std::vector<double> &Y)
+{
+    double s = 0;
+    for (size_t i = 0; i < X.size(); i++) s += X[i] * Y[i];
+    return s;
+}
+double vnorm(const std::vector<double> &X) { return std::sqrt(std::fabs(vdot(X, X))); }
+void mask(const std::vector<int> &ld, std::vector<double> &X)
+{
+    for (int i : ld) X[i] = 0;
+}
+struct CSR {
+    int64_t n;
+    const int64_t *rowptr;
+    const int32_t *col;
+    const double *val;
+};
+void spmv(const CSR &A, const std::vector<double> &X, std::vector<double> &Y)
+{
+    const int64_t R = A.n;
+#pragma omp parallel for schedule(static)
+    for (int64_t r = 0; r < R; r++) {
+        double s = 0;
+        for (int64_t k = A.rowptr[r]; k < A.rowptr[r + 1]; k++) s += A.val[k] * X[A.col[k]];
+        Y[r] = s;
+    }
+}
+
+/* iteration<double> (src/algebra/iter.h:36-179) */
+struct Iter {
+    double rhsn = 1.0, res = std::numeric_limits<double>::max(), resmax;
+    int maxiter, nit = 0, status = FG_ITER_UNDEFINED;
+    Iter(double r, int m) : resmax(r), maxiter(m) {}
+    bool converged(double nr)
+    {
+        res = std::fabs(nr);
+        if (std::isnan(res)) {
+            status = FG_ITER_CANNOT_CONVERGE;
+            return false;
+        }
+        bool cv = res <= rhsn * resmax;
+        if (cv) status = FG_ITER_CONVERGED;
+        return cv;
+    }
+    void inc()
+    {
+        nit++;
+        if (nit >= maxiter) status = FG_ITER_OVERFLOW;
+    }
+};
+
+/* algebra::bicg_dir, 5-argument overload (src/algebra/bicg.h:162-234) */
+void bicg_dir(Iter &iter, const CSR &A, std::vector<double> &x, const std::vector<double> &rhs,
+              const std::vector<int> &ld)
+{
+    double rho_1 = 0, rho_2 = 0, alpha = 0, beta = 0, omega = 0;
+    const size_t DIM = x.size();
+    std::vector<double> pv(DIM), phat(DIM), shat(DIM), r(DIM), rt(DIM), s(DIM), t(DIM), v(DIM),
+        dp(DIM), b(rhs);
+    /* build_diag_precond (src/algebra/sparseMat.h:173-183) */
+    for (size_t i = 0; i < DIM; i++) {
+        double c = 0;
+        for (int64_t k = A.rowptr[i]; k < A.rowptr[i + 1]; k++)
+            if (A.col[k] == (int)i) c = A.val[k];
+        dp[i] = 1.0 / c;
+    }
+    mask(ld, b);
+    mask(ld, dp);
+    iter.rhsn = vnorm(b);
+    r = b;
+    spmv(A, x, v);
+    for (size_t i = 0; i < DIM; i++) r[i] -= v[i];
+    mask(ld, r);
+    rt = r;
+    pv = r;
+    while (!iter.converged(vnorm(r))) {
+        rho_1 = vdot(rt, r);
+        if (iter.nit > 0) {
+            if (rho_2 == 0 || omega == 0) {
+                iter.status = FG_ITER_CANNOT_CONVERGE;
+                break;
+            }
+            beta = (rho_1 / rho_2) * (alpha / omega);
+            for (size_t i = 0; i < DIM; i++) v[i] *= omega;
+            for (size_t i = 0; i < DIM; i++) pv[i] -= v[i];
+            for (size_t i = 0; i < DIM; i++) pv[i] *= beta;
+            for (size_t i = 0; i < DIM; i++) pv[i] += r[i];
+        }
+        for (size_t i = 0; i < DIM; i++) phat[i] = dp[i] * pv[i];
+        spmv(A, phat, v);
+        mask(ld, v);
+        alpha = rho_1 / vdot(v, rt);
+        s = r;
+        for (size_t i = 0; i < DIM; i++) s[i] += (-alpha) * v[i];
+        if (iter.converged(vnorm(s))) {
+            for (size_t i = 0; i < DIM; i++) x[i] += alpha * phat[i];
+            break;
+        } else if (iter.status == FG_ITER_OVERFLOW || iter.status == FG_ITER_CANNOT_CONVERGE) {
+            break;
+        }
+        for (size_t i = 0; i < DIM; i++) shat[i] = dp[i] * s[i];
+        spmv(A, shat, t);
+        mask(ld, t);
+        omega = vdot(t, s) / vdot(t, t);
+        for (size_t i = 0; i < DIM; i++) x[i] += alpha * phat[i];
+        for (size_t i = 0; i < DIM; i++) x[i] += omega * shat[i];
+        for (size_t i = 0; i < DIM; i++) t[i] *= omega;
+        r = s;
+        for (size_t i = 0; i < DIM; i++) r[i] -= t[i];
+        rho_2 = rho_1;
+        iter.inc();
+    }
+}
+} // namespace
+
+extern "C" {
+
+/* LinAlgebra::solve (src/solver.cpp:6-90) */
+int orc_solve(orc_problem *p, double dt, orc_solve_info *info)
+{
+    std::fill(p->val.begin(), p->val.end(), 0.0);
+    /* buildMat<4> (src/solver.h:319-339): K(2*i+di, 2*j+dj) += Kp(di*4+ie, dj*4+je) */
+    for (int k : p->magTet) {
+        const TetGeo &t = p->tet[k];
+        const double *Kp = &p->Kp[64 * k];
+        for (int ie = 0; ie < 4; ie++)
+            for (int je = 0; je < 4; je++)
+                for (int di = 0; di < 2; di++)
+                    for (int dj = 0; dj < 2; dj++)
+                        p->K(2 * t.ind[ie] + di, 2 * t.ind[je] + dj) += Kp[(di * 4 + ie) * 8 + dj * 4 + je];
+    }
+    std::fill(p->L.begin(), p->L.end(), 0.0);
+    for (int k : p->magTet) {
+        const TetGeo &t = p->tet[k];
+        for (int ie = 0; ie < 4; ie++)
+            for (int di = 0; di < 2; di++) p->L[2 * t.ind[ie] + di] += p->Lp[8 * k + di * 4 + ie];
+    }
+    for (int k : p->magTri) {
+        const TriGeo &f = p->tri[k];
+        for (int ie = 0; ie < 3; ie++)
+            for (int di = 0; di < 2; di++) p->L[2 * f.ind[ie] + di] += p->LpT[6 * k + di * 3 + ie];
+    }
+    mask(p->lvd, p->L);
+    for (int i : p->lvd) p->K(i, i) = 1.0;
+    /* buildInitGuess (src/linear_algebra.cpp:13-24) */
+    std::fill(p->X.begin(), p->X.end(), 0.0);
+    for (int i = 0; i < p->N; i++)
+        if (p->magNode[i]) {
+            const Node &n = p->node[i];
+            p->X[2 * i] = dot(n.d[1].v, n.ep) / GAMMA0;
+            p->X[2 * i + 1] = dot(n.d[1].v, n.eq) / GAMMA0;
+        }
+    Iter iter(p->tol, p->max_iter);
+    CSR A{2 * (int64_t)p->N, p->rowptr.data(), p->col.data(), p->val.data()};
+    bicg_dir(iter, A, p->X, p->L, p->lvd);
+    if (info) {
+        info->status = iter.status;
+        info->iterations = iter.nit;
+        info->residual = iter.res;
+        info->rhs_norm = iter.rhsn;
+    }
+    if (iter.status == FG_ITER_OVERFLOW || iter.status == FG_ITER_CANNOT_CONVERGE || iter.res > iter.resmax)
+        return 1;
+    double v2max = 0.0;
+    for (int i = 0; i < p->N; i++)
+        if (p->magNode[i]) {
+            double vp = p->X[2 * i], vq = p->X[2 * i + 1];
+            double v2 = vp * vp + vq * vq;
+            if (v2 > v2max) v2max = v2;
+            /* mesh::updateNode -> Node::make_evol (src/mesh.h:185-186, src/node.h:116-122) */
+            Node &n = p->node[i];
+            n.d[1].v = (vp * GAMMA0) * n.ep + (vq * GAMMA0) * n.eq;
+            n.d[1].u = normalized(n.d[0].u + dt * n.d[1].v);
+        }
+    p->v_max = GAMMA0 * std::sqrt(v2max);
+    if (info) info->v_max = p->v_max;
+    return 0;
+}
+
+/* algebra::bicg_dir (5-argument overload) on a caller-supplied CSR matrix */
+int orc_bicg_dir(int64_t n, const int64_t *rowptr, const int32_t *col, const double *val,
+                 const double *rhs, double *x, int n_mask, const int32_t *mask_idx, double tol,
+                 int max_iter, orc_solve_info *info)
+{
+    CSR A{n, rowptr, col, val};
+    std::vector<double> X(x, x + n), b(rhs, rhs + n);
+    std::vector<int> ld(mask_idx, mask_idx + n_mask);
+    Iter iter(tol, max_iter);
+    bicg_dir(iter, A, X, b, ld);
+    std::copy(X.begin(), X.end(), x);
+    if (info) {
+        info->status = iter.status;
+        info->iterations = iter.nit;
+        info->residual = iter.res;
+        info->rhs_norm = iter.rhsn;
+        info->v_max = 0;
+    }
+    return iter.status;
+}
+
+void orc_system_size(const orc_problem *p, int64_t *rows, int64_t *nnz)
+{
+    *rows = 2 * p->N;
+    *nnz = (int64_t)p->col.size();
+}
+
+void orc_get_system(const orc_problem *p, int64_t *rowptr, int32_t *col, double *val, double *rhs)
+{
+    if (rowptr) std::copy(p->rowptr.begin(), p->rowptr.end(), rowptr);
+    if (col) std::copy(p->col.begin(), p->col.end(), col);
+    if (val) std::copy(p->val.begin(), p->val.end(), val);
+    if (rhs) std::copy(p->L.begin(), p->L.end(), rhs);
+}
+
+void orc_get_solution(const orc_problem *p, double *x) { std::copy(p->X.begin(), p->X.end(), x); }
+
+void orc_get_tet_element(const orc_problem *p, int k, double *Kp, double *Lp)
+{
+    if (Kp) std::memcpy(Kp, &p->Kp[64 * k], 64 * sizeof(double));
+    if (Lp) std::memcpy(Lp, &p->Lp[8 * k], 8 * sizeof(double));
+}
+
+} // extern "C"
+
+namespace {
+/* fmm::calc_charges + the exact potential (src/fmm_demag.h:155-224, src/tetra.cpp:347-359,
+ * src/triangle.cpp:45-78): phi_i = (sum_s q_s / |x_i - y_s| + corr_i) / (4 pi) */
+void demag_one(orc_problem *p, bool of_v)
+{
+    const int nt = (int)p->magTet.size(), nf = (int)p->magTri.size();
+    const size_t S = (size_t)TNPI * nt + (size_t)FNPI * nf;
+    p->src_xyz.resize(3 * S);
+    std::vector<double> &q = of_v ? p->src_qv : p->src_qu;
+    std::vector<double> &corr = of_v ? p->corr_v : p->corr_u;
+    q.assign(S, 0.0);
+    size_t ns = 0;
+    for (int k : p->magTet) { /* Tet::charges: -Ms * w_g * div(u) */
+        const TetGeo &t = p->tet[k];
+        V3 vn[4];
+        V3 P[4];
+        for (int i = 0; i < 4; i++) {
+            const Node &n = p->node[t.ind[i]];
+            vn[i] = of_v ? n.d[1].v : n.d[1].u;
+            P[i] = n.p;
+        }
+        double dud_sum = 0;
+        for (int i = 0; i < 4; i++) dud_sum += vn[i].x * t.da[i][0];
+        double sy = 0, sz = 0;
+        for (int i = 0; i < 4; i++) sy += vn[i].y * t.da[i][1];
+        for (int i = 0; i < 4; i++) sz += vn[i].z * t.da[i][2];
+        dud_sum = dud_sum + sy + sz;
+        const double Ms = p->vol[t.reg].Ms;
+        for (int g = 0; g < TNPI; g++) {
+            V3 G{0, 0, 0};
+            for (int i = 0; i < 4; i++) G = G + ta[i][g] * P[i];
+            p->src_xyz[3 * ns] = G.x;
+            p->src_xyz[3 * ns + 1] = G.y;
+            p->src_xyz[3 * ns + 2] = G.z;
+            q[ns] = -Ms * t.w[g] * dud_sum;
+            ns++;
+        }
+    }
+    std::fill(corr.begin(), corr.end(), 0.0);
+    for (int k : p->magTri) { /* Tri::charges + correctionCharges */
+        const TriGeo &f = p->tri[k];
+        V3 vn[3], P[3];
+        for (int i = 0; i < 3; i++) {
+            const Node &n = p->node[f.ind[i]];
+            vn[i] = of_v ? n.d[1].v : n.d[1].u;
+            P[i] = n.p;
+        }
+        double qg[FNPI];
+        V3 G[FNPI];
+        for (int g = 0; g < FNPI; g++) {
+            V3 U{0, 0, 0}, Gp{0, 0, 0};
+            for (int i = 0; i < 3; i++) {
+                U = U + fa[i][g] * vn[i];
+                Gp = Gp + fa[i][g] * P[i];
+            }
+            qg[g] = (f.dMs != 0.0) ? f.dMs * (f.w[g] * dot(U, f.n)) : 0.0;
+            G[g] = Gp;
+            p->src_xyz[3 * ns] = Gp.x;
+            p->src_xyz[3 * ns + 1] = Gp.y;
+            p->src_xyz[3 * ns + 2] = Gp.z;
+            q[ns] = qg[g];
+            ns++;
+        }
+        double s[3] = {dot(vn[0], f.n), dot(vn[1], f.n), dot(vn[2], f.n)};
+        for (int i = 0; i < 3; i++) {
+            for (int g = 0; g < FNPI; g++) corr[f.ind[i]] -= qg[g] / norm(P[i] - G[g]);
+            corr[f.ind[i]] += tri_potential(P, s, f.surf, f.dMs, i);
+        }
+    }
+    const int N = p->N;
+    const double *sx = p->src_xyz.data();
+    const double *qq = q.data();
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int i = 0; i < N; i++) {
+        if (!p->magNode[i]) continue;
+        const V3 x = p->node[i].p;
+        double pot = 0;
+        for (size_t s2 = 0; s2 < S; s2++) {
+            double dx = x.x - sx[3 * s2], dy = x.y - sx[3 * s2 + 1], dz = x.z - sx[3 * s2 + 2];
+            pot += qq[s2] / std::sqrt(dx * dx + dy * dy + dz * dz);
+        }
+        double val = (pot + corr[i]) / (4 * M_PI);
+        if (of_v) p->node[i].d[1].phiv = val;
+        else p->node[i].d[1].phi = val;
+    }
+}
+} // namespace
+
+extern "C" {
+
+/* fmm::calc_demag (src/fmm_demag.h:96-101) */
+void orc_calc_demag(orc_problem *p)
+{
+    demag_one(p, false);
+    demag_one(p, true);
+}
+
+int64_t orc_n_sources(const orc_problem *p) { return (int64_t)p->src_qu.size(); }
+
+void orc_get_sources(const orc_problem *p, double *xyz, double *q_u, double *q_v)
+{
+    if (xyz) std::copy(p->src_xyz.begin(), p->src_xyz.end(), xyz);
+    if (q_u) std::copy(p->src_qu.begin(), p->src_qu.end(), q_u);
+    if (q_v) std::copy(p->src_qv.begin(), p->src_qv.end(), q_v);
+}
+
+void orc_get_corrections(const orc_problem *p, double *cu, double *cv)
+{
+    if (cu) std::copy(p->corr_u.begin(), p->corr_u.end(), cu);
+    if (cv) std::copy(p->corr_v.begin(), p->corr_v.end(), cv);
+}
+
+/* Fem::energy (src/energy.cpp:5-68) for a uniform field; element energies src/tetra.cpp:309-391,
+ * src/triangle.cpp:38-43,80-85 */
+void orc_energy(orc_problem *p, const double Hext_[3], double E[4])
+{
+    V3 Hext = v3(Hext_);
+    for (int k = 0; k < 4; k++) E[k] = 0;
+    for (int k : p->magTet) {
+        const TetGeo &t = p->tet[k];
+        const fg_volume_region &prm = p->vol[t.reg];
+        V3 u[4];
+        double phi[4];
+        for (int i = 0; i < 4; i++) {
+            u[i] = p->node[t.ind[i]].d[1].u;
+            phi[i] = p->node[t.ind[i]].d[1].phi;
+        }
+        V3 U[TNPI], dU[3];
+        double ph[TNPI];
+        for (int g = 0; g < TNPI; g++) {
+            V3 s{0, 0, 0};
+            double sp = 0;
+            for (int i = 0; i < 4; i++) {
+                s = s + ta[i][g] * u[i];
+                sp += phi[i] * ta[i][g];
+            }
+            U[g] = s;
+            ph[g] = sp;
+        }
+        for (int d = 0; d < 3; d++) {
+            V3 s{0, 0, 0};
+            for (int i = 0; i < 4; i++) s = s + t.da[i][d] * u[i];
+            dU[d] = s;
+        }
+        double dens = dot(dU[0], dU[0]) + dot(dU[1], dU[1]) + dot(dU[2], dU[2]);
+        double ex = 0, dm = 0, an = 0, ze = 0;
+        for (int g = 0; g < TNPI; g++) ex += t.w[g] * dens;
+        E[FG_E_EXCHANGE] += prm.Ae * ex;
+        double divu = dU[0].x + dU[1].y + dU[2].z;
+        for (int g = 0; g < TNPI; g++) dm += t.w[g] * (divu * ph[g]);
+        E[FG_E_DEMAG] += -0.5 * MU0 * prm.Ms * dm;
+        if (prm.K != 0.0) {
+            V3 uk = v3(prm.uk);
+            for (int g = 0; g < TNPI; g++) an += t.w[g] * sq(dot(uk, U[g]));
+            E[FG_E_ANISOTROPY] += -prm.K * an;
+        }
+        if (prm.K3 != 0.0) {
+            double c = 0;
+            for (int g = 0; g < TNPI; g++) {
+                double a0 = dot(U[g], v3(prm.ex)), a1 = dot(U[g], v3(prm.ey)), a2 = dot(U[g], v3(prm.ez));
+                c += t.w[g] * (sq(a0 * a1) + sq(a1 * a2) + sq(a2 * a0));
+            }
+            E[FG_E_ANISOTROPY] += prm.K3 * c;
+        }
+        for (int g = 0; g < TNPI; g++) ze += t.w[g] * dot(U[g], Hext);
+        E[FG_E_ZEEMAN] += -MU0 * prm.Ms * ze;
+    }
+    for (int k : p->magTri) {
+        const TriGeo &f = p->tri[k];
+        V3 U[FNPI];
+        double ph[FNPI];
+        for (int g = 0; g < FNPI; g++) {
+            V3 s{0, 0, 0};
+            double sp = 0;
+            for (int i = 0; i < 3; i++) {
+                s = s + fa[i][g] * p->node[f.ind[i]].d[1].u;
+                sp += p->node[f.ind[i]].d[1].phi * fa[i][g];
+            }
+            U[g] = s;
+            ph[g] = sp;
+        }
+        if (!p->surf.empty() && p->surf[f.reg].Ks != 0.0) {
+            V3 uk = v3(p->surf[f.reg].uk);
+            double a = 0;
+            for (int g = 0; g < FNPI; g++) a += f.w[g] * sq(dot(U[g], uk));
+            E[FG_E_ANISOTROPY] += -p->surf[f.reg].Ks * a;
+        }
+        double d = 0;
+        for (int g = 0; g < FNPI; g++) d += (dot(U[g], f.n) * ph[g]) * f.w[g];
+        E[FG_E_DEMAG] += 0.5 * MU0 * f.dMs * d;
+    }
+}
+
+/* Fem::evolution -> Node::evolution (src/node.h:107) */
+void orc_evolution(orc_problem *p)
+{
+    for (auto &n : p->node) n.d[0] = n.d[1];
+}
+
+/* mesh::max_angle (src/mesh.h:295-306) */
+double orc_max_angle(const orc_problem *p)
+{
+    double m = 1.0;
+    for (auto &e : p->edges) {
+        double d = dot(p->node[e.first].d[1].u, p->node[e.second].d[1].u);
+        m = std::min(m, d);
+    }
+    return std::acos(m);
+}
+
+/* mesh::avg (src/mesh.cpp:89-106) on NEXT u (which 0) or v (which 1) */
+void orc_average(const orc_problem *p, int which, double out[3])
+{
+    for (int c = 0; c < 3; c++) {
+        double sum = 0;
+        for (int k : p->magTet) {
+            const TetGeo &t = p->tet[k];
+            double s = 0;
+            for (int g = 0; g < TNPI; g++) {
+                double val = 0;
+                for (int i = 0; i < 4; i++) {
+                    const DataNode &d = p->node[t.ind[i]].d[1];
+                    val += comp(which ? d.v : d.u, c) * ta[i][g];
+                }
+                s += t.w[g] * val;
+            }
+            sum += s;
+        }
+        out[c] = sum / p->totalMagVol;
+    }
+}
+
+/* ---------------- elementary entry points for the known-answer tests ---------------- */
+int orc_tet_geometry(const double *pos, int32_t *ind, double *da, double *w)
+{
+    V3 P[4];
+    for (int i = 0; i < 4; i++) P[i] = v3(pos + 3 * i);
+    int id[4] = {ind[0], ind[1], ind[2], ind[3]};
+    double d[4][3];
+    int rc = tet_geometry(P, id, d, w, nullptr);
+    if (rc) return rc;
+    for (int i = 0; i < 4; i++) {
+        ind[i] = id[i];
+        for (int k = 0; k < 3; k++) da[3 * i + k] = d[i][k];
+    }
+    return 0;
+}
+
+void orc_alpha_eff(double dt, double alpha, const double *uHeff, int n, double *out)
+{
+    for (int i = 0; i < n; i++) out[i] = alpha_eff(dt, alpha, uHeff[i]);
+}
+
+void orc_tet_lumping(const double *da_, const double *w, const double *u_nod, const double *aeff,
+                     double prefactor, double *AE_)
+{
+    double da[4][3];
+    V3 u[4];
+    for (int i = 0; i < 4; i++) {
+        for (int k = 0; k < 3; k++) da[i][k] = da_[3 * i + k];
+        u[i] = v3(u_nod + 3 * i);
+    }
+    double AE[12][12];
+    std::memcpy(AE, AE_, sizeof(AE));
+    tet_lumping(da, w, u, aeff, prefactor, AE);
+    std::memcpy(AE_, AE, sizeof(AE));
+}
+
+void orc_tet_integrales(const double *da_, const double *w, const double *u_, const double *v_,
+                        const double *phi, const double *phiv, const double *ep_, const double *eq_,
+                        const fg_volume_region *prm, const double *Hext, double dt, double prefactor,
+                        double *Kp_, double *Lp)
+{
+    double da[4][3];
+    V3 u[4], v[4], ep[4], eq[4];
+    for (int i = 0; i < 4; i++) {
+        for (int k = 0; k < 3; k++) da[i][k] = da_[3 * i + k];
+        u[i] = v3(u_ + 3 * i);
+        v[i] = v3(v_ + 3 * i);
+        ep[i] = v3(ep_ + 3 * i);
+        eq[i] = v3(eq_ + 3 * i);
+    }
+    double Kp[8][8];
+    tet_integrales(da, w, u, v, phi, phiv, ep, eq, *prm, v3(Hext), dt, prefactor, Kp, Lp);
+    std::memcpy(Kp_, Kp, sizeof(Kp));
+}
+
+double orc_tri_potential(const double *pos, const double *s, double dMs, int i)
+{
+    V3 P[3] = {v3(pos), v3(pos + 3), v3(pos + 6)};
+    double surf = 0.5 * norm(cross(P[1] - P[0], P[2] - P[0]));
+    return tri_potential(P, s, surf, dMs, i);
+}
+
+void orc_set_basis(const double *u, double r, double *ep, double *eq)
+{
+    V3 e, q;
+    set_basis(v3(u), r, e, q);
+    ep[0] = e.x; ep[1] = e.y; ep[2] = e.z;
+    eq[0] = q.x; eq[1] = q.y; eq[2] = q.z;
+}
+
+void orc_mt_uniform(uint32_t seed, int n, double *out)
+{
+    std::mt19937 gen(seed);
+    std::uniform_real_distribution<> distrib(0.0, 1.0);
+    for (int i = 0; i < n; i++) out[i] = distrib(gen);
+}
+
+void orc_basis_angles(uint32_t seed, int n, double *out)
+{
+    srand(seed);
+    for (int i = 0; i < n; i++) {
+        std::mt19937 gen(rand());
+        std::uniform_real_distribution<> distrib(0.0, 1.0);
+        out[i] = M_2_PI * distrib(gen);
+    }
+}
+
+/* ---------------- the reference time loop ---------------- */
+int orc_run(const fg_mesh *mesh, const double *u0, const orc_run_settings *s, double *evol,
+            int32_t *n_rows)
+{
+    *n_rows = 0;
+    orc_problem *p = orc_create(mesh, s->tol, s->max_iter);
+    if (!p) return -1;
+    srand(s->seed); /* src/main.cpp:211 */
+    auto next_angle = []() {
+        std::mt19937 gen(rand());
+        std::uniform_real_distribution<> distrib(0.0, 1.0);
+        return M_2_PI * distrib(gen);
+    };
+    /* Mesh::init_distrib (src/mesh.h:221-277): NEXT u, phi = phiv = 0; CURRENT u too */
+    std::vector<double> u(3 * p->N);
+    for (int i = 0; i < p->N; i++) {
+        V3 m = p->magNode[i] ? normalized(v3(u0 + 3 * i)) : V3{NAN, NAN, NAN};
+        u[3 * i] = m.x; u[3 * i + 1] = m.y; u[3 * i + 2] = m.z;
+    }
+    orc_set_state(p, FG_BOTH, u.data(), nullptr, nullptr, nullptr);
+    orc_base_projection(p, next_angle()); /* LinAlgebra ctor (src/linear_algebra.h:49) */
+    V3 Hext = (1.0 / MU0) * v3(s->Bext);  /* Settings::getField (src/settings.h:290-297) */
+    double Hx[3] = {Hext.x, Hext.y, Hext.z};
+    /* timing (src/time_integration.h:6-59) */
+    const double DTMIN = s->dt_min, DTMAX = s->dt_max, TAUR = 100. * DTMAX;
+    double t = 0, dt = 0, prefactor = 1;
+    auto set_dt = [&](double d) {
+        dt = d;
+        double tt = d / TAUR;
+        prefactor = 1. + tt * std::fabs(std::log(tt));
+    };
+    set_dt(std::sqrt(DTMIN * DTMAX));
+    double E[4];
+    double vmax = 0;
+    auto compute_all = [&](double) { /* src/fem.h:203-224 */
+        orc_calc_demag(p);
+        orc_energy(p, Hx, E);
+        orc_evolution(p);
+    };
+    compute_all(t);
+    const double t_initial = t, t_step = s->evol_time_step;
+    const int step_count = (int)std::lround((s->final_time - t_initial) / t_step);
+    /* TimeStepper (src/time_integration.cpp:11-38) */
+    const double hard_min = DTMIN, hard_max = DTMAX * (1 + FLT_EPSILON);
+    double soft_max = dt;
+    auto stepper = [&](double stride) {
+        double step = std::min(stride, soft_max);
+        if (step > stride - 2 * hard_min && step < stride) step = stride - 2 * hard_min;
+        soft_max = std::max(soft_max, std::min(step * 1.1, hard_max));
+        return step;
+    };
+    auto set_soft_limit = [&](double mx) { soft_max = std::min(soft_max, mx); };
+    int status = 0, nt = 0;
+    for (int step_nb = 0; step_nb <= step_count; step_nb++) {
+        double t_target = t_initial + step_nb * t_step;
+        while (t < t_target) {
+            set_dt(stepper(t_target - t));
+            bool last_step = (dt == t_target - t);
+            if (dt < DTMIN) {
+                status = 1;
+                goto bailout;
+            }
+            orc_base_projection(p, next_angle());
+            orc_prepare_elements(p, Hx, dt, prefactor);
+            orc_solve_info info;
+            bool err = orc_solve(p, dt, &info) != 0;
+            vmax = p->v_max;
+            if (s->verbose)
+                std::printf("t=%.6e dt=%.4e status=%d nit=%d res=%.3e\n", t, dt, info.status, info.iterations,
+                            info.residual);
+            if (err) {
+                set_soft_limit(dt / 2);
+                continue;
+            }
+            double dumax = dt * vmax;
+            set_soft_limit((s->du_max / vmax) * 0.95);
+            if (dumax > s->du_max) continue;
+            compute_all(t);
+            nt++;
+            if (last_step) t = t_target;
+            else t += dt;
+        }
+        if (*n_rows < s->max_rows) { /* Fem::saver (src/save.cpp:13-162) */
+            double *row = evol + 12 * (*n_rows);
+            double m[3];
+            orc_average(p, 0, m);
+            row[0] = t;
+            row[1] = m[0]; row[2] = m[1]; row[3] = m[2];
+            row[4] = E[FG_E_EXCHANGE]; row[5] = E[FG_E_ANISOTROPY]; row[6] = E[FG_E_DEMAG];
+            row[7] = E[FG_E_ZEEMAN]; row[8] = E[0] + E[1] + E[2] + E[3];
+            row[9] = dt; row[10] = vmax * dt; row[11] = nt;
+            (*n_rows)++;
+        }
+    }
+bailout:
+    orc_destroy(p);
+    return status;
+}
+
+} // extern "C"

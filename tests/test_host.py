@@ -1,0 +1,136 @@
+"""Host side (CPU): gmsh reading, preprocessing (orientation, controlTriangles, sortNodes),
+generators and .sol files."""
+import os
+
+import numpy as np
+import pytest
+
+from feellgood_amd.mesh import HostMesh
+
+
+def test_read_ellipsoid_counts(golden_dir):
+    """ut_readMesh.cpp:18-75: examples/ellipsoid.msh holds 167 nodes, 499 tets, 274 triangles."""
+    m = HostMesh.read(os.path.join(golden_dir, "ellipsoid.msh"))
+    assert m.raw_counts() == (167, 499, 274)
+    # naming the surface region reads its 274 triangles; none is then created
+    m.prepare({"ellipsoid_volume": {}}, {"ellipsoid_surface": {}})
+    assert (m.n_nodes, m.n_tets, m.n_tris) == (167, 499, 274)
+    assert set(m.arrays()["tri_region"]) == {0}
+    # leaving it unnamed: controlTriangles creates the 274 boundary triangles itself
+    m2 = HostMesh.read(os.path.join(golden_dir, "ellipsoid.msh"))
+    m2.prepare({"ellipsoid_volume": {}})
+    assert m2.n_tris == 274 and set(m2.arrays()["tri_region"]) == {0}  # the created region, index n_surf
+
+
+def _outward_ok(m):
+    a = m.arrays()
+    pos, tets, tris, dMs = a["pos"], a["tets"], a["tris"], a["dMs"]
+    # positive orientation of every tetrahedron
+    P = pos[tets]
+    mixed = np.einsum("ij,ij->i", P[:, 1] - P[:, 0], np.cross(P[:, 2] - P[:, 0], P[:, 3] - P[:, 0]))
+    assert (mixed > 0).all()
+    # dMs * n points out of the magnet: closed surface => sum dMs * area * n = 0, and the
+    # centroid of the body lies on the inner side of every face
+    Q = pos[tris]
+    nrm = np.cross(Q[:, 1] - Q[:, 0], Q[:, 2] - Q[:, 0])
+    c = pos.mean(axis=0)
+    side = np.einsum("ij,ij->i", (Q.mean(axis=1) - c), nrm) * dMs
+    return side, (dMs[:, None] * nrm).sum(axis=0), np.abs(dMs[:, None] * nrm).sum()
+
+
+def test_cuboid_control_triangles_and_sorting():
+    m = HostMesh.cuboid((30.0, 20.0, 10.0), (6, 4, 2), jitter=0.1, seed=11)
+    assert m.raw_counts()[1] == 6 * 48
+    m.prepare({"volume": dict(Ms=8e5)})
+    nx, ny, nz = 6, 4, 2
+    assert m.n_nodes == (nx + 1) * (ny + 1) * (nz + 1)
+    assert m.n_tris == 4 * (nx * ny + ny * nz + nx * nz)  # 2 triangles per boundary quad
+    side, closure, scale = _outward_ok(m)
+    assert (side > 0).all()
+    assert np.abs(closure).max() < 1e-12 * scale
+    assert np.all(np.abs(m.arrays()["dMs"]) == 8e5)
+    pos = m.arrays()["pos"]
+    assert np.all(np.diff(pos[:, 0]) >= 0)  # sorted along the longest axis (x)
+    l, c = m.bbox()
+    assert np.allclose(l, [30e-9, 20e-9, 10e-9]) and np.allclose(c, [15e-9, 10e-9, 5e-9])
+    ni = m.node_index()
+    assert sorted(ni) == list(range(m.n_nodes))
+
+
+def test_interface_triangles_signed_dMs(tmp_path):
+    """Two volume regions: the interface gets one triangle per face with dMs = Ms1 - Ms2 in the
+    triangle's orientation (src/mesh.cpp:227-242)."""
+    path = tmp_path / "two.msh"
+    HostMesh.cuboid((20.0, 10.0, 10.0), (4, 2, 2)).write_msh(path)
+    # relabel the tets of the right half (centroid x > 10) to a second volume region
+    txt = path.read_text().splitlines()
+    nodes = {}
+    lines = txt
+    i_nodes = lines.index("$Nodes")
+    nn = int(lines[i_nodes + 1])
+    for k in range(nn):
+        parts = lines[i_nodes + 2 + k].split()
+        nodes[int(parts[0])] = float(parts[1])
+    i_el = lines.index("$Elements")
+    ne = int(lines[i_el + 1])
+    for k in range(ne):
+        parts = lines[i_el + 2 + k].split()
+        if parts[1] == "4":
+            xs = [nodes[int(v)] for v in parts[5:9]]
+            if sum(xs) / 4 > 10:
+                parts[3] = "301"
+        lines[i_el + 2 + k] = " ".join(parts)
+    i_pn = lines.index("$PhysicalNames")
+    lines[i_pn + 1] = str(int(lines[i_pn + 1]) + 1)
+    lines.insert(i_pn + 2, '3 301 "right"')
+    path.write_text("\n".join(lines) + "\n")
+    m = HostMesh.read(path)
+    m.prepare({"volume": dict(Ms=8e5), "right": dict(Ms=5e5)})
+    arr = m.arrays()
+    pos, tris, dMs = arr["pos"], arr["tris"], arr["dMs"]
+    xs = pos[tris][:, :, 0]
+    iface = np.all(np.abs(xs - 10e-9) < 1e-15, axis=1)
+    assert iface.sum() == 8
+    nrm = np.cross(pos[tris[:, 1]] - pos[tris[:, 0]], pos[tris[:, 2]] - pos[tris[:, 0]])
+    # dMs * n_x on the interface is (8e5 - 5e5) pointing from the left (Ms 8e5) into the right
+    assert np.allclose(np.sign(nrm[iface, 0]) * dMs[iface], 3e5)
+
+
+@pytest.mark.parametrize("k,msg", [(1, "instances of the same surface triangle"),
+                                   (2, "belongs to no tetrahedron"),
+                                   (3, "shared by 3 tetrahedra"),
+                                   (4, "internal triangle")])
+def test_control_triangles_errors(golden_dir, k, msg):
+    """ut_readMesh.cpp:77-127 with unit-tests/meshes/bad_cuboid_{1..4}.msh."""
+    m = HostMesh.read(os.path.join(golden_dir, f"bad_cuboid_{k}.msh"))
+    with pytest.raises(RuntimeError, match=msg):
+        m.prepare({"whole_volume": {}}, {"whole_surface": {}})
+
+
+def test_missing_region_is_an_error(golden_dir):
+    m = HostMesh.read(os.path.join(golden_dir, "ellipsoid.msh"))
+    with pytest.raises(RuntimeError, match="not found"):
+        m.prepare({"no_such_volume": {}})
+
+
+def test_sol_round_trip(tmp_path):
+    m = HostMesh.cuboid((10.0, 10.0, 30.0), (2, 2, 6), jitter=0.1, seed=3)
+    m.prepare({"volume": {}})
+    rng = np.random.default_rng(0)
+    u = rng.normal(size=(m.n_nodes, 3))
+    u /= np.linalg.norm(u, axis=1, keepdims=True)
+    phi = rng.normal(size=m.n_nodes)
+    m.write_sol(tmp_path / "a.sol", u, phi, t=1.5e-12, precision=15)
+    u2, phi2, t = m.read_sol(tmp_path / "a.sol")
+    assert t == 1.5e-12
+    assert np.allclose(u2, u, rtol=1e-14) and np.allclose(phi2, phi, rtol=1e-14)
+    # file order is the original node numbering
+    text = (tmp_path / "a.sol").read_text().splitlines()
+    assert text[0].startswith("## time:") and text[1].startswith("0\t")
+
+
+def test_ellipsoid_generator_valid():
+    m = HostMesh.ellipsoid((0, 0, 0), (30.0, 20.0, 8.0), 2.0)
+    m.prepare({"volume": dict(Ms=1e6)})
+    side, closure, scale = _outward_ok(m)
+    assert np.abs(closure).max() < 1e-12 * scale
