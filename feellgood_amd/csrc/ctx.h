@@ -1,0 +1,153 @@
+// ctx.h -- the engine context: device buffers, the host-built plan and the launchers.
+#pragma once
+
+#include <string>
+#include <vector>
+
+#include "../../include/fgmi.h"
+#include "common.h"
+
+struct FmmPlan;  // fmm.hip
+
+namespace fg {
+
+struct KernelStat {
+    std::string name;
+    int64_t launches = 0;
+    double ms = 0;
+};
+
+struct Ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    fg_options opt{};
+    std::string err;
+
+    // sizes
+    int N = 0;         // nodes
+    int Tm = 0;        // magnetic tetrahedra
+    int Fm = 0;        // charged magnetic triangles (magTri)
+    int Fk = 0;        // triangles with surface anisotropy in the RHS
+    int64_t S = 0;     // demag sources = 5 Tm + 4 Fm
+    int nslices = 0;   // SELL slices of the stiffness stencil
+    int64_t nnzS = 0;  // stored stencil entries (with padding)
+    int64_t nnz_real = 0;  // stencil entries without padding = N + 2 E_mag
+    int n_regions = 0;
+    double totalMagVol = 0;
+
+    // host copies kept for introspection
+    std::vector<int32_t> h_mag;          // [N]
+    std::vector<int64_t> h_rowptr;       // stencil CSR (unpadded), for fg_get_system
+    std::vector<int32_t> h_col;
+    std::vector<double> h_val;
+
+    // node data
+    NodeRec *rec[2] = {nullptr, nullptr};  // CURRENT = rec[cur], NEXT = rec[1-cur]
+    int cur = 0;
+    double4 *pos4 = nullptr;  // node positions
+    double4 *ep4 = nullptr, *eq4 = nullptr;
+    int32_t *mag = nullptr;
+    double *mass = nullptr;   // lumped mass m_i = sum_t sum_g a_ig w_g
+    double *sdiag = nullptr;  // S'_ii
+    double4 *Dg = nullptr;    // per-node 2x2 block without the stencil: D_i B(i,i) + m_i X(i)
+    double2 *dprec = nullptr; // Jacobi preconditioner, masked
+    // Krylov vectors (2 unknowns per node)
+    double2 *b = nullptr, *x = nullptr, *r = nullptr, *rt = nullptr, *p = nullptr, *ph = nullptr,
+            *v = nullptr, *s = nullptr, *sh = nullptr, *t = nullptr;
+    double4 *w = nullptr;     // lifted 3-D vector fed to the stencil
+    KrylovCtl *ctl = nullptr;
+    KrylovCtl *h_ctl = nullptr;  // pinned host mirror
+    double *part = nullptr;      // [8][MAX_PART] partial sums
+    double *h_part = nullptr;    // pinned host mirror of small results
+
+    // stiffness stencil S' (SELL-64)
+    int64_t *slice_ptr = nullptr;
+    int32_t *slice_w = nullptr;
+    int32_t *scol = nullptr;
+    double *sval = nullptr;
+
+    // tetrahedra (magnetic only)
+    int4 *tet_nodes = nullptr;
+    double *geo = nullptr;    // SoA [17][Tm]: da (12, row-major 4x3), w (5)
+    int32_t *tet_reg = nullptr;
+    DevRegion *regions = nullptr;
+    double4 *elem_out = nullptr;  // [4 Tm + 3 Fk] (D contribution, BE xyz)
+    int32_t *inc_ptr = nullptr, *inc_idx = nullptr;  // node -> elem_out entries
+
+    // surface anisotropy triangles
+    int4 *trik_nodes = nullptr;
+    double *trik_geo = nullptr;  // SoA [9][Fk]: w0..3, Kbis, uk xyz, pad
+
+    // charged triangles
+    int4 *tri_nodes = nullptr;
+    double *tri_geo = nullptr;   // SoA [16][Fm]: n xyz, w0..3, dMs, surf, pad x3, Ks, uk xyz
+    double2 *tri_corr = nullptr; // [3][Fm] vertex corrections (u, v)
+    int32_t *cinc_ptr = nullptr, *cinc_idx = nullptr;  // node -> tri_corr entries
+    double2 *corr = nullptr;     // [N]
+
+    // demag sources
+    double4 *src_pos = nullptr;  // [S]
+    double2 *src_q = nullptr;    // [S] (q from u, q from v)
+    float4 *src_pos_f = nullptr; // FP32 copies for the mixed-precision P2P
+    FmmPlan *fmm = nullptr;
+
+    // energies / reductions
+    double *epart = nullptr;  // [8][MAX_PART]
+    double *res = nullptr;    // [32] small results: energies 0..5, min dot 8, average 16..18
+
+    // profiling
+    std::vector<KernelStat> stats;
+    struct Pending {
+        int id;
+        hipEvent_t a, b;
+    };
+    std::vector<Pending> pending;
+    std::vector<hipEvent_t> event_pool;
+
+    int blocks_for(int64_t n) const
+    {
+        int64_t b = (n + BLOCK - 1) / BLOCK;
+        return (int)(b < MAX_PART ? (b > 0 ? b : 1) : MAX_PART);
+    }
+    int kid(const char *name);
+    void tic(int id, hipEvent_t *a);
+    void toc(int id, hipEvent_t a);
+    void flush_stats();
+};
+
+// launch-time profiling scope: records HIP events on the context stream around a kernel
+struct Prof {
+    Ctx &c;
+    int id;
+    hipEvent_t a = nullptr;
+    Prof(Ctx &ctx, const char *name) : c(ctx), id(ctx.opt.profile ? ctx.kid(name) : -1)
+    {
+        if (id >= 0) c.tic(id, &a);
+    }
+    ~Prof()
+    {
+        if (id >= 0) c.toc(id, a);
+    }
+};
+
+// ---- launchers (kernels.hip) ----
+void launch_element(Ctx &c, const double Hext[3], double dt);
+void launch_basis(Ctx &c, double r);
+void launch_node_assemble(Ctx &c, double r, double c_stencil, double dt);
+void launch_r0(Ctx &c, double c_stencil);
+void launch_iteration(Ctx &c, double c_stencil);
+void launch_update(Ctx &c, double dt);
+void launch_spmv_debug(Ctx &c, double c_stencil, const double2 *xin, double2 *yout);
+void launch_charges(Ctx &c);
+void launch_direct_p2p(Ctx &c);
+void launch_energy(Ctx &c, const double Hext[3]);
+void launch_max_angle(Ctx &c);
+void launch_average(Ctx &c, int which);
+void launch_evolution(Ctx &c);
+
+// ---- FMM (fmm.hip) ----
+int fmm_create(Ctx &c);
+void fmm_destroy(Ctx &c);
+void launch_fmm(Ctx &c);
+
+}  // namespace fg

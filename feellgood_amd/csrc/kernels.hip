@@ -1,0 +1,1158 @@
+// kernels.hip -- element pass, node assembly, Krylov, update, demag sources, energies.
+//
+// Data layout (all FP64):
+//   node state     NodeRec[N] x 2 (CURRENT / NEXT, swapped by evolution)      64 B per node
+//   basis          ep4[N], eq4[N] (double4)
+//   tet geometry   SoA geo[17][Tm] (da 4x3 row-major, w[5]); nodes int4[Tm]
+//   element output elem_out[4 Tm (+3 Fk)] double4 = (alpha_eff-weighted mass, BE x, y, z)
+//   stiffness S'   SELL-64 (col int32, val f64), S'_ij = sum_t Abis_t sum_g w_g (da_a . da_b)
+//   Krylov vectors double2[N] (the two tangent unknowns of a node side by side)
+//
+// The LLG matrix of FeeLLGood (src/tetra.cpp:210-307 assembled by src/solver.h:319-339) factors as
+//   K[(i,di),(j,dj)] = c S'_ij (e_r(di),i . e_s(dj),j) + delta_ij (D_i B_ii + m_i X_i)[di][dj]
+// with c = prefactor * theta * dt * gamma0, rows r = (eq, ep), columns s = (ep, eq), B_ij the
+// basis dot products, X_i the projection of [u_i]x and D_i the alpha_eff-weighted lumped mass.
+// The SpMV therefore lifts x to a 3-D field w = x_p ep + x_q eq, applies the scalar stencil S' to
+// it and projects back, reading 12 bytes per stored entry instead of a 2x2 FP64 block.
+#include "ctx.h"
+
+#include <cmath>
+
+namespace fg {
+
+// ------------------------------------------------------------------ reductions
+__device__ inline double wave_sum(double v)
+{
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+__device__ inline double wave_max(double v)
+{
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+    return v;
+}
+__device__ inline double wave_min(double v)
+{
+    for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+// Sum of NV per-thread values over the block, stored to part[slot[k] * MAX_PART + blockIdx.x].
+template <int NV>
+__device__ inline void block_store(const double (&v)[NV], const int (&slot)[NV], double *part)
+{
+    __shared__ double sh[NV][BLOCK / 64];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    double r[NV];
+#pragma unroll
+    for (int k = 0; k < NV; k++) r[k] = wave_sum(v[k]);
+    if (lane == 0)
+#pragma unroll
+        for (int k = 0; k < NV; k++) sh[k][wid] = r[k];
+    __syncthreads();
+    if (threadIdx.x == 0)
+#pragma unroll
+        for (int k = 0; k < NV; k++) {
+            double s = sh[k][0];
+            for (int q = 1; q < BLOCK / 64; q++) s += sh[k][q];
+            part[slot[k] * MAX_PART + blockIdx.x] = s;
+        }
+}
+
+// Deterministic sum of nb partials of a slot; every thread of the block gets the result.
+template <int NV>
+__device__ inline void block_load(const double *part, const int (&slot)[NV], int nb, double (&out)[NV])
+{
+    __shared__ double res[NV];
+    if (threadIdx.x < 64) {
+#pragma unroll
+        for (int k = 0; k < NV; k++) {
+            double s = 0;
+            for (int i = threadIdx.x; i < nb; i += 64) s += part[slot[k] * MAX_PART + i];
+            s = wave_sum(s);
+            if (threadIdx.x == 0) res[k] = s;
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < NV; k++) out[k] = res[k];
+}
+
+// ------------------------------------------------------------------ element pass
+// Tetra::calc_alpha_eff (src/tetra.cpp:47-75)
+__device__ inline double alpha_eff(double dt, double alpha, double h)
+{
+    const double reduced_dt = GAMMA0 * dt;
+    const double M = 2. * alpha * 0.1 / reduced_dt;
+    if (h > 0.) return alpha + reduced_dt / 2. * (h > M ? M : h);
+    if (h < -M) return alpha / (1. + reduced_dt / (2. * alpha) * M);
+    return alpha / (1. - reduced_dt / (2. * alpha) * h);
+}
+
+// One thread per magnetic tetrahedron: the basis-free part of Tet::integrales
+// (src/tetra.cpp:210-307): for every local node a, the alpha_eff-weighted lumped mass
+// sum_g a_ag w_g alpha_eff,g and the 3-D right-hand side BE_a.
+__global__ __launch_bounds__(BLOCK) void k_element(int Tm, const int4 *__restrict__ tn,
+                                                   const double *__restrict__ geo,
+                                                   const int32_t *__restrict__ treg,
+                                                   const DevRegion *__restrict__ regs,
+                                                   const NodeRec *__restrict__ cur, double Hx,
+                                                   double Hy, double Hz, double dt,
+                                                   double4 *__restrict__ out)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= Tm) return;
+    const int4 q = tn[t];
+    const int nd[4] = {q.x, q.y, q.z, q.w};
+    double da[4][3], w[5];
+#pragma unroll
+    for (int c = 0; c < 12; c++) da[c / 3][c % 3] = geo[(size_t)c * Tm + t];
+#pragma unroll
+    for (int g = 0; g < 5; g++) w[g] = geo[(size_t)(12 + g) * Tm + t];
+    const DevRegion R = regs[treg[t]];
+    d3 u[4], v[4];
+    double ph[4], phv[4];
+#pragma unroll
+    for (int a = 0; a < 4; a++) {
+        const NodeRec n = cur[nd[a]];
+        u[a] = mk(n.ux, n.uy, n.uz);
+        v[a] = mk(n.vx, n.vy, n.vz);
+        ph[a] = n.phi;
+        phv[a] = n.phiv;
+    }
+    d3 dU[3];
+#pragma unroll
+    for (int k = 0; k < 3; k++) dU[k] = da[0][k] * u[0] + da[1][k] * u[1] + da[2][k] * u[2] + da[3][k] * u[3];
+    d3 Hd = mk(0, 0, 0), Hv = mk(0, 0, 0);
+#pragma unroll
+    for (int a = 0; a < 4; a++) {
+        const d3 g = mk(da[a][0], da[a][1], da[a][2]);
+        Hd = Hd - ph[a] * g;
+        Hv = Hv - phv[a] * g;
+    }
+    const double exch = -R.Abis * (dot(dU[0], dU[0]) + dot(dU[1], dU[1]) + dot(dU[2], dU[2]));
+    const d3 H0 = Hd + mk(Hx, Hy, Hz);
+    const double sdt = THETA * dt;  // s_dt / gamma0
+    const d3 uk = mk(R.uk[0], R.uk[1], R.uk[2]);
+    const d3 ex = mk(R.ex[0], R.ex[1], R.ex[2]), ey = mk(R.ey[0], R.ey[1], R.ey[2]),
+             ez = mk(R.ez[0], R.ez[1], R.ez[2]);
+    double contrib[4] = {0, 0, 0, 0};
+    d3 BE[4];
+    d3 gex[4];
+#pragma unroll
+    for (int a = 0; a < 4; a++) {
+        BE[a] = mk(0, 0, 0);
+        gex[a] = da[a][0] * dU[0] + da[a][1] * dU[1] + da[a][2] * dU[2];
+    }
+#pragma unroll
+    for (int g = 0; g < 5; g++) {
+        d3 U = mk(0, 0, 0), V = mk(0, 0, 0);
+#pragma unroll
+        for (int a = 0; a < 4; a++) {
+            U = U + tet_a(a, g) * u[a];
+            V = V + tet_a(a, g) * v[a];
+        }
+        double uH = exch;
+        d3 Han = mk(0, 0, 0);
+        if (R.hasK) {  // Tet::calc_aniso_uniax (src/tetra.cpp:171-181)
+            Han = Han + (R.Kbis * dot(uk, U + sdt * V)) * uk;
+            const double c = dot(U, uk);
+            uH += R.Kbis * (c * c);
+        }
+        if (R.hasK3) {  // Tet::calc_aniso_cub (src/tetra.cpp:183-208)
+            const d3 bu = mk(dot(ex, U), dot(ey, U), dot(ez, U));
+            const d3 bv = mk(dot(ex, V), dot(ey, V), dot(ez, V));
+            const d3 b3 = mk(bu.x * (1.0 - bu.x * bu.x), bu.y * (1.0 - bu.y * bu.y), bu.z * (1.0 - bu.z * bu.z));
+            const d3 tm = mk(bv.x * ex.x * (1 - 3 * bu.x * bu.x), bv.y * ex.y * (1 - 3 * bu.y * bu.y),
+                             bv.z * ex.z * (1 - 3 * bu.z * bu.z));
+            Han = Han + (-R.K3bis) * (b3.x * ex + b3.y * ey + b3.z * ez + sdt * tm);
+            uH += -R.K3bis * dot(bu, b3);
+        }
+        uH += dot(U, H0);
+        const double ae = alpha_eff(dt, R.alpha, uH);
+        const d3 H = H0 + (Han + sdt * Hv);
+        const double wg = w[g];
+#pragma unroll
+        for (int a = 0; a < 4; a++) {
+            const double aw = wg * tet_a(a, g);
+            contrib[a] += tet_a(a, g) * (wg * ae);
+            BE[a] = BE[a] - (wg * R.Abis) * gex[a];
+            BE[a] = BE[a] + aw * H;
+        }
+    }
+#pragma unroll
+    for (int a = 0; a < 4; a++) out[(size_t)a * Tm + t] = make_double4(contrib[a], BE[a].x, BE[a].y, BE[a].z);
+}
+
+// Tri::integrales (src/triangle.cpp:6-36): surface anisotropy right-hand side, 3-D form.
+__global__ __launch_bounds__(BLOCK) void k_tri_aniso(int Fk, const int4 *__restrict__ tn,
+                                                     const double *__restrict__ tg,
+                                                     const NodeRec *__restrict__ cur,
+                                                     double4 *__restrict__ out)
+{
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= Fk) return;
+    const int4 q = tn[f];
+    const int nd[3] = {q.x, q.y, q.z};
+    d3 u[3];
+    for (int a = 0; a < 3; a++) {
+        const NodeRec n = cur[nd[a]];
+        u[a] = mk(n.ux, n.uy, n.uz);
+    }
+    const double Kbis = tg[(size_t)4 * Fk + f];
+    const d3 uk = mk(tg[(size_t)5 * Fk + f], tg[(size_t)6 * Fk + f], tg[(size_t)7 * Fk + f]);
+    d3 BE[3] = {mk(0, 0, 0), mk(0, 0, 0), mk(0, 0, 0)};
+    for (int g = 0; g < 4; g++) {
+        const d3 U = tri_a(0, g) * u[0] + tri_a(1, g) * u[1] + tri_a(2, g) * u[2];
+        const double pre = tg[(size_t)g * Fk + f] * Kbis * dot(uk, U);
+        for (int a = 0; a < 3; a++) BE[a] = BE[a] + (pre * tri_a(a, g)) * uk;
+    }
+    for (int a = 0; a < 3; a++) out[(size_t)a * Fk + f] = make_double4(0.0, BE[a].x, BE[a].y, BE[a].z);
+}
+
+// Node::setBasis (src/node.h:73-102)
+__device__ inline void set_basis(d3 u, double cr, double sr, d3 &ep, d3 &eq)
+{
+    const double ax = fabs(u.x), ay = fabs(u.y), az = fabs(u.z);
+    int k = 0;
+    if (ay < ax) k = 1;
+    if (az < (k == 0 ? ax : ay)) k = 2;
+    d3 e = mk(k == 0, k == 1, k == 2);
+    e = e - dot(e, u) * u;
+    const double n2 = dot(e, e);
+    if (n2 > 0) {
+        const double n = sqrt(n2);
+        e = mk(e.x / n, e.y / n, e.z / n);
+    }
+    const d3 qv = cross(u, e);
+    ep = cr * e - sr * qv;
+    eq = sr * e + cr * qv;
+}
+
+__global__ __launch_bounds__(BLOCK) void k_basis(int N, double cr, double sr, const NodeRec *__restrict__ cur,
+                                                 double4 *__restrict__ ep4, double4 *__restrict__ eq4)
+{
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x) {
+        const NodeRec n = cur[i];
+        d3 ep, eq;
+        set_basis(mk(n.ux, n.uy, n.uz), cr, sr, ep, eq);
+        ep4[i] = make_double4(ep.x, ep.y, ep.z, 0.0);
+        eq4[i] = make_double4(eq.x, eq.y, eq.z, 0.0);
+    }
+}
+
+// One thread per node: basis, gathered element contributions, diagonal block, Jacobi
+// preconditioner, right-hand side and initial guess (LinAlgebra::solve assembly part,
+// src/solver.cpp:11-59; buildInitGuess src/linear_algebra.cpp:13-24).
+__global__ __launch_bounds__(BLOCK) void k_node_assemble(
+    int N, double cr, double sr, double cst, const NodeRec *__restrict__ cur,
+    const NodeRec *__restrict__ vsrc, const int32_t *__restrict__ mag,
+    const int32_t *__restrict__ inc_ptr, const int32_t *__restrict__ inc_idx,
+    const double4 *__restrict__ eout, const double *__restrict__ mass,
+    const double *__restrict__ sdiag, double4 *__restrict__ ep4, double4 *__restrict__ eq4,
+    double4 *__restrict__ Dg, double2 *__restrict__ dprec, double2 *__restrict__ b,
+    double2 *__restrict__ x, double4 *__restrict__ w, double *__restrict__ part)
+{
+    double bb = 0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x) {
+        const NodeRec n = cur[i];
+        const d3 u = mk(n.ux, n.uy, n.uz);
+        d3 ep, eq;
+        set_basis(u, cr, sr, ep, eq);
+        ep4[i] = make_double4(ep.x, ep.y, ep.z, 0.0);
+        eq4[i] = make_double4(eq.x, eq.y, eq.z, 0.0);
+        if (!mag[i]) {
+            Dg[i] = make_double4(0, 0, 0, 0);
+            dprec[i] = make_double2(0, 0);
+            b[i] = make_double2(0, 0);
+            x[i] = make_double2(0, 0);
+            w[i] = make_double4(0, 0, 0, 0);
+            continue;
+        }
+        double D = 0;
+        d3 F = mk(0, 0, 0);
+        for (int k = inc_ptr[i]; k < inc_ptr[i + 1]; k++) {
+            const double4 e = eout[inc_idx[k]];
+            D += e.x;
+            F = F + mk(e.y, e.z, e.w);
+        }
+        const double m = mass[i];
+        // B = [[eq.ep, eq.eq], [ep.ep, ep.eq]];  X = projection of [u]x
+        const double B00 = dot(eq, ep), B01 = dot(eq, eq), B10 = dot(ep, ep), B11 = dot(ep, eq);
+        const d3 uep = cross(u, ep), ueq = cross(u, eq);
+        const double X00 = dot(eq, uep), X01 = dot(eq, ueq), X10 = dot(ep, uep), X11 = dot(ep, ueq);
+        const double4 dg = make_double4(D * B00 + m * X00, D * B01 + m * X01, D * B10 + m * X10, D * B11 + m * X11);
+        Dg[i] = dg;
+        const double sd = cst * sdiag[i];
+        dprec[i] = make_double2(1.0 / (sd * B00 + dg.x), 1.0 / (sd * B11 + dg.w));
+        const double2 bi = make_double2(dot(eq, F), dot(ep, F));
+        b[i] = bi;
+        bb += bi.x * bi.x + bi.y * bi.y;
+        const NodeRec nv = vsrc[i];
+        const d3 vv = mk(nv.vx, nv.vy, nv.vz);
+        const double2 xi = make_double2(dot(vv, ep) / GAMMA0, dot(vv, eq) / GAMMA0);
+        x[i] = xi;
+        const d3 wi = xi.x * ep + xi.y * eq;
+        w[i] = make_double4(wi.x, wi.y, wi.z, 0.0);
+    }
+    const double vals[1] = {bb};
+    const int slots[1] = {0};
+    block_store<1>(vals, slots, part);
+}
+
+// ------------------------------------------------------------------ stencil SpMV
+// y_i = c [eq_i . z_i, ep_i . z_i] + Dg_i x_i with z_i = sum_j S'_ij w_j, zero for
+// non-magnetic rows (applyMask after mult, src/algebra/bicg.h:201,216).
+__device__ inline double2 stencil_row(int i, double cst, const int64_t *__restrict__ sptr,
+                                      const int32_t *__restrict__ swid, const int32_t *__restrict__ scol,
+                                      const double *__restrict__ sval, const double4 *__restrict__ w,
+                                      const double4 *__restrict__ ep4, const double4 *__restrict__ eq4,
+                                      const double4 *__restrict__ Dg, double2 xi)
+{
+    const int sl = i >> 6, lane = i & 63;
+    const int64_t off = sptr[sl];
+    const int width = swid[sl];
+    double zx = 0, zy = 0, zz = 0;
+    for (int k = 0; k < width; k++) {
+        const int64_t e = off + (int64_t)k * SELL_C + lane;
+        const int j = scol[e];
+        const double sv = sval[e];
+        const double4 wj = w[j];
+        zx += sv * wj.x;
+        zy += sv * wj.y;
+        zz += sv * wj.z;
+    }
+    const double4 p = ep4[i], q = eq4[i], g = Dg[i];
+    double2 y;
+    y.x = cst * (q.x * zx + q.y * zy + q.z * zz) + (g.x * xi.x + g.y * xi.y);
+    y.y = cst * (p.x * zx + p.y * zy + p.z * zz) + (g.z * xi.x + g.w * xi.y);
+    return y;
+}
+
+#define STENCIL_ARGS                                                                          \
+    const int64_t *__restrict__ sptr, const int32_t *__restrict__ swid,                       \
+        const int32_t *__restrict__ scol, const double *__restrict__ sval,                    \
+        const double4 *__restrict__ ep4, const double4 *__restrict__ eq4,                     \
+        const double4 *__restrict__ Dg, const int32_t *__restrict__ mag
+#define STENCIL_PASS c.slice_ptr, c.slice_w, c.scol, c.sval, c.ep4, c.eq4, c.Dg, c.mag
+
+// r = b - A x (masked); rt = r; p = r; partials ||r||^2, rt.r  (src/algebra/bicg.h:172-184)
+__global__ __launch_bounds__(BLOCK) void k_r0(int N, double cst, STENCIL_ARGS, const double4 *__restrict__ w,
+                                              const double2 *__restrict__ x, const double2 *__restrict__ b,
+                                              double2 *__restrict__ r, double2 *__restrict__ rt,
+                                              double2 *__restrict__ p, KrylovCtl *ctl, double *__restrict__ part)
+{
+    double rr = 0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x) {
+        double2 ri = make_double2(0, 0);
+        if (mag[i]) {
+            const double2 y = stencil_row(i, cst, sptr, swid, scol, sval, w, ep4, eq4, Dg, x[i]);
+            const double2 bi = b[i];
+            ri = make_double2(bi.x - y.x, bi.y - y.y);
+        }
+        r[i] = ri;
+        rt[i] = ri;
+        p[i] = ri;
+        rr += ri.x * ri.x + ri.y * ri.y;
+    }
+    // rt = r, so rt . r = ||r||^2 for the first iteration
+    const double vals[2] = {rr, rr};
+    const int slots[2] = {1, 2};
+    block_store<2>(vals, slots, part);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        ctl->nit = 0;
+        ctl->status = -1;
+        ctl->done = 0;
+        ctl->rho1 = ctl->rho2 = ctl->alpha = ctl->omega = ctl->beta = 0.0;
+    }
+}
+
+// ||b|| into the control block (one block)
+__global__ void k_set_rhsn(int nb, KrylovCtl *ctl, const double *__restrict__ part)
+{
+    const int slots[1] = {0};
+    double v[1];
+    block_load<1>(part, slots, nb, v);
+    if (threadIdx.x == 0) ctl->rhsn = sqrt(fabs(v[0]));
+}
+
+// (C) loop head: convergence on ||r||, rho_1 = rt.r, beta, p = r + beta (p - omega v),
+// p^ = D p, w = lift(p^)  (src/algebra/bicg.h:180-199)
+__global__ __launch_bounds__(BLOCK) void k_iter_p(int N, int nb, KrylovCtl *ctl, const double4 *__restrict__ ep4,
+                                                  const double4 *__restrict__ eq4,
+                                                  const double2 *__restrict__ dprec,
+                                                  const double2 *__restrict__ r, const double2 *__restrict__ v,
+                                                  double2 *__restrict__ p, double2 *__restrict__ ph,
+                                                  double4 *__restrict__ w, const double *__restrict__ part)
+{
+    if (ctl->done) return;
+    const int slots[2] = {1, 2};
+    double red[2];
+    block_load<2>(part, slots, nb, red);
+    const double res = sqrt(fabs(red[0]));
+    const double rho1 = red[1];
+    const int nit = ctl->nit;
+    int status = ctl->status, done = 0;
+    if (isnan(res)) {
+        status = 2;
+        done = 1;
+    } else if (res <= ctl->rhsn * ctl->tol) {
+        status = 0;
+        done = 1;
+    }
+    double beta = 0;
+    const double omega = ctl->omega;
+    if (!done && nit > 0) {
+        if (ctl->rho2 == 0 || omega == 0) {
+            status = 2;
+            done = 1;
+        } else {
+            beta = (rho1 / ctl->rho2) * (ctl->alpha / omega);
+        }
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        ctl->res = res;
+        ctl->status = status;
+        ctl->rho1 = rho1;
+        ctl->beta = beta;
+        if (done) ctl->done = 1;
+    }
+    if (done) return;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x) {
+        double2 pi;
+        const double2 ri = r[i];
+        if (nit > 0) {
+            const double2 vi = v[i];
+            pi = p[i];
+            pi.x = (pi.x - omega * vi.x) * beta + ri.x;
+            pi.y = (pi.y - omega * vi.y) * beta + ri.y;
+        } else {
+            pi = ri;
+        }
+        p[i] = pi;
+        const double2 d = dprec[i];
+        const double2 h = make_double2(d.x * pi.x, d.y * pi.y);
+        ph[i] = h;
+        const double4 e = ep4[i], q = eq4[i];
+        w[i] = make_double4(h.x * e.x + h.y * q.x, h.x * e.y + h.y * q.y, h.x * e.z + h.y * q.z, 0.0);
+    }
+}
+
+// (D) v = A p^ (masked); partial v . rt
+__global__ __launch_bounds__(BLOCK) void k_spmv_v(int N, double cst, STENCIL_ARGS, const KrylovCtl *ctl,
+                                                  const double4 *__restrict__ w, const double2 *__restrict__ ph,
+                                                  const double2 *__restrict__ rt, double2 *__restrict__ v,
+                                                  double *__restrict__ part)
+{
+    if (ctl->done) return;
+    double vr = 0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x) {
+        double2 y = make_double2(0, 0);
+        if (mag[i]) y = stencil_row(i, cst, sptr, swid, scol, sval, w, ep4, eq4, Dg, ph[i]);
+        v[i] = y;
+        const double2 q = rt[i];
+        vr += y.x * q.x + y.y * q.y;
+    }
+    const double vals[1] = {vr};
+    const int slots[1] = {3};
+    block_store<1>(vals, slots, part);
+}
+
+// (E) alpha = rho_1 / (v . rt); s = r - alpha v; s^ = D s; w = lift(s^); partial ||s||^2
+__global__ __launch_bounds__(BLOCK) void k_iter_s(int N, int nb, KrylovCtl *ctl, const double4 *__restrict__ ep4,
+                                                  const double4 *__restrict__ eq4,
+                                                  const double2 *__restrict__ dprec,
+                                                  const double2 *__restrict__ r, const double2 *__restrict__ v,
+                                                  double2 *__restrict__ s, double2 *__restrict__ sh,
+                                                  double4 *__restrict__ w, double *__restrict__ part)
+{
+    if (ctl->done) return;
+    const int slots[1] = {3};
+    double red[1];
+    block_load<1>(part, slots, nb, red);
+    const double alpha = ctl->rho1 / red[0];
+    if (blockIdx.x == 0 && threadIdx.x == 0) ctl->alpha = alpha;
+    double ss = 0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x) {
+        const double2 ri = r[i], vi = v[i];
+        const double2 si = make_double2(ri.x + (-alpha) * vi.x, ri.y + (-alpha) * vi.y);
+        s[i] = si;
+        ss += si.x * si.x + si.y * si.y;
+        const double2 d = dprec[i];
+        const double2 h = make_double2(d.x * si.x, d.y * si.y);
+        sh[i] = h;
+        const double4 e = ep4[i], q = eq4[i];
+        w[i] = make_double4(h.x * e.x + h.y * q.x, h.x * e.y + h.y * q.y, h.x * e.z + h.y * q.z, 0.0);
+    }
+    const double vals[1] = {ss};
+    const int slots2[1] = {4};
+    block_store<1>(vals, slots2, part);
+}
+
+// (F) convergence on ||s|| (x += alpha p^ when converged), else t = A s^ (masked);
+// partials t.s, t.t  (src/algebra/bicg.h:206-217)
+__global__ __launch_bounds__(BLOCK) void k_spmv_t(int N, int nb, double cst, STENCIL_ARGS, KrylovCtl *ctl,
+                                                  const double4 *__restrict__ w, const double2 *__restrict__ sh,
+                                                  const double2 *__restrict__ s, const double2 *__restrict__ ph,
+                                                  double2 *__restrict__ x, double2 *__restrict__ t,
+                                                  double *__restrict__ part)
+{
+    if (ctl->done) return;
+    const int slots[1] = {4};
+    double red[1];
+    block_load<1>(part, slots, nb, red);
+    const double res = sqrt(fabs(red[0]));
+    int status = ctl->status, done = 0;
+    bool conv = false;
+    if (isnan(res)) {
+        status = 2;
+        done = 1;
+    } else if (res <= ctl->rhsn * ctl->tol) {
+        status = 0;
+        done = 1;
+        conv = true;
+    } else if (status == 1 || status == 2) {
+        done = 1;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0 && done) {
+        ctl->res = res;
+        ctl->status = status;
+        ctl->done = 1;
+    }
+    if (done) {
+        if (conv) {
+            const double alpha = ctl->alpha;
+            for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x) {
+                double2 xi = x[i];
+                const double2 h = ph[i];
+                xi.x += alpha * h.x;
+                xi.y += alpha * h.y;
+                x[i] = xi;
+            }
+        }
+        return;
+    }
+    double ts = 0, tt = 0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x) {
+        double2 y = make_double2(0, 0);
+        if (mag[i]) y = stencil_row(i, cst, sptr, swid, scol, sval, w, ep4, eq4, Dg, sh[i]);
+        t[i] = y;
+        const double2 si = s[i];
+        ts += y.x * si.x + y.y * si.y;
+        tt += y.x * y.x + y.y * y.y;
+    }
+    const double vals[2] = {ts, tt};
+    const int slots2[2] = {5, 6};
+    block_store<2>(vals, slots2, part);
+}
+
+// (G) omega = (t.s)/(t.t); x += alpha p^ + omega s^; r = s - omega t; partials ||r||^2, rt.r
+__global__ __launch_bounds__(BLOCK) void k_iter_x(int N, int nb, KrylovCtl *ctl, const double2 *__restrict__ ph,
+                                                  const double2 *__restrict__ sh, const double2 *__restrict__ s,
+                                                  const double2 *__restrict__ t, const double2 *__restrict__ rt,
+                                                  double2 *__restrict__ x, double2 *__restrict__ r,
+                                                  double *__restrict__ part)
+{
+    if (ctl->done) return;
+    const int slots[2] = {5, 6};
+    double red[2];
+    block_load<2>(part, slots, nb, red);
+    const double omega = red[0] / red[1];
+    const double alpha = ctl->alpha;
+    double rr = 0, rtr = 0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x) {
+        double2 xi = x[i];
+        const double2 h = ph[i], hs = sh[i], si = s[i], ti = t[i], q = rt[i];
+        xi.x += alpha * h.x;
+        xi.y += alpha * h.y;
+        xi.x += omega * hs.x;
+        xi.y += omega * hs.y;
+        x[i] = xi;
+        const double2 ri = make_double2(si.x - omega * ti.x, si.y - omega * ti.y);
+        r[i] = ri;
+        rr += ri.x * ri.x + ri.y * ri.y;
+        rtr += q.x * ri.x + q.y * ri.y;
+    }
+    const double vals[2] = {rr, rtr};
+    const int slots2[2] = {1, 2};
+    block_store<2>(vals, slots2, part);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        ctl->omega = omega;
+        ctl->rho2 = ctl->rho1;
+        const int nit = ctl->nit + 1;
+        ctl->nit = nit;
+        if (nit >= ctl->max_iter) ctl->status = 1;
+    }
+}
+
+// debug: y = A x without masking side effects
+__global__ __launch_bounds__(BLOCK) void k_spmv_plain(int N, double cst, STENCIL_ARGS, const double4 *__restrict__ w,
+                                                      const double2 *__restrict__ xin, double2 *__restrict__ y)
+{
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x) {
+        double2 yi = make_double2(0, 0);
+        if (mag[i]) yi = stencil_row(i, cst, sptr, swid, scol, sval, w, ep4, eq4, Dg, xin[i]);
+        y[i] = yi;
+    }
+}
+
+__global__ __launch_bounds__(BLOCK) void k_lift(int N, const double4 *__restrict__ ep4,
+                                                const double4 *__restrict__ eq4,
+                                                const double2 *__restrict__ xin, double4 *__restrict__ w)
+{
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x) {
+        const double2 h = xin[i];
+        const double4 e = ep4[i], q = eq4[i];
+        w[i] = make_double4(h.x * e.x + h.y * q.x, h.x * e.y + h.y * q.y, h.x * e.z + h.y * q.z, 0.0);
+    }
+}
+
+// ------------------------------------------------------------------ update
+// v = gamma0 (x_p ep + x_q eq); u_next = normalize(u + dt v); partial max |x_i|^2
+// (src/solver.cpp:74-88, src/node.h:116-122)
+__global__ __launch_bounds__(BLOCK) void k_update(int N, double dt, const int32_t *__restrict__ mag,
+                                                  const double2 *__restrict__ x, const double4 *__restrict__ ep4,
+                                                  const double4 *__restrict__ eq4,
+                                                  const NodeRec *__restrict__ cur, NodeRec *__restrict__ next,
+                                                  double *__restrict__ part)
+{
+    double m2 = 0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x) {
+        if (!mag[i]) continue;
+        const double2 xi = x[i];
+        m2 = fmax(m2, xi.x * xi.x + xi.y * xi.y);
+        const double4 e = ep4[i], q = eq4[i];
+        const double vp = xi.x * GAMMA0, vq = xi.y * GAMMA0;
+        const d3 v = vp * mk(e.x, e.y, e.z) + vq * mk(q.x, q.y, q.z);
+        const NodeRec n = cur[i];
+        d3 u = mk(n.ux, n.uy, n.uz) + dt * v;
+        const double n2 = dot(u, u);
+        if (n2 > 0) {
+            const double nn = sqrt(n2);
+            u = mk(u.x / nn, u.y / nn, u.z / nn);
+        }
+        NodeRec o = next[i];
+        o.ux = u.x;
+        o.uy = u.y;
+        o.uz = u.z;
+        o.vx = v.x;
+        o.vy = v.y;
+        o.vz = v.z;
+        next[i] = o;
+    }
+    __shared__ double sh[BLOCK / 64];
+    m2 = wave_max(m2);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = m2;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double s = sh[0];
+        for (int q = 1; q < BLOCK / 64; q++) s = fmax(s, sh[q]);
+        part[7 * MAX_PART + blockIdx.x] = s;
+    }
+}
+
+__global__ void k_finish_update(int nb, KrylovCtl *ctl, const double *__restrict__ part)
+{
+    if (threadIdx.x < 64) {
+        double m = 0;
+        for (int i = threadIdx.x; i < nb; i += 64) m = fmax(m, part[7 * MAX_PART + i]);
+        m = wave_max(m);
+        if (threadIdx.x == 0) ctl->vmax2 = m;
+    }
+}
+
+// ------------------------------------------------------------------ demag sources
+// Tet::charges (src/tetra.cpp:347-359): q_g = -Ms w_g div(u), for u and v NEXT at once.
+__global__ __launch_bounds__(BLOCK) void k_charges_tet(int Tm, const int4 *__restrict__ tn,
+                                                       const double *__restrict__ geo,
+                                                       const int32_t *__restrict__ treg,
+                                                       const DevRegion *__restrict__ regs,
+                                                       const NodeRec *__restrict__ nxt,
+                                                       double2 *__restrict__ q)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= Tm) return;
+    const int4 qn = tn[t];
+    const int nd[4] = {qn.x, qn.y, qn.z, qn.w};
+    double du = 0, dv = 0, duy = 0, dvy = 0, duz = 0, dvz = 0;
+#pragma unroll
+    for (int a = 0; a < 4; a++) {
+        const NodeRec n = nxt[nd[a]];
+        const double gx = geo[(size_t)(3 * a) * Tm + t], gy = geo[(size_t)(3 * a + 1) * Tm + t],
+                     gz = geo[(size_t)(3 * a + 2) * Tm + t];
+        du += n.ux * gx;
+        dv += n.vx * gx;
+        duy += n.uy * gy;
+        dvy += n.vy * gy;
+        duz += n.uz * gz;
+        dvz += n.vz * gz;
+    }
+    du = du + duy + duz;
+    dv = dv + dvy + dvz;
+    const double Ms = regs[treg[t]].Ms;
+#pragma unroll
+    for (int g = 0; g < 5; g++) {
+        const double wg = geo[(size_t)(12 + g) * Tm + t];
+        q[(size_t)5 * t + g] = make_double2(-Ms * wg * du, -Ms * wg * dv);
+    }
+}
+
+// Tri::potential (src/triangle.cpp:87-125): analytic potential at vertex i of the linear surface
+// charge density dMs * sum_j s_j lambda_j
+__device__ inline double tri_potential(const d3 P[3], const double s[3], double surf, double dMs, int i)
+{
+    const int ii = (i + 1) % 3, iii = (i + 2) % 3;
+    const d3 p12 = P[ii] - P[i], p13 = P[iii] - P[i];
+    const double b = sqrt(dot(p12, p12));
+    const double t = dot(p12, p13) / b;
+    const double s2 = 2. * surf;
+    const double h = s2 / b;
+    const double c = (t - b) / h;
+    const double fth = sqrt(1.0 + (t / h) * (t / h)), fc = sqrt(1.0 + c * c);
+    const double r = h * fth;
+    const double lg = log((c * t + h + fc * r) / (b * (c + fc)));
+    const double xi = b * lg / fc;
+    const double pot = xi * s[i] + ((xi * (h + c * t) - b * (r - b)) * s[ii] + b * (r - b - c * xi) * s[iii]) * b /
+                                       (s2 * (1 + c * c));
+    return 0.5 * dMs * pot;
+}
+
+// Tri::charges + Tri::correctionCharges (src/triangle.cpp:45-78) for u and v NEXT at once.
+__global__ __launch_bounds__(BLOCK) void k_charges_tri(int Fm, int64_t src_off, const int4 *__restrict__ tn,
+                                                       const double *__restrict__ tg,
+                                                       const double4 *__restrict__ pos4,
+                                                       const NodeRec *__restrict__ nxt,
+                                                       double2 *__restrict__ q, double2 *__restrict__ corr3)
+{
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= Fm) return;
+    const int4 qn = tn[f];
+    const int nd[3] = {qn.x, qn.y, qn.z};
+    const d3 n = mk(tg[f], tg[(size_t)Fm + f], tg[(size_t)2 * Fm + f]);
+    const double dMs = tg[(size_t)7 * Fm + f], surf = tg[(size_t)8 * Fm + f];
+    d3 P[3], u[3], v[3];
+    for (int a = 0; a < 3; a++) {
+        const double4 p = pos4[nd[a]];
+        P[a] = mk(p.x, p.y, p.z);
+        const NodeRec r = nxt[nd[a]];
+        u[a] = mk(r.ux, r.uy, r.uz);
+        v[a] = mk(r.vx, r.vy, r.vz);
+    }
+    double qu[4], qv[4];
+    d3 G[4];
+    for (int g = 0; g < 4; g++) {
+        const double wg = tg[(size_t)(3 + g) * Fm + f];
+        const d3 U = tri_a(0, g) * u[0] + tri_a(1, g) * u[1] + tri_a(2, g) * u[2];
+        const d3 V = tri_a(0, g) * v[0] + tri_a(1, g) * v[1] + tri_a(2, g) * v[2];
+        G[g] = tri_a(0, g) * P[0] + tri_a(1, g) * P[1] + tri_a(2, g) * P[2];
+        qu[g] = dMs != 0.0 ? dMs * (wg * dot(U, n)) : 0.0;
+        qv[g] = dMs != 0.0 ? dMs * (wg * dot(V, n)) : 0.0;
+        q[src_off + (int64_t)4 * f + g] = make_double2(qu[g], qv[g]);
+    }
+    const double su[3] = {dot(u[0], n), dot(u[1], n), dot(u[2], n)};
+    const double sv[3] = {dot(v[0], n), dot(v[1], n), dot(v[2], n)};
+    for (int a = 0; a < 3; a++) {
+        double cu = 0, cv = 0;
+        for (int g = 0; g < 4; g++) {
+            const d3 d = P[a] - G[g];
+            const double dist = sqrt(dot(d, d));
+            cu -= qu[g] / dist;
+            cv -= qv[g] / dist;
+        }
+        cu += tri_potential(P, su, surf, dMs, a);
+        cv += tri_potential(P, sv, surf, dMs, a);
+        corr3[(size_t)a * Fm + f] = make_double2(cu, cv);
+    }
+}
+
+__global__ __launch_bounds__(BLOCK) void k_corr_gather(int N, const int32_t *__restrict__ cptr,
+                                                       const int32_t *__restrict__ cidx,
+                                                       const double2 *__restrict__ corr3, double2 *__restrict__ corr)
+{
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x) {
+        double cu = 0, cv = 0;
+        for (int k = cptr[i]; k < cptr[i + 1]; k++) {
+            const double2 e = corr3[cidx[k]];
+            cu += e.x;
+            cv += e.y;
+        }
+        corr[i] = make_double2(cu, cv);
+    }
+}
+
+// Exact potentials phi_i = (sum_s q_s / |x_i - y_s| + corr_i) / (4 pi) (src/fmm_demag.h:211-223),
+// FP64, sources staged through LDS in tiles of BLOCK.
+__global__ __launch_bounds__(BLOCK) void k_p2p_direct(int N, int64_t S, const int32_t *__restrict__ mag,
+                                                      const double4 *__restrict__ pos4,
+                                                      const double4 *__restrict__ spos,
+                                                      const double2 *__restrict__ q,
+                                                      const double2 *__restrict__ corr, NodeRec *__restrict__ nxt)
+{
+    __shared__ double4 ts[BLOCK];
+    __shared__ double2 tq[BLOCK];
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool act = i < N && mag[i];
+    double4 xi = make_double4(0, 0, 0, 0);
+    if (i < N) xi = pos4[i];
+    double pu = 0, pv = 0;
+    for (int64_t base = 0; base < S; base += BLOCK) {
+        const int64_t k = base + threadIdx.x;
+        if (k < S) {
+            ts[threadIdx.x] = spos[k];
+            tq[threadIdx.x] = q[k];
+        } else {
+            ts[threadIdx.x] = make_double4(1e30, 1e30, 1e30, 0);
+            tq[threadIdx.x] = make_double2(0, 0);
+        }
+        __syncthreads();
+        if (act) {
+#pragma unroll 8
+            for (int j = 0; j < BLOCK; j++) {
+                const double4 y = ts[j];
+                const double dx = xi.x - y.x, dy = xi.y - y.y, dz = xi.z - y.z;
+                const double ir = 1.0 / sqrt(dx * dx + dy * dy + dz * dz);
+                const double2 qq = tq[j];
+                pu += qq.x * ir;
+                pv += qq.y * ir;
+            }
+        }
+        __syncthreads();
+    }
+    if (act) {
+        const double2 c = corr[i];
+        NodeRec r = nxt[i];
+        r.phi = (pu + c.x) / (4 * M_PI);
+        r.phiv = (pv + c.y) / (4 * M_PI);
+        nxt[i] = r;
+    }
+}
+
+// ------------------------------------------------------------------ energies
+// Fem::energy (src/energy.cpp:5-68), element energies src/tetra.cpp:309-391 and
+// src/triangle.cpp:38-43,80-85, on the NEXT state; partials per block, slots 0..3 = E[4].
+__global__ __launch_bounds__(BLOCK) void k_energy_tet(int Tm, const int4 *__restrict__ tn,
+                                                      const double *__restrict__ geo,
+                                                      const int32_t *__restrict__ treg,
+                                                      const DevRegion *__restrict__ regs,
+                                                      const NodeRec *__restrict__ nxt, double Hx, double Hy,
+                                                      double Hz, double *__restrict__ part)
+{
+    double e[4] = {0, 0, 0, 0};
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < Tm; t += gridDim.x * blockDim.x) {
+        const int4 q = tn[t];
+        const int nd[4] = {q.x, q.y, q.z, q.w};
+        const DevRegion R = regs[treg[t]];
+        d3 u[4];
+        double ph[4];
+        for (int a = 0; a < 4; a++) {
+            const NodeRec n = nxt[nd[a]];
+            u[a] = mk(n.ux, n.uy, n.uz);
+            ph[a] = n.phi;
+        }
+        double da[4][3], w[5];
+        for (int c = 0; c < 12; c++) da[c / 3][c % 3] = geo[(size_t)c * Tm + t];
+        for (int g = 0; g < 5; g++) w[g] = geo[(size_t)(12 + g) * Tm + t];
+        d3 dU[3];
+        for (int k = 0; k < 3; k++) dU[k] = da[0][k] * u[0] + da[1][k] * u[1] + da[2][k] * u[2] + da[3][k] * u[3];
+        const double dens = dot(dU[0], dU[0]) + dot(dU[1], dU[1]) + dot(dU[2], dU[2]);
+        const double divu = dU[0].x + dU[1].y + dU[2].z;
+        double ex = 0, dm = 0, an = 0, ze = 0;
+        const d3 H = mk(Hx, Hy, Hz);
+        for (int g = 0; g < 5; g++) {
+            d3 U = mk(0, 0, 0);
+            double pg = 0;
+            for (int a = 0; a < 4; a++) {
+                U = U + tet_a(a, g) * u[a];
+                pg += ph[a] * tet_a(a, g);
+            }
+            ex += w[g] * dens;
+            dm += w[g] * (divu * pg);
+            if (R.hasK) {
+                const double c = dot(mk(R.uk[0], R.uk[1], R.uk[2]), U);
+                an += w[g] * (-R.K) * (c * c);
+            }
+            if (R.hasK3) {
+                const double a0 = dot(U, mk(R.ex[0], R.ex[1], R.ex[2])), a1 = dot(U, mk(R.ey[0], R.ey[1], R.ey[2])),
+                             a2 = dot(U, mk(R.ez[0], R.ez[1], R.ez[2]));
+                an += w[g] * R.K3 * ((a0 * a1) * (a0 * a1) + (a1 * a2) * (a1 * a2) + (a2 * a0) * (a2 * a0));
+            }
+            ze += w[g] * dot(U, H);
+        }
+        e[0] += R.Ae * ex;
+        e[1] += an;
+        e[2] += -0.5 * MU0 * R.Ms * dm;
+        e[3] += -MU0 * R.Ms * ze;
+    }
+    const int slots[4] = {0, 1, 2, 3};
+    block_store<4>(e, slots, part);
+}
+
+__global__ __launch_bounds__(BLOCK) void k_energy_tri(int Fm, const int4 *__restrict__ tn,
+                                                      const double *__restrict__ tg,
+                                                      const double *__restrict__ tks, // Ks, uk per tri (4 SoA)
+                                                      const NodeRec *__restrict__ nxt, double *__restrict__ part)
+{
+    double e[2] = {0, 0};
+    for (int f = blockIdx.x * blockDim.x + threadIdx.x; f < Fm; f += gridDim.x * blockDim.x) {
+        const int4 q = tn[f];
+        const int nd[3] = {q.x, q.y, q.z};
+        const d3 n = mk(tg[f], tg[(size_t)Fm + f], tg[(size_t)2 * Fm + f]);
+        const double dMs = tg[(size_t)7 * Fm + f];
+        const double Ks = tks[f];
+        const d3 uk = mk(tks[(size_t)Fm + f], tks[(size_t)2 * Fm + f], tks[(size_t)3 * Fm + f]);
+        d3 u[3];
+        double ph[3];
+        for (int a = 0; a < 3; a++) {
+            const NodeRec r = nxt[nd[a]];
+            u[a] = mk(r.ux, r.uy, r.uz);
+            ph[a] = r.phi;
+        }
+        double d = 0, an = 0;
+        for (int g = 0; g < 4; g++) {
+            const d3 U = tri_a(0, g) * u[0] + tri_a(1, g) * u[1] + tri_a(2, g) * u[2];
+            const double pg = ph[0] * tri_a(0, g) + ph[1] * tri_a(1, g) + ph[2] * tri_a(2, g);
+            const double wg = tg[(size_t)(3 + g) * Fm + f];
+            d += (dot(U, n) * pg) * wg;
+            if (Ks != 0.0) {
+                const double c = dot(U, uk);
+                an += wg * (c * c);
+            }
+        }
+        e[0] += -Ks * an;
+        e[1] += 0.5 * MU0 * dMs * d;
+    }
+    const int slots[2] = {4, 5};
+    block_store<2>(e, slots, part);
+}
+
+// final sums of the partials: out[k] = sum of slot k, k < nslots; slots >= nfirst use nb_b blocks
+__global__ void k_sum_finish(int nslots, int nfirst, int nb_a, int nb_b, const double *__restrict__ part,
+                             double *__restrict__ out)
+{
+    if (threadIdx.x < 64) {
+        for (int k = 0; k < nslots; k++) {
+            const int nb = k < nfirst ? nb_a : nb_b;
+            double s = 0;
+            for (int i = threadIdx.x; i < nb; i += 64) s += part[k * MAX_PART + i];
+            s = wave_sum(s);
+            if (threadIdx.x == 0) out[k] = s;
+        }
+    }
+}
+
+// mesh::max_angle (src/mesh.h:295-306) over the magnetic edges of the stencil pattern
+__global__ __launch_bounds__(BLOCK) void k_max_angle(int N, const int64_t *__restrict__ sptr,
+                                                     const int32_t *__restrict__ swid,
+                                                     const int32_t *__restrict__ scol,
+                                                     const NodeRec *__restrict__ nxt, double *__restrict__ part)
+{
+    double mn = 1.0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x) {
+        const int sl = i >> 6, lane = i & 63;
+        const int64_t off = sptr[sl];
+        const NodeRec a = nxt[i];
+        for (int k = 0; k < swid[sl]; k++) {
+            const int j = scol[off + (int64_t)k * SELL_C + lane];
+            if (j <= i) continue;
+            const NodeRec bq = nxt[j];
+            mn = fmin(mn, a.ux * bq.ux + a.uy * bq.uy + a.uz * bq.uz);
+        }
+    }
+    __shared__ double sh[BLOCK / 64];
+    mn = wave_min(mn);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = mn;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double s = sh[0];
+        for (int q = 1; q < BLOCK / 64; q++) s = fmin(s, sh[q]);
+        part[blockIdx.x] = s;
+    }
+}
+
+__global__ void k_min_finish(int nb, const double *__restrict__ part, double *__restrict__ out)
+{
+    if (threadIdx.x < 64) {
+        double m = 1.0;
+        for (int i = threadIdx.x; i < nb; i += 64) m = fmin(m, part[i]);
+        m = wave_min(m);
+        if (threadIdx.x == 0) out[0] = m;
+    }
+}
+
+// mesh::avg (src/mesh.cpp:89-106): sum_t sum_g w_g U_g (which 0: u, 1: v), slots 0..2
+__global__ __launch_bounds__(BLOCK) void k_average(int Tm, int which, const int4 *__restrict__ tn,
+                                                   const double *__restrict__ geo,
+                                                   const NodeRec *__restrict__ nxt, double *__restrict__ part)
+{
+    double e[3] = {0, 0, 0};
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < Tm; t += gridDim.x * blockDim.x) {
+        const int4 q = tn[t];
+        const int nd[4] = {q.x, q.y, q.z, q.w};
+        d3 u[4];
+        for (int a = 0; a < 4; a++) {
+            const NodeRec n = nxt[nd[a]];
+            u[a] = which ? mk(n.vx, n.vy, n.vz) : mk(n.ux, n.uy, n.uz);
+        }
+        for (int g = 0; g < 5; g++) {
+            const double wg = geo[(size_t)(12 + g) * Tm + t];
+            const d3 U = tet_a(0, g) * u[0] + tet_a(1, g) * u[1] + tet_a(2, g) * u[2] + tet_a(3, g) * u[3];
+            e[0] += wg * U.x;
+            e[1] += wg * U.y;
+            e[2] += wg * U.z;
+        }
+    }
+    const int slots[3] = {0, 1, 2};
+    block_store<3>(e, slots, part);
+}
+
+// Node::evolution (src/node.h:107): CURRENT = NEXT
+__global__ __launch_bounds__(BLOCK) void k_evolution(int N, const NodeRec *__restrict__ nxt, NodeRec *__restrict__ cur)
+{
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x) cur[i] = nxt[i];
+}
+
+// ------------------------------------------------------------------ launchers
+static inline int grid1(int64_t n) { return (int)((n + BLOCK - 1) / BLOCK); }
+
+void launch_element(Ctx &c, const double Hext[3], double dt)
+{
+    const NodeRec *cur = c.rec[c.cur];
+    if (c.Tm > 0) {
+        Prof pr(c, "element");
+        k_element<<<grid1(c.Tm), BLOCK, 0, c.stream>>>(c.Tm, c.tet_nodes, c.geo, c.tet_reg, c.regions, cur, Hext[0],
+                                                       Hext[1], Hext[2], dt, c.elem_out);
+    }
+    if (c.Fk > 0) {
+        Prof pr(c, "tri_aniso");
+        k_tri_aniso<<<grid1(c.Fk), BLOCK, 0, c.stream>>>(c.Fk, c.trik_nodes, c.trik_geo, cur,
+                                                         c.elem_out + (size_t)4 * c.Tm);
+    }
+}
+
+void launch_node_assemble(Ctx &c, double r, double cst, double /*dt*/)
+{
+    const int nb = c.blocks_for(c.N);
+    const NodeRec *vsrc = c.rec[1 - c.cur];  // buildInitGuess reads v NEXT
+    {
+        Prof pr(c, "node_assemble");
+        k_node_assemble<<<nb, BLOCK, 0, c.stream>>>(c.N, cos(r), sin(r), cst, c.rec[c.cur], vsrc, c.mag, c.inc_ptr,
+                                                    c.inc_idx, c.elem_out, c.mass, c.sdiag, c.ep4, c.eq4, c.Dg,
+                                                    c.dprec, c.b, c.x, c.w, c.part);
+    }
+    k_set_rhsn<<<1, 64, 0, c.stream>>>(nb, c.ctl, c.part);
+}
+
+void launch_basis(Ctx &c, double r)
+{
+    k_basis<<<c.blocks_for(c.N), BLOCK, 0, c.stream>>>(c.N, cos(r), sin(r), c.rec[c.cur], c.ep4, c.eq4);
+}
+
+void launch_r0(Ctx &c, double cst)
+{
+    const int nb = c.blocks_for(c.N);
+    Prof pr(c, "spmv_r0");
+    k_r0<<<nb, BLOCK, 0, c.stream>>>(c.N, cst, STENCIL_PASS, c.w, c.x, c.b, c.r, c.rt, c.p, c.ctl, c.part);
+}
+
+void launch_iteration(Ctx &c, double cst)
+{
+    const int nb = c.blocks_for(c.N);
+    {
+        Prof pr(c, "iter_p");
+        k_iter_p<<<nb, BLOCK, 0, c.stream>>>(c.N, nb, c.ctl, c.ep4, c.eq4, c.dprec, c.r, c.v, c.p, c.ph, c.w, c.part);
+    }
+    {
+        Prof pr(c, "spmv");
+        k_spmv_v<<<nb, BLOCK, 0, c.stream>>>(c.N, cst, STENCIL_PASS, c.ctl, c.w, c.ph, c.rt, c.v, c.part);
+    }
+    {
+        Prof pr(c, "iter_s");
+        k_iter_s<<<nb, BLOCK, 0, c.stream>>>(c.N, nb, c.ctl, c.ep4, c.eq4, c.dprec, c.r, c.v, c.s, c.sh, c.w, c.part);
+    }
+    {
+        Prof pr(c, "spmv");
+        k_spmv_t<<<nb, BLOCK, 0, c.stream>>>(c.N, nb, cst, STENCIL_PASS, c.ctl, c.w, c.sh, c.s, c.ph, c.x, c.t, c.part);
+    }
+    {
+        Prof pr(c, "iter_x");
+        k_iter_x<<<nb, BLOCK, 0, c.stream>>>(c.N, nb, c.ctl, c.ph, c.sh, c.s, c.t, c.rt, c.x, c.r, c.part);
+    }
+}
+
+void launch_update(Ctx &c, double dt)
+{
+    const int nb = c.blocks_for(c.N);
+    {
+        Prof pr(c, "update");
+        k_update<<<nb, BLOCK, 0, c.stream>>>(c.N, dt, c.mag, c.x, c.ep4, c.eq4, c.rec[c.cur], c.rec[1 - c.cur], c.part);
+    }
+    k_finish_update<<<1, 64, 0, c.stream>>>(nb, c.ctl, c.part);
+}
+
+void launch_spmv_debug(Ctx &c, double cst, const double2 *xin, double2 *yout)
+{
+    const int nb = c.blocks_for(c.N);
+    k_lift<<<nb, BLOCK, 0, c.stream>>>(c.N, c.ep4, c.eq4, xin, c.w);
+    k_spmv_plain<<<nb, BLOCK, 0, c.stream>>>(c.N, cst, STENCIL_PASS, c.w, xin, yout);
+}
+
+void launch_charges(Ctx &c)
+{
+    const NodeRec *nxt = c.rec[1 - c.cur];
+    if (c.Tm > 0) {
+        Prof pr(c, "charges_tet");
+        k_charges_tet<<<grid1(c.Tm), BLOCK, 0, c.stream>>>(c.Tm, c.tet_nodes, c.geo, c.tet_reg, c.regions, nxt, c.src_q);
+    }
+    if (c.Fm > 0) {
+        Prof pr(c, "charges_tri");
+        k_charges_tri<<<grid1(c.Fm), BLOCK, 0, c.stream>>>(c.Fm, (int64_t)5 * c.Tm, c.tri_nodes, c.tri_geo, c.pos4,
+                                                           nxt, c.src_q, c.tri_corr);
+    }
+    Prof pr(c, "corr_gather");
+    k_corr_gather<<<c.blocks_for(c.N), BLOCK, 0, c.stream>>>(c.N, c.cinc_ptr, c.cinc_idx, c.tri_corr, c.corr);
+}
+
+void launch_direct_p2p(Ctx &c)
+{
+    Prof pr(c, "p2p_direct");
+    k_p2p_direct<<<grid1(c.N), BLOCK, 0, c.stream>>>(c.N, c.S, c.mag, c.pos4, c.src_pos, c.src_q, c.corr,
+                                                     c.rec[1 - c.cur]);
+}
+
+void launch_energy(Ctx &c, const double Hext[3])
+{
+    const NodeRec *nxt = c.rec[1 - c.cur];
+    const int nbt = c.blocks_for(c.Tm), nbf = c.blocks_for(c.Fm);
+    {
+        Prof pr(c, "energy");
+        k_energy_tet<<<nbt, BLOCK, 0, c.stream>>>(c.Tm, c.tet_nodes, c.geo, c.tet_reg, c.regions, nxt, Hext[0], Hext[1],
+                                                  Hext[2], c.epart);
+        if (c.Fm > 0)
+            k_energy_tri<<<nbf, BLOCK, 0, c.stream>>>(c.Fm, c.tri_nodes, c.tri_geo, c.tri_geo + (size_t)12 * c.Fm, nxt,
+                                                      c.epart);
+    }
+    k_sum_finish<<<1, 64, 0, c.stream>>>(6, 4, nbt, c.Fm > 0 ? nbf : 0, c.epart, c.res);
+}
+
+void launch_max_angle(Ctx &c)
+{
+    const int nb = c.blocks_for(c.N);
+    Prof pr(c, "max_angle");
+    k_max_angle<<<nb, BLOCK, 0, c.stream>>>(c.N, c.slice_ptr, c.slice_w, c.scol, c.rec[1 - c.cur], c.epart);
+    k_min_finish<<<1, 64, 0, c.stream>>>(nb, c.epart, c.res + 8);
+}
+
+void launch_evolution(Ctx &c)
+{
+    Prof pr(c, "evolution");
+    k_evolution<<<c.blocks_for(c.N), BLOCK, 0, c.stream>>>(c.N, c.rec[1 - c.cur], c.rec[c.cur]);
+}
+
+void launch_average(Ctx &c, int which)
+{
+    const int nb = c.blocks_for(c.Tm);
+    k_average<<<nb, BLOCK, 0, c.stream>>>(c.Tm, which, c.tet_nodes, c.geo, c.rec[1 - c.cur], c.epart);
+    k_sum_finish<<<1, 64, 0, c.stream>>>(3, 3, nb, 0, c.epart, c.res + 16);
+}
+
+}  // namespace fg

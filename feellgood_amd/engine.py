@@ -1,0 +1,156 @@
+"""Python mirror of the fg_* C-ABI (include/fgmi.h): one Engine per fg_ctx.
+
+The method names follow the reference's LinAlgebra / fmm / Fem calls that they replace
+(base_projection, prepare_elements, solve, calc_demag, energy, evolution, max_angle).
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import native as nv
+
+
+class EngineError(RuntimeError):
+    pass
+
+
+def options(tol=1e-6, max_iter=700, demag="direct", fmm_order=8, fmm_leaf=128, precision="fp64",
+            device=0, profile=False):
+    o = nv.Options()
+    nv.hip_lib().fg_default_options(C.byref(o))
+    o.tol = tol
+    o.max_iter = max_iter
+    o.demag_method = {"direct": nv.FG_DEMAG_DIRECT, "fmm": nv.FG_DEMAG_FMM}[demag]
+    o.fmm_order = fmm_order
+    o.fmm_leaf = fmm_leaf
+    o.precision = {"fp64": nv.FG_PREC_FP64, "mixed": nv.FG_PREC_MIXED}[precision]
+    o.device = device
+    o.profile = 1 if profile else 0
+    return o
+
+
+class Engine:
+    def __init__(self, mesh, opts=None, **kw):
+        self.lib = nv.hip_lib()
+        self.N = mesh.n_nodes
+        self._mesh = mesh
+        o = opts if opts is not None else options(**kw)
+        h = C.c_void_p()
+        rc = self.lib.fg_create(C.byref(mesh), C.byref(o), C.byref(h))
+        if rc != nv.FG_OK:
+            raise EngineError(f"fg_create: {rc}: {self.lib.fg_last_error(None).decode()}")
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self.lib.fg_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _chk(self, rc, what):
+        if rc < 0:
+            raise EngineError(f"{what}: {rc}: {self.lib.fg_last_error(self._h).decode()}")
+        return rc
+
+    def set_state(self, which, u=None, v=None, phi=None, phiv=None):
+        c = lambda a: None if a is None else np.ascontiguousarray(a, dtype=np.float64).reshape(-1)
+        u, v, phi, phiv = c(u), c(v), c(phi), c(phiv)
+        self._chk(self.lib.fg_set_state(self._h, which, nv.dptr(u), nv.dptr(v), nv.dptr(phi), nv.dptr(phiv)),
+                  "fg_set_state")
+
+    def get_state(self, which):
+        u = np.zeros(3 * self.N); v = np.zeros(3 * self.N)
+        phi = np.zeros(self.N); phiv = np.zeros(self.N)
+        self._chk(self.lib.fg_get_state(self._h, which, nv.dptr(u), nv.dptr(v), nv.dptr(phi), nv.dptr(phiv)),
+                  "fg_get_state")
+        return u.reshape(-1, 3), v.reshape(-1, 3), phi, phiv
+
+    def base_projection(self, r):
+        self._chk(self.lib.fg_base_projection(self._h, r), "fg_base_projection")
+
+    def prepare_elements(self, Hext, dt, prefactor):
+        H = np.ascontiguousarray(Hext, dtype=np.float64)
+        self._chk(self.lib.fg_prepare_elements(self._h, nv.dptr(H), dt, prefactor), "fg_prepare_elements")
+
+    def solve(self, dt):
+        """Returns (failed, SolveInfo) like LinAlgebra::solve's bool (true = failure)."""
+        info = nv.SolveInfo()
+        rc = self._chk(self.lib.fg_solve(self._h, dt, C.byref(info)), "fg_solve")
+        return rc == nv.FG_SOLVE_FAILED, info
+
+    def calc_demag(self):
+        self._chk(self.lib.fg_calc_demag(self._h), "fg_calc_demag")
+
+    def energy(self, Hext):
+        H = np.ascontiguousarray(Hext, dtype=np.float64)
+        E = np.zeros(4)
+        self._chk(self.lib.fg_energy(self._h, nv.dptr(H), nv.dptr(E)), "fg_energy")
+        return E
+
+    def evolution(self):
+        self._chk(self.lib.fg_evolution(self._h), "fg_evolution")
+
+    def max_angle(self):
+        a = C.c_double()
+        self._chk(self.lib.fg_max_angle(self._h, C.byref(a)), "fg_max_angle")
+        return a.value
+
+    def average(self, which=0):
+        out = np.zeros(3)
+        self._chk(self.lib.fg_average(self._h, which, nv.dptr(out)), "fg_average")
+        return out
+
+    def basis(self):
+        ep = np.zeros(3 * self.N); eq = np.zeros(3 * self.N)
+        self._chk(self.lib.fg_get_basis(self._h, nv.dptr(ep), nv.dptr(eq)), "fg_get_basis")
+        return ep.reshape(-1, 3), eq.reshape(-1, 3)
+
+    def system(self):
+        rows, nnz = C.c_int64(), C.c_int64()
+        self._chk(self.lib.fg_system_size(self._h, C.byref(rows), C.byref(nnz)), "fg_system_size")
+        rp = np.zeros(rows.value + 1, np.int64); col = np.zeros(nnz.value, np.int32)
+        val = np.zeros(nnz.value); rhs = np.zeros(rows.value)
+        self._chk(self.lib.fg_get_system(self._h, nv.lptr(rp), nv.iptr(col), nv.dptr(val), nv.dptr(rhs)),
+                  "fg_get_system")
+        return rp, col, val, rhs
+
+    def solution(self):
+        x = np.zeros(2 * self.N)
+        self._chk(self.lib.fg_get_solution(self._h, nv.dptr(x)), "fg_get_solution")
+        return x
+
+    def synchronize(self):
+        self._chk(self.lib.fg_synchronize(self._h), "fg_synchronize")
+
+    def kernel_stats(self):
+        out = {}
+        k = 0
+        name = C.create_string_buffer(64)
+        n = C.c_int64()
+        ms = C.c_double()
+        while self.lib.fg_kernel_stats(self._h, k, name, 64, C.byref(n), C.byref(ms)) == nv.FG_OK:
+            out[name.value.decode()] = (n.value, ms.value)
+            k += 1
+        return out
+
+    def reset_kernel_stats(self):
+        self.lib.fg_reset_kernel_stats(self._h)
+
+    def step(self, r, Hext, dt, prefactor):
+        """One LLG step of the reference loop (src/time_integration.cpp:193-231) at fixed dt:
+        base_projection, prepare_elements, solve, then compute_all (demag, energy, evolution).
+        Returns (failed, SolveInfo, E)."""
+        self.base_projection(r)
+        self.prepare_elements(Hext, dt, prefactor)
+        failed, info = self.solve(dt)
+        if failed:
+            return True, info, None
+        self.calc_demag()
+        E = self.energy(Hext)
+        self.evolution()
+        return False, info, E
