@@ -1,0 +1,320 @@
+#!/usr/bin/env python3
+"""LLG steps/s of the gfx950 engine on the 2M-tetrahedron film (one MI355X).
+
+Workload (the configuration the headline metric is quoted on): thin film 1000 x 500 x 20 nm,
+333 x 167 x 6 cells split into 6 tetrahedra each (2,001,996 tets, 392,784 nodes), interior node
+coordinates jittered by +/-10% of the cell size (seed 2); Ms = 8e5 A/m, A = 1.3e-11 J/m,
+alpha = 0.02; S-state m = (cos p, sin p, 0), p = 0.3 sin(pi x / L); B = 25 mT at 170 deg in the
+plane; fixed dt = 5e-14 s; BiCGSTAB tolerance 1e-10.
+
+One step = the reference's hot loop body (src/time_integration.cpp:193-244): base_projection,
+prepareElements, solve (assembly + BiCGSTAB + update), compute_all (demag u and v, energies,
+evolution) and max_angle. Inputs are resident in HBM before the timed region.
+
+Prints one JSON line (see the keys below). Options: --gpus N (only 1 is supported), --steps K,
+--warmup W, --demag {fmm,direct}, --precision {fp64,mixed}, --no-cpu, --small (the 48k-tet cube).
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "LLG steps/s at 2M tets: learned+mixed vs plain FP64 GPU at |dm|<=1e-6"
+MU0 = 1.25663706127e-6
+HBM_PEAK_GBS = 8000.0
+FP64_VALU_TFLOPS = 78.6
+FP32_VALU_TFLOPS = 157.3
+
+
+def film_workload():
+    return dict(name="film", lengths=(1000.0, 500.0, 20.0), cells=(333, 167, 6), jitter=0.1, seed=2,
+                Ms=8e5, Ae=1.3e-11, alpha=0.02, B=0.025, angle=170.0, dt=5e-14, tol=1e-10)
+
+
+def cube_workload():
+    return dict(name="cube", lengths=(100.0, 100.0, 100.0), cells=(20, 20, 20), jitter=0.0, seed=1,
+                Ms=8e5, Ae=1.3e-11, alpha=0.5, B=0.0, angle=0.0, dt=1e-14, tol=1e-10)
+
+
+def build_mesh(w):
+    from feellgood_amd.mesh import HostMesh
+    m = HostMesh.cuboid(w["lengths"], w["cells"], jitter=w["jitter"], seed=w["seed"])
+    m.prepare({"volume": dict(Ms=w["Ms"], Ae=w["Ae"], alpha=w["alpha"])})
+    return m
+
+
+def initial_state(m, w):
+    pos = m.arrays()["pos"]
+    if w["name"] == "film":
+        L = w["lengths"][0] * 1e-9
+        ph = 0.3 * np.sin(np.pi * pos[:, 0] / L)
+        u = np.stack([np.cos(ph), np.sin(ph), np.zeros_like(ph)], axis=1)
+    else:
+        rng = np.random.default_rng(w["seed"])
+        u = np.array([1.0, 1.0, 0.0]) / math.sqrt(2) + rng.uniform(-0.05, 0.05, size=(m.n_nodes, 3))
+        u /= np.linalg.norm(u, axis=1, keepdims=True)
+    return u
+
+
+def field(w):
+    a = math.radians(w["angle"])
+    return np.array([math.cos(a), math.sin(a), 0.0]) * w["B"] / MU0
+
+
+def prefactor(dt):
+    # timing::set_dt with dt_min = dt_max = dt (src/time_integration.h:37-42)
+    tt = dt / (100.0 * dt)
+    return 1.0 + tt * abs(math.log(tt))
+
+
+# ------------------------------------------------------------------ byte / flop models
+def kernel_model(name, n):
+    """(bound, algorithmic bytes or flops per launch) of the engine's kernels; n holds the sizes."""
+    N, T, nnz, F, S = n["N"], n["T"], n["nnz"], n["F"], n["S"]
+    models = {
+        "spmv": ("hbm", 12 * nnz + 180 * N),
+        "spmv_r0": ("hbm", 12 * nnz + 180 * N),
+        "iter_p": ("hbm", 192 * N),
+        "iter_s": ("hbm", 176 * N),
+        "iter_x": ("hbm", 128 * N),
+        "node_assemble": ("hbm", 332 * N + 144 * T),
+        "element": ("hbm", 284 * T + 64 * N),
+        "update": ("hbm", 164 * N),
+        "evolution": ("hbm", 128 * N),
+        "charges_tet": ("hbm", 16 * T + 96 * T + 4 * T + 80 * T + 64 * N),
+        "charges_tri": ("hbm", 16 * F + 128 * F + 64 * F + 48 * F + 96 * N),
+        "corr_gather": ("hbm", 8 * N + 4 * 3 * F + 16 * 3 * F + 16 * N),
+        "energy": ("hbm", 16 * T + 136 * T + 4 * T + 64 * N),
+        "max_angle": ("hbm", 4 * nnz + 64 * N),
+        "p2p_direct": ("valu64", 2 * 10.0 * N * S),
+    }
+    return models.get(name)
+
+
+def run_gpu(args, w, m, u0):
+    from feellgood_amd import native as nv
+    from feellgood_amd.engine import Engine, options
+    opts = options(tol=w["tol"], max_iter=10000, demag=args.demag, fmm_order=args.fmm_order,
+                   fmm_leaf=args.fmm_leaf, precision=args.precision, profile=False)
+    g = Engine(m.mesh, opts)
+    H = field(w)
+    dt = w["dt"]
+    pref = prefactor(dt)
+    rng = np.random.default_rng(w["seed"])
+    # compute_all at t0 (src/time_integration.cpp:136)
+    g.set_state(nv.FG_BOTH, u=u0)
+    g.calc_demag()
+    g.energy(H)
+    g.evolution()
+    iters = []
+    fails = 0
+
+    def step():
+        nonlocal fails
+        g.base_projection((2 / math.pi) * rng.uniform())
+        g.prepare_elements(H, dt, pref)
+        failed, info = g.solve(dt)
+        if failed:
+            fails += 1
+            return info
+        g.calc_demag()
+        g.energy(H)
+        g.evolution()
+        g.max_angle()
+        iters.append(info.iterations)
+        return info
+
+    for _ in range(args.warmup):
+        step()
+    iters.clear()
+    g.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    g.synchronize()
+    t1 = time.perf_counter()
+    clean_ms = (t1 - t0) * 1e3 / args.steps
+    clean_iters = list(iters)
+    # profiled pass (HIP events on the engine stream around every kernel)
+    prof_steps = max(1, min(args.steps, args.prof_steps))
+    g.close()
+    opts.profile = 1
+    g = Engine(m.mesh, opts)
+    g.set_state(nv.FG_BOTH, u=u0)
+    g.calc_demag()
+    g.energy(H)
+    g.evolution()
+    step()
+    g.reset_kernel_stats()
+    iters.clear()
+    g.synchronize()
+    tp0 = time.perf_counter()
+    for _ in range(prof_steps):
+        step()
+    g.synchronize()
+    tp1 = time.perf_counter()
+    stats = g.kernel_stats()
+    g.close()
+    return dict(ms_per_step=clean_ms, iters=clean_iters, fails=fails, stats=stats,
+                prof_ms_per_step=(tp1 - tp0) * 1e3 / prof_steps, prof_steps=prof_steps)
+
+
+def roofline(stats, sizes):
+    best = None
+    for name, (launches, ms) in stats.items():
+        if launches == 0:
+            continue
+        if best is None or ms > best[2]:
+            best = (name, launches, ms)
+    if best is None:
+        return None, {}
+    per_kernel = {}
+    for name, (launches, ms) in stats.items():
+        mdl = kernel_model(name, sizes)
+        entry = {"launches": launches, "total_ms": round(ms, 4), "avg_us": round(1e3 * ms / max(launches, 1), 3)}
+        if mdl:
+            bound, work = mdl
+            avg_s = ms / 1e3 / max(launches, 1)
+            if bound == "hbm":
+                entry["GBps"] = round(work / avg_s / 1e9, 1)
+                entry["frac_hbm"] = round(work / avg_s / 1e9 / HBM_PEAK_GBS, 4)
+            else:
+                entry["TFLOPs"] = round(work / avg_s / 1e12, 2)
+        per_kernel[name] = entry
+    name, launches, ms = best
+    mdl = kernel_model(name, sizes)
+    if mdl is None:
+        return {"kernel": name, "bound": None}, per_kernel
+    bound, work = mdl
+    avg_s = ms / 1e3 / launches
+    if bound == "hbm":
+        achieved = work / avg_s / 1e9
+        rl = {"kernel": name, "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+              "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "bytes_per_launch": int(work),
+              "avg_launch_us": round(avg_s * 1e6, 3)}
+    else:
+        achieved = work / avg_s / 1e12
+        rl = {"kernel": name, "bound": "valu", "achieved": round(achieved, 2), "peak": FP64_VALU_TFLOPS,
+              "unit": "TFLOP/s", "frac": round(achieved / FP64_VALU_TFLOPS, 4), "traffic": None,
+              "flops_per_launch": work, "avg_launch_us": round(avg_s * 1e6, 3)}
+    return rl, per_kernel
+
+
+def cpu_baseline(w, m, u0, n_targets=256):
+    """The FP64 oracle on the host cores: one full step of element pass, assembly, BiCGSTAB, update
+    and energies on the whole mesh, plus the exact (direct-sum) demag of u and v evaluated for
+    n_targets nodes and scaled to all nodes."""
+    import oracle
+    from feellgood_amd import native as nv
+    cores = os.cpu_count() or 1
+    t0 = time.perf_counter()
+    p = oracle.Problem(m.mesh, tol=w["tol"], max_iter=10000)
+    p.set_state(nv.FG_BOTH, u=u0)
+    t_create = time.perf_counter() - t0
+    H = field(w)
+    dt = w["dt"]
+    t1 = time.perf_counter()
+    p.base_projection(0.3)
+    p.prepare_elements(H, dt, prefactor(dt))
+    rc, info = p.solve(dt)
+    p.energy(H)
+    p.evolution()
+    p.max_angle()
+    t_step = time.perf_counter() - t1
+    targets = np.arange(0, m.n_nodes, max(1, m.n_nodes // n_targets), dtype=np.int32)[:n_targets]
+    t2 = time.perf_counter()
+    p.demag_sample(targets)
+    t_sample = time.perf_counter() - t2
+    S = 5 * m.n_tets + 4 * m.n_tris
+    t_demag = t_sample * m.n_nodes / len(targets)
+    per_step = t_step + t_demag
+    return {"value": round(1.0 / per_step, 8), "unit": "steps/s", "cores": cores, "kind": "port",
+            "sample": (f"FP64 oracle with OpenMP on {cores} host threads: one full step of element pass, "
+                       f"assembly, BiCGSTAB ({info.iterations} iterations), update, energies and max_angle "
+                       f"on the whole mesh ({t_step:.2f} s) plus the direct-sum demag of u and v (charges "
+                       f"for all {S} sources, potentials at {len(targets)} of {m.n_nodes} nodes in "
+                       f"{t_sample:.2f} s, scaled to all nodes: {t_demag:.1f} s); setup {t_create:.1f} s "
+                       f"excluded"),
+            "seconds_per_step": round(per_step, 3)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--demag", default="direct", choices=["fmm", "direct"])
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
+    ap.add_argument("--fmm-order", type=int, default=8)
+    ap.add_argument("--fmm-leaf", type=int, default=128)
+    ap.add_argument("--prof-steps", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--small", action="store_true", help="the 48k-tet cube instead of the film")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this engine runs on one GPU; --gpus {args.gpus} is not supported "
+              "(the multi-GPU sharding is a design in DESIGN.md, not an implementation)", file=sys.stderr)
+        sys.exit(2)
+    w = cube_workload() if args.small else film_workload()
+    t0 = time.perf_counter()
+    m = build_mesh(w)
+    u0 = initial_state(m, w)
+    print(f"[bench] mesh {m.n_nodes} nodes {m.n_tets} tets {m.n_tris} tris built in "
+          f"{time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
+    res = run_gpu(args, w, m, u0)
+    print(f"[bench] gpu {res['ms_per_step']:.2f} ms/step, iterations {res['iters']}", file=sys.stderr, flush=True)
+    sizes = dict(N=m.n_nodes, T=m.n_tets, F=m.n_tris, S=5 * m.n_tets + 4 * m.n_tris)
+    # stencil entries N + 2E: E from the Kuhn grid connectivity is not needed exactly; use the
+    # engine's own count
+    from feellgood_amd.engine import Engine, options
+    import ctypes as C
+    g = Engine(m.mesh, options(tol=w["tol"]))
+    rows, nz = C.c_int64(), C.c_int64()
+    g.lib.fg_system_size(g._h, C.byref(rows), C.byref(nz))
+    g.close()
+    sizes["nnz"] = nz.value // 4
+    rl, per_kernel = roofline(res["stats"], sizes)
+    out = {
+        "metric": METRIC,
+        "value": round(1e3 / res["ms_per_step"], 4),
+        "unit": "steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(res["ms_per_step"], 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if args.precision == "fp64" else "f32+f64",
+        "data": "synthetic",
+        "config": {"workload": (f"{w['name']} {w['lengths'][0]:g}x{w['lengths'][1]:g}x{w['lengths'][2]:g} nm, "
+                                f"{w['cells'][0]}x{w['cells'][1]}x{w['cells'][2]} cells x 6 tets "
+                                f"({m.n_tets} tets, {m.n_nodes} nodes, {m.n_tris} surface triangles), "
+                                f"jitter {w['jitter']}, dt {w['dt']:g} s, tol {w['tol']:g}"),
+                   "mode": f"plain {args.precision}" if args.precision == "fp64" else "mixed",
+                   "initial_guess": "previous v (reference buildInitGuess)",
+                   "demag": args.demag, "fmm_order": args.fmm_order if args.demag == "fmm" else None,
+                   "parallelism": "single GPU"},
+        "bicgstab_iterations_per_step": (round(float(np.mean(res["iters"])), 2) if res["iters"] else None),
+        "solver_failures": res["fails"],
+        "roofline": rl,
+        "kernels": per_kernel,
+        "profiled_ms_per_step": round(res["prof_ms_per_step"], 4),
+    }
+    if not args.no_cpu:
+        try:
+            out["cpu_baseline"] = cpu_baseline(w, m, u0)
+        except Exception as e:  # the baseline must not hide the GPU measurement
+            out["cpu_baseline"] = {"error": str(e)}
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -50,6 +50,7 @@ SYMBOLS = {
     "orc_get_solution": (None, [C.c_void_p, _d]),
     "orc_get_tet_element": (None, [C.c_void_p, C.c_int, _d, _d]),
     "orc_calc_demag": (None, [C.c_void_p]),
+    "orc_demag_sample": (None, [C.c_void_p, _i32, C.c_int, _d, _d]),
     "orc_n_sources": (C.c_int64, [C.c_void_p]),
     "orc_get_sources": (None, [C.c_void_p, _d, _d, _d]),
     "orc_get_corrections": (None, [C.c_void_p, _d, _d]),
@@ -185,6 +186,12 @@ class Problem:
 
     def calc_demag(self):
         lib().orc_calc_demag(self._h)
+
+    def demag_sample(self, targets):
+        t = np.ascontiguousarray(targets, dtype=np.int32)
+        phi = np.zeros(len(t)); phiv = np.zeros(len(t))
+        lib().orc_demag_sample(self._h, t.ctypes.data_as(_i32), len(t), d(phi), d(phiv))
+        return phi, phiv
 
     def sources(self):
         n = lib().orc_n_sources(self._h)

@@ -841,7 +841,7 @@ void orc_get_tet_element(const orc_problem *p, int k, double *Kp, double *Lp)
 namespace {
 /* fmm::calc_charges + the exact potential (src/fmm_demag.h:155-224, src/tetra.cpp:347-359,
  * src/triangle.cpp:45-78): phi_i = (sum_s q_s / |x_i - y_s| + corr_i) / (4 pi) */
-void demag_one(orc_problem *p, bool of_v)
+void demag_charges(orc_problem *p, bool of_v)
 {
     const int nt = (int)p->magTet.size(), nf = (int)p->magTri.size();
     const size_t S = (size_t)TNPI * nt + (size_t)FNPI * nf;
@@ -907,19 +907,31 @@ void demag_one(orc_problem *p, bool of_v)
             corr[f.ind[i]] += tri_potential(P, s, f.surf, f.dMs, i);
         }
     }
-    const int N = p->N;
+}
+
+double direct_potential(const orc_problem *p, bool of_v, int i)
+{
+    const std::vector<double> &q = of_v ? p->src_qv : p->src_qu;
+    const std::vector<double> &corr = of_v ? p->corr_v : p->corr_u;
+    const V3 x = p->node[i].p;
     const double *sx = p->src_xyz.data();
-    const double *qq = q.data();
+    const size_t S = q.size();
+    double pot = 0;
+    for (size_t s2 = 0; s2 < S; s2++) {
+        double dx = x.x - sx[3 * s2], dy = x.y - sx[3 * s2 + 1], dz = x.z - sx[3 * s2 + 2];
+        pot += q[s2] / std::sqrt(dx * dx + dy * dy + dz * dz);
+    }
+    return (pot + corr[i]) / (4 * M_PI);
+}
+
+void demag_one(orc_problem *p, bool of_v)
+{
+    demag_charges(p, of_v);
+    const int N = p->N;
 #pragma omp parallel for schedule(dynamic, 64)
     for (int i = 0; i < N; i++) {
         if (!p->magNode[i]) continue;
-        const V3 x = p->node[i].p;
-        double pot = 0;
-        for (size_t s2 = 0; s2 < S; s2++) {
-            double dx = x.x - sx[3 * s2], dy = x.y - sx[3 * s2 + 1], dz = x.z - sx[3 * s2 + 2];
-            pot += qq[s2] / std::sqrt(dx * dx + dy * dy + dz * dz);
-        }
-        double val = (pot + corr[i]) / (4 * M_PI);
+        double val = direct_potential(p, of_v, i);
         if (of_v) p->node[i].d[1].phiv = val;
         else p->node[i].d[1].phi = val;
     }
@@ -933,6 +945,19 @@ void orc_calc_demag(orc_problem *p)
 {
     demag_one(p, false);
     demag_one(p, true);
+}
+
+/* The charges of both potentials and the exact potentials at n given targets only (CPU-baseline
+ * sample of the demag cost); the node state is not modified. */
+void orc_demag_sample(orc_problem *p, const int32_t *targets, int n, double *phi, double *phiv)
+{
+    demag_charges(p, false);
+    demag_charges(p, true);
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int k = 0; k < n; k++) {
+        phi[k] = direct_potential(p, false, targets[k]);
+        phiv[k] = direct_potential(p, true, targets[k]);
+    }
 }
 
 int64_t orc_n_sources(const orc_problem *p) { return (int64_t)p->src_qu.size(); }

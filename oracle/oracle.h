@@ -55,6 +55,8 @@ void orc_get_solution(const orc_problem *p, double *x);
 /* Element matrices of tet k after orc_prepare_elements: Kp[64] row-major, Lp[8]. */
 void orc_get_tet_element(const orc_problem *p, int k, double *Kp, double *Lp);
 void orc_calc_demag(orc_problem *p);
+/* Charges of u and v NEXT and the exact potentials at n targets only (state untouched). */
+void orc_demag_sample(orc_problem *p, const int32_t *targets, int n, double *phi, double *phiv);
 /* Charges of the last demag call: sources [n_src] (tets then tris), corrections [n_nodes]. */
 int64_t orc_n_sources(const orc_problem *p);
 void orc_get_sources(const orc_problem *p, double *xyz, double *q_u, double *q_v);
