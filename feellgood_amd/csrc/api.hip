@@ -423,6 +423,10 @@ void build_plan(Ctx &c, const fg_mesh *m)
     c.src_pos = dalloc<double4>(c.S);
     upload(c.src_pos, spos, st);
     c.src_q = dalloc<double2>(c.S);
+    std::vector<int64_t> rank(c.S);
+    for (int64_t k = 0; k < c.S; k++) rank[k] = k;
+    c.src_rank = dalloc<int64_t>(c.S);
+    upload(c.src_rank, rank, st);
     HIPCHK(hipStreamSynchronize(st));
 }
 
@@ -432,7 +436,7 @@ void free_all(Ctx &c)
                     c.rt, c.p, c.ph, c.v, c.s, c.sh, c.t, c.w, c.ctl, c.part, c.epart, c.res, c.slice_ptr, c.slice_w,
                     c.scol, c.sval, c.tet_nodes, c.geo, c.tet_reg, c.regions, c.elem_out, c.inc_ptr, c.inc_idx,
                     c.trik_nodes, c.trik_geo, c.tri_nodes, c.tri_geo, c.tri_corr, c.cinc_ptr, c.cinc_idx, c.corr,
-                    c.src_pos, c.src_q, c.src_pos_f};
+                    c.src_pos, c.src_q, c.src_pos_f, c.src_rank};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     if (c.h_ctl) (void)hipHostFree(c.h_ctl);

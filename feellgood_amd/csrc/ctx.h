@@ -86,10 +86,12 @@ struct Ctx {
     double2 *corr = nullptr;     // [N]
 
     // demag sources
-    double4 *src_pos = nullptr;  // [S]
+    double4 *src_pos = nullptr;  // [S] in tree order when the FMM is used
+    int64_t *src_rank = nullptr; // [S] source index (5 per magnetic tet, 4 per charged tri) -> slot
     double2 *src_q = nullptr;    // [S] (q from u, q from v)
     float4 *src_pos_f = nullptr; // FP32 copies for the mixed-precision P2P
     FmmPlan *fmm = nullptr;
+    double *fmm_W = nullptr;     // cell width per tree level
 
     // energies / reductions
     double *epart = nullptr;  // [8][MAX_PART]

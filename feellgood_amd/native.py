@@ -84,6 +84,8 @@ HIP_SYMBOLS = {
     "fg_kernel_stats": (C.c_int, [C.c_void_p, C.c_int, C.c_char_p, C.c_int, _i64, _d]),
     "fg_reset_kernel_stats": (C.c_int, [C.c_void_p]),
     "fg_synchronize": (C.c_int, [C.c_void_p]),
+    "fg_fmm_reference": (C.c_int, [_d, C.c_int64, _d, C.c_int64, _d, _d, C.c_int, C.c_int, _d, _d,
+                                   C.POINTER(C.c_int)]),
     "fg_last_error": (C.c_char_p, [C.c_void_p]),
 }
 

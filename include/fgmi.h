@@ -183,6 +183,12 @@ int fg_get_solution(fg_ctx *ctx, double *x);
  * kernel k. Returns FG_ERR_RANGE past the last kernel. */
 int fg_kernel_stats(fg_ctx *ctx, int k, char *name, int name_len, int64_t *launches, double *ms);
 int fg_reset_kernel_stats(fg_ctx *ctx);
+/* Host (CPU) evaluation of the same octree FMM as the device pass: potentials sum_s q_s/|x_t - y_s|
+ * at T targets from S sources (xyz arrays of 3*n doubles), for two charge sets. No GPU needed;
+ * used to check the device pass and the expansion order. leaf_level receives the tree depth. */
+int fg_fmm_reference(const double *src_xyz, int64_t S, const double *tgt_xyz, int64_t T,
+                     const double *q_u, const double *q_v, int order, int leaf, double *pot_u,
+                     double *pot_v, int *leaf_level);
 /* Wait for all work queued on the context's stream. */
 int fg_synchronize(fg_ctx *ctx);
 /* Last error message of this context (or of fg_create when ctx is NULL). */

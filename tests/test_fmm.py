@@ -1,0 +1,132 @@
+"""The octree FMM of the demag potential.
+
+CPU: the host evaluation of the FMM (fg_fmm_reference, the same plan and operators as the device
+pass) against exact direct sums of the oracle's charges: the error falls with the expansion order
+and is bounded at each order.
+GPU: the device FMM against the host FMM (same algorithm: 1e-10) and against the oracle's exact
+potentials (bounded by the truncation error), then one LLG step with FMM demag against the exact
+oracle step (|dm| <= 1e-6).
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import oracle
+from feellgood_amd import native as nv
+from feellgood_amd.mesh import HostMesh
+
+
+def film(nx=20, ny=10, nz=3, seed=3):
+    m = HostMesh.cuboid((10.0 * nx, 10.0 * ny, 10.0 * nz), (nx, ny, nz), jitter=0.1, seed=seed)
+    m.prepare({"volume": dict(Ms=8e5, Ae=1.3e-11, alpha=0.02)})
+    return m
+
+
+def charged(m, seed=1):
+    p = oracle.Problem(m.mesh)
+    rng = np.random.default_rng(seed)
+    u = rng.normal(size=(m.n_nodes, 3)) * 0.3 + [1, 0.5, 0]
+    u /= np.linalg.norm(u, axis=1, keepdims=True)
+    v = rng.normal(size=(m.n_nodes, 3)) * 1e9
+    p.set_state(nv.FG_BOTH, u=u, v=v)
+    p.calc_demag()
+    return p, u, v
+
+
+def fmm_ref(xyz, qu, qv, tgt, order, leaf):
+    pu = np.zeros(len(tgt))
+    pv = np.zeros(len(tgt))
+    L = C.c_int()
+    rc = nv.hip_lib().fg_fmm_reference(nv.dptr(np.ascontiguousarray(xyz.reshape(-1))), len(qu),
+                                       nv.dptr(np.ascontiguousarray(tgt.reshape(-1))), len(tgt),
+                                       nv.dptr(qu), nv.dptr(qv), order, leaf, nv.dptr(pu), nv.dptr(pv),
+                                       C.byref(L))
+    assert rc == 0
+    return pu, pv, L.value
+
+
+def test_host_fmm_converges_with_order():
+    m = film()
+    p, _, _ = charged(m)
+    xyz, qu, qv = p.sources()
+    cu, cv = p.corrections()
+    _, _, phi, phiv = p.get_state(nv.FG_NEXT)
+    tgt = m.arrays()["pos"]
+    exact = (phi * 4 * np.pi - cu, phiv * 4 * np.pi - cv)
+    errs = []
+    for order in (4, 6, 8):
+        pu, pv, L = fmm_ref(xyz, qu, qv, tgt, order, 64)
+        assert L >= 3
+        e = max(np.abs(pu - exact[0]).max() / np.abs(exact[0]).max(),
+                np.abs(pv - exact[1]).max() / np.abs(exact[1]).max())
+        errs.append(e)
+    assert errs[0] < 1e-2 and errs[1] < 3e-3 and errs[2] < 6e-4
+    assert errs[2] < errs[1] < errs[0]
+
+
+def test_host_fmm_point_charges():
+    """Random point charges in a cube: FMM vs brute force at orders 4..10 (exact tail bound)."""
+    rng = np.random.default_rng(5)
+    src = rng.uniform(-1, 1, size=(4000, 3))
+    tgt = rng.uniform(-1, 1, size=(300, 3))
+    qu = rng.normal(size=4000)
+    qv = rng.uniform(0, 1, size=4000)
+    r = np.linalg.norm(tgt[:, None, :] - src[None, :, :], axis=2)
+    ex_u = (qu / r).sum(axis=1)
+    ex_v = (qv / r).sum(axis=1)
+    prev = 1.0
+    for order in (4, 6, 8, 10):
+        pu, pv, L = fmm_ref(src, qu, qv, tgt, order, 40)
+        e = max(np.abs(pu - ex_u).max() / np.abs(ex_u).max(), np.abs(pv - ex_v).max() / np.abs(ex_v).max())
+        assert e < prev
+        prev = e
+    assert prev < 1e-5
+
+
+@pytest.mark.gpu
+def test_device_fmm_matches_host_fmm_and_direct():
+    from feellgood_amd.engine import Engine
+    m = film()
+    p, u, v = charged(m)
+    xyz, qu, qv = p.sources()
+    cu, cv = p.corrections()
+    _, _, phi0, phiv0 = p.get_state(nv.FG_NEXT)
+    tgt = m.arrays()["pos"]
+    for order, leaf in ((8, 64), (6, 128), (10, 32)):
+        g = Engine(m.mesh, demag="fmm", fmm_order=order, fmm_leaf=leaf)
+        g.set_state(nv.FG_BOTH, u=u, v=v)
+        g.calc_demag()
+        _, _, phi1, phiv1 = g.get_state(nv.FG_NEXT)
+        pu, pv, _ = fmm_ref(xyz, qu, qv, tgt, order, leaf)
+        ref_u = (pu + cu) / (4 * np.pi)
+        ref_v = (pv + cv) / (4 * np.pi)
+        assert np.abs(phi1 - ref_u).max() <= 1e-10 * np.abs(ref_u).max(), (order, leaf)
+        assert np.abs(phiv1 - ref_v).max() <= 1e-10 * np.abs(ref_v).max(), (order, leaf)
+        err = np.abs(phi1 - phi0).max() / np.abs(phi0).max()
+        assert err < {6: 3e-3, 8: 6e-4, 10: 2e-4}[order], (order, err)
+        g.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("order", [8, 10])
+def test_step_with_fmm_against_exact_oracle(order):
+    """One LLG step from the same state, FMM demag vs the oracle's exact demag: |dm| <= 1e-6."""
+    from feellgood_amd.engine import Engine
+    m = film(24, 12, 3)
+    _, u, _ = charged(m)
+    v = np.zeros_like(u)
+    H = np.array([-0.025, 0.0044, 0.0]) / nv.MU0
+    dt = 5e-14
+    pref = 1 + 0.01 * abs(np.log(0.01))
+    p = oracle.Problem(m.mesh, tol=1e-10, max_iter=1000)
+    g = Engine(m.mesh, tol=1e-10, max_iter=1000, demag="fmm", fmm_order=order, fmm_leaf=64)
+    for o in (p, g):
+        o.set_state(nv.FG_BOTH, u=u, v=v)
+        o.calc_demag()
+        o.evolution()
+        o.base_projection(0.3)
+        o.prepare_elements(H, dt, pref)
+        o.solve(dt)
+    dm = np.abs(g.get_state(nv.FG_NEXT)[0] - p.get_state(nv.FG_NEXT)[0]).max()
+    assert dm <= 1e-6, dm
