@@ -90,6 +90,8 @@ struct Ctx {
     int64_t *src_rank = nullptr; // [S] source index (5 per magnetic tet, 4 per charged tri) -> slot
     double2 *src_q = nullptr;    // [S] (q from u, q from v)
     float4 *src_pos_f = nullptr; // FP32 copies for the mixed-precision P2P
+    double2 *qe = nullptr;       // [Tm + 3 Fm] element charge values for the FMM near-field operator:
+                                 // -Ms div u per tetrahedron, u_a . n per triangle vertex (u, v)
     FmmPlan *fmm = nullptr;
     double *fmm_W = nullptr;     // cell width per tree level
 

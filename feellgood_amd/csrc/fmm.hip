@@ -8,8 +8,9 @@
 // neighbours that are not neighbours) and near lists (adjacent leaves), the irregular-harmonic
 // table of the 316 M2L offsets and the regular table of the 8 child offsets.
 //
-// Device pass per call: P2M (leaves) -> M2M (upward) -> M2L (every level) -> L2L (downward) ->
-// L2P + P2P fused per target.
+// Device pass per call: near field (a precomputed sparse operator on the element charge values)
+// -> P2M (leaves) -> M2M (upward) -> M2L (levels 2..L-2) -> L2L (downward) -> per group of targets
+// of a level-(L-1) cell: M2P of the cells of levels L-1 and L in the far field, and L2P.
 #include <algorithm>
 #include <cmath>
 #include <cstdlib>
@@ -36,9 +37,15 @@ struct FmmPlan {
     std::vector<int32_t> il_ptr, il_src, il_off;  // interaction lists (target cells)
     std::vector<int32_t> leaf_sb, leaf_se, leaf_tb, leaf_te;  // ranges per leaf (index - lev_off[L])
     std::vector<int32_t> near_ptr, near_leaf;  // per leaf: source leaves adjacent (leaf indices)
-    int ls = 3;                        // first level evaluated multipole-to-target (M2P)
-    std::vector<int32_t> m2p_ptr, m2p_cell;  // per leaf: source cells of the ancestors' lists, levels >= ls
-    std::vector<int32_t> lcell;        // per leaf: ancestor at level ls-1 holding the local expansion, or -1
+    // Evaluation groups: the targets of one level-(L-1) cell. M2L/L2L run on levels 2..ls-1, the
+    // group's targets evaluate the local expansion of its level-(L-2) ancestor (lcell) and the
+    // multipoles of its source list (the cell's interaction list and the children of its
+    // neighbours) directly; u_mask bit o says whether a list entry applies to the targets in child
+    // octant o (entries adjacent to that child belong to the near field).
+    int ls = 2;
+    std::vector<int32_t> grp_tb, grp_te, grp_lcell, u_ptr, u_cell;
+    std::vector<uint8_t> u_mask, tgt_oct;
+    std::vector<int32_t> tgt_grp;
     // sorted points
     std::vector<int64_t> src_rank;    // original source index -> sorted position
     std::vector<double4> src_sorted;
@@ -52,11 +59,20 @@ struct FmmPlan {
     double4 *d_cen = nullptr;
     int32_t *d_parent = nullptr, *d_oct = nullptr, *d_ch_ptr = nullptr, *d_ch_idx = nullptr;
     int32_t *d_il_ptr = nullptr, *d_il_src = nullptr, *d_il_off = nullptr;
-    int32_t *d_leaf_sb = nullptr, *d_leaf_se = nullptr, *d_near_ptr = nullptr, *d_near_leaf = nullptr;
-    int32_t *d_m2p_ptr = nullptr, *d_m2p_cell = nullptr, *d_lcell = nullptr;
-    int32_t *d_tgt_node = nullptr, *d_tgt_leaf = nullptr;
+    int32_t *d_leaf_sb = nullptr, *d_leaf_se = nullptr;
+    int32_t *d_grp_tb = nullptr, *d_grp_te = nullptr, *d_grp_lcell = nullptr, *d_u_ptr = nullptr, *d_u_cell = nullptr;
+    uint8_t *d_u_mask = nullptr, *d_tgt_oct = nullptr;
+    double2 *d_Itf = nullptr;  // irregular table with both signs of m: [343][(2p+1)^2]
+    double2 *d_Rcf = nullptr;  // regular child-offset table with both signs of m: [8][(p+1)^2]
+    // near-field operator: per sorted target, (element charge index, coefficient) entries
+    int64_t *d_nf_ptr = nullptr;
+    int32_t *d_nf_col = nullptr;
+    double *d_nf_val = nullptr;
+    double2 *d_pot = nullptr;  // [T] near-field + correction potential
+    int64_t nf_nnz = 0;
+    int32_t *d_tgt_node = nullptr;
     double4 *d_tgt_pos = nullptr;
-    double2 *d_It = nullptr, *d_Rc = nullptr;
+    double2 *d_Rc = nullptr;
     double2 *d_M = nullptr, *d_L = nullptr;  // [ncell][2][nc]
     std::vector<int32_t> src_cells_lev_off, tgt_cells_lev_off;  // offsets into the lists
     std::vector<int32_t> src_cells, tgt_cells;
@@ -89,17 +105,9 @@ inline void unmorton(uint32_t c, int l, int &ix, int &iy, int &iz)
 
 namespace fg {
 
-// Mean targets per cell at or below which a level is evaluated multipole-to-target
-// (FG_FMM_M2P_TARGETS overrides; 0 forces M2L everywhere).
-static int fmm_m2p_threshold()
-{
-    const char *e = std::getenv("FG_FMM_M2P_TARGETS");
-    return e ? std::atoi(e) : 24;
-}
-
 // Host construction of the plan from source positions (original order) and target nodes.
 std::string fmm_build_host(FmmPlan &P, const std::vector<double4> &src, const std::vector<int32_t> &tnode,
-                           const std::vector<double4> &tpos, int p, int leaf_target, int m2p_threshold)
+                           const std::vector<double4> &tpos, int p, int leaf_target)
 {
     if (p < 2 || p > 10 || (p & 1)) return "fmm_order must be even, 2..10";
     P.p = p;
@@ -328,46 +336,55 @@ std::string fmm_build_host(FmmPlan &P, const std::vector<double4> &src, const st
         P.src_cells_lev_off[l + 1] = (int32_t)P.src_cells.size();
         P.tgt_cells_lev_off[l + 1] = (int32_t)P.tgt_cells.size();
     }
-    // evaluation split: levels with few targets per cell are evaluated multipole-to-target
-    // (one interaction serves few targets there, a translation would cost more than the
-    // evaluations it saves); coarser levels use M2L, L2L and L2P at level ls-1
-    P.ls = L + 1;
-    for (int l = 2; l <= L; l++) {
-        const int ncl = P.tgt_cells_lev_off[l + 1] - P.tgt_cells_lev_off[l];
-        if (ncl > 0 && (double)T / ncl <= (double)m2p_threshold) {
-            P.ls = l;
-            break;
-        }
-    }
-    P.m2p_ptr.assign(NL + 1, 0);
-    P.lcell.assign(NL, -1);
+    // evaluation groups (level gl = L-1): the targets of one cell, contiguous in Morton order
+    P.ls = std::max(2, L - 1);
     {
-        std::vector<int32_t> anc(L + 1);
-        for (int li = 0; li < NL; li++) {
-            const int ci = P.lev_off[L] + li;
-            int a = ci;
-            for (int l = L; l >= 0; l--) {
-                anc[l] = a;
-                a = l > 0 ? P.parent[a] : -1;
+        const int gl = L - 1, shg = 3 * (LMAX - gl), n = 1 << gl, nL = 1 << L;
+        P.tgt_oct.resize(T);
+        P.tgt_grp.resize(T);
+        P.u_ptr.assign(1, 0);
+        for (size_t k = 0; k < T;) {
+            const uint32_t gc = tcode[tord[k]] >> shg;
+            size_t e = k;
+            while (e < T && (tcode[tord[e]] >> shg) == gc) e++;
+            const int g = (int)P.grp_tb.size();
+            for (size_t q = k; q < e; q++) {
+                P.tgt_oct[q] = (uint8_t)((tcode[tord[q]] >> shL) & 7u);
+                P.tgt_grp[q] = g;
             }
-            if (P.ls - 1 >= 2) P.lcell[li] = anc[P.ls - 1];
-            int cnt = 0;
-            if (P.has_tgt[ci])
-                for (int l = P.ls; l <= L; l++) cnt += P.il_ptr[anc[l] + 1] - P.il_ptr[anc[l]];
-            P.m2p_ptr[li + 1] = P.m2p_ptr[li] + cnt;
-        }
-        P.m2p_cell.resize(P.m2p_ptr[NL]);
-        for (int li = 0; li < NL; li++) {
-            const int ci = P.lev_off[L] + li;
-            if (!P.has_tgt[ci]) continue;
-            int a = ci;
-            for (int l = L; l >= 0; l--) {
-                anc[l] = a;
-                a = l > 0 ? P.parent[a] : -1;
+            const int gci = index[gl].at(gc);
+            P.grp_tb.push_back((int32_t)k);
+            P.grp_te.push_back((int32_t)e);
+            P.grp_lcell.push_back(L - 2 >= 2 ? P.parent[gci] : -1);
+            for (int q = P.il_ptr[gci]; q < P.il_ptr[gci + 1]; q++) {
+                P.u_cell.push_back(P.il_src[q]);
+                P.u_mask.push_back(0xff);
             }
-            int pos = P.m2p_ptr[li];
-            for (int l = P.ls; l <= L; l++)
-                for (int k = P.il_ptr[anc[l]]; k < P.il_ptr[anc[l] + 1]; k++) P.m2p_cell[pos++] = P.il_src[k];
+            int px, py, pz;
+            unmorton(gc, gl, px, py, pz);
+            for (int dx = -1; dx <= 1; dx++)
+                for (int dy = -1; dy <= 1; dy++)
+                    for (int dz = -1; dz <= 1; dz++) {
+                        const int qx = px + dx, qy = py + dy, qz = pz + dz;
+                        if (qx < 0 || qy < 0 || qz < 0 || qx >= n || qy >= n || qz >= n) continue;
+                        for (int o = 0; o < 8; o++) {
+                            const int sx = 2 * qx + ((o >> 2) & 1), sy = 2 * qy + ((o >> 1) & 1), sz = 2 * qz + (o & 1);
+                            if (sx >= nL || sy >= nL || sz >= nL) continue;
+                            auto it = index[L].find(morton(sx, sy, sz));
+                            if (it == index[L].end() || !P.has_src[it->second]) continue;
+                            uint8_t mask = 0;
+                            for (int t = 0; t < 8; t++) {
+                                const int tx = 2 * px + ((t >> 2) & 1), ty = 2 * py + ((t >> 1) & 1), tz = 2 * pz + (t & 1);
+                                if (std::abs(tx - sx) > 1 || std::abs(ty - sy) > 1 || std::abs(tz - sz) > 1)
+                                    mask |= (uint8_t)(1u << t);
+                            }
+                            if (!mask) continue;
+                            P.u_cell.push_back(it->second);
+                            P.u_mask.push_back(mask);
+                        }
+                    }
+            P.u_ptr.push_back((int32_t)P.u_cell.size());
+            k = e;
         }
     }
     // tables
@@ -432,7 +449,7 @@ void fmm_eval_host(const FmmPlan &P, const double *qu, const double *qv, std::ve
             }
     for (int ci = 0; ci < NC; ci++) {
         const int l = (int)P.cen[ci].w;
-        if (l >= P.ls) continue;
+        if (l >= P.ls || !P.has_tgt[ci]) continue;
         const double iw = 1.0 / P.W[l];
         for (int k = P.il_ptr[ci]; k < P.il_ptr[ci + 1]; k++)
             for (int s = 0; s < 2; s++)
@@ -463,18 +480,19 @@ void fmm_eval_host(const FmmPlan &P, const double *qu, const double *qv, std::ve
     pot_u.assign(T, 0.0);
     pot_v.assign(T, 0.0);
     for (size_t t = 0; t < T; t++) {
-        const int li = P.tgt_leaf[t];
+        const int li = P.tgt_leaf[t], g = P.tgt_grp[t];
         const double4 y = P.tgt_pos[t];
         double acc[2] = {0, 0};
-        if (P.lcell[li] >= 0) {
-            const int ci = P.lcell[li];
+        if (P.grp_lcell[g] >= 0) {
+            const int ci = P.grp_lcell[g];
             const double4 c = P.cen[ci];
             const double iw = 1.0 / P.W[(int)c.w];
             for (int s = 0; s < 2; s++)
                 acc[s] = l2p_eval(&Lx[((size_t)ci * 2 + s) * nc], p, (y.x - c.x) * iw, (y.y - c.y) * iw, (y.z - c.z) * iw);
         }
-        for (int k = P.m2p_ptr[li]; k < P.m2p_ptr[li + 1]; k++) {
-            const int sc = P.m2p_cell[k];
+        for (int k = P.u_ptr[g]; k < P.u_ptr[g + 1]; k++) {
+            if (!((P.u_mask[k] >> P.tgt_oct[t]) & 1)) continue;
+            const int sc = P.u_cell[k];
             const double4 c = P.cen[sc];
             const double iw = 1.0 / P.W[(int)c.w];
             double su = 0, sv = 0;
@@ -515,12 +533,6 @@ static T *dput(const std::vector<T> &h)
     return (T *)d;
 }
 
-__device__ inline double wsum64(double v)
-{
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    return v;
-}
-
 // P2M: a group of G lanes per source leaf, each lane a strided subset of the leaf's sources; the
 // lanes' accumulators are combined with cross-lane sums. blockIdx.y = charge set.
 template <int P, int G>
@@ -558,66 +570,116 @@ __global__ __launch_bounds__(64) void k_fmm_p2m(int nleaf, int cell0, double iw,
     }
 }
 
-// M2M: one thread per (cell with sources at this level, set, coefficient)
-__global__ __launch_bounds__(256) void k_fmm_m2m(int ncells, const int32_t *__restrict__ cells, int p, int nc,
-                                                 const int32_t *__restrict__ ch_ptr, const int32_t *__restrict__ ch_idx,
-                                                 const int32_t *__restrict__ oct, const double2 *__restrict__ Rc,
-                                                 double2 *__restrict__ M)
+// Logical block index with the dispatcher's round-robin over the 8 XCDs undone: consecutive
+// logical blocks (neighbouring cells, which share source data) run on the same XCD and its L2.
+__device__ inline int xcd_block(int b, int nb)
 {
-    const int tid = blockIdx.x * blockDim.x + threadIdx.x;
-    const int q = tid % nc, set = (tid / nc) & 1, k = tid / (2 * nc);
-    if (k >= ncells) return;
-    int n = 0;
-    while (fidx(n + 1, 0) <= q) n++;
-    const int m = q - fidx(n, 0);
-    const int ci = cells[k];
-    double re = 0, im = 0;
-    for (int e = ch_ptr[ci]; e < ch_ptr[ci + 1]; e++) {
-        const int ch = ch_idx[e];
-        const cplx v = m2m_coef(M + ((size_t)ch * 2 + set) * nc, Rc + (size_t)oct[ch] * nc, n, m);
-        re += v.re;
-        im += v.im;
-    }
-    M[((size_t)ci * 2 + set) * nc + q] = make_double2(re, im);
+    const int q = nb >> 3, r = nb & 7, x = b & 7;
+    return x * q + min(x, r) + (b >> 3);
 }
 
-// M2L: one block per target cell (any level), lanes = coefficients x sets; the source expansion
-// and the offset's irregular table are staged in LDS.
+// M2M: one wavefront per parent cell with sources; lane (n, m) accumulates both charge sets over
+// the children. The children's expansions are read at wave-uniform addresses; the child-offset
+// table (both signs of m) is indexed per lane.
 template <int P>
-__global__ __launch_bounds__(256) void k_fmm_m2l(int ncells, const int32_t *__restrict__ cells,
-                                                 const double4 *__restrict__ cen, const double *__restrict__ Wlev,
-                                                 const int32_t *__restrict__ il_ptr, const int32_t *__restrict__ il_src,
-                                                 const int32_t *__restrict__ il_off, const double2 *__restrict__ It,
-                                                 const double2 *__restrict__ M, double2 *__restrict__ Lx)
+__global__ __launch_bounds__(128) void k_fmm_m2m(const int32_t *__restrict__ cells, const int32_t *__restrict__ ch_ptr,
+                                                const int32_t *__restrict__ ch_idx, const int32_t *__restrict__ oct,
+                                                const double2 *__restrict__ Rcf, double2 *__restrict__ M)
 {
-    constexpr int NCo = fcount(P), NIo = fcount(2 * P);
-    __shared__ double2 sM[2 * NCo];
-    __shared__ double2 sI[NIo];
+    constexpr int NCo = fcount(P), NF = (P + 1) * (P + 1);
     const int ci = cells[blockIdx.x];
-    const int t = threadIdx.x;
-    const int q = t % NCo, set = t / NCo;
-    const bool act = t < 2 * NCo;
-    int j = 0;
-    while (fidx(j + 1, 0) <= q) j++;
-    const int k = q - fidx(j, 0);
-    double re = 0, im = 0;
-    for (int e = il_ptr[ci]; e < il_ptr[ci + 1]; e++) {
-        const int sc = il_src[e];
-        const double2 *src = M + (size_t)sc * 2 * NCo;
-        for (int u = t; u < 2 * NCo; u += blockDim.x) sM[u] = src[u];
-        const double2 *tab = It + (size_t)il_off[e] * NIo;
-        for (int u = t; u < NIo; u += blockDim.x) sI[u] = tab[u];
-        __syncthreads();
-        if (act) {
-            const cplx v = m2l_coef(sM + set * NCo, sI, P, j, k);
-            re += v.re;
-            im += v.im;
+    const int q = threadIdx.x;  // blockDim = 64 * ceil(NCo / 64)
+    int n = 0;
+    while (n < P && fidx(n + 1, 0) <= q) n++;
+    const int m = q - fidx(n, 0);
+    double ur = 0, ui = 0, vr = 0, vi = 0;
+    for (int e = ch_ptr[ci]; e < ch_ptr[ci + 1]; e++) {
+        const int ch = ch_idx[e];
+        const double2 *Mu = M + (size_t)ch * 2 * NCo, *Mv = Mu + NCo;
+        const double2 *R = Rcf + (size_t)oct[ch] * NF;
+        double sc = 1.0;
+#pragma unroll
+        for (int k = 0; k <= P; k++, sc *= 0.5) {
+#pragma unroll
+            for (int l = -k; l <= k; l++) {
+                const int ml = m - l, nk = n - k;
+                if (q >= NCo || nk < 0 || ml > nk || -ml > nk) continue;
+                const double2 r = R[fidx_full(nk, ml)];
+                const double2 a = Mu[fidx(k, l < 0 ? -l : l)], b = Mv[fidx(k, l < 0 ? -l : l)];
+                // X_k^{-l} = (-1)^l conj(X_k^l)
+                const double s = (l < 0 && (l & 1)) ? -sc : sc;
+                const double ai = l < 0 ? -a.y : a.y, bi = l < 0 ? -b.y : b.y;
+                ur += s * (r.x * a.x - r.y * ai);
+                ui += s * (r.x * ai + r.y * a.x);
+                vr += s * (r.x * b.x - r.y * bi);
+                vi += s * (r.x * bi + r.y * b.x);
+            }
         }
-        __syncthreads();
     }
-    if (act) {
-        const double iw = 1.0 / Wlev[(int)cen[ci].w];
-        Lx[((size_t)ci * 2 + set) * NCo + q] = make_double2(re * iw, im * iw);
+    if (q < NCo) {
+        M[(size_t)ci * 2 * NCo + q] = make_double2(ur, ui);
+        M[((size_t)ci * 2 + 1) * NCo + q] = make_double2(vr, vi);
+    }
+}
+
+// M2L: one block of 4 wavefronts per target cell, split into NG entry groups of NW wavefronts
+// (NW = 1 up to order 8); group w takes the list entries w, w+NG, ...; thread (j, k) accumulates
+// both charge sets; the groups' sums are combined through LDS.
+template <int P>
+__global__ __launch_bounds__(256) void k_fmm_m2l(const int32_t *__restrict__ cells, const double4 *__restrict__ cen,
+                                                 const double *__restrict__ Wlev, const int32_t *__restrict__ il_ptr,
+                                                 const int32_t *__restrict__ il_src, const int32_t *__restrict__ il_off,
+                                                 const double2 *__restrict__ Itf, const double2 *__restrict__ M,
+                                                 double2 *__restrict__ Lx)
+{
+    constexpr int NCo = fcount(P), NIF = (2 * P + 1) * (2 * P + 1);
+    constexpr int NW = (NCo + 63) / 64, NG = 4 / NW;  // waves per coefficient set, entry groups
+    __shared__ double4 red[NG][64 * NW];
+    const int ci = cells[xcd_block(blockIdx.x, gridDim.x)];
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / (64 * NW));
+    const int q = threadIdx.x % (64 * NW);
+    int j = 0;
+    while (j < P && fidx(j + 1, 0) <= q) j++;
+    const int k = q - fidx(j, 0);
+    double ur = 0, ui = 0, vr = 0, vi = 0;
+    if (q < NCo) {
+        for (int e = il_ptr[ci] + wave; e < il_ptr[ci + 1]; e += NG) {
+            const int sc = il_src[e];
+            const double2 *Mu = M + (size_t)sc * 2 * NCo, *Mv = Mu + NCo;
+            const double2 *I = Itf + (size_t)il_off[e] * NIF;
+#pragma unroll
+            for (int n = 0; n <= P; n++) {
+#pragma unroll
+                for (int m = -n; m <= n; m++) {
+                    // (-1)^m M_n^m I_{n+j}^{-(m+k)}
+                    const double2 a = Mu[fidx(n, m < 0 ? -m : m)], b = Mv[fidx(n, m < 0 ? -m : m)];
+                    const double2 t = I[fidx_full(n + j, -(m + k))];
+                    // for m < 0, M_n^m = (-1)^m conj(M_n^|m|): the two factors (-1)^m cancel
+                    const double s = (m > 0 && (m & 1)) ? -1.0 : 1.0;
+                    const double ai = m < 0 ? -a.y : a.y, bi = m < 0 ? -b.y : b.y;
+                    ur += s * (a.x * t.x - ai * t.y);
+                    ui += s * (a.x * t.y + ai * t.x);
+                    vr += s * (b.x * t.x - bi * t.y);
+                    vi += s * (b.x * t.y + bi * t.x);
+                }
+            }
+        }
+    }
+    red[wave][q] = make_double4(ur, ui, vr, vi);
+    __syncthreads();
+    if (threadIdx.x < NCo) {
+        double4 a = red[0][q];
+#pragma unroll
+        for (int w = 1; w < NG; w++) {
+            const double4 b = red[w][q];
+            a.x += b.x;
+            a.y += b.y;
+            a.z += b.z;
+            a.w += b.w;
+        }
+        const double f = (((j + k) & 1) ? -1.0 : 1.0) / Wlev[(int)cen[ci].w];
+        Lx[(size_t)ci * 2 * NCo + q] = make_double2(a.x * f, a.y * f);
+        Lx[((size_t)ci * 2 + 1) * NCo + q] = make_double2(a.z * f, a.w * f);
     }
 }
 
@@ -639,74 +701,157 @@ __global__ __launch_bounds__(256) void k_fmm_l2l(int ncells, const int32_t *__re
     d.y += v.im;
 }
 
-// L2P + M2P + P2P: one wavefront per target; lanes stride over the sources of the adjacent leaves
-// and over the cells evaluated multipole-to-target.
-__global__ __launch_bounds__(256) void k_fmm_eval(int T, int p, int nc, const double *__restrict__ Wlev,
-                                                  const int32_t *__restrict__ tgt_node,
-                                                  const int32_t *__restrict__ tgt_leaf,
-                                                  const double4 *__restrict__ tpos, const double4 *__restrict__ cen,
-                                                  const double2 *__restrict__ Lx, const double2 *__restrict__ M,
-                                                  const int32_t *__restrict__ lcell,
-                                                  const int32_t *__restrict__ m2p_ptr,
-                                                  const int32_t *__restrict__ m2p_cell,
-                                                  const int32_t *__restrict__ near_ptr,
+__device__ inline double wsum64(double v)
+{
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// Near-field operator, per-source weights in tree order: a tetrahedron's Gauss point carries w_g
+// (its charge is w_g times the element value -Ms div u); a triangle's Gauss point carries
+// dMs w_g lambda_a(g) for its vertices a = 0..2 (the charge is their sum against u_a . n).
+__global__ __launch_bounds__(256) void k_nf_weights(int Tm, int Fm, const double *__restrict__ geo,
+                                                    const double *__restrict__ tg, const int64_t *__restrict__ rank,
+                                                    double4 *__restrict__ wsrc)
+{
+    const int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (s < (int64_t)5 * Tm) {
+        const int t = (int)(s / 5), g = (int)(s % 5);
+        wsrc[rank[s]] = make_double4(geo[(size_t)(12 + g) * Tm + t], 0, 0, 0);
+    } else if (s < (int64_t)5 * Tm + (int64_t)4 * Fm) {
+        const int64_t r = s - (int64_t)5 * Tm;
+        const int f = (int)(r / 4), g = (int)(r % 4);
+        const double w = tg[(size_t)(3 + g) * Fm + f] * tg[(size_t)7 * Fm + f];
+        wsrc[rank[s]] = make_double4(w * tri_a(0, g), w * tri_a(1, g), w * tri_a(2, g), 0);
+    }
+}
+
+// Near-field operator entries: one wavefront per target; for each adjacent leaf, lanes take the
+// leaf's element runs (consecutive Gauss points of one element) and sum weight / distance.
+__global__ __launch_bounds__(256) void k_nf_build(int T, const int32_t *__restrict__ tgt_leaf,
+                                                  const double4 *__restrict__ tpos, const int32_t *__restrict__ near_ptr,
                                                   const int32_t *__restrict__ near_leaf,
-                                                  const int32_t *__restrict__ sb, const int32_t *__restrict__ se,
-                                                  const double4 *__restrict__ spos, const double2 *__restrict__ q,
-                                                  const double2 *__restrict__ corr, NodeRec *__restrict__ nxt)
+                                                  const int32_t *__restrict__ tp_ptr, const int32_t *__restrict__ tp_col,
+                                                  const int32_t *__restrict__ tp_src, const int32_t *__restrict__ tp_nk,
+                                                  const double4 *__restrict__ spos, const double4 *__restrict__ wsrc,
+                                                  const int64_t *__restrict__ nf_ptr, int32_t *__restrict__ nf_col,
+                                                  double *__restrict__ nf_val)
 {
     const int t = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const int lane = threadIdx.x & 63;
     if (t >= T) return;
     const int li = tgt_leaf[t];
     const double4 y = tpos[t];
-    double pu = 0, pv = 0;
+    int64_t pos = nf_ptr[t];
     for (int e = near_ptr[li]; e < near_ptr[li + 1]; e++) {
         const int sl = near_leaf[e];
-        for (int k = sb[sl] + lane; k < se[sl]; k += 64) {
-            const double4 x = spos[k];
-            const double dx = y.x - x.x, dy = y.y - x.y, dz = y.z - x.z;
-            const double ir = 1.0 / sqrt(dx * dx + dy * dy + dz * dz);
-            const double2 qq = q[k];
-            pu += qq.x * ir;
-            pv += qq.y * ir;
+        const int b = tp_ptr[sl], n = tp_ptr[sl + 1] - b;
+        for (int j = lane; j < n; j += 64) {
+            const int s0 = tp_src[b + j], nk = tp_nk[b + j];
+            const int cnt = nk & 15, kind = nk >> 4;
+            double v = 0;
+            for (int s = s0; s < s0 + cnt; s++) {
+                const double4 x = spos[s], w = wsrc[s];
+                const double dx = y.x - x.x, dy = y.y - x.y, dz = y.z - x.z;
+                const double wk = kind <= 1 ? w.x : (kind == 2 ? w.y : w.z);
+                v += wk / sqrt(dx * dx + dy * dy + dz * dz);
+            }
+            nf_col[pos + j] = tp_col[b + j];
+            nf_val[pos + j] = v;
         }
+        pos += n;
     }
-    for (int e = m2p_ptr[li] + lane; e < m2p_ptr[li + 1]; e += 64) {
-        const int sc = m2p_cell[e];
-        const double4 c = cen[sc];
-        const double iw = 1.0 / Wlev[(int)c.w];
-        double su = 0, sv = 0;
-        m2p_eval2(M + (size_t)sc * 2 * nc, M + ((size_t)sc * 2 + 1) * nc, p, (y.x - c.x) * iw, (y.y - c.y) * iw,
-                  (y.z - c.z) * iw, su, sv);
-        pu += su * iw;
-        pv += sv * iw;
-    }
-    const int lc = lcell[li];
-    if (lc >= 0 && lane < 2) {
-        const double4 c = cen[lc];
-        const double iw = 1.0 / Wlev[(int)c.w];
-        const double v = l2p_eval(Lx + ((size_t)lc * 2 + lane) * nc, p, (y.x - c.x) * iw, (y.y - c.y) * iw, (y.z - c.z) * iw);
-        if (lane == 0) pu += v;
-        else pv += v;
+}
+
+// Near field + vertex corrections: one wavefront per target (tree order).
+__global__ __launch_bounds__(256) void k_nf_apply(int T, const int64_t *__restrict__ nf_ptr,
+                                                  const int32_t *__restrict__ nf_col, const double *__restrict__ nf_val,
+                                                  const double2 *__restrict__ qe, const int32_t *__restrict__ tgt_node,
+                                                  const double2 *__restrict__ corr, double2 *__restrict__ pot)
+{
+    const int t = ((int)xcd_block(blockIdx.x, gridDim.x) * (int)blockDim.x + (int)threadIdx.x) >> 6;
+    const int lane = threadIdx.x & 63;
+    if (t >= T) return;
+    double pu = 0, pv = 0;
+    for (int64_t e = nf_ptr[t] + lane; e < nf_ptr[t + 1]; e += 64) {
+        const double w = nf_val[e];
+        const double2 q = qe[nf_col[e]];
+        pu += w * q.x;
+        pv += w * q.y;
     }
     pu = wsum64(pu);
     pv = wsum64(pv);
     if (lane == 0) {
-        const int node = tgt_node[t];
-        const double2 c = corr[node];
-        NodeRec r = nxt[node];
-        r.phi = (pu + c.x) / (4 * M_PI);
-        r.phiv = (pv + c.y) / (4 * M_PI);
-        nxt[node] = r;
+        const double2 c = corr[tgt_node[t]];
+        pot[t] = make_double2(pu + c.x, pv + c.y);
     }
 }
 
-__global__ void k_zero2(size_t n, double2 *__restrict__ a)
+// Far field: one block per evaluation group. Threads are (target slot, list slice) pairs; the
+// targets of a slice read the same multipole at the same time. The slices are summed through LDS,
+// then the L2P of the group's level-(L-2) ancestor is added and phi, phi_v are written.
+template <int P>
+__global__ __launch_bounds__(256) void k_fmm_m2p(const int32_t *__restrict__ grp_tb, const int32_t *__restrict__ grp_te,
+                                                 const int32_t *__restrict__ grp_lcell,
+                                                 const int32_t *__restrict__ u_ptr, const int32_t *__restrict__ u_cell,
+                                                 const uint8_t *__restrict__ u_mask,
+                                                 const uint8_t *__restrict__ tgt_oct, const double4 *__restrict__ tpos,
+                                                 const double4 *__restrict__ cen, const double *__restrict__ Wlev,
+                                                 const double2 *__restrict__ M, const double2 *__restrict__ Lx,
+                                                 const double2 *__restrict__ pot, const int32_t *__restrict__ tgt_node,
+                                                 NodeRec *__restrict__ nxt)
 {
-    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
-        a[i] = make_double2(0, 0);
+    constexpr int NCo = fcount(P);
+    __shared__ double2 red[256];
+    const int g = xcd_block(blockIdx.x, gridDim.x);
+    const int tb = grp_tb[g], te = grp_te[g], ub = u_ptr[g], ue = u_ptr[g + 1], lc = grp_lcell[g];
+    for (int t0 = tb; t0 < te; t0 += 64) {
+        const int n = min(64, te - t0);
+        const int nsl = 256 / n;
+        const int slot = (int)threadIdx.x % n, slice = (int)threadIdx.x / n;
+        double pu = 0, pv = 0;
+        if (slice < nsl) {
+            const int t = t0 + slot;
+            const double4 y = tpos[t];
+            const int oc = tgt_oct[t];
+            for (int e = ub + slice; e < ue; e += nsl) {
+                if (!((u_mask[e] >> oc) & 1)) continue;
+                const int sc = u_cell[e];
+                const double4 c = cen[sc];
+                const double iw = 1.0 / Wlev[(int)c.w];
+                double su = 0, sv = 0;
+                m2p_eval2_t<P>(M + (size_t)sc * 2 * NCo, M + ((size_t)sc * 2 + 1) * NCo, (y.x - c.x) * iw,
+                               (y.y - c.y) * iw, (y.z - c.z) * iw, su, sv);
+                pu += su * iw;
+                pv += sv * iw;
+            }
+        }
+        red[threadIdx.x] = make_double2(pu, pv);
+        __syncthreads();
+        if ((int)threadIdx.x < n) {
+            double su = 0, sv = 0;
+            for (int s = 0; s < nsl; s++) {
+                const double2 r = red[s * n + threadIdx.x];
+                su += r.x;
+                sv += r.y;
+            }
+            const int t = t0 + threadIdx.x;
+            if (lc >= 0) {
+                const double4 y = tpos[t], c = cen[lc];
+                const double iw = 1.0 / Wlev[(int)c.w];
+                l2p_eval2_t<P>(Lx + (size_t)lc * 2 * NCo, Lx + ((size_t)lc * 2 + 1) * NCo, (y.x - c.x) * iw,
+                               (y.y - c.y) * iw, (y.z - c.z) * iw, su, sv);
+            }
+            const double2 pn = pot[t];
+            NodeRec *r = nxt + tgt_node[t];
+            r->phi = (pn.x + su) / (4 * M_PI);
+            r->phiv = (pn.y + sv) / (4 * M_PI);
+        }
+        __syncthreads();
+    }
 }
+
+void fmm_build_near(Ctx &c, FmmPlan &P);
 
 // ---- host side of the device pass ----
 int fmm_create(Ctx &c)
@@ -727,8 +872,8 @@ int fmm_create(Ctx &c)
             }
         auto *P = new FmmPlan();
         c.fmm = P;
-        const int leaf = c.opt.fmm_leaf > 0 ? c.opt.fmm_leaf : 128;
-        std::string e = fmm_build_host(*P, src, tnode, tpos, c.opt.fmm_order, leaf, fmm_m2p_threshold());
+        const int leaf = c.opt.fmm_leaf > 0 ? c.opt.fmm_leaf : 64;
+        std::string e = fmm_build_host(*P, src, tnode, tpos, c.opt.fmm_order, leaf);
         if (!e.empty()) throw e;
         // sources in tree order; the charge kernels scatter through src_rank
         if (hipMemcpy(c.src_pos, P->src_sorted.data(), sizeof(double4) * c.S, hipMemcpyHostToDevice) != hipSuccess)
@@ -745,18 +890,37 @@ int fmm_create(Ctx &c)
         P->d_il_off = dput(P->il_off);
         P->d_leaf_sb = dput(P->leaf_sb);
         P->d_leaf_se = dput(P->leaf_se);
-        P->d_near_ptr = dput(P->near_ptr);
-        P->d_near_leaf = dput(P->near_leaf);
-        P->d_m2p_ptr = dput(P->m2p_ptr);
-        P->d_m2p_cell = dput(P->m2p_cell);
-        P->d_lcell = dput(P->lcell);
+        P->d_grp_tb = dput(P->grp_tb);
+        P->d_grp_te = dput(P->grp_te);
+        P->d_grp_lcell = dput(P->grp_lcell);
+        P->d_u_ptr = dput(P->u_ptr);
+        P->d_u_cell = dput(P->u_cell);
+        P->d_u_mask = dput(P->u_mask);
+        P->d_tgt_oct = dput(P->tgt_oct);
         P->d_tgt_node = dput(P->tgt_node);
-        P->d_tgt_leaf = dput(P->tgt_leaf);
         P->d_tgt_pos = dput(P->tgt_pos);
-        P->d_It = dput(P->It);
         P->d_Rc = dput(P->Rc);
         P->d_src_cells = dput(P->src_cells);
         P->d_tgt_cells = dput(P->tgt_cells);
+        {
+            // offset tables with both signs of m
+            const int p = P->p, NIF = (2 * p + 1) * (2 * p + 1), NF = (p + 1) * (p + 1);
+            std::vector<double2> itf((size_t)343 * NIF), rcf((size_t)8 * NF);
+            for (int o = 0; o < 343; o++)
+                for (int n = 0; n <= 2 * p; n++)
+                    for (int m = -n; m <= n; m++) {
+                        const cplx v = fget(&P->It[(size_t)o * P->ni], n, m);
+                        itf[(size_t)o * NIF + fidx_full(n, m)] = make_double2(v.re, v.im);
+                    }
+            for (int o = 0; o < 8; o++)
+                for (int n = 0; n <= p; n++)
+                    for (int m = -n; m <= n; m++) {
+                        const cplx v = fget(&P->Rc[(size_t)o * P->nc], n, m);
+                        rcf[(size_t)o * NF + fidx_full(n, m)] = make_double2(v.re, v.im);
+                    }
+            P->d_Itf = dput(itf);
+            P->d_Rcf = dput(rcf);
+        }
         const size_t nexp = (size_t)P->ncell() * 2 * P->nc;
         void *d = nullptr;
         if (hipMalloc(&d, nexp * sizeof(double2)) != hipSuccess) throw std::string("hipMalloc M");
@@ -765,6 +929,7 @@ int fmm_create(Ctx &c)
         P->d_L = (double2 *)d;
         std::vector<double> W(P->W.begin(), P->W.end());
         c.fmm_W = dput(W);
+        fmm_build_near(c, *P);
         return FG_OK;
     } catch (const std::string &e) {
         c.err = e;
@@ -772,18 +937,92 @@ int fmm_create(Ctx &c)
     }
 }
 
+// Near-field operator: per leaf, the runs of consecutive tree-order sources that belong to one
+// element (a tetrahedron, or a triangle expanded to its three vertices); per target, one entry per
+// run in its adjacent leaves, coefficient sum_g weight_g / |y - x_g| computed once on the device.
+void fmm_build_near(Ctx &c, FmmPlan &P)
+{
+    const int L = P.L, NL = P.lev_off[L + 1] - P.lev_off[L];
+    const int64_t S = c.S, T5 = (int64_t)5 * c.Tm;
+    std::vector<int64_t> orig(S);
+    for (int64_t k = 0; k < S; k++) orig[P.src_rank[k]] = k;
+    std::vector<int32_t> tp_ptr(NL + 1, 0), tp_col, tp_src, tp_nk;
+    tp_col.reserve(S / 3);
+    tp_src.reserve(S / 3);
+    tp_nk.reserve(S / 3);
+    for (int li = 0; li < NL; li++) {
+        for (int k = P.leaf_sb[li]; k < P.leaf_se[li];) {
+            const int64_t o = orig[k];
+            const bool tet = o < T5;
+            const int64_t el = tet ? o / 5 : (o - T5) / 4;
+            int e = k + 1;
+            while (e < P.leaf_se[li]) {
+                const int64_t o2 = orig[e];
+                if ((o2 < T5) != tet || (tet ? o2 / 5 : (o2 - T5) / 4) != el) break;
+                e++;
+            }
+            if (tet) {
+                tp_col.push_back((int32_t)el);
+                tp_src.push_back(k);
+                tp_nk.push_back(e - k);
+            } else {
+                for (int a = 0; a < 3; a++) {
+                    tp_col.push_back((int32_t)(c.Tm + 3 * el + a));
+                    tp_src.push_back(k);
+                    tp_nk.push_back((e - k) | ((1 + a) << 4));
+                }
+            }
+            k = e;
+        }
+        tp_ptr[li + 1] = (int32_t)tp_col.size();
+    }
+    const int T = (int)P.tgt_node.size();
+    std::vector<int64_t> nf_ptr(T + 1, 0);
+    for (int t = 0; t < T; t++) {
+        const int li = P.tgt_leaf[t];
+        int64_t n = 0;
+        for (int e = P.near_ptr[li]; e < P.near_ptr[li + 1]; e++) n += tp_ptr[P.near_leaf[e] + 1] - tp_ptr[P.near_leaf[e]];
+        nf_ptr[t + 1] = nf_ptr[t] + n;
+    }
+    P.nf_nnz = nf_ptr[T];
+    int32_t *d_tp_ptr = dput(tp_ptr), *d_tp_col = dput(tp_col), *d_tp_src = dput(tp_src), *d_tp_nk = dput(tp_nk);
+    int32_t *d_near_ptr = dput(P.near_ptr), *d_near_leaf = dput(P.near_leaf), *d_tgt_leaf = dput(P.tgt_leaf);
+    P.d_nf_ptr = dput(nf_ptr);
+    void *d = nullptr;
+    if (hipMalloc(&d, std::max<int64_t>(1, P.nf_nnz) * sizeof(int32_t)) != hipSuccess) throw std::string("hipMalloc near-field columns");
+    P.d_nf_col = (int32_t *)d;
+    if (hipMalloc(&d, std::max<int64_t>(1, P.nf_nnz) * sizeof(double)) != hipSuccess) throw std::string("hipMalloc near-field values");
+    P.d_nf_val = (double *)d;
+    if (hipMalloc(&d, sizeof(double4) * S) != hipSuccess) throw std::string("hipMalloc weights");
+    double4 *wsrc = (double4 *)d;
+    if (hipMalloc(&d, sizeof(double2) * std::max(1, T)) != hipSuccess) throw std::string("hipMalloc pot");
+    P.d_pot = (double2 *)d;
+    if (hipMalloc(&d, sizeof(double2) * ((size_t)c.Tm + 3 * (size_t)c.Fm)) != hipSuccess) throw std::string("hipMalloc qe");
+    c.qe = (double2 *)d;
+    k_nf_weights<<<(unsigned)((S + 255) / 256), 256, 0, c.stream>>>(c.Tm, c.Fm, c.geo, c.tri_geo, c.src_rank, wsrc);
+    if (T > 0)
+        k_nf_build<<<(unsigned)(((int64_t)T * 64 + 255) / 256), 256, 0, c.stream>>>(
+            T, d_tgt_leaf, P.d_tgt_pos, d_near_ptr, d_near_leaf, d_tp_ptr, d_tp_col, d_tp_src, d_tp_nk, c.src_pos,
+            wsrc, P.d_nf_ptr, P.d_nf_col, P.d_nf_val);
+    const hipError_t err = hipStreamSynchronize(c.stream);
+    void *tmp[] = {d_tp_ptr, d_tp_col, d_tp_src, d_tp_nk, d_near_ptr, d_near_leaf, d_tgt_leaf, wsrc};
+    for (void *p : tmp) (void)hipFree(p);
+    if (err != hipSuccess) throw std::string("near-field build: ") + hipGetErrorString(err);
+}
+
 void fmm_destroy(Ctx &c)
 {
     FmmPlan *P = c.fmm;
     if (!P) return;
     void *ptrs[] = {P->d_cen, P->d_parent, P->d_oct, P->d_ch_ptr, P->d_ch_idx, P->d_il_ptr, P->d_il_src,
-                    P->d_il_off, P->d_leaf_sb, P->d_leaf_se, P->d_near_ptr, P->d_near_leaf, P->d_tgt_node,
-                    P->d_tgt_leaf, P->d_tgt_pos, P->d_It, P->d_Rc, P->d_M, P->d_L, P->d_src_cells, P->d_tgt_cells,
-                    P->d_m2p_ptr, P->d_m2p_cell, P->d_lcell,
-                    c.fmm_W};
+                    P->d_il_off, P->d_leaf_sb, P->d_leaf_se, P->d_grp_tb, P->d_grp_te, P->d_grp_lcell, P->d_u_ptr,
+                    P->d_u_cell, P->d_u_mask, P->d_tgt_oct, P->d_tgt_node, P->d_tgt_pos, P->d_Rc, P->d_Itf,
+                    P->d_Rcf, P->d_M, P->d_L, P->d_src_cells, P->d_tgt_cells, P->d_nf_ptr, P->d_nf_col,
+                    P->d_nf_val, P->d_pot, c.qe, c.fmm_W};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     c.fmm_W = nullptr;
+    c.qe = nullptr;
     delete P;
     c.fmm = nullptr;
 }
@@ -794,11 +1033,12 @@ static void run_fmm(Ctx &c)
     FmmPlan &P = *c.fmm;
     const int L = P.L, nc = P.nc;
     const int NL = P.lev_off[L + 1] - P.lev_off[L];
-    const size_t nexp = (size_t)P.ncell() * 2 * nc;
+    const int T = (int)P.tgt_node.size();
     {
-        Prof pr(c, "fmm_clear");
-        k_zero2<<<1024, 256, 0, c.stream>>>(nexp, P.d_M);
-        k_zero2<<<1024, 256, 0, c.stream>>>(nexp, P.d_L);
+        Prof pr(c, "fmm_near");
+        if (T > 0)
+            k_nf_apply<<<(unsigned)(((int64_t)T * 64 + 255) / 256), 256, 0, c.stream>>>(
+                T, P.d_nf_ptr, P.d_nf_col, P.d_nf_val, c.qe, P.d_tgt_node, c.corr, P.d_pot);
     }
     {
         Prof pr(c, "fmm_p2m");
@@ -812,18 +1052,16 @@ static void run_fmm(Ctx &c)
         for (int l = L - 1; l >= 2; l--) {
             const int n = P.src_cells_lev_off[l + 1] - P.src_cells_lev_off[l];
             if (n == 0) continue;
-            const int threads = n * 2 * nc;
-            k_fmm_m2m<<<(threads + 255) / 256, 256, 0, c.stream>>>(n, P.d_src_cells + P.src_cells_lev_off[l], PP, nc,
-                                                                   P.d_ch_ptr, P.d_ch_idx, P.d_oct, P.d_Rc, P.d_M);
+            k_fmm_m2m<PP><<<n, 64 * ((fcount(PP) + 63) / 64), 0, c.stream>>>(P.d_src_cells + P.src_cells_lev_off[l], P.d_ch_ptr, P.d_ch_idx,
+                                                  P.d_oct, P.d_Rcf, P.d_M);
         }
     }
     {
         Prof pr(c, "fmm_m2l");
-        const int n = P.tgt_cells_lev_off[std::min(P.ls, L + 1)] - P.tgt_cells_lev_off[2];
-        const int bs = ((2 * nc + 63) / 64) * 64;
+        const int n = P.tgt_cells_lev_off[P.ls] - P.tgt_cells_lev_off[2];
         if (n > 0)
-            k_fmm_m2l<PP><<<n, bs, 0, c.stream>>>(n, P.d_tgt_cells + P.tgt_cells_lev_off[2], P.d_cen, c.fmm_W,
-                                                   P.d_il_ptr, P.d_il_src, P.d_il_off, P.d_It, P.d_M, P.d_L);
+            k_fmm_m2l<PP><<<n, 256, 0, c.stream>>>(P.d_tgt_cells + P.tgt_cells_lev_off[2], P.d_cen, c.fmm_W,
+                                                   P.d_il_ptr, P.d_il_src, P.d_il_off, P.d_Itf, P.d_M, P.d_L);
     }
     {
         Prof pr(c, "fmm_l2l");
@@ -836,13 +1074,12 @@ static void run_fmm(Ctx &c)
         }
     }
     {
-        Prof pr(c, "fmm_eval");
-        const int T = (int)P.tgt_node.size();
-        k_fmm_eval<<<(T * 64 + 255) / 256, 256, 0, c.stream>>>(T, PP, nc, c.fmm_W, P.d_tgt_node, P.d_tgt_leaf,
-                                                              P.d_tgt_pos, P.d_cen, P.d_L, P.d_M, P.d_lcell,
-                                                              P.d_m2p_ptr, P.d_m2p_cell, P.d_near_ptr, P.d_near_leaf,
-                                                              P.d_leaf_sb, P.d_leaf_se, c.src_pos, c.src_q, c.corr,
-                                                              c.rec[1 - c.cur]);
+        Prof pr(c, "fmm_m2p");
+        const int ng = (int)P.grp_tb.size();
+        if (ng > 0)
+            k_fmm_m2p<PP><<<ng, 256, 0, c.stream>>>(P.d_grp_tb, P.d_grp_te, P.d_grp_lcell, P.d_u_ptr, P.d_u_cell,
+                                                    P.d_u_mask, P.d_tgt_oct, P.d_tgt_pos, P.d_cen, c.fmm_W, P.d_M,
+                                                    P.d_L, P.d_pot, P.d_tgt_node, c.rec[1 - c.cur]);
     }
 }
 
@@ -872,7 +1109,7 @@ extern "C" int fg_fmm_reference(const double *src_xyz, int64_t S, const double *
             tnode[i] = (int32_t)i;
         }
         FmmPlan P;
-        std::string e = fmm_build_host(P, src, tnode, tpos, order, leaf, fmm_m2p_threshold());
+        std::string e = fmm_build_host(P, src, tnode, tpos, order, leaf);
         if (!e.empty()) return FG_ERR_ARG;
         std::vector<double> pu, pv;
         fmm_eval_host(P, q_u, q_v, pu, pv, tnode);

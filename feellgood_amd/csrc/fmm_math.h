@@ -220,6 +220,96 @@ __host__ __device__ inline void m2p_eval2(const double2 *Mu, const double2 *Mv, 
     }
 }
 
+// Unrolled forms for a compile-time order P (device kernels).
+template <int P>
+__host__ __device__ inline void m2p_eval2_t(const double2 *__restrict__ Mu, const double2 *__restrict__ Mv, double x,
+                                            double y, double z, double &su, double &sv)
+{
+    const double r2 = x * x + y * y + z * z;
+    const double ir2 = 1.0 / r2;
+    double dre = 1.0 / sqrt(r2), dim = 0.0;
+#pragma unroll
+    for (int m = 0; m <= P; m++) {
+        if (m > 0) {
+            const double f = -(2.0 * m - 1.0) * ir2;
+            const double nre = f * (x * dre - y * dim), nim = f * (x * dim + y * dre);
+            dre = nre;
+            dim = nim;
+        }
+        const double w = m == 0 ? 1.0 : 2.0;
+        double2 a = Mu[fidx(m, m)], b = Mv[fidx(m, m)];
+        su += w * (a.x * dre + a.y * dim);
+        sv += w * (b.x * dre + b.y * dim);
+        if (m + 1 <= P) {
+            const double a1 = (2.0 * m + 1.0) * z * ir2;
+            double p2re = dre, p2im = dim, p1re = a1 * dre, p1im = a1 * dim;
+            a = Mu[fidx(m + 1, m)];
+            b = Mv[fidx(m + 1, m)];
+            su += w * (a.x * p1re + a.y * p1im);
+            sv += w * (b.x * p1re + b.y * p1im);
+#pragma unroll
+            for (int n = m + 2; n <= P; n++) {
+                const double aa = (2.0 * n - 1.0) * z, bb = (double)(n + m - 1) * (double)(n - m - 1);
+                const double cre = (aa * p1re - bb * p2re) * ir2, cim = (aa * p1im - bb * p2im) * ir2;
+                a = Mu[fidx(n, m)];
+                b = Mv[fidx(n, m)];
+                su += w * (a.x * cre + a.y * cim);
+                sv += w * (b.x * cre + b.y * cim);
+                p2re = p1re;
+                p2im = p1im;
+                p1re = cre;
+                p1im = cim;
+            }
+        }
+    }
+}
+
+// L2P of two expansions at once (unrolled): pu += sum L_u R, pv += sum L_v R.
+template <int P>
+__host__ __device__ inline void l2p_eval2_t(const double2 *__restrict__ Lu, const double2 *__restrict__ Lv, double x,
+                                            double y, double z, double &pu, double &pv)
+{
+    const double r2 = x * x + y * y + z * z;
+    double dre = 1.0, dim = 0.0;
+#pragma unroll
+    for (int m = 0; m <= P; m++) {
+        if (m > 0) {
+            const double f = -1.0 / (2.0 * m);
+            const double nre = f * (x * dre - y * dim), nim = f * (x * dim + y * dre);
+            dre = nre;
+            dim = nim;
+        }
+        const double w = m == 0 ? 1.0 : 2.0;
+        double2 a = Lu[fidx(m, m)], b = Lv[fidx(m, m)];
+        pu += w * (a.x * dre - a.y * dim);
+        pv += w * (b.x * dre - b.y * dim);
+        if (m + 1 <= P) {
+            double p2re = dre, p2im = dim, p1re = z * dre, p1im = z * dim;
+            a = Lu[fidx(m + 1, m)];
+            b = Lv[fidx(m + 1, m)];
+            pu += w * (a.x * p1re - a.y * p1im);
+            pv += w * (b.x * p1re - b.y * p1im);
+#pragma unroll
+            for (int n = m + 2; n <= P; n++) {
+                const double inv = 1.0 / ((double)(n + m) * (double)(n - m));
+                const double aa = (2.0 * n - 1.0) * z;
+                const double cre = (aa * p1re - r2 * p2re) * inv, cim = (aa * p1im - r2 * p2im) * inv;
+                a = Lu[fidx(n, m)];
+                b = Lv[fidx(n, m)];
+                pu += w * (a.x * cre - a.y * cim);
+                pv += w * (b.x * cre - b.y * cim);
+                p2re = p1re;
+                p2im = p1im;
+                p1re = cre;
+                p1im = cim;
+            }
+        }
+    }
+}
+
+// Index of X_n^m (any sign of m) in the tables that store both signs: n^2 + n + m.
+__host__ __device__ constexpr inline int fidx_full(int n, int m) { return n * n + n + m; }
+
 // M2M target coefficient (n, m >= 0): sum over the child's (k, l) of R_{n-k}^{m-l}(d) M_k^l, with d
 // the child-to-parent centre offset in parent widths and the child expansion normalised by the
 // child width (factor 2^-k).

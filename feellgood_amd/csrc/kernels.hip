@@ -669,7 +669,7 @@ __global__ __launch_bounds__(BLOCK) void k_charges_tet(int Tm, const int4 *__res
                                                        const DevRegion *__restrict__ regs,
                                                        const NodeRec *__restrict__ nxt,
                                                        const int64_t *__restrict__ rank,
-                                                       double2 *__restrict__ q)
+                                                       double2 *__restrict__ q, double2 *__restrict__ qe)
 {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= Tm) return;
@@ -696,6 +696,7 @@ __global__ __launch_bounds__(BLOCK) void k_charges_tet(int Tm, const int4 *__res
         const double wg = geo[(size_t)(12 + g) * Tm + t];
         q[rank[(size_t)5 * t + g]] = make_double2(-Ms * wg * du, -Ms * wg * dv);
     }
+    if (qe) qe[t] = make_double2(-Ms * du, -Ms * dv);
 }
 
 // Tri::potential (src/triangle.cpp:87-125): analytic potential at vertex i of the linear surface
@@ -724,7 +725,8 @@ __global__ __launch_bounds__(BLOCK) void k_charges_tri(int Fm, int64_t src_off, 
                                                        const double4 *__restrict__ pos4,
                                                        const NodeRec *__restrict__ nxt,
                                                        const int64_t *__restrict__ rank,
-                                                       double2 *__restrict__ q, double2 *__restrict__ corr3)
+                                                       double2 *__restrict__ q, double2 *__restrict__ corr3,
+                                                       double2 *__restrict__ qe, int Tm)
 {
     const int f = blockIdx.x * blockDim.x + threadIdx.x;
     if (f >= Fm) return;
@@ -753,6 +755,8 @@ __global__ __launch_bounds__(BLOCK) void k_charges_tri(int Fm, int64_t src_off, 
     }
     const double su[3] = {dot(u[0], n), dot(u[1], n), dot(u[2], n)};
     const double sv[3] = {dot(v[0], n), dot(v[1], n), dot(v[2], n)};
+    if (qe)
+        for (int a = 0; a < 3; a++) qe[Tm + 3 * f + a] = make_double2(su[a], sv[a]);
     for (int a = 0; a < 3; a++) {
         double cu = 0, cv = 0;
         for (int g = 0; g < 4; g++) {
@@ -1104,12 +1108,12 @@ void launch_charges(Ctx &c)
     if (c.Tm > 0) {
         Prof pr(c, "charges_tet");
         k_charges_tet<<<grid1(c.Tm), BLOCK, 0, c.stream>>>(c.Tm, c.tet_nodes, c.geo, c.tet_reg, c.regions, nxt,
-                                                           c.src_rank, c.src_q);
+                                                           c.src_rank, c.src_q, c.qe);
     }
     if (c.Fm > 0) {
         Prof pr(c, "charges_tri");
         k_charges_tri<<<grid1(c.Fm), BLOCK, 0, c.stream>>>(c.Fm, (int64_t)5 * c.Tm, c.tri_nodes, c.tri_geo, c.pos4,
-                                                           nxt, c.src_rank, c.src_q, c.tri_corr);
+                                                           nxt, c.src_rank, c.src_q, c.tri_corr, c.qe, c.Tm);
     }
     Prof pr(c, "corr_gather");
     k_corr_gather<<<c.blocks_for(c.N), BLOCK, 0, c.stream>>>(c.N, c.cinc_ptr, c.cinc_idx, c.tri_corr, c.corr);
