@@ -75,6 +75,42 @@ def prefactor(dt):
 
 
 # ------------------------------------------------------------------ byte / flop models
+def m2p_flops(p):
+    """FP64 operations of one multipole-to-target evaluation of two expansions of order p (the
+    streamed irregular recurrence of m2p_eval2_t plus the caller's scaling)."""
+    f = 5 + 1 + 2                      # r^2, 1/r^2, 1/r
+    f += 7 * p                         # diagonal recurrence I_m^m, m = 1..p
+    f += 4 * p                         # first off-diagonal I_{m+1}^m
+    f += 9 * sum(max(0, p - m - 1) for m in range(p + 1))  # column recurrence, n >= m+2
+    f += 10 * (p + 1) * (p + 2) // 2   # two dot products, weighted
+    return f + 10                      # offset, scaling and accumulation
+
+
+def m2l_flops(p):
+    """FP64 operations of one M2L translation of two expansions: every output (j, k >= 0) sums
+    (p+1)^2 complex products per charge set."""
+    return (p + 1) * (p + 2) // 2 * (p + 1) ** 2 * 2 * 8
+
+
+def m2m_flops(p):
+    """FP64 operations of one child-to-parent M2M translation of two expansions."""
+    terms = 0
+    for n in range(p + 1):
+        for m in range(n + 1):
+            for k in range(n + 1):
+                for l in range(-k, k + 1):
+                    if abs(m - l) <= n - k:
+                        terms += 1
+    return terms * 2 * 8
+
+
+def p2m_flops(p):
+    """FP64 operations of accumulating one source into both expansions (the regular recurrence is
+    evaluated once per charge set)."""
+    rec = 5 + 7 * p + 2 * p + 9 * sum(max(0, p - m - 1) for m in range(p + 1))
+    return 2 * (rec + 2 * (p + 1) * (p + 2) // 2)
+
+
 def kernel_model(name, n):
     """(bound, algorithmic bytes or flops per launch) of the engine's kernels; n holds the sizes."""
     N, T, nnz, F, S = n["N"], n["T"], n["nnz"], n["F"], n["S"]
@@ -95,6 +131,18 @@ def kernel_model(name, n):
         "max_angle": ("hbm", 4 * nnz + 64 * N),
         "p2p_direct": ("valu64", 2 * 10.0 * N * S),
     }
+    f = n.get("fmm")
+    if f:
+        p, Tg = f["order"], f["targets"]
+        models.update({
+            # coefficient rows + shared columns + element values read once + per-target I/O
+            "fmm_near": ("hbm", 8 * f["near_coefs"] + 4 * f["near_columns"] + 16 * (T + 3 * F)
+                         + (8 + 16 + 16 + 4) * Tg + 12 * f["target_leaves"]),
+            "fmm_m2p": ("valu64", m2p_flops(p) * f["m2p_pairs"]),
+            "fmm_m2l": ("valu64", m2l_flops(p) * f["m2l_pairs"]),
+            "fmm_m2m": ("valu64", m2m_flops(p) * (f["cells"] - f["leaves"])),
+            "fmm_p2m": ("valu64", p2m_flops(p) * f["sources"]),
+        })
     return models.get(name)
 
 
@@ -161,9 +209,23 @@ def run_gpu(args, w, m, u0):
     g.synchronize()
     tp1 = time.perf_counter()
     stats = g.kernel_stats()
+    fmm = g.fmm_info() if args.demag == "fmm" else None
     g.close()
-    return dict(ms_per_step=clean_ms, iters=clean_iters, fails=fails, stats=stats,
+    return dict(ms_per_step=clean_ms, iters=clean_iters, fails=fails, stats=stats, fmm=fmm,
                 prof_ms_per_step=(tp1 - tp0) * 1e3 / prof_steps, prof_steps=prof_steps)
+
+
+def measured_traffic(kernel, args):
+    """HBM bytes per launch of the dominant kernel from the committed rocprofv3 PMC summary
+    (FETCH_SIZE and WRITE_SIZE scaled by a copy kernel), when it describes this configuration."""
+    path = os.path.join(ROOT, "profiles", "film_fmm_fp64_traffic.json")
+    if args.small or args.precision != "fp64" or args.demag != "fmm" or args.fmm_order != 8 or args.fmm_leaf != 64:
+        return None
+    try:
+        d = json.load(open(path))
+    except (OSError, ValueError):
+        return None
+    return d.get("bytes_per_launch") if d.get("kernel") == kernel else None
 
 
 def roofline(stats, sizes):
@@ -187,6 +249,7 @@ def roofline(stats, sizes):
                 entry["frac_hbm"] = round(work / avg_s / 1e9 / HBM_PEAK_GBS, 4)
             else:
                 entry["TFLOPs"] = round(work / avg_s / 1e12, 2)
+                entry["frac_fp64"] = round(work / avg_s / 1e12 / FP64_VALU_TFLOPS, 4)
         per_kernel[name] = entry
     name, launches, ms = best
     mdl = kernel_model(name, sizes)
@@ -250,10 +313,10 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--demag", default="direct", choices=["fmm", "direct"])
+    ap.add_argument("--demag", default="fmm", choices=["fmm", "direct"])
     ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
     ap.add_argument("--fmm-order", type=int, default=8)
-    ap.add_argument("--fmm-leaf", type=int, default=128)
+    ap.add_argument("--fmm-leaf", type=int, default=64)
     ap.add_argument("--prof-steps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--small", action="store_true", help="the 48k-tet cube instead of the film")
@@ -280,7 +343,10 @@ def main():
     g.lib.fg_system_size(g._h, C.byref(rows), C.byref(nz))
     g.close()
     sizes["nnz"] = nz.value // 4
+    sizes["fmm"] = res["fmm"]
     rl, per_kernel = roofline(res["stats"], sizes)
+    if rl and rl.get("bound"):
+        rl["traffic"] = measured_traffic(rl["kernel"], args)
     out = {
         "metric": METRIC,
         "value": round(1e3 / res["ms_per_step"], 4),
@@ -307,6 +373,7 @@ def main():
         "roofline": rl,
         "kernels": per_kernel,
         "profiled_ms_per_step": round(res["prof_ms_per_step"], 4),
+        "fmm_plan": res["fmm"],
     }
     if not args.no_cpu:
         try:

@@ -524,7 +524,7 @@ void fg_default_options(fg_options *o)
     o->max_iter = 700;
     o->demag_method = FG_DEMAG_DIRECT;
     o->fmm_order = 8;
-    o->fmm_leaf = 128;
+    o->fmm_leaf = 64;
     o->precision = FG_PREC_FP64;
     o->device = 0;
     o->profile = 0;
@@ -798,6 +798,14 @@ int fg_system_size(fg_ctx *x, int64_t *rows, int64_t *nnz)
     if (!x || !rows || !nnz) return FG_ERR_ARG;
     *rows = 2 * (int64_t)x->c.N;
     *nnz = 4 * x->c.nnz_real;
+    return FG_OK;
+}
+
+int fg_fmm_info(fg_ctx *x, int64_t info[16])
+{
+    if (!x || !info) return FG_ERR_ARG;
+    if (!x->c.fmm) return FG_ERR_STATE;
+    fmm_info(x->c, info);
     return FG_OK;
 }
 

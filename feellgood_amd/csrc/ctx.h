@@ -153,5 +153,6 @@ void launch_evolution(Ctx &c);
 int fmm_create(Ctx &c);
 void fmm_destroy(Ctx &c);
 void launch_fmm(Ctx &c);
+void fmm_info(const Ctx &c, int64_t info[16]);
 
 }  // namespace fg

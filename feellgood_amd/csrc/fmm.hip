@@ -65,10 +65,14 @@ struct FmmPlan {
     double2 *d_Itf = nullptr;  // irregular table with both signs of m: [343][(2p+1)^2]
     double2 *d_Rcf = nullptr;  // regular child-offset table with both signs of m: [8][(p+1)^2]
     // near-field operator: per sorted target, (element charge index, coefficient) entries
-    int64_t *d_nf_ptr = nullptr;
-    int32_t *d_nf_col = nullptr;
+    int64_t *d_nf_ptr = nullptr;  // [T+1] start of each target's coefficient row
     double *d_nf_val = nullptr;
+    int ntl = 0;                  // leaves with targets
+    int32_t *d_tl_tb = nullptr, *d_tl_te = nullptr;  // their target ranges
+    int64_t *d_lc_ptr = nullptr;  // [ntl+1] shared column list of each such leaf
+    int32_t *d_lc_col = nullptr;
     double2 *d_pot = nullptr;  // [T] near-field + correction potential
+    int64_t m2p_pairs = 0, m2l_pairs = 0, lc_cols = 0;
     int64_t nf_nnz = 0;
     int32_t *d_tgt_node = nullptr;
     double4 *d_tgt_pos = nullptr;
@@ -726,16 +730,16 @@ __global__ __launch_bounds__(256) void k_nf_weights(int Tm, int Fm, const double
     }
 }
 
-// Near-field operator entries: one wavefront per target; for each adjacent leaf, lanes take the
-// leaf's element runs (consecutive Gauss points of one element) and sum weight / distance.
+// Near-field operator coefficients: one wavefront per target; for each adjacent leaf, lanes take
+// the leaf's element runs (consecutive Gauss points of one element) and sum weight / distance.
+// The columns are the same for all targets of a leaf and are stored once per leaf.
 __global__ __launch_bounds__(256) void k_nf_build(int T, const int32_t *__restrict__ tgt_leaf,
                                                   const double4 *__restrict__ tpos, const int32_t *__restrict__ near_ptr,
                                                   const int32_t *__restrict__ near_leaf,
                                                   const int32_t *__restrict__ tp_ptr, const int32_t *__restrict__ tp_col,
                                                   const int32_t *__restrict__ tp_src, const int32_t *__restrict__ tp_nk,
                                                   const double4 *__restrict__ spos, const double4 *__restrict__ wsrc,
-                                                  const int64_t *__restrict__ nf_ptr, int32_t *__restrict__ nf_col,
-                                                  double *__restrict__ nf_val)
+                                                  const int64_t *__restrict__ nf_ptr, double *__restrict__ nf_val)
 {
     const int t = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const int lane = threadIdx.x & 63;
@@ -756,40 +760,60 @@ __global__ __launch_bounds__(256) void k_nf_build(int T, const int32_t *__restri
                 const double wk = kind <= 1 ? w.x : (kind == 2 ? w.y : w.z);
                 v += wk / sqrt(dx * dx + dy * dy + dz * dz);
             }
-            nf_col[pos + j] = tp_col[b + j];
             nf_val[pos + j] = v;
         }
         pos += n;
     }
 }
 
-// Near field + vertex corrections: one wavefront per target (tree order).
-__global__ __launch_bounds__(256) void k_nf_apply(int T, const int64_t *__restrict__ nf_ptr,
-                                                  const int32_t *__restrict__ nf_col, const double *__restrict__ nf_val,
-                                                  const double2 *__restrict__ qe, const int32_t *__restrict__ tgt_node,
-                                                  const double2 *__restrict__ corr, double2 *__restrict__ pot)
+// Near field + vertex corrections: one wavefront per leaf with targets. The leaf's targets share
+// one column list (the runs of the adjacent leaves); each element value is gathered once and
+// applied to up to four targets' coefficient rows at a time.
+__global__ __launch_bounds__(256) void k_nf_apply(int ntl, const int32_t *__restrict__ tl_tb,
+                                                  const int32_t *__restrict__ tl_te, const int64_t *__restrict__ lc_ptr,
+                                                  const int32_t *__restrict__ lc_col, const int64_t *__restrict__ nf_ptr,
+                                                  const double *__restrict__ nf_val, const double2 *__restrict__ qe,
+                                                  const int32_t *__restrict__ tgt_node, const double2 *__restrict__ corr,
+                                                  double2 *__restrict__ pot)
 {
-    const int t = ((int)xcd_block(blockIdx.x, gridDim.x) * (int)blockDim.x + (int)threadIdx.x) >> 6;
+    const int w = ((int)xcd_block(blockIdx.x, gridDim.x) * (int)blockDim.x + (int)threadIdx.x) >> 6;
     const int lane = threadIdx.x & 63;
-    if (t >= T) return;
-    double pu = 0, pv = 0;
-    for (int64_t e = nf_ptr[t] + lane; e < nf_ptr[t + 1]; e += 64) {
-        const double w = nf_val[e];
-        const double2 q = qe[nf_col[e]];
-        pu += w * q.x;
-        pv += w * q.y;
-    }
-    pu = wsum64(pu);
-    pv = wsum64(pv);
-    if (lane == 0) {
-        const double2 c = corr[tgt_node[t]];
-        pot[t] = make_double2(pu + c.x, pv + c.y);
+    if (w >= ntl) return;
+    const int tb = tl_tb[w], te = tl_te[w];
+    const int64_t cb = lc_ptr[w];
+    const int nc = (int)(lc_ptr[w + 1] - cb);
+    for (int t0 = tb; t0 < te; t0 += 4) {
+        const int nt = min(4, te - t0);
+        int64_t vb[4];
+#pragma unroll
+        for (int i = 0; i < 4; i++) vb[i] = nf_ptr[t0 + min(i, nt - 1)];
+        double au[4] = {0, 0, 0, 0}, av[4] = {0, 0, 0, 0};
+        for (int j = lane; j < nc; j += 64) {
+            const double2 q = qe[lc_col[cb + j]];
+#pragma unroll
+            for (int i = 0; i < 4; i++)
+                if (i < nt) {
+                    const double v = nf_val[vb[i] + j];
+                    au[i] += v * q.x;
+                    av[i] += v * q.y;
+                }
+        }
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            if (i >= nt) break;
+            const double su = wsum64(au[i]), sv = wsum64(av[i]);
+            if (lane == 0) {
+                const double2 c = corr[tgt_node[t0 + i]];
+                pot[t0 + i] = make_double2(su + c.x, sv + c.y);
+            }
+        }
     }
 }
 
-// Far field: one block per evaluation group. Threads are (target slot, list slice) pairs; the
-// targets of a slice read the same multipole at the same time. The slices are summed through LDS,
-// then the L2P of the group's level-(L-2) ancestor is added and phi, phi_v are written.
+// Far field: one block per evaluation group. The group's source list is walked in chunks of CH
+// cells whose multipoles are staged in LDS; threads are (target slot, chunk slice) pairs. The
+// slices are summed through LDS, then the L2P of the group's level-(L-2) ancestor is added and
+// phi, phi_v are written.
 template <int P>
 __global__ __launch_bounds__(256) void k_fmm_m2p(const int32_t *__restrict__ grp_tb, const int32_t *__restrict__ grp_te,
                                                  const int32_t *__restrict__ grp_lcell,
@@ -801,46 +825,65 @@ __global__ __launch_bounds__(256) void k_fmm_m2p(const int32_t *__restrict__ grp
                                                  const double2 *__restrict__ pot, const int32_t *__restrict__ tgt_node,
                                                  NodeRec *__restrict__ nxt)
 {
-    constexpr int NCo = fcount(P);
+    constexpr int NCo = fcount(P), CW = 2 * NCo, CH = 16;
+    __shared__ double2 sM[CH * CW];
+    __shared__ double4 sC[CH];
+    __shared__ int sMask[CH];
     __shared__ double2 red[256];
     const int g = xcd_block(blockIdx.x, gridDim.x);
+    const int tid = threadIdx.x;
     const int tb = grp_tb[g], te = grp_te[g], ub = u_ptr[g], ue = u_ptr[g + 1], lc = grp_lcell[g];
     for (int t0 = tb; t0 < te; t0 += 64) {
         const int n = min(64, te - t0);
-        const int nsl = 256 / n;
-        const int slot = (int)threadIdx.x % n, slice = (int)threadIdx.x / n;
-        double pu = 0, pv = 0;
-        if (slice < nsl) {
-            const int t = t0 + slot;
-            const double4 y = tpos[t];
-            const int oc = tgt_oct[t];
-            for (int e = ub + slice; e < ue; e += nsl) {
-                if (!((u_mask[e] >> oc) & 1)) continue;
-                const int sc = u_cell[e];
-                const double4 c = cen[sc];
-                const double iw = 1.0 / Wlev[(int)c.w];
-                double su = 0, sv = 0;
-                m2p_eval2_t<P>(M + (size_t)sc * 2 * NCo, M + ((size_t)sc * 2 + 1) * NCo, (y.x - c.x) * iw,
-                               (y.y - c.y) * iw, (y.z - c.z) * iw, su, sv);
-                pu += su * iw;
-                pv += sv * iw;
-            }
+        const int nsl = min(256 / n, CH);
+        const int slot = tid % n, slice = tid / n;
+        const bool act = slice < nsl;
+        double4 y = make_double4(0, 0, 0, 0);
+        int oc = 0;
+        if (act) {
+            y = tpos[t0 + slot];
+            oc = tgt_oct[t0 + slot];
         }
-        red[threadIdx.x] = make_double2(pu, pv);
+        double pu = 0, pv = 0;
+        for (int cb = ub; cb < ue; cb += CH) {
+            const int nch = min(CH, ue - cb);
+            for (int i = tid; i < nch * CW; i += 256) {
+                const int jj = i / CW;
+                sM[i] = M[(size_t)u_cell[cb + jj] * CW + (i - jj * CW)];
+            }
+            if (tid < nch) {
+                const double4 c = cen[u_cell[cb + tid]];
+                sC[tid] = make_double4(c.x, c.y, c.z, 1.0 / Wlev[(int)c.w]);
+                sMask[tid] = u_mask[cb + tid];
+            }
+            __syncthreads();
+            if (act)
+                for (int jj = slice; jj < nch; jj += nsl) {
+                    if (!((sMask[jj] >> oc) & 1)) continue;
+                    const double4 c = sC[jj];
+                    double su = 0, sv = 0;
+                    m2p_eval2_t<P>(sM + jj * CW, sM + jj * CW + NCo, (y.x - c.x) * c.w, (y.y - c.y) * c.w,
+                                   (y.z - c.z) * c.w, su, sv);
+                    pu += su * c.w;
+                    pv += sv * c.w;
+                }
+            __syncthreads();
+        }
+        red[tid] = make_double2(pu, pv);
         __syncthreads();
-        if ((int)threadIdx.x < n) {
+        if (tid < n) {
             double su = 0, sv = 0;
             for (int s = 0; s < nsl; s++) {
-                const double2 r = red[s * n + threadIdx.x];
+                const double2 r = red[s * n + tid];
                 su += r.x;
                 sv += r.y;
             }
-            const int t = t0 + threadIdx.x;
+            const int t = t0 + tid;
             if (lc >= 0) {
-                const double4 y = tpos[t], c = cen[lc];
+                const double4 yy = tpos[t], c = cen[lc];
                 const double iw = 1.0 / Wlev[(int)c.w];
-                l2p_eval2_t<P>(Lx + (size_t)lc * 2 * NCo, Lx + ((size_t)lc * 2 + 1) * NCo, (y.x - c.x) * iw,
-                               (y.y - c.y) * iw, (y.z - c.z) * iw, su, sv);
+                l2p_eval2_t<P>(Lx + (size_t)lc * CW, Lx + (size_t)lc * CW + NCo, (yy.x - c.x) * iw, (yy.y - c.y) * iw,
+                               (yy.z - c.z) * iw, su, sv);
             }
             const double2 pn = pot[t];
             NodeRec *r = nxt + tgt_node[t];
@@ -930,6 +973,14 @@ int fmm_create(Ctx &c)
         std::vector<double> W(P->W.begin(), P->W.end());
         c.fmm_W = dput(W);
         fmm_build_near(c, *P);
+        // interaction counts (benchmark work models)
+        for (size_t g = 0; g < P->grp_tb.size(); g++)
+            for (int t = P->grp_tb[g]; t < P->grp_te[g]; t++)
+                for (int k = P->u_ptr[g]; k < P->u_ptr[g + 1]; k++) P->m2p_pairs += (P->u_mask[k] >> P->tgt_oct[t]) & 1;
+        for (int q = P->tgt_cells_lev_off[2]; q < P->tgt_cells_lev_off[P->ls]; q++) {
+            const int ci = P->tgt_cells[q];
+            P->m2l_pairs += P->il_ptr[ci + 1] - P->il_ptr[ci];
+        }
         return FG_OK;
     } catch (const std::string &e) {
         c.err = e;
@@ -985,12 +1036,29 @@ void fmm_build_near(Ctx &c, FmmPlan &P)
         nf_ptr[t + 1] = nf_ptr[t] + n;
     }
     P.nf_nnz = nf_ptr[T];
+    // leaves with targets and their shared column lists
+    std::vector<int32_t> tl_tb, tl_te, lc_col;
+    std::vector<int64_t> lc_ptr(1, 0);
+    for (int li = 0; li < NL; li++) {
+        if (P.leaf_te[li] <= P.leaf_tb[li]) continue;
+        tl_tb.push_back(P.leaf_tb[li]);
+        tl_te.push_back(P.leaf_te[li]);
+        for (int e = P.near_ptr[li]; e < P.near_ptr[li + 1]; e++) {
+            const int sl = P.near_leaf[e];
+            lc_col.insert(lc_col.end(), tp_col.begin() + tp_ptr[sl], tp_col.begin() + tp_ptr[sl + 1]);
+        }
+        lc_ptr.push_back((int64_t)lc_col.size());
+    }
+    P.ntl = (int)tl_tb.size();
+    P.lc_cols = (int64_t)lc_col.size();
+    P.d_tl_tb = dput(tl_tb);
+    P.d_tl_te = dput(tl_te);
+    P.d_lc_ptr = dput(lc_ptr);
+    P.d_lc_col = dput(lc_col);
     int32_t *d_tp_ptr = dput(tp_ptr), *d_tp_col = dput(tp_col), *d_tp_src = dput(tp_src), *d_tp_nk = dput(tp_nk);
     int32_t *d_near_ptr = dput(P.near_ptr), *d_near_leaf = dput(P.near_leaf), *d_tgt_leaf = dput(P.tgt_leaf);
     P.d_nf_ptr = dput(nf_ptr);
     void *d = nullptr;
-    if (hipMalloc(&d, std::max<int64_t>(1, P.nf_nnz) * sizeof(int32_t)) != hipSuccess) throw std::string("hipMalloc near-field columns");
-    P.d_nf_col = (int32_t *)d;
     if (hipMalloc(&d, std::max<int64_t>(1, P.nf_nnz) * sizeof(double)) != hipSuccess) throw std::string("hipMalloc near-field values");
     P.d_nf_val = (double *)d;
     if (hipMalloc(&d, sizeof(double4) * S) != hipSuccess) throw std::string("hipMalloc weights");
@@ -1003,7 +1071,7 @@ void fmm_build_near(Ctx &c, FmmPlan &P)
     if (T > 0)
         k_nf_build<<<(unsigned)(((int64_t)T * 64 + 255) / 256), 256, 0, c.stream>>>(
             T, d_tgt_leaf, P.d_tgt_pos, d_near_ptr, d_near_leaf, d_tp_ptr, d_tp_col, d_tp_src, d_tp_nk, c.src_pos,
-            wsrc, P.d_nf_ptr, P.d_nf_col, P.d_nf_val);
+            wsrc, P.d_nf_ptr, P.d_nf_val);
     const hipError_t err = hipStreamSynchronize(c.stream);
     void *tmp[] = {d_tp_ptr, d_tp_col, d_tp_src, d_tp_nk, d_near_ptr, d_near_leaf, d_tgt_leaf, wsrc};
     for (void *p : tmp) (void)hipFree(p);
@@ -1017,8 +1085,8 @@ void fmm_destroy(Ctx &c)
     void *ptrs[] = {P->d_cen, P->d_parent, P->d_oct, P->d_ch_ptr, P->d_ch_idx, P->d_il_ptr, P->d_il_src,
                     P->d_il_off, P->d_leaf_sb, P->d_leaf_se, P->d_grp_tb, P->d_grp_te, P->d_grp_lcell, P->d_u_ptr,
                     P->d_u_cell, P->d_u_mask, P->d_tgt_oct, P->d_tgt_node, P->d_tgt_pos, P->d_Rc, P->d_Itf,
-                    P->d_Rcf, P->d_M, P->d_L, P->d_src_cells, P->d_tgt_cells, P->d_nf_ptr, P->d_nf_col,
-                    P->d_nf_val, P->d_pot, c.qe, c.fmm_W};
+                    P->d_Rcf, P->d_M, P->d_L, P->d_src_cells, P->d_tgt_cells, P->d_nf_ptr, P->d_nf_val,
+                    P->d_tl_tb, P->d_tl_te, P->d_lc_ptr, P->d_lc_col, P->d_pot, c.qe, c.fmm_W};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     c.fmm_W = nullptr;
@@ -1033,12 +1101,12 @@ static void run_fmm(Ctx &c)
     FmmPlan &P = *c.fmm;
     const int L = P.L, nc = P.nc;
     const int NL = P.lev_off[L + 1] - P.lev_off[L];
-    const int T = (int)P.tgt_node.size();
     {
         Prof pr(c, "fmm_near");
-        if (T > 0)
-            k_nf_apply<<<(unsigned)(((int64_t)T * 64 + 255) / 256), 256, 0, c.stream>>>(
-                T, P.d_nf_ptr, P.d_nf_col, P.d_nf_val, c.qe, P.d_tgt_node, c.corr, P.d_pot);
+        if (P.ntl > 0)
+            k_nf_apply<<<(unsigned)(((int64_t)P.ntl * 64 + 255) / 256), 256, 0, c.stream>>>(
+                P.ntl, P.d_tl_tb, P.d_tl_te, P.d_lc_ptr, P.d_lc_col, P.d_nf_ptr, P.d_nf_val, c.qe, P.d_tgt_node,
+                c.corr, P.d_pot);
     }
     {
         Prof pr(c, "fmm_p2m");
@@ -1081,6 +1149,26 @@ static void run_fmm(Ctx &c)
                                                     P.d_u_mask, P.d_tgt_oct, P.d_tgt_pos, P.d_cen, c.fmm_W, P.d_M,
                                                     P.d_L, P.d_pot, P.d_tgt_node, c.rec[1 - c.cur]);
     }
+}
+
+void fmm_info(const Ctx &c, int64_t info[16])
+{
+    const FmmPlan &P = *c.fmm;
+    for (int k = 0; k < 16; k++) info[k] = 0;
+    info[0] = P.L;
+    info[1] = P.ncell();
+    info[2] = P.lev_off[P.L + 1] - P.lev_off[P.L];
+    info[3] = (int64_t)P.grp_tb.size();
+    info[4] = (int64_t)P.u_cell.size();
+    info[5] = P.m2p_pairs;
+    info[6] = P.m2l_pairs;
+    info[7] = P.nf_nnz;
+    info[8] = P.lc_cols;
+    info[9] = (int64_t)P.tgt_node.size();
+    info[10] = (int64_t)P.src_sorted.size();
+    info[11] = P.p;
+    info[12] = P.ntl;
+    info[13] = P.src_cells_lev_off[P.L] - P.src_cells_lev_off[2];  // M2M target cells
 }
 
 void launch_fmm(Ctx &c)

@@ -138,6 +138,15 @@ class Engine:
             k += 1
         return out
 
+    def fmm_info(self):
+        """Octree plan statistics of an FMM context (keys as in fg_fmm_info), or None."""
+        a = np.zeros(16, dtype=np.int64)
+        if self.lib.fg_fmm_info(self._h, nv.lptr(a)) != nv.FG_OK:
+            return None
+        keys = ["leaf_level", "cells", "leaves", "groups", "list_entries", "m2p_pairs", "m2l_pairs",
+                "near_coefs", "near_columns", "targets", "sources", "order", "target_leaves", "m2m_cells"]
+        return {k: int(a[i]) for i, k in enumerate(keys)}
+
     def reset_kernel_stats(self):
         self.lib.fg_reset_kernel_stats(self._h)
 

@@ -83,6 +83,7 @@ HIP_SYMBOLS = {
     "fg_get_solution": (C.c_int, [C.c_void_p, _d]),
     "fg_kernel_stats": (C.c_int, [C.c_void_p, C.c_int, C.c_char_p, C.c_int, _i64, _d]),
     "fg_reset_kernel_stats": (C.c_int, [C.c_void_p]),
+    "fg_fmm_info": (C.c_int, [C.c_void_p, _i64]),
     "fg_synchronize": (C.c_int, [C.c_void_p]),
     "fg_fmm_reference": (C.c_int, [_d, C.c_int64, _d, C.c_int64, _d, _d, C.c_int, C.c_int, _d, _d,
                                    C.POINTER(C.c_int)]),

@@ -117,7 +117,7 @@ typedef struct fg_options {
     int32_t max_iter;      /* finite_element_solver.max(iter) (default 700) */
     int32_t demag_method;  /* FG_DEMAG_DIRECT or FG_DEMAG_FMM */
     int32_t fmm_order;     /* expansion degree p of the FMM (default 8) */
-    int32_t fmm_leaf;      /* target number of sources per leaf (default 128; 0 = autotune) */
+    int32_t fmm_leaf;      /* target mean number of sources per leaf (default 64; 0 = default) */
     int32_t precision;     /* FG_PREC_FP64 or FG_PREC_MIXED */
     int32_t device;        /* HIP device ordinal */
     int32_t profile;       /* nonzero: time every kernel with HIP events */
@@ -189,6 +189,12 @@ int fg_reset_kernel_stats(fg_ctx *ctx);
 int fg_fmm_reference(const double *src_xyz, int64_t S, const double *tgt_xyz, int64_t T,
                      const double *q_u, const double *q_v, int order, int leaf, double *pot_u,
                      double *pot_v, int *leaf_level);
+/* Octree FMM plan of an FG_DEMAG_FMM context (FG_ERR_STATE otherwise): info[0] leaf level,
+ * [1] cells, [2] leaves, [3] evaluation groups, [4] source-list entries, [5] multipole-to-target
+ * pairs, [6] M2L pairs, [7] near-field coefficients, [8] near-field shared columns, [9] targets,
+ * [10] sources, [11] expansion order, [12] leaves with targets, [13] M2M parent cells. Used by the
+ * benchmark's work models. */
+int fg_fmm_info(fg_ctx *ctx, int64_t info[16]);
 /* Wait for all work queued on the context's stream. */
 int fg_synchronize(fg_ctx *ctx);
 /* Last error message of this context (or of fg_create when ctx is NULL). */
