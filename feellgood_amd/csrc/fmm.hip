@@ -113,7 +113,7 @@ namespace fg {
 std::string fmm_build_host(FmmPlan &P, const std::vector<double4> &src, const std::vector<int32_t> &tnode,
                            const std::vector<double4> &tpos, int p, int leaf_target)
 {
-    if (p < 2 || p > 10 || (p & 1)) return "fmm_order must be even, 2..10";
+    if (p != 4 && p != 6 && p != 8 && p != 10) return "fmm_order must be 4, 6, 8 or 10";
     P.p = p;
     P.nc = fcount(p);
     P.ni = fcount(2 * p);

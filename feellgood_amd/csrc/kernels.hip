@@ -1,7 +1,7 @@
 // kernels.hip -- element pass, node assembly, Krylov, update, demag sources, energies.
 //
 // Data layout (all FP64):
-//   node state     NodeRec[N] x 2 (CURRENT / NEXT, swapped by evolution)      64 B per node
+//   node state     NodeRec[N] x 2 (CURRENT / NEXT; evolution copies NEXT)      64 B per node
 //   basis          ep4[N], eq4[N] (double4)
 //   tet geometry   SoA geo[17][Tm] (da 4x3 row-major, w[5]); nodes int4[Tm]
 //   element output elem_out[4 Tm (+3 Fk)] double4 = (alpha_eff-weighted mass, BE x, y, z)

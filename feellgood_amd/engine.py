@@ -14,7 +14,7 @@ class EngineError(RuntimeError):
     pass
 
 
-def options(tol=1e-6, max_iter=700, demag="direct", fmm_order=8, fmm_leaf=128, precision="fp64",
+def options(tol=1e-6, max_iter=700, demag="direct", fmm_order=8, fmm_leaf=64, precision="fp64",
             device=0, profile=False):
     o = nv.Options()
     nv.hip_lib().fg_default_options(C.byref(o))

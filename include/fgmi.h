@@ -116,7 +116,7 @@ typedef struct fg_options {
     double tol;            /* finite_element_solver.tolerance (default 1e-6) */
     int32_t max_iter;      /* finite_element_solver.max(iter) (default 700) */
     int32_t demag_method;  /* FG_DEMAG_DIRECT or FG_DEMAG_FMM */
-    int32_t fmm_order;     /* expansion degree p of the FMM (default 8) */
+    int32_t fmm_order;     /* expansion degree p of the FMM: 4, 6, 8 or 10 (default 8) */
     int32_t fmm_leaf;      /* target mean number of sources per leaf (default 64; 0 = default) */
     int32_t precision;     /* FG_PREC_FP64 or FG_PREC_MIXED */
     int32_t device;        /* HIP device ordinal */
