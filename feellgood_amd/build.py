@@ -57,13 +57,15 @@ def build_hip():
 def build_host():
     os.makedirs(LIB, exist_ok=True)
     srcs = sorted(glob.glob(os.path.join(HERE, "host", "*.cpp")))
-    lib_srcs = [s for s in srcs if not s.endswith("driver.cpp")]
+    drv_only = ("driver.cpp", "settings.cpp")
+    lib_srcs = [s for s in srcs if not s.endswith(drv_only)]
     tmp = os.path.join(LIB, "libfgmi_host.so.tmp")
     _run(["g++"] + HOST_FLAGS + ["-shared", "-o", tmp] + lib_srcs)
     os.replace(tmp, os.path.join(LIB, "libfgmi_host.so"))
     drv = os.path.join(HERE, "host", "driver.cpp")
     if os.path.exists(drv):
         _run(["g++"] + HOST_FLAGS + ["-o", os.path.join(LIB, "fgmi_run"), drv,
+                                     os.path.join(HERE, "host", "settings.cpp"),
                                      "-L" + LIB, "-lfgmi_host", "-lfgmi", "-Wl,-rpath,$ORIGIN",
                                      "-Wl,-rpath-link," + "/opt/rocm/lib"])
 

@@ -1,0 +1,569 @@
+// driver.cpp -- fgmi_run, the drop-in time loop: reads FeeLLGood settings (YAML or JSON, a file
+// or "-" for stdin) and the .msh mesh, runs the reference's integration loop on the gfx950
+// engine through the C-ABI, and writes the .evol and .sol outputs.
+//
+// Follows src/main.cpp:103-232 (options, random seed), src/settings.cpp:314-616 (settings and
+// defaults from src/default-settings.yml), src/time_integration.cpp:11-38,136-254 (TimeStepper
+// and the loop with its retry semantics) and src/save.cpp:13-186 (.evol columns, .sol files).
+// Not supported, reported as errors: expressions in initial_magnetization or Bext (only constant
+// vectors and .sol files), region-qualified evol columns, Thiele_length, recentering and spin
+// accumulation.
+#include <unistd.h>
+
+#include <algorithm>
+#include <cerrno>
+#include <cfloat>
+#include <chrono>
+#include <climits>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <ctime>
+#include <fstream>
+#include <iostream>
+#include <random>
+#include <sstream>
+#include <string>
+#include <sys/stat.h>
+#include <vector>
+
+#include "../../include/fgmi.h"
+#include "../../include/fgmi_host.h"
+#include "settings.h"
+
+namespace {
+
+using fgh::Node;
+using fgh::SettingsError;
+
+constexpr double MU0 = 1.25663706127e-6;
+const char *VERSION = "feellgood_amd 1.0 (gfx950)";
+
+struct Config {
+    std::string settings_name;
+    std::string out_dir = ".", basename;
+    double evol_dt = 5e-13, final_time = 1e-10;
+    std::vector<std::string> cols = {"t", "<Mx>", "<My>", "<Mz>", "E_ex", "E_demag", "E_zeeman", "E_tot"};
+    long mag_every = 100;  // 0: no .sol output
+    std::vector<std::pair<std::string, std::string>> metadata;
+    std::string mesh_file;
+    double length_unit = 1e-9;
+    std::vector<std::string> vol_names, surf_names;
+    std::vector<fg_volume_region> vol;
+    std::vector<fg_surface_region> surf;
+    std::string init_sol;
+    double m0[3] = {0, 0, 1};
+    bool has_t0 = false;
+    double t0 = 0;
+    double Bext[3] = {0, 0, 0};
+    long max_iter = 700;
+    double tol = 1e-6;
+    double du_max = 0.02, dt_min = 1e-16, dt_max = 5e-13;
+    int demag = FG_DEMAG_FMM, fmm_order = 8, fmm_leaf = 64, precision = FG_PREC_FP64, device = 0;
+};
+
+void normalize(double v[3], const std::string &where)
+{
+    const double n = std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+    if (n == 0) throw SettingsError(where + " should not be the null vector");
+    for (int k = 0; k < 3; k++) v[k] /= n;
+}
+
+fg_volume_region default_volume()
+{
+    // src/default-settings.yml: mesh.volume_regions.__default__
+    fg_volume_region r{};
+    r.alpha = 0.5;
+    r.Ae = 1e-11;
+    r.Ms = 795774.7;
+    r.uk[2] = 1;
+    r.ex[0] = 1;
+    r.ey[1] = 1;
+    r.ez[2] = 1;
+    return r;
+}
+
+void check_map(const Node *n, const std::string &where)
+{
+    if (n && !n->is_null() && n->kind != Node::Map) throw SettingsError(where + " should be a map");
+}
+
+Config read_config(const Node &root)
+{
+    Config c;
+    if (root.kind != Node::Map) throw SettingsError("the settings should be a map");
+    const Node *o = root.get("outputs");
+    check_map(o, "outputs");
+    if (o && o->kind == Node::Map) {
+        if (auto n = o->get("directory"); n && !n->is_null()) {
+            c.out_dir = fgh::as_string(*n, "outputs.directory");
+            if (c.out_dir.empty()) c.out_dir = ".";
+            if (c.out_dir.size() > 1 && c.out_dir.back() == '/') c.out_dir.pop_back();
+        }
+        if (auto n = o->get("file_basename"); n && !n->is_null()) c.basename = fgh::as_string(*n, "outputs.file_basename");
+        if (auto n = o->get("evol_time_step"); n && !n->is_null()) c.evol_dt = fgh::as_double(*n, "outputs.evol_time_step");
+        if (auto n = o->get("final_time"); n && !n->is_null()) c.final_time = fgh::as_double(*n, "outputs.final_time");
+        if (auto n = o->get("evol_columns"); n && !n->is_null()) {
+            if (n->kind != Node::Seq || n->items.empty()) throw SettingsError("outputs.evol_columns should be a list");
+            c.cols.clear();
+            for (auto &it : n->items) c.cols.push_back(fgh::as_string(it, "outputs.evol_columns"));
+        }
+        if (auto n = o->get("mag_config_every"); n && !n->is_null()) {
+            if (n->kind == Node::Scalar && (n->text == "false" || n->text == "False")) c.mag_every = 0;
+            else c.mag_every = fgh::as_long(*n, "outputs.mag_config_every");
+        }
+        if (auto n = o->get("metadata"); n && !n->is_null()) {
+            if (n->kind != Node::Map) throw SettingsError("outputs.metadata should be a map.");
+            for (auto &f : n->fields) c.metadata.emplace_back(f.first, f.second.is_null() ? "" : f.second.text);
+        }
+    }
+    const Node *m = root.get("mesh");
+    if (!m || m->kind != Node::Map) throw SettingsError("mesh.filename is required");
+    if (auto n = m->get("filename"); n && !n->is_null()) c.mesh_file = fgh::as_string(*n, "mesh.filename");
+    if (c.mesh_file.empty()) throw SettingsError("mesh.filename is required");
+    if (auto n = m->get("length_unit"); n && !n->is_null()) c.length_unit = fgh::as_double(*n, "mesh.length_unit");
+    const Node *vr = m->get("volume_regions");
+    check_map(vr, "mesh.volume_regions");
+    if (vr && vr->kind == Node::Map)
+        for (auto &f : vr->fields) {
+            const std::string w = "mesh.volume_regions." + f.first;
+            check_map(&f.second, w);
+            fg_volume_region r = default_volume();
+            const Node &p = f.second;
+            if (p.kind == Node::Map) {
+                if (auto n = p.get("Ae")) r.Ae = fgh::as_double(*n, w + ".Ae");
+                if (auto n = p.get("Ms")) r.Ms = fgh::as_double(*n, w + ".Ms");
+                if (auto n = p.get("K")) r.K = fgh::as_double(*n, w + ".K");
+                if (auto n = p.get("uk")) fgh::as_vec3(*n, r.uk, w + ".uk");
+                if (auto n = p.get("K3")) r.K3 = fgh::as_double(*n, w + ".K3");
+                if (auto n = p.get("ex")) fgh::as_vec3(*n, r.ex, w + ".ex");
+                if (auto n = p.get("ey")) fgh::as_vec3(*n, r.ey, w + ".ey");
+                if (auto n = p.get("ez")) fgh::as_vec3(*n, r.ez, w + ".ez");
+                if (auto n = p.get("alpha_LLG")) r.alpha = fgh::as_double(*n, w + ".alpha_LLG");
+            }
+            normalize(r.uk, w + ".uk");
+            normalize(r.ex, w + ".ex");
+            normalize(r.ey, w + ".ey");
+            normalize(r.ez, w + ".ez");
+            c.vol_names.push_back(f.first);
+            c.vol.push_back(r);
+        }
+    if (c.vol.empty()) throw SettingsError("mesh.volume_regions should name at least one region");
+    const Node *sr = m->get("surface_regions");
+    check_map(sr, "mesh.surface_regions");
+    if (sr && sr->kind == Node::Map)
+        for (auto &f : sr->fields) {
+            const std::string w = "mesh.surface_regions." + f.first;
+            check_map(&f.second, w);
+            fg_surface_region r{};
+            r.uk[2] = 1;
+            const Node &p = f.second;
+            if (p.kind == Node::Map) {
+                if (auto n = p.get("Ks")) r.Ks = fgh::as_double(*n, w + ".Ks");
+                if (auto n = p.get("uk")) fgh::as_vec3(*n, r.uk, w + ".uk");
+                if (auto n = p.get("suppress_charges")) r.suppress_charges = fgh::as_bool(*n, w + ".suppress_charges");
+            }
+            normalize(r.uk, w + ".uk");
+            c.surf_names.push_back(f.first);
+            c.surf.push_back(r);
+        }
+    if (auto n = root.get("initial_magnetization"); n && !n->is_null()) {
+        if (n->kind == Node::Scalar) c.init_sol = n->text;
+        else {
+            try {
+                fgh::as_vec3(*n, c.m0, "initial_magnetization");
+            } catch (const SettingsError &) {
+                throw SettingsError("initial_magnetization: only a .sol file name or three numbers are supported "
+                                    "(expressions of x, y, z are not)");
+            }
+        }
+    }
+    if (auto n = root.get("initial_time"); n && !n->is_null()) {
+        c.has_t0 = true;
+        c.t0 = fgh::as_double(*n, "initial_time");
+    }
+    if (auto n = root.get("Bext"); n && !n->is_null()) {
+        try {
+            fgh::as_vec3(*n, c.Bext, "Bext");
+        } catch (const SettingsError &) {
+            throw SettingsError("Bext: only three numbers are supported (expressions of t are not)");
+        }
+    }
+    if (auto r = root.get("recentering"); r && r->kind == Node::Map)
+        if (auto n = r->get("enable"); n && !n->is_null() && fgh::as_bool(*n, "recentering.enable"))
+            throw SettingsError("recentering is not supported");
+    if (auto r = root.get("spin_accumulation"); r && r->kind == Node::Map)
+        if (auto n = r->get("enable"); n && !n->is_null() && fgh::as_bool(*n, "spin_accumulation.enable"))
+            throw SettingsError("spin_accumulation is not supported");
+    if (auto s = root.get("finite_element_solver"); s && s->kind == Node::Map) {
+        if (auto n = s->get("max(iter)"); n && !n->is_null()) c.max_iter = fgh::as_long(*n, "finite_element_solver.max(iter)");
+        if (auto n = s->get("tolerance"); n && !n->is_null()) c.tol = fgh::as_double(*n, "finite_element_solver.tolerance");
+    }
+    if (auto s = root.get("time_integration"); s && s->kind == Node::Map) {
+        if (auto n = s->get("max(du)"); n && !n->is_null()) c.du_max = fgh::as_double(*n, "time_integration.max(du)");
+        if (auto n = s->get("min(dt)"); n && !n->is_null()) c.dt_min = fgh::as_double(*n, "time_integration.min(dt)");
+        if (auto n = s->get("max(dt)"); n && !n->is_null()) c.dt_max = fgh::as_double(*n, "time_integration.max(dt)");
+    }
+    // engine section (not in the reference): demag method, FMM order and leaf size, precision
+    if (auto s = root.get("gpu"); s && s->kind == Node::Map) {
+        if (auto n = s->get("demag"); n && !n->is_null()) {
+            const std::string v = fgh::as_string(*n, "gpu.demag");
+            if (v == "fmm") c.demag = FG_DEMAG_FMM;
+            else if (v == "direct") c.demag = FG_DEMAG_DIRECT;
+            else throw SettingsError("gpu.demag should be fmm or direct");
+        }
+        if (auto n = s->get("fmm_order"); n && !n->is_null()) c.fmm_order = (int)fgh::as_long(*n, "gpu.fmm_order");
+        if (auto n = s->get("fmm_leaf"); n && !n->is_null()) c.fmm_leaf = (int)fgh::as_long(*n, "gpu.fmm_leaf");
+        if (auto n = s->get("precision"); n && !n->is_null()) {
+            const std::string v = fgh::as_string(*n, "gpu.precision");
+            if (v == "fp64") c.precision = FG_PREC_FP64;
+            else if (v == "mixed") c.precision = FG_PREC_MIXED;
+            else throw SettingsError("gpu.precision should be fp64 or mixed");
+        }
+        if (auto n = s->get("device"); n && !n->is_null()) c.device = (int)fgh::as_long(*n, "gpu.device");
+    }
+    for (auto &col : c.cols) {
+        static const char *known[] = {"iter", "t", "dt", "max_dm", "max_angle", "<Mx>", "<My>", "<Mz>", "<dMx/dt>",
+                                      "<dMy/dt>", "<dMz/dt>", "E_ex", "E_aniso", "E_demag", "E_zeeman", "E_tot",
+                                      "Hx", "Hy", "Hz"};
+        if (std::find_if(std::begin(known), std::end(known), [&](const char *k) { return col == k; }) == std::end(known))
+            throw SettingsError("outputs.evol_columns: unsupported column '" + col + "'");
+    }
+    if (c.basename.empty()) {
+        std::string b = c.mesh_file;
+        const size_t sl = b.rfind('/');
+        if (sl != std::string::npos) b = b.substr(sl + 1);
+        const size_t dot = b.rfind('.');
+        if (dot != std::string::npos && dot > 0) b = b.substr(0, dot);
+        c.basename = b;
+    }
+    return c;
+}
+
+std::string date_string()
+{
+    const std::time_t t = std::time(nullptr);
+    char buf[64];
+    std::strftime(buf, sizeof buf, "%Y-%m-%dT%H:%M:%S%z", std::localtime(&t));
+    return buf;
+}
+
+std::string common_metadata(const Config &c)
+{
+    std::ostringstream ss;
+    ss << "## feeLLGood version: " << VERSION << "\n";
+    char name[HOST_NAME_MAX + 1];
+    if (gethostname(name, sizeof name) == 0) ss << "## hostname: " << name << "\n";
+    ss << "## real-world time: " << date_string() << "\n";
+    ss << "## settings file: " << c.settings_name << "\n";
+    for (auto &kv : c.metadata) {
+        std::string v = kv.second;
+        if (!v.empty() && v.back() == '\n') v.pop_back();
+        std::string esc;
+        for (char ch : v) esc += ch == '\n' ? std::string("\n##  ") : std::string(1, ch);
+        ss << "## " << kv.first << ": " << esc << "\n";
+    }
+    return ss.str();
+}
+
+struct Engine {
+    fg_ctx *ctx = nullptr;
+    void check(int rc, const char *what)
+    {
+        if (rc < 0) {
+            std::ostringstream o;
+            o << what << ": " << fg_last_error(ctx);
+            throw std::runtime_error(o.str());
+        }
+    }
+};
+
+int run(const Config &c, unsigned seed, bool verbose, bool check_only)
+{
+    char err[512] = {0};
+    fgh_mesh *mesh = fgh_read_msh(c.mesh_file.c_str(), err, sizeof err);
+    if (!mesh) throw std::runtime_error(std::string("mesh: ") + err);
+    std::vector<const char *> vn, sn;
+    for (auto &s : c.vol_names) vn.push_back(s.c_str());
+    for (auto &s : c.surf_names) sn.push_back(s.c_str());
+    if (fgh_prepare(mesh, c.length_unit, (int)vn.size(), vn.data(), c.vol.data(), (int)sn.size(), sn.data(),
+                    c.surf.data()) != FG_OK)
+        throw std::runtime_error(std::string("mesh: ") + fgh_last_error());
+    fg_mesh view;
+    fgh_view(mesh, &view);
+    const int N = view.n_nodes;
+    std::cout << "mesh:              " << c.mesh_file << " (" << N << " nodes, " << view.n_tets << " tetrahedra, "
+              << view.n_tris << " triangles)\n";
+    if (check_only) {
+        // settings and mesh only, no device: what the run would use
+        std::cout.precision(10);
+        for (size_t k = 0; k < c.vol.size(); k++)
+            std::cout << "volume region:     " << c.vol_names[k] << " Ae " << c.vol[k].Ae << " Ms " << c.vol[k].Ms
+                      << " K " << c.vol[k].K << " uk " << c.vol[k].uk[0] << " " << c.vol[k].uk[1] << " "
+                      << c.vol[k].uk[2] << " K3 " << c.vol[k].K3 << " alpha " << c.vol[k].alpha << "\n";
+        for (size_t k = 0; k < c.surf.size(); k++)
+            std::cout << "surface region:    " << c.surf_names[k] << " Ks " << c.surf[k].Ks << " suppress_charges "
+                      << c.surf[k].suppress_charges << "\n";
+        std::cout << "Bext:              " << c.Bext[0] << " " << c.Bext[1] << " " << c.Bext[2] << "\n";
+        std::cout << "initial:           " << (c.init_sol.empty() ? "constant" : c.init_sol) << " " << c.m0[0] << " "
+                  << c.m0[1] << " " << c.m0[2] << "\n";
+        std::cout << "time:              final " << c.final_time << " evol " << c.evol_dt << " dt " << c.dt_min << " "
+                  << c.dt_max << " du " << c.du_max << "\n";
+        std::cout << "solver:            tol " << c.tol << " max_iter " << c.max_iter << "\n";
+        std::cout << "outputs:           " << c.out_dir << "/" << c.basename << " every " << c.mag_every << " columns";
+        for (auto &col : c.cols) std::cout << " " << col;
+        std::cout << "\n";
+        fgh_free(mesh);
+        return 0;
+    }
+
+    std::vector<double> u(3 * (size_t)N), phi(N, 0.0), v(3 * (size_t)N, 0.0);
+    double t = c.has_t0 ? c.t0 : 0.0;
+    if (!c.init_sol.empty()) {
+        double ts = 0;
+        if (fgh_read_sol(mesh, c.init_sol.c_str(), u.data(), phi.data(), &ts) != FG_OK)
+            throw std::runtime_error(std::string("initial magnetization: ") + fgh_last_error());
+        if (!c.has_t0) t = ts;
+    } else {
+        double m0[3] = {c.m0[0], c.m0[1], c.m0[2]};
+        normalize(m0, "initial_magnetization");
+        for (int i = 0; i < N; i++)
+            for (int k = 0; k < 3; k++) u[3 * (size_t)i + k] = m0[k];
+    }
+
+    Engine e;
+    fg_options opt;
+    fg_default_options(&opt);
+    opt.tol = c.tol;
+    opt.max_iter = (int)c.max_iter;
+    opt.demag_method = c.demag;
+    opt.fmm_order = c.fmm_order;
+    opt.fmm_leaf = c.fmm_leaf;
+    opt.precision = c.precision;
+    opt.device = c.device;
+    if (fg_create(&view, &opt, &e.ctx) != FG_OK) throw std::runtime_error(std::string("fg_create: ") + fg_last_error(nullptr));
+
+    // src/main.cpp:211 and the per-step draw of src/linear_algebra.cpp:5-10
+    srand(seed);
+    auto next_angle = []() {
+        std::mt19937 gen(rand());
+        std::uniform_real_distribution<> distrib(0.0, 1.0);
+        return M_2_PI * distrib(gen);
+    };
+    e.check(fg_set_state(e.ctx, FG_BOTH, u.data(), nullptr, phi.data(), nullptr), "fg_set_state");
+    e.check(fg_base_projection(e.ctx, next_angle()), "fg_base_projection");  // LinAlgebra ctor
+
+    const double Hext[3] = {c.Bext[0] / MU0, c.Bext[1] / MU0, c.Bext[2] / MU0};
+    // timing (src/time_integration.h:6-59)
+    const double DTMIN = c.dt_min, DTMAX = c.dt_max, TAUR = 100. * DTMAX;
+    double dt = 0, prefactor = 1;
+    auto set_dt = [&](double d) {
+        dt = d;
+        const double tt = d / TAUR;
+        prefactor = 1. + tt * std::fabs(std::log(tt));
+    };
+    set_dt(std::sqrt(DTMIN * DTMAX));
+    double E[4] = {0, 0, 0, 0};
+    double vmax = 0;
+    auto compute_all = [&]() {  // src/fem.h:203-224
+        e.check(fg_calc_demag(e.ctx), "fg_calc_demag");
+        e.check(fg_energy(e.ctx, Hext, E), "fg_energy");
+        e.check(fg_evolution(e.ctx), "fg_evolution");
+    };
+    compute_all();
+
+    mkdir(c.out_dir.c_str(), 0777);
+    const std::string base = c.out_dir + "/" + c.basename;
+    std::ofstream fout(base + ".evol");
+    if (!fout) throw std::runtime_error("cannot open " + base + ".evol");
+    fout << common_metadata(c) << "## columns: ";
+    for (size_t k = 0; k < c.cols.size(); k++) fout << c.cols[k] << (k + 1 < c.cols.size() ? "\t" : "\n");
+    fout.precision(16);
+
+    const double t_initial = t, t_step = c.evol_dt;
+    const int step_count = (int)std::lround((c.final_time - t_initial) / t_step);
+    // TimeStepper (src/time_integration.cpp:11-38)
+    const double hard_min = DTMIN, hard_max = DTMAX * (1 + FLT_EPSILON);
+    double soft_max = dt;
+    auto stepper = [&](double stride) {
+        double step = std::min(stride, soft_max);
+        if (step > stride - 2 * hard_min && step < stride) step = stride - 2 * hard_min;
+        soft_max = std::max(soft_max, std::min(step * 1.1, hard_max));
+        return step;
+    };
+    auto set_soft_limit = [&](double mx) { soft_max = std::min(soft_max, mx); };
+
+    int status = 0, nt = 0, flag = 0, nt_output = 0, bad = 0;
+    long total_iter = 0;
+    double max_angle = 0;
+    e.check(fg_max_angle(e.ctx, &max_angle), "fg_max_angle");
+    const auto wall0 = std::chrono::steady_clock::now();
+    for (int step_nb = 0; step_nb <= step_count; step_nb++) {
+        const double t_target = t_initial + step_nb * t_step;
+        while (t < t_target) {
+            set_dt(stepper(t_target - t));
+            const bool last_step = (dt == t_target - t);
+            if (verbose)
+                std::cout << "evol step = " << nt_output << ", step = " << nt << ", t = " << t << ", dt = " << dt
+                          << (flag ? "  (retry)" : "") << '\n';
+            if (dt < DTMIN) {
+                std::cout << "\n**ABORTED**: dt < DTMIN\n";
+                status = 1;
+                goto bailout;
+            }
+            e.check(fg_base_projection(e.ctx, next_angle()), "fg_base_projection");
+            e.check(fg_prepare_elements(e.ctx, Hext, dt, prefactor), "fg_prepare_elements");
+            fg_solve_info info;
+            const int rc = fg_solve(e.ctx, dt, &info);
+            e.check(rc, "fg_solve");
+            vmax = info.v_max;
+            total_iter += info.iterations;
+            if (rc == FG_SOLVE_FAILED) {
+                flag++;
+                bad++;
+                set_soft_limit(dt / 2);
+                continue;
+            }
+            const double dumax = dt * vmax;
+            set_soft_limit((c.du_max / vmax) * 0.95);
+            if (dumax > c.du_max) {
+                flag++;
+                bad++;
+                continue;
+            }
+            compute_all();
+            nt++;
+            flag = 0;
+            if (last_step) t = t_target;
+            else t += dt;
+            double a;
+            e.check(fg_max_angle(e.ctx, &a), "fg_max_angle");
+            max_angle = std::max(max_angle, a);
+        }
+        {  // Fem::saver (src/save.cpp:13-162)
+            double mu[3] = {0, 0, 0}, mv[3] = {0, 0, 0}, ang = 0;
+            e.check(fg_average(e.ctx, 0, mu), "fg_average");
+            for (size_t k = 0; k < c.cols.size(); k++) {
+                const std::string &col = c.cols[k];
+                if (col == "iter") fout << nt_output;
+                else if (col == "t") fout << t;
+                else if (col == "dt") fout << dt;
+                else if (col == "max_dm") fout << vmax * dt;
+                else if (col == "max_angle") {
+                    e.check(fg_max_angle(e.ctx, &ang), "fg_max_angle");
+                    fout << ang;
+                } else if (col == "<Mx>") fout << mu[0];
+                else if (col == "<My>") fout << mu[1];
+                else if (col == "<Mz>") fout << mu[2];
+                else if (col[0] == '<') {
+                    e.check(fg_average(e.ctx, 1, mv), "fg_average");
+                    fout << mv[col[3] - 'x'];
+                } else if (col == "E_ex") fout << E[FG_E_EXCHANGE];
+                else if (col == "E_aniso") fout << E[FG_E_ANISOTROPY];
+                else if (col == "E_demag") fout << E[FG_E_DEMAG];
+                else if (col == "E_zeeman") fout << E[FG_E_ZEEMAN];
+                else if (col == "E_tot") fout << E[0] + E[1] + E[2] + E[3];
+                else if (col == "Hx") fout << Hext[0];
+                else if (col == "Hy") fout << Hext[1];
+                else if (col == "Hz") fout << Hext[2];
+                fout << (k + 1 < c.cols.size() ? "\t" : "\n");
+            }
+            fout << std::flush;
+            if (c.mag_every && (nt_output % c.mag_every) == 0) {
+                const std::string name = base + "_iter" + std::to_string(nt_output) + ".sol";
+                e.check(fg_get_state(e.ctx, FG_NEXT, u.data(), nullptr, phi.data(), nullptr), "fg_get_state");
+                std::ostringstream md;
+                md << common_metadata(c) << "## time: " << std::scientific << t << "\n"
+                   << "## columns: idx\tmx\tmy\tmz\tphi\n";
+                if (fgh_write_sol(mesh, name.c_str(), md.str().c_str(), 7, u.data(), phi.data()) != FG_OK)
+                    throw std::runtime_error(std::string("writing ") + name + ": " + fgh_last_error());
+                if (verbose) std::cout << " " << name << "\n";
+            }
+            nt_output++;
+        }
+    }
+bailout:
+    fout.close();
+    {
+        const double wall = std::chrono::duration<double>(std::chrono::steady_clock::now() - wall0).count();
+        std::cout << "steps:             " << nt << " accepted, " << bad << " rejected, " << total_iter
+                  << " BiCGSTAB iterations\n";
+        std::cout << "max angle:         " << max_angle << " rad\n";
+        std::cout << "wall time:         " << wall << " s (" << (nt > 0 ? wall / nt * 1e3 : 0.0) << " ms per step)\n";
+    }
+    fg_destroy(e.ctx);
+    fgh_free(mesh);
+    return status;
+}
+
+void usage(const char *prog)
+{
+    std::cout << "Usage: " << prog << " [options] settings_file\n"
+              << "  settings_file    YAML or JSON settings; '-' reads standard input\n"
+              << "  -v, --verbose    print every integration step\n"
+              << "  --seed N         fixed random seed (default: random)\n"
+              << "  --check          read the settings and the mesh, print them, run nothing\n"
+              << "  -h, --help       this help\n"
+              << "  -V, --version    version\n";
+}
+
+}  // namespace
+
+int main(int argc, char *argv[])
+{
+    bool verbose = false, fixed_seed = false, check_only = false;
+    unsigned seed = 0;
+    std::string file;
+    for (int k = 1; k < argc; k++) {
+        const std::string a = argv[k];
+        if (a == "-v" || a == "--verbose") verbose = true;
+        else if (a == "--check") check_only = true;
+        else if (a == "--seed" && k + 1 < argc) {
+            fixed_seed = true;
+            seed = (unsigned)std::strtoul(argv[++k], nullptr, 10);
+        } else if (a == "-h" || a == "--help") {
+            usage(argv[0]);
+            return 0;
+        } else if (a == "-V" || a == "--version") {
+            std::cout << VERSION << "\n";
+            return 0;
+        } else if (a == "-" || a[0] != '-') {
+            if (!file.empty()) {
+                usage(argv[0]);
+                return 1;
+            }
+            file = a;
+        } else {
+            std::cerr << "unknown option " << a << "\n";
+            usage(argv[0]);
+            return 1;
+        }
+    }
+    if (file.empty()) {
+        usage(argv[0]);
+        return 1;
+    }
+    if (!fixed_seed) seed = std::random_device{}();
+    try {
+        std::string text;
+        if (file == "-") {
+            std::ostringstream ss;
+            ss << std::cin.rdbuf();
+            text = ss.str();
+        } else {
+            std::ifstream in(file);
+            if (!in) throw std::runtime_error("cannot open " + file);
+            std::ostringstream ss;
+            ss << in.rdbuf();
+            text = ss.str();
+        }
+        Config c = read_config(fgh::parse_settings(text));
+        c.settings_name = file == "-" ? "standard input" : file;
+        std::cout << "random seed:       " << seed << "\n";
+        return run(c, seed, verbose, check_only);
+    } catch (const std::exception &ex) {
+        std::cerr << "error: " << ex.what() << "\n";
+        return 1;
+    }
+}

@@ -1,0 +1,172 @@
+"""fgmi_run, the C++ drop-in time loop.
+
+CPU: the settings reader (YAML block style with comments, JSON as ci-tests/full_test.py feeds it on
+stdin, the reference's defaults from src/default-settings.yml) and its error reports, through
+`fgmi_run --check`, which reads settings and mesh and runs nothing.
+GPU: the reference's own end-to-end golden (ci-tests/full_test.py:21-50: examples/ellipsoid.msh,
+--seed 2, adaptive time steps) run by fgmi_run on the engine; the final <M> and E_tot must match
+the published values to the test's threshold 1e-5, and the FP64 oracle's run of the same loop much
+closer.
+"""
+import json
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from feellgood_amd import native as nv
+
+DRIVER = os.path.join(nv.LIB_DIR, "fgmi_run")
+
+
+def full_test_settings(golden_dir, outdir, demag="direct"):
+    return {
+        "outputs": {"directory": str(outdir), "file_basename": "full_test", "evol_time_step": 1e-12,
+                    "final_time": 2e-11, "mag_config_every": False},
+        "mesh": {"filename": os.path.join(golden_dir, "ellipsoid.msh"), "length_unit": 1e-9,
+                 "volume_regions": {"ellipsoid_volume": {"K": 3e5, "uk": [0, 1, 0]}}},
+        "initial_magnetization": [0, 0, 1],
+        "Bext": [1, 0, -1],
+        "demagnetizing_field_solver": {"nb_threads": 3},
+        "time_integration": {"min(dt)": 5e-14, "max(dt)": 1e-12, "max(du)": 0.1},
+        "gpu": {"demag": demag},
+    }
+
+
+def check(args, text=None):
+    r = subprocess.run([DRIVER, "--check"] + args, input=text, capture_output=True, text=True, timeout=60)
+    return r.returncode, r.stdout, r.stderr
+
+
+def parsed(out):
+    d = {}
+    for ln in out.splitlines():
+        k, _, v = ln.partition(":")
+        d.setdefault(k.strip(), []).append(v.split())
+    return d
+
+
+def test_json_settings_on_stdin(golden_dir, tmp_path):
+    rc, out, err = check(["--seed", "2", "-"], json.dumps(full_test_settings(golden_dir, tmp_path)))
+    assert rc == 0, err
+    d = parsed(out)
+    assert d["mesh"][0][1:] == ["(167", "nodes,", "499", "tetrahedra,", "274", "triangles)"]
+    vol = d["volume region"][0]
+    assert vol[0] == "ellipsoid_volume"
+    # defaults of src/default-settings.yml where the settings are silent
+    assert float(vol[2]) == 1e-11 and float(vol[4]) == 795774.7 and float(vol[6]) == 3e5
+    assert [float(x) for x in vol[8:11]] == [0, 1, 0] and float(vol[14]) == 0.5
+    assert [float(x) for x in d["Bext"][0]] == [1, 0, -1]
+    assert d["time"][0] == ["final", "2e-11", "evol", "1e-12", "dt", "5e-14", "1e-12", "du", "0.1"]
+    assert d["solver"][0] == ["tol", "1e-06", "max_iter", "700"]
+    assert d["outputs"][0][2] == "0"  # mag_config_every: false
+    assert d["outputs"][0][4:] == ["t", "<Mx>", "<My>", "<Mz>", "E_ex", "E_demag", "E_zeeman", "E_tot"]
+
+
+YAML = """# a settings file in block style
+outputs:
+  directory: {out}/   # trailing slash is dropped
+  file_basename: blocky
+  evol_time_step: 5e-13
+  evol_columns:
+    - t
+    - <Mx>
+    - "E_tot"
+  mag_config_every: 10
+  metadata:
+    Author: someone
+mesh:
+  filename: {msh}
+  length_unit: 1e-9
+  volume_regions:
+    ellipsoid_volume:
+      Ms: 8e5
+      alpha_LLG: 0.02
+      uk: [1, 1, 0]    # normalised on reading
+  surface_regions:
+    ellipsoid_surface:
+      Ks: 1e-4
+      suppress_charges: true
+initial_magnetization: [1, 0, 0]
+Bext: [0.0, 0.0, 0.01]
+finite_element_solver:
+  max(iter): 500
+  tolerance: 1e-10
+time_integration:
+  max(du): 0.02
+  min(dt): 1e-15
+  max(dt): 1e-13
+"""
+
+
+def test_yaml_block_settings(golden_dir, tmp_path):
+    p = tmp_path / "s.yml"
+    p.write_text(YAML.format(out=tmp_path, msh=os.path.join(golden_dir, "ellipsoid.msh")))
+    rc, out, err = check([str(p)])
+    assert rc == 0, err
+    d = parsed(out)
+    vol = d["volume region"][0]
+    assert float(vol[4]) == 8e5 and float(vol[14]) == 0.02
+    assert np.allclose([float(x) for x in vol[8:11]], [2 ** -0.5, 2 ** -0.5, 0])
+    surf = d["surface region"][0]
+    assert surf[0] == "ellipsoid_surface" and float(surf[2]) == 1e-4 and surf[4] == "1"
+    assert d["solver"][0] == ["tol", "1e-10", "max_iter", "500"]
+    assert d["outputs"][0][0] == f"{tmp_path}/blocky" and d["outputs"][0][2] == "10"
+    assert d["outputs"][0][4:] == ["t", "<Mx>", "E_tot"]
+
+
+@pytest.mark.parametrize("patch,msg", [
+    ({"Bext": ["0", "0", "0.1*t"]}, "Bext: only three numbers"),
+    ({"initial_magnetization": ["x", "y", "0"]}, "initial_magnetization: only"),
+    ({"outputs": {"evol_columns": ["t", "nonsense"]}}, "unsupported column 'nonsense'"),
+    ({"recentering": {"enable": True}}, "recentering is not supported"),
+    ({"time_integration": {"max(dt)": "soon"}}, "time_integration.max(dt) should be a number"),
+    ({"mesh": {"filename": "missing.msh", "volume_regions": {"v": {}}}}, "missing.msh"),
+])
+def test_settings_errors(golden_dir, tmp_path, patch, msg):
+    s = full_test_settings(golden_dir, tmp_path)
+    s.update(patch)
+    rc, out, err = check(["-"], json.dumps(s))
+    assert rc == 1 and msg in err, (out, err)
+
+
+def test_yaml_syntax_errors(tmp_path):
+    p = tmp_path / "bad.yml"
+    p.write_text("mesh:\n\tfilename: x.msh\n")
+    rc, _, err = check([str(p)])
+    assert rc == 1 and "line 2: tab in indentation" in err
+    p.write_text("mesh:\n  filename: a\n  filename: b\n")
+    rc, _, err = check([str(p)])
+    assert rc == 1 and "duplicate key 'filename'" in err
+
+
+def last_evol_row(path):
+    rows = [ln for ln in open(path) if ln.strip() and not ln.startswith("#")]
+    return [float(x) for x in rows[-1].split()], len(rows)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("demag", ["direct", "fmm"])
+def test_full_test_golden_on_gpu(golden_dir, tmp_path, demag):
+    """ci-tests/full_test.py:45-50 expected values (NPI = 5 build): m = (0.307542, 0.475877,
+    -0.823989), E_tot = -4.445605e-19, threshold 1e-5; the oracle's run of the same loop is the
+    tighter reference."""
+    s = full_test_settings(golden_dir, tmp_path, demag)
+    r = subprocess.run([DRIVER, "--seed", "2", "-"], input=json.dumps(s), capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    row, n = last_evol_row(tmp_path / "full_test.evol")
+    assert n == 21
+    m, e_tot = np.array(row[1:4]), row[7]
+    assert np.linalg.norm(m - [0.307542, 0.475877, -0.823989]) < 1e-5, m
+    assert abs((e_tot - -4.445605e-19) / -4.445605e-19) < 1e-5, e_tot
+    import oracle
+    from feellgood_amd.mesh import HostMesh
+    hm = HostMesh.read(os.path.join(golden_dir, "ellipsoid.msh"))
+    hm.prepare({"ellipsoid_volume": dict(K=3e5, uk=(0, 1, 0))})
+    st, evol = oracle.run(hm.mesh, np.tile([0, 0, 1.0], (hm.n_nodes, 1)), final_time=2e-11, evol_time_step=1e-12,
+                          dt_min=5e-14, dt_max=1e-12, du_max=0.1, Bext=(1, 0, -1), seed=2)
+    assert st == 0 and len(evol) == 21
+    tol = 1e-8 if demag == "direct" else 1e-6
+    assert np.linalg.norm(m - evol[-1][1:4]) < tol, (m, evol[-1][1:4])
+    assert abs(e_tot - evol[-1][8]) < tol * abs(evol[-1][8])
