@@ -401,6 +401,15 @@ void build_plan(Ctx &c, const fg_mesh *m)
     upload(c.tet_reg, mreg, st);
     c.regions = dalloc<DevRegion>(regs.size());
     upload(c.regions, regs, st);
+    {
+        // typical charge magnitude -Ms w div u ~ Ms w / h = Ms w^(2/3)
+        double ms = 0, wsum = 0;
+        for (auto &r : regs) ms = std::max(ms, std::fabs(r.Ms));
+        for (int t = 0; t < Tm; t++)
+            for (int g = 0; g < 5; g++) wsum += std::fabs(da_w[(size_t)17 * t + 12 + g]);
+        const double w = wsum / (5.0 * Tm);
+        c.charge_unit = (ms > 0 && w > 0) ? ms * std::cbrt(w * w) : 1.0;
+    }
     c.elem_out = dalloc<double4>((size_t)4 * Tm + (size_t)3 * Fk);
     c.inc_ptr = dalloc<int32_t>(N + 1);
     upload(c.inc_ptr, incp, st);

@@ -37,6 +37,7 @@ struct Ctx {
 
     // host copies kept for introspection
     std::vector<int32_t> h_mag;          // [N]
+    double charge_unit = 1.0;            // typical |charge| (Ms_max w^(2/3)), scales the FP32 FMM pass
     std::vector<int64_t> h_rowptr;       // stencil CSR (unpadded), for fg_get_system
     std::vector<int32_t> h_col;
     std::vector<double> h_val;

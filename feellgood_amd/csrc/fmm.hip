@@ -72,6 +72,9 @@ struct FmmPlan {
     int64_t *d_lc_ptr = nullptr;  // [ntl+1] shared column list of each such leaf
     int32_t *d_lc_col = nullptr;
     double2 *d_pot = nullptr;  // [T] near-field + correction potential
+    float *d_nf_valf = nullptr;  // FP32 coefficients (mixed precision; d_nf_val is then released)
+    bool mixed = false;
+    double mscale = 1.0;         // multipoles times mscale (a power of two) in the FP32 pass
     int64_t m2p_pairs = 0, m2l_pairs = 0, lc_cols = 0;
     int64_t nf_nnz = 0;
     int32_t *d_tgt_node = nullptr;
@@ -537,43 +540,6 @@ static T *dput(const std::vector<T> &h)
     return (T *)d;
 }
 
-// P2M: a group of G lanes per source leaf, each lane a strided subset of the leaf's sources; the
-// lanes' accumulators are combined with cross-lane sums. blockIdx.y = charge set.
-template <int P, int G>
-__global__ __launch_bounds__(64) void k_fmm_p2m(int nleaf, int cell0, double iw, const double4 *__restrict__ cen,
-                                                const int32_t *__restrict__ sb, const int32_t *__restrict__ se,
-                                                const double4 *__restrict__ spos, const double2 *__restrict__ q,
-                                                double2 *__restrict__ M)
-{
-    constexpr int NCo = fcount(P);
-    const int lane = threadIdx.x & (G - 1);
-    const int li = (blockIdx.x * 64 + threadIdx.x) / G;
-    const int set = blockIdx.y;
-    double acc[2 * NCo];
-#pragma unroll
-    for (int k = 0; k < 2 * NCo; k++) acc[k] = 0.0;
-    const bool act = li < nleaf;
-    if (act) {
-        const double4 c = cen[cell0 + li];
-        for (int k = sb[li] + lane; k < se[li]; k += G) {
-            const double4 x = spos[k];
-            const double2 qq = q[k];
-            p2m_accumulate<P>((x.x - c.x) * iw, (x.y - c.y) * iw, (x.z - c.z) * iw, set ? qq.y : qq.x, acc);
-        }
-    }
-#pragma unroll
-    for (int k = 0; k < 2 * NCo; k++) {
-        double v = acc[k];
-        for (int o = G / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-        acc[k] = v;
-    }
-    if (act && lane == 0) {
-        double2 *dst = M + ((size_t)(cell0 + li) * 2 + set) * NCo;
-#pragma unroll
-        for (int k = 0; k < NCo; k++) dst[k] = make_double2(acc[2 * k], acc[2 * k + 1]);
-    }
-}
-
 // Logical block index with the dispatcher's round-robin over the 8 XCDs undone: consecutive
 // logical blocks (neighbouring cells, which share source data) run on the same XCD and its L2.
 __device__ inline int xcd_block(int b, int nb)
@@ -582,47 +548,175 @@ __device__ inline int xcd_block(int b, int nb)
     return x * q + min(x, r) + (b >> 3);
 }
 
-// M2M: one wavefront per parent cell with sources; lane (n, m) accumulates both charge sets over
-// the children. The children's expansions are read at wave-uniform addresses; the child-offset
-// table (both signs of m) is indexed per lane.
+// P2M: P+1 lanes per source leaf, lane m owning the column m of both expansions (coefficients
+// (m..P, m)); 64 / (P+1) leaves per wavefront. The lanes of a leaf walk its sources together
+// (one broadcast load per source); lane m forms R_m^m = (-1/2)^m / m! (x + iy)^m and runs the
+// column recurrence with its own divisors, computed once.
 template <int P>
-__global__ __launch_bounds__(128) void k_fmm_m2m(const int32_t *__restrict__ cells, const int32_t *__restrict__ ch_ptr,
-                                                const int32_t *__restrict__ ch_idx, const int32_t *__restrict__ oct,
-                                                const double2 *__restrict__ Rcf, double2 *__restrict__ M)
+__global__ __launch_bounds__(64) void k_fmm_p2m(int nleaf, int cell0, double iw, const double4 *__restrict__ cen,
+                                                const int32_t *__restrict__ sb, const int32_t *__restrict__ se,
+                                                const double4 *__restrict__ spos, const double2 *__restrict__ q,
+                                                double2 *__restrict__ M)
+{
+    constexpr int NCo = fcount(P), LPW = 64 / (P + 1);
+    const int lane = threadIdx.x;
+    const int g = lane / (P + 1), m = lane - g * (P + 1);
+    const int li = blockIdx.x * LPW + g;
+    const bool act = g < LPW && li < nleaf;
+    // R_{m+k}^m = ca[k] z R_{m+k-1}^m - cb[k] r^2 R_{m+k-2}^m, k >= 2
+    double ca[P + 1], cb[P + 1];
+#pragma unroll
+    for (int k = 0; k <= P; k++) {
+        const int n = m + k;
+        const double inv = (k >= 2 && n <= P) ? 1.0 / ((double)(n + m) * (double)(n - m)) : 0.0;
+        ca[k] = (2.0 * n - 1.0) * inv;
+        cb[k] = inv;
+    }
+    double cm = 1.0;
+#pragma unroll 1
+    for (int k = 1; k <= m; k++) cm *= -0.5 / (double)k;
+    double ur[P + 1], ui[P + 1], vr[P + 1], vi[P + 1];
+#pragma unroll
+    for (int k = 0; k <= P; k++) ur[k] = ui[k] = vr[k] = vi[k] = 0.0;
+    if (act) {
+        const double4 c = cen[cell0 + li];
+        const int s0 = sb[li], s1 = se[li];
+        for (int s = s0; s < s1; s += 2) {
+            // two sources per pass: independent recurrences interleaved (a missing second source
+            // carries zero charge)
+            double x[2], y[2], z[2], r2[2], qu[2], qv[2];
+#pragma unroll
+            for (int h = 0; h < 2; h++) {
+                const int k = min(s + h, s1 - 1);
+                const double4 p = spos[k];
+                const double2 qq = q[k];
+                const bool ok = s + h < s1;
+                x[h] = (p.x - c.x) * iw;
+                y[h] = (p.y - c.y) * iw;
+                z[h] = (p.z - c.z) * iw;
+                r2[h] = x[h] * x[h] + y[h] * y[h] + z[h] * z[h];
+                qu[h] = ok ? qq.x : 0.0;
+                qv[h] = ok ? qq.y : 0.0;
+            }
+            double pr[2] = {1.0, 1.0}, pi[2] = {0.0, 0.0};
+#pragma unroll
+            for (int k = 1; k <= P; k++)
+                if (k <= m)
+#pragma unroll
+                    for (int h = 0; h < 2; h++) {
+                        const double t = pr[h] * x[h] - pi[h] * y[h];
+                        pi[h] = pr[h] * y[h] + pi[h] * x[h];
+                        pr[h] = t;
+                    }
+            double r0r[2], r0i[2], r1r[2], r1i[2];
+#pragma unroll
+            for (int h = 0; h < 2; h++) {
+                r0r[h] = cm * pr[h];  // R_m^m
+                r0i[h] = cm * pi[h];
+                r1r[h] = z[h] * r0r[h];  // R_{m+1}^m
+                r1i[h] = z[h] * r0i[h];
+                ur[0] += qu[h] * r0r[h];
+                ui[0] += qu[h] * r0i[h];
+                vr[0] += qv[h] * r0r[h];
+                vi[0] += qv[h] * r0i[h];
+                ur[1] += qu[h] * r1r[h];
+                ui[1] += qu[h] * r1i[h];
+                vr[1] += qv[h] * r1r[h];
+                vi[1] += qv[h] * r1i[h];
+            }
+#pragma unroll
+            for (int k = 2; k <= P; k++)
+#pragma unroll
+                for (int h = 0; h < 2; h++) {
+                    const double a = ca[k] * z[h], b = cb[k] * r2[h];
+                    const double cr = a * r1r[h] - b * r0r[h], ci = a * r1i[h] - b * r0i[h];
+                    ur[k] += qu[h] * cr;
+                    ui[k] += qu[h] * ci;
+                    vr[k] += qv[h] * cr;
+                    vi[k] += qv[h] * ci;
+                    r0r[h] = r1r[h];
+                    r0i[h] = r1i[h];
+                    r1r[h] = cr;
+                    r1i[h] = ci;
+                }
+        }
+        double2 *Mu = M + (size_t)(cell0 + li) * 2 * NCo, *Mv = Mu + NCo;
+#pragma unroll
+        for (int k = 0; k <= P; k++)
+            if (m + k <= P) {
+                Mu[fidx(m + k, m)] = make_double2(ur[k], ui[k]);
+                Mv[fidx(m + k, m)] = make_double2(vr[k], vi[k]);
+            }
+    }
+}
+
+// M2M: one block of 4 wavefronts per parent cell with sources. The children's expansions are
+// staged in LDS; the block splits the children into NG groups of NW wavefronts (NW = 1 up to order
+// 8), thread (n, m) of a group accumulates both charge sets over its children (read back as LDS
+// broadcasts; the child-offset table, both signs of m, is indexed per thread from global memory)
+// and the groups' sums are combined through LDS.
+template <int P>
+__global__ __launch_bounds__(256) void k_fmm_m2m(const int32_t *__restrict__ cells, const int32_t *__restrict__ ch_ptr,
+                                                 const int32_t *__restrict__ ch_idx, const int32_t *__restrict__ oct,
+                                                 const double2 *__restrict__ Rcf, double2 *__restrict__ M)
 {
     constexpr int NCo = fcount(P), NF = (P + 1) * (P + 1);
-    const int ci = cells[blockIdx.x];
-    const int q = threadIdx.x;  // blockDim = 64 * ceil(NCo / 64)
+    constexpr int NW = (NCo + 63) / 64, NG = 4 / NW;
+    __shared__ double2 sM[8][2 * NCo];
+    __shared__ int sOct[8];
+    __shared__ double4 red[NG][64 * NW];
+    const int ci = cells[xcd_block(blockIdx.x, gridDim.x)];
+    const int c0 = ch_ptr[ci], nch = ch_ptr[ci + 1] - c0;
+    for (int i = threadIdx.x; i < nch * 2 * NCo; i += blockDim.x) {
+        const int e = i / (2 * NCo), r = i - e * 2 * NCo;
+        sM[e][r] = M[(size_t)ch_idx[c0 + e] * 2 * NCo + r];
+    }
+    if ((int)threadIdx.x < nch) sOct[threadIdx.x] = oct[ch_idx[c0 + threadIdx.x]];
+    __syncthreads();
+    const int grp = __builtin_amdgcn_readfirstlane(threadIdx.x / (64 * NW));
+    const int q = threadIdx.x % (64 * NW);
     int n = 0;
     while (n < P && fidx(n + 1, 0) <= q) n++;
     const int m = q - fidx(n, 0);
     double ur = 0, ui = 0, vr = 0, vi = 0;
-    for (int e = ch_ptr[ci]; e < ch_ptr[ci + 1]; e++) {
-        const int ch = ch_idx[e];
-        const double2 *Mu = M + (size_t)ch * 2 * NCo, *Mv = Mu + NCo;
-        const double2 *R = Rcf + (size_t)oct[ch] * NF;
-        double sc = 1.0;
+    if (q < NCo)
+        for (int e = grp; e < nch; e += NG) {
+            const double2 *Mu = sM[e], *Mv = sM[e] + NCo;
+            const double2 *R = Rcf + (size_t)sOct[e] * NF;
+            double sc = 1.0;
 #pragma unroll
-        for (int k = 0; k <= P; k++, sc *= 0.5) {
+            for (int k = 0; k <= P; k++, sc *= 0.5) {
 #pragma unroll
-            for (int l = -k; l <= k; l++) {
-                const int ml = m - l, nk = n - k;
-                if (q >= NCo || nk < 0 || ml > nk || -ml > nk) continue;
-                const double2 r = R[fidx_full(nk, ml)];
-                const double2 a = Mu[fidx(k, l < 0 ? -l : l)], b = Mv[fidx(k, l < 0 ? -l : l)];
-                // X_k^{-l} = (-1)^l conj(X_k^l)
-                const double s = (l < 0 && (l & 1)) ? -sc : sc;
-                const double ai = l < 0 ? -a.y : a.y, bi = l < 0 ? -b.y : b.y;
-                ur += s * (r.x * a.x - r.y * ai);
-                ui += s * (r.x * ai + r.y * a.x);
-                vr += s * (r.x * b.x - r.y * bi);
-                vi += s * (r.x * bi + r.y * b.x);
+                for (int l = -k; l <= k; l++) {
+                    const int ml = m - l, nk = n - k;
+                    if (nk < 0 || ml > nk || -ml > nk) continue;
+                    const double2 r = R[fidx_full(nk, ml)];
+                    const double2 a = Mu[fidx(k, l < 0 ? -l : l)], b = Mv[fidx(k, l < 0 ? -l : l)];
+                    // X_k^{-l} = (-1)^l conj(X_k^l)
+                    const double s = (l < 0 && (l & 1)) ? -sc : sc;
+                    const double ai = l < 0 ? -a.y : a.y, bi = l < 0 ? -b.y : b.y;
+                    ur += s * (r.x * a.x - r.y * ai);
+                    ui += s * (r.x * ai + r.y * a.x);
+                    vr += s * (r.x * b.x - r.y * bi);
+                    vi += s * (r.x * bi + r.y * b.x);
+                }
             }
         }
-    }
-    if (q < NCo) {
-        M[(size_t)ci * 2 * NCo + q] = make_double2(ur, ui);
-        M[((size_t)ci * 2 + 1) * NCo + q] = make_double2(vr, vi);
+    red[grp][q] = make_double4(ur, ui, vr, vi);
+    __syncthreads();
+    if ((int)threadIdx.x < NCo) {
+        const int qq = threadIdx.x;
+        double4 a = red[0][qq];
+#pragma unroll
+        for (int w = 1; w < NG; w++) {
+            const double4 b = red[w][qq];
+            a.x += b.x;
+            a.y += b.y;
+            a.z += b.z;
+            a.w += b.w;
+        }
+        M[(size_t)ci * 2 * NCo + qq] = make_double2(a.x, a.y);
+        M[((size_t)ci * 2 + 1) * NCo + qq] = make_double2(a.z, a.w);
     }
 }
 
@@ -711,6 +805,12 @@ __device__ inline double wsum64(double v)
     return v;
 }
 
+__global__ void k_to_float(int64_t n, const double *__restrict__ a, float *__restrict__ b)
+{
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        b[i] = (float)a[i];
+}
+
 // Near-field operator, per-source weights in tree order: a tetrahedron's Gauss point carries w_g
 // (its charge is w_g times the element value -Ms div u); a triangle's Gauss point carries
 // dMs w_g lambda_a(g) for its vertices a = 0..2 (the charge is their sum against u_a . n).
@@ -768,11 +868,13 @@ __global__ __launch_bounds__(256) void k_nf_build(int T, const int32_t *__restri
 
 // Near field + vertex corrections: one wavefront per leaf with targets. The leaf's targets share
 // one column list (the runs of the adjacent leaves); each element value is gathered once and
-// applied to up to four targets' coefficient rows at a time.
+// applied to up to four targets' coefficient rows at a time. Coefficients are FP64, or FP32 in the
+// mixed-precision mode (products and sums stay FP64).
+template <class TC>
 __global__ __launch_bounds__(256) void k_nf_apply(int ntl, const int32_t *__restrict__ tl_tb,
                                                   const int32_t *__restrict__ tl_te, const int64_t *__restrict__ lc_ptr,
                                                   const int32_t *__restrict__ lc_col, const int64_t *__restrict__ nf_ptr,
-                                                  const double *__restrict__ nf_val, const double2 *__restrict__ qe,
+                                                  const TC *__restrict__ nf_val, const double2 *__restrict__ qe,
                                                   const int32_t *__restrict__ tgt_node, const double2 *__restrict__ corr,
                                                   double2 *__restrict__ pot)
 {
@@ -793,7 +895,7 @@ __global__ __launch_bounds__(256) void k_nf_apply(int ntl, const int32_t *__rest
 #pragma unroll
             for (int i = 0; i < 4; i++)
                 if (i < nt) {
-                    const double v = nf_val[vb[i] + j];
+                    const double v = (double)nf_val[vb[i] + j];
                     au[i] += v * q.x;
                     av[i] += v * q.y;
                 }
@@ -811,11 +913,11 @@ __global__ __launch_bounds__(256) void k_nf_apply(int ntl, const int32_t *__rest
 }
 
 // Far field: one block per evaluation group. The group's source list is walked in chunks of CH
-// cells whose multipoles are staged in LDS; threads are (target slot, chunk slice) pairs. The
-// slices are summed through LDS, then the L2P of the group's level-(L-2) ancestor is added and
-// phi, phi_v are written.
-template <int P>
-__global__ __launch_bounds__(256) void k_fmm_m2p(const int32_t *__restrict__ grp_tb, const int32_t *__restrict__ grp_te,
+// cells whose multipoles are staged in LDS; a thread evaluates a chunk slice for two targets at
+// once (one load of each coefficient serves both). The slices are summed through LDS, then the
+// L2P of the group's level-(L-2) ancestor is added and phi, phi_v are written.
+template <int P, class R>
+__global__ __launch_bounds__(128) void k_fmm_m2p(const int32_t *__restrict__ grp_tb, const int32_t *__restrict__ grp_te,
                                                  const int32_t *__restrict__ grp_lcell,
                                                  const int32_t *__restrict__ u_ptr, const int32_t *__restrict__ u_cell,
                                                  const uint8_t *__restrict__ u_mask,
@@ -823,33 +925,40 @@ __global__ __launch_bounds__(256) void k_fmm_m2p(const int32_t *__restrict__ grp
                                                  const double4 *__restrict__ cen, const double *__restrict__ Wlev,
                                                  const double2 *__restrict__ M, const double2 *__restrict__ Lx,
                                                  const double2 *__restrict__ pot, const int32_t *__restrict__ tgt_node,
-                                                 NodeRec *__restrict__ nxt)
+                                                 NodeRec *__restrict__ nxt, double mscale)
 {
-    constexpr int NCo = fcount(P), CW = 2 * NCo, CH = 16;
-    __shared__ double2 sM[CH * CW];
+    using R2 = typename vec2_of<R>::type;
+    constexpr int NCo = fcount(P), CW = 2 * NCo, CH = 16, NT = 128, TMAX = 64;
+    __shared__ R2 sM[CH * CW];
     __shared__ double4 sC[CH];
     __shared__ int sMask[CH];
-    __shared__ double2 red[256];
+    __shared__ double4 red[NT];
     const int g = xcd_block(blockIdx.x, gridDim.x);
     const int tid = threadIdx.x;
     const int tb = grp_tb[g], te = grp_te[g], ub = u_ptr[g], ue = u_ptr[g + 1], lc = grp_lcell[g];
-    for (int t0 = tb; t0 < te; t0 += 64) {
-        const int n = min(64, te - t0);
-        const int nsl = min(256 / n, CH);
-        const int slot = tid % n, slice = tid / n;
+    for (int t0 = tb; t0 < te; t0 += TMAX) {
+        const int n = min(TMAX, te - t0);
+        const int np = (n + 1) >> 1;  // target pairs
+        const int nsl = min(NT / np, CH);
+        const int slot = tid % np, slice = tid / np;
         const bool act = slice < nsl;
-        double4 y = make_double4(0, 0, 0, 0);
-        int oc = 0;
+        const int ta = t0 + 2 * slot, tb2 = min(ta + 1, t0 + n - 1);
+        double4 ya = make_double4(0, 0, 0, 0), yb = ya;
+        int oa = 0, ob = 0;
         if (act) {
-            y = tpos[t0 + slot];
-            oc = tgt_oct[t0 + slot];
+            ya = tpos[ta];
+            yb = tpos[tb2];
+            oa = tgt_oct[ta];
+            ob = tb2 == ta ? 8 : tgt_oct[tb2];  // bit 8 of the mask is never set
         }
-        double pu = 0, pv = 0;
+        double ua = 0, va = 0, ub2 = 0, vb = 0;
         for (int cb = ub; cb < ue; cb += CH) {
             const int nch = min(CH, ue - cb);
-            for (int i = tid; i < nch * CW; i += 256) {
+            for (int i = tid; i < nch * CW; i += NT) {
                 const int jj = i / CW;
-                sM[i] = M[(size_t)u_cell[cb + jj] * CW + (i - jj * CW)];
+                const double2 v = M[(size_t)u_cell[cb + jj] * CW + (i - jj * CW)];
+                sM[i].x = (R)(v.x * mscale);
+                sM[i].y = (R)(v.y * mscale);
             }
             if (tid < nch) {
                 const double4 c = cen[u_cell[cb + tid]];
@@ -859,25 +968,37 @@ __global__ __launch_bounds__(256) void k_fmm_m2p(const int32_t *__restrict__ grp
             __syncthreads();
             if (act)
                 for (int jj = slice; jj < nch; jj += nsl) {
-                    if (!((sMask[jj] >> oc) & 1)) continue;
+                    const int mk = sMask[jj];
+                    const bool ha = (mk >> oa) & 1, hb = (mk >> ob) & 1;
+                    if (!ha && !hb) continue;
                     const double4 c = sC[jj];
-                    double su = 0, sv = 0;
-                    m2p_eval2_t<P>(sM + jj * CW, sM + jj * CW + NCo, (y.x - c.x) * c.w, (y.y - c.y) * c.w,
-                                   (y.z - c.z) * c.w, su, sv);
-                    pu += su * c.w;
-                    pv += sv * c.w;
+                    R s0 = 0, s1 = 0, s2 = 0, s3 = 0;
+                    m2p_eval2x2_t<P, R>(sM + jj * CW, sM + jj * CW + NCo, (R)((ya.x - c.x) * c.w), (R)((ya.y - c.y) * c.w),
+                                        (R)((ya.z - c.z) * c.w), (R)((yb.x - c.x) * c.w), (R)((yb.y - c.y) * c.w),
+                                        (R)((yb.z - c.z) * c.w), s0, s1, s2, s3);
+                    if (ha) {
+                        ua += (double)s0 * c.w;
+                        va += (double)s1 * c.w;
+                    }
+                    if (hb) {
+                        ub2 += (double)s2 * c.w;
+                        vb += (double)s3 * c.w;
+                    }
                 }
             __syncthreads();
         }
-        red[tid] = make_double2(pu, pv);
+        red[tid] = make_double4(ua, va, ub2, vb);
         __syncthreads();
         if (tid < n) {
+            const int p = tid >> 1, h = tid & 1;
             double su = 0, sv = 0;
             for (int s = 0; s < nsl; s++) {
-                const double2 r = red[s * n + tid];
-                su += r.x;
-                sv += r.y;
+                const double4 r = red[s * np + p];
+                su += h ? r.z : r.x;
+                sv += h ? r.w : r.y;
             }
+            su /= mscale;
+            sv /= mscale;
             const int t = t0 + tid;
             if (lc >= 0) {
                 const double4 yy = tpos[t], c = cen[lc];
@@ -972,6 +1093,10 @@ int fmm_create(Ctx &c)
         P->d_L = (double2 *)d;
         std::vector<double> W(P->W.begin(), P->W.end());
         c.fmm_W = dput(W);
+        P->mixed = c.opt.precision == FG_PREC_MIXED;
+        // FP32 pass: charges of order charge_unit times the unit-width regular harmonics; scale
+        // them to order one by a power of two (exact in both directions)
+        P->mscale = P->mixed ? std::ldexp(1.0, -(int)std::lround(std::log2(c.charge_unit))) : 1.0;
         fmm_build_near(c, *P);
         // interaction counts (benchmark work models)
         for (size_t g = 0; g < P->grp_tb.size(); g++)
@@ -1072,10 +1197,19 @@ void fmm_build_near(Ctx &c, FmmPlan &P)
         k_nf_build<<<(unsigned)(((int64_t)T * 64 + 255) / 256), 256, 0, c.stream>>>(
             T, d_tgt_leaf, P.d_tgt_pos, d_near_ptr, d_near_leaf, d_tp_ptr, d_tp_col, d_tp_src, d_tp_nk, c.src_pos,
             wsrc, P.d_nf_ptr, P.d_nf_val);
+    if (P.mixed && P.nf_nnz > 0) {
+        if (hipMalloc(&d, P.nf_nnz * sizeof(float)) != hipSuccess) throw std::string("hipMalloc near-field values");
+        P.d_nf_valf = (float *)d;
+        k_to_float<<<4096, 256, 0, c.stream>>>(P.nf_nnz, P.d_nf_val, P.d_nf_valf);
+    }
     const hipError_t err = hipStreamSynchronize(c.stream);
     void *tmp[] = {d_tp_ptr, d_tp_col, d_tp_src, d_tp_nk, d_near_ptr, d_near_leaf, d_tgt_leaf, wsrc};
     for (void *p : tmp) (void)hipFree(p);
     if (err != hipSuccess) throw std::string("near-field build: ") + hipGetErrorString(err);
+    if (P.d_nf_valf) {
+        (void)hipFree(P.d_nf_val);
+        P.d_nf_val = nullptr;
+    }
 }
 
 void fmm_destroy(Ctx &c)
@@ -1086,7 +1220,7 @@ void fmm_destroy(Ctx &c)
                     P->d_il_off, P->d_leaf_sb, P->d_leaf_se, P->d_grp_tb, P->d_grp_te, P->d_grp_lcell, P->d_u_ptr,
                     P->d_u_cell, P->d_u_mask, P->d_tgt_oct, P->d_tgt_node, P->d_tgt_pos, P->d_Rc, P->d_Itf,
                     P->d_Rcf, P->d_M, P->d_L, P->d_src_cells, P->d_tgt_cells, P->d_nf_ptr, P->d_nf_val,
-                    P->d_tl_tb, P->d_tl_te, P->d_lc_ptr, P->d_lc_col, P->d_pot, c.qe, c.fmm_W};
+                    P->d_nf_valf, P->d_tl_tb, P->d_tl_te, P->d_lc_ptr, P->d_lc_col, P->d_pot, c.qe, c.fmm_W};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     c.fmm_W = nullptr;
@@ -1103,24 +1237,26 @@ static void run_fmm(Ctx &c)
     const int NL = P.lev_off[L + 1] - P.lev_off[L];
     {
         Prof pr(c, "fmm_near");
-        if (P.ntl > 0)
-            k_nf_apply<<<(unsigned)(((int64_t)P.ntl * 64 + 255) / 256), 256, 0, c.stream>>>(
-                P.ntl, P.d_tl_tb, P.d_tl_te, P.d_lc_ptr, P.d_lc_col, P.d_nf_ptr, P.d_nf_val, c.qe, P.d_tgt_node,
-                c.corr, P.d_pot);
+        const unsigned nb = (unsigned)(((int64_t)P.ntl * 64 + 255) / 256);
+        if (P.ntl > 0 && P.d_nf_valf)
+            k_nf_apply<float><<<nb, 256, 0, c.stream>>>(P.ntl, P.d_tl_tb, P.d_tl_te, P.d_lc_ptr, P.d_lc_col, P.d_nf_ptr,
+                                                        P.d_nf_valf, c.qe, P.d_tgt_node, c.corr, P.d_pot);
+        else if (P.ntl > 0)
+            k_nf_apply<double><<<nb, 256, 0, c.stream>>>(P.ntl, P.d_tl_tb, P.d_tl_te, P.d_lc_ptr, P.d_lc_col,
+                                                         P.d_nf_ptr, P.d_nf_val, c.qe, P.d_tgt_node, c.corr, P.d_pot);
     }
     {
         Prof pr(c, "fmm_p2m");
-        constexpr int G = 8;
-        dim3 grid((NL * G + 63) / 64, 2);
-        k_fmm_p2m<PP, G><<<grid, 64, 0, c.stream>>>(NL, P.lev_off[L], 1.0 / P.W[L], P.d_cen, P.d_leaf_sb, P.d_leaf_se,
-                                                    c.src_pos, c.src_q, P.d_M);
+        constexpr int LPW = 64 / (PP + 1);
+        k_fmm_p2m<PP><<<(NL + LPW - 1) / LPW, 64, 0, c.stream>>>(NL, P.lev_off[L], 1.0 / P.W[L], P.d_cen, P.d_leaf_sb,
+                                                                 P.d_leaf_se, c.src_pos, c.src_q, P.d_M);
     }
     {
         Prof pr(c, "fmm_m2m");
         for (int l = L - 1; l >= 2; l--) {
             const int n = P.src_cells_lev_off[l + 1] - P.src_cells_lev_off[l];
             if (n == 0) continue;
-            k_fmm_m2m<PP><<<n, 64 * ((fcount(PP) + 63) / 64), 0, c.stream>>>(P.d_src_cells + P.src_cells_lev_off[l], P.d_ch_ptr, P.d_ch_idx,
+            k_fmm_m2m<PP><<<n, 256, 0, c.stream>>>(P.d_src_cells + P.src_cells_lev_off[l], P.d_ch_ptr, P.d_ch_idx,
                                                   P.d_oct, P.d_Rcf, P.d_M);
         }
     }
@@ -1145,9 +1281,16 @@ static void run_fmm(Ctx &c)
         Prof pr(c, "fmm_m2p");
         const int ng = (int)P.grp_tb.size();
         if (ng > 0)
-            k_fmm_m2p<PP><<<ng, 256, 0, c.stream>>>(P.d_grp_tb, P.d_grp_te, P.d_grp_lcell, P.d_u_ptr, P.d_u_cell,
-                                                    P.d_u_mask, P.d_tgt_oct, P.d_tgt_pos, P.d_cen, c.fmm_W, P.d_M,
-                                                    P.d_L, P.d_pot, P.d_tgt_node, c.rec[1 - c.cur]);
+            if (P.mixed)
+                k_fmm_m2p<PP, float><<<ng, 128, 0, c.stream>>>(P.d_grp_tb, P.d_grp_te, P.d_grp_lcell, P.d_u_ptr, P.d_u_cell,
+                                                               P.d_u_mask, P.d_tgt_oct, P.d_tgt_pos, P.d_cen, c.fmm_W,
+                                                               P.d_M, P.d_L, P.d_pot, P.d_tgt_node, c.rec[1 - c.cur],
+                                                               P.mscale);
+            else
+                k_fmm_m2p<PP, double><<<ng, 128, 0, c.stream>>>(P.d_grp_tb, P.d_grp_te, P.d_grp_lcell, P.d_u_ptr,
+                                                                P.d_u_cell, P.d_u_mask, P.d_tgt_oct, P.d_tgt_pos, P.d_cen,
+                                                                c.fmm_W, P.d_M, P.d_L, P.d_pot, P.d_tgt_node,
+                                                                c.rec[1 - c.cur], 1.0);
     }
 }
 

@@ -66,7 +66,8 @@ extern "C" {
 
 /* ---- precision modes ---- */
 #define FG_PREC_FP64 0        /* plain FP64 everywhere */
-#define FG_PREC_MIXED 1       /* FP32 inner Krylov + FP64 refinement, FP32 near-field P2P */
+#define FG_PREC_MIXED 1       /* FMM demag: FP32 near-field coefficients and FP32 multipole-to-target
+                                 evaluation (FP64 sums); everything else FP64 */
 
 /* Material of a volume region (Tetra::prm, src/tetra.h:226-252). Ms <= 0: non-magnetic. */
 typedef struct fg_volume_region {

@@ -130,3 +130,38 @@ def test_step_with_fmm_against_exact_oracle(order):
         o.solve(dt)
     dm = np.abs(g.get_state(nv.FG_NEXT)[0] - p.get_state(nv.FG_NEXT)[0]).max()
     assert dm <= 1e-6, dm
+
+
+@pytest.mark.gpu
+def test_mixed_precision_fmm_step():
+    """FP32 near-field coefficients and FP32 multipole-to-target evaluation: potentials within 1e-5
+    relative of the FP64 FMM, and one LLG step within |dm| <= 1e-6 of the exact FP64 oracle."""
+    from feellgood_amd.engine import Engine
+    m = film(24, 12, 3)
+    p, u, v = charged(m)
+    _, _, phi0, phiv0 = p.get_state(nv.FG_NEXT)
+    pots = {}
+    for prec in ("fp64", "mixed"):
+        g = Engine(m.mesh, demag="fmm", fmm_order=8, fmm_leaf=64, precision=prec)
+        g.set_state(nv.FG_BOTH, u=u, v=v)
+        g.calc_demag()
+        pots[prec] = g.get_state(nv.FG_NEXT)[2:]
+        g.close()
+    for k in range(2):
+        ref = pots["fp64"][k]
+        assert np.abs(pots["mixed"][k] - ref).max() <= 1e-5 * np.abs(ref).max()
+    v0 = np.zeros_like(u)
+    H = np.array([-0.025, 0.0044, 0.0]) / nv.MU0
+    dt = 5e-14
+    pref = 1 + 0.01 * abs(np.log(0.01))
+    o = oracle.Problem(m.mesh, tol=1e-10, max_iter=1000)
+    g = Engine(m.mesh, tol=1e-10, max_iter=1000, demag="fmm", fmm_order=8, fmm_leaf=64, precision="mixed")
+    for e in (o, g):
+        e.set_state(nv.FG_BOTH, u=u, v=v0)
+        e.calc_demag()
+        e.evolution()
+        e.base_projection(0.3)
+        e.prepare_elements(H, dt, pref)
+        e.solve(dt)
+    dm = np.abs(g.get_state(nv.FG_NEXT)[0] - o.get_state(nv.FG_NEXT)[0]).max()
+    assert dm <= 1e-6, dm
