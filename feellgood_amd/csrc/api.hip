@@ -585,6 +585,7 @@ int fg_destroy(fg_ctx *x)
     (void)hipSetDevice(x->c.device);
     (void)hipStreamSynchronize(x->c.stream);
     fmm_destroy(x->c);
+    guess_destroy(x->c);
     free_all(x->c);
     (void)hipStreamDestroy(x->c.stream);
     delete x;
@@ -676,6 +677,7 @@ int fg_solve(fg_ctx *x, double dt, fg_solve_info *info)
         if (dt != x->dt) return FG_ERR_ARG;
         const double cst = stencil_coef(x);
         launch_node_assemble(c, x->r, cst, dt);
+        guess_on_assembly(c);
         x->basis_valid = true;
         launch_r0(c, cst);
         // BiCGSTAB: batches of iterations, the device control block says when to stop
@@ -702,6 +704,7 @@ int fg_solve(fg_ctx *x, double dt, fg_solve_info *info)
         }
         if (failed) return FG_SOLVE_FAILED;
         launch_update(c, dt);
+        guess_on_solved(c);
         HIPCHK(hipMemcpyAsync(c.h_ctl, c.ctl, sizeof(KrylovCtl), hipMemcpyDeviceToHost, c.stream));
         HIPCHK(hipStreamSynchronize(c.stream));
         if (info) info->v_max = GAMMA0 * std::sqrt(c.h_ctl->vmax2);
@@ -808,6 +811,68 @@ int fg_system_size(fg_ctx *x, int64_t *rows, int64_t *nnz)
     *rows = 2 * (int64_t)x->c.N;
     *nnz = 4 * x->c.nnz_real;
     return FG_OK;
+}
+
+int fg_guess_init(fg_ctx *x, uint64_t seed)
+{
+    return guarded(x, [&]() {
+        guess_init(x->c, seed);
+        return FG_OK;
+    });
+}
+
+int fg_guess_set(fg_ctx *x, const float *params, int64_t n)
+{
+    if (!params || n != FG_GUESS_NPARAM) return FG_ERR_ARG;
+    return guarded(x, [&]() {
+        if (!x->c.guess) guess_init(x->c, 0);
+        guess_set(x->c, params);
+        return FG_OK;
+    });
+}
+
+int fg_guess_get(fg_ctx *x, float *params, int64_t n)
+{
+    if (!params || n != FG_GUESS_NPARAM) return FG_ERR_ARG;
+    return guarded(x, [&]() {
+        if (!x->c.guess) return FG_ERR_STATE;
+        guess_get(x->c, params);
+        return FG_OK;
+    });
+}
+
+int fg_guess_enable(fg_ctx *x, int on)
+{
+    if (!x) return FG_ERR_ARG;
+    return guess_enable(x->c, on) ? FG_OK : FG_ERR_STATE;
+}
+
+int fg_guess_train(fg_ctx *x, float lr, double *loss, double *loss_ref)
+{
+    return guarded(x, [&]() {
+        if (!x->c.guess) return FG_ERR_STATE;
+        guess_train(x->c, lr, loss, loss_ref);
+        return FG_OK;
+    });
+}
+
+int fg_guess_gradient(fg_ctx *x, float *grad, double *loss, double *loss_ref)
+{
+    return guarded(x, [&]() {
+        if (!x->c.guess) return FG_ERR_STATE;
+        guess_gradient(x->c, grad, loss, loss_ref);
+        return FG_OK;
+    });
+}
+
+int fg_guess_features(fg_ctx *x, float *feat)
+{
+    if (!feat) return FG_ERR_ARG;
+    return guarded(x, [&]() {
+        if (!x->c.guess) return FG_ERR_STATE;
+        guess_batch(x->c, feat);
+        return FG_OK;
+    });
 }
 
 int fg_fmm_info(fg_ctx *x, int64_t info[16])

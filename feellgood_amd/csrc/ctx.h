@@ -11,6 +11,8 @@ struct FmmPlan;  // fmm.hip
 
 namespace fg {
 
+struct GuessNet;  // guess.hip
+
 struct KernelStat {
     std::string name;
     int64_t launches = 0;
@@ -94,6 +96,7 @@ struct Ctx {
     double2 *qe = nullptr;       // [Tm + 3 Fm] element charge values for the FMM near-field operator:
                                  // -Ms div u per tetrahedron, u_a . n per triangle vertex (u, v)
     FmmPlan *fmm = nullptr;
+    GuessNet *guess = nullptr;   // learned initial guess (allocated on first use)
     double *fmm_W = nullptr;     // cell width per tree level
 
     // energies / reductions
@@ -149,6 +152,18 @@ void launch_energy(Ctx &c, const double Hext[3]);
 void launch_max_angle(Ctx &c);
 void launch_average(Ctx &c, int which);
 void launch_evolution(Ctx &c);
+
+// ---- learned initial guess (guess.hip) ----
+void guess_init(Ctx &c, uint64_t seed);
+void guess_set(Ctx &c, const float *params);
+void guess_get(Ctx &c, float *params);
+void guess_destroy(Ctx &c);
+void guess_on_assembly(Ctx &c);
+void guess_on_solved(Ctx &c);
+void guess_train(Ctx &c, float lr, double *loss, double *loss_ref);
+void guess_gradient(Ctx &c, float *grad, double *loss, double *loss_ref);
+void guess_batch(Ctx &c, float *feat);
+bool guess_enable(Ctx &c, int on);
 
 // ---- FMM (fmm.hip) ----
 int fmm_create(Ctx &c);

@@ -1280,13 +1280,12 @@ static void run_fmm(Ctx &c)
     {
         Prof pr(c, "fmm_m2p");
         const int ng = (int)P.grp_tb.size();
-        if (ng > 0)
-            if (P.mixed)
+        if (ng > 0 && P.mixed)
                 k_fmm_m2p<PP, float><<<ng, 128, 0, c.stream>>>(P.d_grp_tb, P.d_grp_te, P.d_grp_lcell, P.d_u_ptr, P.d_u_cell,
                                                                P.d_u_mask, P.d_tgt_oct, P.d_tgt_pos, P.d_cen, c.fmm_W,
                                                                P.d_M, P.d_L, P.d_pot, P.d_tgt_node, c.rec[1 - c.cur],
                                                                P.mscale);
-            else
+        else if (ng > 0)
                 k_fmm_m2p<PP, double><<<ng, 128, 0, c.stream>>>(P.d_grp_tb, P.d_grp_te, P.d_grp_lcell, P.d_u_ptr,
                                                                 P.d_u_cell, P.d_u_mask, P.d_tgt_oct, P.d_tgt_pos, P.d_cen,
                                                                 c.fmm_W, P.d_M, P.d_L, P.d_pot, P.d_tgt_node,

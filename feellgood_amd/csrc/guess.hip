@@ -1,0 +1,615 @@
+// guess.hip -- the learned initial guess of the BiCGSTAB solve.
+//
+// FeeLLGood starts each solve from the previous v projected on the new tangent plane
+// (LinAlgebra::buildInitGuess, src/linear_algebra.cpp:13-24). This network proposes a correction
+// to that guess; the Krylov solve to the same tolerance follows unchanged, so a poor proposal
+// costs iterations, never accuracy.
+//
+// Network, per magnetic node i (weights stored in BF16, arithmetic in FP32):
+//   features f_i = [u_i, s_h h_i, s_v v_i]  with h_i = F_t,i / m_i the tangent part of the
+//                  assembled right-hand side over the lumped mass, v_i the previous v (9 values)
+//   aggregation a_i = mean of f_j over the magnetic mesh-edge neighbours j of i
+//   x = [f_i, a_i] (18) -> z1 = W1 x + b1 -> h1 = GELU(z1) (64) -> z2 = W2 h1 + b2 -> h2 = GELU(z2)
+//   (64) -> o = W3 h2 + b3 (3) -> v_pred = P_u (v_i + s_out o), P_u the tangent projection
+//   guess x0_i = (v_pred . ep_i, v_pred . eq_i) / gamma0
+// GELU is the tanh form 0.5 z (1 + tanh(sqrt(2/pi) (z + 0.044715 z^3))).
+// Training: loss = sum_i |v_pred,i - v*_i|^2 / (n_mag s_out^2) against the converged solution v*
+// of the same solve; Adam on FP32 master weights; the forward pass uses their BF16 rounding.
+// The scales are the root mean squares of h, v* and v* - P_u v over the first training batch, so
+// the reference guess has loss 1 on it; W3 and b3 start at zero (the network starts as the
+// reference guess).
+#include <cmath>
+#include <cstring>
+#include <random>
+#include <string>
+#include <vector>
+
+#include "ctx.h"
+
+namespace fg {
+
+namespace {
+
+constexpr int NF = 9, NX = 18, NH = 64, NO = 3;
+constexpr int OFF_W1 = 0, OFF_B1 = OFF_W1 + NH * NX, OFF_W2 = OFF_B1 + NH, OFF_B2 = OFF_W2 + NH * NH,
+              OFF_W3 = OFF_B2 + NH, OFF_B3 = OFF_W3 + NO * NH, NTRAIN = OFF_B3 + NO;
+constexpr int OFF_S = NTRAIN, NPARAM = NTRAIN + 3;
+static_assert(NPARAM == FG_GUESS_NPARAM, "parameter layout and header disagree");
+constexpr int GB = 128;  // threads per block of the per-node kernels
+
+__device__ inline float gelu(float z)
+{
+    const float k = 0.7978845608028654f;  // sqrt(2/pi)
+    return 0.5f * z * (1.0f + tanhf(k * (z + 0.044715f * z * z * z)));
+}
+__device__ inline float gelu_grad(float z)
+{
+    const float k = 0.7978845608028654f;
+    const float t = tanhf(k * (z + 0.044715f * z * z * z));
+    return 0.5f * (1.0f + t) + 0.5f * z * (1.0f - t * t) * k * (1.0f + 3.0f * 0.044715f * z * z);
+}
+
+// round-to-nearest-even BF16 value of a float, returned as a float
+__host__ __device__ inline float bf16_round(float x)
+{
+    uint32_t b;
+    __builtin_memcpy(&b, &x, 4);
+    b = (b + 0x7FFFu + ((b >> 16) & 1u)) & 0xFFFF0000u;
+    float y;
+    __builtin_memcpy(&y, &b, 4);
+    return y;
+}
+
+}  // namespace
+
+struct GuessNet {
+    int N = 0, nmag = 0, nblk = 0;
+    bool enabled = false, have_batch = false, scaled = false;
+    int adam_t = 0;
+    float *params = nullptr;                 // FP32 master + scales [NPARAM]
+    uint16_t *wbf = nullptr;                 // BF16 copy of the trainable block [NTRAIN]
+    float *adam_m = nullptr, *adam_v = nullptr, *grad = nullptr;
+    float *feat = nullptr;                   // [N][9] raw features of the last solve (u, h, v)
+    // training activations and deltas, [N][k]
+    float *xin = nullptr, *z1 = nullptr, *z2 = nullptr, *dz1 = nullptr, *dz2 = nullptr, *dout = nullptr;
+    float *gpart = nullptr;                  // per-block partial gradients
+    double *lpart = nullptr;                 // per-block loss partials (2 per block)
+    double *h_l = nullptr;                   // pinned
+};
+
+namespace {
+
+// BF16 rounding of the trainable block (biases are used in FP32 from the master copy)
+__global__ void k_guess_round(const float *__restrict__ p, uint16_t *__restrict__ wbf)
+{
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < NTRAIN; i += gridDim.x * blockDim.x) {
+        const float r = bf16_round(p[i]);
+        uint32_t b;
+        __builtin_memcpy(&b, &r, 4);
+        wbf[i] = (uint16_t)(b >> 16);
+    }
+}
+
+__device__ inline float bf16_to_float(uint16_t h)
+{
+    const uint32_t b = (uint32_t)h << 16;
+    float f;
+    __builtin_memcpy(&f, &b, 4);
+    return f;
+}
+
+// raw features: u, h = (b_y ep + b_x eq) / m (b = (eq.F, ep.F)), previous v
+__global__ __launch_bounds__(256) void k_guess_features(int N, const NodeRec *__restrict__ cur,
+                                                        const NodeRec *__restrict__ vsrc,
+                                                        const double2 *__restrict__ b,
+                                                        const double4 *__restrict__ ep4,
+                                                        const double4 *__restrict__ eq4,
+                                                        const double *__restrict__ mass,
+                                                        const int32_t *__restrict__ mag, float *__restrict__ feat)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    float f[NF] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    if (mag[i]) {
+        const NodeRec n = cur[i], v = vsrc[i];
+        const double2 bi = b[i];
+        const double4 p = ep4[i], q = eq4[i];
+        const double im = 1.0 / mass[i];
+        f[0] = (float)n.ux;
+        f[1] = (float)n.uy;
+        f[2] = (float)n.uz;
+        f[3] = (float)((bi.y * p.x + bi.x * q.x) * im);
+        f[4] = (float)((bi.y * p.y + bi.x * q.y) * im);
+        f[5] = (float)((bi.y * p.z + bi.x * q.z) * im);
+        f[6] = (float)v.vx;
+        f[7] = (float)v.vy;
+        f[8] = (float)v.vz;
+    }
+    for (int k = 0; k < NF; k++) feat[(size_t)i * NF + k] = f[k];
+}
+
+struct SWeights {
+    float W1[NH * NX], b1[NH], W2[NH * NH], b2[NH], W3[NO * NH], b3[NO], s[3];
+};
+
+__device__ inline void load_weights(SWeights &S, const float *__restrict__ p, const uint16_t *__restrict__ wbf)
+{
+    for (int k = threadIdx.x; k < NH * NX; k += blockDim.x) S.W1[k] = bf16_to_float(wbf[OFF_W1 + k]);
+    for (int k = threadIdx.x; k < NH * NH; k += blockDim.x) S.W2[k] = bf16_to_float(wbf[OFF_W2 + k]);
+    for (int k = threadIdx.x; k < NO * NH; k += blockDim.x) S.W3[k] = bf16_to_float(wbf[OFF_W3 + k]);
+    for (int k = threadIdx.x; k < NH; k += blockDim.x) {
+        S.b1[k] = p[OFF_B1 + k];
+        S.b2[k] = p[OFF_B2 + k];
+    }
+    if (threadIdx.x < NO) S.b3[threadIdx.x] = p[OFF_B3 + threadIdx.x];
+    if (threadIdx.x < 3) S.s[threadIdx.x] = p[OFF_S + threadIdx.x];
+    __syncthreads();
+}
+
+// network input x = [f_i, mean f_j] with the feature scales applied
+__device__ inline void gather_input(int i, const float *__restrict__ feat, const int32_t *__restrict__ mag,
+                                    const int64_t *__restrict__ sptr, const int32_t *__restrict__ swid,
+                                    const int32_t *__restrict__ scol, const float *s, float x[NX])
+{
+    const float sc[NF] = {1, 1, 1, s[0], s[0], s[0], s[1], s[1], s[1]};
+    for (int k = 0; k < NF; k++) x[k] = feat[(size_t)i * NF + k] * sc[k];
+    float a[NF] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    int cnt = 0;
+    const int sl = i >> 6, lane = i & 63;
+    const int64_t off = sptr[sl];
+    const int width = swid[sl];
+    for (int k = 0; k < width; k++) {
+        const int j = scol[off + (int64_t)k * SELL_C + lane];
+        if (j == i || !mag[j]) continue;
+        cnt++;
+        for (int q = 0; q < NF; q++) a[q] += feat[(size_t)j * NF + q];
+    }
+    const float ic = cnt > 0 ? 1.0f / (float)cnt : 0.0f;
+    for (int k = 0; k < NF; k++) x[NF + k] = a[k] * ic * sc[k];
+}
+
+// Forward pass per magnetic node. mode 0: write the guess x, w. mode 1 (training): store the
+// input and pre-activations, and the output delta and loss partials against the solution v*
+// (NEXT v) and the reference guess.
+template <int MODE>
+__global__ __launch_bounds__(GB) void k_guess_forward(
+    int N, double inv_norm, const float *__restrict__ feat, const int32_t *__restrict__ mag,
+    const int64_t *__restrict__ sptr, const int32_t *__restrict__ swid, const int32_t *__restrict__ scol,
+    const float *__restrict__ p, const uint16_t *__restrict__ wbf, const double4 *__restrict__ ep4,
+    const double4 *__restrict__ eq4, const NodeRec *__restrict__ nxt, double2 *__restrict__ xg,
+    double4 *__restrict__ wg, float *__restrict__ xin, float *__restrict__ z1o, float *__restrict__ z2o,
+    float *__restrict__ dout, double *__restrict__ lpart)
+{
+    __shared__ SWeights S;
+    __shared__ double red[2][GB];
+    load_weights(S, p, wbf);
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    double l_net = 0, l_ref = 0;
+    if (i < N && mag[i]) {
+        float x[NX];
+        gather_input(i, feat, mag, sptr, swid, scol, S.s, x);
+        float h1[NH];
+#pragma unroll
+        for (int o = 0; o < NH; o++) {
+            float z = S.b1[o];
+#pragma unroll
+            for (int k = 0; k < NX; k++) z += S.W1[o * NX + k] * x[k];
+            if (MODE == 1) z1o[(size_t)i * NH + o] = z;
+            h1[o] = gelu(z);
+        }
+        float out[NO] = {S.b3[0], S.b3[1], S.b3[2]};
+        for (int o = 0; o < NH; o++) {
+            float z = S.b2[o];
+#pragma unroll
+            for (int k = 0; k < NH; k++) z += S.W2[o * NH + k] * h1[k];
+            if (MODE == 1) z2o[(size_t)i * NH + o] = z;
+            const float h = gelu(z);
+            out[0] += S.W3[o] * h;
+            out[1] += S.W3[NH + o] * h;
+            out[2] += S.W3[2 * NH + o] * h;
+        }
+        // v_pred = P_u (v + s_out o)
+        const double u[3] = {feat[(size_t)i * NF], feat[(size_t)i * NF + 1], feat[(size_t)i * NF + 2]};
+        const double vp[3] = {feat[(size_t)i * NF + 6], feat[(size_t)i * NF + 7], feat[(size_t)i * NF + 8]};
+        const double so = S.s[2];
+        double v[3], vr[3];
+        for (int k = 0; k < 3; k++) {
+            v[k] = vp[k] + so * (double)out[k];
+            vr[k] = vp[k];
+        }
+        const double un = u[0] * v[0] + u[1] * v[1] + u[2] * v[2], ur = u[0] * vr[0] + u[1] * vr[1] + u[2] * vr[2];
+        for (int k = 0; k < 3; k++) {
+            v[k] -= un * u[k];
+            vr[k] -= ur * u[k];
+        }
+        if (MODE == 0) {
+            const double4 ep = ep4[i], eq = eq4[i];
+            const double2 xx = make_double2((v[0] * ep.x + v[1] * ep.y + v[2] * ep.z) / GAMMA0,
+                                            (v[0] * eq.x + v[1] * eq.y + v[2] * eq.z) / GAMMA0);
+            xg[i] = xx;
+            wg[i] = make_double4(xx.x * ep.x + xx.y * eq.x, xx.x * ep.y + xx.y * eq.y, xx.x * ep.z + xx.y * eq.z, 0.0);
+        } else {
+            for (int k = 0; k < NX; k++) xin[(size_t)i * NX + k] = x[k];
+            const NodeRec n = nxt[i];
+            const double vs[3] = {n.vx, n.vy, n.vz};
+            double e[3];
+            for (int k = 0; k < 3; k++) {
+                e[k] = v[k] - vs[k];
+                l_net += e[k] * e[k];
+                l_ref += (vr[k] - vs[k]) * (vr[k] - vs[k]);
+            }
+            // dL/do = s_out P_u dL/dv_pred, dL/dv_pred = 2 e inv_norm
+            const double eu = u[0] * e[0] + u[1] * e[1] + u[2] * e[2];
+            for (int k = 0; k < 3; k++) dout[(size_t)i * NO + k] = (float)(2.0 * inv_norm * so * (e[k] - eu * u[k]));
+        }
+    } else if (MODE == 1 && i < N) {
+        for (int k = 0; k < NX; k++) xin[(size_t)i * NX + k] = 0.f;
+        for (int k = 0; k < NH; k++) {
+            z1o[(size_t)i * NH + k] = 0.f;
+            z2o[(size_t)i * NH + k] = 0.f;
+        }
+        for (int k = 0; k < NO; k++) dout[(size_t)i * NO + k] = 0.f;
+    }
+    if (MODE == 1) {
+        red[0][threadIdx.x] = l_net;
+        red[1][threadIdx.x] = l_ref;
+        __syncthreads();
+        for (int s = GB / 2; s > 0; s >>= 1) {
+            if ((int)threadIdx.x < s) {
+                red[0][threadIdx.x] += red[0][threadIdx.x + s];
+                red[1][threadIdx.x] += red[1][threadIdx.x + s];
+            }
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) {
+            lpart[2 * blockIdx.x] = red[0][0] * inv_norm;
+            lpart[2 * blockIdx.x + 1] = red[1][0] * inv_norm;
+        }
+    }
+}
+
+// Backward through the layers per node: dz2 = (W3^T dout) * GELU'(z2), dz1 = (W2^T dz2) * GELU'(z1)
+__global__ __launch_bounds__(GB) void k_guess_backward(int N, const float *__restrict__ p,
+                                                      const uint16_t *__restrict__ wbf,
+                                                      const float *__restrict__ z1, const float *__restrict__ z2,
+                                                      const float *__restrict__ dout, float *__restrict__ dz1,
+                                                      float *__restrict__ dz2)
+{
+    __shared__ SWeights S;
+    load_weights(S, p, wbf);
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    const float d0 = dout[(size_t)i * NO], d1 = dout[(size_t)i * NO + 1], d2 = dout[(size_t)i * NO + 2];
+    float g2[NH];
+#pragma unroll
+    for (int o = 0; o < NH; o++) {
+        const float dh = S.W3[o] * d0 + S.W3[NH + o] * d1 + S.W3[2 * NH + o] * d2;
+        g2[o] = dh * gelu_grad(z2[(size_t)i * NH + o]);
+        dz2[(size_t)i * NH + o] = g2[o];
+    }
+    for (int k = 0; k < NH; k++) {
+        float dh = 0;
+#pragma unroll
+        for (int o = 0; o < NH; o++) dh += S.W2[o * NH + k] * g2[o];
+        dz1[(size_t)i * NH + k] = dh * gelu_grad(z1[(size_t)i * NH + k]);
+    }
+}
+
+// Partial weight gradients over a chunk of nodes: G[r][c] = sum_n A[n][r] B[n][c] with an implicit
+// last column B[n][rb] = 1 (the bias gradient). B is an activation array, or the GELU of a
+// pre-activation array when act is set. One partial of ra*(rb+1) values per block.
+constexpr int WG_CHUNK = 2048, WG_TILE = 32;
+__global__ __launch_bounds__(256) void k_guess_wgrad(int N, const float *__restrict__ A, int ra,
+                                                     const float *__restrict__ B, int rb, int act,
+                                                     float *__restrict__ part)
+{
+    __shared__ float sA[WG_TILE][NH];
+    __shared__ float sB[WG_TILE][NH + 1];
+    const int n0 = blockIdx.x * WG_CHUNK, n1 = min(N, n0 + WG_CHUNK);
+    const int ne = ra * (rb + 1);
+    float acc[17];
+    for (int q = 0; q < 17; q++) acc[q] = 0.f;
+    for (int t0 = n0; t0 < n1; t0 += WG_TILE) {
+        const int nt = min(WG_TILE, n1 - t0);
+        for (int k = threadIdx.x; k < WG_TILE * NH; k += blockDim.x) {
+            const int r = k / NH, c = k % NH;
+            float a = 0.f, bv = 0.f;
+            if (r < nt && c < ra) a = A[(size_t)(t0 + r) * ra + c];
+            if (r < nt && c < rb) {
+                bv = B[(size_t)(t0 + r) * rb + c];
+                if (act) bv = gelu(bv);
+            }
+            sA[r][c] = a;
+            sB[r][c] = bv;
+        }
+        __syncthreads();
+        for (int q = 0; q < 17; q++) {
+            const int e = threadIdx.x + q * 256;
+            if (e >= ne) break;
+            const int r = e / (rb + 1), c = e % (rb + 1);
+            float s = 0.f;
+            if (c < rb)
+                for (int n = 0; n < nt; n++) s += sA[n][r] * sB[n][c];
+            else
+                for (int n = 0; n < nt; n++) s += sA[n][r];
+            acc[q] += s;
+        }
+        __syncthreads();
+    }
+    for (int q = 0; q < 17; q++) {
+        const int e = threadIdx.x + q * 256;
+        if (e < ne) part[(size_t)blockIdx.x * ne + e] = acc[q];
+    }
+}
+
+// grad[dst(e)] = sum over blocks of part[blk][e], scattered into the parameter layout:
+// weights at wofs + r * rb + c, bias at bofs + r
+__global__ void k_guess_reduce(int nblk, int ra, int rb, const float *__restrict__ part, int wofs, int bofs,
+                               float *__restrict__ grad)
+{
+    const int ne = ra * (rb + 1);
+    for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < ne; e += gridDim.x * blockDim.x) {
+        double s = 0;
+        for (int b = 0; b < nblk; b++) s += part[(size_t)b * ne + e];
+        const int r = e / (rb + 1), c = e % (rb + 1);
+        grad[c < rb ? wofs + r * rb + c : bofs + r] = (float)s;
+    }
+}
+
+__global__ void k_guess_adam(float *__restrict__ p, const float *__restrict__ g, float *__restrict__ m,
+                             float *__restrict__ v, float lr, float c1, float c2)
+{
+    const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < NTRAIN; i += gridDim.x * blockDim.x) {
+        const float gi = g[i];
+        m[i] = b1 * m[i] + (1 - b1) * gi;
+        v[i] = b2 * v[i] + (1 - b2) * gi * gi;
+        p[i] -= lr * (m[i] * c1) / (sqrtf(v[i] * c2) + eps);
+    }
+}
+
+// sums for the scales: |h|^2, |v*|^2, |v* - P_u v|^2 over magnetic nodes
+__global__ __launch_bounds__(256) void k_guess_moments(int N, const float *__restrict__ feat,
+                                                       const int32_t *__restrict__ mag,
+                                                       const NodeRec *__restrict__ nxt, double *__restrict__ part)
+{
+    __shared__ double red[3][256];
+    double s[3] = {0, 0, 0};
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x) {
+        if (!mag[i]) continue;
+        const float *f = feat + (size_t)i * NF;
+        const NodeRec n = nxt[i];
+        const double vs[3] = {n.vx, n.vy, n.vz};
+        const double ur = (double)f[0] * f[6] + (double)f[1] * f[7] + (double)f[2] * f[8];
+        for (int k = 0; k < 3; k++) {
+            s[0] += (double)f[3 + k] * f[3 + k];
+            s[1] += vs[k] * vs[k];
+            const double d = vs[k] - (f[6 + k] - ur * f[k]);
+            s[2] += d * d;
+        }
+    }
+    for (int k = 0; k < 3; k++) red[k][threadIdx.x] = s[k];
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if ((int)threadIdx.x < h)
+            for (int k = 0; k < 3; k++) red[k][threadIdx.x] += red[k][threadIdx.x + h];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        for (int k = 0; k < 3; k++) part[3 * blockIdx.x + k] = red[k][0];
+}
+
+template <class T>
+void galloc(T **p, size_t n)
+{
+    if (hipMalloc((void **)p, std::max<size_t>(1, n) * sizeof(T)) != hipSuccess) throw std::string("hipMalloc (guess)");
+}
+
+void sync_or_throw(Ctx &c, const char *what)
+{
+    const hipError_t e = hipStreamSynchronize(c.stream);
+    if (e != hipSuccess) throw std::string(what) + ": " + hipGetErrorString(e);
+}
+
+}  // namespace
+
+void guess_destroy(Ctx &c)
+{
+    GuessNet *g = c.guess;
+    if (!g) return;
+    void *ptrs[] = {g->params, g->wbf, g->adam_m, g->adam_v, g->grad, g->feat, g->xin, g->z1, g->z2,
+                    g->dz1, g->dz2, g->dout, g->gpart, g->lpart};
+    for (void *p : ptrs)
+        if (p) (void)hipFree(p);
+    if (g->h_l) (void)hipHostFree(g->h_l);
+    delete g;
+    c.guess = nullptr;
+}
+
+// Allocate the network and draw initial weights (W1, W2 ~ N(0, 1/fan_in), W3 = 0, biases 0).
+void guess_init(Ctx &c, uint64_t seed)
+{
+    if (!c.guess) {
+        auto *g = new GuessNet();
+        c.guess = g;
+        g->N = c.N;
+        for (int32_t m : c.h_mag) g->nmag += m != 0;
+        g->nblk = (c.N + WG_CHUNK - 1) / WG_CHUNK;
+        galloc(&g->params, NPARAM);
+        galloc(&g->wbf, NTRAIN);
+        galloc(&g->adam_m, NTRAIN);
+        galloc(&g->adam_v, NTRAIN);
+        galloc(&g->grad, NTRAIN);
+        galloc(&g->feat, (size_t)c.N * NF);
+        galloc(&g->lpart, 2 * ((c.N + GB - 1) / GB) + 3 * 1024);
+        if (hipHostMalloc((void **)&g->h_l, 3 * 1024 * sizeof(double), hipHostMallocDefault) != hipSuccess)
+            throw std::string("hipHostMalloc (guess)");
+    }
+    GuessNet *g = c.guess;
+    std::vector<float> p(NPARAM, 0.f);
+    std::mt19937_64 gen(seed);
+    std::normal_distribution<float> nd(0.f, 1.f);
+    for (int k = 0; k < NH * NX; k++) p[OFF_W1 + k] = nd(gen) / std::sqrt((float)NX);
+    for (int k = 0; k < NH * NH; k++) p[OFF_W2 + k] = nd(gen) / std::sqrt((float)NH);
+    guess_set(c, p.data());
+    g->scaled = false;
+}
+
+void guess_set(Ctx &c, const float *params)
+{
+    GuessNet *g = c.guess;
+    if (hipMemcpy(g->params, params, NPARAM * sizeof(float), hipMemcpyHostToDevice) != hipSuccess)
+        throw std::string("hipMemcpy (guess)");
+    (void)hipMemset(g->adam_m, 0, NTRAIN * sizeof(float));
+    (void)hipMemset(g->adam_v, 0, NTRAIN * sizeof(float));
+    g->adam_t = 0;
+    g->scaled = params[OFF_S] > 0 && params[OFF_S + 1] > 0 && params[OFF_S + 2] > 0;
+    k_guess_round<<<32, 256, 0, c.stream>>>(g->params, g->wbf);
+    sync_or_throw(c, "guess set");
+}
+
+void guess_get(Ctx &c, float *params)
+{
+    sync_or_throw(c, "guess get");
+    if (hipMemcpy(params, c.guess->params, NPARAM * sizeof(float), hipMemcpyDeviceToHost) != hipSuccess)
+        throw std::string("hipMemcpy (guess)");
+}
+
+// after the assembly of a solve: record the features (always, so that plain solves feed training)
+// and, when enabled, overwrite the initial guess
+void guess_on_assembly(Ctx &c)
+{
+    GuessNet *g = c.guess;
+    if (!g) return;
+    const NodeRec *vsrc = c.rec[1 - c.cur];
+    {
+        Prof pr(c, "guess_features");
+        k_guess_features<<<(c.N + 255) / 256, 256, 0, c.stream>>>(c.N, c.rec[c.cur], vsrc, c.b, c.ep4, c.eq4, c.mass,
+                                                                  c.mag, g->feat);
+    }
+    g->have_batch = false;  // the solution of this solve is not known yet
+    if (g->enabled && g->scaled) {
+        Prof pr(c, "guess_forward");
+        k_guess_forward<0><<<(c.N + GB - 1) / GB, GB, 0, c.stream>>>(
+            c.N, 0.0, g->feat, c.mag, c.slice_ptr, c.slice_w, c.scol, g->params, g->wbf, c.ep4, c.eq4, nullptr, c.x,
+            c.w, nullptr, nullptr, nullptr, nullptr, nullptr);
+    }
+}
+
+void guess_on_solved(Ctx &c)
+{
+    if (c.guess) c.guess->have_batch = true;
+}
+
+static void ensure_training_buffers(GuessNet *g)
+{
+    if (g->xin) return;
+    galloc(&g->xin, (size_t)g->N * NX);
+    galloc(&g->z1, (size_t)g->N * NH);
+    galloc(&g->z2, (size_t)g->N * NH);
+    galloc(&g->dz1, (size_t)g->N * NH);
+    galloc(&g->dz2, (size_t)g->N * NH);
+    galloc(&g->dout, (size_t)g->N * NO);
+    galloc(&g->gpart, (size_t)g->nblk * NH * (NH + 1));
+}
+
+static void compute_scales(Ctx &c)
+{
+    GuessNet *g = c.guess;
+    const int nb = 1024;
+    k_guess_moments<<<nb, 256, 0, c.stream>>>(c.N, g->feat, c.mag, c.rec[1 - c.cur], g->lpart);
+    if (hipMemcpyAsync(g->h_l, g->lpart, 3 * nb * sizeof(double), hipMemcpyDeviceToHost, c.stream) != hipSuccess)
+        throw std::string("hipMemcpy (guess moments)");
+    sync_or_throw(c, "guess moments");
+    double s[3] = {0, 0, 0};
+    for (int b = 0; b < nb; b++)
+        for (int k = 0; k < 3; k++) s[k] += g->h_l[3 * b + k];
+    const double n = std::max(1, g->nmag);
+    float sc[3];
+    sc[0] = s[0] > 0 ? (float)(1.0 / std::sqrt(s[0] / n)) : 1.f;  // 1 / rms|h|
+    sc[1] = s[1] > 0 ? (float)(1.0 / std::sqrt(s[1] / n)) : 1.f;  // 1 / rms|v*|
+    sc[2] = s[2] > 0 ? (float)std::sqrt(s[2] / n) : 1.f;          // rms|v* - P v|
+    if (hipMemcpy(g->params + OFF_S, sc, 3 * sizeof(float), hipMemcpyHostToDevice) != hipSuccess)
+        throw std::string("hipMemcpy (guess scales)");
+    g->scaled = true;
+}
+
+// loss of the network and of the reference guess on the last solve, gradient into g->grad
+static void loss_and_gradient(Ctx &c, double *loss, double *loss_ref)
+{
+    GuessNet *g = c.guess;
+    ensure_training_buffers(g);
+    float so;
+    if (hipMemcpy(&so, g->params + OFF_S + 2, sizeof(float), hipMemcpyDeviceToHost) != hipSuccess)
+        throw std::string("hipMemcpy (guess)");
+    const double inv_norm = 1.0 / (std::max(1, g->nmag) * (double)so * (double)so);
+    const int nbf = (c.N + GB - 1) / GB;
+    k_guess_forward<1><<<nbf, GB, 0, c.stream>>>(c.N, inv_norm, g->feat, c.mag, c.slice_ptr, c.slice_w, c.scol,
+                                                 g->params, g->wbf, c.ep4, c.eq4, c.rec[1 - c.cur], nullptr, nullptr,
+                                                 g->xin, g->z1, g->z2, g->dout, g->lpart);
+    k_guess_backward<<<nbf, GB, 0, c.stream>>>(c.N, g->params, g->wbf, g->z1, g->z2, g->dout, g->dz1, g->dz2);
+    // weight gradients: (dout, h2) -> W3, b3; (dz2, h1) -> W2, b2; (dz1, x) -> W1, b1
+    struct L {
+        const float *A;
+        int ra;
+        const float *B;
+        int rb, act, wofs, bofs;
+    } layers[3] = {{g->dout, NO, g->z2, NH, 1, OFF_W3, OFF_B3},
+                   {g->dz2, NH, g->z1, NH, 1, OFF_W2, OFF_B2},
+                   {g->dz1, NH, g->xin, NX, 0, OFF_W1, OFF_B1}};
+    for (auto &l : layers) {
+        k_guess_wgrad<<<g->nblk, 256, 0, c.stream>>>(c.N, l.A, l.ra, l.B, l.rb, l.act, g->gpart);
+        k_guess_reduce<<<32, 256, 0, c.stream>>>(g->nblk, l.ra, l.rb, g->gpart, l.wofs, l.bofs, g->grad);
+    }
+    if (hipMemcpyAsync(g->h_l, g->lpart, 2 * nbf * sizeof(double), hipMemcpyDeviceToHost, c.stream) != hipSuccess)
+        throw std::string("hipMemcpy (guess loss)");
+    sync_or_throw(c, "guess training");
+    double a = 0, b = 0;
+    for (int k = 0; k < nbf; k++) {
+        a += g->h_l[2 * k];
+        b += g->h_l[2 * k + 1];
+    }
+    if (loss) *loss = a;
+    if (loss_ref) *loss_ref = b;
+}
+
+void guess_gradient(Ctx &c, float *grad, double *loss, double *loss_ref)
+{
+    GuessNet *g = c.guess;
+    if (!g->have_batch) throw std::string("no solved step to train on");
+    if (!g->scaled) compute_scales(c);
+    loss_and_gradient(c, loss, loss_ref);
+    if (grad && hipMemcpy(grad, g->grad, NTRAIN * sizeof(float), hipMemcpyDeviceToHost) != hipSuccess)
+        throw std::string("hipMemcpy (guess gradient)");
+}
+
+void guess_train(Ctx &c, float lr, double *loss, double *loss_ref)
+{
+    GuessNet *g = c.guess;
+    if (!g->have_batch) throw std::string("no solved step to train on");
+    if (!g->scaled) compute_scales(c);
+    loss_and_gradient(c, loss, loss_ref);
+    g->adam_t++;
+    const float c1 = 1.0f / (1.0f - std::pow(0.9f, (float)g->adam_t));
+    const float c2 = 1.0f / (1.0f - std::pow(0.999f, (float)g->adam_t));
+    k_guess_adam<<<32, 256, 0, c.stream>>>(g->params, g->grad, g->adam_m, g->adam_v, lr, c1, c2);
+    k_guess_round<<<32, 256, 0, c.stream>>>(g->params, g->wbf);
+    sync_or_throw(c, "guess update");
+}
+
+bool guess_enable(Ctx &c, int on)
+{
+    if (!c.guess) return false;
+    c.guess->enabled = on != 0;
+    return true;
+}
+
+void guess_batch(Ctx &c, float *feat)
+{
+    GuessNet *g = c.guess;
+    sync_or_throw(c, "guess batch");
+    if (hipMemcpy(feat, g->feat, (size_t)c.N * NF * sizeof(float), hipMemcpyDeviceToHost) != hipSuccess)
+        throw std::string("hipMemcpy (guess batch)");
+}
+
+}  // namespace fg

@@ -147,6 +147,38 @@ class Engine:
                 "near_coefs", "near_columns", "targets", "sources", "order", "target_leaves", "m2m_cells"]
         return {k: int(a[i]) for i, k in enumerate(keys)}
 
+    # ---- learned initial guess ----
+    def guess_init(self, seed=0):
+        self._chk(self.lib.fg_guess_init(self._h, seed), "fg_guess_init")
+
+    def guess_set(self, params):
+        p = np.ascontiguousarray(params, dtype=np.float32)
+        self._chk(self.lib.fg_guess_set(self._h, nv.fptr(p), p.size), "fg_guess_set")
+
+    def guess_get(self):
+        p = np.zeros(nv.FG_GUESS_NPARAM, dtype=np.float32)
+        self._chk(self.lib.fg_guess_get(self._h, nv.fptr(p), p.size), "fg_guess_get")
+        return p
+
+    def guess_enable(self, on=True):
+        self._chk(self.lib.fg_guess_enable(self._h, 1 if on else 0), "fg_guess_enable")
+
+    def guess_train(self, lr):
+        a, b = C.c_double(), C.c_double()
+        self._chk(self.lib.fg_guess_train(self._h, lr, C.byref(a), C.byref(b)), "fg_guess_train")
+        return a.value, b.value
+
+    def guess_gradient(self):
+        g = np.zeros(nv.FG_GUESS_NPARAM - 3, dtype=np.float32)
+        a, b = C.c_double(), C.c_double()
+        self._chk(self.lib.fg_guess_gradient(self._h, nv.fptr(g), C.byref(a), C.byref(b)), "fg_guess_gradient")
+        return g, a.value, b.value
+
+    def guess_features(self):
+        f = np.zeros(9 * self.N, dtype=np.float32)
+        self._chk(self.lib.fg_guess_features(self._h, nv.fptr(f)), "fg_guess_features")
+        return f.reshape(self.N, 9)
+
     def reset_kernel_stats(self):
         self.lib.fg_reset_kernel_stats(self._h)
 

@@ -61,6 +61,8 @@ class SolveInfo(C.Structure):
 _d = C.POINTER(C.c_double)
 _i32 = C.POINTER(C.c_int32)
 _i64 = C.POINTER(C.c_int64)
+_f32 = C.POINTER(C.c_float)
+FG_GUESS_NPARAM = 5574
 
 # name -> (restype, argtypes); every symbol of include/fgmi.h
 HIP_SYMBOLS = {
@@ -84,6 +86,13 @@ HIP_SYMBOLS = {
     "fg_kernel_stats": (C.c_int, [C.c_void_p, C.c_int, C.c_char_p, C.c_int, _i64, _d]),
     "fg_reset_kernel_stats": (C.c_int, [C.c_void_p]),
     "fg_fmm_info": (C.c_int, [C.c_void_p, _i64]),
+    "fg_guess_init": (C.c_int, [C.c_void_p, C.c_uint64]),
+    "fg_guess_set": (C.c_int, [C.c_void_p, _f32, C.c_int64]),
+    "fg_guess_get": (C.c_int, [C.c_void_p, _f32, C.c_int64]),
+    "fg_guess_enable": (C.c_int, [C.c_void_p, C.c_int]),
+    "fg_guess_train": (C.c_int, [C.c_void_p, C.c_float, _d, _d]),
+    "fg_guess_gradient": (C.c_int, [C.c_void_p, _f32, _d, _d]),
+    "fg_guess_features": (C.c_int, [C.c_void_p, _f32]),
     "fg_synchronize": (C.c_int, [C.c_void_p]),
     "fg_fmm_reference": (C.c_int, [_d, C.c_int64, _d, C.c_int64, _d, _d, C.c_int, C.c_int, _d, _d,
                                    C.POINTER(C.c_int)]),
@@ -151,6 +160,13 @@ def iptr(a):
         return None
     assert a.dtype == np.int32 and a.flags.c_contiguous
     return a.ctypes.data_as(_i32)
+
+
+def fptr(a):
+    if a is None:
+        return None
+    assert a.dtype == np.float32 and a.flags.c_contiguous
+    return a.ctypes.data_as(_f32)
 
 
 def lptr(a):

@@ -68,6 +68,8 @@ SYMBOLS = {
     "orc_mt_uniform": (None, [C.c_uint32, C.c_int, _d]),
     "orc_basis_angles": (None, [C.c_uint32, C.c_int, _d]),
     "orc_run": (C.c_int, [C.POINTER(nv.Mesh), _d, C.POINTER(RunSettings), _d, _i32]),
+    "orc_guess_eval": (None, [C.POINTER(C.c_float), C.c_int, C.c_int, C.POINTER(C.c_float),
+                              C.POINTER(C.c_int64), _i32, _i32, _d, _d, _d, _d, _d]),
 }
 
 _lib = None
@@ -240,3 +242,22 @@ def run(mesh, u0, final_time, evol_time_step, dt_min, dt_max, du_max, Bext, seed
     u0 = np.ascontiguousarray(u0, dtype=np.float64).reshape(-1)
     st = lib().orc_run(C.byref(mesh), d(u0), C.byref(s), d(evol), C.byref(n))
     return st, evol[:12 * n.value].reshape(-1, 12)
+
+
+def guess_eval(params, feat, adj_ptr, adj_idx, mag, v_target=None, round_bf16=True, want_grad=True):
+    """The learned-guess network on the CPU: (v_pred [n,3], loss, loss_ref, grad)."""
+    n = len(mag)
+    params = np.ascontiguousarray(params, dtype=np.float32)
+    feat = np.ascontiguousarray(feat, dtype=np.float32).reshape(-1)
+    adj_ptr = np.ascontiguousarray(adj_ptr, dtype=np.int64)
+    adj_idx = np.ascontiguousarray(adj_idx, dtype=np.int32)
+    mag = np.ascontiguousarray(mag, dtype=np.int32)
+    vt = None if v_target is None else np.ascontiguousarray(v_target, dtype=np.float64).reshape(-1)
+    vp = np.zeros(3 * n)
+    grad = np.zeros(len(params) - 3) if want_grad else None
+    loss, loss_ref = C.c_double(), C.c_double()
+    lib().orc_guess_eval(params.ctypes.data_as(C.POINTER(C.c_float)), 1 if round_bf16 else 0, n,
+                         feat.ctypes.data_as(C.POINTER(C.c_float)), adj_ptr.ctypes.data_as(C.POINTER(C.c_int64)),
+                         adj_idx.ctypes.data_as(_i32), mag.ctypes.data_as(_i32), d(vt), d(vp), C.byref(loss),
+                         C.byref(loss_ref), d(grad))
+    return vp.reshape(n, 3), loss.value, loss_ref.value, grad

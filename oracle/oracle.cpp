@@ -1282,3 +1282,135 @@ bailout:
 }
 
 } // extern "C"
+
+/* ---------------- learned initial guess: the network of the engine's guess, restated ----------------
+ * Same parameter layout and formulas as the engine (documented in include/fgmi.h): features
+ * f = [u, s_h h, s_v v], neighbour mean over the given adjacency (magnetic neighbours only), input
+ * [f, mean], 18 -> 64 -> 64 -> 3 with the tanh GELU, v_pred = P_u (v + s_out o), loss
+ * sum |v_pred - v*|^2 / (n_mag s_out^2). Arithmetic in double; with round_bf16 the weight
+ * matrices are first rounded to BF16 (round to nearest even) as the engine's forward pass uses
+ * them. Gradients are with respect to the (rounded) weights and the biases. */
+namespace {
+const int GNF = 9, GNX = 18, GNH = 64, GNO = 3;
+const int G_W1 = 0, G_B1 = G_W1 + GNH * GNX, G_W2 = G_B1 + GNH, G_B2 = G_W2 + GNH * GNH, G_W3 = G_B2 + GNH,
+          G_B3 = G_W3 + GNO * GNH, G_NTRAIN = G_B3 + GNO, G_S = G_NTRAIN;
+double g_bf16(float x)
+{
+    uint32_t b;
+    std::memcpy(&b, &x, 4);
+    b = (b + 0x7FFFu + ((b >> 16) & 1u)) & 0xFFFF0000u;
+    float y;
+    std::memcpy(&y, &b, 4);
+    return y;
+}
+double g_gelu(double z) { return 0.5 * z * (1.0 + std::tanh(0.7978845608028654 * (z + 0.044715 * z * z * z))); }
+double g_gelu_grad(double z)
+{
+    const double k = 0.7978845608028654, t = std::tanh(k * (z + 0.044715 * z * z * z));
+    return 0.5 * (1.0 + t) + 0.5 * z * (1.0 - t * t) * k * (1.0 + 3.0 * 0.044715 * z * z);
+}
+} // namespace
+
+extern "C" {
+
+void orc_guess_eval(const float *params, int round_bf16, int n, const float *feat, const int64_t *adj_ptr,
+                    const int32_t *adj_idx, const int32_t *mag, const double *v_target, double *v_pred,
+                    double *loss, double *loss_ref, double *grad)
+{
+    std::vector<double> W(G_NTRAIN);
+    for (int k = 0; k < G_NTRAIN; k++) {
+        const bool is_w = (k >= G_W1 && k < G_B1) || (k >= G_W2 && k < G_B2) || (k >= G_W3 && k < G_B3);
+        W[k] = (round_bf16 && is_w) ? g_bf16(params[k]) : (double)params[k];
+    }
+    const double sh = params[G_S], sv = params[G_S + 1], so = params[G_S + 2];
+    int nmag = 0;
+    for (int i = 0; i < n; i++) nmag += mag[i] != 0;
+    const double inv_norm = 1.0 / (std::max(1, nmag) * so * so);
+    if (grad) std::fill(grad, grad + G_NTRAIN, 0.0);
+    double L = 0, Lr = 0;
+    const double sc[GNF] = {1, 1, 1, sh, sh, sh, sv, sv, sv};
+    for (int i = 0; i < n; i++) {
+        if (!mag[i]) {
+            if (v_pred) v_pred[3 * i] = v_pred[3 * i + 1] = v_pred[3 * i + 2] = 0;
+            continue;
+        }
+        double x[GNX], a[GNF] = {0};
+        int cnt = 0;
+        for (int k = 0; k < GNF; k++) x[k] = feat[(size_t)i * GNF + k] * sc[k];
+        for (int64_t e = adj_ptr[i]; e < adj_ptr[i + 1]; e++) {
+            const int j = adj_idx[e];
+            if (j == i || !mag[j]) continue;
+            cnt++;
+            for (int k = 0; k < GNF; k++) a[k] += feat[(size_t)j * GNF + k];
+        }
+        for (int k = 0; k < GNF; k++) x[GNF + k] = cnt ? a[k] / cnt * sc[k] : 0.0;
+        double z1[GNH], h1[GNH], z2[GNH], h2[GNH], out[GNO];
+        for (int o = 0; o < GNH; o++) {
+            double z = W[G_B1 + o];
+            for (int k = 0; k < GNX; k++) z += W[G_W1 + o * GNX + k] * x[k];
+            z1[o] = z;
+            h1[o] = g_gelu(z);
+        }
+        for (int o = 0; o < GNH; o++) {
+            double z = W[G_B2 + o];
+            for (int k = 0; k < GNH; k++) z += W[G_W2 + o * GNH + k] * h1[k];
+            z2[o] = z;
+            h2[o] = g_gelu(z);
+        }
+        for (int c = 0; c < GNO; c++) {
+            double z = W[G_B3 + c];
+            for (int k = 0; k < GNH; k++) z += W[G_W3 + c * GNH + k] * h2[k];
+            out[c] = z;
+        }
+        const double u[3] = {feat[(size_t)i * GNF], feat[(size_t)i * GNF + 1], feat[(size_t)i * GNF + 2]};
+        const double vp[3] = {feat[(size_t)i * GNF + 6], feat[(size_t)i * GNF + 7], feat[(size_t)i * GNF + 8]};
+        double v[3], vr[3];
+        for (int k = 0; k < 3; k++) {
+            v[k] = vp[k] + so * out[k];
+            vr[k] = vp[k];
+        }
+        const double un = u[0] * v[0] + u[1] * v[1] + u[2] * v[2], ur = u[0] * vr[0] + u[1] * vr[1] + u[2] * vr[2];
+        for (int k = 0; k < 3; k++) {
+            v[k] -= un * u[k];
+            vr[k] -= ur * u[k];
+        }
+        if (v_pred)
+            for (int k = 0; k < 3; k++) v_pred[3 * i + k] = v[k];
+        if (!v_target) continue;
+        double e[3];
+        for (int k = 0; k < 3; k++) {
+            e[k] = v[k] - v_target[3 * i + k];
+            L += e[k] * e[k] * inv_norm;
+            Lr += (vr[k] - v_target[3 * i + k]) * (vr[k] - v_target[3 * i + k]) * inv_norm;
+        }
+        if (!grad) continue;
+        const double eu = u[0] * e[0] + u[1] * e[1] + u[2] * e[2];
+        double d3o[GNO];
+        for (int k = 0; k < 3; k++) d3o[k] = 2.0 * inv_norm * so * (e[k] - eu * u[k]);
+        double dh2[GNH] = {0}, dh1[GNH] = {0};
+        for (int c = 0; c < GNO; c++) {
+            grad[G_B3 + c] += d3o[c];
+            for (int k = 0; k < GNH; k++) {
+                grad[G_W3 + c * GNH + k] += d3o[c] * h2[k];
+                dh2[k] += W[G_W3 + c * GNH + k] * d3o[c];
+            }
+        }
+        for (int o = 0; o < GNH; o++) {
+            const double dz = dh2[o] * g_gelu_grad(z2[o]);
+            grad[G_B2 + o] += dz;
+            for (int k = 0; k < GNH; k++) {
+                grad[G_W2 + o * GNH + k] += dz * h1[k];
+                dh1[k] += W[G_W2 + o * GNH + k] * dz;
+            }
+        }
+        for (int o = 0; o < GNH; o++) {
+            const double dz = dh1[o] * g_gelu_grad(z1[o]);
+            grad[G_B1 + o] += dz;
+            for (int k = 0; k < GNX; k++) grad[G_W1 + o * GNX + k] += dz * x[k];
+        }
+    }
+    if (loss) *loss = L;
+    if (loss_ref) *loss_ref = Lr;
+}
+
+} // extern "C"
