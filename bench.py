@@ -11,8 +11,12 @@ One step = the reference's hot loop body (src/time_integration.cpp:193-244): bas
 prepareElements, solve (assembly + BiCGSTAB + update), compute_all (demag u and v, energies,
 evolution) and max_angle. Inputs are resident in HBM before the timed region.
 
+The measured configuration (--mode, default learned+mixed: the trained initial-guess network and
+the mixed-precision FMM pass) is timed beside the plain FP64 engine on the same steps; the JSON
+line carries both rates, their ratio and the per-step |dm| of the fast mode against plain FP64.
+
 Prints one JSON line (see the keys below). Options: --gpus N (only 1 is supported), --steps K,
---warmup W, --demag {fmm,direct}, --precision {fp64,mixed}, --no-cpu, --small (the 48k-tet cube).
+--warmup W, --mode, --demag {fmm,direct}, --guess weights.npy, --no-cpu, --small (48k-tet cube).
 """
 import argparse
 import json
@@ -138,7 +142,7 @@ def kernel_model(name, n):
             # coefficient rows + shared columns + element values read once + per-target I/O
             "fmm_near": ("hbm", 8 * f["near_coefs"] + 4 * f["near_columns"] + 16 * (T + 3 * F)
                          + (8 + 16 + 16 + 4) * Tg + 12 * f["target_leaves"]),
-            "fmm_m2p": ("valu64", m2p_flops(p) * f["m2p_pairs"]),
+            "fmm_m2p": ("valu32" if n.get("precision") == "mixed" else "valu64", m2p_flops(p) * f["m2p_pairs"]),
             "fmm_m2l": ("valu64", m2l_flops(p) * f["m2l_pairs"]),
             "fmm_m2m": ("valu64", m2m_flops(p) * (f["cells"] - f["leaves"])),
             "fmm_p2m": ("valu64", p2m_flops(p) * f["sources"]),
@@ -146,66 +150,76 @@ def kernel_model(name, n):
     return models.get(name)
 
 
-def run_gpu(args, w, m, u0):
-    from feellgood_amd import native as nv
+def make_engine(args, w, m, precision, guess, profile):
     from feellgood_amd.engine import Engine, options
     opts = options(tol=w["tol"], max_iter=10000, demag=args.demag, fmm_order=args.fmm_order,
-                   fmm_leaf=args.fmm_leaf, precision=args.precision, profile=False)
+                   fmm_leaf=args.fmm_leaf, precision=precision, profile=profile)
     g = Engine(m.mesh, opts)
-    H = field(w)
-    dt = w["dt"]
-    pref = prefactor(dt)
-    rng = np.random.default_rng(w["seed"])
-    # compute_all at t0 (src/time_integration.cpp:136)
-    g.set_state(nv.FG_BOTH, u=u0)
-    g.calc_demag()
-    g.energy(H)
-    g.evolution()
-    iters = []
-    fails = 0
+    if guess is not None:
+        g.guess_set(guess)
+        g.guess_enable(True)
+    return g
 
-    def step():
-        nonlocal fails
-        g.base_projection((2 / math.pi) * rng.uniform())
-        g.prepare_elements(H, dt, pref)
-        failed, info = g.solve(dt)
+
+class Stepper:
+    """The reference loop body at fixed dt on one engine, with its own basis-angle stream."""
+
+    def __init__(self, g, w, u0):
+        from feellgood_amd import native as nv
+        self.g, self.H, self.dt = g, field(w), w["dt"]
+        self.pref = prefactor(self.dt)
+        self.rng = np.random.default_rng(w["seed"])
+        self.iters, self.fails = [], 0
+        # compute_all at t0 (src/time_integration.cpp:136)
+        g.set_state(nv.FG_BOTH, u=u0)
+        g.calc_demag()
+        g.energy(self.H)
+        g.evolution()
+
+    def angle(self):
+        return (2 / math.pi) * self.rng.uniform()
+
+    def step(self, r=None):
+        g = self.g
+        g.base_projection(self.angle() if r is None else r)
+        g.prepare_elements(self.H, self.dt, self.pref)
+        failed, info = g.solve(self.dt)
         if failed:
-            fails += 1
+            self.fails += 1
             return info
         g.calc_demag()
-        g.energy(H)
+        g.energy(self.H)
         g.evolution()
         g.max_angle()
-        iters.append(info.iterations)
+        self.iters.append(info.iterations)
         return info
 
+
+def run_gpu(args, w, m, u0, precision, guess):
+    """Clean timing of K steps after W warmup steps, then a profiled engine for per-kernel times."""
+    g = make_engine(args, w, m, precision, guess, False)
+    st = Stepper(g, w, u0)
     for _ in range(args.warmup):
-        step()
-    iters.clear()
+        st.step()
+    st.iters.clear()
     g.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        step()
+        st.step()
     g.synchronize()
     t1 = time.perf_counter()
     clean_ms = (t1 - t0) * 1e3 / args.steps
-    clean_iters = list(iters)
-    # profiled pass (HIP events on the engine stream around every kernel)
-    prof_steps = max(1, min(args.steps, args.prof_steps))
+    clean_iters, fails = list(st.iters), st.fails
     g.close()
-    opts.profile = 1
-    g = Engine(m.mesh, opts)
-    g.set_state(nv.FG_BOTH, u=u0)
-    g.calc_demag()
-    g.energy(H)
-    g.evolution()
-    step()
+    prof_steps = max(1, min(args.steps, args.prof_steps))
+    g = make_engine(args, w, m, precision, guess, True)
+    st = Stepper(g, w, u0)
+    st.step()
     g.reset_kernel_stats()
-    iters.clear()
     g.synchronize()
     tp0 = time.perf_counter()
     for _ in range(prof_steps):
-        step()
+        st.step()
     g.synchronize()
     tp1 = time.perf_counter()
     stats = g.kernel_stats()
@@ -215,11 +229,31 @@ def run_gpu(args, w, m, u0):
                 prof_ms_per_step=(tp1 - tp0) * 1e3 / prof_steps, prof_steps=prof_steps)
 
 
+def step_parity(args, w, m, u0, precision, guess, steps=3):
+    """Per-step |dm| of the fast mode against the plain FP64 engine: both start every step from the
+    plain engine's state with the same basis angle; max over `steps` steps (untimed)."""
+    from feellgood_amd import native as nv
+    ref = make_engine(args, w, m, "fp64", None, False)
+    fast = make_engine(args, w, m, precision, guess, False)
+    a, b = Stepper(ref, w, u0), Stepper(fast, w, u0)
+    dm = 0.0
+    for _ in range(steps):
+        u, v, phi, phiv = ref.get_state(nv.FG_CURRENT)
+        fast.set_state(nv.FG_BOTH, u=u, v=v, phi=phi, phiv=phiv)
+        r = a.angle()
+        a.step(r)
+        b.step(r)
+        dm = max(dm, float(np.abs(ref.get_state(nv.FG_CURRENT)[0] - fast.get_state(nv.FG_CURRENT)[0]).max()))
+    ref.close()
+    fast.close()
+    return dm
+
+
 def measured_traffic(kernel, args):
     """HBM bytes per launch of the dominant kernel from the committed rocprofv3 PMC summary
     (FETCH_SIZE and WRITE_SIZE scaled by a copy kernel), when it describes this configuration."""
     path = os.path.join(ROOT, "profiles", "film_fmm_fp64_traffic.json")
-    if args.small or args.precision != "fp64" or args.demag != "fmm" or args.fmm_order != 8 or args.fmm_leaf != 64:
+    if args.small or args.mode != "plain" or args.demag != "fmm" or args.fmm_order != 8 or args.fmm_leaf != 64:
         return None
     try:
         d = json.load(open(path))
@@ -248,8 +282,9 @@ def roofline(stats, sizes):
                 entry["GBps"] = round(work / avg_s / 1e9, 1)
                 entry["frac_hbm"] = round(work / avg_s / 1e9 / HBM_PEAK_GBS, 4)
             else:
+                pk = FP32_VALU_TFLOPS if bound == "valu32" else FP64_VALU_TFLOPS
                 entry["TFLOPs"] = round(work / avg_s / 1e12, 2)
-                entry["frac_fp64"] = round(work / avg_s / 1e12 / FP64_VALU_TFLOPS, 4)
+                entry["frac_valu"] = round(work / avg_s / 1e12 / pk, 4)
         per_kernel[name] = entry
     name, launches, ms = best
     mdl = kernel_model(name, sizes)
@@ -264,9 +299,11 @@ def roofline(stats, sizes):
               "avg_launch_us": round(avg_s * 1e6, 3)}
     else:
         achieved = work / avg_s / 1e12
-        rl = {"kernel": name, "bound": "valu", "achieved": round(achieved, 2), "peak": FP64_VALU_TFLOPS,
-              "unit": "TFLOP/s", "frac": round(achieved / FP64_VALU_TFLOPS, 4), "traffic": None,
-              "flops_per_launch": work, "avg_launch_us": round(avg_s * 1e6, 3)}
+        pk = FP32_VALU_TFLOPS if bound == "valu32" else FP64_VALU_TFLOPS
+        rl = {"kernel": name, "bound": "valu", "achieved": round(achieved, 2), "peak": pk,
+              "unit": "TFLOP/s", "frac": round(achieved / pk, 4), "traffic": None,
+              "flops_per_launch": work, "avg_launch_us": round(avg_s * 1e6, 3),
+              "precision": "fp32" if bound == "valu32" else "fp64"}
     return rl, per_kernel
 
 
@@ -308,15 +345,25 @@ def cpu_baseline(w, m, u0, n_targets=256):
             "seconds_per_step": round(per_step, 3)}
 
 
+def default_guess_weights(w):
+    """Trained network parameters committed for this workload, or None."""
+    path = os.path.join(ROOT, "feellgood_amd", "weights", f"{w['name']}_guess.npy")
+    if not os.path.exists(path):
+        return None, None
+    return np.load(path, allow_pickle=False).astype(np.float32), os.path.relpath(path, ROOT)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--mode", default="learned+mixed", choices=["learned+mixed", "mixed", "plain"],
+                    help="the measured configuration; the plain FP64 engine is timed beside it")
     ap.add_argument("--demag", default="fmm", choices=["fmm", "direct"])
-    ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
     ap.add_argument("--fmm-order", type=int, default=8)
     ap.add_argument("--fmm-leaf", type=int, default=64)
+    ap.add_argument("--guess", default=None, help="network parameters (.npy); default: the committed ones")
     ap.add_argument("--prof-steps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--small", action="store_true", help="the 48k-tet cube instead of the film")
@@ -331,11 +378,36 @@ def main():
     u0 = initial_state(m, w)
     print(f"[bench] mesh {m.n_nodes} nodes {m.n_tets} tets {m.n_tris} tris built in "
           f"{time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
-    res = run_gpu(args, w, m, u0)
-    print(f"[bench] gpu {res['ms_per_step']:.2f} ms/step, iterations {res['iters']}", file=sys.stderr, flush=True)
+    precision = "fp64" if args.mode == "plain" else "mixed"
+    guess, guess_src = None, None
+    if args.mode == "learned+mixed":
+        if args.guess:
+            guess, guess_src = np.load(args.guess, allow_pickle=False).astype(np.float32), args.guess
+        else:
+            guess, guess_src = default_guess_weights(w)
+    args.precision = precision
+    res = run_gpu(args, w, m, u0, precision, guess)
+    print(f"[bench] {args.mode}: {res['ms_per_step']:.2f} ms/step, iterations {res['iters']}", file=sys.stderr,
+          flush=True)
+    plain = None
+    if args.mode != "plain":
+        g = make_engine(args, w, m, "fp64", None, False)
+        st = Stepper(g, w, u0)
+        for _ in range(args.warmup):
+            st.step()
+        st.iters.clear()
+        g.synchronize()
+        t1 = time.perf_counter()
+        for _ in range(args.steps):
+            st.step()
+        g.synchronize()
+        plain = dict(ms=(time.perf_counter() - t1) * 1e3 / args.steps, iters=list(st.iters))
+        g.close()
+        print(f"[bench] plain fp64: {plain['ms']:.2f} ms/step, iterations {plain['iters']}", file=sys.stderr, flush=True)
+        dm = step_parity(args, w, m, u0, precision, guess)
+        print(f"[bench] per-step |dm| against plain fp64: {dm:.3e}", file=sys.stderr, flush=True)
     sizes = dict(N=m.n_nodes, T=m.n_tets, F=m.n_tris, S=5 * m.n_tets + 4 * m.n_tris)
-    # stencil entries N + 2E: E from the Kuhn grid connectivity is not needed exactly; use the
-    # engine's own count
+    # stencil entries N + 2E from the engine's own count
     from feellgood_amd.engine import Engine, options
     import ctypes as C
     g = Engine(m.mesh, options(tol=w["tol"]))
@@ -344,9 +416,12 @@ def main():
     g.close()
     sizes["nnz"] = nz.value // 4
     sizes["fmm"] = res["fmm"]
+    sizes["precision"] = precision
     rl, per_kernel = roofline(res["stats"], sizes)
     if rl and rl.get("bound"):
         rl["traffic"] = measured_traffic(rl["kernel"], args)
+    mode_txt = {"plain": "plain FP64", "mixed": "mixed precision (FP32 FMM near field and M2P, FP64 elsewhere)",
+                "learned+mixed": "learned initial guess + mixed precision"}[args.mode]
     out = {
         "metric": METRIC,
         "value": round(1e3 / res["ms_per_step"], 4),
@@ -358,14 +433,15 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if args.precision == "fp64" else "f32+f64",
+        "dtype": "f64" if precision == "fp64" else "f32+f64",
         "data": "synthetic",
         "config": {"workload": (f"{w['name']} {w['lengths'][0]:g}x{w['lengths'][1]:g}x{w['lengths'][2]:g} nm, "
                                 f"{w['cells'][0]}x{w['cells'][1]}x{w['cells'][2]} cells x 6 tets "
                                 f"({m.n_tets} tets, {m.n_nodes} nodes, {m.n_tris} surface triangles), "
                                 f"jitter {w['jitter']}, dt {w['dt']:g} s, tol {w['tol']:g}"),
-                   "mode": f"plain {args.precision}" if args.precision == "fp64" else "mixed",
-                   "initial_guess": "previous v (reference buildInitGuess)",
+                   "mode": mode_txt,
+                   "initial_guess": (f"learned ({guess_src})" if guess is not None else
+                                     "previous v (reference buildInitGuess)"),
                    "demag": args.demag, "fmm_order": args.fmm_order if args.demag == "fmm" else None,
                    "parallelism": "single GPU"},
         "bicgstab_iterations_per_step": (round(float(np.mean(res["iters"])), 2) if res["iters"] else None),
@@ -375,6 +451,11 @@ def main():
         "profiled_ms_per_step": round(res["prof_ms_per_step"], 4),
         "fmm_plan": res["fmm"],
     }
+    if plain:
+        out["plain_fp64"] = {"steps_per_s": round(1e3 / plain["ms"], 4), "ms_per_step": round(plain["ms"], 4),
+                             "bicgstab_iterations_per_step": round(float(np.mean(plain["iters"])), 2)}
+        out["speedup_vs_plain_fp64"] = round(plain["ms"] / res["ms_per_step"], 4)
+        out["max_step_dm_vs_plain_fp64"] = dm
     if not args.no_cpu:
         try:
             out["cpu_baseline"] = cpu_baseline(w, m, u0)

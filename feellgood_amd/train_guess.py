@@ -1,0 +1,115 @@
+"""Training of the learned initial guess on the GPU, from the steps the plain solver generates.
+
+For each seeded applied-field direction the state restarts from the workload's initial
+magnetisation; every solve of the plain solver (reference initial guess, or the network once it is
+enabled) leaves its features and its converged solution on the device, and `fg_guess_train` takes
+Adam steps on that pair. Held-out directions then measure BiCGSTAB iterations per step with and
+without the network, from the same states and basis angles.
+
+    python -m feellgood_amd.train_guess --workload film --directions 8 --steps 40 --out w.npy
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+MU0 = 1.25663706127e-6
+
+
+def field_direction(rng, bmax=0.05):
+    """Seeded applied field: in-plane angle U(0, 2 pi), small out-of-plane tilt, |B| U(0, bmax) T."""
+    th = rng.uniform(0, 2 * math.pi)
+    tilt = rng.uniform(-0.2, 0.2)
+    b = rng.uniform(0.0, bmax)
+    d = np.array([math.cos(th), math.sin(th), tilt])
+    return d / np.linalg.norm(d) * b / MU0
+
+
+def run_direction(g, u0, H, dt, pref, steps, rng, train=None, inner=4, lr=2e-3):
+    """Steps from u0 under H; returns the iteration counts and the training losses."""
+    from feellgood_amd import native as nv
+    g.set_state(nv.FG_BOTH, u=u0)
+    g.calc_demag()
+    g.evolution()
+    its, losses = [], []
+    for k in range(steps):
+        g.base_projection(rng.uniform(0, 2 / math.pi))
+        g.prepare_elements(H, dt, pref)
+        failed, info = g.solve(dt)
+        if failed:
+            raise RuntimeError(f"solver failure at step {k}")
+        its.append(info.iterations)
+        if train and k >= 1:
+            for _ in range(inner):
+                loss, ref = g.guess_train(lr)
+            losses.append(loss / ref)
+        g.calc_demag()
+        g.evolution()
+    return its, losses
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
+    ap.add_argument("--workload", default="film", choices=["film", "cube"])
+    ap.add_argument("--directions", type=int, default=8)
+    ap.add_argument("--held-out", type=int, default=2)
+    ap.add_argument("--steps", type=int, default=40)
+    ap.add_argument("--epochs", type=int, default=1)
+    ap.add_argument("--inner", type=int, default=4, help="Adam steps per solved step")
+    ap.add_argument("--lr", type=float, default=2e-3)
+    ap.add_argument("--seed", type=int, default=3)
+    ap.add_argument("--out", default=None)
+    ap.add_argument("--demag", default="fmm")
+    args = ap.parse_args(argv)
+
+    import bench
+    from feellgood_amd.engine import Engine, options
+    w = bench.film_workload() if args.workload == "film" else bench.cube_workload()
+    m = bench.build_mesh(w)
+    u0 = bench.initial_state(m, w)
+    dt = w["dt"]
+    pref = bench.prefactor(dt)
+    g = Engine(m.mesh, options(tol=w["tol"], max_iter=10000, demag=args.demag))
+    g.guess_init(seed=args.seed)
+    rng = np.random.default_rng(args.seed)
+    train_fields = [field_direction(rng) for _ in range(args.directions)]
+    test_fields = [field_direction(rng) for _ in range(args.held_out)]
+    t0 = time.perf_counter()
+    hist = []
+    for ep in range(args.epochs):
+        for d, H in enumerate(train_fields):
+            its, losses = run_direction(g, u0, H, dt, pref, args.steps, np.random.default_rng(1000 * ep + d),
+                                        train=True, inner=args.inner, lr=args.lr)
+            hist.append(float(np.mean(losses[-5:])))
+            print(f"[train] epoch {ep} direction {d}: mean loss/ref {np.mean(losses):.3f} "
+                  f"(last 5: {hist[-1]:.3f}), iterations {np.mean(its):.2f}", flush=True)
+    t_train = time.perf_counter() - t0
+    res = {}
+    for on in (False, True):
+        g.guess_enable(on)
+        tot = []
+        for d, H in enumerate(test_fields):
+            its, _ = run_direction(g, u0, H, dt, pref, args.steps, np.random.default_rng(777 + d))
+            tot += its
+        res["learned" if on else "reference"] = float(np.mean(tot))
+    g.guess_enable(False)
+    out = {"workload": args.workload, "train_directions": args.directions, "steps": args.steps,
+           "epochs": args.epochs, "train_seconds": round(t_train, 1), "final_loss_ratio": hist[-1],
+           "iterations_per_step_reference": res["reference"], "iterations_per_step_learned": res["learned"]}
+    print(json.dumps(out), flush=True)
+    if args.out:
+        np.save(args.out, g.guess_get())
+    g.close()
+    return out
+
+
+if __name__ == "__main__":
+    main()

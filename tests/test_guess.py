@@ -93,7 +93,7 @@ def cube(n=8, seed=4):
 
 def adjacency(m):
     a = m.arrays()
-    tn = a["tet_nodes"].reshape(-1, 4)
+    tn = a["tets"]
     n = m.n_nodes
     nb = [set([i]) for i in range(n)]
     for t in tn:
@@ -110,7 +110,7 @@ def adjacency(m):
 def lumped_mass(m):
     a = m.arrays()
     pos = a["pos"]
-    tn = a["tet_nodes"].reshape(-1, 4)
+    tn = a["tets"]
     d = pos[tn[:, 1:]] - pos[tn[:, :1]]
     vol = np.abs(np.linalg.det(d)) / 6
     mass = np.zeros(m.n_nodes)
