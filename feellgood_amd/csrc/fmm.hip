@@ -36,6 +36,7 @@ struct FmmPlan {
     std::vector<int32_t> ch_ptr, ch_idx;      // children with sources (M2M)
     std::vector<int32_t> il_ptr, il_src, il_off;  // interaction lists (target cells)
     std::vector<int32_t> leaf_sb, leaf_se, leaf_tb, leaf_te;  // ranges per leaf (index - lev_off[L])
+    std::vector<uint32_t> leaf_code;  // Morton code of each leaf at level L
     std::vector<int32_t> near_ptr, near_leaf;  // per leaf: source leaves adjacent (leaf indices)
     // Evaluation groups: the targets of one level-(L-1) cell. M2L/L2L run on levels 2..ls-1, the
     // group's targets evaluate the local expansion of its level-(L-2) ancestor (lcell) and the
@@ -285,6 +286,7 @@ std::string fmm_build_host(FmmPlan &P, const std::vector<double4> &src, const st
         }
     // leaves: ranges and near lists
     const int NL = P.lev_off[L + 1] - P.lev_off[L];
+    P.leaf_code.assign(codes[L].begin(), codes[L].end());
     P.leaf_sb.assign(NL, 0);
     P.leaf_se.assign(NL, 0);
     P.leaf_tb.assign(NL, 0);
@@ -830,39 +832,38 @@ __global__ __launch_bounds__(256) void k_nf_weights(int Tm, int Fm, const double
     }
 }
 
-// Near-field operator coefficients: one wavefront per target; for each adjacent leaf, lanes take
-// the leaf's element runs (consecutive Gauss points of one element) and sum weight / distance.
-// The columns are the same for all targets of a leaf and are stored once per leaf.
-__global__ __launch_bounds__(256) void k_nf_build(int T, const int32_t *__restrict__ tgt_leaf,
-                                                  const double4 *__restrict__ tpos, const int32_t *__restrict__ near_ptr,
-                                                  const int32_t *__restrict__ near_leaf,
-                                                  const int32_t *__restrict__ tp_ptr, const int32_t *__restrict__ tp_col,
-                                                  const int32_t *__restrict__ tp_src, const int32_t *__restrict__ tp_nk,
+// Near-field operator coefficients: one wavefront per target; lanes take the entries of the
+// target's leaf. An entry is one element (a tetrahedron, or one vertex of a triangle) with the mask
+// of its Gauss points that lie in leaves adjacent to the target's; its coefficient is
+// sum over those points of weight / distance. The columns are the same for all targets of a leaf
+// and are stored once per leaf.
+__global__ __launch_bounds__(256) void k_nf_build(int T, const int32_t *__restrict__ tgt_tl,
+                                                  const double4 *__restrict__ tpos,
+                                                  const int64_t *__restrict__ lc_ptr,
+                                                  const int32_t *__restrict__ lc_base,
+                                                  const int32_t *__restrict__ lc_nk, const int64_t *__restrict__ rank,
                                                   const double4 *__restrict__ spos, const double4 *__restrict__ wsrc,
                                                   const int64_t *__restrict__ nf_ptr, double *__restrict__ nf_val)
 {
     const int t = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const int lane = threadIdx.x & 63;
     if (t >= T) return;
-    const int li = tgt_leaf[t];
+    const int tl = tgt_tl[t];
     const double4 y = tpos[t];
-    int64_t pos = nf_ptr[t];
-    for (int e = near_ptr[li]; e < near_ptr[li + 1]; e++) {
-        const int sl = near_leaf[e];
-        const int b = tp_ptr[sl], n = tp_ptr[sl + 1] - b;
-        for (int j = lane; j < n; j += 64) {
-            const int s0 = tp_src[b + j], nk = tp_nk[b + j];
-            const int cnt = nk & 15, kind = nk >> 4;
-            double v = 0;
-            for (int s = s0; s < s0 + cnt; s++) {
-                const double4 x = spos[s], w = wsrc[s];
-                const double dx = y.x - x.x, dy = y.y - x.y, dz = y.z - x.z;
-                const double wk = kind <= 1 ? w.x : (kind == 2 ? w.y : w.z);
-                v += wk / sqrt(dx * dx + dy * dy + dz * dz);
-            }
-            nf_val[pos + j] = v;
+    const int64_t b = lc_ptr[tl], n = lc_ptr[tl + 1] - b, pos = nf_ptr[t];
+    for (int64_t j = lane; j < n; j += 64) {
+        const int base = lc_base[b + j], nk = lc_nk[b + j];
+        const int kind = nk >> 8;
+        double v = 0;
+        for (int g = 0; g < 5; g++) {
+            if (!((nk >> g) & 1)) continue;
+            const int64_t s = rank[(int64_t)base + g];
+            const double4 x = spos[s], w = wsrc[s];
+            const double dx = y.x - x.x, dy = y.y - x.y, dz = y.z - x.z;
+            const double wk = kind <= 1 ? w.x : (kind == 2 ? w.y : w.z);
+            v += wk / sqrt(dx * dx + dy * dy + dz * dz);
         }
-        pos += n;
+        nf_val[pos + j] = v;
     }
 }
 
@@ -1113,75 +1114,101 @@ int fmm_create(Ctx &c)
     }
 }
 
-// Near-field operator: per leaf, the runs of consecutive tree-order sources that belong to one
-// element (a tetrahedron, or a triangle expanded to its three vertices); per target, one entry per
-// run in its adjacent leaves, coefficient sum_g weight_g / |y - x_g| computed once on the device.
+// Near-field operator. For each element (a tetrahedron; a triangle expanded to its three vertices)
+// and each target leaf adjacent to a leaf holding one of its Gauss points, one entry with the mask
+// of the Gauss points in leaves adjacent to that target leaf: an element split over several leaves
+// gives one entry, not one per leaf. The entries are grouped by target leaf (shared columns); the
+// coefficients sum_g weight_g / |y - x_g| are computed once on the device.
 void fmm_build_near(Ctx &c, FmmPlan &P)
 {
     const int L = P.L, NL = P.lev_off[L + 1] - P.lev_off[L];
-    const int64_t S = c.S, T5 = (int64_t)5 * c.Tm;
-    std::vector<int64_t> orig(S);
-    for (int64_t k = 0; k < S; k++) orig[P.src_rank[k]] = k;
-    std::vector<int32_t> tp_ptr(NL + 1, 0), tp_col, tp_src, tp_nk;
-    tp_col.reserve(S / 3);
-    tp_src.reserve(S / 3);
-    tp_nk.reserve(S / 3);
-    for (int li = 0; li < NL; li++) {
-        for (int k = P.leaf_sb[li]; k < P.leaf_se[li];) {
-            const int64_t o = orig[k];
-            const bool tet = o < T5;
-            const int64_t el = tet ? o / 5 : (o - T5) / 4;
-            int e = k + 1;
-            while (e < P.leaf_se[li]) {
-                const int64_t o2 = orig[e];
-                if ((o2 < T5) != tet || (tet ? o2 / 5 : (o2 - T5) / 4) != el) break;
-                e++;
-            }
-            if (tet) {
-                tp_col.push_back((int32_t)el);
-                tp_src.push_back(k);
-                tp_nk.push_back(e - k);
-            } else {
-                for (int a = 0; a < 3; a++) {
-                    tp_col.push_back((int32_t)(c.Tm + 3 * el + a));
-                    tp_src.push_back(k);
-                    tp_nk.push_back((e - k) | ((1 + a) << 4));
-                }
-            }
-            k = e;
+    const int64_t S = c.S;
+    const int Tm = c.Tm, Fm = c.Fm;
+    std::vector<int32_t> leaf_of(S);
+    for (int li = 0; li < NL; li++)
+        for (int k = P.leaf_sb[li]; k < P.leaf_se[li]; k++) leaf_of[k] = li;
+    std::vector<int32_t> lx(NL), ly(NL), lz(NL);
+    for (int li = 0; li < NL; li++) unmorton(P.leaf_code[li], L, lx[li], ly[li], lz[li]);
+    // target leaves and, per source leaf, the target leaves adjacent to it
+    std::vector<int32_t> tl_of_leaf(NL, -1), tl_tb, tl_te;
+    for (int li = 0; li < NL; li++)
+        if (P.leaf_te[li] > P.leaf_tb[li]) {
+            tl_of_leaf[li] = (int32_t)tl_tb.size();
+            tl_tb.push_back(P.leaf_tb[li]);
+            tl_te.push_back(P.leaf_te[li]);
         }
-        tp_ptr[li + 1] = (int32_t)tp_col.size();
+    const int ntl = (int)tl_tb.size();
+    std::vector<int32_t> rn_ptr(NL + 1, 0), rn_idx;
+    for (int li = 0; li < NL; li++)
+        for (int e = P.near_ptr[li]; e < P.near_ptr[li + 1]; e++) rn_ptr[P.near_leaf[e] + 1]++;
+    for (int li = 0; li < NL; li++) rn_ptr[li + 1] += rn_ptr[li];
+    rn_idx.resize(rn_ptr[NL]);
+    {
+        std::vector<int32_t> fill(rn_ptr.begin(), rn_ptr.end() - 1);
+        for (int li = 0; li < NL; li++)
+            for (int e = P.near_ptr[li]; e < P.near_ptr[li + 1]; e++) rn_idx[fill[P.near_leaf[e]]++] = li;
     }
+    // entries: (target leaf, column, base source index, mask | kind << 8), two passes (count, fill)
+    const int64_t nelem = (int64_t)Tm + Fm;
+    auto visit = [&](auto &&emit) {
+        std::vector<int32_t> cand;
+        for (int64_t e = 0; e < nelem; e++) {
+            const bool tet = e < Tm;
+            const int ng = tet ? 5 : 4;
+            const int64_t base = tet ? 5 * e : 5 * (int64_t)Tm + 4 * (e - Tm);
+            int lg[5];
+            cand.clear();
+            for (int g = 0; g < ng; g++) {
+                lg[g] = leaf_of[P.src_rank[base + g]];
+                for (int q = rn_ptr[lg[g]]; q < rn_ptr[lg[g] + 1]; q++) cand.push_back(rn_idx[q]);
+            }
+            std::sort(cand.begin(), cand.end());
+            cand.erase(std::unique(cand.begin(), cand.end()), cand.end());
+            for (int tli : cand) {
+                int mask = 0;
+                for (int g = 0; g < ng; g++)
+                    if (std::abs(lx[tli] - lx[lg[g]]) <= 1 && std::abs(ly[tli] - ly[lg[g]]) <= 1 &&
+                        std::abs(lz[tli] - lz[lg[g]]) <= 1)
+                        mask |= 1 << g;
+                const int t = tl_of_leaf[tli];
+                if (tet) emit(t, (int32_t)e, (int32_t)base, mask);
+                else
+                    for (int a = 0; a < 3; a++)
+                        emit(t, (int32_t)(Tm + 3 * (e - Tm) + a), (int32_t)base, mask | ((1 + a) << 8));
+            }
+        }
+    };
+    std::vector<int64_t> lc_ptr(ntl + 1, 0);
+    visit([&](int t, int32_t, int32_t, int) { lc_ptr[t + 1]++; });
+    for (int t = 0; t < ntl; t++) lc_ptr[t + 1] += lc_ptr[t];
+    const int64_t ncol = lc_ptr[ntl];
+    std::vector<int32_t> lc_col(ncol), lc_base(ncol), lc_nk(ncol);
+    {
+        std::vector<int64_t> fill(lc_ptr.begin(), lc_ptr.end() - 1);
+        visit([&](int t, int32_t col, int32_t base, int nk) {
+            const int64_t q = fill[t]++;
+            lc_col[q] = col;
+            lc_base[q] = base;
+            lc_nk[q] = nk;
+        });
+    }
+    // coefficient rows: one per target, as long as its leaf's column list
     const int T = (int)P.tgt_node.size();
+    std::vector<int32_t> tgt_tl(T);
     std::vector<int64_t> nf_ptr(T + 1, 0);
     for (int t = 0; t < T; t++) {
-        const int li = P.tgt_leaf[t];
-        int64_t n = 0;
-        for (int e = P.near_ptr[li]; e < P.near_ptr[li + 1]; e++) n += tp_ptr[P.near_leaf[e] + 1] - tp_ptr[P.near_leaf[e]];
-        nf_ptr[t + 1] = nf_ptr[t] + n;
+        const int tl = tl_of_leaf[P.tgt_leaf[t]];
+        tgt_tl[t] = tl;
+        nf_ptr[t + 1] = nf_ptr[t] + (lc_ptr[tl + 1] - lc_ptr[tl]);
     }
     P.nf_nnz = nf_ptr[T];
-    // leaves with targets and their shared column lists
-    std::vector<int32_t> tl_tb, tl_te, lc_col;
-    std::vector<int64_t> lc_ptr(1, 0);
-    for (int li = 0; li < NL; li++) {
-        if (P.leaf_te[li] <= P.leaf_tb[li]) continue;
-        tl_tb.push_back(P.leaf_tb[li]);
-        tl_te.push_back(P.leaf_te[li]);
-        for (int e = P.near_ptr[li]; e < P.near_ptr[li + 1]; e++) {
-            const int sl = P.near_leaf[e];
-            lc_col.insert(lc_col.end(), tp_col.begin() + tp_ptr[sl], tp_col.begin() + tp_ptr[sl + 1]);
-        }
-        lc_ptr.push_back((int64_t)lc_col.size());
-    }
-    P.ntl = (int)tl_tb.size();
-    P.lc_cols = (int64_t)lc_col.size();
+    P.ntl = ntl;
+    P.lc_cols = ncol;
     P.d_tl_tb = dput(tl_tb);
     P.d_tl_te = dput(tl_te);
     P.d_lc_ptr = dput(lc_ptr);
     P.d_lc_col = dput(lc_col);
-    int32_t *d_tp_ptr = dput(tp_ptr), *d_tp_col = dput(tp_col), *d_tp_src = dput(tp_src), *d_tp_nk = dput(tp_nk);
-    int32_t *d_near_ptr = dput(P.near_ptr), *d_near_leaf = dput(P.near_leaf), *d_tgt_leaf = dput(P.tgt_leaf);
+    int32_t *d_lc_base = dput(lc_base), *d_lc_nk = dput(lc_nk), *d_tgt_tl = dput(tgt_tl);
     P.d_nf_ptr = dput(nf_ptr);
     void *d = nullptr;
     if (hipMalloc(&d, std::max<int64_t>(1, P.nf_nnz) * sizeof(double)) != hipSuccess) throw std::string("hipMalloc near-field values");
@@ -1190,20 +1217,20 @@ void fmm_build_near(Ctx &c, FmmPlan &P)
     double4 *wsrc = (double4 *)d;
     if (hipMalloc(&d, sizeof(double2) * std::max(1, T)) != hipSuccess) throw std::string("hipMalloc pot");
     P.d_pot = (double2 *)d;
-    if (hipMalloc(&d, sizeof(double2) * ((size_t)c.Tm + 3 * (size_t)c.Fm)) != hipSuccess) throw std::string("hipMalloc qe");
+    if (hipMalloc(&d, sizeof(double2) * ((size_t)Tm + 3 * (size_t)Fm)) != hipSuccess) throw std::string("hipMalloc qe");
     c.qe = (double2 *)d;
-    k_nf_weights<<<(unsigned)((S + 255) / 256), 256, 0, c.stream>>>(c.Tm, c.Fm, c.geo, c.tri_geo, c.src_rank, wsrc);
+    k_nf_weights<<<(unsigned)((S + 255) / 256), 256, 0, c.stream>>>(Tm, Fm, c.geo, c.tri_geo, c.src_rank, wsrc);
     if (T > 0)
         k_nf_build<<<(unsigned)(((int64_t)T * 64 + 255) / 256), 256, 0, c.stream>>>(
-            T, d_tgt_leaf, P.d_tgt_pos, d_near_ptr, d_near_leaf, d_tp_ptr, d_tp_col, d_tp_src, d_tp_nk, c.src_pos,
-            wsrc, P.d_nf_ptr, P.d_nf_val);
+            T, d_tgt_tl, P.d_tgt_pos, P.d_lc_ptr, d_lc_base, d_lc_nk, c.src_rank, c.src_pos, wsrc, P.d_nf_ptr,
+            P.d_nf_val);
     if (P.mixed && P.nf_nnz > 0) {
         if (hipMalloc(&d, P.nf_nnz * sizeof(float)) != hipSuccess) throw std::string("hipMalloc near-field values");
         P.d_nf_valf = (float *)d;
         k_to_float<<<4096, 256, 0, c.stream>>>(P.nf_nnz, P.d_nf_val, P.d_nf_valf);
     }
     const hipError_t err = hipStreamSynchronize(c.stream);
-    void *tmp[] = {d_tp_ptr, d_tp_col, d_tp_src, d_tp_nk, d_near_ptr, d_near_leaf, d_tgt_leaf, wsrc};
+    void *tmp[] = {d_lc_base, d_lc_nk, d_tgt_tl, wsrc};
     for (void *p : tmp) (void)hipFree(p);
     if (err != hipSuccess) throw std::string("near-field build: ") + hipGetErrorString(err);
     if (P.d_nf_valf) {
