@@ -95,7 +95,7 @@ void build_plan(Ctx &c, const fg_mesh *m)
     std::vector<int32_t> mtets;  // 4 per magnetic tet
     std::vector<int32_t> mreg;
     std::vector<double> da_w;    // 17 per magnetic tet, AoS during construction
-    std::vector<char> magn(N, 0);
+    std::vector<char> magn(N, 0), tet_is_mag(m->n_tets, 0), tri_is_mag(std::max(0, m->n_tris), 0);
     std::vector<std::pair<int, int>> edges;
     edges.reserve((size_t)6 * m->n_tets);
     for (int t = 0; t < m->n_tets; t++) {
@@ -139,6 +139,7 @@ void build_plan(Ctx &c, const fg_mesh *m)
         }
         for (int p = 0; p < 5; p++) g[12 + p] = detJ * TET_PDS[p];
         mtets.insert(mtets.end(), q, q + 4);
+        tet_is_mag[t] = 1;
         mreg.push_back(rg);
         da_w.insert(da_w.end(), g, g + 17);
         for (int a = 0; a < 4; a++) magn[q[a]] = 1;
@@ -243,11 +244,31 @@ void build_plan(Ctx &c, const fg_mesh *m)
         const bool supp = (m->n_surf_regions > 0 && rg >= 0 && rg < m->n_surf_regions) && m->surf[rg].suppress_charges;
         if (mag && !supp) {
             magtri.push_back(f);
+            tri_is_mag[f] = 1;
             if (m->n_surf_regions > 0 && rg >= 0 && rg < m->n_surf_regions && m->surf[rg].Ks != 0) kstri.push_back(f);
         }
     }
     c.Fm = (int)magtri.size();
     c.Fk = (int)kstri.size();
+    // instance ranges: the input ranges (one instance unless a batch) mapped to magnetic elements
+    if (c.in_node_off.empty()) {
+        c.in_node_off = {0, N};
+        c.in_tet_off = {0, m->n_tets};
+        c.in_tri_off = {0, std::max(0, m->n_tris)};
+    }
+    {
+        const size_t K = c.in_node_off.size() - 1;
+        c.inst_node_off = c.in_node_off;
+        c.inst_mtet_off.assign(K + 1, 0);
+        c.inst_mtri_off.assign(K + 1, 0);
+        for (size_t k = 0; k < K; k++) {
+            int64_t nt = 0, nf = 0;
+            for (int64_t t = c.in_tet_off[k]; t < c.in_tet_off[k + 1]; t++) nt += tet_is_mag[t];
+            for (int64_t f = c.in_tri_off[k]; f < c.in_tri_off[k + 1]; f++) nf += tri_is_mag[f];
+            c.inst_mtet_off[k + 1] = c.inst_mtet_off[k] + nt;
+            c.inst_mtri_off[k + 1] = c.inst_mtri_off[k] + nf;
+        }
+    }
     for (int f : kstri)
         for (int a = 0; a < 3; a++) incp[m->tri_nodes[3 * f + a] + 1]++;
     for (int i = 0; i < N; i++) incp[i + 1] += incp[i];
@@ -436,6 +457,11 @@ void build_plan(Ctx &c, const fg_mesh *m)
     for (int64_t k = 0; k < c.S; k++) rank[k] = k;
     c.src_rank = dalloc<int64_t>(c.S);
     upload(c.src_rank, rank, st);
+    if (c.inst_node_off.size() > 2) {
+        c.d_inst_node_off = dalloc<int64_t>(c.inst_node_off.size());
+        upload(c.d_inst_node_off, c.inst_node_off, st);
+        c.inst_bn2 = dalloc<double>(c.inst_node_off.size());
+    }
     HIPCHK(hipStreamSynchronize(st));
 }
 
@@ -445,7 +471,7 @@ void free_all(Ctx &c)
                     c.rt, c.p, c.ph, c.v, c.s, c.sh, c.t, c.w, c.ctl, c.part, c.epart, c.res, c.slice_ptr, c.slice_w,
                     c.scol, c.sval, c.tet_nodes, c.geo, c.tet_reg, c.regions, c.elem_out, c.inc_ptr, c.inc_idx,
                     c.trik_nodes, c.trik_geo, c.tri_nodes, c.tri_geo, c.tri_corr, c.cinc_ptr, c.cinc_idx, c.corr,
-                    c.src_pos, c.src_q, c.src_pos_f, c.src_rank};
+                    c.src_pos, c.src_q, c.src_pos_f, c.src_rank, c.d_inst_node_off, c.inst_bn2};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     if (c.h_ctl) (void)hipHostFree(c.h_ctl);
@@ -539,7 +565,10 @@ void fg_default_options(fg_options *o)
     o->profile = 0;
 }
 
-int fg_create(const fg_mesh *mesh, const fg_options *opt, fg_ctx **out)
+namespace {
+
+int create_impl(const fg_mesh *mesh, const fg_options *opt, fg_ctx **out, const std::vector<int64_t> *node_off,
+                const std::vector<int64_t> *tet_off, const std::vector<int64_t> *tri_off)
 {
     if (!mesh || !out || mesh->n_nodes <= 0 || mesh->n_tets <= 0 || !mesh->pos || !mesh->tet_nodes ||
         !mesh->tet_region || !mesh->vol || mesh->n_vol_regions <= 0 || (mesh->n_tris > 0 && (!mesh->tri_nodes || !mesh->tri_region || !mesh->tri_dMs))) {
@@ -555,6 +584,11 @@ int fg_create(const fg_mesh *mesh, const fg_options *opt, fg_ctx **out)
         delete x;
         return FG_ERR_ARG;
     }
+    if (node_off) {
+        c.in_node_off = *node_off;
+        c.in_tet_off = *tet_off;
+        c.in_tri_off = *tri_off;
+    }
     c.device = c.opt.device;
     try {
         HIPCHK(hipSetDevice(c.device));
@@ -568,6 +602,7 @@ int fg_create(const fg_mesh *mesh, const fg_options *opt, fg_ctx **out)
         g_create_err = e;
         if (c.stream) {
             (void)hipStreamSynchronize(c.stream);
+            fmm_destroy(c);
             free_all(c);
             (void)hipStreamDestroy(c.stream);
         }
@@ -576,6 +611,85 @@ int fg_create(const fg_mesh *mesh, const fg_options *opt, fg_ctx **out)
                                                                                                : FG_ERR_DEVICE;
     }
     *out = x;
+    return FG_OK;
+}
+
+}  // namespace
+
+int fg_create(const fg_mesh *mesh, const fg_options *opt, fg_ctx **out)
+{
+    return create_impl(mesh, opt, out, nullptr, nullptr, nullptr);
+}
+
+int fg_create_batch(const fg_mesh *meshes, int k, const fg_options *opt, fg_ctx **out)
+{
+    if (!meshes || k <= 0 || !out) {
+        g_create_err = "fg_create_batch: invalid arguments";
+        return FG_ERR_ARG;
+    }
+    // the union of the instances: nodes, elements and regions side by side
+    std::vector<double> pos, dMs;
+    std::vector<int32_t> tets, treg, tris, freg;
+    std::vector<fg_volume_region> vol;
+    std::vector<fg_surface_region> surf;
+    std::vector<int64_t> node_off(1, 0), tet_off(1, 0), tri_off(1, 0);
+    int64_t surf_total = 0;
+    for (int q = 0; q < k; q++) surf_total += std::max(0, meshes[q].n_surf_regions);
+    for (int q = 0; q < k; q++) {
+        const fg_mesh &m = meshes[q];
+        if (m.n_nodes <= 0 || m.n_tets <= 0 || !m.pos || !m.tet_nodes || !m.tet_region || !m.vol ||
+            m.n_vol_regions <= 0 || (m.n_tris > 0 && (!m.tri_nodes || !m.tri_region || !m.tri_dMs))) {
+            g_create_err = "fg_create_batch: invalid mesh " + std::to_string(q);
+            return FG_ERR_ARG;
+        }
+        const int32_t no = (int32_t)node_off.back(), vo = (int32_t)vol.size(), so = (int32_t)surf.size();
+        pos.insert(pos.end(), m.pos, m.pos + 3 * (size_t)m.n_nodes);
+        for (int64_t t = 0; t < 4 * (int64_t)m.n_tets; t++) tets.push_back(m.tet_nodes[t] + no);
+        for (int t = 0; t < m.n_tets; t++)
+            treg.push_back(m.tet_region[t] >= 0 && m.tet_region[t] < m.n_vol_regions ? m.tet_region[t] + vo : -1);
+        for (int64_t f = 0; f < 3 * (int64_t)std::max(0, m.n_tris); f++) tris.push_back(m.tri_nodes[f] + no);
+        for (int f = 0; f < m.n_tris; f++) {
+            const int r = m.tri_region[f];
+            // regions beyond the instance's list (created triangles) keep the default parameters
+            freg.push_back(r >= 0 && r < m.n_surf_regions ? r + so : (int32_t)surf_total);
+            dMs.push_back(m.tri_dMs[f]);
+        }
+        vol.insert(vol.end(), m.vol, m.vol + m.n_vol_regions);
+        if (m.n_surf_regions > 0) surf.insert(surf.end(), m.surf, m.surf + m.n_surf_regions);
+        node_off.push_back(node_off.back() + m.n_nodes);
+        tet_off.push_back(tet_off.back() + m.n_tets);
+        tri_off.push_back(tri_off.back() + std::max(0, m.n_tris));
+    }
+    if (node_off.back() > INT32_MAX || tet_off.back() > INT32_MAX || tri_off.back() > INT32_MAX) {
+        g_create_err = "fg_create_batch: the union exceeds int32 indices";
+        return FG_ERR_RANGE;
+    }
+    fg_mesh u{};
+    u.n_nodes = (int32_t)node_off.back();
+    u.n_tets = (int32_t)tet_off.back();
+    u.n_tris = (int32_t)tri_off.back();
+    u.n_vol_regions = (int32_t)vol.size();
+    u.n_surf_regions = (int32_t)surf.size();
+    u.pos = pos.data();
+    u.tet_nodes = tets.data();
+    u.tet_region = treg.data();
+    u.tri_nodes = tris.empty() ? nullptr : tris.data();
+    u.tri_region = freg.empty() ? nullptr : freg.data();
+    u.tri_dMs = dMs.empty() ? nullptr : dMs.data();
+    u.vol = vol.data();
+    u.surf = surf.empty() ? nullptr : surf.data();
+    return create_impl(&u, opt, out, &node_off, &tet_off, &tri_off);
+}
+
+int fg_batch_layout(fg_ctx *x, int64_t *node_off, int64_t *mtet_off, int *k)
+{
+    if (!x || !k) return FG_ERR_ARG;
+    const Ctx &c = x->c;
+    *k = (int)c.inst_node_off.size() - 1;
+    for (int q = 0; q <= *k; q++) {
+        if (node_off) node_off[q] = c.inst_node_off[q];
+        if (mtet_off) mtet_off[q] = c.inst_mtet_off[q];
+    }
     return FG_OK;
 }
 

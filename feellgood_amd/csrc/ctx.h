@@ -39,6 +39,12 @@ struct Ctx {
 
     // host copies kept for introspection
     std::vector<int32_t> h_mag;          // [N]
+    // independent instances held side by side (one unless the context is a batch): node,
+    // magnetic-tetrahedron and charged-triangle ranges, size K + 1
+    std::vector<int64_t> inst_node_off, inst_mtet_off, inst_mtri_off;
+    std::vector<int64_t> in_node_off, in_tet_off, in_tri_off;  // the same ranges in the input mesh
+    int64_t *d_inst_node_off = nullptr;  // batch only: node ranges on the device
+    double *inst_bn2 = nullptr;          // batch only: ||b_k||^2 per instance
     double charge_unit = 1.0;            // typical |charge| (Ms_max w^(2/3)), scales the FP32 FMM pass
     std::vector<int64_t> h_rowptr;       // stencil CSR (unpadded), for fg_get_system
     std::vector<int32_t> h_col;
@@ -97,7 +103,6 @@ struct Ctx {
                                  // -Ms div u per tetrahedron, u_a . n per triangle vertex (u, v)
     FmmPlan *fmm = nullptr;
     GuessNet *guess = nullptr;   // learned initial guess (allocated on first use)
-    double *fmm_W = nullptr;     // cell width per tree level
 
     // energies / reductions
     double *epart = nullptr;  // [8][MAX_PART]

@@ -36,7 +36,6 @@ struct FmmPlan {
     std::vector<int32_t> ch_ptr, ch_idx;      // children with sources (M2M)
     std::vector<int32_t> il_ptr, il_src, il_off;  // interaction lists (target cells)
     std::vector<int32_t> leaf_sb, leaf_se, leaf_tb, leaf_te;  // ranges per leaf (index - lev_off[L])
-    std::vector<uint32_t> leaf_code;  // Morton code of each leaf at level L
     std::vector<int32_t> near_ptr, near_leaf;  // per leaf: source leaves adjacent (leaf indices)
     // Evaluation groups: the targets of one level-(L-1) cell. M2L/L2L run on levels 2..ls-1, the
     // group's targets evaluate the local expansion of its level-(L-2) ancestor (lcell) and the
@@ -84,7 +83,13 @@ struct FmmPlan {
     double2 *d_M = nullptr, *d_L = nullptr;  // [ncell][2][nc]
     std::vector<int32_t> src_cells_lev_off, tgt_cells_lev_off;  // offsets into the lists
     std::vector<int32_t> src_cells, tgt_cells;
-    int32_t *d_src_cells = nullptr, *d_tgt_cells = nullptr;
+    // Launch schedule in global cell ids (a device plan may hold several independent trees, one
+    // per instance, side by side): leaves, M2M parents per level, M2L cells, L2L cells per level.
+    int Lmax = 2;
+    std::vector<int32_t> leaf_cell, leaf_ix, leaf_iy, leaf_iz;
+    std::vector<double> cell_iw;  // 1 / cell width
+    std::vector<int32_t> m2m_lev_off, m2m_cells, m2l_cells, l2l_lev_off, l2l_cells;
+    int32_t *d_leaf_cell = nullptr, *d_m2m_cells = nullptr, *d_m2l_cells = nullptr, *d_l2l_cells = nullptr;
     int ncell() const { return (int)cen.size(); }
 };
 
@@ -286,7 +291,6 @@ std::string fmm_build_host(FmmPlan &P, const std::vector<double4> &src, const st
         }
     // leaves: ranges and near lists
     const int NL = P.lev_off[L + 1] - P.lev_off[L];
-    P.leaf_code.assign(codes[L].begin(), codes[L].end());
     P.leaf_sb.assign(NL, 0);
     P.leaf_se.assign(NL, 0);
     P.leaf_tb.assign(NL, 0);
@@ -395,6 +399,29 @@ std::string fmm_build_host(FmmPlan &P, const std::vector<double4> &src, const st
             P.u_ptr.push_back((int32_t)P.u_cell.size());
             k = e;
         }
+    }
+    // launch schedule of this tree
+    P.Lmax = L;
+    P.leaf_cell.resize(NL);
+    P.leaf_ix.resize(NL);
+    P.leaf_iy.resize(NL);
+    P.leaf_iz.resize(NL);
+    for (int li = 0; li < NL; li++) {
+        P.leaf_cell[li] = P.lev_off[L] + li;
+        unmorton(codes[L][li], L, P.leaf_ix[li], P.leaf_iy[li], P.leaf_iz[li]);
+    }
+    P.cell_iw.resize(NC);
+    for (int ci = 0; ci < NC; ci++) P.cell_iw[ci] = 1.0 / P.W[(int)P.cen[ci].w];
+    P.m2m_lev_off.assign(L + 2, 0);
+    P.l2l_lev_off.assign(L + 2, 0);
+    for (int l = 0; l <= L; l++) {
+        for (int ci = P.lev_off[l]; ci < P.lev_off[l + 1]; ci++) {
+            if (l >= 2 && l < L && P.has_src[ci]) P.m2m_cells.push_back(ci);
+            if (l >= 3 && l < P.ls && P.has_tgt[ci]) P.l2l_cells.push_back(ci);
+            if (l >= 2 && l < P.ls && P.has_tgt[ci]) P.m2l_cells.push_back(ci);
+        }
+        P.m2m_lev_off[l + 1] = (int32_t)P.m2m_cells.size();
+        P.l2l_lev_off[l + 1] = (int32_t)P.l2l_cells.size();
     }
     // tables
     P.It.assign((size_t)343 * P.ni, make_double2(0, 0));
@@ -526,6 +553,95 @@ void fmm_eval_host(const FmmPlan &P, const double *qu, const double *qv, std::ve
     }
 }
 
+
+// Several independent trees side by side in one device plan: cells, leaves, sources, targets,
+// groups and lists are concatenated with offsets; src_glob[k][s] is the global original index of
+// part k's source s (the charge kernels address sources by that index).
+void fmm_merge(std::vector<FmmPlan> &parts, const std::vector<std::vector<int64_t>> &src_glob, int64_t S_total,
+               FmmPlan &M)
+{
+    const FmmPlan &P0 = parts[0];
+    M.p = P0.p;
+    M.nc = P0.nc;
+    M.ni = P0.ni;
+    M.It = P0.It;
+    M.Rc = P0.Rc;
+    M.Lmax = 0;
+    for (auto &P : parts) M.Lmax = std::max(M.Lmax, P.Lmax);
+    M.src_rank.assign(S_total, -1);
+    M.m2m_lev_off.assign(M.Lmax + 2, 0);
+    M.l2l_lev_off.assign(M.Lmax + 2, 0);
+    M.ch_ptr.assign(1, 0);
+    M.il_ptr.assign(1, 0);
+    M.near_ptr.assign(1, 0);
+    M.u_ptr.assign(1, 0);
+    std::vector<std::vector<int32_t>> m2m_lev(M.Lmax + 1), l2l_lev(M.Lmax + 1);
+    for (size_t k = 0; k < parts.size(); k++) {
+        const FmmPlan &P = parts[k];
+        const int32_t co = (int32_t)M.cen.size(), lo = (int32_t)M.leaf_cell.size(), so = (int32_t)M.src_sorted.size(),
+                      to = (int32_t)M.tgt_node.size(), go = (int32_t)M.grp_tb.size();
+        const int NCk = P.ncell();
+        for (int c = 0; c < NCk; c++) {
+            M.cen.push_back(P.cen[c]);
+            M.cell_iw.push_back(P.cell_iw[c]);
+            M.parent.push_back(P.parent[c] >= 0 ? P.parent[c] + co : -1);
+            M.oct.push_back(P.oct[c]);
+            M.has_src.push_back(P.has_src[c]);
+            M.has_tgt.push_back(P.has_tgt[c]);
+            for (int e = P.ch_ptr[c]; e < P.ch_ptr[c + 1]; e++) M.ch_idx.push_back(P.ch_idx[e] + co);
+            M.ch_ptr.push_back((int32_t)M.ch_idx.size());
+            for (int e = P.il_ptr[c]; e < P.il_ptr[c + 1]; e++) {
+                M.il_src.push_back(P.il_src[e] + co);
+                M.il_off.push_back(P.il_off[e]);
+            }
+            M.il_ptr.push_back((int32_t)M.il_src.size());
+        }
+        const int NLk = (int)P.leaf_cell.size();
+        for (int li = 0; li < NLk; li++) {
+            M.leaf_cell.push_back(P.leaf_cell[li] + co);
+            M.leaf_ix.push_back(P.leaf_ix[li]);
+            M.leaf_iy.push_back(P.leaf_iy[li]);
+            M.leaf_iz.push_back(P.leaf_iz[li]);
+            M.leaf_sb.push_back(P.leaf_sb[li] + so);
+            M.leaf_se.push_back(P.leaf_se[li] + so);
+            M.leaf_tb.push_back(P.leaf_tb[li] + to);
+            M.leaf_te.push_back(P.leaf_te[li] + to);
+            for (int e = P.near_ptr[li]; e < P.near_ptr[li + 1]; e++) M.near_leaf.push_back(P.near_leaf[e] + lo);
+            M.near_ptr.push_back((int32_t)M.near_leaf.size());
+        }
+        for (size_t s = 0; s < P.src_sorted.size(); s++) M.src_sorted.push_back(P.src_sorted[s]);
+        for (size_t s = 0; s < P.src_rank.size(); s++) M.src_rank[src_glob[k][s]] = so + P.src_rank[s];
+        for (size_t t = 0; t < P.tgt_node.size(); t++) {
+            M.tgt_node.push_back(P.tgt_node[t]);
+            M.tgt_pos.push_back(P.tgt_pos[t]);
+            M.tgt_leaf.push_back(P.tgt_leaf[t] + lo);
+            M.tgt_oct.push_back(P.tgt_oct[t]);
+            M.tgt_grp.push_back(P.tgt_grp[t] + go);
+        }
+        for (size_t g = 0; g < P.grp_tb.size(); g++) {
+            M.grp_tb.push_back(P.grp_tb[g] + to);
+            M.grp_te.push_back(P.grp_te[g] + to);
+            M.grp_lcell.push_back(P.grp_lcell[g] >= 0 ? P.grp_lcell[g] + co : -1);
+            for (int e = P.u_ptr[g]; e < P.u_ptr[g + 1]; e++) {
+                M.u_cell.push_back(P.u_cell[e] + co);
+                M.u_mask.push_back(P.u_mask[e]);
+            }
+            M.u_ptr.push_back((int32_t)M.u_cell.size());
+        }
+        for (int l = 0; l <= P.Lmax; l++) {
+            for (int q = P.m2m_lev_off[l]; q < P.m2m_lev_off[l + 1]; q++) m2m_lev[l].push_back(P.m2m_cells[q] + co);
+            for (int q = P.l2l_lev_off[l]; q < P.l2l_lev_off[l + 1]; q++) l2l_lev[l].push_back(P.l2l_cells[q] + co);
+        }
+        for (int32_t c : P.m2l_cells) M.m2l_cells.push_back(c + co);
+    }
+    for (int l = 0; l <= M.Lmax; l++) {
+        M.m2m_cells.insert(M.m2m_cells.end(), m2m_lev[l].begin(), m2m_lev[l].end());
+        M.m2m_lev_off[l + 1] = (int32_t)M.m2m_cells.size();
+        M.l2l_cells.insert(M.l2l_cells.end(), l2l_lev[l].begin(), l2l_lev[l].end());
+        M.l2l_lev_off[l + 1] = (int32_t)M.l2l_cells.size();
+    }
+}
+
 }  // namespace fg
 
 // =====================================================================================  device
@@ -555,10 +671,10 @@ __device__ inline int xcd_block(int b, int nb)
 // (one broadcast load per source); lane m forms R_m^m = (-1/2)^m / m! (x + iy)^m and runs the
 // column recurrence with its own divisors, computed once.
 template <int P>
-__global__ __launch_bounds__(64) void k_fmm_p2m(int nleaf, int cell0, double iw, const double4 *__restrict__ cen,
-                                                const int32_t *__restrict__ sb, const int32_t *__restrict__ se,
-                                                const double4 *__restrict__ spos, const double2 *__restrict__ q,
-                                                double2 *__restrict__ M)
+__global__ __launch_bounds__(64) void k_fmm_p2m(int nleaf, const int32_t *__restrict__ leaf_cell,
+                                                const double4 *__restrict__ cen, const int32_t *__restrict__ sb,
+                                                const int32_t *__restrict__ se, const double4 *__restrict__ spos,
+                                                const double2 *__restrict__ q, double2 *__restrict__ M)
 {
     constexpr int NCo = fcount(P), LPW = 64 / (P + 1);
     const int lane = threadIdx.x;
@@ -580,8 +696,10 @@ __global__ __launch_bounds__(64) void k_fmm_p2m(int nleaf, int cell0, double iw,
     double ur[P + 1], ui[P + 1], vr[P + 1], vi[P + 1];
 #pragma unroll
     for (int k = 0; k <= P; k++) ur[k] = ui[k] = vr[k] = vi[k] = 0.0;
+    const int cell = act ? leaf_cell[li] : 0;
     if (act) {
-        const double4 c = cen[cell0 + li];
+        const double4 c = cen[cell];
+        const double iw = c.w;
         const int s0 = sb[li], s1 = se[li];
         for (int s = s0; s < s1; s += 2) {
             // two sources per pass: independent recurrences interleaved (a missing second source
@@ -642,7 +760,7 @@ __global__ __launch_bounds__(64) void k_fmm_p2m(int nleaf, int cell0, double iw,
                     r1i[h] = ci;
                 }
         }
-        double2 *Mu = M + (size_t)(cell0 + li) * 2 * NCo, *Mv = Mu + NCo;
+        double2 *Mu = M + (size_t)cell * 2 * NCo, *Mv = Mu + NCo;
 #pragma unroll
         for (int k = 0; k <= P; k++)
             if (m + k <= P) {
@@ -727,7 +845,7 @@ __global__ __launch_bounds__(256) void k_fmm_m2m(const int32_t *__restrict__ cel
 // both charge sets; the groups' sums are combined through LDS.
 template <int P>
 __global__ __launch_bounds__(256) void k_fmm_m2l(const int32_t *__restrict__ cells, const double4 *__restrict__ cen,
-                                                 const double *__restrict__ Wlev, const int32_t *__restrict__ il_ptr,
+                                                 const int32_t *__restrict__ il_ptr,
                                                  const int32_t *__restrict__ il_src, const int32_t *__restrict__ il_off,
                                                  const double2 *__restrict__ Itf, const double2 *__restrict__ M,
                                                  double2 *__restrict__ Lx)
@@ -777,7 +895,7 @@ __global__ __launch_bounds__(256) void k_fmm_m2l(const int32_t *__restrict__ cel
             a.z += b.z;
             a.w += b.w;
         }
-        const double f = (((j + k) & 1) ? -1.0 : 1.0) / Wlev[(int)cen[ci].w];
+        const double f = (((j + k) & 1) ? -1.0 : 1.0) * cen[ci].w;
         Lx[(size_t)ci * 2 * NCo + q] = make_double2(a.x * f, a.y * f);
         Lx[((size_t)ci * 2 + 1) * NCo + q] = make_double2(a.z * f, a.w * f);
     }
@@ -923,7 +1041,7 @@ __global__ __launch_bounds__(128) void k_fmm_m2p(const int32_t *__restrict__ grp
                                                  const int32_t *__restrict__ u_ptr, const int32_t *__restrict__ u_cell,
                                                  const uint8_t *__restrict__ u_mask,
                                                  const uint8_t *__restrict__ tgt_oct, const double4 *__restrict__ tpos,
-                                                 const double4 *__restrict__ cen, const double *__restrict__ Wlev,
+                                                 const double4 *__restrict__ cen,
                                                  const double2 *__restrict__ M, const double2 *__restrict__ Lx,
                                                  const double2 *__restrict__ pot, const int32_t *__restrict__ tgt_node,
                                                  NodeRec *__restrict__ nxt, double mscale)
@@ -963,7 +1081,7 @@ __global__ __launch_bounds__(128) void k_fmm_m2p(const int32_t *__restrict__ grp
             }
             if (tid < nch) {
                 const double4 c = cen[u_cell[cb + tid]];
-                sC[tid] = make_double4(c.x, c.y, c.z, 1.0 / Wlev[(int)c.w]);
+                sC[tid] = c;
                 sMask[tid] = u_mask[cb + tid];
             }
             __syncthreads();
@@ -1003,7 +1121,7 @@ __global__ __launch_bounds__(128) void k_fmm_m2p(const int32_t *__restrict__ grp
             const int t = t0 + tid;
             if (lc >= 0) {
                 const double4 yy = tpos[t], c = cen[lc];
-                const double iw = 1.0 / Wlev[(int)c.w];
+                const double iw = c.w;
                 l2p_eval2_t<P>(Lx + (size_t)lc * CW, Lx + (size_t)lc * CW + NCo, (yy.x - c.x) * iw, (yy.y - c.y) * iw,
                                (yy.z - c.z) * iw, su, sv);
             }
@@ -1025,27 +1143,50 @@ int fmm_create(Ctx &c)
         std::vector<double4> src(c.S);
         if (hipMemcpy(src.data(), c.src_pos, sizeof(double4) * c.S, hipMemcpyDeviceToHost) != hipSuccess)
             throw std::string("hipMemcpy src");
-        std::vector<int32_t> tnode;
-        std::vector<double4> tpos;
         std::vector<double4> p4(c.N);
         if (hipMemcpy(p4.data(), c.pos4, sizeof(double4) * c.N, hipMemcpyDeviceToHost) != hipSuccess)
             throw std::string("hipMemcpy pos");
-        for (int i = 0; i < c.N; i++)
-            if (c.h_mag[i]) {
-                tnode.push_back(i);
-                tpos.push_back(p4[i]);
-            }
+        const int leaf = c.opt.fmm_leaf > 0 ? c.opt.fmm_leaf : 64;
+        // one tree per instance (one instance unless the context is a batch), merged side by side
+        const int K = (int)c.inst_node_off.size() - 1;
+        std::vector<FmmPlan> parts(K);
+        std::vector<std::vector<int64_t>> glob(K);
+        for (int k = 0; k < K; k++) {
+            std::vector<double4> s;
+            std::vector<int64_t> &gl = glob[k];
+            for (int64_t t = c.inst_mtet_off[k]; t < c.inst_mtet_off[k + 1]; t++)
+                for (int g = 0; g < 5; g++) gl.push_back(5 * t + g);
+            for (int64_t q = c.inst_mtri_off[k]; q < c.inst_mtri_off[k + 1]; q++)
+                for (int g = 0; g < 4; g++) gl.push_back(5 * (int64_t)c.Tm + 4 * q + g);
+            s.reserve(gl.size());
+            for (int64_t o : gl) s.push_back(src[o]);
+            std::vector<int32_t> tnode;
+            std::vector<double4> tpos;
+            for (int64_t i = c.inst_node_off[k]; i < c.inst_node_off[k + 1]; i++)
+                if (c.h_mag[i]) {
+                    tnode.push_back((int32_t)i);
+                    tpos.push_back(p4[i]);
+                }
+            std::string e = fmm_build_host(parts[k], s, tnode, tpos, c.opt.fmm_order, leaf);
+            if (!e.empty()) throw e + " (instance " + std::to_string(k) + ")";
+        }
         auto *P = new FmmPlan();
         c.fmm = P;
-        const int leaf = c.opt.fmm_leaf > 0 ? c.opt.fmm_leaf : 64;
-        std::string e = fmm_build_host(*P, src, tnode, tpos, c.opt.fmm_order, leaf);
-        if (!e.empty()) throw e;
+        fmm_merge(parts, glob, c.S, *P);
+        parts.clear();
+        for (int64_t r : P->src_rank)
+            if (r < 0) throw std::string("FMM plan: a source belongs to no instance");
         // sources in tree order; the charge kernels scatter through src_rank
         if (hipMemcpy(c.src_pos, P->src_sorted.data(), sizeof(double4) * c.S, hipMemcpyHostToDevice) != hipSuccess)
             throw std::string("hipMemcpy sorted sources");
         if (hipMemcpy(c.src_rank, P->src_rank.data(), sizeof(int64_t) * c.S, hipMemcpyHostToDevice) != hipSuccess)
             throw std::string("hipMemcpy rank");
-        P->d_cen = dput(P->cen);
+        {
+            // device centres carry the inverse cell width in w
+            std::vector<double4> cw(P->cen);
+            for (size_t q = 0; q < cw.size(); q++) cw[q].w = P->cell_iw[q];
+            P->d_cen = dput(cw);
+        }
         P->d_parent = dput(P->parent);
         P->d_oct = dput(P->oct);
         P->d_ch_ptr = dput(P->ch_ptr);
@@ -1065,8 +1206,10 @@ int fmm_create(Ctx &c)
         P->d_tgt_node = dput(P->tgt_node);
         P->d_tgt_pos = dput(P->tgt_pos);
         P->d_Rc = dput(P->Rc);
-        P->d_src_cells = dput(P->src_cells);
-        P->d_tgt_cells = dput(P->tgt_cells);
+        P->d_leaf_cell = dput(P->leaf_cell);
+        P->d_m2m_cells = dput(P->m2m_cells);
+        P->d_m2l_cells = dput(P->m2l_cells);
+        P->d_l2l_cells = dput(P->l2l_cells);
         {
             // offset tables with both signs of m
             const int p = P->p, NIF = (2 * p + 1) * (2 * p + 1), NF = (p + 1) * (p + 1);
@@ -1092,8 +1235,6 @@ int fmm_create(Ctx &c)
         P->d_M = (double2 *)d;
         if (hipMalloc(&d, nexp * sizeof(double2)) != hipSuccess) throw std::string("hipMalloc L");
         P->d_L = (double2 *)d;
-        std::vector<double> W(P->W.begin(), P->W.end());
-        c.fmm_W = dput(W);
         P->mixed = c.opt.precision == FG_PREC_MIXED;
         // FP32 pass: charges of order charge_unit times the unit-width regular harmonics; scale
         // them to order one by a power of two (exact in both directions)
@@ -1103,10 +1244,7 @@ int fmm_create(Ctx &c)
         for (size_t g = 0; g < P->grp_tb.size(); g++)
             for (int t = P->grp_tb[g]; t < P->grp_te[g]; t++)
                 for (int k = P->u_ptr[g]; k < P->u_ptr[g + 1]; k++) P->m2p_pairs += (P->u_mask[k] >> P->tgt_oct[t]) & 1;
-        for (int q = P->tgt_cells_lev_off[2]; q < P->tgt_cells_lev_off[P->ls]; q++) {
-            const int ci = P->tgt_cells[q];
-            P->m2l_pairs += P->il_ptr[ci + 1] - P->il_ptr[ci];
-        }
+        for (int32_t ci : P->m2l_cells) P->m2l_pairs += P->il_ptr[ci + 1] - P->il_ptr[ci];
         return FG_OK;
     } catch (const std::string &e) {
         c.err = e;
@@ -1121,14 +1259,13 @@ int fmm_create(Ctx &c)
 // coefficients sum_g weight_g / |y - x_g| are computed once on the device.
 void fmm_build_near(Ctx &c, FmmPlan &P)
 {
-    const int L = P.L, NL = P.lev_off[L + 1] - P.lev_off[L];
+    const int NL = (int)P.leaf_cell.size();
     const int64_t S = c.S;
     const int Tm = c.Tm, Fm = c.Fm;
     std::vector<int32_t> leaf_of(S);
     for (int li = 0; li < NL; li++)
         for (int k = P.leaf_sb[li]; k < P.leaf_se[li]; k++) leaf_of[k] = li;
-    std::vector<int32_t> lx(NL), ly(NL), lz(NL);
-    for (int li = 0; li < NL; li++) unmorton(P.leaf_code[li], L, lx[li], ly[li], lz[li]);
+    const std::vector<int32_t> &lx = P.leaf_ix, &ly = P.leaf_iy, &lz = P.leaf_iz;
     // target leaves and, per source leaf, the target leaves adjacent to it
     std::vector<int32_t> tl_of_leaf(NL, -1), tl_tb, tl_te;
     for (int li = 0; li < NL; li++)
@@ -1246,11 +1383,11 @@ void fmm_destroy(Ctx &c)
     void *ptrs[] = {P->d_cen, P->d_parent, P->d_oct, P->d_ch_ptr, P->d_ch_idx, P->d_il_ptr, P->d_il_src,
                     P->d_il_off, P->d_leaf_sb, P->d_leaf_se, P->d_grp_tb, P->d_grp_te, P->d_grp_lcell, P->d_u_ptr,
                     P->d_u_cell, P->d_u_mask, P->d_tgt_oct, P->d_tgt_node, P->d_tgt_pos, P->d_Rc, P->d_Itf,
-                    P->d_Rcf, P->d_M, P->d_L, P->d_src_cells, P->d_tgt_cells, P->d_nf_ptr, P->d_nf_val,
-                    P->d_nf_valf, P->d_tl_tb, P->d_tl_te, P->d_lc_ptr, P->d_lc_col, P->d_pot, c.qe, c.fmm_W};
+                    P->d_Rcf, P->d_M, P->d_L, P->d_leaf_cell, P->d_m2m_cells, P->d_m2l_cells, P->d_l2l_cells,
+                    P->d_nf_ptr, P->d_nf_val,
+                    P->d_nf_valf, P->d_tl_tb, P->d_tl_te, P->d_lc_ptr, P->d_lc_col, P->d_pot, c.qe};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
-    c.fmm_W = nullptr;
     c.qe = nullptr;
     delete P;
     c.fmm = nullptr;
@@ -1260,8 +1397,8 @@ template <int PP>
 static void run_fmm(Ctx &c)
 {
     FmmPlan &P = *c.fmm;
-    const int L = P.L, nc = P.nc;
-    const int NL = P.lev_off[L + 1] - P.lev_off[L];
+    const int nc = P.nc;
+    const int NL = (int)P.leaf_cell.size();
     {
         Prof pr(c, "fmm_near");
         const unsigned nb = (unsigned)(((int64_t)P.ntl * 64 + 255) / 256);
@@ -1275,32 +1412,33 @@ static void run_fmm(Ctx &c)
     {
         Prof pr(c, "fmm_p2m");
         constexpr int LPW = 64 / (PP + 1);
-        k_fmm_p2m<PP><<<(NL + LPW - 1) / LPW, 64, 0, c.stream>>>(NL, P.lev_off[L], 1.0 / P.W[L], P.d_cen, P.d_leaf_sb,
-                                                                 P.d_leaf_se, c.src_pos, c.src_q, P.d_M);
+        if (NL > 0)
+            k_fmm_p2m<PP><<<(NL + LPW - 1) / LPW, 64, 0, c.stream>>>(NL, P.d_leaf_cell, P.d_cen, P.d_leaf_sb,
+                                                                     P.d_leaf_se, c.src_pos, c.src_q, P.d_M);
     }
     {
         Prof pr(c, "fmm_m2m");
-        for (int l = L - 1; l >= 2; l--) {
-            const int n = P.src_cells_lev_off[l + 1] - P.src_cells_lev_off[l];
+        for (int l = P.Lmax - 1; l >= 2; l--) {
+            const int n = P.m2m_lev_off[l + 1] - P.m2m_lev_off[l];
             if (n == 0) continue;
-            k_fmm_m2m<PP><<<n, 256, 0, c.stream>>>(P.d_src_cells + P.src_cells_lev_off[l], P.d_ch_ptr, P.d_ch_idx,
-                                                  P.d_oct, P.d_Rcf, P.d_M);
+            k_fmm_m2m<PP><<<n, 256, 0, c.stream>>>(P.d_m2m_cells + P.m2m_lev_off[l], P.d_ch_ptr, P.d_ch_idx,
+                                                   P.d_oct, P.d_Rcf, P.d_M);
         }
     }
     {
         Prof pr(c, "fmm_m2l");
-        const int n = P.tgt_cells_lev_off[P.ls] - P.tgt_cells_lev_off[2];
+        const int n = (int)P.m2l_cells.size();
         if (n > 0)
-            k_fmm_m2l<PP><<<n, 256, 0, c.stream>>>(P.d_tgt_cells + P.tgt_cells_lev_off[2], P.d_cen, c.fmm_W,
-                                                   P.d_il_ptr, P.d_il_src, P.d_il_off, P.d_Itf, P.d_M, P.d_L);
+            k_fmm_m2l<PP><<<n, 256, 0, c.stream>>>(P.d_m2l_cells, P.d_cen, P.d_il_ptr, P.d_il_src, P.d_il_off,
+                                                   P.d_Itf, P.d_M, P.d_L);
     }
     {
         Prof pr(c, "fmm_l2l");
-        for (int l = 3; l < P.ls; l++) {
-            const int n = P.tgt_cells_lev_off[l + 1] - P.tgt_cells_lev_off[l];
+        for (int l = 3; l <= P.Lmax; l++) {
+            const int n = P.l2l_lev_off[l + 1] - P.l2l_lev_off[l];
             if (n == 0) continue;
             const int threads = n * 2 * nc;
-            k_fmm_l2l<<<(threads + 255) / 256, 256, 0, c.stream>>>(n, P.d_tgt_cells + P.tgt_cells_lev_off[l], PP, nc,
+            k_fmm_l2l<<<(threads + 255) / 256, 256, 0, c.stream>>>(n, P.d_l2l_cells + P.l2l_lev_off[l], PP, nc,
                                                                    P.d_parent, P.d_oct, P.d_Rc, P.d_L);
         }
     }
@@ -1308,15 +1446,13 @@ static void run_fmm(Ctx &c)
         Prof pr(c, "fmm_m2p");
         const int ng = (int)P.grp_tb.size();
         if (ng > 0 && P.mixed)
-                k_fmm_m2p<PP, float><<<ng, 128, 0, c.stream>>>(P.d_grp_tb, P.d_grp_te, P.d_grp_lcell, P.d_u_ptr, P.d_u_cell,
-                                                               P.d_u_mask, P.d_tgt_oct, P.d_tgt_pos, P.d_cen, c.fmm_W,
-                                                               P.d_M, P.d_L, P.d_pot, P.d_tgt_node, c.rec[1 - c.cur],
-                                                               P.mscale);
+            k_fmm_m2p<PP, float><<<ng, 128, 0, c.stream>>>(P.d_grp_tb, P.d_grp_te, P.d_grp_lcell, P.d_u_ptr, P.d_u_cell,
+                                                           P.d_u_mask, P.d_tgt_oct, P.d_tgt_pos, P.d_cen, P.d_M, P.d_L,
+                                                           P.d_pot, P.d_tgt_node, c.rec[1 - c.cur], P.mscale);
         else if (ng > 0)
-                k_fmm_m2p<PP, double><<<ng, 128, 0, c.stream>>>(P.d_grp_tb, P.d_grp_te, P.d_grp_lcell, P.d_u_ptr,
-                                                                P.d_u_cell, P.d_u_mask, P.d_tgt_oct, P.d_tgt_pos, P.d_cen,
-                                                                c.fmm_W, P.d_M, P.d_L, P.d_pot, P.d_tgt_node,
-                                                                c.rec[1 - c.cur], 1.0);
+            k_fmm_m2p<PP, double><<<ng, 128, 0, c.stream>>>(P.d_grp_tb, P.d_grp_te, P.d_grp_lcell, P.d_u_ptr,
+                                                            P.d_u_cell, P.d_u_mask, P.d_tgt_oct, P.d_tgt_pos, P.d_cen,
+                                                            P.d_M, P.d_L, P.d_pot, P.d_tgt_node, c.rec[1 - c.cur], 1.0);
     }
 }
 
@@ -1324,9 +1460,9 @@ void fmm_info(const Ctx &c, int64_t info[16])
 {
     const FmmPlan &P = *c.fmm;
     for (int k = 0; k < 16; k++) info[k] = 0;
-    info[0] = P.L;
+    info[0] = P.Lmax;
     info[1] = P.ncell();
-    info[2] = P.lev_off[P.L + 1] - P.lev_off[P.L];
+    info[2] = (int64_t)P.leaf_cell.size();
     info[3] = (int64_t)P.grp_tb.size();
     info[4] = (int64_t)P.u_cell.size();
     info[5] = P.m2p_pairs;
@@ -1337,7 +1473,7 @@ void fmm_info(const Ctx &c, int64_t info[16])
     info[10] = (int64_t)P.src_sorted.size();
     info[11] = P.p;
     info[12] = P.ntl;
-    info[13] = P.src_cells_lev_off[P.L] - P.src_cells_lev_off[2];  // M2M target cells
+    info[13] = (int64_t)P.m2m_cells.size();  // M2M parent cells
 }
 
 void launch_fmm(Ctx &c)

@@ -376,6 +376,36 @@ __global__ void k_set_rhsn(int nb, KrylovCtl *ctl, const double *__restrict__ pa
     if (threadIdx.x == 0) ctl->rhsn = sqrt(fabs(v[0]));
 }
 
+// Batched instances: ||b_k||^2 of each instance (one block per instance), then the stopping scale
+// min over instances with b_k != 0, so that ||r|| <= tol min_k ||b_k|| gives every instance its own
+// criterion ||r_k|| <= tol ||b_k||.
+__global__ __launch_bounds__(BLOCK) void k_inst_norm2(const int64_t *__restrict__ off, const double2 *__restrict__ b,
+                                                     double *__restrict__ out)
+{
+    __shared__ double red[BLOCK];
+    double s = 0;
+    for (int64_t i = off[blockIdx.x] + threadIdx.x; i < off[blockIdx.x + 1]; i += blockDim.x) {
+        const double2 v = b[i];
+        s += v.x * v.x + v.y * v.y;
+    }
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int h = BLOCK / 2; h > 0; h >>= 1) {
+        if ((int)threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[blockIdx.x] = red[0];
+}
+
+__global__ void k_inst_rhsn(int K, const double *__restrict__ n2, KrylovCtl *ctl)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    double mn = -1;
+    for (int k = 0; k < K; k++)
+        if (n2[k] > 0 && (mn < 0 || n2[k] < mn)) mn = n2[k];
+    if (mn > 0) ctl->rhsn = sqrt(mn);
+}
+
 // (C) loop head: convergence on ||r||, rho_1 = rt.r, beta, p = r + beta (p - omega v),
 // p^ = D p, w = lift(p^)  (src/algebra/bicg.h:180-199)
 __global__ __launch_bounds__(BLOCK) void k_iter_p(int N, int nb, KrylovCtl *ctl, const double4 *__restrict__ ep4,
@@ -1046,6 +1076,11 @@ void launch_node_assemble(Ctx &c, double r, double cst, double /*dt*/)
                                                     c.dprec, c.b, c.x, c.w, c.part);
     }
     k_set_rhsn<<<1, 64, 0, c.stream>>>(nb, c.ctl, c.part);
+    const int K = (int)c.inst_node_off.size() - 1;
+    if (K > 1) {
+        k_inst_norm2<<<K, BLOCK, 0, c.stream>>>(c.d_inst_node_off, c.b, c.inst_bn2);
+        k_inst_rhsn<<<1, 64, 0, c.stream>>>(K, c.inst_bn2, c.ctl);
+    }
 }
 
 void launch_basis(Ctx &c, double r)

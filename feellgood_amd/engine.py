@@ -41,6 +41,31 @@ class Engine:
             raise EngineError(f"fg_create: {rc}: {self.lib.fg_last_error(None).decode()}")
         self._h = h
 
+    @classmethod
+    def batch(cls, meshes, opts=None, **kw):
+        """One context over independent meshes side by side (fg_create_batch)."""
+        self = cls.__new__(cls)
+        self.lib = nv.hip_lib()
+        arr = (nv.Mesh * len(meshes))(*[m for m in meshes])
+        self._meshes = meshes
+        self.N = sum(m.n_nodes for m in meshes)
+        o = opts if opts is not None else options(**kw)
+        h = C.c_void_p()
+        rc = self.lib.fg_create_batch(arr, len(meshes), C.byref(o), C.byref(h))
+        if rc != nv.FG_OK:
+            raise EngineError(f"fg_create_batch: {rc}: {self.lib.fg_last_error(None).decode()}")
+        self._h = h
+        return self
+
+    def layout(self):
+        """(node offsets, magnetic-tetrahedron offsets) of the instances, each of size k + 1."""
+        k = C.c_int()
+        self._chk(self.lib.fg_batch_layout(self._h, None, None, C.byref(k)), "fg_batch_layout")
+        no = np.zeros(k.value + 1, np.int64)
+        to = np.zeros(k.value + 1, np.int64)
+        self._chk(self.lib.fg_batch_layout(self._h, nv.lptr(no), nv.lptr(to), C.byref(k)), "fg_batch_layout")
+        return no, to
+
     def close(self):
         if getattr(self, "_h", None):
             self.lib.fg_destroy(self._h)

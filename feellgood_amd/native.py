@@ -68,6 +68,8 @@ FG_GUESS_NPARAM = 5574
 HIP_SYMBOLS = {
     "fg_default_options": (None, [C.POINTER(Options)]),
     "fg_create": (C.c_int, [C.POINTER(Mesh), C.POINTER(Options), C.POINTER(C.c_void_p)]),
+    "fg_create_batch": (C.c_int, [C.POINTER(Mesh), C.c_int, C.POINTER(Options), C.POINTER(C.c_void_p)]),
+    "fg_batch_layout": (C.c_int, [C.c_void_p, _i64, _i64, C.POINTER(C.c_int)]),
     "fg_destroy": (C.c_int, [C.c_void_p]),
     "fg_set_state": (C.c_int, [C.c_void_p, C.c_int, _d, _d, _d, _d]),
     "fg_get_state": (C.c_int, [C.c_void_p, C.c_int, _d, _d, _d, _d]),

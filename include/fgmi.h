@@ -142,6 +142,17 @@ void fg_default_options(fg_options *opt);
 
 /* Build the device plan (matrix pattern, gather lists, geometry, octree) and upload the mesh. */
 int fg_create(const fg_mesh *mesh, const fg_options *opt, fg_ctx **out);
+/* Batched independent instances: k meshes held side by side in one context, so that every verb
+ * below runs each kernel once over all of them. Nothing couples the instances: each has its own
+ * octree (the demag of one never sees another), and BiCGSTAB stops when ||r|| <= tol min_k ||b_k||
+ * (k over instances with b_k != 0), which meets every instance's own criterion ||r_k|| <= tol
+ * ||b_k||. States are exchanged for the union: node ranges as fg_batch_layout reports them, in
+ * input order. One basis angle and one applied field serve all instances. No counterpart in the
+ * reference (one geometry per process there). */
+int fg_create_batch(const fg_mesh *meshes, int k, const fg_options *opt, fg_ctx **out);
+/* Instance layout of a context: k, node offsets [k+1] and magnetic-tetrahedron offsets [k+1]
+ * (either array may be NULL; a plain context reports k = 1). */
+int fg_batch_layout(fg_ctx *ctx, int64_t *node_off, int64_t *mtet_off, int *k);
 int fg_destroy(fg_ctx *ctx);
 
 /* Copy a node state in (which = FG_CURRENT, FG_NEXT or FG_BOTH). NULL pointers write zeros.
