@@ -458,9 +458,21 @@ void build_plan(Ctx &c, const fg_mesh *m)
     c.src_rank = dalloc<int64_t>(c.S);
     upload(c.src_rank, rank, st);
     if (c.inst_node_off.size() > 2) {
-        c.d_inst_node_off = dalloc<int64_t>(c.inst_node_off.size());
+        const size_t K = c.inst_node_off.size() - 1;
+        c.d_inst_node_off = dalloc<int64_t>(K + 1);
         upload(c.d_inst_node_off, c.inst_node_off, st);
-        c.inst_bn2 = dalloc<double>(c.inst_node_off.size());
+        c.inst_bn2 = dalloc<double>(K + 1);
+        std::vector<int32_t> ni(N);
+        std::vector<longlong4> rs(K);
+        for (size_t k = 0; k < K; k++) {
+            for (int64_t i = c.inst_node_off[k]; i < c.inst_node_off[k + 1]; i++) ni[i] = (int32_t)k;
+            rs[k] = make_longlong4(5 * c.inst_mtet_off[k], 5 * c.inst_mtet_off[k + 1],
+                                   5 * (int64_t)Tm + 4 * c.inst_mtri_off[k], 5 * (int64_t)Tm + 4 * c.inst_mtri_off[k + 1]);
+        }
+        c.d_node_inst = dalloc<int32_t>(N);
+        upload(c.d_node_inst, ni, st);
+        c.d_inst_src = dalloc<longlong4>(K);
+        upload(c.d_inst_src, rs, st);
     }
     HIPCHK(hipStreamSynchronize(st));
 }
@@ -471,7 +483,7 @@ void free_all(Ctx &c)
                     c.rt, c.p, c.ph, c.v, c.s, c.sh, c.t, c.w, c.ctl, c.part, c.epart, c.res, c.slice_ptr, c.slice_w,
                     c.scol, c.sval, c.tet_nodes, c.geo, c.tet_reg, c.regions, c.elem_out, c.inc_ptr, c.inc_idx,
                     c.trik_nodes, c.trik_geo, c.tri_nodes, c.tri_geo, c.tri_corr, c.cinc_ptr, c.cinc_idx, c.corr,
-                    c.src_pos, c.src_q, c.src_pos_f, c.src_rank, c.d_inst_node_off, c.inst_bn2};
+                    c.src_pos, c.src_q, c.src_pos_f, c.src_rank, c.d_inst_node_off, c.inst_bn2, c.d_node_inst, c.d_inst_src};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     if (c.h_ctl) (void)hipHostFree(c.h_ctl);

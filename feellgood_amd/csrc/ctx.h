@@ -45,6 +45,8 @@ struct Ctx {
     std::vector<int64_t> in_node_off, in_tet_off, in_tri_off;  // the same ranges in the input mesh
     int64_t *d_inst_node_off = nullptr;  // batch only: node ranges on the device
     double *inst_bn2 = nullptr;          // batch only: ||b_k||^2 per instance
+    int32_t *d_node_inst = nullptr;      // batch only: instance of each node
+    longlong4 *d_inst_src = nullptr;     // batch only: tetrahedron and triangle source ranges
     double charge_unit = 1.0;            // typical |charge| (Ms_max w^(2/3)), scales the FP32 FMM pass
     std::vector<int64_t> h_rowptr;       // stencil CSR (unpadded), for fg_get_system
     std::vector<int32_t> h_col;

@@ -817,12 +817,16 @@ __global__ __launch_bounds__(BLOCK) void k_corr_gather(int N, const int32_t *__r
 }
 
 // Exact potentials phi_i = (sum_s q_s / |x_i - y_s| + corr_i) / (4 pi) (src/fmm_demag.h:211-223),
-// FP64, sources staged through LDS in tiles of BLOCK.
+// FP64, sources staged through LDS in tiles of BLOCK. In a batch (node_inst != nullptr) a node
+// sees only its own instance's sources: tetrahedron sources [r.x, r.y), triangle sources [r.z, r.w)
+// with r = inst_src[instance].
 __global__ __launch_bounds__(BLOCK) void k_p2p_direct(int N, int64_t S, const int32_t *__restrict__ mag,
                                                       const double4 *__restrict__ pos4,
                                                       const double4 *__restrict__ spos,
                                                       const double2 *__restrict__ q,
-                                                      const double2 *__restrict__ corr, NodeRec *__restrict__ nxt)
+                                                      const double2 *__restrict__ corr, NodeRec *__restrict__ nxt,
+                                                      const int32_t *__restrict__ node_inst,
+                                                      const longlong4 *__restrict__ inst_src)
 {
     __shared__ double4 ts[BLOCK];
     __shared__ double2 tq[BLOCK];
@@ -830,6 +834,8 @@ __global__ __launch_bounds__(BLOCK) void k_p2p_direct(int N, int64_t S, const in
     const bool act = i < N && mag[i];
     double4 xi = make_double4(0, 0, 0, 0);
     if (i < N) xi = pos4[i];
+    longlong4 r = make_longlong4(0, S, 0, 0);
+    if (node_inst && i < N) r = inst_src[node_inst[i]];
     double pu = 0, pv = 0;
     for (int64_t base = 0; base < S; base += BLOCK) {
         const int64_t k = base + threadIdx.x;
@@ -842,14 +848,28 @@ __global__ __launch_bounds__(BLOCK) void k_p2p_direct(int N, int64_t S, const in
         }
         __syncthreads();
         if (act) {
+            const bool all = (base >= r.x && base + BLOCK <= r.y) || (base >= r.z && base + BLOCK <= r.w);
+            if (all) {
 #pragma unroll 8
-            for (int j = 0; j < BLOCK; j++) {
-                const double4 y = ts[j];
-                const double dx = xi.x - y.x, dy = xi.y - y.y, dz = xi.z - y.z;
-                const double ir = 1.0 / sqrt(dx * dx + dy * dy + dz * dz);
-                const double2 qq = tq[j];
-                pu += qq.x * ir;
-                pv += qq.y * ir;
+                for (int j = 0; j < BLOCK; j++) {
+                    const double4 y = ts[j];
+                    const double dx = xi.x - y.x, dy = xi.y - y.y, dz = xi.z - y.z;
+                    const double ir = 1.0 / sqrt(dx * dx + dy * dy + dz * dz);
+                    const double2 qq = tq[j];
+                    pu += qq.x * ir;
+                    pv += qq.y * ir;
+                }
+            } else {
+                for (int j = 0; j < BLOCK; j++) {
+                    const int64_t s = base + j;
+                    if (!((s >= r.x && s < r.y) || (s >= r.z && s < r.w))) continue;
+                    const double4 y = ts[j];
+                    const double dx = xi.x - y.x, dy = xi.y - y.y, dz = xi.z - y.z;
+                    const double ir = 1.0 / sqrt(dx * dx + dy * dy + dz * dz);
+                    const double2 qq = tq[j];
+                    pu += qq.x * ir;
+                    pv += qq.y * ir;
+                }
             }
         }
         __syncthreads();
@@ -1158,7 +1178,7 @@ void launch_direct_p2p(Ctx &c)
 {
     Prof pr(c, "p2p_direct");
     k_p2p_direct<<<grid1(c.N), BLOCK, 0, c.stream>>>(c.N, c.S, c.mag, c.pos4, c.src_pos, c.src_q, c.corr,
-                                                     c.rec[1 - c.cur]);
+                                                     c.rec[1 - c.cur], c.d_node_inst, c.d_inst_src);
 }
 
 void launch_energy(Ctx &c, const double Hext[3])
