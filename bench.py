@@ -317,15 +317,18 @@ def run_gpu(args, w, m, u0, precision, guess):
 
 def step_parity(args, w, m, u0, precision, guess, steps=3):
     """Per-step |dm| of the fast mode against the plain FP64 engine: both start every step from the
-    plain engine's state with the same basis angle; max over `steps` steps (untimed)."""
+    plain engine's u, v with the same basis angle, each with the demag potentials of its own
+    precision; max over `steps` steps (untimed)."""
     from feellgood_amd import native as nv
     ref = make_engine(args, w, m, "fp64", None, False)
     fast = make_engine(args, w, m, precision, guess, False)
     a, b = Stepper(ref, w, u0), Stepper(fast, w, u0)
     dm = 0.0
     for _ in range(steps):
-        u, v, phi, phiv = ref.get_state(nv.FG_CURRENT)
-        fast.set_state(nv.FG_BOTH, u=u, v=v, phi=phi, phiv=phiv)
+        u, v, _, _ = ref.get_state(nv.FG_CURRENT)
+        fast.set_state(nv.FG_BOTH, u=u, v=v)
+        fast.calc_demag()
+        fast.evolution()
         r = a.angle()
         a.step(r)
         b.step(r)

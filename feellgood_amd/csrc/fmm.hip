@@ -76,6 +76,7 @@ struct FmmPlan {
     bool mixed = false;
     double mscale = 1.0;         // multipoles times mscale (a power of two) in the FP32 pass
     int64_t m2p_pairs = 0, m2l_pairs = 0, lc_cols = 0;
+    int64_t m2p_slots = 0;  // lane slots the M2P launch mapping executes (useful pairs / slots = SIMD efficiency)
     int64_t nf_nnz = 0;
     int32_t *d_tgt_node = nullptr;
     double4 *d_tgt_pos = nullptr;
@@ -666,107 +667,42 @@ __device__ inline int xcd_block(int b, int nb)
     return x * q + min(x, r) + (b >> 3);
 }
 
-// P2M: P+1 lanes per source leaf, lane m owning the column m of both expansions (coefficients
-// (m..P, m)); 64 / (P+1) leaves per wavefront. The lanes of a leaf walk its sources together
-// (one broadcast load per source); lane m forms R_m^m = (-1/2)^m / m! (x + iy)^m and runs the
-// column recurrence with its own divisors, computed once.
-template <int P>
+// P2M: a group of G lanes per source leaf, each lane a strided subset of the leaf's sources; the
+// lanes' accumulators are combined with cross-lane sums. blockIdx.y = charge set.
+template <int P, int G>
 __global__ __launch_bounds__(64) void k_fmm_p2m(int nleaf, const int32_t *__restrict__ leaf_cell,
                                                 const double4 *__restrict__ cen, const int32_t *__restrict__ sb,
                                                 const int32_t *__restrict__ se, const double4 *__restrict__ spos,
                                                 const double2 *__restrict__ q, double2 *__restrict__ M)
 {
-    constexpr int NCo = fcount(P), LPW = 64 / (P + 1);
-    const int lane = threadIdx.x;
-    const int g = lane / (P + 1), m = lane - g * (P + 1);
-    const int li = blockIdx.x * LPW + g;
-    const bool act = g < LPW && li < nleaf;
-    // R_{m+k}^m = ca[k] z R_{m+k-1}^m - cb[k] r^2 R_{m+k-2}^m, k >= 2
-    double ca[P + 1], cb[P + 1];
+    constexpr int NCo = fcount(P);
+    const int lane = threadIdx.x & (G - 1);
+    const int li = (blockIdx.x * 64 + threadIdx.x) / G;
+    const int set = blockIdx.y;
+    double acc[2 * NCo];
 #pragma unroll
-    for (int k = 0; k <= P; k++) {
-        const int n = m + k;
-        const double inv = (k >= 2 && n <= P) ? 1.0 / ((double)(n + m) * (double)(n - m)) : 0.0;
-        ca[k] = (2.0 * n - 1.0) * inv;
-        cb[k] = inv;
-    }
-    double cm = 1.0;
-#pragma unroll 1
-    for (int k = 1; k <= m; k++) cm *= -0.5 / (double)k;
-    double ur[P + 1], ui[P + 1], vr[P + 1], vi[P + 1];
-#pragma unroll
-    for (int k = 0; k <= P; k++) ur[k] = ui[k] = vr[k] = vi[k] = 0.0;
+    for (int k = 0; k < 2 * NCo; k++) acc[k] = 0.0;
+    const bool act = li < nleaf;
     const int cell = act ? leaf_cell[li] : 0;
     if (act) {
         const double4 c = cen[cell];
         const double iw = c.w;
-        const int s0 = sb[li], s1 = se[li];
-        for (int s = s0; s < s1; s += 2) {
-            // two sources per pass: independent recurrences interleaved (a missing second source
-            // carries zero charge)
-            double x[2], y[2], z[2], r2[2], qu[2], qv[2];
-#pragma unroll
-            for (int h = 0; h < 2; h++) {
-                const int k = min(s + h, s1 - 1);
-                const double4 p = spos[k];
-                const double2 qq = q[k];
-                const bool ok = s + h < s1;
-                x[h] = (p.x - c.x) * iw;
-                y[h] = (p.y - c.y) * iw;
-                z[h] = (p.z - c.z) * iw;
-                r2[h] = x[h] * x[h] + y[h] * y[h] + z[h] * z[h];
-                qu[h] = ok ? qq.x : 0.0;
-                qv[h] = ok ? qq.y : 0.0;
-            }
-            double pr[2] = {1.0, 1.0}, pi[2] = {0.0, 0.0};
-#pragma unroll
-            for (int k = 1; k <= P; k++)
-                if (k <= m)
-#pragma unroll
-                    for (int h = 0; h < 2; h++) {
-                        const double t = pr[h] * x[h] - pi[h] * y[h];
-                        pi[h] = pr[h] * y[h] + pi[h] * x[h];
-                        pr[h] = t;
-                    }
-            double r0r[2], r0i[2], r1r[2], r1i[2];
-#pragma unroll
-            for (int h = 0; h < 2; h++) {
-                r0r[h] = cm * pr[h];  // R_m^m
-                r0i[h] = cm * pi[h];
-                r1r[h] = z[h] * r0r[h];  // R_{m+1}^m
-                r1i[h] = z[h] * r0i[h];
-                ur[0] += qu[h] * r0r[h];
-                ui[0] += qu[h] * r0i[h];
-                vr[0] += qv[h] * r0r[h];
-                vi[0] += qv[h] * r0i[h];
-                ur[1] += qu[h] * r1r[h];
-                ui[1] += qu[h] * r1i[h];
-                vr[1] += qv[h] * r1r[h];
-                vi[1] += qv[h] * r1i[h];
-            }
-#pragma unroll
-            for (int k = 2; k <= P; k++)
-#pragma unroll
-                for (int h = 0; h < 2; h++) {
-                    const double a = ca[k] * z[h], b = cb[k] * r2[h];
-                    const double cr = a * r1r[h] - b * r0r[h], ci = a * r1i[h] - b * r0i[h];
-                    ur[k] += qu[h] * cr;
-                    ui[k] += qu[h] * ci;
-                    vr[k] += qv[h] * cr;
-                    vi[k] += qv[h] * ci;
-                    r0r[h] = r1r[h];
-                    r0i[h] = r1i[h];
-                    r1r[h] = cr;
-                    r1i[h] = ci;
-                }
+        for (int k = sb[li] + lane; k < se[li]; k += G) {
+            const double4 x = spos[k];
+            const double2 qq = q[k];
+            p2m_accumulate<P>((x.x - c.x) * iw, (x.y - c.y) * iw, (x.z - c.z) * iw, set ? qq.y : qq.x, acc);
         }
-        double2 *Mu = M + (size_t)cell * 2 * NCo, *Mv = Mu + NCo;
+    }
 #pragma unroll
-        for (int k = 0; k <= P; k++)
-            if (m + k <= P) {
-                Mu[fidx(m + k, m)] = make_double2(ur[k], ui[k]);
-                Mv[fidx(m + k, m)] = make_double2(vr[k], vi[k]);
-            }
+    for (int k = 0; k < 2 * NCo; k++) {
+        double v = acc[k];
+        for (int o = G / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+        acc[k] = v;
+    }
+    if (act && lane == 0) {
+        double2 *dst = M + ((size_t)cell * 2 + set) * NCo;
+#pragma unroll
+        for (int k = 0; k < NCo; k++) dst[k] = make_double2(acc[2 * k], acc[2 * k + 1]);
     }
 }
 
@@ -1031,12 +967,16 @@ __global__ __launch_bounds__(256) void k_nf_apply(int ntl, const int32_t *__rest
     }
 }
 
-// Far field: one block per evaluation group. The group's source list is walked in chunks of CH
-// cells whose multipoles are staged in LDS; a thread evaluates a chunk slice for two targets at
-// once (one load of each coefficient serves both). The slices are summed through LDS, then the
-// L2P of the group's level-(L-2) ancestor is added and phi, phi_v are written.
+// Far field: one block per evaluation group. Threads are (target slot, slice) pairs: with n
+// targets in the current chunk of at most 64, nsl = min(256 / n, CH) slices, and the group's source
+// list is walked in stages of nsl cells, one cell per slice, whose multipoles are staged in LDS
+// (in the real type R: FP64, or FP32 scaled by mscale in the mixed-precision pass), so that every
+// stage keeps n * nsl of the 256 lanes busy. Each target's slices are summed through LDS in FP64,
+// then the L2P of the group's level-(L-2) ancestor is added and phi, phi_v are written.
+template <class R> constexpr int m2p_stage_cells() { return sizeof(R) == 8 ? 16 : 32; }
+
 template <int P, class R>
-__global__ __launch_bounds__(128) void k_fmm_m2p(const int32_t *__restrict__ grp_tb, const int32_t *__restrict__ grp_te,
+__global__ __launch_bounds__(256) void k_fmm_m2p(const int32_t *__restrict__ grp_tb, const int32_t *__restrict__ grp_te,
                                                  const int32_t *__restrict__ grp_lcell,
                                                  const int32_t *__restrict__ u_ptr, const int32_t *__restrict__ u_cell,
                                                  const uint8_t *__restrict__ u_mask,
@@ -1047,32 +987,27 @@ __global__ __launch_bounds__(128) void k_fmm_m2p(const int32_t *__restrict__ grp
                                                  NodeRec *__restrict__ nxt, double mscale)
 {
     using R2 = typename vec2_of<R>::type;
-    constexpr int NCo = fcount(P), CW = 2 * NCo, CH = 16, NT = 128, TMAX = 64;
+    constexpr int NCo = fcount(P), CW = 2 * NCo, CH = m2p_stage_cells<R>(), NT = 256;
     __shared__ R2 sM[CH * CW];
     __shared__ double4 sC[CH];
     __shared__ int sMask[CH];
-    __shared__ double4 red[NT];
+    __shared__ double2 red[NT];
     const int g = xcd_block(blockIdx.x, gridDim.x);
     const int tid = threadIdx.x;
     const int tb = grp_tb[g], te = grp_te[g], ub = u_ptr[g], ue = u_ptr[g + 1], lc = grp_lcell[g];
-    for (int t0 = tb; t0 < te; t0 += TMAX) {
-        const int n = min(TMAX, te - t0);
-        const int np = (n + 1) >> 1;  // target pairs
-        const int nsl = min(NT / np, CH);
-        const int slot = tid % np, slice = tid / np;
-        const bool act = slice < nsl;
-        const int ta = t0 + 2 * slot, tb2 = min(ta + 1, t0 + n - 1);
-        double4 ya = make_double4(0, 0, 0, 0), yb = ya;
-        int oa = 0, ob = 0;
-        if (act) {
-            ya = tpos[ta];
-            yb = tpos[tb2];
-            oa = tgt_oct[ta];
-            ob = tb2 == ta ? 8 : tgt_oct[tb2];  // bit 8 of the mask is never set
+    for (int t0 = tb; t0 < te; t0 += 64) {
+        const int n = min(64, te - t0);
+        const int nsl = min(NT / n, CH);
+        const int slot = tid % n, slice = tid / n;
+        double4 y = make_double4(0, 0, 0, 0);
+        int oc = 0;
+        if (slice < nsl) {
+            y = tpos[t0 + slot];
+            oc = tgt_oct[t0 + slot];
         }
-        double ua = 0, va = 0, ub2 = 0, vb = 0;
-        for (int cb = ub; cb < ue; cb += CH) {
-            const int nch = min(CH, ue - cb);
+        double pu = 0, pv = 0;
+        for (int cb = ub; cb < ue; cb += nsl) {
+            const int nch = min(nsl, ue - cb);
             for (int i = tid; i < nch * CW; i += NT) {
                 const int jj = i / CW;
                 const double2 v = M[(size_t)u_cell[cb + jj] * CW + (i - jj * CW)];
@@ -1080,41 +1015,28 @@ __global__ __launch_bounds__(128) void k_fmm_m2p(const int32_t *__restrict__ grp
                 sM[i].y = (R)(v.y * mscale);
             }
             if (tid < nch) {
-                const double4 c = cen[u_cell[cb + tid]];
-                sC[tid] = c;
+                sC[tid] = cen[u_cell[cb + tid]];
                 sMask[tid] = u_mask[cb + tid];
             }
             __syncthreads();
-            if (act)
-                for (int jj = slice; jj < nch; jj += nsl) {
-                    const int mk = sMask[jj];
-                    const bool ha = (mk >> oa) & 1, hb = (mk >> ob) & 1;
-                    if (!ha && !hb) continue;
-                    const double4 c = sC[jj];
-                    R s0 = 0, s1 = 0, s2 = 0, s3 = 0;
-                    m2p_eval2x2_t<P, R>(sM + jj * CW, sM + jj * CW + NCo, (R)((ya.x - c.x) * c.w), (R)((ya.y - c.y) * c.w),
-                                        (R)((ya.z - c.z) * c.w), (R)((yb.x - c.x) * c.w), (R)((yb.y - c.y) * c.w),
-                                        (R)((yb.z - c.z) * c.w), s0, s1, s2, s3);
-                    if (ha) {
-                        ua += (double)s0 * c.w;
-                        va += (double)s1 * c.w;
-                    }
-                    if (hb) {
-                        ub2 += (double)s2 * c.w;
-                        vb += (double)s3 * c.w;
-                    }
-                }
+            if (slice < nch && ((sMask[slice] >> oc) & 1)) {
+                const double4 c = sC[slice];
+                R su = 0, sv = 0;
+                m2p_eval2_r<P, R>(sM + slice * CW, sM + slice * CW + NCo, (R)((y.x - c.x) * c.w),
+                                  (R)((y.y - c.y) * c.w), (R)((y.z - c.z) * c.w), su, sv);
+                pu += (double)su * c.w;
+                pv += (double)sv * c.w;
+            }
             __syncthreads();
         }
-        red[tid] = make_double4(ua, va, ub2, vb);
+        red[tid] = make_double2(pu, pv);
         __syncthreads();
         if (tid < n) {
-            const int p = tid >> 1, h = tid & 1;
             double su = 0, sv = 0;
             for (int s = 0; s < nsl; s++) {
-                const double4 r = red[s * np + p];
-                su += h ? r.z : r.x;
-                sv += h ? r.w : r.y;
+                const double2 r = red[s * n + tid];
+                su += r.x;
+                sv += r.y;
             }
             su /= mscale;
             sv /= mscale;
@@ -1135,6 +1057,24 @@ __global__ __launch_bounds__(128) void k_fmm_m2p(const int32_t *__restrict__ grp
 }
 
 void fmm_build_near(Ctx &c, FmmPlan &P);
+
+// Lane slots executed by k_fmm_m2p's mapping: per 64-target chunk of a group and per stage of nsl
+// source cells, the wavefronts that hold the stage's n * nch (target, cell) lanes.
+static int64_t m2p_lane_slots(const FmmPlan &P)
+{
+    constexpr int NT = 256;
+    const int CH = P.mixed ? m2p_stage_cells<float>() : m2p_stage_cells<double>();
+    int64_t slots = 0;
+    for (size_t g = 0; g < P.grp_tb.size(); g++)
+        for (int t0 = P.grp_tb[g]; t0 < P.grp_te[g]; t0 += 64) {
+            const int n = std::min(64, P.grp_te[g] - t0), nsl = std::min(NT / n, CH);
+            for (int cb = P.u_ptr[g]; cb < P.u_ptr[g + 1]; cb += nsl) {
+                const int nch = std::min(nsl, P.u_ptr[g + 1] - cb);
+                slots += 64 * (int64_t)((n * nch + 63) / 64);
+            }
+        }
+    return slots;
+}
 
 // ---- host side of the device pass ----
 int fmm_create(Ctx &c)
@@ -1244,6 +1184,7 @@ int fmm_create(Ctx &c)
         for (size_t g = 0; g < P->grp_tb.size(); g++)
             for (int t = P->grp_tb[g]; t < P->grp_te[g]; t++)
                 for (int k = P->u_ptr[g]; k < P->u_ptr[g + 1]; k++) P->m2p_pairs += (P->u_mask[k] >> P->tgt_oct[t]) & 1;
+        P->m2p_slots = m2p_lane_slots(*P);
         for (int32_t ci : P->m2l_cells) P->m2l_pairs += P->il_ptr[ci + 1] - P->il_ptr[ci];
         return FG_OK;
     } catch (const std::string &e) {
@@ -1411,10 +1352,10 @@ static void run_fmm(Ctx &c)
     }
     {
         Prof pr(c, "fmm_p2m");
-        constexpr int LPW = 64 / (PP + 1);
+        constexpr int G = 8;
         if (NL > 0)
-            k_fmm_p2m<PP><<<(NL + LPW - 1) / LPW, 64, 0, c.stream>>>(NL, P.d_leaf_cell, P.d_cen, P.d_leaf_sb,
-                                                                     P.d_leaf_se, c.src_pos, c.src_q, P.d_M);
+            k_fmm_p2m<PP, G><<<dim3((NL * G + 63) / 64, 2), 64, 0, c.stream>>>(NL, P.d_leaf_cell, P.d_cen, P.d_leaf_sb,
+                                                                               P.d_leaf_se, c.src_pos, c.src_q, P.d_M);
     }
     {
         Prof pr(c, "fmm_m2m");
@@ -1446,11 +1387,11 @@ static void run_fmm(Ctx &c)
         Prof pr(c, "fmm_m2p");
         const int ng = (int)P.grp_tb.size();
         if (ng > 0 && P.mixed)
-            k_fmm_m2p<PP, float><<<ng, 128, 0, c.stream>>>(P.d_grp_tb, P.d_grp_te, P.d_grp_lcell, P.d_u_ptr, P.d_u_cell,
+            k_fmm_m2p<PP, float><<<ng, 256, 0, c.stream>>>(P.d_grp_tb, P.d_grp_te, P.d_grp_lcell, P.d_u_ptr, P.d_u_cell,
                                                            P.d_u_mask, P.d_tgt_oct, P.d_tgt_pos, P.d_cen, P.d_M, P.d_L,
                                                            P.d_pot, P.d_tgt_node, c.rec[1 - c.cur], P.mscale);
         else if (ng > 0)
-            k_fmm_m2p<PP, double><<<ng, 128, 0, c.stream>>>(P.d_grp_tb, P.d_grp_te, P.d_grp_lcell, P.d_u_ptr,
+            k_fmm_m2p<PP, double><<<ng, 256, 0, c.stream>>>(P.d_grp_tb, P.d_grp_te, P.d_grp_lcell, P.d_u_ptr,
                                                             P.d_u_cell, P.d_u_mask, P.d_tgt_oct, P.d_tgt_pos, P.d_cen,
                                                             P.d_M, P.d_L, P.d_pot, P.d_tgt_node, c.rec[1 - c.cur], 1.0);
     }
@@ -1474,6 +1415,7 @@ void fmm_info(const Ctx &c, int64_t info[16])
     info[11] = P.p;
     info[12] = P.ntl;
     info[13] = (int64_t)P.m2m_cells.size();  // M2M parent cells
+    info[14] = P.m2p_slots;
 }
 
 void launch_fmm(Ctx &c)

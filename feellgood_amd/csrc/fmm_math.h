@@ -224,163 +224,110 @@ __host__ __device__ inline void m2p_eval2(const double2 *Mu, const double2 *Mv, 
 // recurrences are carried as running doubles (stepped by the inline constants 1 and 2) instead of
 // per-term literals: gfx950's three-operand FP64 instructions take no literal operands, so
 // literals would each occupy a scalar register pair.
-template <int P>
-__host__ __device__ inline void m2p_eval2_t(const double2 *__restrict__ Mu, const double2 *__restrict__ Mv, double x,
-                                            double y, double z, double &su, double &sv)
-{
-    const double r2 = x * x + y * y + z * z;
-    const double ir2 = 1.0 / r2;
-    const double z2 = z + z;
-    double dre = 1.0 / sqrt(r2), dim = 0.0;
-    double tm = 1.0;  // 2m + 1
-#pragma unroll
-    for (int m = 0; m <= P; m++) {
-        if (m > 0) {
-            // I_m^m = -(2m-1) (x + iy) I_{m-1}^{m-1} / r^2
-            const double f = (2.0 - tm) * ir2;
-            const double nre = f * (x * dre - y * dim), nim = f * (x * dim + y * dre);
-            dre = nre;
-            dim = nim;
-        }
-        double2 a = Mu[fidx(m, m)], b = Mv[fidx(m, m)];
-        double au = a.x * dre + a.y * dim, av = b.x * dre + b.y * dim;
-        if (m + 1 <= P) {
-            // I_{m+1}^m = (2m+1) z I_m^m / r^2
-            const double a1 = tm * z * ir2;
-            double p2re = dre, p2im = dim, p1re = a1 * dre, p1im = a1 * dim;
-            a = Mu[fidx(m + 1, m)];
-            b = Mv[fidx(m + 1, m)];
-            au += a.x * p1re + a.y * p1im;
-            av += b.x * p1re + b.y * p1im;
-            double tz = (tm + 2.0) * z;  // (2n-1) z at n = m+2
-            double bb = tm;              // (n+m-1)(n-m-1) at n = m+2
-            double step = tm + 2.0;      // bb(n+1) - bb(n) = 2n - 1
-#pragma unroll
-            for (int n = m + 2; n <= P; n++) {
-                const double cre = (tz * p1re - bb * p2re) * ir2, cim = (tz * p1im - bb * p2im) * ir2;
-                a = Mu[fidx(n, m)];
-                b = Mv[fidx(n, m)];
-                au += a.x * cre + a.y * cim;
-                av += b.x * cre + b.y * cim;
-                p2re = p1re;
-                p2im = p1im;
-                p1re = cre;
-                p1im = cim;
-                tz += z2;
-                bb += step;
-                step += 2.0;
-            }
-        }
-        if (m == 0) {
-            su += au;
-            sv += av;
-        } else {
-            su += au + au;
-            sv += av + av;
-        }
-        tm += 2.0;
-    }
-}
-
 // Real-type helpers of the templated evaluators (FP64, or FP32 for the mixed-precision pass).
 template <class R> struct vec2_of;
 template <> struct vec2_of<double> { using type = double2; };
 template <> struct vec2_of<float> { using type = float2; };
-__host__ __device__ inline double inv_of(double x) { return 1.0 / x; }
-__host__ __device__ inline double rsqrt_of(double x) { return 1.0 / sqrt(x); }
-__host__ __device__ inline float inv_of(float x) { return 1.0f / x; }
-__host__ __device__ inline float rsqrt_of(float x) { return 1.0f / sqrtf(x); }
 
-// M2P of two expansions at two targets at once (independent recurrences interleaved; each
-// coefficient is loaded once for both targets), in the real type R. The recurrence factors are
-// scaled by 1/r^2 and carried as running values.
-template <int P, class R>
-__host__ __device__ inline void m2p_eval2x2_t(const typename vec2_of<R>::type *__restrict__ Mu,
-                                              const typename vec2_of<R>::type *__restrict__ Mv, R x0, R y0, R z0,
-                                              R x1, R y1, R z1, R &su0, R &sv0, R &su1, R &sv1)
+// Reciprocal and reciprocal square root for the device evaluators: the hardware approximations
+// refined by Newton steps (two in FP64, each doubling the correct bits, to full precision for the
+// O(1) arguments here, in place of the IEEE-exact division sequences), used as is in FP32.
+__device__ inline double fast_inv(double x)
 {
-    R ir2[2], zr[2], dre[2], dim[2], xx[2] = {x0, x1}, yy[2] = {y0, y1};
-    {
-        const R r20 = x0 * x0 + y0 * y0 + z0 * z0, r21 = x1 * x1 + y1 * y1 + z1 * z1;
-        ir2[0] = inv_of(r20);
-        ir2[1] = inv_of(r21);
-        dre[0] = rsqrt_of(r20);
-        dre[1] = rsqrt_of(r21);
-        zr[0] = z0 * ir2[0];
-        zr[1] = z1 * ir2[1];
-        dim[0] = dim[1] = R(0);
+    double r = __builtin_amdgcn_rcp(x);
+    double e = fma(-x, r, 1.0);
+    r = fma(r, e, r);
+    e = fma(-x, r, 1.0);
+    return fma(r, e, r);
+}
+__device__ inline double fast_rsqrt(double x)
+{
+    double y = __builtin_amdgcn_rsq(x);
+    double e = fma(-x * y, y, 1.0);  // 1 - x y^2
+    y = fma(0.5 * y, e, y);
+    e = fma(-x * y, y, 1.0);
+    return fma(0.5 * y, e, y);
+}
+__device__ inline float fast_inv(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ inline float fast_rsqrt(float x) { return __builtin_amdgcn_rsqf(x); }
+
+// M2P of two expansions at one target in the real type R: su += sum M_u,n^m conj(I_n^m(x)), same
+// for sv, x in cell widths. The irregular harmonics are streamed column by column (diagonal
+// I_m^m, then I_{m+1}^m and the three-term recurrence in n) with the recurrence factors scaled by
+// 1/r^2 and carried as running values. The m = 0 column is real (M_n^0 and I_n^0 are real for
+// real charges), so only its real parts are formed; each product is its own fused multiply-add
+// into the running sum.
+template <int P, class R>
+__device__ inline void m2p_eval2_r(const typename vec2_of<R>::type *__restrict__ Mu,
+                                   const typename vec2_of<R>::type *__restrict__ Mv, R x, R y, R z, R &su, R &sv)
+{
+    const R r2 = x * x + y * y + z * z;
+    const R ir2 = fast_inv(r2), zr = z * ir2, zr2 = zr + zr, ir22 = ir2 + ir2;
+    R dre = fast_rsqrt(r2), dim = R(0);
+    R au = Mu[0].x * dre, av = Mv[0].x * dre;
+    if (P >= 1) {
+        R p2 = dre, p1 = zr * dre;
+        au = fma(Mu[fidx(1, 0)].x, p1, au);
+        av = fma(Mv[fidx(1, 0)].x, p1, av);
+        R tz = R(3) * zr, bb = ir2, st = R(3) * ir2;
+#pragma unroll
+        for (int n = 2; n <= P; n++) {
+            const R c = tz * p1 - bb * p2;
+            au = fma(Mu[fidx(n, 0)].x, c, au);
+            av = fma(Mv[fidx(n, 0)].x, c, av);
+            p2 = p1;
+            p1 = c;
+            tz += zr2;
+            bb += st;
+            st += ir22;
+        }
     }
+    R hu = R(0), hv = R(0);
     R tm = R(1);  // 2m + 1
 #pragma unroll
-    for (int m = 0; m <= P; m++) {
-        R au[2], av[2];
-#pragma unroll
-        for (int t = 0; t < 2; t++)
-            if (m > 0) {
-                const R f = (R(2) - tm) * ir2[t];
-                const R nre = f * (xx[t] * dre[t] - yy[t] * dim[t]), nim = f * (xx[t] * dim[t] + yy[t] * dre[t]);
-                dre[t] = nre;
-                dim[t] = nim;
-            }
+    for (int m = 1; m <= P; m++) {
+        tm += R(2);
+        const R f = (R(2) - tm) * ir2;
+        const R nre = f * (x * dre - y * dim), nim = f * (x * dim + y * dre);
+        dre = nre;
+        dim = nim;
         auto a = Mu[fidx(m, m)];
         auto b = Mv[fidx(m, m)];
-#pragma unroll
-        for (int t = 0; t < 2; t++) {
-            au[t] = a.x * dre[t] + a.y * dim[t];
-            av[t] = b.x * dre[t] + b.y * dim[t];
-        }
+        hu = fma(a.x, dre, hu);
+        hu = fma(a.y, dim, hu);
+        hv = fma(b.x, dre, hv);
+        hv = fma(b.y, dim, hv);
         if (m + 1 <= P) {
-            R p2re[2], p2im[2], p1re[2], p1im[2], tz[2], bb[2], st[2];
-#pragma unroll
-            for (int t = 0; t < 2; t++) {
-                const R a1 = tm * zr[t];
-                p2re[t] = dre[t];
-                p2im[t] = dim[t];
-                p1re[t] = a1 * dre[t];
-                p1im[t] = a1 * dim[t];
-                tz[t] = (tm + R(2)) * zr[t];  // (2n-1) z / r^2 at n = m+2
-                bb[t] = tm * ir2[t];          // (n+m-1)(n-m-1) / r^2 at n = m+2
-                st[t] = (tm + R(2)) * ir2[t]; // its increment (2n-1) / r^2
-            }
+            const R a1 = tm * zr;
+            R p2re = dre, p2im = dim, p1re = a1 * dre, p1im = a1 * dim;
             a = Mu[fidx(m + 1, m)];
             b = Mv[fidx(m + 1, m)];
-#pragma unroll
-            for (int t = 0; t < 2; t++) {
-                au[t] += a.x * p1re[t] + a.y * p1im[t];
-                av[t] += b.x * p1re[t] + b.y * p1im[t];
-            }
+            hu = fma(a.x, p1re, hu);
+            hu = fma(a.y, p1im, hu);
+            hv = fma(b.x, p1re, hv);
+            hv = fma(b.y, p1im, hv);
+            R tz = (tm + R(2)) * zr, bb = tm * ir2, st = (tm + R(2)) * ir2;
 #pragma unroll
             for (int n = m + 2; n <= P; n++) {
+                const R cre = tz * p1re - bb * p2re, cim = tz * p1im - bb * p2im;
                 a = Mu[fidx(n, m)];
                 b = Mv[fidx(n, m)];
-#pragma unroll
-                for (int t = 0; t < 2; t++) {
-                    const R cre = tz[t] * p1re[t] - bb[t] * p2re[t], cim = tz[t] * p1im[t] - bb[t] * p2im[t];
-                    au[t] += a.x * cre + a.y * cim;
-                    av[t] += b.x * cre + b.y * cim;
-                    p2re[t] = p1re[t];
-                    p2im[t] = p1im[t];
-                    p1re[t] = cre;
-                    p1im[t] = cim;
-                    tz[t] += zr[t] + zr[t];
-                    bb[t] += st[t];
-                    st[t] += ir2[t] + ir2[t];
-                }
+                hu = fma(a.x, cre, hu);
+                hu = fma(a.y, cim, hu);
+                hv = fma(b.x, cre, hv);
+                hv = fma(b.y, cim, hv);
+                p2re = p1re;
+                p2im = p1im;
+                p1re = cre;
+                p1im = cim;
+                tz += zr2;
+                bb += st;
+                st += ir22;
             }
         }
-        if (m == 0) {
-            su0 += au[0];
-            sv0 += av[0];
-            su1 += au[1];
-            sv1 += av[1];
-        } else {
-            su0 += au[0] + au[0];
-            sv0 += av[0] + av[0];
-            su1 += au[1] + au[1];
-            sv1 += av[1] + av[1];
-        }
-        tm += R(2);
     }
+    su += au + (hu + hu);
+    sv += av + (hv + hv);
 }
 
 // L2P of two expansions at once: pu += sum L_u R, pv += sum L_v R. Not unrolled (it runs once per

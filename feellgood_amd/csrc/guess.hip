@@ -442,7 +442,8 @@ void guess_init(Ctx &c, uint64_t seed)
         galloc(&g->grad, NTRAIN);
         galloc(&g->feat, (size_t)c.N * NF);
         galloc(&g->lpart, 2 * ((c.N + GB - 1) / GB) + 3 * 1024);
-        if (hipHostMalloc((void **)&g->h_l, 3 * 1024 * sizeof(double), hipHostMallocDefault) != hipSuccess)
+        const size_t nl = std::max<size_t>(3 * 1024, 2 * (size_t)((c.N + GB - 1) / GB));
+        if (hipHostMalloc((void **)&g->h_l, nl * sizeof(double), hipHostMallocDefault) != hipSuccess)
             throw std::string("hipHostMalloc (guess)");
     }
     GuessNet *g = c.guess;

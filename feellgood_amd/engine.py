@@ -169,7 +169,8 @@ class Engine:
         if self.lib.fg_fmm_info(self._h, nv.lptr(a)) != nv.FG_OK:
             return None
         keys = ["leaf_level", "cells", "leaves", "groups", "list_entries", "m2p_pairs", "m2l_pairs",
-                "near_coefs", "near_columns", "targets", "sources", "order", "target_leaves", "m2m_cells"]
+                "near_coefs", "near_columns", "targets", "sources", "order", "target_leaves", "m2m_cells",
+                "m2p_slots"]
         return {k: int(a[i]) for i, k in enumerate(keys)}
 
     # ---- learned initial guess ----

@@ -230,7 +230,8 @@ int fg_guess_features(fg_ctx *ctx, float *feat);
 /* Octree FMM plan of an FG_DEMAG_FMM context (FG_ERR_STATE otherwise): info[0] leaf level,
  * [1] cells, [2] leaves, [3] evaluation groups, [4] source-list entries, [5] multipole-to-target
  * pairs, [6] M2L pairs, [7] near-field coefficients, [8] near-field shared columns, [9] targets,
- * [10] sources, [11] expansion order, [12] leaves with targets, [13] M2M parent cells. Used by the
+ * [10] sources, [11] expansion order, [12] leaves with targets, [13] M2M parent cells, [14] lane
+ * slots of the multipole-to-target launch (pairs / slots = its SIMD efficiency). Used by the
  * benchmark's work models. */
 int fg_fmm_info(fg_ctx *ctx, int64_t info[16]);
 /* Wait for all work queued on the context's stream. */
