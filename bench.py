@@ -88,6 +88,7 @@ def bench_dots(args):
     def make(leaf, profile=False):
         return Engine.batch([m.mesh for m in meshes], options(tol=w["tol"], max_iter=10000, demag="fmm",
                                                               fmm_order=args.fmm_order, fmm_leaf=leaf,
+                                                              fmm_far_order=args.fmm_far_order,
                                                               precision=prec, profile=profile))
     leaf, tune = tune_fmm_leaf(make, u0)
     print(f"[bench] autotuned fmm_leaf {leaf}: {tune}", file=sys.stderr, flush=True)
@@ -166,14 +167,25 @@ def prefactor(dt):
 
 # ------------------------------------------------------------------ byte / flop models
 def m2p_flops(p):
-    """FP64 operations of one multipole-to-target evaluation of two expansions of order p (the
-    streamed irregular recurrence of m2p_eval2_t plus the caller's scaling)."""
-    f = 5 + 1 + 2                      # r^2, 1/r^2, 1/r
-    f += 7 * p                         # diagonal recurrence I_m^m, m = 1..p
-    f += 4 * p                         # first off-diagonal I_{m+1}^m
-    f += 9 * sum(max(0, p - m - 1) for m in range(p + 1))  # column recurrence, n >= m+2
-    f += 10 * (p + 1) * (p + 2) // 2   # two dot products, weighted
-    return f + 10                      # offset, scaling and accumulation
+    """FP64 (or FP32) operations of one multipole-to-target evaluation of two expansions of degree
+    p, as m2p_eval2_r performs them: setup (r^2, refined 1/r^2 and 1/r), the real m = 0 column, the
+    complex columns m >= 1 (diagonal step, first off-diagonal, three-term recurrence) with four
+    fused multiply-adds per term, and the caller's offset and scaling."""
+    f = 25                                              # setup
+    f += 2 + (5 if p >= 1 else 0) + 10 * max(0, p - 1)  # m = 0 column
+    for m in range(1, p + 1):
+        f += 18 + (15 if m < p else 0) + 17 * max(0, p - m - 1)
+    return f + 14
+
+
+FAR_DEGREE = {4: 2, 6: 4, 8: 5, 10: 6}  # lower degree of the far list entries, per order
+
+
+def far_degree(p, far_order, precision):
+    """Degree the far list entries are evaluated to (fg_options.fmm_far_order semantics)."""
+    if far_order:
+        return far_order
+    return FAR_DEGREE[p] if precision == "mixed" else p
 
 
 def m2l_flops(p):
@@ -228,7 +240,9 @@ def kernel_model(name, n):
             # coefficient rows + shared columns + element values read once + per-target I/O
             "fmm_near": ("hbm", 8 * f["near_coefs"] + 4 * f["near_columns"] + 16 * (T + 3 * F)
                          + (8 + 16 + 16 + 4) * Tg + 12 * f["target_leaves"]),
-            "fmm_m2p": ("valu32" if n.get("precision") == "mixed" else "valu64", m2p_flops(p) * f["m2p_pairs"]),
+            "fmm_m2p": ("valu32" if n.get("precision") == "mixed" else "valu64",
+                        m2p_flops(p) * (f["m2p_pairs"] - f["m2p_far_pairs"])
+                        + m2p_flops(far_degree(p, n.get("far_order"), n.get("precision"))) * f["m2p_far_pairs"]),
             "fmm_m2l": ("valu64", m2l_flops(p) * f["m2l_pairs"]),
             "fmm_m2m": ("valu64", m2m_flops(p) * (f["cells"] - f["leaves"])),
             "fmm_p2m": ("valu64", p2m_flops(p) * f["sources"]),
@@ -239,7 +253,7 @@ def kernel_model(name, n):
 def make_engine(args, w, m, precision, guess, profile):
     from feellgood_amd.engine import Engine, options
     opts = options(tol=w["tol"], max_iter=10000, demag=args.demag, fmm_order=args.fmm_order,
-                   fmm_leaf=args.fmm_leaf, precision=precision, profile=profile)
+                   fmm_leaf=args.fmm_leaf, precision=precision, profile=profile, fmm_far_order=args.fmm_far_order)
     g = Engine(m.mesh, opts)
     if guess is not None:
         g.guess_set(guess)
@@ -452,6 +466,8 @@ def main():
     ap.add_argument("--demag", default="fmm", choices=["fmm", "direct"])
     ap.add_argument("--fmm-order", type=int, default=8)
     ap.add_argument("--fmm-leaf", type=int, default=64)
+    ap.add_argument("--fmm-far-order", type=int, default=0,
+                    help="degree of the far source cells (0: the order in FP64, the lower far degree in mixed)")
     ap.add_argument("--guess", default=None, help="network parameters (.npy); default: the committed ones")
     ap.add_argument("--prof-steps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
@@ -510,6 +526,7 @@ def main():
     sizes["nnz"] = nz.value // 4
     sizes["fmm"] = res["fmm"]
     sizes["precision"] = precision
+    sizes["far_order"] = args.fmm_far_order
     rl, per_kernel = roofline(res["stats"], sizes)
     if rl and rl.get("bound"):
         rl["traffic"] = measured_traffic(rl["kernel"], args)

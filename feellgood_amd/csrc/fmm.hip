@@ -25,7 +25,7 @@
 using namespace fg;
 
 struct FmmPlan {
-    int p = 8, L = 2, nc = 0, ni = 0;
+    int p = 8, pf = 6, L = 2, nc = 0, ni = 0;  // pf: degree of the far entries of the group lists
     double H = 0, C[3] = {0, 0, 0};
     std::vector<double> W;            // cell width per level
     std::vector<int> lev_off;         // first cell of each level (size L+2)
@@ -41,9 +41,12 @@ struct FmmPlan {
     // group's targets evaluate the local expansion of its level-(L-2) ancestor (lcell) and the
     // multipoles of its source list (the cell's interaction list and the children of its
     // neighbours) directly; u_mask bit o says whether a list entry applies to the targets in child
-    // octant o (entries adjacent to that child belong to the near field).
+    // octant o (entries adjacent to that child belong to the near field). Each group's list holds
+    // first the entries at Chebyshev distance 2 (in widths of the entry's level) from some of the
+    // targets they serve, evaluated to degree p, then from u_far[g] on those at distance 3 from all
+    // of them, evaluated to degree pf.
     int ls = 2;
-    std::vector<int32_t> grp_tb, grp_te, grp_lcell, u_ptr, u_cell;
+    std::vector<int32_t> grp_tb, grp_te, grp_lcell, u_ptr, u_cell, u_far;
     std::vector<uint8_t> u_mask, tgt_oct;
     std::vector<int32_t> tgt_grp;
     // sorted points
@@ -60,7 +63,8 @@ struct FmmPlan {
     int32_t *d_parent = nullptr, *d_oct = nullptr, *d_ch_ptr = nullptr, *d_ch_idx = nullptr;
     int32_t *d_il_ptr = nullptr, *d_il_src = nullptr, *d_il_off = nullptr;
     int32_t *d_leaf_sb = nullptr, *d_leaf_se = nullptr;
-    int32_t *d_grp_tb = nullptr, *d_grp_te = nullptr, *d_grp_lcell = nullptr, *d_u_ptr = nullptr, *d_u_cell = nullptr;
+    int32_t *d_grp_tb = nullptr, *d_grp_te = nullptr, *d_grp_lcell = nullptr, *d_u_ptr = nullptr, *d_u_cell = nullptr,
+            *d_u_far = nullptr;
     uint8_t *d_u_mask = nullptr, *d_tgt_oct = nullptr;
     double2 *d_Itf = nullptr;  // irregular table with both signs of m: [343][(2p+1)^2]
     double2 *d_Rcf = nullptr;  // regular child-offset table with both signs of m: [8][(p+1)^2]
@@ -76,7 +80,7 @@ struct FmmPlan {
     bool mixed = false;
     double mscale = 1.0;         // multipoles times mscale (a power of two) in the FP32 pass
     int64_t m2p_pairs = 0, m2l_pairs = 0, lc_cols = 0;
-    int64_t m2p_slots = 0;  // lane slots the M2P launch mapping executes (useful pairs / slots = SIMD efficiency)
+    int64_t m2p_slots = 0, m2p_far_pairs = 0;  // lane slots the M2P launch mapping executes (useful pairs / slots = SIMD efficiency)
     int64_t nf_nnz = 0;
     int32_t *d_tgt_node = nullptr;
     double4 *d_tgt_pos = nullptr;
@@ -119,12 +123,20 @@ inline void unmorton(uint32_t c, int l, int &ix, int &iy, int &iz)
 
 namespace fg {
 
+// Lower degree of the far list entries (Chebyshev distance 3 from every target they serve): the
+// truncation error of a cell three widths away at this degree stays below that of the cells two
+// widths away at degree p (the film's FMM error grows by at most 5 % over one degree everywhere).
+constexpr int fmm_far_degree(int p) { return p <= 6 ? p - 2 : p == 8 ? 5 : 6; }
+
 // Host construction of the plan from source positions (original order) and target nodes.
 std::string fmm_build_host(FmmPlan &P, const std::vector<double4> &src, const std::vector<int32_t> &tnode,
-                           const std::vector<double4> &tpos, int p, int leaf_target)
+                           const std::vector<double4> &tpos, int p, int pf, int leaf_target)
 {
     if (p != 4 && p != 6 && p != 8 && p != 10) return "fmm_order must be 4, 6, 8 or 10";
+    if (pf == 0) pf = fmm_far_degree(p);  // callers resolve the precision-dependent default
+    if (pf != p && pf != fmm_far_degree(p)) return "fmm_far_order must be 0, fmm_order or the default far degree";
     P.p = p;
+    P.pf = pf;
     P.nc = fcount(p);
     P.ni = fcount(2 * p);
     const size_t S = src.size(), T = tpos.size();
@@ -370,9 +382,12 @@ std::string fmm_build_host(FmmPlan &P, const std::vector<double4> &src, const st
             P.grp_tb.push_back((int32_t)k);
             P.grp_te.push_back((int32_t)e);
             P.grp_lcell.push_back(L - 2 >= 2 ? P.parent[gci] : -1);
+            std::vector<int32_t> far_cell;
+            std::vector<uint8_t> far_mask;
             for (int q = P.il_ptr[gci]; q < P.il_ptr[gci + 1]; q++) {
-                P.u_cell.push_back(P.il_src[q]);
-                P.u_mask.push_back(0xff);
+                const int o = P.il_off[q], d = std::max({std::abs(o / 49 - 3), std::abs(o / 7 % 7 - 3), std::abs(o % 7 - 3)});
+                (d >= 3 ? far_cell : P.u_cell).push_back(P.il_src[q]);
+                (d >= 3 ? far_mask : P.u_mask).push_back(0xff);
             }
             int px, py, pz;
             unmorton(gc, gl, px, py, pz);
@@ -386,17 +401,29 @@ std::string fmm_build_host(FmmPlan &P, const std::vector<double4> &src, const st
                             if (sx >= nL || sy >= nL || sz >= nL) continue;
                             auto it = index[L].find(morton(sx, sy, sz));
                             if (it == index[L].end() || !P.has_src[it->second]) continue;
-                            uint8_t mask = 0;
+                            uint8_t mn = 0, mf = 0;  // octants at distance 2, at distance 3
                             for (int t = 0; t < 8; t++) {
                                 const int tx = 2 * px + ((t >> 2) & 1), ty = 2 * py + ((t >> 1) & 1), tz = 2 * pz + (t & 1);
-                                if (std::abs(tx - sx) > 1 || std::abs(ty - sy) > 1 || std::abs(tz - sz) > 1)
-                                    mask |= (uint8_t)(1u << t);
+                                const int d = std::max({std::abs(tx - sx), std::abs(ty - sy), std::abs(tz - sz)});
+                                if (d >= 3)
+                                    mf |= (uint8_t)(1u << t);
+                                else if (d == 2)
+                                    mn |= (uint8_t)(1u << t);
                             }
-                            if (!mask) continue;
-                            P.u_cell.push_back(it->second);
-                            P.u_mask.push_back(mask);
+                            // an entry near to some of its octants is evaluated to degree p for all
+                            // of them (splitting it would stage the cell twice)
+                            if (mn) {
+                                P.u_cell.push_back(it->second);
+                                P.u_mask.push_back(mn | mf);
+                            } else if (mf) {
+                                far_cell.push_back(it->second);
+                                far_mask.push_back(mf);
+                            }
                         }
                     }
+            P.u_far.push_back((int32_t)P.u_cell.size());
+            P.u_cell.insert(P.u_cell.end(), far_cell.begin(), far_cell.end());
+            P.u_mask.insert(P.u_mask.end(), far_mask.begin(), far_mask.end());
             P.u_ptr.push_back((int32_t)P.u_cell.size());
             k = e;
         }
@@ -441,6 +468,36 @@ std::string fmm_build_host(FmmPlan &P, const std::vector<double4> &src, const st
     return "";
 }
 
+// Host P2M of one leaf in the device's form: both column halves of p2m_accumulate_cols, then the
+// factorial scale.
+template <int P, unsigned MASK>
+static void p2m_half_host(const FmmPlan &Pl, int li, int ci, const std::vector<double> (&qs)[2], std::vector<double2> &M)
+{
+    constexpr int NCo = fcount(P), NH = p2m_half_count<P>(MASK);
+    const double4 c = Pl.cen[ci];
+    const double iw = 1.0 / Pl.W[Pl.L];
+    for (int s = 0; s < 2; s++) {
+        double acc[2 * NH] = {};
+        for (int k = Pl.leaf_sb[li]; k < Pl.leaf_se[li]; k++) {
+            const double4 x = Pl.src_sorted[k];
+            p2m_accumulate_cols<P, MASK>((x.x - c.x) * iw, (x.y - c.y) * iw, (x.z - c.z) * iw, qs[s][k], acc);
+        }
+        for (int m = 0; m <= P; m++) {
+            if (!((MASK >> m) & 1)) continue;
+            const int o = p2m_col_off<P>(MASK, m);
+            for (int n = m; n <= P; n++)
+                M[((size_t)ci * 2 + s) * NCo + fidx(n, m)] =
+                    make_double2(acc[2 * (o + n - m)] * p2m_scale(n, m), acc[2 * (o + n - m) + 1] * p2m_scale(n, m));
+        }
+    }
+}
+template <int P>
+static void p2m_leaf_host(const FmmPlan &Pl, int li, int ci, const std::vector<double> (&qs)[2], std::vector<double2> &M)
+{
+    p2m_half_host<P, p2m_half_mask<P>(0)>(Pl, li, ci, qs, M);
+    p2m_half_host<P, p2m_half_mask<P>(1)>(Pl, li, ci, qs, M);
+}
+
 // Host evaluation of the whole FMM (reference for the device pass): qs[2][S] in the original
 // source order; out[2][T] potentials sum q/r at the targets in the original target order.
 void fmm_eval_host(const FmmPlan &P, const double *qu, const double *qv, std::vector<double> &pot_u,
@@ -454,21 +511,15 @@ void fmm_eval_host(const FmmPlan &P, const double *qu, const double *qv, std::ve
         qs[1][P.src_rank[i]] = qv[i];
     }
     std::vector<double2> M((size_t)NC * 2 * nc, make_double2(0, 0)), Lx((size_t)NC * 2 * nc, make_double2(0, 0));
-    std::vector<double2> R(nc);
     const int NL = P.lev_off[L + 1] - P.lev_off[L];
     for (int li = 0; li < NL; li++) {
         const int ci = P.lev_off[L] + li;
-        const double4 c = P.cen[ci];
-        const double iw = 1.0 / P.W[L];
-        for (int k = P.leaf_sb[li]; k < P.leaf_se[li]; k++) {
-            const double4 x = P.src_sorted[k];
-            regular((x.x - c.x) * iw, (x.y - c.y) * iw, (x.z - c.z) * iw, p, R.data());
-            for (int s = 0; s < 2; s++)
-                for (int q = 0; q < nc; q++) {
-                    double2 &m = M[((size_t)ci * 2 + s) * nc + q];
-                    m.x += qs[s][k] * R[q].x;
-                    m.y += qs[s][k] * R[q].y;
-                }
+        switch (p) {
+        case 4: p2m_leaf_host<4>(P, li, ci, qs, M); break;
+        case 6: p2m_leaf_host<6>(P, li, ci, qs, M); break;
+        case 8: p2m_leaf_host<8>(P, li, ci, qs, M); break;
+        case 10: p2m_leaf_host<10>(P, li, ci, qs, M); break;
+        default: break;
         }
     }
     for (int l = L - 1; l >= 2; l--)
@@ -533,8 +584,8 @@ void fmm_eval_host(const FmmPlan &P, const double *qu, const double *qv, std::ve
             const double4 c = P.cen[sc];
             const double iw = 1.0 / P.W[(int)c.w];
             double su = 0, sv = 0;
-            m2p_eval2(&M[(size_t)sc * 2 * nc], &M[((size_t)sc * 2 + 1) * nc], p, (y.x - c.x) * iw, (y.y - c.y) * iw,
-                      (y.z - c.z) * iw, su, sv);
+            m2p_eval2(&M[(size_t)sc * 2 * nc], &M[((size_t)sc * 2 + 1) * nc], k < P.u_far[g] ? p : P.pf,
+                      (y.x - c.x) * iw, (y.y - c.y) * iw, (y.z - c.z) * iw, su, sv);
             acc[0] += su * iw;
             acc[1] += sv * iw;
         }
@@ -563,6 +614,7 @@ void fmm_merge(std::vector<FmmPlan> &parts, const std::vector<std::vector<int64_
 {
     const FmmPlan &P0 = parts[0];
     M.p = P0.p;
+    M.pf = P0.pf;
     M.nc = P0.nc;
     M.ni = P0.ni;
     M.It = P0.It;
@@ -623,6 +675,7 @@ void fmm_merge(std::vector<FmmPlan> &parts, const std::vector<std::vector<int64_
             M.grp_tb.push_back(P.grp_tb[g] + to);
             M.grp_te.push_back(P.grp_te[g] + to);
             M.grp_lcell.push_back(P.grp_lcell[g] >= 0 ? P.grp_lcell[g] + co : -1);
+            M.u_far.push_back(P.u_far[g] - P.u_ptr[g] + (int32_t)M.u_cell.size());
             for (int e = P.u_ptr[g]; e < P.u_ptr[g + 1]; e++) {
                 M.u_cell.push_back(P.u_cell[e] + co);
                 M.u_mask.push_back(P.u_mask[e]);
@@ -667,43 +720,73 @@ __device__ inline int xcd_block(int b, int nb)
     return x * q + min(x, r) + (b >> 3);
 }
 
-// P2M: a group of G lanes per source leaf, each lane a strided subset of the leaf's sources; the
-// lanes' accumulators are combined with cross-lane sums. blockIdx.y = charge set.
-template <int P, int G>
+// Sum over each aligned group of 8 lanes, left in every lane of the group: data-parallel-primitive
+// moves (swap within pairs, swap pairs within quads, mirror within the half row) of the two
+// 32-bit halves, no LDS round trip.
+template <int CTRL>
+__device__ __forceinline__ double dpp_mov(double v)
+{
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)(b & 0xffffffffll), CTRL, 0xf, 0xf, true);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xf, 0xf, true);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+__device__ __forceinline__ double sum8_dpp(double v)
+{
+    v += dpp_mov<0xB1>(v);   // quad_perm [1,0,3,2]
+    v += dpp_mov<0x4E>(v);   // quad_perm [2,3,0,1]
+    v += dpp_mov<0x141>(v);  // row_half_mirror
+    return v;
+}
+
+// P2M: wavefronts over 64 / G source leaves; G lanes per leaf, each a strided subset of the leaf's
+// sources. Launch H accumulates the coefficient columns of half H (a balanced split,
+// p2m_half_mask), so each lane holds about half of one expansion; the lanes' sums are combined
+// with cross-lane sums. blockIdx.y = charge set.
+template <int P, int G, unsigned MASK>
+__device__ __forceinline__ void p2m_half(int li, int lane, int set, bool act, int cell, const double4 *__restrict__ cen,
+                                         const int32_t *__restrict__ sb, const int32_t *__restrict__ se,
+                                         const double4 *__restrict__ spos, const double2 *__restrict__ q,
+                                         double2 *__restrict__ M)
+{
+    constexpr int NCo = fcount(P), NH = p2m_half_count<P>(MASK);
+    double acc[2 * NH];
+#pragma unroll
+    for (int k = 0; k < 2 * NH; k++) acc[k] = 0.0;
+    const double4 c = cen[cell];
+    const double iw = c.w;
+    const int kb = act ? sb[li] + lane : 0, ke = act ? se[li] : 0;
+    for (int k = kb; k < ke; k += G) {
+        const double4 x = spos[k];
+        const double2 qq = q[k];
+        p2m_accumulate_cols<P, MASK>((x.x - c.x) * iw, (x.y - c.y) * iw, (x.z - c.z) * iw, set ? qq.y : qq.x, acc);
+    }
+    static_assert(G == 8, "the cross-lane sum spans 8 lanes");
+#pragma unroll
+    for (int k = 0; k < 2 * NH; k++) acc[k] = sum8_dpp(acc[k]);
+    if (act && lane == 0) {
+        double2 *dst = M + ((size_t)cell * 2 + set) * NCo;
+#pragma unroll
+        for (int m = 0; m <= P; m++) {
+            if (!((MASK >> m) & 1)) continue;
+            const int o = p2m_col_off<P>(MASK, m);
+#pragma unroll
+            for (int n = m; n <= P; n++)
+                dst[fidx(n, m)] = make_double2(acc[2 * (o + n - m)] * p2m_scale(n, m), acc[2 * (o + n - m) + 1] * p2m_scale(n, m));
+        }
+    }
+}
+
+template <int P, int G, int H>
 __global__ __launch_bounds__(64) void k_fmm_p2m(int nleaf, const int32_t *__restrict__ leaf_cell,
                                                 const double4 *__restrict__ cen, const int32_t *__restrict__ sb,
                                                 const int32_t *__restrict__ se, const double4 *__restrict__ spos,
                                                 const double2 *__restrict__ q, double2 *__restrict__ M)
 {
-    constexpr int NCo = fcount(P);
     const int lane = threadIdx.x & (G - 1);
     const int li = (blockIdx.x * 64 + threadIdx.x) / G;
-    const int set = blockIdx.y;
-    double acc[2 * NCo];
-#pragma unroll
-    for (int k = 0; k < 2 * NCo; k++) acc[k] = 0.0;
     const bool act = li < nleaf;
-    const int cell = act ? leaf_cell[li] : 0;
-    if (act) {
-        const double4 c = cen[cell];
-        const double iw = c.w;
-        for (int k = sb[li] + lane; k < se[li]; k += G) {
-            const double4 x = spos[k];
-            const double2 qq = q[k];
-            p2m_accumulate<P>((x.x - c.x) * iw, (x.y - c.y) * iw, (x.z - c.z) * iw, set ? qq.y : qq.x, acc);
-        }
-    }
-#pragma unroll
-    for (int k = 0; k < 2 * NCo; k++) {
-        double v = acc[k];
-        for (int o = G / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-        acc[k] = v;
-    }
-    if (act && lane == 0) {
-        double2 *dst = M + ((size_t)cell * 2 + set) * NCo;
-#pragma unroll
-        for (int k = 0; k < NCo; k++) dst[k] = make_double2(acc[2 * k], acc[2 * k + 1]);
-    }
+    p2m_half<P, G, p2m_half_mask<P>(H)>(li, lane, blockIdx.y, act, act ? leaf_cell[li] : 0, cen, sb, se, spos, q, M);
 }
 
 // M2M: one block of 4 wavefronts per parent cell with sources. The children's expansions are
@@ -971,14 +1054,52 @@ __global__ __launch_bounds__(256) void k_nf_apply(int ntl, const int32_t *__rest
 // targets in the current chunk of at most 64, nsl = min(256 / n, CH) slices, and the group's source
 // list is walked in stages of nsl cells, one cell per slice, whose multipoles are staged in LDS
 // (in the real type R: FP64, or FP32 scaled by mscale in the mixed-precision pass), so that every
-// stage keeps n * nsl of the 256 lanes busy. Each target's slices are summed through LDS in FP64,
-// then the L2P of the group's level-(L-2) ancestor is added and phi, phi_v are written.
+// stage keeps n * nsl of the 256 lanes busy. Entries from u_far[g] on are evaluated to degree PF.
+// Each target's slices are summed through LDS in FP64, then the L2P of the group's level-(L-2)
+// ancestor is added and phi, phi_v are written.
 template <class R> constexpr int m2p_stage_cells() { return sizeof(R) == 8 ? 16 : 32; }
 
-template <int P, class R>
+// Stages of list entries [b, e) of a group, evaluated to degree Q <= P (expansions stored to P):
+// the first fcount(Q) coefficients of each set are staged, one cell per slice.
+template <int P, int Q, class R>
+__device__ __forceinline__ void m2p_stages(int b, int e, int nsl, int slice, int oc, double4 y,
+                                           const int32_t *__restrict__ u_cell, const uint8_t *__restrict__ u_mask,
+                                           const double4 *__restrict__ cen, const double2 *__restrict__ M,
+                                           double mscale, typename vec2_of<R>::type *sM, double4 *sC, int *sMask,
+                                           double &pu, double &pv)
+{
+    constexpr int NCo = fcount(P), NQ = fcount(Q), CWQ = 2 * NQ, NT = 256;
+    const int tid = threadIdx.x;
+    for (int cb = b; cb < e; cb += nsl) {
+        const int nch = min(nsl, e - cb);
+        for (int i = tid; i < nch * CWQ; i += NT) {
+            const int jj = i / CWQ, r = i - jj * CWQ, set = r >= NQ ? 1 : 0;
+            const double2 v = M[((size_t)u_cell[cb + jj] * 2 + set) * NCo + (r - set * NQ)];
+            sM[i].x = (R)(v.x * mscale);
+            sM[i].y = (R)(v.y * mscale);
+        }
+        if (tid < nch) {
+            sC[tid] = cen[u_cell[cb + tid]];
+            sMask[tid] = u_mask[cb + tid];
+        }
+        __syncthreads();
+        if (slice < nch && ((sMask[slice] >> oc) & 1)) {
+            const double4 c = sC[slice];
+            R su = 0, sv = 0;
+            m2p_eval2_r<Q, R>(sM + slice * CWQ, sM + slice * CWQ + NQ, (R)((y.x - c.x) * c.w), (R)((y.y - c.y) * c.w),
+                              (R)((y.z - c.z) * c.w), su, sv);
+            pu += (double)su * c.w;
+            pv += (double)sv * c.w;
+        }
+        __syncthreads();
+    }
+}
+
+template <int P, int PF, class R>
 __global__ __launch_bounds__(256) void k_fmm_m2p(const int32_t *__restrict__ grp_tb, const int32_t *__restrict__ grp_te,
                                                  const int32_t *__restrict__ grp_lcell,
-                                                 const int32_t *__restrict__ u_ptr, const int32_t *__restrict__ u_cell,
+                                                 const int32_t *__restrict__ u_ptr, const int32_t *__restrict__ u_far,
+                                                 const int32_t *__restrict__ u_cell,
                                                  const uint8_t *__restrict__ u_mask,
                                                  const uint8_t *__restrict__ tgt_oct, const double4 *__restrict__ tpos,
                                                  const double4 *__restrict__ cen,
@@ -994,7 +1115,7 @@ __global__ __launch_bounds__(256) void k_fmm_m2p(const int32_t *__restrict__ grp
     __shared__ double2 red[NT];
     const int g = xcd_block(blockIdx.x, gridDim.x);
     const int tid = threadIdx.x;
-    const int tb = grp_tb[g], te = grp_te[g], ub = u_ptr[g], ue = u_ptr[g + 1], lc = grp_lcell[g];
+    const int tb = grp_tb[g], te = grp_te[g], ub = u_ptr[g], ue = u_ptr[g + 1], uf = u_far[g], lc = grp_lcell[g];
     for (int t0 = tb; t0 < te; t0 += 64) {
         const int n = min(64, te - t0);
         const int nsl = min(NT / n, CH);
@@ -1006,28 +1127,11 @@ __global__ __launch_bounds__(256) void k_fmm_m2p(const int32_t *__restrict__ grp
             oc = tgt_oct[t0 + slot];
         }
         double pu = 0, pv = 0;
-        for (int cb = ub; cb < ue; cb += nsl) {
-            const int nch = min(nsl, ue - cb);
-            for (int i = tid; i < nch * CW; i += NT) {
-                const int jj = i / CW;
-                const double2 v = M[(size_t)u_cell[cb + jj] * CW + (i - jj * CW)];
-                sM[i].x = (R)(v.x * mscale);
-                sM[i].y = (R)(v.y * mscale);
-            }
-            if (tid < nch) {
-                sC[tid] = cen[u_cell[cb + tid]];
-                sMask[tid] = u_mask[cb + tid];
-            }
-            __syncthreads();
-            if (slice < nch && ((sMask[slice] >> oc) & 1)) {
-                const double4 c = sC[slice];
-                R su = 0, sv = 0;
-                m2p_eval2_r<P, R>(sM + slice * CW, sM + slice * CW + NCo, (R)((y.x - c.x) * c.w),
-                                  (R)((y.y - c.y) * c.w), (R)((y.z - c.z) * c.w), su, sv);
-                pu += (double)su * c.w;
-                pv += (double)sv * c.w;
-            }
-            __syncthreads();
+        if constexpr (PF == P) {
+            m2p_stages<P, P, R>(ub, ue, nsl, slice, oc, y, u_cell, u_mask, cen, M, mscale, sM, sC, sMask, pu, pv);
+        } else {
+            m2p_stages<P, P, R>(ub, uf, nsl, slice, oc, y, u_cell, u_mask, cen, M, mscale, sM, sC, sMask, pu, pv);
+            m2p_stages<P, PF, R>(uf, ue, nsl, slice, oc, y, u_cell, u_mask, cen, M, mscale, sM, sC, sMask, pu, pv);
         }
         red[tid] = make_double2(pu, pv);
         __syncthreads();
@@ -1107,7 +1211,10 @@ int fmm_create(Ctx &c)
                     tnode.push_back((int32_t)i);
                     tpos.push_back(p4[i]);
                 }
-            std::string e = fmm_build_host(parts[k], s, tnode, tpos, c.opt.fmm_order, leaf);
+            // default far degree: one degree in FP64, the lower far degree in mixed precision
+            const int far = c.opt.fmm_far_order ? c.opt.fmm_far_order
+                                                : c.opt.precision == FG_PREC_MIXED ? 0 : c.opt.fmm_order;
+            std::string e = fmm_build_host(parts[k], s, tnode, tpos, c.opt.fmm_order, far, leaf);
             if (!e.empty()) throw e + " (instance " + std::to_string(k) + ")";
         }
         auto *P = new FmmPlan();
@@ -1140,6 +1247,7 @@ int fmm_create(Ctx &c)
         P->d_grp_te = dput(P->grp_te);
         P->d_grp_lcell = dput(P->grp_lcell);
         P->d_u_ptr = dput(P->u_ptr);
+        P->d_u_far = dput(P->u_far);
         P->d_u_cell = dput(P->u_cell);
         P->d_u_mask = dput(P->u_mask);
         P->d_tgt_oct = dput(P->tgt_oct);
@@ -1183,7 +1291,11 @@ int fmm_create(Ctx &c)
         // interaction counts (benchmark work models)
         for (size_t g = 0; g < P->grp_tb.size(); g++)
             for (int t = P->grp_tb[g]; t < P->grp_te[g]; t++)
-                for (int k = P->u_ptr[g]; k < P->u_ptr[g + 1]; k++) P->m2p_pairs += (P->u_mask[k] >> P->tgt_oct[t]) & 1;
+                for (int k = P->u_ptr[g]; k < P->u_ptr[g + 1]; k++) {
+                    const int on = (P->u_mask[k] >> P->tgt_oct[t]) & 1;
+                    P->m2p_pairs += on;
+                    if (k >= P->u_far[g]) P->m2p_far_pairs += on;
+                }
         P->m2p_slots = m2p_lane_slots(*P);
         for (int32_t ci : P->m2l_cells) P->m2l_pairs += P->il_ptr[ci + 1] - P->il_ptr[ci];
         return FG_OK;
@@ -1323,7 +1435,7 @@ void fmm_destroy(Ctx &c)
     if (!P) return;
     void *ptrs[] = {P->d_cen, P->d_parent, P->d_oct, P->d_ch_ptr, P->d_ch_idx, P->d_il_ptr, P->d_il_src,
                     P->d_il_off, P->d_leaf_sb, P->d_leaf_se, P->d_grp_tb, P->d_grp_te, P->d_grp_lcell, P->d_u_ptr,
-                    P->d_u_cell, P->d_u_mask, P->d_tgt_oct, P->d_tgt_node, P->d_tgt_pos, P->d_Rc, P->d_Itf,
+                    P->d_u_far, P->d_u_cell, P->d_u_mask, P->d_tgt_oct, P->d_tgt_node, P->d_tgt_pos, P->d_Rc, P->d_Itf,
                     P->d_Rcf, P->d_M, P->d_L, P->d_leaf_cell, P->d_m2m_cells, P->d_m2l_cells, P->d_l2l_cells,
                     P->d_nf_ptr, P->d_nf_val,
                     P->d_nf_valf, P->d_tl_tb, P->d_tl_te, P->d_lc_ptr, P->d_lc_col, P->d_pot, c.qe};
@@ -1354,8 +1466,12 @@ static void run_fmm(Ctx &c)
         Prof pr(c, "fmm_p2m");
         constexpr int G = 8;
         if (NL > 0)
-            k_fmm_p2m<PP, G><<<dim3((NL * G + 63) / 64, 2), 64, 0, c.stream>>>(NL, P.d_leaf_cell, P.d_cen, P.d_leaf_sb,
-                                                                               P.d_leaf_se, c.src_pos, c.src_q, P.d_M);
+        {
+            k_fmm_p2m<PP, G, 0><<<dim3((NL * G + 63) / 64, 2), 64, 0, c.stream>>>(NL, P.d_leaf_cell, P.d_cen, P.d_leaf_sb,
+                                                                                  P.d_leaf_se, c.src_pos, c.src_q, P.d_M);
+            k_fmm_p2m<PP, G, 1><<<dim3((NL * G + 63) / 64, 2), 64, 0, c.stream>>>(NL, P.d_leaf_cell, P.d_cen, P.d_leaf_sb,
+                                                                                  P.d_leaf_se, c.src_pos, c.src_q, P.d_M);
+        }
     }
     {
         Prof pr(c, "fmm_m2m");
@@ -1386,14 +1502,18 @@ static void run_fmm(Ctx &c)
     {
         Prof pr(c, "fmm_m2p");
         const int ng = (int)P.grp_tb.size();
-        if (ng > 0 && P.mixed)
-            k_fmm_m2p<PP, float><<<ng, 256, 0, c.stream>>>(P.d_grp_tb, P.d_grp_te, P.d_grp_lcell, P.d_u_ptr, P.d_u_cell,
-                                                           P.d_u_mask, P.d_tgt_oct, P.d_tgt_pos, P.d_cen, P.d_M, P.d_L,
-                                                           P.d_pot, P.d_tgt_node, c.rec[1 - c.cur], P.mscale);
-        else if (ng > 0)
-            k_fmm_m2p<PP, double><<<ng, 256, 0, c.stream>>>(P.d_grp_tb, P.d_grp_te, P.d_grp_lcell, P.d_u_ptr,
-                                                            P.d_u_cell, P.d_u_mask, P.d_tgt_oct, P.d_tgt_pos, P.d_cen,
-                                                            P.d_M, P.d_L, P.d_pot, P.d_tgt_node, c.rec[1 - c.cur], 1.0);
+        if (ng > 0) {
+            auto go = [&](auto kern, double ms) {
+                kern<<<ng, 256, 0, c.stream>>>(P.d_grp_tb, P.d_grp_te, P.d_grp_lcell, P.d_u_ptr, P.d_u_far, P.d_u_cell,
+                                               P.d_u_mask, P.d_tgt_oct, P.d_tgt_pos, P.d_cen, P.d_M, P.d_L, P.d_pot,
+                                               P.d_tgt_node, c.rec[1 - c.cur], ms);
+            };
+            const bool lowf = P.pf != PP;
+            if (P.mixed)
+                go(lowf ? k_fmm_m2p<PP, fmm_far_degree(PP), float> : k_fmm_m2p<PP, PP, float>, P.mscale);
+            else
+                go(lowf ? k_fmm_m2p<PP, fmm_far_degree(PP), double> : k_fmm_m2p<PP, PP, double>, 1.0);
+        }
     }
 }
 
@@ -1416,6 +1536,7 @@ void fmm_info(const Ctx &c, int64_t info[16])
     info[12] = P.ntl;
     info[13] = (int64_t)P.m2m_cells.size();  // M2M parent cells
     info[14] = P.m2p_slots;
+    info[15] = P.m2p_far_pairs;
 }
 
 void launch_fmm(Ctx &c)
@@ -1433,7 +1554,8 @@ void launch_fmm(Ctx &c)
 
 // ---- host reference of the FMM (tests) ----
 extern "C" int fg_fmm_reference(const double *src_xyz, int64_t S, const double *tgt_xyz, int64_t T, const double *q_u,
-                                const double *q_v, int order, int leaf, double *pot_u, double *pot_v, int *leaf_level)
+                                const double *q_v, int order, int far_order, int leaf, double *pot_u, double *pot_v,
+                                int *leaf_level)
 {
     try {
         std::vector<double4> src(S), tpos(T);
@@ -1444,7 +1566,7 @@ extern "C" int fg_fmm_reference(const double *src_xyz, int64_t S, const double *
             tnode[i] = (int32_t)i;
         }
         FmmPlan P;
-        std::string e = fmm_build_host(P, src, tnode, tpos, order, leaf);
+        std::string e = fmm_build_host(P, src, tnode, tpos, order, far_order, leaf);
         if (!e.empty()) return FG_ERR_ARG;
         std::vector<double> pu, pv;
         fmm_eval_host(P, q_u, q_v, pu, pv, tnode);

@@ -145,6 +145,100 @@ __host__ __device__ inline void p2m_accumulate(double x, double y, double z, dou
     }
 }
 
+// Column split of the P2M accumulators between two lanes: columns m (P + 1 - m coefficients each),
+// largest first, go to the half with the smaller total. p2m_half_mask<P>(h) is the bit set of
+// half h's columns, p2m_col_off the position of column m's first coefficient among them.
+template <int P>
+__host__ __device__ constexpr unsigned p2m_half_mask(int h)
+{
+    unsigned a = 0;
+    int na = 0, nb = 0;
+    for (int m = 0; m <= P; m++) {
+        if (na <= nb) {
+            a |= 1u << m;
+            na += P + 1 - m;
+        } else {
+            nb += P + 1 - m;
+        }
+    }
+    return h == 0 ? a : ((1u << (P + 1)) - 1) & ~a;
+}
+template <int P>
+__host__ __device__ constexpr int p2m_col_off(unsigned mask, int m)
+{
+    int o = 0;
+    for (int k = 0; k < m; k++)
+        if ((mask >> k) & 1) o += P + 1 - k;
+    return o;
+}
+template <int P>
+__host__ __device__ constexpr int p2m_half_count(unsigned mask)
+{
+    return p2m_col_off<P>(mask, P + 1);
+}
+
+// P2M accumulation of one charge into the columns of MASK only (acc holds their coefficients,
+// column m's at p2m_col_off, real and imaginary parts interleaved). It accumulates the unscaled
+// harmonics Q_n^m = (n+m)! (n-m)! R_n^m, whose recurrences have integer factors,
+//   Q_m^m = -(2m-1)(x+iy) Q_{m-1}^{m-1},  Q_{m+1}^m = (2m+1) z Q_m^m,
+//   Q_n^m = (2n-1) z Q_{n-1}^m - ((n-1)^2 - m^2) r^2 Q_{n-2}^m,
+// carried as running sums (no per-term constants to hold in registers); p2m_finish applies the
+// factorial scale once per expansion. The diagonal chain runs up to the highest column of MASK.
+template <int P, unsigned MASK>
+__host__ __device__ inline void p2m_accumulate_cols(double x, double y, double z, double q,
+                                                    double (&acc)[2 * p2m_half_count<P>(MASK)])
+{
+    const double r2 = x * x + y * y + z * z, z2 = z + z, r22 = r2 + r2;
+    double dre = q, dim = 0.0;
+    double tm = 1.0;  // -(2m - 1) once stepped
+#pragma unroll
+    for (int m = 0; m <= P; m++) {
+        if ((MASK >> m) == 0) break;
+        if (m > 0) {
+            tm -= 2.0;
+            const double nre = tm * (x * dre - y * dim), nim = tm * (x * dim + y * dre);
+            dre = nre;
+            dim = nim;
+        }
+        if (!((MASK >> m) & 1)) continue;
+        const int o = 2 * p2m_col_off<P>(MASK, m);
+        acc[o] += dre;
+        acc[o + 1] += dim;
+        if (m + 1 <= P) {
+            // (2m+1) z = -(tm - 2) z for m > 0, z for m = 0
+            const double a1 = (m == 0 ? 1.0 : 2.0 - tm) * z;
+            double p2re = dre, p2im = dim;
+            double p1re = a1 * dre, p1im = a1 * dim;
+            acc[o + 2] += p1re;
+            acc[o + 3] += p1im;
+            double az = a1 + z2;       // (2n-1) z at n = m+2
+            double kr2 = (double)(2 * m + 1) * r2, dk = (double)(2 * m + 3) * r2;  // ((n-1)^2 - m^2) r^2 and its step
+#pragma unroll
+            for (int n = m + 2; n <= P; n++) {
+                const double cre = az * p1re - kr2 * p2re, cim = az * p1im - kr2 * p2im;
+                acc[o + 2 * (n - m)] += cre;
+                acc[o + 2 * (n - m) + 1] += cim;
+                p2re = p1re;
+                p2im = p1im;
+                p1re = cre;
+                p1im = cim;
+                az += z2;
+                kr2 += dk;
+                dk += r22;
+            }
+        }
+    }
+}
+
+// Factorial scale of p2m_accumulate_cols' sums: coefficient (n, m) times 1 / ((n+m)! (n-m)!).
+__host__ __device__ constexpr double p2m_scale(int n, int m)
+{
+    double f = 1.0;
+    for (int k = 2; k <= n + m; k++) f *= k;
+    for (int k = 2; k <= n - m; k++) f *= k;
+    return 1.0 / f;
+}
+
 // L2P: sum_{n,m} L_n^m R_n^m(x), x in cell widths.
 __host__ __device__ inline double l2p_eval(const double2 *L, int p, double x, double y, double z)
 {

@@ -15,7 +15,7 @@ class EngineError(RuntimeError):
 
 
 def options(tol=1e-6, max_iter=700, demag="direct", fmm_order=8, fmm_leaf=64, precision="fp64",
-            device=0, profile=False):
+            device=0, profile=False, fmm_far_order=0):
     o = nv.Options()
     nv.hip_lib().fg_default_options(C.byref(o))
     o.tol = tol
@@ -26,6 +26,7 @@ def options(tol=1e-6, max_iter=700, demag="direct", fmm_order=8, fmm_leaf=64, pr
     o.precision = {"fp64": nv.FG_PREC_FP64, "mixed": nv.FG_PREC_MIXED}[precision]
     o.device = device
     o.profile = 1 if profile else 0
+    o.fmm_far_order = fmm_far_order
     return o
 
 
@@ -170,7 +171,7 @@ class Engine:
             return None
         keys = ["leaf_level", "cells", "leaves", "groups", "list_entries", "m2p_pairs", "m2l_pairs",
                 "near_coefs", "near_columns", "targets", "sources", "order", "target_leaves", "m2m_cells",
-                "m2p_slots"]
+                "m2p_slots", "m2p_far_pairs"]
         return {k: int(a[i]) for i, k in enumerate(keys)}
 
     # ---- learned initial guess ----

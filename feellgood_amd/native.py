@@ -49,7 +49,7 @@ class Mesh(C.Structure):
 class Options(C.Structure):
     _fields_ = [("tol", C.c_double), ("max_iter", C.c_int32), ("demag_method", C.c_int32),
                 ("fmm_order", C.c_int32), ("fmm_leaf", C.c_int32), ("precision", C.c_int32),
-                ("device", C.c_int32), ("profile", C.c_int32), ("reserved", C.c_int32)]
+                ("device", C.c_int32), ("profile", C.c_int32), ("fmm_far_order", C.c_int32)]
 
 
 class SolveInfo(C.Structure):
@@ -96,8 +96,8 @@ HIP_SYMBOLS = {
     "fg_guess_gradient": (C.c_int, [C.c_void_p, _f32, _d, _d]),
     "fg_guess_features": (C.c_int, [C.c_void_p, _f32]),
     "fg_synchronize": (C.c_int, [C.c_void_p]),
-    "fg_fmm_reference": (C.c_int, [_d, C.c_int64, _d, C.c_int64, _d, _d, C.c_int, C.c_int, _d, _d,
-                                   C.POINTER(C.c_int)]),
+    "fg_fmm_reference": (C.c_int, [_d, C.c_int64, _d, C.c_int64, _d, _d, C.c_int, C.c_int, C.c_int, _d,
+                                   _d, C.POINTER(C.c_int)]),
     "fg_last_error": (C.c_char_p, [C.c_void_p]),
 }
 

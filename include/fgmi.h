@@ -122,7 +122,11 @@ typedef struct fg_options {
     int32_t precision;     /* FG_PREC_FP64 or FG_PREC_MIXED */
     int32_t device;        /* HIP device ordinal */
     int32_t profile;       /* nonzero: time every kernel with HIP events */
-    int32_t reserved;
+    int32_t fmm_far_order; /* degree of the multipole evaluations of source cells three cell widths
+                              from all their targets: fmm_order (one degree), or the lower far
+                              degree 2, 4, 5, 6 for fmm_order 4, 6, 8, 10 (FMM error up by under
+                              5 %); 0 = default: fmm_order in FP64, the lower degree in
+                              FG_PREC_MIXED */
 } fg_options;
 
 /* Result of fg_solve (iteration<T> state, src/algebra/iter.h:36-179, and LinAlgebra::v_max). */
@@ -197,10 +201,11 @@ int fg_kernel_stats(fg_ctx *ctx, int k, char *name, int name_len, int64_t *launc
 int fg_reset_kernel_stats(fg_ctx *ctx);
 /* Host (CPU) evaluation of the same octree FMM as the device pass: potentials sum_s q_s/|x_t - y_s|
  * at T targets from S sources (xyz arrays of 3*n doubles), for two charge sets. No GPU needed;
- * used to check the device pass and the expansion order. leaf_level receives the tree depth. */
+ * used to check the device pass and the expansion orders (far_order as fg_options.fmm_far_order).
+ * leaf_level receives the tree depth. */
 int fg_fmm_reference(const double *src_xyz, int64_t S, const double *tgt_xyz, int64_t T,
-                     const double *q_u, const double *q_v, int order, int leaf, double *pot_u,
-                     double *pot_v, int *leaf_level);
+                     const double *q_u, const double *q_v, int order, int far_order, int leaf,
+                     double *pot_u, double *pot_v, int *leaf_level);
 /* ---- learned initial guess ----
  * No counterpart in the reference: when enabled, a per-node network (features u, the tangent
  * right-hand side over the lumped mass, the previous v, and their mean over mesh-edge neighbours;
@@ -231,8 +236,8 @@ int fg_guess_features(fg_ctx *ctx, float *feat);
  * [1] cells, [2] leaves, [3] evaluation groups, [4] source-list entries, [5] multipole-to-target
  * pairs, [6] M2L pairs, [7] near-field coefficients, [8] near-field shared columns, [9] targets,
  * [10] sources, [11] expansion order, [12] leaves with targets, [13] M2M parent cells, [14] lane
- * slots of the multipole-to-target launch (pairs / slots = its SIMD efficiency). Used by the
- * benchmark's work models. */
+ * slots of the multipole-to-target launch (pairs / slots = its SIMD efficiency), [15] pairs of [5]
+ * evaluated at the far degree. Used by the benchmark's work models. */
 int fg_fmm_info(fg_ctx *ctx, int64_t info[16]);
 /* Wait for all work queued on the context's stream. */
 int fg_synchronize(fg_ctx *ctx);

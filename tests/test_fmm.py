@@ -34,13 +34,13 @@ def charged(m, seed=1):
     return p, u, v
 
 
-def fmm_ref(xyz, qu, qv, tgt, order, leaf):
+def fmm_ref(xyz, qu, qv, tgt, order, leaf, far=0):
     pu = np.zeros(len(tgt))
     pv = np.zeros(len(tgt))
     L = C.c_int()
     rc = nv.hip_lib().fg_fmm_reference(nv.dptr(np.ascontiguousarray(xyz.reshape(-1))), len(qu),
                                        nv.dptr(np.ascontiguousarray(tgt.reshape(-1))), len(tgt),
-                                       nv.dptr(qu), nv.dptr(qv), order, leaf, nv.dptr(pu), nv.dptr(pv),
+                                       nv.dptr(qu), nv.dptr(qv), order, far, leaf, nv.dptr(pu), nv.dptr(pv),
                                        C.byref(L))
     assert rc == 0
     return pu, pv, L.value
@@ -63,6 +63,29 @@ def test_host_fmm_converges_with_order():
         errs.append(e)
     assert errs[0] < 1e-2 and errs[1] < 3e-3 and errs[2] < 6e-4
     assert errs[2] < errs[1] < errs[0]
+
+
+def test_host_fmm_far_degree():
+    """Source cells three widths away evaluated at the lower far degree: the error stays within
+    1.15x of the one-degree FMM at every order, and other far degrees are refused."""
+    m = film()
+    p, _, _ = charged(m)
+    xyz, qu, qv = p.sources()
+    cu, cv = p.corrections()
+    _, _, phi, phiv = p.get_state(nv.FG_NEXT)
+    tgt = m.arrays()["pos"]
+    exact = phi * 4 * np.pi - cu
+    for order in (6, 8, 10):
+        one, _, _ = fmm_ref(xyz, qu, qv, tgt, order, 64, far=order)
+        low, _, _ = fmm_ref(xyz, qu, qv, tgt, order, 64)
+        low2, _, _ = fmm_ref(xyz, qu, qv, tgt, order, 64, far={6: 4, 8: 5, 10: 6}[order])
+        assert np.array_equal(low, low2)
+        e1 = np.abs(one - exact).max() / np.abs(exact).max()
+        e2 = np.abs(low - exact).max() / np.abs(exact).max()
+        assert e2 <= 1.15 * e1, (order, e1, e2)
+        assert not np.array_equal(one, low)
+    with pytest.raises(AssertionError):
+        fmm_ref(xyz, qu, qv, tgt, 8, 64, far=6)
 
 
 def test_host_fmm_point_charges():
@@ -93,16 +116,17 @@ def test_device_fmm_matches_host_fmm_and_direct():
     cu, cv = p.corrections()
     _, _, phi0, phiv0 = p.get_state(nv.FG_NEXT)
     tgt = m.arrays()["pos"]
-    for order, leaf in ((8, 64), (6, 128), (10, 32)):
-        g = Engine(m.mesh, demag="fmm", fmm_order=order, fmm_leaf=leaf)
+    # FP64 default: one degree; the lower far degree on request (both loops of the far field)
+    for order, leaf, far in ((8, 64, 0), (8, 64, 5), (6, 128, 0), (10, 32, 0), (10, 32, 6)):
+        g = Engine(m.mesh, demag="fmm", fmm_order=order, fmm_leaf=leaf, fmm_far_order=far)
         g.set_state(nv.FG_BOTH, u=u, v=v)
         g.calc_demag()
         _, _, phi1, phiv1 = g.get_state(nv.FG_NEXT)
-        pu, pv, _ = fmm_ref(xyz, qu, qv, tgt, order, leaf)
+        pu, pv, _ = fmm_ref(xyz, qu, qv, tgt, order, leaf, far=far or order)
         ref_u = (pu + cu) / (4 * np.pi)
         ref_v = (pv + cv) / (4 * np.pi)
-        assert np.abs(phi1 - ref_u).max() <= 1e-10 * np.abs(ref_u).max(), (order, leaf)
-        assert np.abs(phiv1 - ref_v).max() <= 1e-10 * np.abs(ref_v).max(), (order, leaf)
+        assert np.abs(phi1 - ref_u).max() <= 1e-10 * np.abs(ref_u).max(), (order, leaf, far)
+        assert np.abs(phiv1 - ref_v).max() <= 1e-10 * np.abs(ref_v).max(), (order, leaf, far)
         err = np.abs(phi1 - phi0).max() / np.abs(phi0).max()
         assert err < {6: 3e-3, 8: 6e-4, 10: 2e-4}[order], (order, err)
         g.close()
