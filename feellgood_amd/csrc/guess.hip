@@ -36,17 +36,23 @@ constexpr int OFF_W1 = 0, OFF_B1 = OFF_W1 + NH * NX, OFF_W2 = OFF_B1 + NH, OFF_B
 constexpr int OFF_S = NTRAIN, NPARAM = NTRAIN + 3;
 static_assert(NPARAM == FG_GUESS_NPARAM, "parameter layout and header disagree");
 constexpr int GB = 128;  // threads per block of the per-node kernels
+constexpr int NXP = 20;  // inputs padded to whole MFMA k-steps of 4
 
-__device__ inline float gelu(float z)
+// GELU in its tanh form, written as 0.5 (1 + tanh(a)) = 1 / (1 + exp(-2a)) with
+// a = sqrt(2/pi) (z + 0.044715 z^3), evaluated by the hardware exp2 and reciprocal (about one ulp
+// each): exp(-2a) = 2^(c (z + 0.044715 z^3)), c = -2 sqrt(2/pi) log2(e).
+__device__ inline float gelu_sigma(float z)
 {
-    const float k = 0.7978845608028654f;  // sqrt(2/pi)
-    return 0.5f * z * (1.0f + tanhf(k * (z + 0.044715f * z * z * z)));
+    const float c = -2.3022082f;
+    const float u = fmaf(0.044715f * z * z, z, z);
+    return __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(c * u));
 }
+__device__ inline float gelu(float z) { return z * gelu_sigma(z); }
+// d/dz z s(z) = s + z s (1 - s) 2 sqrt(2/pi) (1 + 3 0.044715 z^2)
 __device__ inline float gelu_grad(float z)
 {
-    const float k = 0.7978845608028654f;
-    const float t = tanhf(k * (z + 0.044715f * z * z * z));
-    return 0.5f * (1.0f + t) + 0.5f * z * (1.0f - t * t) * k * (1.0f + 3.0f * 0.044715f * z * z);
+    const float s = gelu_sigma(z);
+    return s + z * s * (1.0f - s) * 1.5957691f * (1.0f + 0.134145f * z * z);
 }
 
 // round-to-nearest-even BF16 value of a float, returned as a float
@@ -75,6 +81,7 @@ struct GuessNet {
     float *gpart = nullptr;                  // per-block partial gradients
     double *lpart = nullptr;                 // per-block loss partials (2 per block)
     double *h_l = nullptr;                   // pinned
+    float *xq = nullptr;                     // [N][NXP] network inputs of the inference pass
 };
 
 namespace {
@@ -166,6 +173,166 @@ __device__ inline void gather_input(int i, const float *__restrict__ feat, const
     }
     const float ic = cnt > 0 ? 1.0f / (float)cnt : 0.0f;
     for (int k = 0; k < NF; k++) x[NF + k] = a[k] * ic * sc[k];
+}
+
+// Network inputs of the inference pass, padded to NXP (zero rows for non-magnetic nodes).
+__global__ __launch_bounds__(256) void k_guess_input(int N, const float *__restrict__ feat,
+                                                     const int32_t *__restrict__ mag,
+                                                     const int64_t *__restrict__ sptr, const int32_t *__restrict__ swid,
+                                                     const int32_t *__restrict__ scol, const float *__restrict__ p,
+                                                     float *__restrict__ xq)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    float x[NX] = {};
+    if (mag[i]) {
+        const float s[3] = {p[OFF_S], p[OFF_S + 1], p[OFF_S + 2]};
+        gather_input(i, feat, mag, sptr, swid, scol, s, x);
+    }
+    float4 *dst = reinterpret_cast<float4 *>(xq + (size_t)i * NXP);
+#pragma unroll
+    for (int k = 0; k < 4; k++) dst[k] = make_float4(x[4 * k], x[4 * k + 1], x[4 * k + 2], x[4 * k + 3]);
+    dst[4] = make_float4(x[16], x[17], 0.f, 0.f);
+}
+
+// Inference on the matrix cores. Each wavefront takes 64 nodes as four 16-node column tiles and
+// computes the layers transposed, Z^T = W X^T, with v_mfma_f32_16x16x4f32 (A: 16x4 weights, lane
+// l holding row l%16, k l/16; B: 4x16 activations, lane l holding k l/16, column l%16; D: lane l
+// holding rows 4(l/16)+j, j < 4, column l%16). Hidden layer k-steps are taken as the features
+// {16R + 4g + j : g < 4} for fixed (R, j), which are exactly the D registers (R, j) of the lanes
+// of group g = l/16: each layer's output feeds the next as its B operand without moving data.
+// The weight fragments are staged in LDS in lane order (one conflict-free read per fragment).
+// The three outputs (rows 0..2, lanes 0..15) go through LDS to one node per lane for v_pred.
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+struct MfmaNet {
+    float4 W2f[16 * 64];  // [(R2 * 4 + R) * 64 + l], j = 0..3
+    float W1f[20 * 64];   // [(R * 5 + s) * 64 + l]
+    float4 W3f[4 * 64];   // [R * 64 + l], j = 0..3
+    float b1[NH], b2[NH], b3[4], s[3];
+    float o[256 * NO];
+};
+
+__global__ __launch_bounds__(256) void k_guess_infer(int N, const float *__restrict__ xq,
+                                                     const float *__restrict__ feat, const int32_t *__restrict__ mag,
+                                                     const float *__restrict__ p, const uint16_t *__restrict__ wbf,
+                                                     const double4 *__restrict__ ep4, const double4 *__restrict__ eq4,
+                                                     double2 *__restrict__ xg, double4 *__restrict__ wg)
+{
+    __shared__ MfmaNet S;
+    const int tid = threadIdx.x, l = tid & 63, wv = tid >> 6, g4 = l >> 4, ln = l & 15;
+    for (int e = tid; e < 16 * 64; e += 256) {
+        const int f = e >> 6, q = e & 63, R2 = f >> 2, R = f & 3;
+        const int row = 16 * R2 + (q & 15), col = 16 * R + 4 * (q >> 4);
+        const uint16_t *w = wbf + OFF_W2 + row * NH + col;
+        S.W2f[e] = make_float4(bf16_to_float(w[0]), bf16_to_float(w[1]), bf16_to_float(w[2]), bf16_to_float(w[3]));
+    }
+    for (int e = tid; e < 20 * 64; e += 256) {
+        const int f = e >> 6, q = e & 63, R = f / 5, s = f - 5 * R, k = 4 * s + (q >> 4);
+        S.W1f[e] = k < NX ? bf16_to_float(wbf[OFF_W1 + (16 * R + (q & 15)) * NX + k]) : 0.f;
+    }
+    for (int e = tid; e < 4 * 64; e += 256) {
+        const int R = e >> 6, q = e & 63, o = q & 15, col = 16 * R + 4 * (q >> 4);
+        float v[4] = {0.f, 0.f, 0.f, 0.f};
+        if (o < NO)
+            for (int j = 0; j < 4; j++) v[j] = bf16_to_float(wbf[OFF_W3 + o * NH + col + j]);
+        S.W3f[e] = make_float4(v[0], v[1], v[2], v[3]);
+    }
+    if (tid < NH) {
+        S.b1[tid] = p[OFF_B1 + tid];
+        S.b2[tid] = p[OFF_B2 + tid];
+    }
+    if (tid < 4) S.b3[tid] = tid < NO ? p[OFF_B3 + tid] : 0.f;
+    if (tid < 3) S.s[tid] = p[OFF_S + tid];
+    __syncthreads();
+    for (int base = blockIdx.x * 256; base < N; base += gridDim.x * 256) {
+        const int wbase = base + 64 * wv;
+        // B operands of layer 1: inputs 4s + g4 of node wbase + 16T + ln
+        float xb[4][5];
+#pragma unroll
+        for (int T = 0; T < 4; T++) {
+            const int node = wbase + 16 * T + ln;
+#pragma unroll
+            for (int s = 0; s < 5; s++) xb[T][s] = node < N ? xq[(size_t)node * NXP + 4 * s + g4] : 0.f;
+        }
+        f32x4 h1[4][4];
+#pragma unroll
+        for (int R = 0; R < 4; R++) {
+            f32x4 a0;
+#pragma unroll
+            for (int j = 0; j < 4; j++) a0[j] = S.b1[16 * R + 4 * g4 + j];
+#pragma unroll
+            for (int T = 0; T < 4; T++) h1[R][T] = a0;
+#pragma unroll
+            for (int s = 0; s < 5; s++) {
+                const float a = S.W1f[(R * 5 + s) * 64 + l];
+#pragma unroll
+                for (int T = 0; T < 4; T++) h1[R][T] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, xb[T][s], h1[R][T], 0, 0, 0);
+            }
+#pragma unroll
+            for (int T = 0; T < 4; T++)
+#pragma unroll
+                for (int j = 0; j < 4; j++) h1[R][T][j] = gelu(h1[R][T][j]);
+        }
+        f32x4 h2[4][4];
+#pragma unroll
+        for (int R2 = 0; R2 < 4; R2++) {
+            f32x4 a0;
+#pragma unroll
+            for (int j = 0; j < 4; j++) a0[j] = S.b2[16 * R2 + 4 * g4 + j];
+#pragma unroll
+            for (int T = 0; T < 4; T++) h2[R2][T] = a0;
+#pragma unroll
+            for (int R = 0; R < 4; R++) {
+                const float4 a = S.W2f[(R2 * 4 + R) * 64 + l];
+                const float av[4] = {a.x, a.y, a.z, a.w};
+#pragma unroll
+                for (int j = 0; j < 4; j++)
+#pragma unroll
+                    for (int T = 0; T < 4; T++)
+                        h2[R2][T] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[j], h1[R][T][j], h2[R2][T], 0, 0, 0);
+            }
+#pragma unroll
+            for (int T = 0; T < 4; T++)
+#pragma unroll
+                for (int j = 0; j < 4; j++) h2[R2][T][j] = gelu(h2[R2][T][j]);
+        }
+        f32x4 o3[4];
+#pragma unroll
+        for (int T = 0; T < 4; T++)
+#pragma unroll
+            for (int j = 0; j < 4; j++) o3[T][j] = 4 * g4 + j < NO ? S.b3[4 * g4 + j] : 0.f;
+#pragma unroll
+        for (int R = 0; R < 4; R++) {
+            const float4 a = S.W3f[R * 64 + l];
+            const float av[4] = {a.x, a.y, a.z, a.w};
+#pragma unroll
+            for (int j = 0; j < 4; j++)
+#pragma unroll
+                for (int T = 0; T < 4; T++) o3[T] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[j], h2[R][T][j], o3[T], 0, 0, 0);
+        }
+        if (g4 == 0)
+#pragma unroll
+            for (int T = 0; T < 4; T++)
+#pragma unroll
+                for (int j = 0; j < NO; j++) S.o[(64 * wv + 16 * T + ln) * NO + j] = o3[T][j];
+        __syncthreads();
+        const int i = base + tid;
+        if (i < N && mag[i]) {
+            // v_pred = P_u (v + s_out o), then the guess in the tangent basis
+            const double u[3] = {feat[(size_t)i * NF], feat[(size_t)i * NF + 1], feat[(size_t)i * NF + 2]};
+            const double so = S.s[2];
+            double v[3];
+            for (int k = 0; k < 3; k++) v[k] = (double)feat[(size_t)i * NF + 6 + k] + so * (double)S.o[tid * NO + k];
+            const double un = u[0] * v[0] + u[1] * v[1] + u[2] * v[2];
+            for (int k = 0; k < 3; k++) v[k] -= un * u[k];
+            const double4 ep = ep4[i], eq = eq4[i];
+            const double2 xx = make_double2((v[0] * ep.x + v[1] * ep.y + v[2] * ep.z) / GAMMA0,
+                                            (v[0] * eq.x + v[1] * eq.y + v[2] * eq.z) / GAMMA0);
+            xg[i] = xx;
+            wg[i] = make_double4(xx.x * ep.x + xx.y * eq.x, xx.x * ep.y + xx.y * eq.y, xx.x * ep.z + xx.y * eq.z, 0.0);
+        }
+        __syncthreads();
+    }
 }
 
 // Forward pass per magnetic node. mode 0: write the guess x, w. mode 1 (training): store the
@@ -418,7 +585,7 @@ void guess_destroy(Ctx &c)
     GuessNet *g = c.guess;
     if (!g) return;
     void *ptrs[] = {g->params, g->wbf, g->adam_m, g->adam_v, g->grad, g->feat, g->xin, g->z1, g->z2,
-                    g->dz1, g->dz2, g->dout, g->gpart, g->lpart};
+                    g->dz1, g->dz2, g->dout, g->gpart, g->lpart, g->xq};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     if (g->h_l) (void)hipHostFree(g->h_l);
@@ -490,10 +657,15 @@ void guess_on_assembly(Ctx &c)
     }
     g->have_batch = false;  // the solution of this solve is not known yet
     if (g->enabled && g->scaled) {
+        if (!g->xq) galloc(&g->xq, (size_t)c.N * NXP);
+        {
+            Prof pr(c, "guess_input");
+            k_guess_input<<<(c.N + 255) / 256, 256, 0, c.stream>>>(c.N, g->feat, c.mag, c.slice_ptr, c.slice_w, c.scol,
+                                                                   g->params, g->xq);
+        }
         Prof pr(c, "guess_forward");
-        k_guess_forward<0><<<(c.N + GB - 1) / GB, GB, 0, c.stream>>>(
-            c.N, 0.0, g->feat, c.mag, c.slice_ptr, c.slice_w, c.scol, g->params, g->wbf, c.ep4, c.eq4, nullptr, c.x,
-            c.w, nullptr, nullptr, nullptr, nullptr, nullptr);
+        const int nb = std::min((c.N + 255) / 256, 2048);
+        k_guess_infer<<<nb, 256, 0, c.stream>>>(c.N, g->xq, g->feat, c.mag, g->params, g->wbf, c.ep4, c.eq4, c.x, c.w);
     }
 }
 

@@ -182,6 +182,53 @@ def test_features_forward_and_gradient_match_oracle():
 
 
 @pytest.mark.gpu
+def test_inference_pass_matches_oracle():
+    """The matrix-core inference pass that seeds the solve: with a tolerance the initial residual
+    already meets, the solve stops at iteration 0 and NEXT v is the guess, v_pred projected on the
+    tangent basis; compared with the oracle's network on the recorded features."""
+    from feellgood_amd.engine import Engine
+    m = cube()
+    u0 = initial_u(m)
+    dt, pref, H = 2e-14, 1 + 0.01 * abs(math.log(0.01)), field(1.1)
+    ref = Engine(m.mesh, tol=1e-10, max_iter=1000, demag="direct")
+    ref.set_state(nv.FG_BOTH, u=u0)
+    ref.calc_demag()
+    ref.evolution()
+    ref.guess_init(seed=3)
+    ref.base_projection(0.2)
+    ref.prepare_elements(H, dt, pref)
+    assert not ref.solve(dt)[0]
+    ref.guess_gradient()  # sets the feature and output scales from this solve
+    p = ref.guess_get()
+    rng = np.random.default_rng(8)
+    p[W3:B3 + 3] = rng.normal(size=B3 + 3 - W3) * 0.2
+    ref.close()
+    g = Engine(m.mesh, tol=1e300, max_iter=10, demag="direct")
+    g.set_state(nv.FG_BOTH, u=u0, v=rng.normal(size=(m.n_nodes, 3)) * 1e9)
+    g.calc_demag()
+    g.evolution()
+    g.guess_init(seed=3)
+    g.guess_set(p)
+    g.guess_enable(True)
+    g.base_projection(0.7)
+    g.prepare_elements(H, dt, pref)
+    failed, info = g.solve(dt)
+    assert not failed and info.iterations == 0
+    feat = g.guess_features()
+    ptr, idx = adjacency(m)
+    vp, *_ = oracle.guess_eval(p, feat, ptr, idx, np.ones(m.n_nodes, dtype=np.int32),
+                               np.zeros((m.n_nodes, 3)), want_grad=False)
+    ep, eq = g.basis()
+    proj = np.sum(vp * ep, axis=1, keepdims=True) * ep + np.sum(vp * eq, axis=1, keepdims=True) * eq
+    v = g.get_state(nv.FG_NEXT)[1]
+    # the network's own contribution must be visible against the previous-v part
+    prev = feat[:, 6:].astype(np.float64)
+    assert np.abs(vp - prev).max() > 1e-3 * np.abs(prev).max()
+    assert np.allclose(v, proj, rtol=0, atol=2e-5 * np.abs(proj).max())
+    g.close()
+
+
+@pytest.mark.gpu
 def test_training_beats_the_reference_guess_on_held_out_fields():
     from feellgood_amd.engine import Engine
     m = cube(10)
