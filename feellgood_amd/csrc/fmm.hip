@@ -67,7 +67,12 @@ struct FmmPlan {
             *d_u_far = nullptr;
     uint8_t *d_u_mask = nullptr, *d_tgt_oct = nullptr;
     double2 *d_Itf = nullptr;  // irregular table with both signs of m: [343][(2p+1)^2]
-    double2 *d_Rcf = nullptr;  // regular child-offset table with both signs of m: [8][(p+1)^2]
+    // M2M and L2L as real linear operators per child octant (xlate_layout packing), and their
+    // launch schedules: M2M parents per level with their children by octant (-1: none); L2L
+    // parents per level with the children that receive a local expansion
+    double *d_Tm2m = nullptr, *d_Tl2l = nullptr;
+    std::vector<int32_t> m2m_child, l2l_par, l2l_par_off, l2l_child;
+    int32_t *d_m2m_child = nullptr, *d_l2l_par = nullptr, *d_l2l_child = nullptr;
     // near-field operator: per sorted target, (element charge index, coefficient) entries
     int64_t *d_nf_ptr = nullptr;  // [T+1] start of each target's coefficient row
     double *d_nf_val = nullptr;
@@ -789,73 +794,105 @@ __global__ __launch_bounds__(64) void k_fmm_p2m(int nleaf, const int32_t *__rest
     p2m_half<P, G, p2m_half_mask<P>(H)>(li, lane, blockIdx.y, act, act ? leaf_cell[li] : 0, cen, sb, se, spos, q, M);
 }
 
-// M2M: one block of 4 wavefronts per parent cell with sources. The children's expansions are
-// staged in LDS; the block splits the children into NG groups of NW wavefronts (NW = 1 up to order
-// 8), thread (n, m) of a group accumulates both charge sets over its children (read back as LDS
-// broadcasts; the child-offset table, both signs of m, is indexed per thread from global memory)
-// and the groups' sums are combined through LDS.
-template <int P>
-__global__ __launch_bounds__(256) void k_fmm_m2m(const int32_t *__restrict__ cells, const int32_t *__restrict__ ch_ptr,
-                                                 const int32_t *__restrict__ ch_idx, const int32_t *__restrict__ oct,
-                                                 const double2 *__restrict__ Rcf, double2 *__restrict__ M)
+// Translations between expansions of degree P in real form: NR = 2 fcount(P) reals, real j being
+// component j & 1 of coefficient j >> 1. M2M (K = 0) maps child to parent, L2L (K = 1) parent to
+// child; each is a fixed NR x NR matrix per child octant (expansions are normalised by the cell
+// width, so it is the same at every level). They are applied on the matrix cores as products
+// Y = T X with v_mfma_f64_16x16x4f64 (A: lane l holds T[l%16][l/16] of a 16x4 tile; B: lane l
+// holds X[l/16][l%16] of a 4x16 tile; D: lane l holds Y[l/16 + 4t][l%16], t < 4). T is stored in
+// fragment order [octant][row tile][k-step][lane] (rows padded to 16, columns to 4). M2M couples
+// output degree n to input degrees <= n and L2L to degrees >= n, so each row tile runs only the
+// k-steps [kb, ke) its degrees need.
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+template <int P> struct XL {
+    static constexpr int NCo = fcount(P), NR = 2 * NCo, NT = (NR + 15) / 16, NKS = (NR + 3) / 4;
+    static constexpr int frag_len() { return 8 * NT * NKS * 64; }
+};
+struct XRange {
+    int b[9], e[9];
+};
+
+// One wavefront per (16-column tile, charge set, row tile); a column is a parent cell. M2M
+// (K = 0): parent = sum over octants of T_o child_o, written to the parent. L2L (K = 1):
+// child_o += T_o parent for each octant with a receiving child (the parent's fragments are loaded
+// once for all octants).
+template <int P, int K>
+__global__ __launch_bounds__(256) void k_fmm_xlate(int n, const int32_t *__restrict__ cells,
+                                                   const int32_t *__restrict__ child, const double *__restrict__ Top,
+                                                   XRange kr, double *__restrict__ X)
 {
-    constexpr int NCo = fcount(P), NF = (P + 1) * (P + 1);
-    constexpr int NW = (NCo + 63) / 64, NG = 4 / NW;
-    __shared__ double2 sM[8][2 * NCo];
-    __shared__ int sOct[8];
-    __shared__ double4 red[NG][64 * NW];
-    const int ci = cells[xcd_block(blockIdx.x, gridDim.x)];
-    const int c0 = ch_ptr[ci], nch = ch_ptr[ci + 1] - c0;
-    for (int i = threadIdx.x; i < nch * 2 * NCo; i += blockDim.x) {
-        const int e = i / (2 * NCo), r = i - e * 2 * NCo;
-        sM[e][r] = M[(size_t)ch_idx[c0 + e] * 2 * NCo + r];
-    }
-    if ((int)threadIdx.x < nch) sOct[threadIdx.x] = oct[ch_idx[c0 + threadIdx.x]];
-    __syncthreads();
-    const int grp = __builtin_amdgcn_readfirstlane(threadIdx.x / (64 * NW));
-    const int q = threadIdx.x % (64 * NW);
-    int n = 0;
-    while (n < P && fidx(n + 1, 0) <= q) n++;
-    const int m = q - fidx(n, 0);
-    double ur = 0, ui = 0, vr = 0, vi = 0;
-    if (q < NCo)
-        for (int e = grp; e < nch; e += NG) {
-            const double2 *Mu = sM[e], *Mv = sM[e] + NCo;
-            const double2 *R = Rcf + (size_t)sOct[e] * NF;
-            double sc = 1.0;
+    using L = XL<P>;
+    constexpr int NR = L::NR, NT = L::NT, NKS = L::NKS;
+    const int gw = blockIdx.x * 4 + (threadIdx.x >> 6), l = threadIdx.x & 63;
+    const int R = gw % NT, s = (gw / NT) & 1, ct = gw / (2 * NT);
+    const int col = l & 15, kq = l >> 4;
+    const int pi = ct * 16 + col;
+    if (ct * 16 >= n) return;  // whole wavefront past the level (uniform)
+    const bool act = pi < n;
+    const int pc = act ? cells[pi] : 0;
+    const int kb = kr.b[R], ke = kr.e[R];
+    const double *A = Top + (size_t)R * NKS * 64 + l;
+    if (K == 0) {
+        f64x4 acc = {0.0, 0.0, 0.0, 0.0};
+        for (int o = 0; o < 8; o++) {
+            const int ch = act ? child[(size_t)pi * 8 + o] : -1;
+            const double *xs = X + ((size_t)(ch < 0 ? 0 : ch) * 2 + s) * NR;
+            const double *Ao = A + (size_t)o * NT * NKS * 64;
+            double bv[NKS], av[NKS];
 #pragma unroll
-            for (int k = 0; k <= P; k++, sc *= 0.5) {
+            for (int ks = 0; ks < NKS; ks++) {
+                if (ks < kb || ks >= ke) continue;
+                const int k = 4 * ks + kq;
+                bv[ks] = ch >= 0 && k < NR ? xs[k] : 0.0;
+                av[ks] = Ao[ks * 64];
+            }
 #pragma unroll
-                for (int l = -k; l <= k; l++) {
-                    const int ml = m - l, nk = n - k;
-                    if (nk < 0 || ml > nk || -ml > nk) continue;
-                    const double2 r = R[fidx_full(nk, ml)];
-                    const double2 a = Mu[fidx(k, l < 0 ? -l : l)], b = Mv[fidx(k, l < 0 ? -l : l)];
-                    // X_k^{-l} = (-1)^l conj(X_k^l)
-                    const double s = (l < 0 && (l & 1)) ? -sc : sc;
-                    const double ai = l < 0 ? -a.y : a.y, bi = l < 0 ? -b.y : b.y;
-                    ur += s * (r.x * a.x - r.y * ai);
-                    ui += s * (r.x * ai + r.y * a.x);
-                    vr += s * (r.x * b.x - r.y * bi);
-                    vi += s * (r.x * bi + r.y * b.x);
+            for (int ks = 0; ks < NKS; ks++) {
+                if (ks < kb || ks >= ke) continue;
+                acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av[ks], bv[ks], acc, 0, 0, 0);
+            }
+        }
+        if (act) {
+            double *d = X + ((size_t)pc * 2 + s) * NR;
+#pragma unroll
+            for (int t = 0; t < 4; t++) {
+                const int r = 16 * R + kq + 4 * t;
+                if (r < NR) d[r] = acc[t];
+            }
+        }
+    } else {
+        const double *xs = X + ((size_t)pc * 2 + s) * NR;
+        double bv[NKS];
+#pragma unroll
+        for (int ks = 0; ks < NKS; ks++) {
+            if (ks < kb || ks >= ke) continue;
+            const int k = 4 * ks + kq;
+            bv[ks] = act && k < NR ? xs[k] : 0.0;
+        }
+        for (int o = 0; o < 8; o++) {
+            const int ch = act ? child[(size_t)pi * 8 + o] : -1;
+            const double *Ao = A + (size_t)o * NT * NKS * 64;
+            double av[NKS];
+#pragma unroll
+            for (int ks = 0; ks < NKS; ks++) {
+                if (ks < kb || ks >= ke) continue;
+                av[ks] = Ao[ks * 64];
+            }
+            f64x4 acc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+            for (int ks = 0; ks < NKS; ks++) {
+                if (ks < kb || ks >= ke) continue;
+                acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av[ks], bv[ks], acc, 0, 0, 0);
+            }
+            if (ch >= 0) {
+                double *d = X + ((size_t)ch * 2 + s) * NR;
+#pragma unroll
+                for (int t = 0; t < 4; t++) {
+                    const int r = 16 * R + kq + 4 * t;
+                    if (r < NR) d[r] += acc[t];
                 }
             }
         }
-    red[grp][q] = make_double4(ur, ui, vr, vi);
-    __syncthreads();
-    if ((int)threadIdx.x < NCo) {
-        const int qq = threadIdx.x;
-        double4 a = red[0][qq];
-#pragma unroll
-        for (int w = 1; w < NG; w++) {
-            const double4 b = red[w][qq];
-            a.x += b.x;
-            a.y += b.y;
-            a.z += b.z;
-            a.w += b.w;
-        }
-        M[(size_t)ci * 2 * NCo + qq] = make_double2(a.x, a.y);
-        M[((size_t)ci * 2 + 1) * NCo + qq] = make_double2(a.z, a.w);
     }
 }
 
@@ -920,23 +957,6 @@ __global__ __launch_bounds__(256) void k_fmm_m2l(const int32_t *__restrict__ cel
     }
 }
 
-// L2L: one thread per (cell with targets at this level, set, coefficient): L += shift(parent)
-__global__ __launch_bounds__(256) void k_fmm_l2l(int ncells, const int32_t *__restrict__ cells, int p, int nc,
-                                                 const int32_t *__restrict__ parent, const int32_t *__restrict__ oct,
-                                                 const double2 *__restrict__ Rc, double2 *__restrict__ Lx)
-{
-    const int tid = blockIdx.x * blockDim.x + threadIdx.x;
-    const int q = tid % nc, set = (tid / nc) & 1, kk = tid / (2 * nc);
-    if (kk >= ncells) return;
-    int j = 0;
-    while (fidx(j + 1, 0) <= q) j++;
-    const int k = q - fidx(j, 0);
-    const int ci = cells[kk];
-    const cplx v = l2l_coef(Lx + ((size_t)parent[ci] * 2 + set) * nc, Rc + (size_t)oct[ci] * nc, p, j, k);
-    double2 &d = Lx[((size_t)ci * 2 + set) * nc + q];
-    d.x += v.re;
-    d.y += v.im;
-}
 
 __device__ inline double wsum64(double v)
 {
@@ -1180,6 +1200,109 @@ static int64_t m2p_lane_slots(const FmmPlan &P)
     return slots;
 }
 
+// The M2M (K = 0) or L2L (K = 1) operators of the 8 child octants in real form, in the fragment
+// order k_fmm_xlate reads; built by applying the reference translations (m2m_coef, l2l_coef) to
+// unit expansions, so device and host translations are the same linear maps.
+template <int P>
+static std::vector<double> xlate_pack(const FmmPlan &Pl, int K)
+{
+    using L = XL<P>;
+    constexpr int NR = L::NR, NCo = L::NCo, NT = L::NT, NKS = L::NKS;
+    std::vector<double> out((size_t)L::frag_len(), 0.0), T((size_t)NR * NR);
+    std::vector<double2> e(NCo);
+    for (int o = 0; o < 8; o++) {
+        const double2 *Rd = &Pl.Rc[(size_t)o * Pl.nc];
+        for (int j = 0; j < NR; j++) {
+            std::fill(e.begin(), e.end(), make_double2(0, 0));
+            e[j >> 1] = (j & 1) ? make_double2(0, 1) : make_double2(1, 0);
+            for (int n = 0; n <= P; n++)
+                for (int m = 0; m <= n; m++) {
+                    const cplx v = K == 0 ? m2m_coef(e.data(), Rd, n, m) : l2l_coef(e.data(), Rd, P, n, m);
+                    T[(size_t)(2 * fidx(n, m)) * NR + j] = v.re;
+                    T[(size_t)(2 * fidx(n, m) + 1) * NR + j] = v.im;
+                }
+        }
+        for (int R = 0; R < NT; R++)
+            for (int ks = 0; ks < NKS; ks++)
+                for (int l = 0; l < 64; l++) {
+                    const int r = 16 * R + (l & 15), k = 4 * ks + (l >> 4);
+                    out[(((size_t)o * NT + R) * NKS + ks) * 64 + l] = r < NR && k < NR ? T[(size_t)r * NR + k] : 0.0;
+                }
+    }
+    return out;
+}
+
+// k-step range of each row tile: the input reals whose degree the tile's output degrees couple to.
+template <int P>
+static XRange xlate_range(int K)
+{
+    using L = XL<P>;
+    XRange x{};
+    auto deg = [](int j) {
+        int n = 0;
+        while (fidx(n + 1, 0) <= (j >> 1)) n++;
+        return n;
+    };
+    for (int R = 0; R < L::NT; R++) {
+        const int r0 = 16 * R, r1 = std::min(16 * R + 15, L::NR - 1);
+        if (K == 0) {
+            const int dmax = deg(r1);  // inputs of degree <= dmax: reals [0, 2 fcount(dmax))
+            x.b[R] = 0;
+            x.e[R] = (2 * fcount(dmax) + 3) / 4;
+        } else {
+            const int dmin = deg(r0);  // inputs of degree >= dmin: reals [2 fidx(dmin, 0), NR)
+            x.b[R] = (2 * fidx(dmin, 0)) / 4;
+            x.e[R] = L::NKS;
+        }
+    }
+    return x;
+}
+
+static std::vector<double> xlate_pack(const FmmPlan &Pl, int K)
+{
+    switch (Pl.p) {
+    case 4: return xlate_pack<4>(Pl, K);
+    case 6: return xlate_pack<6>(Pl, K);
+    case 8: return xlate_pack<8>(Pl, K);
+    default: return xlate_pack<10>(Pl, K);
+    }
+}
+
+// Launch schedules of k_fmm_xlate: M2M children of each parent by octant; L2L parents per level
+// (distinct parents of the level's receiving cells) with their receiving children.
+static void xlate_schedules(FmmPlan &P)
+{
+    P.m2m_child.assign(P.m2m_cells.size() * 8, -1);
+    for (size_t k = 0; k < P.m2m_cells.size(); k++) {
+        const int ci = P.m2m_cells[k];
+        for (int e = P.ch_ptr[ci]; e < P.ch_ptr[ci + 1]; e++) {
+            const int ch = P.ch_idx[e];
+            P.m2m_child[k * 8 + P.oct[ch]] = ch;
+        }
+    }
+    P.l2l_par.clear();
+    P.l2l_child.clear();
+    P.l2l_par_off.assign(P.Lmax + 2, 0);
+    for (int l = 0; l <= P.Lmax; l++) {
+        std::unordered_map<int32_t, int32_t> slot;
+        for (int q = P.l2l_lev_off[l]; q < P.l2l_lev_off[l + 1]; q++) {
+            const int ci = P.l2l_cells[q], pa = P.parent[ci];
+            auto it = slot.find(pa);
+            int s;
+            if (it == slot.end()) {
+                s = (int)P.l2l_par.size();
+                slot[pa] = s;
+                P.l2l_par.push_back(pa);
+                P.l2l_child.insert(P.l2l_child.end(), 8, -1);
+            } else {
+                s = it->second;
+            }
+            P.l2l_child[(size_t)s * 8 + P.oct[ci]] = ci;
+        }
+        P.l2l_par_off[l + 1] = (int32_t)P.l2l_par.size();
+    }
+}
+
 // ---- host side of the device pass ----
 int fmm_create(Ctx &c)
 {
@@ -1234,10 +1357,6 @@ int fmm_create(Ctx &c)
             for (size_t q = 0; q < cw.size(); q++) cw[q].w = P->cell_iw[q];
             P->d_cen = dput(cw);
         }
-        P->d_parent = dput(P->parent);
-        P->d_oct = dput(P->oct);
-        P->d_ch_ptr = dput(P->ch_ptr);
-        P->d_ch_idx = dput(P->ch_idx);
         P->d_il_ptr = dput(P->il_ptr);
         P->d_il_src = dput(P->il_src);
         P->d_il_off = dput(P->il_off);
@@ -1253,30 +1372,28 @@ int fmm_create(Ctx &c)
         P->d_tgt_oct = dput(P->tgt_oct);
         P->d_tgt_node = dput(P->tgt_node);
         P->d_tgt_pos = dput(P->tgt_pos);
-        P->d_Rc = dput(P->Rc);
         P->d_leaf_cell = dput(P->leaf_cell);
         P->d_m2m_cells = dput(P->m2m_cells);
         P->d_m2l_cells = dput(P->m2l_cells);
-        P->d_l2l_cells = dput(P->l2l_cells);
         {
             // offset tables with both signs of m
             const int p = P->p, NIF = (2 * p + 1) * (2 * p + 1), NF = (p + 1) * (p + 1);
-            std::vector<double2> itf((size_t)343 * NIF), rcf((size_t)8 * NF);
+            std::vector<double2> itf((size_t)343 * NIF);
             for (int o = 0; o < 343; o++)
                 for (int n = 0; n <= 2 * p; n++)
                     for (int m = -n; m <= n; m++) {
                         const cplx v = fget(&P->It[(size_t)o * P->ni], n, m);
                         itf[(size_t)o * NIF + fidx_full(n, m)] = make_double2(v.re, v.im);
                     }
-            for (int o = 0; o < 8; o++)
-                for (int n = 0; n <= p; n++)
-                    for (int m = -n; m <= n; m++) {
-                        const cplx v = fget(&P->Rc[(size_t)o * P->nc], n, m);
-                        rcf[(size_t)o * NF + fidx_full(n, m)] = make_double2(v.re, v.im);
-                    }
             P->d_Itf = dput(itf);
-            P->d_Rcf = dput(rcf);
+            (void)NF;
         }
+        P->d_Tm2m = dput(xlate_pack(*P, 0));
+        P->d_Tl2l = dput(xlate_pack(*P, 1));
+        xlate_schedules(*P);
+        P->d_m2m_child = dput(P->m2m_child);
+        P->d_l2l_par = dput(P->l2l_par);
+        P->d_l2l_child = dput(P->l2l_child);
         const size_t nexp = (size_t)P->ncell() * 2 * P->nc;
         void *d = nullptr;
         if (hipMalloc(&d, nexp * sizeof(double2)) != hipSuccess) throw std::string("hipMalloc M");
@@ -1436,7 +1553,7 @@ void fmm_destroy(Ctx &c)
     void *ptrs[] = {P->d_cen, P->d_parent, P->d_oct, P->d_ch_ptr, P->d_ch_idx, P->d_il_ptr, P->d_il_src,
                     P->d_il_off, P->d_leaf_sb, P->d_leaf_se, P->d_grp_tb, P->d_grp_te, P->d_grp_lcell, P->d_u_ptr,
                     P->d_u_far, P->d_u_cell, P->d_u_mask, P->d_tgt_oct, P->d_tgt_node, P->d_tgt_pos, P->d_Rc, P->d_Itf,
-                    P->d_Rcf, P->d_M, P->d_L, P->d_leaf_cell, P->d_m2m_cells, P->d_m2l_cells, P->d_l2l_cells,
+                    P->d_Tm2m, P->d_Tl2l, P->d_m2m_child, P->d_l2l_par, P->d_l2l_child, P->d_M, P->d_L, P->d_leaf_cell, P->d_m2m_cells, P->d_m2l_cells, P->d_l2l_cells,
                     P->d_nf_ptr, P->d_nf_val,
                     P->d_nf_valf, P->d_tl_tb, P->d_tl_te, P->d_lc_ptr, P->d_lc_col, P->d_pot, c.qe};
     for (void *p : ptrs)
@@ -1450,7 +1567,6 @@ template <int PP>
 static void run_fmm(Ctx &c)
 {
     FmmPlan &P = *c.fmm;
-    const int nc = P.nc;
     const int NL = (int)P.leaf_cell.size();
     {
         Prof pr(c, "fmm_near");
@@ -1475,11 +1591,14 @@ static void run_fmm(Ctx &c)
     }
     {
         Prof pr(c, "fmm_m2m");
+        static const XRange xr0 = xlate_range<PP>(0);
         for (int l = P.Lmax - 1; l >= 2; l--) {
             const int n = P.m2m_lev_off[l + 1] - P.m2m_lev_off[l];
             if (n == 0) continue;
-            k_fmm_m2m<PP><<<n, 256, 0, c.stream>>>(P.d_m2m_cells + P.m2m_lev_off[l], P.d_ch_ptr, P.d_ch_idx,
-                                                   P.d_oct, P.d_Rcf, P.d_M);
+            const int waves = (n + 15) / 16 * 2 * XL<PP>::NT;
+            k_fmm_xlate<PP, 0><<<(waves + 3) / 4, 256, 0, c.stream>>>(n, P.d_m2m_cells + P.m2m_lev_off[l],
+                                                                    P.d_m2m_child + (size_t)8 * P.m2m_lev_off[l],
+                                                                    P.d_Tm2m, xr0, (double *)P.d_M);
         }
     }
     {
@@ -1491,12 +1610,14 @@ static void run_fmm(Ctx &c)
     }
     {
         Prof pr(c, "fmm_l2l");
+        static const XRange xr1 = xlate_range<PP>(1);
         for (int l = 3; l <= P.Lmax; l++) {
-            const int n = P.l2l_lev_off[l + 1] - P.l2l_lev_off[l];
+            const int n = P.l2l_par_off[l + 1] - P.l2l_par_off[l];
             if (n == 0) continue;
-            const int threads = n * 2 * nc;
-            k_fmm_l2l<<<(threads + 255) / 256, 256, 0, c.stream>>>(n, P.d_l2l_cells + P.l2l_lev_off[l], PP, nc,
-                                                                   P.d_parent, P.d_oct, P.d_Rc, P.d_L);
+            const int waves = (n + 15) / 16 * 2 * XL<PP>::NT;
+            k_fmm_xlate<PP, 1><<<(waves + 3) / 4, 256, 0, c.stream>>>(n, P.d_l2l_par + P.l2l_par_off[l],
+                                                                    P.d_l2l_child + (size_t)8 * P.l2l_par_off[l],
+                                                                    P.d_Tl2l, xr1, (double *)P.d_L);
         }
     }
     {
