@@ -807,52 +807,83 @@ typedef double f64x4 __attribute__((ext_vector_type(4)));
 template <int P> struct XL {
     static constexpr int NCo = fcount(P), NR = 2 * NCo, NT = (NR + 15) / 16, NKS = (NR + 3) / 4;
     static constexpr int frag_len() { return 8 * NT * NKS * 64; }
-};
-struct XRange {
-    int b[9], e[9];
+    static constexpr int deg(int j)
+    {
+        int n = 0;
+        while (fidx(n + 1, 0) <= (j >> 1)) n++;
+        return n;
+    }
+    // k-steps [kb, ke) of row tile R: the input reals whose degrees its output degrees couple to
+    static constexpr int kb(int K, int R) { return K == 0 ? 0 : (2 * fidx(deg(16 * R), 0)) / 4; }
+    static constexpr int ke(int K, int R)
+    {
+        return K == 0 ? (2 * fcount(deg(16 * R + 15 < NR ? 16 * R + 15 : NR - 1)) + 3) / 4 : NKS;
+    }
 };
 
-// One wavefront per (16-column tile, charge set, row tile); a column is a parent cell. M2M
-// (K = 0): parent = sum over octants of T_o child_o, written to the parent. L2L (K = 1):
-// child_o += T_o parent for each octant with a receiving child (the parent's fragments are loaded
-// once for all octants).
-template <int P, int K>
-__global__ __launch_bounds__(256) void k_fmm_xlate(int n, const int32_t *__restrict__ cells,
-                                                   const int32_t *__restrict__ child, const double *__restrict__ Top,
-                                                   XRange kr, double *__restrict__ X)
+// One block per (16-column tile, charge set, row tile); a column is a parent cell, and wavefront
+// w takes child octants 2w and 2w + 1 (short chains: the top levels have few cells, so a level
+// takes as long as one block). M2M (K = 0): parent
+// = sum over octants of T_o child_o, the four partial tiles summed through LDS and written to the
+// parent. L2L (K = 1): child_o += T_o parent for each octant with a receiving child.
+template <int P, int K, int R>
+__device__ __forceinline__ void xlate_tile(int n, const int32_t *__restrict__ cells, const int32_t *__restrict__ child,
+                                           const double *__restrict__ Top, double *__restrict__ X, f64x4 (*red)[64])
 {
     using L = XL<P>;
-    constexpr int NR = L::NR, NT = L::NT, NKS = L::NKS;
-    const int gw = blockIdx.x * 4 + (threadIdx.x >> 6), l = threadIdx.x & 63;
-    const int R = gw % NT, s = (gw / NT) & 1, ct = gw / (2 * NT);
+    constexpr int NR = L::NR, NT = L::NT, NKS = L::NKS, kb = L::kb(K, R), ke = L::ke(K, R);
+    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+    const int s = (blockIdx.x / NT) & 1, ct = blockIdx.x / (2 * NT);
     const int col = l & 15, kq = l >> 4;
     const int pi = ct * 16 + col;
-    if (ct * 16 >= n) return;  // whole wavefront past the level (uniform)
     const bool act = pi < n;
     const int pc = act ? cells[pi] : 0;
-    const int kb = kr.b[R], ke = kr.e[R];
     const double *A = Top + (size_t)R * NKS * 64 + l;
-    if (K == 0) {
-        f64x4 acc = {0.0, 0.0, 0.0, 0.0};
-        for (int o = 0; o < 8; o++) {
-            const int ch = act ? child[(size_t)pi * 8 + o] : -1;
-            const double *xs = X + ((size_t)(ch < 0 ? 0 : ch) * 2 + s) * NR;
-            const double *Ao = A + (size_t)o * NT * NKS * 64;
-            double bv[NKS], av[NKS];
+    double pv[NKS];  // L2L: the parent's fragments, the same for every octant
+    if (K == 1) {
+        const double *xs = X + ((size_t)pc * 2 + s) * NR;
 #pragma unroll
-            for (int ks = 0; ks < NKS; ks++) {
-                if (ks < kb || ks >= ke) continue;
-                const int k = 4 * ks + kq;
-                bv[ks] = ch >= 0 && k < NR ? xs[k] : 0.0;
-                av[ks] = Ao[ks * 64];
-            }
+        for (int ks = 0; ks < NKS; ks++) {
+            if (ks < kb || ks >= ke) continue;
+            const int k = 4 * ks + kq;
+            pv[ks] = act && k < NR ? xs[k] : 0.0;
+        }
+    }
+    f64x4 acc = {0.0, 0.0, 0.0, 0.0};
+    for (int oo = 0; oo < 2; oo++) {
+        const int o = 2 * w + oo;
+        const int ch = act ? child[(size_t)pi * 8 + o] : -1;
+        const double *Ao = A + (size_t)o * NT * NKS * 64;
+        double av[NKS], bv[NKS];
+        const double *xs = X + ((size_t)(ch < 0 ? 0 : ch) * 2 + s) * NR;
 #pragma unroll
-            for (int ks = 0; ks < NKS; ks++) {
-                if (ks < kb || ks >= ke) continue;
-                acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av[ks], bv[ks], acc, 0, 0, 0);
+        for (int ks = 0; ks < NKS; ks++) {
+            if (ks < kb || ks >= ke) continue;
+            const int k = 4 * ks + kq;
+            av[ks] = Ao[ks * 64];
+            bv[ks] = K == 1 ? pv[ks] : (ch >= 0 && k < NR ? xs[k] : 0.0);
+        }
+        if (K == 1) acc = f64x4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int ks = 0; ks < NKS; ks++) {
+            if (ks < kb || ks >= ke) continue;
+            acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av[ks], bv[ks], acc, 0, 0, 0);
+        }
+        if (K == 1 && ch >= 0) {
+            double *d = X + ((size_t)ch * 2 + s) * NR;
+#pragma unroll
+            for (int t = 0; t < 4; t++) {
+                const int r = 16 * R + kq + 4 * t;
+                if (r < NR) d[r] += acc[t];
             }
         }
-        if (act) {
+    }
+    if (K == 0) {
+        if (w > 0) red[w - 1][l] = acc;
+        __syncthreads();
+        if (w == 0 && act) {
+#pragma unroll
+            for (int q = 0; q < 3; q++) acc += red[q][l];
             double *d = X + ((size_t)pc * 2 + s) * NR;
 #pragma unroll
             for (int t = 0; t < 4; t++) {
@@ -860,40 +891,31 @@ __global__ __launch_bounds__(256) void k_fmm_xlate(int n, const int32_t *__restr
                 if (r < NR) d[r] = acc[t];
             }
         }
-    } else {
-        const double *xs = X + ((size_t)pc * 2 + s) * NR;
-        double bv[NKS];
-#pragma unroll
-        for (int ks = 0; ks < NKS; ks++) {
-            if (ks < kb || ks >= ke) continue;
-            const int k = 4 * ks + kq;
-            bv[ks] = act && k < NR ? xs[k] : 0.0;
-        }
-        for (int o = 0; o < 8; o++) {
-            const int ch = act ? child[(size_t)pi * 8 + o] : -1;
-            const double *Ao = A + (size_t)o * NT * NKS * 64;
-            double av[NKS];
-#pragma unroll
-            for (int ks = 0; ks < NKS; ks++) {
-                if (ks < kb || ks >= ke) continue;
-                av[ks] = Ao[ks * 64];
-            }
-            f64x4 acc = {0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-            for (int ks = 0; ks < NKS; ks++) {
-                if (ks < kb || ks >= ke) continue;
-                acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av[ks], bv[ks], acc, 0, 0, 0);
-            }
-            if (ch >= 0) {
-                double *d = X + ((size_t)ch * 2 + s) * NR;
-#pragma unroll
-                for (int t = 0; t < 4; t++) {
-                    const int r = 16 * R + kq + 4 * t;
-                    if (r < NR) d[r] += acc[t];
-                }
-            }
-        }
     }
+}
+
+template <int P, int K, int R>
+__device__ __forceinline__ void xlate_tile_dispatch(int r, int n, const int32_t *__restrict__ cells,
+                                                    const int32_t *__restrict__ child, const double *__restrict__ Top,
+                                                    double *__restrict__ X, f64x4 (*red)[64])
+{
+    if constexpr (R < XL<P>::NT) {
+        if (r == R)
+            xlate_tile<P, K, R>(n, cells, child, Top, X, red);
+        else
+            xlate_tile_dispatch<P, K, R + 1>(r, n, cells, child, Top, X, red);
+    }
+}
+
+template <int P, int K>
+__global__ __launch_bounds__(256) void k_fmm_xlate(int n, const int32_t *__restrict__ cells,
+                                                   const int32_t *__restrict__ child, const double *__restrict__ Top,
+                                                   double *__restrict__ X)
+{
+    constexpr int NT = XL<P>::NT;
+    __shared__ f64x4 red[3][64];  // partial tiles of wavefronts 1..3 (M2M)
+    const int R = blockIdx.x % NT;
+    xlate_tile_dispatch<P, K, 0>(R, n, cells, child, Top, X, red);
 }
 
 // M2L: one block of 4 wavefronts per target cell, split into NG entry groups of NW wavefronts
@@ -1232,32 +1254,6 @@ static std::vector<double> xlate_pack(const FmmPlan &Pl, int K)
     return out;
 }
 
-// k-step range of each row tile: the input reals whose degree the tile's output degrees couple to.
-template <int P>
-static XRange xlate_range(int K)
-{
-    using L = XL<P>;
-    XRange x{};
-    auto deg = [](int j) {
-        int n = 0;
-        while (fidx(n + 1, 0) <= (j >> 1)) n++;
-        return n;
-    };
-    for (int R = 0; R < L::NT; R++) {
-        const int r0 = 16 * R, r1 = std::min(16 * R + 15, L::NR - 1);
-        if (K == 0) {
-            const int dmax = deg(r1);  // inputs of degree <= dmax: reals [0, 2 fcount(dmax))
-            x.b[R] = 0;
-            x.e[R] = (2 * fcount(dmax) + 3) / 4;
-        } else {
-            const int dmin = deg(r0);  // inputs of degree >= dmin: reals [2 fidx(dmin, 0), NR)
-            x.b[R] = (2 * fidx(dmin, 0)) / 4;
-            x.e[R] = L::NKS;
-        }
-    }
-    return x;
-}
-
 static std::vector<double> xlate_pack(const FmmPlan &Pl, int K)
 {
     switch (Pl.p) {
@@ -1591,14 +1587,12 @@ static void run_fmm(Ctx &c)
     }
     {
         Prof pr(c, "fmm_m2m");
-        static const XRange xr0 = xlate_range<PP>(0);
         for (int l = P.Lmax - 1; l >= 2; l--) {
             const int n = P.m2m_lev_off[l + 1] - P.m2m_lev_off[l];
             if (n == 0) continue;
-            const int waves = (n + 15) / 16 * 2 * XL<PP>::NT;
-            k_fmm_xlate<PP, 0><<<(waves + 3) / 4, 256, 0, c.stream>>>(n, P.d_m2m_cells + P.m2m_lev_off[l],
-                                                                    P.d_m2m_child + (size_t)8 * P.m2m_lev_off[l],
-                                                                    P.d_Tm2m, xr0, (double *)P.d_M);
+            k_fmm_xlate<PP, 0><<<(n + 15) / 16 * 2 * XL<PP>::NT, 256, 0, c.stream>>>(
+                n, P.d_m2m_cells + P.m2m_lev_off[l], P.d_m2m_child + (size_t)8 * P.m2m_lev_off[l], P.d_Tm2m,
+                (double *)P.d_M);
         }
     }
     {
@@ -1610,14 +1604,12 @@ static void run_fmm(Ctx &c)
     }
     {
         Prof pr(c, "fmm_l2l");
-        static const XRange xr1 = xlate_range<PP>(1);
         for (int l = 3; l <= P.Lmax; l++) {
             const int n = P.l2l_par_off[l + 1] - P.l2l_par_off[l];
             if (n == 0) continue;
-            const int waves = (n + 15) / 16 * 2 * XL<PP>::NT;
-            k_fmm_xlate<PP, 1><<<(waves + 3) / 4, 256, 0, c.stream>>>(n, P.d_l2l_par + P.l2l_par_off[l],
-                                                                    P.d_l2l_child + (size_t)8 * P.l2l_par_off[l],
-                                                                    P.d_Tl2l, xr1, (double *)P.d_L);
+            k_fmm_xlate<PP, 1><<<(n + 15) / 16 * 2 * XL<PP>::NT, 256, 0, c.stream>>>(
+                n, P.d_l2l_par + P.l2l_par_off[l], P.d_l2l_child + (size_t)8 * P.l2l_par_off[l], P.d_Tl2l,
+                (double *)P.d_L);
         }
     }
     {
