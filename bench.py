@@ -194,16 +194,32 @@ def m2l_flops(p):
     return (p + 1) * (p + 2) // 2 * (p + 1) ** 2 * 2 * 8
 
 
+def xlate_ksteps(p, kind):
+    """Sum over the 16-row tiles of the k-steps the M2M (kind 0) or L2L (kind 1) operator tiles run
+    (the degree coupling of each tile, as k_fmm_xlate takes it)."""
+    nco = (p + 1) * (p + 2) // 2
+    nr, nks = 2 * nco, (2 * nco + 3) // 4
+    fidx = lambda n, m: n * (n + 1) // 2 + m
+
+    def deg(j):
+        n = 0
+        while fidx(n + 1, 0) <= (j >> 1):
+            n += 1
+        return n
+    tot = 0
+    for r in range((nr + 15) // 16):
+        if kind == 0:
+            tot += (2 * ((deg(min(16 * r + 15, nr - 1)) + 1) * (deg(min(16 * r + 15, nr - 1)) + 2) // 2) + 3) // 4
+        else:
+            tot += nks - (2 * fidx(deg(16 * r), 0)) // 4
+    return tot
+
+
 def m2m_flops(p):
-    """FP64 operations of one child-to-parent M2M translation of two expansions."""
-    terms = 0
-    for n in range(p + 1):
-        for m in range(n + 1):
-            for k in range(n + 1):
-                for l in range(-k, k + 1):
-                    if abs(m - l) <= n - k:
-                        terms += 1
-    return terms * 2 * 8
+    """FP64 matrix-core operations of one parent's M2M, both charge sets: for each of 8 child octants,
+    every 16-row operator tile times the parent's 16-column tile over its k-steps (16x16x4 MFMAs,
+    2 x 1024 flops each, per 16 parents)."""
+    return 2 * 8 * xlate_ksteps(p, 0) * 2 * 16 * 16 * 4 // 16
 
 
 def p2m_flops(p):
@@ -244,7 +260,7 @@ def kernel_model(name, n):
                         m2p_flops(p) * (f["m2p_pairs"] - f["m2p_far_pairs"])
                         + m2p_flops(far_degree(p, n.get("far_order"), n.get("precision"))) * f["m2p_far_pairs"]),
             "fmm_m2l": ("valu64", m2l_flops(p) * f["m2l_pairs"]),
-            "fmm_m2m": ("valu64", m2m_flops(p) * (f["cells"] - f["leaves"])),
+            "fmm_m2m": ("mfma64", m2m_flops(p) * f["m2m_cells"]),
             "fmm_p2m": ("valu64", p2m_flops(p) * f["sources"]),
         })
     return models.get(name)
@@ -354,12 +370,15 @@ def step_parity(args, w, m, u0, precision, guess, steps=3):
 
 def measured_traffic(kernel, args):
     """HBM bytes per launch of the dominant kernel from the committed rocprofv3 PMC summary
-    (FETCH_SIZE and WRITE_SIZE scaled by a copy kernel), when it describes this configuration."""
-    path = os.path.join(ROOT, "profiles", "film_fmm_fp64_traffic.json")
-    if args.small or args.mode != "plain" or args.demag != "fmm" or args.fmm_order != 8 or args.fmm_leaf != 64:
+    (FETCH_SIZE and WRITE_SIZE scaled by a copy kernel), when it describes this configuration: the
+    FP64 far field for --mode plain, the FP32 far field at the lower far degree for the mixed modes
+    (the learned guess does not change that kernel)."""
+    name = "film_fmm_fp64_traffic.json" if args.mode == "plain" else "film_learned_mixed_traffic.json"
+    if (args.small or args.demag != "fmm" or args.fmm_order != 8 or args.fmm_leaf != 64 or args.fmm_far_order != 0
+            or getattr(args, "dots", False)):
         return None
     try:
-        d = json.load(open(path))
+        d = json.load(open(os.path.join(ROOT, "profiles", name)))
     except (OSError, ValueError):
         return None
     return d.get("bytes_per_launch") if d.get("kernel") == kernel else None
