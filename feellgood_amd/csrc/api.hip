@@ -723,6 +723,7 @@ int fg_set_state(fg_ctx *x, int which, const double *u, const double *v, const d
     return guarded(x, [&]() {
         Ctx &c = x->c;
         if (which != FG_CURRENT && which != FG_NEXT && which != FG_BOTH) return FG_ERR_ARG;
+        guess_restart(c);  // a new trajectory: no solve history to extrapolate
         std::vector<NodeRec> h(c.N);
         for (int i = 0; i < c.N; i++) {
             NodeRec &r = h[i];
@@ -803,12 +804,15 @@ int fg_solve(fg_ctx *x, double dt, fg_solve_info *info)
         if (dt != x->dt) return FG_ERR_ARG;
         const double cst = stencil_coef(x);
         launch_node_assemble(c, x->r, cst, dt);
-        guess_on_assembly(c);
+        guess_on_assembly(c, dt);
         x->basis_valid = true;
         launch_r0(c, cst);
-        // BiCGSTAB: batches of iterations, the device control block says when to stop
+        // BiCGSTAB: batches of iterations, the device control block says when to stop. Iteration
+        // n's convergence is tested at the start of iteration n + 1, so the first batch is the
+        // previous solve's count plus one: a solve that converges like the last one needs a single
+        // round trip to the host.
         int launched = 0;
-        int batch = std::max(2, std::min(x->last_nit, c.opt.max_iter));
+        int batch = std::max(2, std::min(x->last_nit + 1, c.opt.max_iter));
         while (true) {
             for (int k = 0; k < batch && launched <= c.opt.max_iter; k++, launched++) launch_iteration(c, cst);
             HIPCHK(hipMemcpyAsync(c.h_ctl, c.ctl, sizeof(KrylovCtl), hipMemcpyDeviceToHost, c.stream));

@@ -165,7 +165,8 @@ void guess_init(Ctx &c, uint64_t seed);
 void guess_set(Ctx &c, const float *params);
 void guess_get(Ctx &c, float *params);
 void guess_destroy(Ctx &c);
-void guess_on_assembly(Ctx &c);
+void guess_on_assembly(Ctx &c, double dt);
+void guess_restart(Ctx &c);
 void guess_on_solved(Ctx &c);
 void guess_train(Ctx &c, float lr, double *loss, double *loss_ref);
 void guess_gradient(Ctx &c, float *grad, double *loss, double *loss_ref);

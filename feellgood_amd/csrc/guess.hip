@@ -10,14 +10,18 @@
 //                  assembled right-hand side over the lumped mass, v_i the previous v (9 values)
 //   aggregation a_i = mean of f_j over the magnetic mesh-edge neighbours j of i
 //   x = [f_i, a_i] (18) -> z1 = W1 x + b1 -> h1 = GELU(z1) (64) -> z2 = W2 h1 + b2 -> h2 = GELU(z2)
-//   (64) -> o = W3 h2 + b3 (3) -> v_pred = P_u (v_i + s_out o), P_u the tangent projection
+//   (64) -> o = W3 h2 + b3 (3) -> v_pred = P_u (v_base,i + s_out o), P_u the tangent projection
 //   guess x0_i = (v_pred . ep_i, v_pred . eq_i) / gamma0
+// The base the network corrects is the linear extrapolation of the last two solutions,
+// v_base = v_i + (dt / dt_prev) (v_i - v_i,prev) (v_i itself until two solves follow a state
+// reset): at a fixed step it cuts the initial error by about the ratio of the step to the time
+// scale of the dynamics.
 // GELU is the tanh form 0.5 z (1 + tanh(sqrt(2/pi) (z + 0.044715 z^3))).
 // Training: loss = sum_i |v_pred,i - v*_i|^2 / (n_mag s_out^2) against the converged solution v*
 // of the same solve; Adam on FP32 master weights; the forward pass uses their BF16 rounding.
-// The scales are the root mean squares of h, v* and v* - P_u v over the first training batch, so
-// the reference guess has loss 1 on it; W3 and b3 start at zero (the network starts as the
-// reference guess).
+// The scales are the root mean squares of h, v* and v* - P_u v_base over the first training
+// batch, so the base has loss 1 on it; W3 and b3 start at zero (the network starts as the base).
+// The reference guess (P_u v_i) is scored on the same batch as loss_ref.
 #include <cmath>
 #include <cstring>
 #include <random>
@@ -31,6 +35,7 @@ namespace fg {
 namespace {
 
 constexpr int NF = 9, NX = 18, NH = 64, NO = 3;
+constexpr int FS = 12;  // feature row: the NF features, then the base v the network corrects
 constexpr int OFF_W1 = 0, OFF_B1 = OFF_W1 + NH * NX, OFF_W2 = OFF_B1 + NH, OFF_B2 = OFF_W2 + NH * NH,
               OFF_W3 = OFF_B2 + NH, OFF_B3 = OFF_W3 + NO * NH, NTRAIN = OFF_B3 + NO;
 constexpr int OFF_S = NTRAIN, NPARAM = NTRAIN + 3;
@@ -75,13 +80,17 @@ struct GuessNet {
     float *params = nullptr;                 // FP32 master + scales [NPARAM]
     uint16_t *wbf = nullptr;                 // BF16 copy of the trainable block [NTRAIN]
     float *adam_m = nullptr, *adam_v = nullptr, *grad = nullptr;
-    float *feat = nullptr;                   // [N][9] raw features of the last solve (u, h, v)
+    float *feat = nullptr;                   // [N][FS] features of the last solve (u, h, v) and base
     // training activations and deltas, [N][k]
     float *xin = nullptr, *z1 = nullptr, *z2 = nullptr, *dz1 = nullptr, *dz2 = nullptr, *dout = nullptr;
     float *gpart = nullptr;                  // per-block partial gradients
     double *lpart = nullptr;                 // per-block loss partials (2 per block)
     double *h_l = nullptr;                   // pinned
     float *xq = nullptr;                     // [N][NXP] network inputs of the inference pass
+    float *vhist = nullptr;                  // [N][3] previous v of the last recorded solve
+    float *ninv = nullptr;                   // [N] 1 / number of magnetic mesh neighbours
+    int nhist = 0;                           // assemblies recorded since the last state reset
+    double dt_hist = 0;                      // step of the last recorded solve
 };
 
 namespace {
@@ -112,7 +121,8 @@ __global__ __launch_bounds__(256) void k_guess_features(int N, const NodeRec *__
                                                         const double4 *__restrict__ ep4,
                                                         const double4 *__restrict__ eq4,
                                                         const double *__restrict__ mass,
-                                                        const int32_t *__restrict__ mag, float *__restrict__ feat)
+                                                        const int32_t *__restrict__ mag, float *__restrict__ feat,
+                                                        float *__restrict__ vhist, float rho, int have_hist)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= N) return;
@@ -132,7 +142,17 @@ __global__ __launch_bounds__(256) void k_guess_features(int N, const NodeRec *__
         f[7] = (float)v.vy;
         f[8] = (float)v.vz;
     }
-    for (int k = 0; k < NF; k++) feat[(size_t)i * NF + k] = f[k];
+    // extrapolated base from this solve's previous v and the one before it; remember this one
+    float vb[3];
+    for (int k = 0; k < 3; k++) {
+        const float v1 = f[6 + k], v2 = have_hist ? vhist[(size_t)i * 3 + k] : v1;
+        vb[k] = v1 + rho * (v1 - v2);
+        vhist[(size_t)i * 3 + k] = v1;
+    }
+    float4 *dst = reinterpret_cast<float4 *>(feat + (size_t)i * FS);
+    dst[0] = make_float4(f[0], f[1], f[2], f[3]);
+    dst[1] = make_float4(f[4], f[5], f[6], f[7]);
+    dst[2] = make_float4(f[8], vb[0], vb[1], vb[2]);
 }
 
 struct SWeights {
@@ -154,30 +174,58 @@ __device__ inline void load_weights(SWeights &S, const float *__restrict__ p, co
 }
 
 // network input x = [f_i, mean f_j] with the feature scales applied
-__device__ inline void gather_input(int i, const float *__restrict__ feat, const int32_t *__restrict__ mag,
+// Non-magnetic nodes have zero features, so the sum runs over all stencil neighbours; ninv holds
+// 1 / (number of magnetic neighbours), fixed by the mesh (k_guess_ncount).
+__device__ inline void gather_input(int i, const float *__restrict__ feat, const float *__restrict__ ninv,
                                     const int64_t *__restrict__ sptr, const int32_t *__restrict__ swid,
                                     const int32_t *__restrict__ scol, const float *s, float x[NX])
 {
     const float sc[NF] = {1, 1, 1, s[0], s[0], s[0], s[1], s[1], s[1]};
-    for (int k = 0; k < NF; k++) x[k] = feat[(size_t)i * NF + k] * sc[k];
+    for (int k = 0; k < NF; k++) x[k] = feat[(size_t)i * FS + k] * sc[k];
     float a[NF] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-    int cnt = 0;
     const int sl = i >> 6, lane = i & 63;
     const int64_t off = sptr[sl];
     const int width = swid[sl];
+#pragma unroll 4
     for (int k = 0; k < width; k++) {
         const int j = scol[off + (int64_t)k * SELL_C + lane];
-        if (j == i || !mag[j]) continue;
-        cnt++;
-        for (int q = 0; q < NF; q++) a[q] += feat[(size_t)j * NF + q];
+        if (j == i) continue;
+        const float4 *fj = reinterpret_cast<const float4 *>(feat + (size_t)j * FS);
+        const float4 f0 = fj[0], f1 = fj[1], f2 = fj[2];
+        a[0] += f0.x;
+        a[1] += f0.y;
+        a[2] += f0.z;
+        a[3] += f0.w;
+        a[4] += f1.x;
+        a[5] += f1.y;
+        a[6] += f1.z;
+        a[7] += f1.w;
+        a[8] += f2.x;
     }
-    const float ic = cnt > 0 ? 1.0f / (float)cnt : 0.0f;
+    const float ic = ninv[i];
     for (int k = 0; k < NF; k++) x[NF + k] = a[k] * ic * sc[k];
+}
+
+// 1 / (number of magnetic stencil neighbours) of every magnetic node (0 without any)
+__global__ __launch_bounds__(256) void k_guess_ncount(int N, const int32_t *__restrict__ mag,
+                                                      const int64_t *__restrict__ sptr, const int32_t *__restrict__ swid,
+                                                      const int32_t *__restrict__ scol, float *__restrict__ ninv)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    const int sl = i >> 6, lane = i & 63;
+    const int64_t off = sptr[sl];
+    int cnt = 0;
+    for (int k = 0; k < swid[sl]; k++) {
+        const int j = scol[off + (int64_t)k * SELL_C + lane];
+        cnt += j != i && mag[j];
+    }
+    ninv[i] = cnt > 0 ? 1.0f / (float)cnt : 0.0f;
 }
 
 // Network inputs of the inference pass, padded to NXP (zero rows for non-magnetic nodes).
 __global__ __launch_bounds__(256) void k_guess_input(int N, const float *__restrict__ feat,
-                                                     const int32_t *__restrict__ mag,
+                                                     const int32_t *__restrict__ mag, const float *__restrict__ ninv,
                                                      const int64_t *__restrict__ sptr, const int32_t *__restrict__ swid,
                                                      const int32_t *__restrict__ scol, const float *__restrict__ p,
                                                      float *__restrict__ xq)
@@ -187,7 +235,7 @@ __global__ __launch_bounds__(256) void k_guess_input(int N, const float *__restr
     float x[NX] = {};
     if (mag[i]) {
         const float s[3] = {p[OFF_S], p[OFF_S + 1], p[OFF_S + 2]};
-        gather_input(i, feat, mag, sptr, swid, scol, s, x);
+        gather_input(i, feat, ninv, sptr, swid, scol, s, x);
     }
     float4 *dst = reinterpret_cast<float4 *>(xq + (size_t)i * NXP);
 #pragma unroll
@@ -318,11 +366,11 @@ __global__ __launch_bounds__(256) void k_guess_infer(int N, const float *__restr
         __syncthreads();
         const int i = base + tid;
         if (i < N && mag[i]) {
-            // v_pred = P_u (v + s_out o), then the guess in the tangent basis
-            const double u[3] = {feat[(size_t)i * NF], feat[(size_t)i * NF + 1], feat[(size_t)i * NF + 2]};
+            // v_pred = P_u (v_base + s_out o), then the guess in the tangent basis
+            const double u[3] = {feat[(size_t)i * FS], feat[(size_t)i * FS + 1], feat[(size_t)i * FS + 2]};
             const double so = S.s[2];
             double v[3];
-            for (int k = 0; k < 3; k++) v[k] = (double)feat[(size_t)i * NF + 6 + k] + so * (double)S.o[tid * NO + k];
+            for (int k = 0; k < 3; k++) v[k] = (double)feat[(size_t)i * FS + NF + k] + so * (double)S.o[tid * NO + k];
             const double un = u[0] * v[0] + u[1] * v[1] + u[2] * v[2];
             for (int k = 0; k < 3; k++) v[k] -= un * u[k];
             const double4 ep = ep4[i], eq = eq4[i];
@@ -341,7 +389,7 @@ __global__ __launch_bounds__(256) void k_guess_infer(int N, const float *__restr
 template <int MODE>
 __global__ __launch_bounds__(GB) void k_guess_forward(
     int N, double inv_norm, const float *__restrict__ feat, const int32_t *__restrict__ mag,
-    const int64_t *__restrict__ sptr, const int32_t *__restrict__ swid, const int32_t *__restrict__ scol,
+    const float *__restrict__ ninv, const int64_t *__restrict__ sptr, const int32_t *__restrict__ swid, const int32_t *__restrict__ scol,
     const float *__restrict__ p, const uint16_t *__restrict__ wbf, const double4 *__restrict__ ep4,
     const double4 *__restrict__ eq4, const NodeRec *__restrict__ nxt, double2 *__restrict__ xg,
     double4 *__restrict__ wg, float *__restrict__ xin, float *__restrict__ z1o, float *__restrict__ z2o,
@@ -354,7 +402,7 @@ __global__ __launch_bounds__(GB) void k_guess_forward(
     double l_net = 0, l_ref = 0;
     if (i < N && mag[i]) {
         float x[NX];
-        gather_input(i, feat, mag, sptr, swid, scol, S.s, x);
+        gather_input(i, feat, ninv, sptr, swid, scol, S.s, x);
         float h1[NH];
 #pragma unroll
         for (int o = 0; o < NH; o++) {
@@ -375,13 +423,13 @@ __global__ __launch_bounds__(GB) void k_guess_forward(
             out[1] += S.W3[NH + o] * h;
             out[2] += S.W3[2 * NH + o] * h;
         }
-        // v_pred = P_u (v + s_out o)
-        const double u[3] = {feat[(size_t)i * NF], feat[(size_t)i * NF + 1], feat[(size_t)i * NF + 2]};
-        const double vp[3] = {feat[(size_t)i * NF + 6], feat[(size_t)i * NF + 7], feat[(size_t)i * NF + 8]};
+        // v_pred = P_u (v_base + s_out o); the reference guess P_u v
+        const double u[3] = {feat[(size_t)i * FS], feat[(size_t)i * FS + 1], feat[(size_t)i * FS + 2]};
+        const double vp[3] = {feat[(size_t)i * FS + 6], feat[(size_t)i * FS + 7], feat[(size_t)i * FS + 8]};
         const double so = S.s[2];
         double v[3], vr[3];
         for (int k = 0; k < 3; k++) {
-            v[k] = vp[k] + so * (double)out[k];
+            v[k] = (double)feat[(size_t)i * FS + NF + k] + so * (double)out[k];
             vr[k] = vp[k];
         }
         const double un = u[0] * v[0] + u[1] * v[1] + u[2] * v[2], ur = u[0] * vr[0] + u[1] * vr[1] + u[2] * vr[2];
@@ -535,7 +583,7 @@ __global__ void k_guess_adam(float *__restrict__ p, const float *__restrict__ g,
     }
 }
 
-// sums for the scales: |h|^2, |v*|^2, |v* - P_u v|^2 over magnetic nodes
+// sums for the scales: |h|^2, |v*|^2, |v* - P_u v_base|^2 over magnetic nodes
 __global__ __launch_bounds__(256) void k_guess_moments(int N, const float *__restrict__ feat,
                                                        const int32_t *__restrict__ mag,
                                                        const NodeRec *__restrict__ nxt, double *__restrict__ part)
@@ -544,14 +592,15 @@ __global__ __launch_bounds__(256) void k_guess_moments(int N, const float *__res
     double s[3] = {0, 0, 0};
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x) {
         if (!mag[i]) continue;
-        const float *f = feat + (size_t)i * NF;
+        const float *f = feat + (size_t)i * FS;
         const NodeRec n = nxt[i];
         const double vs[3] = {n.vx, n.vy, n.vz};
-        const double ur = (double)f[0] * f[6] + (double)f[1] * f[7] + (double)f[2] * f[8];
+        const float *vb = f + NF;
+        const double ur = (double)f[0] * vb[0] + (double)f[1] * vb[1] + (double)f[2] * vb[2];
         for (int k = 0; k < 3; k++) {
             s[0] += (double)f[3 + k] * f[3 + k];
             s[1] += vs[k] * vs[k];
-            const double d = vs[k] - (f[6 + k] - ur * f[k]);
+            const double d = vs[k] - (vb[k] - ur * f[k]);
             s[2] += d * d;
         }
     }
@@ -585,7 +634,7 @@ void guess_destroy(Ctx &c)
     GuessNet *g = c.guess;
     if (!g) return;
     void *ptrs[] = {g->params, g->wbf, g->adam_m, g->adam_v, g->grad, g->feat, g->xin, g->z1, g->z2,
-                    g->dz1, g->dz2, g->dout, g->gpart, g->lpart, g->xq};
+                    g->dz1, g->dz2, g->dout, g->gpart, g->lpart, g->xq, g->vhist, g->ninv};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     if (g->h_l) (void)hipHostFree(g->h_l);
@@ -607,7 +656,10 @@ void guess_init(Ctx &c, uint64_t seed)
         galloc(&g->adam_m, NTRAIN);
         galloc(&g->adam_v, NTRAIN);
         galloc(&g->grad, NTRAIN);
-        galloc(&g->feat, (size_t)c.N * NF);
+        galloc(&g->feat, (size_t)c.N * FS);
+        galloc(&g->vhist, (size_t)c.N * 3);
+        galloc(&g->ninv, (size_t)c.N);
+        k_guess_ncount<<<(c.N + 255) / 256, 256, 0, c.stream>>>(c.N, c.mag, c.slice_ptr, c.slice_w, c.scol, g->ninv);
         galloc(&g->lpart, 2 * ((c.N + GB - 1) / GB) + 3 * 1024);
         const size_t nl = std::max<size_t>(3 * 1024, 2 * (size_t)((c.N + GB - 1) / GB));
         if (hipHostMalloc((void **)&g->h_l, nl * sizeof(double), hipHostMallocDefault) != hipSuccess)
@@ -645,28 +697,40 @@ void guess_get(Ctx &c, float *params)
 
 // after the assembly of a solve: record the features (always, so that plain solves feed training)
 // and, when enabled, overwrite the initial guess
-void guess_on_assembly(Ctx &c)
+void guess_on_assembly(Ctx &c, double dt)
 {
     GuessNet *g = c.guess;
     if (!g) return;
     const NodeRec *vsrc = c.rec[1 - c.cur];
     {
         Prof pr(c, "guess_features");
+        // the previous v is a solution from the second assembly after a reset on, the one before it
+        // from the third: extrapolate only between two solutions
+        const bool ext = g->nhist >= 2 && g->dt_hist > 0;
+        const float rho = ext ? (float)(dt / g->dt_hist) : 0.f;
         k_guess_features<<<(c.N + 255) / 256, 256, 0, c.stream>>>(c.N, c.rec[c.cur], vsrc, c.b, c.ep4, c.eq4, c.mass,
-                                                                  c.mag, g->feat);
+                                                                  c.mag, g->feat, g->vhist, rho, ext ? 1 : 0);
+        g->nhist++;
+        g->dt_hist = dt;
     }
     g->have_batch = false;  // the solution of this solve is not known yet
     if (g->enabled && g->scaled) {
         if (!g->xq) galloc(&g->xq, (size_t)c.N * NXP);
         {
             Prof pr(c, "guess_input");
-            k_guess_input<<<(c.N + 255) / 256, 256, 0, c.stream>>>(c.N, g->feat, c.mag, c.slice_ptr, c.slice_w, c.scol,
+            k_guess_input<<<(c.N + 255) / 256, 256, 0, c.stream>>>(c.N, g->feat, c.mag, g->ninv, c.slice_ptr, c.slice_w,
+                                                                   c.scol,
                                                                    g->params, g->xq);
         }
         Prof pr(c, "guess_forward");
         const int nb = std::min((c.N + 255) / 256, 2048);
         k_guess_infer<<<nb, 256, 0, c.stream>>>(c.N, g->xq, g->feat, c.mag, g->params, g->wbf, c.ep4, c.eq4, c.x, c.w);
     }
+}
+
+void guess_restart(Ctx &c)
+{
+    if (c.guess) c.guess->nhist = 0;
 }
 
 void guess_on_solved(Ctx &c)
@@ -701,7 +765,7 @@ static void compute_scales(Ctx &c)
     float sc[3];
     sc[0] = s[0] > 0 ? (float)(1.0 / std::sqrt(s[0] / n)) : 1.f;  // 1 / rms|h|
     sc[1] = s[1] > 0 ? (float)(1.0 / std::sqrt(s[1] / n)) : 1.f;  // 1 / rms|v*|
-    sc[2] = s[2] > 0 ? (float)std::sqrt(s[2] / n) : 1.f;          // rms|v* - P v|
+    sc[2] = s[2] > 0 ? (float)std::sqrt(s[2] / n) : 1.f;          // rms|v* - P v_base|
     if (hipMemcpy(g->params + OFF_S, sc, 3 * sizeof(float), hipMemcpyHostToDevice) != hipSuccess)
         throw std::string("hipMemcpy (guess scales)");
     g->scaled = true;
@@ -717,7 +781,7 @@ static void loss_and_gradient(Ctx &c, double *loss, double *loss_ref)
         throw std::string("hipMemcpy (guess)");
     const double inv_norm = 1.0 / (std::max(1, g->nmag) * (double)so * (double)so);
     const int nbf = (c.N + GB - 1) / GB;
-    k_guess_forward<1><<<nbf, GB, 0, c.stream>>>(c.N, inv_norm, g->feat, c.mag, c.slice_ptr, c.slice_w, c.scol,
+    k_guess_forward<1><<<nbf, GB, 0, c.stream>>>(c.N, inv_norm, g->feat, c.mag, g->ninv, c.slice_ptr, c.slice_w, c.scol,
                                                  g->params, g->wbf, c.ep4, c.eq4, c.rec[1 - c.cur], nullptr, nullptr,
                                                  g->xin, g->z1, g->z2, g->dout, g->lpart);
     k_guess_backward<<<nbf, GB, 0, c.stream>>>(c.N, g->params, g->wbf, g->z1, g->z2, g->dout, g->dz1, g->dz2);
@@ -781,7 +845,7 @@ void guess_batch(Ctx &c, float *feat)
 {
     GuessNet *g = c.guess;
     sync_or_throw(c, "guess batch");
-    if (hipMemcpy(feat, g->feat, (size_t)c.N * NF * sizeof(float), hipMemcpyDeviceToHost) != hipSuccess)
+    if (hipMemcpy(feat, g->feat, (size_t)c.N * FS * sizeof(float), hipMemcpyDeviceToHost) != hipSuccess)
         throw std::string("hipMemcpy (guess batch)");
 }
 

@@ -202,9 +202,9 @@ class Engine:
         return g, a.value, b.value
 
     def guess_features(self):
-        f = np.zeros(9 * self.N, dtype=np.float32)
+        f = np.zeros(12 * self.N, dtype=np.float32)
         self._chk(self.lib.fg_guess_features(self._h, nv.fptr(f)), "fg_guess_features")
-        return f.reshape(self.N, 9)
+        return f.reshape(self.N, 12)
 
     def reset_kernel_stats(self):
         self.lib.fg_reset_kernel_stats(self._h)

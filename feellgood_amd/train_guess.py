@@ -3,8 +3,10 @@
 For each seeded applied-field direction the state restarts from the workload's initial
 magnetisation; every solve of the plain solver (reference initial guess, or the network once it is
 enabled) leaves its features and its converged solution on the device, and `fg_guess_train` takes
-Adam steps on that pair. Held-out directions then measure BiCGSTAB iterations per step with and
-without the network, from the same states and basis angles.
+Adam steps on that pair. Held-out directions then measure BiCGSTAB iterations per step with the
+reference guess (previous v), with the network's base alone (the linear extrapolation of the last
+two solutions, an untrained head) and with the trained network, from the same states and basis
+angles.
 
     python -m feellgood_amd.train_guess --workload film --directions 8 --steps 40 --out w.npy
 """
@@ -47,7 +49,7 @@ def run_direction(g, u0, H, dt, pref, steps, rng, train=None, inner=4, lr=2e-3):
         if failed:
             raise RuntimeError(f"solver failure at step {k}")
         its.append(info.iterations)
-        if train and k >= 1:
+        if train and k >= 2:  # from the third solve on the base is the extrapolation
             for _ in range(inner):
                 loss, ref = g.guess_train(lr)
             losses.append(loss / ref)
@@ -92,18 +94,24 @@ def main(argv=None):
             print(f"[train] epoch {ep} direction {d}: mean loss/ref {np.mean(losses):.3f} "
                   f"(last 5: {hist[-1]:.3f}), iterations {np.mean(its):.2f}", flush=True)
     t_train = time.perf_counter() - t0
+    trained = g.guess_get()
+    base_only = trained.copy()
+    base_only[5376:5571] = 0  # W3, b3: the untrained head, v_pred = P_u v_base
     res = {}
-    for on in (False, True):
+    for name, params, on in (("reference", trained, False), ("base", base_only, True), ("learned", trained, True)):
+        g.guess_set(params)
         g.guess_enable(on)
         tot = []
         for d, H in enumerate(test_fields):
             its, _ = run_direction(g, u0, H, dt, pref, args.steps, np.random.default_rng(777 + d))
             tot += its
-        res["learned" if on else "reference"] = float(np.mean(tot))
+        res[name] = float(np.mean(tot))
+    g.guess_set(trained)
     g.guess_enable(False)
     out = {"workload": args.workload, "train_directions": args.directions, "steps": args.steps,
            "epochs": args.epochs, "train_seconds": round(t_train, 1), "final_loss_ratio": hist[-1],
-           "iterations_per_step_reference": res["reference"], "iterations_per_step_learned": res["learned"]}
+           "iterations_per_step_reference": res["reference"], "iterations_per_step_base": res["base"],
+           "iterations_per_step_learned": res["learned"]}
     print(json.dumps(out), flush=True)
     if args.out:
         np.save(args.out, g.guess_get())

@@ -209,13 +209,15 @@ int fg_fmm_reference(const double *src_xyz, int64_t S, const double *tgt_xyz, in
 /* ---- learned initial guess ----
  * No counterpart in the reference: when enabled, a per-node network (features u, the tangent
  * right-hand side over the lumped mass, the previous v, and their mean over mesh-edge neighbours;
- * 18 -> 64 -> 64 -> 3, GELU, BF16 weights, FP32 arithmetic) corrects the initial guess that
- * LinAlgebra::buildInitGuess forms (src/linear_algebra.cpp:13-24). BiCGSTAB then solves to the
- * same tolerance, so the guess changes iteration counts, not results beyond the tolerance.
+ * 18 -> 64 -> 64 -> 3, GELU, BF16 weights, FP32 arithmetic) replaces the initial guess that
+ * LinAlgebra::buildInitGuess forms (src/linear_algebra.cpp:13-24, the previous v) by a correction
+ * of its linear extrapolation v + (dt / dt_prev)(v - v_prev) over the last two solutions (the
+ * previous v alone until two solves have followed fg_set_state). BiCGSTAB then solves to the same
+ * tolerance, so the guess changes iteration counts, not results beyond the tolerance.
  * Parameter block (FP32): W1[64][18], b1[64], W2[64][64], b2[64], W3[3][64], b3[3], then the
  * scales s_h, s_v, s_out (set from the first training batch when zero). */
 #define FG_GUESS_NPARAM 5574
-/* Allocate the network with random weights (W3 = 0: it starts as the reference guess). Every
+/* Allocate the network with random weights (W3 = 0: it starts as the extrapolation). Every
  * later fg_solve records the features of its assembly, so plain solves provide training data. */
 int fg_guess_init(fg_ctx *ctx, uint64_t seed);
 /* Load or save the parameter block (n must be FG_GUESS_NPARAM); set allocates when needed. */
@@ -225,11 +227,13 @@ int fg_guess_get(fg_ctx *ctx, float *params, int64_t n);
 int fg_guess_enable(fg_ctx *ctx, int on);
 /* One Adam step (learning rate lr) on the last successful solve: features of its assembly,
  * target its solution. loss: the network's loss before the step; loss_ref: the reference
- * guess's loss on the same batch (both normalised so that loss_ref is about 1). */
+ * guess's loss on the same batch (both normalised by the extrapolation's error on the first
+ * training batch, so that an untrained network scores about 1 there). */
 int fg_guess_train(fg_ctx *ctx, float lr, double *loss, double *loss_ref);
 /* Gradient of the loss on the last solve at the current parameters, [FG_GUESS_NPARAM - 3]. */
 int fg_guess_gradient(fg_ctx *ctx, float *grad, double *loss, double *loss_ref);
-/* Raw features of the last solve, [9 * n_nodes]: u, the tangent RHS over the mass, previous v. */
+/* Raw features of the last solve, [12 * n_nodes]: u, the tangent RHS over the mass, previous v,
+ * and the extrapolated base the network corrects. */
 int fg_guess_features(fg_ctx *ctx, float *feat);
 
 /* Octree FMM plan of an FG_DEMAG_FMM context (FG_ERR_STATE otherwise): info[0] leaf level,

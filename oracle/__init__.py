@@ -69,7 +69,7 @@ SYMBOLS = {
     "orc_basis_angles": (None, [C.c_uint32, C.c_int, _d]),
     "orc_run": (C.c_int, [C.POINTER(nv.Mesh), _d, C.POINTER(RunSettings), _d, _i32]),
     "orc_guess_eval": (None, [C.POINTER(C.c_float), C.c_int, C.c_int, C.POINTER(C.c_float),
-                              C.POINTER(C.c_int64), _i32, _i32, _d, _d, _d, _d, _d]),
+                              C.POINTER(C.c_float), C.POINTER(C.c_int64), _i32, _i32, _d, _d, _d, _d, _d]),
 }
 
 _lib = None
@@ -244,11 +244,18 @@ def run(mesh, u0, final_time, evol_time_step, dt_min, dt_max, du_max, Bext, seed
     return st, evol[:12 * n.value].reshape(-1, 12)
 
 
-def guess_eval(params, feat, adj_ptr, adj_idx, mag, v_target=None, round_bf16=True, want_grad=True):
-    """The learned-guess network on the CPU: (v_pred [n,3], loss, loss_ref, grad)."""
+def guess_eval(params, feat, adj_ptr, adj_idx, mag, v_target=None, round_bf16=True, want_grad=True, v_base=None):
+    """The learned-guess network on the CPU: (v_pred [n,3], loss, loss_ref, grad). feat is [n, 9]
+    (u, h, previous v) or the engine's [n, 12] export, whose last three columns are the base the
+    network corrects (v_base, else the previous v)."""
     n = len(mag)
     params = np.ascontiguousarray(params, dtype=np.float32)
-    feat = np.ascontiguousarray(feat, dtype=np.float32).reshape(-1)
+    feat = np.asarray(feat, dtype=np.float32).reshape(n, -1)
+    if feat.shape[1] == 12 and v_base is None:
+        v_base = feat[:, 9:]
+    feat = np.ascontiguousarray(feat[:, :9]).reshape(-1)
+    vb = None if v_base is None else np.ascontiguousarray(v_base, dtype=np.float32).reshape(-1)
+    fp = C.POINTER(C.c_float)
     adj_ptr = np.ascontiguousarray(adj_ptr, dtype=np.int64)
     adj_idx = np.ascontiguousarray(adj_idx, dtype=np.int32)
     mag = np.ascontiguousarray(mag, dtype=np.int32)
@@ -257,7 +264,8 @@ def guess_eval(params, feat, adj_ptr, adj_idx, mag, v_target=None, round_bf16=Tr
     grad = np.zeros(len(params) - 3) if want_grad else None
     loss, loss_ref = C.c_double(), C.c_double()
     lib().orc_guess_eval(params.ctypes.data_as(C.POINTER(C.c_float)), 1 if round_bf16 else 0, n,
-                         feat.ctypes.data_as(C.POINTER(C.c_float)), adj_ptr.ctypes.data_as(C.POINTER(C.c_int64)),
+                         feat.ctypes.data_as(fp), None if vb is None else vb.ctypes.data_as(fp),
+                         adj_ptr.ctypes.data_as(C.POINTER(C.c_int64)),
                          adj_idx.ctypes.data_as(_i32), mag.ctypes.data_as(_i32), d(vt), d(vp), C.byref(loss),
                          C.byref(loss_ref), d(grad))
     return vp.reshape(n, 3), loss.value, loss_ref.value, grad

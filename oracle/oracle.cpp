@@ -1286,8 +1286,9 @@ bailout:
 /* ---------------- learned initial guess: the network of the engine's guess, restated ----------------
  * Same parameter layout and formulas as the engine (documented in include/fgmi.h): features
  * f = [u, s_h h, s_v v], neighbour mean over the given adjacency (magnetic neighbours only), input
- * [f, mean], 18 -> 64 -> 64 -> 3 with the tanh GELU, v_pred = P_u (v + s_out o), loss
- * sum |v_pred - v*|^2 / (n_mag s_out^2). Arithmetic in double; with round_bf16 the weight
+ * [f, mean], 18 -> 64 -> 64 -> 3 with the tanh GELU, v_pred = P_u (v_base + s_out o) with the
+ * given base per node (the previous v when v_base is NULL), loss sum |v_pred - v*|^2 /
+ * (n_mag s_out^2); loss_ref scores the reference guess P_u v the same way. Arithmetic in double; with round_bf16 the weight
  * matrices are first rounded to BF16 (round to nearest even) as the engine's forward pass uses
  * them. Gradients are with respect to the (rounded) weights and the biases. */
 namespace {
@@ -1313,9 +1314,9 @@ double g_gelu_grad(double z)
 
 extern "C" {
 
-void orc_guess_eval(const float *params, int round_bf16, int n, const float *feat, const int64_t *adj_ptr,
-                    const int32_t *adj_idx, const int32_t *mag, const double *v_target, double *v_pred,
-                    double *loss, double *loss_ref, double *grad)
+void orc_guess_eval(const float *params, int round_bf16, int n, const float *feat, const float *v_base,
+                    const int64_t *adj_ptr, const int32_t *adj_idx, const int32_t *mag, const double *v_target,
+                    double *v_pred, double *loss, double *loss_ref, double *grad)
 {
     std::vector<double> W(G_NTRAIN);
     for (int k = 0; k < G_NTRAIN; k++) {
@@ -1366,7 +1367,7 @@ void orc_guess_eval(const float *params, int round_bf16, int n, const float *fea
         const double vp[3] = {feat[(size_t)i * GNF + 6], feat[(size_t)i * GNF + 7], feat[(size_t)i * GNF + 8]};
         double v[3], vr[3];
         for (int k = 0; k < 3; k++) {
-            v[k] = vp[k] + so * out[k];
+            v[k] = (v_base ? (double)v_base[3 * i + k] : vp[k]) + so * out[k];
             vr[k] = vp[k];
         }
         const double un = u[0] * v[0] + u[1] * v[1] + u[2] * v[2], ur = u[0] * vr[0] + u[1] * vr[1] + u[2] * vr[2];

@@ -113,10 +113,11 @@ int orc_run(const fg_mesh *mesh, const double *u0, const orc_run_settings *s, do
             int32_t *n_rows);
 
 /* The engine's learned-guess network restated (see oracle.cpp): predictions v_pred [3n], loss,
- * the reference guess's loss and the gradient [FG_GUESS_NPARAM - 3]; any output may be NULL. */
-void orc_guess_eval(const float *params, int round_bf16, int n, const float *feat, const int64_t *adj_ptr,
-                    const int32_t *adj_idx, const int32_t *mag, const double *v_target, double *v_pred,
-                    double *loss, double *loss_ref, double *grad);
+ * the reference guess's loss and the gradient [FG_GUESS_NPARAM - 3]; any output may be NULL, and
+ * v_base [3n] (the base the network corrects) may be NULL for the previous v. */
+void orc_guess_eval(const float *params, int round_bf16, int n, const float *feat, const float *v_base,
+                    const int64_t *adj_ptr, const int32_t *adj_idx, const int32_t *mag, const double *v_target,
+                    double *v_pred, double *loss, double *loss_ref, double *grad);
 
 #ifdef __cplusplus
 }

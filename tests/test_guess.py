@@ -74,6 +74,14 @@ def test_oracle_zero_head_is_the_reference_guess():
     ok = mag.astype(bool)
     assert np.allclose(vp[ok], ref[ok], rtol=0, atol=1e-6 * np.abs(ref).max())
     assert L == pytest.approx(Lr, rel=1e-12)
+    # with a base (the engine's 12-column export), the zero head predicts the projected base
+    rng = np.random.default_rng(4)
+    base = (v + rng.normal(size=v.shape) * 3e8).astype(np.float32)
+    vb, L2, Lr2, _ = oracle.guess_eval(p, np.concatenate([feat, base], axis=1), ptr, idx, mag, tgt)
+    b64 = base.astype(np.float64)
+    refb = b64 - np.sum(b64 * u, axis=1, keepdims=True) * u
+    assert np.allclose(vb[ok], refb[ok], rtol=0, atol=1e-6 * np.abs(refb).max())
+    assert Lr2 == pytest.approx(Lr, rel=1e-12) and L2 != pytest.approx(L, rel=1e-3)
 
 
 def test_oracle_prediction_is_tangent():
@@ -138,9 +146,11 @@ def test_features_forward_and_gradient_match_oracle():
     g.evolution()
     g.guess_init(seed=7)
     dt, pref, H = 2e-14, 1 + 0.01 * abs(math.log(0.01)), field(0.4)
+    v_hist = []
     for k in range(3):
         u_before = g.get_state(nv.FG_CURRENT)[0]
         v_before = g.get_state(nv.FG_NEXT)[1]
+        v_hist.append(v_before)
         g.base_projection(0.1 * k)
         g.prepare_elements(H, dt, pref)
         failed, info = g.solve(dt)
@@ -156,12 +166,17 @@ def test_features_forward_and_gradient_match_oracle():
     b = rhs.reshape(-1, 2)
     h = (b[:, 1:2] * ep + b[:, 0:1] * eq) / lumped_mass(m)[:, None]
     assert np.allclose(feat[:, :3], u_before, atol=1e-6)
-    assert np.allclose(feat[:, 6:], v_before, rtol=1e-6, atol=1e-6 * np.abs(v_before).max())
+    assert np.allclose(feat[:, 6:9], v_before, rtol=1e-6, atol=1e-6 * np.abs(v_before).max())
     assert np.allclose(feat[:, 3:6], h, rtol=1e-5, atol=1e-5 * np.abs(h).max())
+    # the base: linear extrapolation of the last two solutions (same dt); at the third assembly
+    # after set_state both previous v's come from solves
+    base = 2 * v_hist[2] - v_hist[1]
+    assert np.allclose(feat[:, 9:], base, rtol=1e-5, atol=1e-5 * np.abs(base).max())
     # gradient on this batch (scales are set from it on the first call)
     grad, loss, loss_ref = g.guess_gradient()
     p = g.guess_get()
-    assert p[-3:].min() > 0 and loss_ref == pytest.approx(1.0, rel=1e-4)
+    # scales set from this batch: the untrained network (the base) scores 1
+    assert p[-3:].min() > 0 and loss == pytest.approx(1.0, rel=1e-4)
     vstar = g.get_state(nv.FG_NEXT)[1]
     ptr, idx = adjacency(m)
     mag = np.ones(m.n_nodes, dtype=np.int32)
@@ -221,8 +236,8 @@ def test_inference_pass_matches_oracle():
     ep, eq = g.basis()
     proj = np.sum(vp * ep, axis=1, keepdims=True) * ep + np.sum(vp * eq, axis=1, keepdims=True) * eq
     v = g.get_state(nv.FG_NEXT)[1]
-    # the network's own contribution must be visible against the previous-v part
-    prev = feat[:, 6:].astype(np.float64)
+    # the network's own contribution must be visible against its base
+    prev = feat[:, 9:].astype(np.float64)
     assert np.abs(vp - prev).max() > 1e-3 * np.abs(prev).max()
     assert np.allclose(v, proj, rtol=0, atol=2e-5 * np.abs(proj).max())
     g.close()
@@ -249,7 +264,7 @@ def test_training_beats_the_reference_guess_on_held_out_fields():
                 g.prepare_elements(H, dt, pref)
                 failed, _ = g.solve(dt)
                 assert not failed
-                if k >= 1:
+                if k >= 2:
                     for _ in range(4):
                         loss, loss_ref = g.guess_train(2e-3)
                     losses.append(loss / loss_ref)
