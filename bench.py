@@ -47,6 +47,11 @@ def cube_workload():
                 Ms=8e5, Ae=1.3e-11, alpha=0.5, B=0.0, angle=0.0, dt=1e-14, tol=1e-10)
 
 
+def wire_workload():
+    return dict(name="wire", radius=50.0, length=4000.0, h=2.66, seed=5, Ms=8e5, Ae=1.3e-11, alpha=0.05,
+                dt=2e-14, tol=1e-10, reversed=200.0, ramp_T_per_s=-0.06 / 1e-10)
+
+
 def dots_workload():
     return dict(name="dots", count=256, seed=4, h=4.4, dt=1e-14, tol=1e-10, alpha=0.1, B=0.0, angle=0.0)
 
@@ -134,8 +139,23 @@ def bench_dots(args):
     print(json.dumps(out), flush=True)
 
 
+def workload_text(w, m):
+    if w["name"] == "wire":
+        return (f"wire r {w['radius']:g} nm, L {w['length']:g} nm, h {w['h']:g} nm ({m.n_tets} tets, {m.n_nodes} "
+                f"nodes, {m.n_tris} surface triangles), +z with a reversed {w['reversed']:g} nm end, field ramp "
+                f"{w['ramp_T_per_s'] * 1e-10:g} T per 0.1 ns along z, dt {w['dt']:g} s, tol {w['tol']:g}")
+    return (f"{w['name']} {w['lengths'][0]:g}x{w['lengths'][1]:g}x{w['lengths'][2]:g} nm, "
+            f"{w['cells'][0]}x{w['cells'][1]}x{w['cells'][2]} cells x 6 tets "
+            f"({m.n_tets} tets, {m.n_nodes} nodes, {m.n_tris} surface triangles), "
+            f"jitter {w['jitter']}, dt {w['dt']:g} s, tol {w['tol']:g}")
+
+
 def build_mesh(w):
     from feellgood_amd.mesh import HostMesh
+    if w["name"] == "wire":
+        m = HostMesh.cylinder((0.0, 0.0), 0.0, w["radius"], w["length"], w["h"])
+        m.prepare({"volume": dict(Ms=w["Ms"], Ae=w["Ae"], alpha=w["alpha"])})
+        return m
     m = HostMesh.cuboid(w["lengths"], w["cells"], jitter=w["jitter"], seed=w["seed"])
     m.prepare({"volume": dict(Ms=w["Ms"], Ae=w["Ae"], alpha=w["alpha"])})
     return m
@@ -143,6 +163,13 @@ def build_mesh(w):
 
 def initial_state(m, w):
     pos = m.arrays()["pos"]
+    if w["name"] == "wire":
+        # +z, a reversed domain over the first `reversed` nm, a seeded +/-0.02 perturbation
+        rng = np.random.default_rng(w["seed"])
+        u = np.zeros((m.n_nodes, 3))
+        u[:, 2] = np.where(pos[:, 2] < w["reversed"] * 1e-9, -1.0, 1.0)
+        u += rng.uniform(-0.02, 0.02, size=u.shape)
+        return u / np.linalg.norm(u, axis=1, keepdims=True)
     if w["name"] == "film":
         L = w["lengths"][0] * 1e-9
         ph = 0.3 * np.sin(np.pi * pos[:, 0] / L)
@@ -154,7 +181,9 @@ def initial_state(m, w):
     return u
 
 
-def field(w):
+def field(w, t=0.0):
+    if "ramp_T_per_s" in w:
+        return np.array([0.0, 0.0, w["ramp_T_per_s"] * t]) / MU0
     a = math.radians(w["angle"])
     return np.array([math.cos(a), math.sin(a), 0.0]) * w["B"] / MU0
 
@@ -282,7 +311,7 @@ class Stepper:
 
     def __init__(self, g, w, u0):
         from feellgood_amd import native as nv
-        self.g, self.H, self.dt = g, field(w), w["dt"]
+        self.g, self.w, self.H, self.dt, self.t = g, w, field(w), w["dt"], 0.0
         self.pref = prefactor(self.dt)
         self.rng = np.random.default_rng(w["seed"])
         self.iters, self.fails = [], 0
@@ -298,11 +327,13 @@ class Stepper:
     def step(self, r=None):
         g = self.g
         g.base_projection(self.angle() if r is None else r)
+        self.H = field(self.w, self.t)  # time-dependent for the wire's ramp
         g.prepare_elements(self.H, self.dt, self.pref)
         failed, info = g.solve(self.dt)
         if failed:
             self.fails += 1
             return info
+        self.t += self.dt
         g.calc_demag()
         g.energy(self.H)
         g.evolution()
@@ -375,7 +406,7 @@ def measured_traffic(kernel, args):
     (the learned guess does not change that kernel)."""
     name = "film_fmm_fp64_traffic.json" if args.mode == "plain" else "film_learned_mixed_traffic.json"
     if (args.small or args.demag != "fmm" or args.fmm_order != 8 or args.fmm_leaf != 64 or args.fmm_far_order != 0
-            or getattr(args, "dots", False)):
+            or getattr(args, "dots", False) or getattr(args, "wire", False)):
         return None
     try:
         d = json.load(open(os.path.join(ROOT, "profiles", name)))
@@ -492,6 +523,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--small", action="store_true", help="the 48k-tet cube instead of the film")
     ap.add_argument("--dots", action="store_true", help="the batched nanodots instead of the film")
+    ap.add_argument("--wire", action="store_true",
+                    help="the 10M-tet nanowire (FMM leaf size autotuned over 32..256) instead of the film")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this engine runs on one GPU; --gpus {args.gpus} is not supported "
@@ -500,19 +533,34 @@ def main():
     if args.dots:
         bench_dots(args)
         return
-    w = cube_workload() if args.small else film_workload()
+    w = cube_workload() if args.small else wire_workload() if args.wire else film_workload()
     t0 = time.perf_counter()
     m = build_mesh(w)
     u0 = initial_state(m, w)
     print(f"[bench] mesh {m.n_nodes} nodes {m.n_tets} tets {m.n_tris} tris built in "
           f"{time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
     precision = "fp64" if args.mode == "plain" else "mixed"
+    tune = None
+    if args.wire:
+        from feellgood_amd.autotune import tune_fmm_leaf
+
+        def make_tuned(leaf):
+            args.fmm_leaf = leaf
+            return make_engine(args, w, m, precision, None, False)
+        args.fmm_leaf, tune = tune_fmm_leaf(make_tuned, u0)
+        print(f"[bench] autotuned fmm_leaf {args.fmm_leaf}: {tune}", file=sys.stderr, flush=True)
     guess, guess_src = None, None
     if args.mode == "learned+mixed":
         if args.guess:
             guess, guess_src = np.load(args.guess, allow_pickle=False).astype(np.float32), args.guess
         else:
             guess, guess_src = default_guess_weights(w)
+        if guess is None:
+            # no trained weights for this workload: the untrained head, i.e. the extrapolated base
+            from feellgood_amd import native as nv
+            guess = np.zeros(nv.FG_GUESS_NPARAM, dtype=np.float32)
+            guess[-3:] = 1.0
+            guess_src = "untrained head: the extrapolation of the last two solutions"
     args.precision = precision
     res = run_gpu(args, w, m, u0, precision, guess)
     print(f"[bench] {args.mode}: {res['ms_per_step']:.2f} ms/step, iterations {res['iters']}", file=sys.stderr,
@@ -564,10 +612,7 @@ def main():
         "vs_baseline": None,
         "dtype": "f64" if precision == "fp64" else "f32+f64",
         "data": "synthetic",
-        "config": {"workload": (f"{w['name']} {w['lengths'][0]:g}x{w['lengths'][1]:g}x{w['lengths'][2]:g} nm, "
-                                f"{w['cells'][0]}x{w['cells'][1]}x{w['cells'][2]} cells x 6 tets "
-                                f"({m.n_tets} tets, {m.n_nodes} nodes, {m.n_tris} surface triangles), "
-                                f"jitter {w['jitter']}, dt {w['dt']:g} s, tol {w['tol']:g}"),
+        "config": {"workload": workload_text(w, m),
                    "mode": mode_txt,
                    "initial_guess": (f"learned ({guess_src})" if guess is not None else
                                      "previous v (reference buildInitGuess)"),
@@ -580,6 +625,9 @@ def main():
         "profiled_ms_per_step": round(res["prof_ms_per_step"], 4),
         "fmm_plan": res["fmm"],
     }
+    if tune:
+        out["config"]["fmm_leaf"] = args.fmm_leaf
+        out["config"]["autotune_ms_per_demag"] = tune
     if plain:
         out["plain_fp64"] = {"steps_per_s": round(1e3 / plain["ms"], 4), "ms_per_step": round(plain["ms"], 4),
                              "bicgstab_iterations_per_step": round(float(np.mean(plain["iters"])), 2)}

@@ -426,6 +426,81 @@ fgh_mesh *fgh_gen_ellipsoid(double cx, double cy, double cz, double a, double b,
     return m;
 }
 
+fgh_mesh *fgh_gen_cylinder(double cx, double cy, double z0, double r, double length, double h)
+{
+    if (!(r > 0 && length > 0 && h > 0)) {
+        g_err = "cylinder needs a positive radius, length and spacing";
+        return nullptr;
+    }
+    /* grid over the bounding box, cells kept when their centre is inside the disc */
+    const int nx = std::max(2, (int)std::ceil(2 * r / h)), nz = std::max(1, (int)std::ceil(length / h));
+    const int ny = nx;
+    const double dx = 2 * r / nx, dy = dx, dz = length / nz;
+    const double x0 = cx - r, y0 = cy - r;
+    std::vector<char> col((size_t)nx * ny, 0);  // the kept columns of cells (the same at every z)
+    for (int i = 0; i < nx; i++)
+        for (int j = 0; j < ny; j++) {
+            const double X = x0 + (i + 0.5) * dx - cx, Y = y0 + (j + 0.5) * dy - cy;
+            col[(size_t)i * ny + j] = X * X + Y * Y <= r * r;
+        }
+    auto kept = [&](int i, int j) { return i >= 0 && j >= 0 && i < nx && j < ny && col[(size_t)i * ny + j]; };
+    /* nodes of kept columns: the (i, j) grid points with a kept cell around them, all z levels */
+    std::vector<int32_t> pid((size_t)(nx + 1) * (ny + 1), -1);
+    int np = 0;
+    std::vector<char> lateral;  // grid point on the lateral boundary (a missing column around it)
+    for (int i = 0; i <= nx; i++)
+        for (int j = 0; j <= ny; j++) {
+            int in = 0, tot = 0;
+            for (int a = -1; a <= 0; a++)
+                for (int b = -1; b <= 0; b++) {
+                    tot++;
+                    in += kept(i + a, j + b);
+                }
+            if (in == 0) continue;
+            pid[(size_t)i * (ny + 1) + j] = np++;
+            lateral.push_back(in < tot);
+        }
+    auto *m = new fgh_mesh();
+    m->pos.resize((size_t)np * (nz + 1) * 3);
+    for (int i = 0; i <= nx; i++)
+        for (int j = 0; j <= ny; j++) {
+            const int p = pid[(size_t)i * (ny + 1) + j];
+            if (p < 0) continue;
+            double X = x0 + i * dx - cx, Y = y0 + j * dy - cy;
+            if (lateral[p]) {
+                /* staircase rim: half-way towards the cylinder surface along the radius */
+                const double rho = std::sqrt(X * X + Y * Y);
+                if (rho > 0) {
+                    const double s = 0.5 * (1.0 + r / rho);
+                    X *= s;
+                    Y *= s;
+                }
+            }
+            for (int k = 0; k <= nz; k++) {
+                double *q = &m->pos[3 * ((size_t)k * np + p)];
+                q[0] = cx + X;
+                q[1] = cy + Y;
+                q[2] = z0 + k * dz;
+            }
+        }
+    auto node = [&](int i, int j, int k) { return (int32_t)((size_t)k * np + pid[(size_t)i * (ny + 1) + j]); };
+    for (int k = 0; k < nz; k++)
+        for (int i = 0; i < nx; i++)
+            for (int j = 0; j < ny; j++) {
+                if (!kept(i, j)) continue;
+                int A = node(i, j, k), B = node(i + 1, j, k), C = node(i + 1, j + 1, k), D = node(i, j + 1, k);
+                int E = node(i, j, k + 1), F = node(i + 1, j, k + 1), G = node(i + 1, j + 1, k + 1),
+                    H = node(i, j + 1, k + 1);
+                const int T[6][4] = {{A, B, D, H}, {A, B, E, H}, {E, F, H, B}, {B, C, D, H}, {B, C, G, H}, {F, B, G, H}};
+                for (auto &t : T) {
+                    for (int q = 0; q < 4; q++) m->tets.push_back(t[q]);
+                    m->tet_phys.push_back(300);
+                }
+            }
+    m->phys_names[{3, 300}] = "volume";
+    return m;
+}
+
 int fgh_append(fgh_mesh *dst, const fgh_mesh *src)
 {
     if (!dst || !src || dst->prepared) return FG_ERR_ARG;

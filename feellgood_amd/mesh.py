@@ -78,6 +78,10 @@ class HostMesh:
     def ellipsoid(cls, centre, axes, h):
         return cls(nv.host_lib().fgh_gen_ellipsoid(*centre, *axes, h))
 
+    @classmethod
+    def cylinder(cls, axis_xy, z0, radius, length, h):
+        return cls(nv.host_lib().fgh_gen_cylinder(axis_xy[0], axis_xy[1], z0, radius, length, h))
+
     def append(self, other):
         rc = nv.host_lib().fgh_append(self._h, other._h)
         if rc != nv.FG_OK:

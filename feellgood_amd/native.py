@@ -105,6 +105,7 @@ HOST_SYMBOLS = {
     "fgh_read_msh": (C.c_void_p, [C.c_char_p, C.c_char_p, C.c_int]),
     "fgh_gen_cuboid": (C.c_void_p, [C.c_double] * 6 + [C.c_int] * 3 + [C.c_double, C.c_uint64]),
     "fgh_gen_ellipsoid": (C.c_void_p, [C.c_double] * 7),
+    "fgh_gen_cylinder": (C.c_void_p, [C.c_double] * 6),
     "fgh_append": (C.c_int, [C.c_void_p, C.c_void_p]),
     "fgh_free": (None, [C.c_void_p]),
     "fgh_raw_counts": (C.c_int, [C.c_void_p, _i64, _i64, _i64]),

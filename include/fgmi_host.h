@@ -36,6 +36,11 @@ fgh_mesh *fgh_gen_cuboid(double x0, double y0, double z0, double lx, double ly, 
  * centres lie inside, split as in fgh_gen_cuboid, then boundary nodes projected radially onto
  * the ellipsoid surface. Volume physical group "volume". */
 fgh_mesh *fgh_gen_ellipsoid(double cx, double cy, double cz, double a, double b, double c, double h);
+/* Cylinder of radius r along z from z0 to z0 + length, axis (cx, cy): the cells of a grid of
+ * spacing about h whose centres lie inside the disc, split as in fgh_gen_cuboid, with the
+ * staircase rim nodes moved half-way towards the cylinder surface along the radius. Volume
+ * physical group "volume". No counterpart in the reference (its meshes come from gmsh). */
+fgh_mesh *fgh_gen_cylinder(double cx, double cy, double z0, double r, double length, double h);
 /* Append another mesh's nodes and elements (physical tags kept, names merged by tag). */
 int fgh_append(fgh_mesh *dst, const fgh_mesh *src);
 void fgh_free(fgh_mesh *m);

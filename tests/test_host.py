@@ -134,3 +134,24 @@ def test_ellipsoid_generator_valid():
     m.prepare({"volume": dict(Ms=1e6)})
     side, closure, scale = _outward_ok(m)
     assert np.abs(closure).max() < 1e-12 * scale
+
+
+def test_cylinder_generator_valid():
+    """Positive tetrahedra, a closed outward surface, and a volume within the staircase error of
+    pi r^2 L (half-way projection of the rim)."""
+    r, L, h = 20.0, 60.0, 2.0
+    m = HostMesh.cylinder((1.0, -2.0), 5.0, r, L, h)
+    m.prepare({"volume": dict(Ms=1e6)})
+    side, closure, scale = _outward_ok(m)
+    assert np.abs(closure).max() < 1e-12 * scale
+    a = m.arrays()
+    pos, tn = a["pos"], a["tets"]
+    vol = np.linalg.det(pos[tn[:, 1:]] - pos[tn[:, :1]]) / 6
+    assert vol.min() > 0
+    exact = np.pi * r * r * L * 1e-27
+    assert abs(vol.sum() - exact) < 0.03 * exact
+    # radial extent: the rim within half a cell diagonal of r (half-way projection), axis at
+    # (1, -2), ends at z0 and z0 + L
+    rho = np.hypot(pos[:, 0] - 1e-9, pos[:, 1] + 2e-9)
+    assert r - h <= rho.max() * 1e9 <= r + h / np.sqrt(2)
+    assert pos[:, 2].min() == pytest.approx(5e-9) and pos[:, 2].max() == pytest.approx(65e-9)
