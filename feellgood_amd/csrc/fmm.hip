@@ -919,8 +919,10 @@ __global__ __launch_bounds__(256) void k_fmm_xlate(int n, const int32_t *__restr
 }
 
 // M2L: one block of 4 wavefronts per target cell, split into NG entry groups of NW wavefronts
-// (NW = 1 up to order 8); group w takes the list entries w, w+NG, ...; thread (j, k) accumulates
-// both charge sets; the groups' sums are combined through LDS.
+// (NW = 1 up to order 8); the groups take the list entries in rounds, group w entry NG r + w.
+// Each round stages, per group, the source's two expansions and the offset's irregular table
+// (both signs of m) in LDS; thread (j, k) then accumulates both charge sets from LDS, and the
+// groups' sums are combined through LDS at the end.
 template <int P>
 __global__ __launch_bounds__(256) void k_fmm_m2l(const int32_t *__restrict__ cells, const double4 *__restrict__ cen,
                                                  const int32_t *__restrict__ il_ptr,
@@ -929,20 +931,29 @@ __global__ __launch_bounds__(256) void k_fmm_m2l(const int32_t *__restrict__ cel
                                                  double2 *__restrict__ Lx)
 {
     constexpr int NCo = fcount(P), NIF = (2 * P + 1) * (2 * P + 1);
-    constexpr int NW = (NCo + 63) / 64, NG = 4 / NW;  // waves per coefficient set, entry groups
-    __shared__ double4 red[NG][64 * NW];
+    constexpr int NW = (NCo + 63) / 64, NG = 4 / NW, GT = 64 * NW;  // waves per group, groups, threads per group
+    __shared__ double4 red[NG][GT];
+    __shared__ double2 sI[NG][NIF];
+    __shared__ double2 sM[NG][2 * NCo];
     const int ci = cells[xcd_block(blockIdx.x, gridDim.x)];
-    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / (64 * NW));
-    const int q = threadIdx.x % (64 * NW);
+    const int grp = __builtin_amdgcn_readfirstlane(threadIdx.x / GT);
+    const int q = threadIdx.x % GT;
     int j = 0;
     while (j < P && fidx(j + 1, 0) <= q) j++;
     const int k = q - fidx(j, 0);
     double ur = 0, ui = 0, vr = 0, vi = 0;
-    if (q < NCo) {
-        for (int e = il_ptr[ci] + wave; e < il_ptr[ci + 1]; e += NG) {
-            const int sc = il_src[e];
-            const double2 *Mu = M + (size_t)sc * 2 * NCo, *Mv = Mu + NCo;
-            const double2 *I = Itf + (size_t)il_off[e] * NIF;
+    const int eb = il_ptr[ci], ne = il_ptr[ci + 1] - eb;
+    for (int r0 = 0; r0 < ne; r0 += NG) {
+        const int e = r0 + grp;
+        if (e < ne) {
+            const double2 *I = Itf + (size_t)il_off[eb + e] * NIF;
+            const double2 *Ms = M + (size_t)il_src[eb + e] * 2 * NCo;
+            for (int t = q; t < NIF; t += GT) sI[grp][t] = I[t];
+            for (int t = q; t < 2 * NCo; t += GT) sM[grp][t] = Ms[t];
+        }
+        __syncthreads();
+        if (e < ne && q < NCo) {
+            const double2 *Mu = sM[grp], *Mv = sM[grp] + NCo, *I = sI[grp];
 #pragma unroll
             for (int n = 0; n <= P; n++) {
 #pragma unroll
@@ -960,8 +971,9 @@ __global__ __launch_bounds__(256) void k_fmm_m2l(const int32_t *__restrict__ cel
                 }
             }
         }
+        __syncthreads();
     }
-    red[wave][q] = make_double4(ur, ui, vr, vi);
+    red[grp][q] = make_double4(ur, ui, vr, vi);
     __syncthreads();
     if (threadIdx.x < NCo) {
         double4 a = red[0][q];
