@@ -218,9 +218,10 @@ def far_degree(p, far_order, precision):
 
 
 def m2l_flops(p):
-    """FP64 operations of one M2L translation of two expansions: every output (j, k >= 0) sums
-    (p+1)^2 complex products per charge set."""
-    return (p + 1) * (p + 2) // 2 * (p + 1) ** 2 * 2 * 8
+    """FP64 matrix-core operations of one M2L translation of both charge sets: the dense real
+    operator of the offset (2 fcount(p) rows and columns) times each set's real coefficients."""
+    nr = (p + 1) * (p + 2)
+    return 2 * 2 * nr * nr
 
 
 def xlate_ksteps(p, kind):
@@ -288,7 +289,7 @@ def kernel_model(name, n):
             "fmm_m2p": ("valu32" if n.get("precision") == "mixed" else "valu64",
                         m2p_flops(p) * (f["m2p_pairs"] - f["m2p_far_pairs"])
                         + m2p_flops(far_degree(p, n.get("far_order"), n.get("precision"))) * f["m2p_far_pairs"]),
-            "fmm_m2l": ("valu64", m2l_flops(p) * f["m2l_pairs"]),
+            "fmm_m2l": ("mfma64", m2l_flops(p) * f["m2l_pairs"]),
             "fmm_m2m": ("mfma64", m2m_flops(p) * f["m2m_cells"]),
             "fmm_p2m": ("valu64", p2m_flops(p) * f["sources"]),
         })

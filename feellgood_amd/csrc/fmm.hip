@@ -15,6 +15,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <map>
 #include <string>
 #include <unordered_map>
 #include <vector>
@@ -71,6 +72,15 @@ struct FmmPlan {
     // launch schedules: M2M parents per level with their children by octant (-1: none); L2L
     // parents per level with the children that receive a local expansion
     double *d_Tm2m = nullptr, *d_Tl2l = nullptr;
+    // M2L on the matrix cores: the operator of every offset code (fragment order), the target
+    // cells in tiles of 64 (one octant class per tile), each tile's offset codes in chunks of up
+    // to M2L_CHUNK with the source cell of every (offset, target) (-1: none), and the partial sums
+    // of every chunk, reduced per target in chunk order
+    double *d_Tm2l = nullptr, *d_m2l_part = nullptr;
+    std::vector<int32_t> m2l_tile_cell, m2l_tile_chunk, m2l_chunk_code, m2l_chunk_src;
+    int32_t *d_m2l_tile_cell = nullptr, *d_m2l_tile_chunk = nullptr, *d_m2l_chunk_code = nullptr,
+            *d_m2l_chunk_src = nullptr;
+    int m2l_ntiles = 0, m2l_nchunks = 0;
     std::vector<int32_t> m2m_child, l2l_par, l2l_par_off, l2l_child;
     int32_t *d_m2m_child = nullptr, *d_l2l_par = nullptr, *d_l2l_child = nullptr;
     // near-field operator: per sorted target, (element charge index, coefficient) entries
@@ -918,6 +928,91 @@ __global__ __launch_bounds__(256) void k_fmm_xlate(int n, const int32_t *__restr
     xlate_tile_dispatch<P, K, 0>(R, n, cells, child, Top, X, red);
 }
 
+// M2L on the matrix cores. The translation from a source cell to a target cell at offset d (in
+// cell widths, d = c_t - c_s) is a fixed NR x NR real matrix T_d up to the factor 1/W of the level
+// (the expansions are width-normalised), built on the host from m2l_coef and stored per offset
+// code in v_mfma_f64_16x16x4f64 fragment order. One block per chunk of up to M2L_CHUNK offsets of
+// a tile of 64 target cells: for each offset the block stages T_d in LDS and wavefront w applies
+// it to the source expansions of targets 16w..16w+15 (both charge sets, two column tiles, every
+// row tile), gathering each target's source from the chunk's table (zero columns where there is
+// none). The chunk's partial sums go to a buffer; k_fmm_m2l_reduce adds a tile's chunks in order
+// and writes L = (1/W) sum (deterministic).
+constexpr int M2L_CHUNK = 4;
+template <int P>
+__global__ __launch_bounds__(256) void k_fmm_m2l_mfma(const int32_t *__restrict__ tile_chunk, int ntiles,
+                                                      const int32_t *__restrict__ chunk_code,
+                                                      const int32_t *__restrict__ chunk_src,
+                                                      const double *__restrict__ Tm2l, const double *__restrict__ X,
+                                                      double *__restrict__ part)
+{
+    using L = XL<P>;
+    constexpr int NR = L::NR, NT = L::NT, NKS = L::NKS, OL = NT * NKS * 64;
+    __shared__ double sA[OL];
+    const int ch = blockIdx.x, tid = threadIdx.x, w = tid >> 6, l = tid & 63;
+    const int col = l & 15, kq = l >> 4;
+    f64x4 acc[NT][2];
+#pragma unroll
+    for (int R = 0; R < NT; R++) acc[R][0] = acc[R][1] = f64x4{0.0, 0.0, 0.0, 0.0};
+    for (int i = 0; i < M2L_CHUNK; i++) {
+        const int code = chunk_code[ch * M2L_CHUNK + i];
+        if (code < 0) break;  // uniform: the chunk's offsets are packed first
+        __syncthreads();
+        const double *A = Tm2l + (size_t)code * OL;
+        for (int e = tid; e < OL; e += 256) sA[e] = A[e];
+        __syncthreads();
+        const int src = chunk_src[((size_t)ch * M2L_CHUNK + i) * 64 + 16 * w + col];
+        double bv[2][NKS];
+#pragma unroll
+        for (int s = 0; s < 2; s++) {
+            const double *xs = X + ((size_t)(src < 0 ? 0 : src) * 2 + s) * NR;
+#pragma unroll
+            for (int ks = 0; ks < NKS; ks++) {
+                const int k = 4 * ks + kq;
+                bv[s][ks] = src >= 0 && k < NR ? xs[k] : 0.0;
+            }
+        }
+#pragma unroll
+        for (int R = 0; R < NT; R++)
+#pragma unroll
+            for (int ks = 0; ks < NKS; ks++) {
+                const double a = sA[(R * NKS + ks) * 64 + l];
+                acc[R][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, bv[0][ks], acc[R][0], 0, 0, 0);
+                acc[R][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, bv[1][ks], acc[R][1], 0, 0, 0);
+            }
+    }
+    // partial sums of this chunk: [chunk][target slot][set][NR]
+    double *d = part + ((size_t)ch * 64 + 16 * w + col) * 2 * NR;
+#pragma unroll
+    for (int s = 0; s < 2; s++)
+#pragma unroll
+        for (int R = 0; R < NT; R++)
+#pragma unroll
+            for (int t = 0; t < 4; t++) {
+                const int r = 16 * R + kq + 4 * t;
+                if (r < NR) d[s * NR + r] = acc[R][s][t];
+            }
+}
+
+// L of every M2L target: (1/W) times the sum of its tile's chunk partials, in chunk order.
+template <int P>
+__global__ __launch_bounds__(256) void k_fmm_m2l_reduce(int ntiles, const int32_t *__restrict__ tile_cell,
+                                                        const int32_t *__restrict__ tile_chunk,
+                                                        const double4 *__restrict__ cen,
+                                                        const double *__restrict__ part, double *__restrict__ Lx)
+{
+    constexpr int NR = 2 * fcount(P);
+    const int64_t id = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    const int64_t per_tile = 64 * 2 * NR;
+    if (id >= (int64_t)ntiles * per_tile) return;
+    const int tile = (int)(id / per_tile), rem = (int)(id - (int64_t)tile * per_tile);
+    const int slot = rem / (2 * NR), sr = rem - slot * 2 * NR;
+    const int cell = tile_cell[tile * 64 + slot];
+    if (cell < 0) return;
+    double sum = 0.0;
+    for (int c = tile_chunk[tile]; c < tile_chunk[tile + 1]; c++) sum += part[((size_t)c * 64 + slot) * 2 * NR + sr];
+    Lx[(size_t)cell * 2 * NR + sr] = sum * cen[cell].w;
+}
+
 // M2L: one block of 4 wavefronts per target cell, split into NG entry groups of NW wavefronts
 // (NW = 1 up to order 8); the groups take the list entries in rounds, group w entry NG r + w.
 // Each round stages, per group, the source's two expansions and the offset's irregular table
@@ -1150,7 +1245,7 @@ __device__ __forceinline__ void m2p_stages(int b, int e, int nsl, int slice, int
 }
 
 template <int P, int PF, class R>
-__global__ __launch_bounds__(256) void k_fmm_m2p(const int32_t *__restrict__ grp_tb, const int32_t *__restrict__ grp_te,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k_fmm_m2p(const int32_t *__restrict__ grp_tb, const int32_t *__restrict__ grp_te,
                                                  const int32_t *__restrict__ grp_lcell,
                                                  const int32_t *__restrict__ u_ptr, const int32_t *__restrict__ u_far,
                                                  const int32_t *__restrict__ u_cell,
@@ -1311,6 +1406,91 @@ static void xlate_schedules(FmmPlan &P)
     }
 }
 
+// The M2L operator of every offset code (dx, dy, dz in [-3, 3]: (dx+3) 49 + (dy+3) 7 + dz+3) in
+// fragment order, from m2l_coef on unit expansions (the level's 1/W is applied in the reduction);
+// codes of adjacent offsets stay zero.
+template <int P>
+static std::vector<double> m2l_pack(const FmmPlan &Pl)
+{
+    using L = XL<P>;
+    constexpr int NR = L::NR, NCo = L::NCo, NT = L::NT, NKS = L::NKS, OL = NT * NKS * 64;
+    std::vector<double> out((size_t)343 * OL, 0.0), T((size_t)NR * NR);
+    std::vector<double2> e(NCo);
+    for (int code = 0; code < 343; code++) {
+        const int dx = code / 49 - 3, dy = code / 7 % 7 - 3, dz = code % 7 - 3;
+        if (std::max({std::abs(dx), std::abs(dy), std::abs(dz)}) <= 1) continue;
+        const double2 *It = &Pl.It[(size_t)code * Pl.ni];
+        for (int j = 0; j < NR; j++) {
+            std::fill(e.begin(), e.end(), make_double2(0, 0));
+            e[j >> 1] = (j & 1) ? make_double2(0, 1) : make_double2(1, 0);
+            for (int n = 0; n <= P; n++)
+                for (int m = 0; m <= n; m++) {
+                    const cplx v = m2l_coef(e.data(), It, P, n, m);
+                    T[(size_t)(2 * fidx(n, m)) * NR + j] = v.re;
+                    T[(size_t)(2 * fidx(n, m) + 1) * NR + j] = v.im;
+                }
+        }
+        for (int R = 0; R < NT; R++)
+            for (int ks = 0; ks < NKS; ks++)
+                for (int l = 0; l < 64; l++) {
+                    const int r = 16 * R + (l & 15), k = 4 * ks + (l >> 4);
+                    out[(size_t)code * OL + ((size_t)R * NKS + ks) * 64 + l] =
+                        r < NR && k < NR ? T[(size_t)r * NR + k] : 0.0;
+                }
+    }
+    return out;
+}
+static std::vector<double> m2l_pack(const FmmPlan &Pl)
+{
+    switch (Pl.p) {
+    case 4: return m2l_pack<4>(Pl);
+    case 6: return m2l_pack<6>(Pl);
+    case 8: return m2l_pack<8>(Pl);
+    default: return m2l_pack<10>(Pl);
+    }
+}
+
+// Tiles of 64 M2L targets of one octant class (so the tile's targets share most offsets), each
+// tile's distinct offset codes in chunks of M2L_CHUNK with the source of every target.
+static void m2l_schedule(FmmPlan &P)
+{
+    std::vector<int32_t> cls[8];
+    for (int32_t ci : P.m2l_cells) cls[P.oct[ci] & 7].push_back(ci);
+    P.m2l_tile_cell.clear();
+    P.m2l_tile_chunk.assign(1, 0);
+    P.m2l_chunk_code.clear();
+    P.m2l_chunk_src.clear();
+    for (int o = 0; o < 8; o++)
+        for (size_t t0 = 0; t0 < cls[o].size(); t0 += 64) {
+            const size_t nt = std::min<size_t>(64, cls[o].size() - t0);
+            std::map<int32_t, std::vector<int32_t>> by_code;  // offset code -> source per target slot
+            for (size_t s = 0; s < nt; s++) {
+                const int ci = cls[o][t0 + s];
+                P.m2l_tile_cell.push_back(ci);
+                for (int e = P.il_ptr[ci]; e < P.il_ptr[ci + 1]; e++) {
+                    auto &v = by_code[P.il_off[e]];
+                    if (v.empty()) v.assign(64, -1);
+                    v[s] = P.il_src[e];
+                }
+            }
+            for (size_t s = nt; s < 64; s++) P.m2l_tile_cell.push_back(-1);
+            int k = 0;
+            for (auto &kv : by_code) {
+                if (k % M2L_CHUNK == 0) {
+                    P.m2l_chunk_code.insert(P.m2l_chunk_code.end(), M2L_CHUNK, -1);
+                    P.m2l_chunk_src.insert(P.m2l_chunk_src.end(), (size_t)M2L_CHUNK * 64, -1);
+                }
+                const size_t c = P.m2l_chunk_code.size() / M2L_CHUNK - 1, i = k % M2L_CHUNK;
+                P.m2l_chunk_code[c * M2L_CHUNK + i] = kv.first;
+                std::copy(kv.second.begin(), kv.second.end(), P.m2l_chunk_src.begin() + (c * M2L_CHUNK + i) * 64);
+                k++;
+            }
+            P.m2l_tile_chunk.push_back((int32_t)(P.m2l_chunk_code.size() / M2L_CHUNK));
+        }
+    P.m2l_ntiles = (int)(P.m2l_tile_chunk.size() - 1);
+    P.m2l_nchunks = (int)(P.m2l_chunk_code.size() / M2L_CHUNK);
+}
+
 // ---- host side of the device pass ----
 int fmm_create(Ctx &c)
 {
@@ -1399,6 +1579,18 @@ int fmm_create(Ctx &c)
         P->d_Tm2m = dput(xlate_pack(*P, 0));
         P->d_Tl2l = dput(xlate_pack(*P, 1));
         xlate_schedules(*P);
+        P->d_Tm2l = dput(m2l_pack(*P));
+        m2l_schedule(*P);
+        P->d_m2l_tile_cell = dput(P->m2l_tile_cell);
+        P->d_m2l_tile_chunk = dput(P->m2l_tile_chunk);
+        P->d_m2l_chunk_code = dput(P->m2l_chunk_code);
+        P->d_m2l_chunk_src = dput(P->m2l_chunk_src);
+        {
+            void *pp = nullptr;
+            const size_t np = std::max<size_t>(1, (size_t)P->m2l_nchunks * 64 * 2 * 2 * P->nc);
+            if (hipMalloc(&pp, np * sizeof(double)) != hipSuccess) throw std::string("hipMalloc M2L partials");
+            P->d_m2l_part = (double *)pp;
+        }
         P->d_m2m_child = dput(P->m2m_child);
         P->d_l2l_par = dput(P->l2l_par);
         P->d_l2l_child = dput(P->l2l_child);
@@ -1561,7 +1753,8 @@ void fmm_destroy(Ctx &c)
     void *ptrs[] = {P->d_cen, P->d_parent, P->d_oct, P->d_ch_ptr, P->d_ch_idx, P->d_il_ptr, P->d_il_src,
                     P->d_il_off, P->d_leaf_sb, P->d_leaf_se, P->d_grp_tb, P->d_grp_te, P->d_grp_lcell, P->d_u_ptr,
                     P->d_u_far, P->d_u_cell, P->d_u_mask, P->d_tgt_oct, P->d_tgt_node, P->d_tgt_pos, P->d_Rc, P->d_Itf,
-                    P->d_Tm2m, P->d_Tl2l, P->d_m2m_child, P->d_l2l_par, P->d_l2l_child, P->d_M, P->d_L, P->d_leaf_cell, P->d_m2m_cells, P->d_m2l_cells, P->d_l2l_cells,
+                    P->d_Tm2m, P->d_Tl2l, P->d_Tm2l, P->d_m2l_part, P->d_m2l_tile_cell, P->d_m2l_tile_chunk,
+                    P->d_m2l_chunk_code, P->d_m2l_chunk_src, P->d_m2m_child, P->d_l2l_par, P->d_l2l_child, P->d_M, P->d_L, P->d_leaf_cell, P->d_m2m_cells, P->d_m2l_cells, P->d_l2l_cells,
                     P->d_nf_ptr, P->d_nf_val,
                     P->d_nf_valf, P->d_tl_tb, P->d_tl_te, P->d_lc_ptr, P->d_lc_col, P->d_pot, c.qe};
     for (void *p : ptrs)
@@ -1609,10 +1802,15 @@ static void run_fmm(Ctx &c)
     }
     {
         Prof pr(c, "fmm_m2l");
-        const int n = (int)P.m2l_cells.size();
-        if (n > 0)
-            k_fmm_m2l<PP><<<n, 256, 0, c.stream>>>(P.d_m2l_cells, P.d_cen, P.d_il_ptr, P.d_il_src, P.d_il_off,
-                                                   P.d_Itf, P.d_M, P.d_L);
+        if (P.m2l_nchunks > 0)
+            k_fmm_m2l_mfma<PP><<<P.m2l_nchunks, 256, 0, c.stream>>>(P.d_m2l_tile_chunk, P.m2l_ntiles, P.d_m2l_chunk_code,
+                                                                   P.d_m2l_chunk_src, P.d_Tm2l, (const double *)P.d_M,
+                                                                   P.d_m2l_part);
+        if (P.m2l_ntiles > 0) {
+            const int64_t n = (int64_t)P.m2l_ntiles * 64 * 2 * 2 * P.nc;
+            k_fmm_m2l_reduce<PP><<<(int)((n + 255) / 256), 256, 0, c.stream>>>(
+                P.m2l_ntiles, P.d_m2l_tile_cell, P.d_m2l_tile_chunk, P.d_cen, P.d_m2l_part, (double *)P.d_L);
+        }
     }
     {
         Prof pr(c, "fmm_l2l");

@@ -357,32 +357,43 @@ __device__ inline void m2p_eval2_r(const typename vec2_of<R>::type *__restrict__
 {
     const R r2 = x * x + y * y + z * z;
     const R ir2 = fast_inv(r2), zr = z * ir2, zr2 = zr + zr, ir22 = ir2 + ir2;
+    // Factors of the three-term recurrence in n shared by every column m: tz[n] = (2n - 1) z / r^2
+    // and un[n] = (n - 1)^2 / r^2; column m takes un[n] - m^2 / r^2 = (n + m - 1)(n - m - 1) / r^2.
+    R tz[P + 1], un[P + 1];
+    tz[1] = zr;
+    un[1] = R(0);
+    {
+        R st = ir2;
+#pragma unroll
+        for (int n = 2; n <= P; n++) {
+            tz[n] = tz[n - 1] + zr2;
+            un[n] = un[n - 1] + st;
+            st += ir22;
+        }
+    }
     R dre = fast_rsqrt(r2), dim = R(0);
     R au = Mu[0].x * dre, av = Mv[0].x * dre;
     if (P >= 1) {
         R p2 = dre, p1 = zr * dre;
         au = fma(Mu[fidx(1, 0)].x, p1, au);
         av = fma(Mv[fidx(1, 0)].x, p1, av);
-        R tz = R(3) * zr, bb = ir2, st = R(3) * ir2;
 #pragma unroll
         for (int n = 2; n <= P; n++) {
-            const R c = tz * p1 - bb * p2;
+            const R c = tz[n] * p1 - un[n] * p2;
             au = fma(Mu[fidx(n, 0)].x, c, au);
             av = fma(Mv[fidx(n, 0)].x, c, av);
             p2 = p1;
             p1 = c;
-            tz += zr2;
-            bb += st;
-            st += ir22;
         }
     }
     R hu = R(0), hv = R(0);
-    R tm = R(1);  // 2m + 1
+    R fd = -ir2, vm = R(0), dv = ir2;  // fd = (1 - 2m) / r^2, vm = m^2 / r^2
 #pragma unroll
     for (int m = 1; m <= P; m++) {
-        tm += R(2);
-        const R f = (R(2) - tm) * ir2;
-        const R nre = f * (x * dre - y * dim), nim = f * (x * dim + y * dre);
+        const R nre = fd * (x * dre - y * dim), nim = fd * (x * dim + y * dre);
+        fd -= ir22;
+        vm += dv;
+        dv += ir22;
         dre = nre;
         dim = nim;
         auto a = Mu[fidx(m, m)];
@@ -392,18 +403,17 @@ __device__ inline void m2p_eval2_r(const typename vec2_of<R>::type *__restrict__
         hv = fma(b.x, dre, hv);
         hv = fma(b.y, dim, hv);
         if (m + 1 <= P) {
-            const R a1 = tm * zr;
-            R p2re = dre, p2im = dim, p1re = a1 * dre, p1im = a1 * dim;
+            R p2re = dre, p2im = dim, p1re = tz[m + 1] * dre, p1im = tz[m + 1] * dim;
             a = Mu[fidx(m + 1, m)];
             b = Mv[fidx(m + 1, m)];
             hu = fma(a.x, p1re, hu);
             hu = fma(a.y, p1im, hu);
             hv = fma(b.x, p1re, hv);
             hv = fma(b.y, p1im, hv);
-            R tz = (tm + R(2)) * zr, bb = tm * ir2, st = (tm + R(2)) * ir2;
 #pragma unroll
             for (int n = m + 2; n <= P; n++) {
-                const R cre = tz * p1re - bb * p2re, cim = tz * p1im - bb * p2im;
+                const R bb = un[n] - vm;
+                const R cre = tz[n] * p1re - bb * p2re, cim = tz[n] * p1im - bb * p2im;
                 a = Mu[fidx(n, m)];
                 b = Mv[fidx(n, m)];
                 hu = fma(a.x, cre, hu);
@@ -414,9 +424,6 @@ __device__ inline void m2p_eval2_r(const typename vec2_of<R>::type *__restrict__
                 p2im = p1im;
                 p1re = cre;
                 p1im = cim;
-                tz += zr2;
-                bb += st;
-                st += ir22;
             }
         }
     }
