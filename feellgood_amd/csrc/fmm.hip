@@ -61,13 +61,10 @@ struct FmmPlan {
     std::vector<double2> Rc;          // [8][nc]
     // device copies
     double4 *d_cen = nullptr;
-    int32_t *d_parent = nullptr, *d_oct = nullptr, *d_ch_ptr = nullptr, *d_ch_idx = nullptr;
-    int32_t *d_il_ptr = nullptr, *d_il_src = nullptr, *d_il_off = nullptr;
     int32_t *d_leaf_sb = nullptr, *d_leaf_se = nullptr;
     int32_t *d_grp_tb = nullptr, *d_grp_te = nullptr, *d_grp_lcell = nullptr, *d_u_ptr = nullptr, *d_u_cell = nullptr,
             *d_u_far = nullptr;
     uint8_t *d_u_mask = nullptr, *d_tgt_oct = nullptr;
-    double2 *d_Itf = nullptr;  // irregular table with both signs of m: [343][(2p+1)^2]
     // M2M and L2L as real linear operators per child octant (xlate_layout packing), and their
     // launch schedules: M2M parents per level with their children by octant (-1: none); L2L
     // parents per level with the children that receive a local expansion
@@ -99,7 +96,6 @@ struct FmmPlan {
     int64_t nf_nnz = 0;
     int32_t *d_tgt_node = nullptr;
     double4 *d_tgt_pos = nullptr;
-    double2 *d_Rc = nullptr;
     double2 *d_M = nullptr, *d_L = nullptr;  // [ncell][2][nc]
     std::vector<int32_t> src_cells_lev_off, tgt_cells_lev_off;  // offsets into the lists
     std::vector<int32_t> src_cells, tgt_cells;
@@ -109,7 +105,7 @@ struct FmmPlan {
     std::vector<int32_t> leaf_cell, leaf_ix, leaf_iy, leaf_iz;
     std::vector<double> cell_iw;  // 1 / cell width
     std::vector<int32_t> m2m_lev_off, m2m_cells, m2l_cells, l2l_lev_off, l2l_cells;
-    int32_t *d_leaf_cell = nullptr, *d_m2m_cells = nullptr, *d_m2l_cells = nullptr, *d_l2l_cells = nullptr;
+    int32_t *d_leaf_cell = nullptr, *d_m2m_cells = nullptr;
     int ncell() const { return (int)cen.size(); }
 };
 
@@ -1013,80 +1009,6 @@ __global__ __launch_bounds__(256) void k_fmm_m2l_reduce(int ntiles, const int32_
     Lx[(size_t)cell * 2 * NR + sr] = sum * cen[cell].w;
 }
 
-// M2L: one block of 4 wavefronts per target cell, split into NG entry groups of NW wavefronts
-// (NW = 1 up to order 8); the groups take the list entries in rounds, group w entry NG r + w.
-// Each round stages, per group, the source's two expansions and the offset's irregular table
-// (both signs of m) in LDS; thread (j, k) then accumulates both charge sets from LDS, and the
-// groups' sums are combined through LDS at the end.
-template <int P>
-__global__ __launch_bounds__(256) void k_fmm_m2l(const int32_t *__restrict__ cells, const double4 *__restrict__ cen,
-                                                 const int32_t *__restrict__ il_ptr,
-                                                 const int32_t *__restrict__ il_src, const int32_t *__restrict__ il_off,
-                                                 const double2 *__restrict__ Itf, const double2 *__restrict__ M,
-                                                 double2 *__restrict__ Lx)
-{
-    constexpr int NCo = fcount(P), NIF = (2 * P + 1) * (2 * P + 1);
-    constexpr int NW = (NCo + 63) / 64, NG = 4 / NW, GT = 64 * NW;  // waves per group, groups, threads per group
-    __shared__ double4 red[NG][GT];
-    __shared__ double2 sI[NG][NIF];
-    __shared__ double2 sM[NG][2 * NCo];
-    const int ci = cells[xcd_block(blockIdx.x, gridDim.x)];
-    const int grp = __builtin_amdgcn_readfirstlane(threadIdx.x / GT);
-    const int q = threadIdx.x % GT;
-    int j = 0;
-    while (j < P && fidx(j + 1, 0) <= q) j++;
-    const int k = q - fidx(j, 0);
-    double ur = 0, ui = 0, vr = 0, vi = 0;
-    const int eb = il_ptr[ci], ne = il_ptr[ci + 1] - eb;
-    for (int r0 = 0; r0 < ne; r0 += NG) {
-        const int e = r0 + grp;
-        if (e < ne) {
-            const double2 *I = Itf + (size_t)il_off[eb + e] * NIF;
-            const double2 *Ms = M + (size_t)il_src[eb + e] * 2 * NCo;
-            for (int t = q; t < NIF; t += GT) sI[grp][t] = I[t];
-            for (int t = q; t < 2 * NCo; t += GT) sM[grp][t] = Ms[t];
-        }
-        __syncthreads();
-        if (e < ne && q < NCo) {
-            const double2 *Mu = sM[grp], *Mv = sM[grp] + NCo, *I = sI[grp];
-#pragma unroll
-            for (int n = 0; n <= P; n++) {
-#pragma unroll
-                for (int m = -n; m <= n; m++) {
-                    // (-1)^m M_n^m I_{n+j}^{-(m+k)}
-                    const double2 a = Mu[fidx(n, m < 0 ? -m : m)], b = Mv[fidx(n, m < 0 ? -m : m)];
-                    const double2 t = I[fidx_full(n + j, -(m + k))];
-                    // for m < 0, M_n^m = (-1)^m conj(M_n^|m|): the two factors (-1)^m cancel
-                    const double s = (m > 0 && (m & 1)) ? -1.0 : 1.0;
-                    const double ai = m < 0 ? -a.y : a.y, bi = m < 0 ? -b.y : b.y;
-                    ur += s * (a.x * t.x - ai * t.y);
-                    ui += s * (a.x * t.y + ai * t.x);
-                    vr += s * (b.x * t.x - bi * t.y);
-                    vi += s * (b.x * t.y + bi * t.x);
-                }
-            }
-        }
-        __syncthreads();
-    }
-    red[grp][q] = make_double4(ur, ui, vr, vi);
-    __syncthreads();
-    if (threadIdx.x < NCo) {
-        double4 a = red[0][q];
-#pragma unroll
-        for (int w = 1; w < NG; w++) {
-            const double4 b = red[w][q];
-            a.x += b.x;
-            a.y += b.y;
-            a.z += b.z;
-            a.w += b.w;
-        }
-        const double f = (((j + k) & 1) ? -1.0 : 1.0) * cen[ci].w;
-        Lx[(size_t)ci * 2 * NCo + q] = make_double2(a.x * f, a.y * f);
-        Lx[((size_t)ci * 2 + 1) * NCo + q] = make_double2(a.z * f, a.w * f);
-    }
-}
-
-
 __device__ inline double wsum64(double v)
 {
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
@@ -1545,9 +1467,6 @@ int fmm_create(Ctx &c)
             for (size_t q = 0; q < cw.size(); q++) cw[q].w = P->cell_iw[q];
             P->d_cen = dput(cw);
         }
-        P->d_il_ptr = dput(P->il_ptr);
-        P->d_il_src = dput(P->il_src);
-        P->d_il_off = dput(P->il_off);
         P->d_leaf_sb = dput(P->leaf_sb);
         P->d_leaf_se = dput(P->leaf_se);
         P->d_grp_tb = dput(P->grp_tb);
@@ -1562,20 +1481,6 @@ int fmm_create(Ctx &c)
         P->d_tgt_pos = dput(P->tgt_pos);
         P->d_leaf_cell = dput(P->leaf_cell);
         P->d_m2m_cells = dput(P->m2m_cells);
-        P->d_m2l_cells = dput(P->m2l_cells);
-        {
-            // offset tables with both signs of m
-            const int p = P->p, NIF = (2 * p + 1) * (2 * p + 1), NF = (p + 1) * (p + 1);
-            std::vector<double2> itf((size_t)343 * NIF);
-            for (int o = 0; o < 343; o++)
-                for (int n = 0; n <= 2 * p; n++)
-                    for (int m = -n; m <= n; m++) {
-                        const cplx v = fget(&P->It[(size_t)o * P->ni], n, m);
-                        itf[(size_t)o * NIF + fidx_full(n, m)] = make_double2(v.re, v.im);
-                    }
-            P->d_Itf = dput(itf);
-            (void)NF;
-        }
         P->d_Tm2m = dput(xlate_pack(*P, 0));
         P->d_Tl2l = dput(xlate_pack(*P, 1));
         xlate_schedules(*P);
@@ -1750,11 +1655,10 @@ void fmm_destroy(Ctx &c)
 {
     FmmPlan *P = c.fmm;
     if (!P) return;
-    void *ptrs[] = {P->d_cen, P->d_parent, P->d_oct, P->d_ch_ptr, P->d_ch_idx, P->d_il_ptr, P->d_il_src,
-                    P->d_il_off, P->d_leaf_sb, P->d_leaf_se, P->d_grp_tb, P->d_grp_te, P->d_grp_lcell, P->d_u_ptr,
-                    P->d_u_far, P->d_u_cell, P->d_u_mask, P->d_tgt_oct, P->d_tgt_node, P->d_tgt_pos, P->d_Rc, P->d_Itf,
+    void *ptrs[] = {P->d_cen, P->d_leaf_sb, P->d_leaf_se, P->d_grp_tb, P->d_grp_te, P->d_grp_lcell, P->d_u_ptr,
+                    P->d_u_far, P->d_u_cell, P->d_u_mask, P->d_tgt_oct, P->d_tgt_node, P->d_tgt_pos,
                     P->d_Tm2m, P->d_Tl2l, P->d_Tm2l, P->d_m2l_part, P->d_m2l_tile_cell, P->d_m2l_tile_chunk,
-                    P->d_m2l_chunk_code, P->d_m2l_chunk_src, P->d_m2m_child, P->d_l2l_par, P->d_l2l_child, P->d_M, P->d_L, P->d_leaf_cell, P->d_m2m_cells, P->d_m2l_cells, P->d_l2l_cells,
+                    P->d_m2l_chunk_code, P->d_m2l_chunk_src, P->d_m2m_child, P->d_l2l_par, P->d_l2l_child, P->d_M, P->d_L, P->d_leaf_cell, P->d_m2m_cells,
                     P->d_nf_ptr, P->d_nf_val,
                     P->d_nf_valf, P->d_tl_tb, P->d_tl_te, P->d_lc_ptr, P->d_lc_col, P->d_pot, c.qe};
     for (void *p : ptrs)

@@ -476,9 +476,6 @@ __host__ __device__ inline void l2p_eval2_t(const double2 *__restrict__ Lu, cons
     }
 }
 
-// Index of X_n^m (any sign of m) in the tables that store both signs: n^2 + n + m.
-__host__ __device__ constexpr inline int fidx_full(int n, int m) { return n * n + n + m; }
-
 // M2M target coefficient (n, m >= 0): sum over the child's (k, l) of R_{n-k}^{m-l}(d) M_k^l, with d
 // the child-to-parent centre offset in parent widths and the child expansion normalised by the
 // child width (factor 2^-k).
