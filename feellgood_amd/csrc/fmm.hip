@@ -16,6 +16,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <map>
+#include <type_traits>
 #include <string>
 #include <unordered_map>
 #include <vector>
@@ -1121,53 +1122,197 @@ __global__ __launch_bounds__(256) void k_nf_apply(int ntl, const int32_t *__rest
     }
 }
 
-// Far field: one block per evaluation group. Threads are (target slot, slice) pairs: with n
-// targets in the current chunk of at most 64, nsl = min(256 / n, CH) slices, and the group's source
-// list is walked in stages of nsl cells, one cell per slice, whose multipoles are staged in LDS
-// (in the real type R: FP64, or FP32 scaled by mscale in the mixed-precision pass), so that every
-// stage keeps n * nsl of the 256 lanes busy. Entries from u_far[g] on are evaluated to degree PF.
-// Each target's slices are summed through LDS in FP64, then the L2P of the group's level-(L-2)
-// ancestor is added and phi, phi_v are written.
-template <class R> constexpr int m2p_stage_cells() { return sizeof(R) == 8 ? 16 : 32; }
+// Far field: multipole-to-target evaluation (M2P) per evaluation group, one wavefront per group.
+// The wavefront's four rows of 16 lanes are (target row, cell slot) pairs: with nt targets in the
+// current chunk of at most 64, nr = ceil(nt / 16) rows cover the targets and cps = 4 / nr cells
+// of the group's list are evaluated at once, one per slot. A row holds its cell's two expansions
+// in VGPRs (row-broadcast layout below), fetched one step ahead of their use; every product with a
+// coefficient is a DPP row-broadcast fused multiply-add, so no lane re-reads the coefficients from
+// memory or LDS. Entries from u_far[g] on are evaluated to degree PF. The cell slots of each target
+// are summed in slot order, then the L2P of the group's level-(L-2) ancestor is added and phi,
+// phi_v are written.
 
-// Stages of list entries [b, e) of a group, evaluated to degree Q <= P (expansions stored to P):
-// the first fcount(Q) coefficients of each set are staged, one cell per slice.
-template <int P, int Q, class R>
-__device__ __forceinline__ void m2p_stages(int b, int e, int nsl, int slice, int oc, double4 y,
-                                           const int32_t *__restrict__ u_cell, const uint8_t *__restrict__ u_mask,
-                                           const double4 *__restrict__ cen, const double2 *__restrict__ M,
-                                           double mscale, typename vec2_of<R>::type *sM, double4 *sC, int *sMask,
-                                           double &pu, double &pv)
+// Row-broadcast fused multiply-add: acc += c[lane L of this lane's row of 16] * x, through DPP
+// row_newbcast on src0 of v_fmac_f64 / v_fmac_f32 (gfx950). Rules for the callers: EXEC is full
+// wherever these run (the broadcasting lane must be active), and a VALU write of a coefficient
+// register is separated from its first DPP read by row_fence (two wait states).
+template <int L>
+__device__ __forceinline__ void fmac_row(double &acc, double c, double x)
 {
-    constexpr int NCo = fcount(P), NQ = fcount(Q), CWQ = 2 * NQ, NT = 256;
-    const int tid = threadIdx.x;
-    for (int cb = b; cb < e; cb += nsl) {
-        const int nch = min(nsl, e - cb);
-        for (int i = tid; i < nch * CWQ; i += NT) {
-            const int jj = i / CWQ, r = i - jj * CWQ, set = r >= NQ ? 1 : 0;
-            const double2 v = M[((size_t)u_cell[cb + jj] * 2 + set) * NCo + (r - set * NQ)];
-            sM[i].x = (R)(v.x * mscale);
-            sM[i].y = (R)(v.y * mscale);
+    asm("v_fmac_f64_dpp %0, %1, %2 row_newbcast:%3 row_mask:0xf bank_mask:0xf" : "+v"(acc) : "v"(c), "v"(x), "i"(L));
+}
+template <int L>
+__device__ __forceinline__ void fmac_row(float &acc, float c, float x)
+{
+    asm("v_fmac_f32_dpp %0, %1, %2 row_newbcast:%3 row_mask:0xf bank_mask:0xf" : "+v"(acc) : "v"(c), "v"(x), "i"(L));
+}
+template <class R>
+__device__ __forceinline__ void row_fence(R &c)
+{
+    asm volatile("s_nop 1" : "+v"(c));
+}
+
+template <int B, int E, class F>
+__device__ __forceinline__ void static_for(F &&f)
+{
+    if constexpr (B < E) {
+        f(std::integral_constant<int, B>());
+        static_for<B + 1, E>(f);
+    }
+}
+
+// M2P of the two expansions of a row's cell, degree Q, at one target per lane (x in cell widths):
+// su = sum M_u,n^m conj(I_n^m(x)), sv likewise. Coefficient double k of a set (k = 2 fidx(n, m),
+// + 1 for the imaginary part) is register k / 16 of lane k % 16 of the row. The irregular
+// harmonics are streamed column by column (diagonal I_m^m, then I_{m+1}^m and the three-term
+// recurrence in n), the recurrence factors of each degree shared by all columns; the m = 0 column
+// is real (M_n^0 and I_n^0 are real for real charges).
+template <int Q, class R, int NQ>
+__device__ __forceinline__ void m2p_rows(const R (&cu)[NQ], const R (&cv)[NQ], R x, R y, R z, R &su, R &sv)
+{
+    const R r2 = x * x + y * y + z * z;
+    const R ir2 = fast_inv(r2), zr = z * ir2, zr2 = zr + zr, ir22 = ir2 + ir2;
+    // tz[n] = (2n - 1) z / r^2 and un[n] = (n - 1)^2 / r^2; column m takes
+    // un[n] - m^2 / r^2 = (n + m - 1)(n - m - 1) / r^2
+    R tz[Q + 1], un[Q + 1];
+    tz[1] = zr;
+    un[1] = R(0);
+    {
+        R st = ir2;
+#pragma unroll
+        for (int n = 2; n <= Q; n++) {
+            tz[n] = tz[n - 1] + zr2;
+            un[n] = un[n - 1] + st;
+            st += ir22;
         }
-        if (tid < nch) {
-            sC[tid] = cen[u_cell[cb + tid]];
-            sMask[tid] = u_mask[cb + tid];
+    }
+    R dre = fast_rsqrt(r2), dim = R(0);
+    R au = R(0), av = R(0);
+    fmac_row<0>(au, cu[0], dre);
+    fmac_row<0>(av, cv[0], dre);
+    if constexpr (Q >= 1) {
+        constexpr int k1 = 2 * fidx(1, 0);
+        R p2 = dre, p1 = zr * dre;
+        fmac_row<k1 % 16>(au, cu[k1 / 16], p1);
+        fmac_row<k1 % 16>(av, cv[k1 / 16], p1);
+        static_for<2, Q + 1>([&](auto nc) {
+            constexpr int n = decltype(nc)::value, k = 2 * fidx(n, 0);
+            const R c = tz[n] * p1 - un[n] * p2;
+            fmac_row<k % 16>(au, cu[k / 16], c);
+            fmac_row<k % 16>(av, cv[k / 16], c);
+            p2 = p1;
+            p1 = c;
+        });
+    }
+    R hu = R(0), hv = R(0);
+    R fd = -ir2, vm = R(0), dv = ir2;  // fd = (1 - 2m) / r^2, vm = m^2 / r^2
+    static_for<1, Q + 1>([&](auto mc) {
+        constexpr int m = decltype(mc)::value, kd = 2 * fidx(m, m);
+        const R nre = fd * (x * dre - y * dim), nim = fd * (x * dim + y * dre);
+        fd -= ir22;
+        vm += dv;
+        dv += ir22;
+        dre = nre;
+        dim = nim;
+        fmac_row<kd % 16>(hu, cu[kd / 16], dre);
+        fmac_row<(kd + 1) % 16>(hu, cu[(kd + 1) / 16], dim);
+        fmac_row<kd % 16>(hv, cv[kd / 16], dre);
+        fmac_row<(kd + 1) % 16>(hv, cv[(kd + 1) / 16], dim);
+        if constexpr (m + 1 <= Q) {
+            constexpr int k1 = 2 * fidx(m + 1, m);
+            R p2re = dre, p2im = dim, p1re = tz[m + 1] * dre, p1im = tz[m + 1] * dim;
+            fmac_row<k1 % 16>(hu, cu[k1 / 16], p1re);
+            fmac_row<(k1 + 1) % 16>(hu, cu[(k1 + 1) / 16], p1im);
+            fmac_row<k1 % 16>(hv, cv[k1 / 16], p1re);
+            fmac_row<(k1 + 1) % 16>(hv, cv[(k1 + 1) / 16], p1im);
+            static_for<m + 2, Q + 1>([&](auto nc) {
+                constexpr int n = decltype(nc)::value, k = 2 * fidx(n, m);
+                const R bb = un[n] - vm;
+                const R cre = tz[n] * p1re - bb * p2re, cim = tz[n] * p1im - bb * p2im;
+                fmac_row<k % 16>(hu, cu[k / 16], cre);
+                fmac_row<(k + 1) % 16>(hu, cu[(k + 1) / 16], cim);
+                fmac_row<k % 16>(hv, cv[k / 16], cre);
+                fmac_row<(k + 1) % 16>(hv, cv[(k + 1) / 16], cim);
+                p2re = p1re;
+                p2im = p1im;
+                p1re = cre;
+                p1im = cim;
+            });
         }
-        __syncthreads();
-        if (slice < nch && ((sMask[slice] >> oc) & 1)) {
-            const double4 c = sC[slice];
-            R su = 0, sv = 0;
-            m2p_eval2_r<Q, R>(sM + slice * CWQ, sM + slice * CWQ + NQ, (R)((y.x - c.x) * c.w), (R)((y.y - c.y) * c.w),
-                              (R)((y.z - c.z) * c.w), su, sv);
-            pu += (double)su * c.w;
-            pv += (double)sv * c.w;
+    });
+    su = au + (hu + hu);
+    sv = av + (hv + hv);
+}
+
+// One row's source cell of a step: both expansions in row-broadcast layout (degree Q of order-P
+// storage), its centre and 1/W, and whether it applies to this lane's target.
+template <int P, int Q>
+struct M2pCell {
+    static constexpr int NK = 2 * fcount(Q), NQ = (NK + 15) / 16;
+    double raw[2][NQ];
+    double4 c;
+    bool on;
+};
+
+// List entries [b, e) of a group, evaluated to degree Q <= P (expansions stored to P): step s puts
+// entry b + s cps + slot in each cell slot; the next step's cells are loaded while the current one
+// is evaluated. Every lane runs every step (invalid lanes evaluate at a dummy offset and add 0).
+template <int P, int Q, class R>
+__device__ __forceinline__ void m2p_range(int b, int e, int cps, int slot, bool tv, int oc, double4 y,
+                                          const int32_t *__restrict__ u_cell, const uint8_t *__restrict__ u_mask,
+                                          const double4 *__restrict__ cen, const double *__restrict__ Md,
+                                          double mscale, double &pu, double &pv)
+{
+    using C = M2pCell<P, Q>;
+    constexpr int NCo = fcount(P), NK = C::NK, NQ = C::NQ;
+    const int j = threadIdx.x & 15;
+    const int nst = (e - b + cps - 1) / cps;  // uniform over the wavefront
+    auto fetch = [&](int s, C &cl) {
+        const int ei = b + s * cps + slot;
+        const bool ev = slot < cps && ei < e;
+        const int ec = ev ? ei : b;
+        const int cell = u_cell[ec];
+        cl.c = cen[cell];
+        cl.on = tv && ev && ((u_mask[ec] >> oc) & 1);
+        const double *src = Md + (size_t)cell * 4 * NCo;
+#pragma unroll
+        for (int q = 0; q < NQ; q++) {
+            // lanes past the last coefficient repeat it; no DPP read takes them
+            const int k = min(16 * q + j, NK - 1);
+            cl.raw[0][q] = src[k];
+            cl.raw[1][q] = src[2 * NCo + k];
         }
-        __syncthreads();
+    };
+    auto eval = [&](C &cl) {
+        R cf[2][NQ];
+#pragma unroll
+        for (int st = 0; st < 2; st++)
+#pragma unroll
+            for (int q = 0; q < NQ; q++) {
+                if constexpr (sizeof(R) == 8)
+                    cf[st][q] = cl.raw[st][q];
+                else
+                    cf[st][q] = (R)(cl.raw[st][q] * mscale);
+                row_fence(cf[st][q]);
+            }
+        const double4 c = cl.c;
+        const R x = cl.on ? (R)((y.x - c.x) * c.w) : R(2), yy = cl.on ? (R)((y.y - c.y) * c.w) : R(0),
+                z = cl.on ? (R)((y.z - c.z) * c.w) : R(0);
+        R su, sv;
+        m2p_rows<Q, R, NQ>(cf[0], cf[1], x, yy, z, su, sv);
+        pu += cl.on ? (double)su * c.w : 0.0;
+        pv += cl.on ? (double)sv * c.w : 0.0;
+    };
+    for (int s = 0; s < nst; s++) {
+        C A;
+        fetch(s, A);
+        eval(A);
     }
 }
 
 template <int P, int PF, class R>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k_fmm_m2p(const int32_t *__restrict__ grp_tb, const int32_t *__restrict__ grp_te,
+__global__ __launch_bounds__(256) void k_fmm_m2p(int ng, const int32_t *__restrict__ grp_tb,
+                                                 const int32_t *__restrict__ grp_te,
                                                  const int32_t *__restrict__ grp_lcell,
                                                  const int32_t *__restrict__ u_ptr, const int32_t *__restrict__ u_far,
                                                  const int32_t *__restrict__ u_cell,
@@ -1178,75 +1323,63 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k
                                                  const double2 *__restrict__ pot, const int32_t *__restrict__ tgt_node,
                                                  NodeRec *__restrict__ nxt, double mscale)
 {
-    using R2 = typename vec2_of<R>::type;
-    constexpr int NCo = fcount(P), CW = 2 * NCo, CH = m2p_stage_cells<R>(), NT = 256;
-    __shared__ R2 sM[CH * CW];
-    __shared__ double4 sC[CH];
-    __shared__ int sMask[CH];
-    __shared__ double2 red[NT];
-    const int g = xcd_block(blockIdx.x, gridDim.x);
-    const int tid = threadIdx.x;
+    constexpr int NCo = fcount(P), CW = 2 * NCo;
+    const int wv = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
+    const int g = xcd_block(blockIdx.x, gridDim.x) * 4 + wv;
+    if (g >= ng) return;  // whole wavefront
+    const int lane = threadIdx.x & 63, row = lane >> 4, j = lane & 15;
     const int tb = grp_tb[g], te = grp_te[g], ub = u_ptr[g], ue = u_ptr[g + 1], uf = u_far[g], lc = grp_lcell[g];
+    const double *Md = (const double *)M;
     for (int t0 = tb; t0 < te; t0 += 64) {
-        const int n = min(64, te - t0);
-        const int nsl = min(NT / n, CH);
-        const int slot = tid % n, slice = tid / n;
-        double4 y = make_double4(0, 0, 0, 0);
-        int oc = 0;
-        if (slice < nsl) {
-            y = tpos[t0 + slot];
-            oc = tgt_oct[t0 + slot];
-        }
+        const int nt = min(64, te - t0), nr = (nt + 15) >> 4, cps = 4 / nr;
+        const int tri = row % nr, slot = row / nr;
+        const int t = t0 + 16 * tri + j;
+        const bool tv = slot < cps && t < t0 + nt;
+        const double4 y = tpos[tv ? t : t0];
+        const int oc = tv ? tgt_oct[t] : 0;
         double pu = 0, pv = 0;
         if constexpr (PF == P) {
-            m2p_stages<P, P, R>(ub, ue, nsl, slice, oc, y, u_cell, u_mask, cen, M, mscale, sM, sC, sMask, pu, pv);
+            m2p_range<P, P, R>(ub, ue, cps, slot, tv, oc, y, u_cell, u_mask, cen, Md, mscale, pu, pv);
         } else {
-            m2p_stages<P, P, R>(ub, uf, nsl, slice, oc, y, u_cell, u_mask, cen, M, mscale, sM, sC, sMask, pu, pv);
-            m2p_stages<P, PF, R>(uf, ue, nsl, slice, oc, y, u_cell, u_mask, cen, M, mscale, sM, sC, sMask, pu, pv);
+            m2p_range<P, P, R>(ub, uf, cps, slot, tv, oc, y, u_cell, u_mask, cen, Md, mscale, pu, pv);
+            m2p_range<P, PF, R>(uf, ue, cps, slot, tv, oc, y, u_cell, u_mask, cen, Md, mscale, pu, pv);
         }
-        red[tid] = make_double2(pu, pv);
-        __syncthreads();
-        if (tid < n) {
-            double su = 0, sv = 0;
-            for (int s = 0; s < nsl; s++) {
-                const double2 r = red[s * n + tid];
-                su += r.x;
-                sv += r.y;
-            }
+        // each target's cell slots, in slot order (all lanes take part in the shuffles)
+        double su = 0, sv = 0;
+        for (int q = 0; q < cps; q++) {
+            const int src = ((q * nr + tri) << 4) + j;
+            su += __shfl(pu, src, 64);
+            sv += __shfl(pv, src, 64);
+        }
+        if (slot == 0 && tv) {
             su /= mscale;
             sv /= mscale;
-            const int t = t0 + tid;
             if (lc >= 0) {
-                const double4 yy = tpos[t], c = cen[lc];
+                const double4 c = cen[lc];
                 const double iw = c.w;
-                l2p_eval2_t<P>(Lx + (size_t)lc * CW, Lx + (size_t)lc * CW + NCo, (yy.x - c.x) * iw, (yy.y - c.y) * iw,
-                               (yy.z - c.z) * iw, su, sv);
+                l2p_eval2_t<P>(Lx + (size_t)lc * CW, Lx + (size_t)lc * CW + NCo, (y.x - c.x) * iw, (y.y - c.y) * iw,
+                               (y.z - c.z) * iw, su, sv);
             }
             const double2 pn = pot[t];
             NodeRec *r = nxt + tgt_node[t];
             r->phi = (pn.x + su) / (4 * M_PI);
             r->phiv = (pn.y + sv) / (4 * M_PI);
         }
-        __syncthreads();
     }
 }
 
 void fmm_build_near(Ctx &c, FmmPlan &P);
 
-// Lane slots executed by k_fmm_m2p's mapping: per 64-target chunk of a group and per stage of nsl
-// source cells, the wavefronts that hold the stage's n * nch (target, cell) lanes.
+// Lane slots executed by k_fmm_m2p's mapping: per 64-target chunk of a group, one wavefront per
+// step of cps cells (degree-P and far entries stepped separately).
 static int64_t m2p_lane_slots(const FmmPlan &P)
 {
-    constexpr int NT = 256;
-    const int CH = P.mixed ? m2p_stage_cells<float>() : m2p_stage_cells<double>();
     int64_t slots = 0;
     for (size_t g = 0; g < P.grp_tb.size(); g++)
         for (int t0 = P.grp_tb[g]; t0 < P.grp_te[g]; t0 += 64) {
-            const int n = std::min(64, P.grp_te[g] - t0), nsl = std::min(NT / n, CH);
-            for (int cb = P.u_ptr[g]; cb < P.u_ptr[g + 1]; cb += nsl) {
-                const int nch = std::min(nsl, P.u_ptr[g + 1] - cb);
-                slots += 64 * (int64_t)((n * nch + 63) / 64);
-            }
+            const int nt = std::min(64, P.grp_te[g] - t0), cps = 4 / ((nt + 15) / 16);
+            const int near = P.u_far[g] - P.u_ptr[g], far = P.u_ptr[g + 1] - P.u_far[g];
+            slots += 64 * (int64_t)((near + cps - 1) / cps + (far + cps - 1) / cps);
         }
     return slots;
 }
@@ -1731,7 +1864,7 @@ static void run_fmm(Ctx &c)
         const int ng = (int)P.grp_tb.size();
         if (ng > 0) {
             auto go = [&](auto kern, double ms) {
-                kern<<<ng, 256, 0, c.stream>>>(P.d_grp_tb, P.d_grp_te, P.d_grp_lcell, P.d_u_ptr, P.d_u_far, P.d_u_cell,
+                kern<<<(ng + 3) / 4, 256, 0, c.stream>>>(ng, P.d_grp_tb, P.d_grp_te, P.d_grp_lcell, P.d_u_ptr, P.d_u_far, P.d_u_cell,
                                                P.d_u_mask, P.d_tgt_oct, P.d_tgt_pos, P.d_cen, P.d_M, P.d_L, P.d_pot,
                                                P.d_tgt_node, c.rec[1 - c.cur], ms);
             };
