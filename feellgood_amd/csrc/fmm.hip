@@ -1100,14 +1100,23 @@ __global__ __launch_bounds__(256) void k_nf_apply(int ntl, const int32_t *__rest
 #pragma unroll
         for (int i = 0; i < 4; i++) vb[i] = nf_ptr[t0 + min(i, nt - 1)];
         double au[4] = {0, 0, 0, 0}, av[4] = {0, 0, 0, 0};
-        for (int j = lane; j < nc; j += 64) {
-            const double2 q = qe[lc_col[cb + j]];
+        constexpr int U = 2;  // column chunks per iteration: both chunks' gathers are issued first
+        for (int j0 = lane; j0 < nc; j0 += 64 * U) {
+            int col[U];
+#pragma unroll
+            for (int u = 0; u < U; u++) col[u] = j0 + 64 * u < nc ? lc_col[cb + j0 + 64 * u] : -1;
+            double2 q[U];
+#pragma unroll
+            for (int u = 0; u < U; u++) q[u] = col[u] >= 0 ? qe[col[u]] : make_double2(0.0, 0.0);
 #pragma unroll
             for (int i = 0; i < 4; i++)
                 if (i < nt) {
-                    const double v = (double)nf_val[vb[i] + j];
-                    au[i] += v * q.x;
-                    av[i] += v * q.y;
+#pragma unroll
+                    for (int u = 0; u < U; u++) {
+                        const double v = col[u] >= 0 ? (double)nf_val[vb[i] + j0 + 64 * u] : 0.0;
+                        au[i] += v * q[u].x;
+                        av[i] += v * q[u].y;
+                    }
                 }
         }
 #pragma unroll
@@ -1126,11 +1135,11 @@ __global__ __launch_bounds__(256) void k_nf_apply(int ntl, const int32_t *__rest
 // The wavefront's four rows of 16 lanes are (target row, cell slot) pairs: with nt targets in the
 // current chunk of at most 64, nr = ceil(nt / 16) rows cover the targets and cps = 4 / nr cells
 // of the group's list are evaluated at once, one per slot. A row holds its cell's two expansions
-// in VGPRs (row-broadcast layout below), fetched one step ahead of their use; every product with a
-// coefficient is a DPP row-broadcast fused multiply-add, so no lane re-reads the coefficients from
-// memory or LDS. Entries from u_far[g] on are evaluated to degree PF. The cell slots of each target
-// are summed in slot order, then the L2P of the group's level-(L-2) ancestor is added and phi,
-// phi_v are written.
+// in VGPRs (row-broadcast layout below), loaded at each step; every product with a coefficient is
+// a DPP row-broadcast fused multiply-add, so no lane re-reads the coefficients from memory or LDS.
+// Entries from u_far[g] on are evaluated to degree PF. The cell slots of each target are summed in
+// slot order, then the L2P of the group's level-(L-2) ancestor is added and phi, phi_v are
+// written.
 
 // Row-broadcast fused multiply-add: acc += c[lane L of this lane's row of 16] * x, through DPP
 // row_newbcast on src0 of v_fmac_f64 / v_fmac_f32 (gfx950). Rules for the callers: EXEC is full
@@ -1255,8 +1264,8 @@ struct M2pCell {
 };
 
 // List entries [b, e) of a group, evaluated to degree Q <= P (expansions stored to P): step s puts
-// entry b + s cps + slot in each cell slot; the next step's cells are loaded while the current one
-// is evaluated. Every lane runs every step (invalid lanes evaluate at a dummy offset and add 0).
+// entry b + s cps + slot in each cell slot (other wavefronts on the SIMD hide the loads' latency).
+// Every lane runs every step (invalid lanes evaluate at a dummy offset and add 0).
 template <int P, int Q, class R>
 __device__ __forceinline__ void m2p_range(int b, int e, int cps, int slot, bool tv, int oc, double4 y,
                                           const int32_t *__restrict__ u_cell, const uint8_t *__restrict__ u_mask,
