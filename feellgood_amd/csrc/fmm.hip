@@ -94,6 +94,11 @@ struct FmmPlan {
     double mscale = 1.0;         // multipoles times mscale (a power of two) in the FP32 pass
     int64_t m2p_pairs = 0, m2l_pairs = 0, lc_cols = 0;
     int64_t m2p_slots = 0, m2p_far_pairs = 0;  // lane slots the M2P launch mapping executes (useful pairs / slots = SIMD efficiency)
+    // M2P launches: the groups of more than M2P_HALF_MAX targets (one cell per row), then the
+    // smaller ones (one cell per half row)
+    std::vector<int32_t> m2p_gid;
+    int m2p_n1 = 0;
+    int32_t *d_m2p_gid = nullptr;
     int64_t nf_nnz = 0;
     int32_t *d_tgt_node = nullptr;
     double4 *d_tgt_pos = nullptr;
@@ -752,7 +757,8 @@ __device__ __forceinline__ double sum8_dpp(double v)
 }
 
 // P2M: wavefronts over 64 / G source leaves; G lanes per leaf, each a strided subset of the leaf's
-// sources. Launch H accumulates the coefficient columns of half H (a balanced split,
+// sources, the next source's position and charge loaded while the current one is accumulated.
+// Launch H accumulates the coefficient columns of half H (a balanced split,
 // p2m_half_mask), so each lane holds about half of one expansion; the lanes' sums are combined
 // with cross-lane sums. blockIdx.y = charge set.
 template <int P, int G, unsigned MASK>
@@ -768,10 +774,16 @@ __device__ __forceinline__ void p2m_half(int li, int lane, int set, bool act, in
     const double4 c = cen[cell];
     const double iw = c.w;
     const int kb = act ? sb[li] + lane : 0, ke = act ? se[li] : 0;
+    double4 x = kb < ke ? spos[kb] : make_double4(0, 0, 0, 0);
+    double qn = kb < ke ? (set ? q[kb].y : q[kb].x) : 0.0;
     for (int k = kb; k < ke; k += G) {
-        const double4 x = spos[k];
-        const double2 qq = q[k];
-        p2m_accumulate_cols<P, MASK>((x.x - c.x) * iw, (x.y - c.y) * iw, (x.z - c.z) * iw, set ? qq.y : qq.x, acc);
+        const double4 xc = x;
+        const double qc = qn;
+        if (k + G < ke) {  // next source's position and charge, loaded under this one's work
+            x = spos[k + G];
+            qn = set ? q[k + G].y : q[k + G].x;
+        }
+        p2m_accumulate_cols<P, MASK>((xc.x - c.x) * iw, (xc.y - c.y) * iw, (xc.z - c.z) * iw, qc, acc);
     }
     static_assert(G == 8, "the cross-lane sum spans 8 lanes");
 #pragma unroll
@@ -1142,18 +1154,37 @@ __global__ __launch_bounds__(256) void k_nf_apply(int ntl, const int32_t *__rest
 // written.
 
 // Row-broadcast fused multiply-add: acc += c[lane L of this lane's row of 16] * x, through DPP
-// row_newbcast on src0 of v_fmac_f64 / v_fmac_f32 (gfx950). Rules for the callers: EXEC is full
-// wherever these run (the broadcasting lane must be active), and a VALU write of a coefficient
-// register is separated from its first DPP read by row_fence (two wait states).
-template <int L>
+// row_newbcast on src0 of v_fmac_f64 / v_fmac_f32 (gfx950), written only in the lanes of the banks
+// (groups of 4 lanes of a row) set in BM. Rules for the callers: EXEC is full wherever these run
+// (the broadcasting lane must be active), and a VALU write of a coefficient register is separated
+// from its first DPP read by row_fence (two wait states).
+template <int L, int BM = 15>
 __device__ __forceinline__ void fmac_row(double &acc, double c, double x)
 {
-    asm("v_fmac_f64_dpp %0, %1, %2 row_newbcast:%3 row_mask:0xf bank_mask:0xf" : "+v"(acc) : "v"(c), "v"(x), "i"(L));
+    asm("v_fmac_f64_dpp %0, %1, %2 row_newbcast:%3 row_mask:0xf bank_mask:%4"
+        : "+v"(acc)
+        : "v"(c), "v"(x), "i"(L), "i"(BM));
 }
-template <int L>
+template <int L, int BM = 15>
 __device__ __forceinline__ void fmac_row(float &acc, float c, float x)
 {
-    asm("v_fmac_f32_dpp %0, %1, %2 row_newbcast:%3 row_mask:0xf bank_mask:0xf" : "+v"(acc) : "v"(c), "v"(x), "i"(L));
+    asm("v_fmac_f32_dpp %0, %1, %2 row_newbcast:%3 row_mask:0xf bank_mask:%4"
+        : "+v"(acc)
+        : "v"(c), "v"(x), "i"(L), "i"(BM));
+}
+// acc += (coefficient double K of this lane's cell) * x. H = 1: one cell per row, K held by lane
+// K % 16 of the row in register K / 16. H = 2: one cell per half row (lanes 0-7, 8-15), K held by
+// lane 8 h + K % 8 of half h in register K / 8; each half takes its own broadcast (two masked
+// instructions).
+template <int H, int K, class R, int NQ>
+__device__ __forceinline__ void fmac_coef(R &acc, const R (&c)[NQ], R x)
+{
+    if constexpr (H == 1) {
+        fmac_row<K % 16>(acc, c[K / 16], x);
+    } else {
+        fmac_row<K % 8, 0x3>(acc, c[K / 8], x);
+        fmac_row<8 + K % 8, 0xc>(acc, c[K / 8], x);
+    }
 }
 template <class R>
 __device__ __forceinline__ void row_fence(R &c)
@@ -1172,11 +1203,11 @@ __device__ __forceinline__ void static_for(F &&f)
 
 // M2P of the two expansions of a row's cell, degree Q, at one target per lane (x in cell widths):
 // su = sum M_u,n^m conj(I_n^m(x)), sv likewise. Coefficient double k of a set (k = 2 fidx(n, m),
-// + 1 for the imaginary part) is register k / 16 of lane k % 16 of the row. The irregular
+// + 1 for the imaginary part) sits where fmac_coef<H, k> reads it. The irregular
 // harmonics are streamed column by column (diagonal I_m^m, then I_{m+1}^m and the three-term
 // recurrence in n), the recurrence factors of each degree shared by all columns; the m = 0 column
 // is real (M_n^0 and I_n^0 are real for real charges).
-template <int Q, class R, int NQ>
+template <int Q, int H, class R, int NQ>
 __device__ __forceinline__ void m2p_rows(const R (&cu)[NQ], const R (&cv)[NQ], R x, R y, R z, R &su, R &sv)
 {
     const R r2 = x * x + y * y + z * z;
@@ -1197,18 +1228,18 @@ __device__ __forceinline__ void m2p_rows(const R (&cu)[NQ], const R (&cv)[NQ], R
     }
     R dre = fast_rsqrt(r2), dim = R(0);
     R au = R(0), av = R(0);
-    fmac_row<0>(au, cu[0], dre);
-    fmac_row<0>(av, cv[0], dre);
+    fmac_coef<H, 0>(au, cu, dre);
+    fmac_coef<H, 0>(av, cv, dre);
     if constexpr (Q >= 1) {
         constexpr int k1 = 2 * fidx(1, 0);
         R p2 = dre, p1 = zr * dre;
-        fmac_row<k1 % 16>(au, cu[k1 / 16], p1);
-        fmac_row<k1 % 16>(av, cv[k1 / 16], p1);
+        fmac_coef<H, k1>(au, cu, p1);
+        fmac_coef<H, k1>(av, cv, p1);
         static_for<2, Q + 1>([&](auto nc) {
             constexpr int n = decltype(nc)::value, k = 2 * fidx(n, 0);
             const R c = tz[n] * p1 - un[n] * p2;
-            fmac_row<k % 16>(au, cu[k / 16], c);
-            fmac_row<k % 16>(av, cv[k / 16], c);
+            fmac_coef<H, k>(au, cu, c);
+            fmac_coef<H, k>(av, cv, c);
             p2 = p1;
             p1 = c;
         });
@@ -1223,25 +1254,25 @@ __device__ __forceinline__ void m2p_rows(const R (&cu)[NQ], const R (&cv)[NQ], R
         dv += ir22;
         dre = nre;
         dim = nim;
-        fmac_row<kd % 16>(hu, cu[kd / 16], dre);
-        fmac_row<(kd + 1) % 16>(hu, cu[(kd + 1) / 16], dim);
-        fmac_row<kd % 16>(hv, cv[kd / 16], dre);
-        fmac_row<(kd + 1) % 16>(hv, cv[(kd + 1) / 16], dim);
+        fmac_coef<H, kd>(hu, cu, dre);
+        fmac_coef<H, kd + 1>(hu, cu, dim);
+        fmac_coef<H, kd>(hv, cv, dre);
+        fmac_coef<H, kd + 1>(hv, cv, dim);
         if constexpr (m + 1 <= Q) {
             constexpr int k1 = 2 * fidx(m + 1, m);
             R p2re = dre, p2im = dim, p1re = tz[m + 1] * dre, p1im = tz[m + 1] * dim;
-            fmac_row<k1 % 16>(hu, cu[k1 / 16], p1re);
-            fmac_row<(k1 + 1) % 16>(hu, cu[(k1 + 1) / 16], p1im);
-            fmac_row<k1 % 16>(hv, cv[k1 / 16], p1re);
-            fmac_row<(k1 + 1) % 16>(hv, cv[(k1 + 1) / 16], p1im);
+            fmac_coef<H, k1>(hu, cu, p1re);
+            fmac_coef<H, k1 + 1>(hu, cu, p1im);
+            fmac_coef<H, k1>(hv, cv, p1re);
+            fmac_coef<H, k1 + 1>(hv, cv, p1im);
             static_for<m + 2, Q + 1>([&](auto nc) {
                 constexpr int n = decltype(nc)::value, k = 2 * fidx(n, m);
                 const R bb = un[n] - vm;
                 const R cre = tz[n] * p1re - bb * p2re, cim = tz[n] * p1im - bb * p2im;
-                fmac_row<k % 16>(hu, cu[k / 16], cre);
-                fmac_row<(k + 1) % 16>(hu, cu[(k + 1) / 16], cim);
-                fmac_row<k % 16>(hv, cv[k / 16], cre);
-                fmac_row<(k + 1) % 16>(hv, cv[(k + 1) / 16], cim);
+                fmac_coef<H, k>(hu, cu, cre);
+                fmac_coef<H, k + 1>(hu, cu, cim);
+                fmac_coef<H, k>(hv, cv, cre);
+                fmac_coef<H, k + 1>(hv, cv, cim);
                 p2re = p1re;
                 p2im = p1im;
                 p1re = cre;
@@ -1253,11 +1284,12 @@ __device__ __forceinline__ void m2p_rows(const R (&cu)[NQ], const R (&cv)[NQ], R
     sv = av + (hv + hv);
 }
 
-// One row's source cell of a step: both expansions in row-broadcast layout (degree Q of order-P
-// storage), its centre and 1/W, and whether it applies to this lane's target.
-template <int P, int Q>
+// The source cell of a row (H = 1) or half row (H = 2) in a step: both expansions in
+// row-broadcast layout (degree Q of order-P storage), its centre and 1/W, and whether it applies
+// to this lane's target.
+template <int P, int Q, int H>
 struct M2pCell {
-    static constexpr int NK = 2 * fcount(Q), NQ = (NK + 15) / 16;
+    static constexpr int LPC = 16 / H, NK = 2 * fcount(Q), NQ = (NK + LPC - 1) / LPC;
     double raw[2][NQ];
     double4 c;
     bool on;
@@ -1266,15 +1298,15 @@ struct M2pCell {
 // List entries [b, e) of a group, evaluated to degree Q <= P (expansions stored to P): step s puts
 // entry b + s cps + slot in each cell slot (other wavefronts on the SIMD hide the loads' latency).
 // Every lane runs every step (invalid lanes evaluate at a dummy offset and add 0).
-template <int P, int Q, class R>
+template <int P, int Q, int H, class R>
 __device__ __forceinline__ void m2p_range(int b, int e, int cps, int slot, bool tv, int oc, double4 y,
                                           const int32_t *__restrict__ u_cell, const uint8_t *__restrict__ u_mask,
                                           const double4 *__restrict__ cen, const double *__restrict__ Md,
                                           double mscale, double &pu, double &pv)
 {
-    using C = M2pCell<P, Q>;
-    constexpr int NCo = fcount(P), NK = C::NK, NQ = C::NQ;
-    const int j = threadIdx.x & 15;
+    using C = M2pCell<P, Q, H>;
+    constexpr int NCo = fcount(P), LPC = C::LPC, NK = C::NK, NQ = C::NQ;
+    const int j = threadIdx.x & (LPC - 1);  // lane within the cell's lanes
     const int nst = (e - b + cps - 1) / cps;  // uniform over the wavefront
     auto fetch = [&](int s, C &cl) {
         const int ei = b + s * cps + slot;
@@ -1287,7 +1319,7 @@ __device__ __forceinline__ void m2p_range(int b, int e, int cps, int slot, bool 
 #pragma unroll
         for (int q = 0; q < NQ; q++) {
             // lanes past the last coefficient repeat it; no DPP read takes them
-            const int k = min(16 * q + j, NK - 1);
+            const int k = min(LPC * q + j, NK - 1);
             cl.raw[0][q] = src[k];
             cl.raw[1][q] = src[2 * NCo + k];
         }
@@ -1308,7 +1340,7 @@ __device__ __forceinline__ void m2p_range(int b, int e, int cps, int slot, bool 
         const R x = cl.on ? (R)((y.x - c.x) * c.w) : R(2), yy = cl.on ? (R)((y.y - c.y) * c.w) : R(0),
                 z = cl.on ? (R)((y.z - c.z) * c.w) : R(0);
         R su, sv;
-        m2p_rows<Q, R, NQ>(cf[0], cf[1], x, yy, z, su, sv);
+        m2p_rows<Q, H, R, NQ>(cf[0], cf[1], x, yy, z, su, sv);
         pu += cl.on ? (double)su * c.w : 0.0;
         pv += cl.on ? (double)sv * c.w : 0.0;
     };
@@ -1319,8 +1351,13 @@ __device__ __forceinline__ void m2p_range(int b, int e, int cps, int slot, bool 
     }
 }
 
-template <int P, int PF, class R>
-__global__ __launch_bounds__(256) void k_fmm_m2p(int ng, const int32_t *__restrict__ grp_tb,
+// H = 1: groups of any size, chunks of up to 64 targets as above. H = 2: groups of at most 8
+// targets; each half row holds its own cell, lanes 8 h + i of row r evaluate cell slot 2 r + h for
+// target i, so a step covers 8 cells (each coefficient product then takes two masked broadcasts).
+// gid lists the groups of the launch.
+template <int P, int PF, int H, class R>
+__global__ __launch_bounds__(256) void k_fmm_m2p(int ng, const int32_t *__restrict__ gid,
+                                                 const int32_t *__restrict__ grp_tb,
                                                  const int32_t *__restrict__ grp_te,
                                                  const int32_t *__restrict__ grp_lcell,
                                                  const int32_t *__restrict__ u_ptr, const int32_t *__restrict__ u_far,
@@ -1334,29 +1371,34 @@ __global__ __launch_bounds__(256) void k_fmm_m2p(int ng, const int32_t *__restri
 {
     constexpr int NCo = fcount(P), CW = 2 * NCo;
     const int wv = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
-    const int g = xcd_block(blockIdx.x, gridDim.x) * 4 + wv;
-    if (g >= ng) return;  // whole wavefront
+    const int gi = xcd_block(blockIdx.x, gridDim.x) * 4 + wv;
+    if (gi >= ng) return;  // whole wavefront
+    const int g = gid[gi];
     const int lane = threadIdx.x & 63, row = lane >> 4, j = lane & 15;
     const int tb = grp_tb[g], te = grp_te[g], ub = u_ptr[g], ue = u_ptr[g + 1], uf = u_far[g], lc = grp_lcell[g];
     const double *Md = (const double *)M;
     for (int t0 = tb; t0 < te; t0 += 64) {
-        const int nt = min(64, te - t0), nr = (nt + 15) >> 4, cps = 4 / nr;
-        const int tri = row % nr, slot = row / nr;
-        const int t = t0 + 16 * tri + j;
-        const bool tv = slot < cps && t < t0 + nt;
+        const int nt = min(64, te - t0);
+        // H = 1: nr target rows, cps = 4 / nr cells per step; H = 2: one target row of 8 lanes
+        // per half, 8 cells per step
+        const int nr = H == 1 ? (nt + 15) >> 4 : 1, cps = H == 1 ? 4 / nr : 8;
+        const int tri = H == 1 ? row % nr : 0, slot = H == 1 ? row / nr : 2 * row + (j >> 3);
+        const int ti = H == 1 ? 16 * tri + j : j & 7;
+        const int t = t0 + ti;
+        const bool tv = slot < cps && ti < nt;
         const double4 y = tpos[tv ? t : t0];
         const int oc = tv ? tgt_oct[t] : 0;
         double pu = 0, pv = 0;
         if constexpr (PF == P) {
-            m2p_range<P, P, R>(ub, ue, cps, slot, tv, oc, y, u_cell, u_mask, cen, Md, mscale, pu, pv);
+            m2p_range<P, P, H, R>(ub, ue, cps, slot, tv, oc, y, u_cell, u_mask, cen, Md, mscale, pu, pv);
         } else {
-            m2p_range<P, P, R>(ub, uf, cps, slot, tv, oc, y, u_cell, u_mask, cen, Md, mscale, pu, pv);
-            m2p_range<P, PF, R>(uf, ue, cps, slot, tv, oc, y, u_cell, u_mask, cen, Md, mscale, pu, pv);
+            m2p_range<P, P, H, R>(ub, uf, cps, slot, tv, oc, y, u_cell, u_mask, cen, Md, mscale, pu, pv);
+            m2p_range<P, PF, H, R>(uf, ue, cps, slot, tv, oc, y, u_cell, u_mask, cen, Md, mscale, pu, pv);
         }
         // each target's cell slots, in slot order (all lanes take part in the shuffles)
         double su = 0, sv = 0;
         for (int q = 0; q < cps; q++) {
-            const int src = ((q * nr + tri) << 4) + j;
+            const int src = H == 1 ? ((q * nr + tri) << 4) + j : ((q >> 1) << 4) + ((q & 1) << 3) + (j & 7);
             su += __shfl(pu, src, 64);
             sv += __shfl(pv, src, 64);
         }
@@ -1379,6 +1421,9 @@ __global__ __launch_bounds__(256) void k_fmm_m2p(int ng, const int32_t *__restri
 
 void fmm_build_near(Ctx &c, FmmPlan &P);
 
+// Groups of at most this many targets are evaluated with one source cell per half row.
+constexpr int M2P_HALF_MAX = 8;
+
 // Lane slots executed by k_fmm_m2p's mapping: per 64-target chunk of a group, one wavefront per
 // step of cps cells (degree-P and far entries stepped separately).
 static int64_t m2p_lane_slots(const FmmPlan &P)
@@ -1386,7 +1431,8 @@ static int64_t m2p_lane_slots(const FmmPlan &P)
     int64_t slots = 0;
     for (size_t g = 0; g < P.grp_tb.size(); g++)
         for (int t0 = P.grp_tb[g]; t0 < P.grp_te[g]; t0 += 64) {
-            const int nt = std::min(64, P.grp_te[g] - t0), cps = 4 / ((nt + 15) / 16);
+            const int nt = std::min(64, P.grp_te[g] - t0);
+            const int cps = P.grp_te[g] - P.grp_tb[g] <= M2P_HALF_MAX ? 8 : 4 / ((nt + 15) / 16);
             const int near = P.u_far[g] - P.u_ptr[g], far = P.u_ptr[g + 1] - P.u_far[g];
             slots += 64 * (int64_t)((near + cps - 1) / cps + (far + cps - 1) / cps);
         }
@@ -1612,6 +1658,13 @@ int fmm_create(Ctx &c)
         P->d_leaf_sb = dput(P->leaf_sb);
         P->d_leaf_se = dput(P->leaf_se);
         P->d_grp_tb = dput(P->grp_tb);
+        P->m2p_gid.clear();
+        for (int pass = 0; pass < 2; pass++) {
+            for (size_t g = 0; g < P->grp_tb.size(); g++)
+                if ((P->grp_te[g] - P->grp_tb[g] <= M2P_HALF_MAX) == (pass == 1)) P->m2p_gid.push_back((int32_t)g);
+            if (pass == 0) P->m2p_n1 = (int)P->m2p_gid.size();
+        }
+        P->d_m2p_gid = dput(P->m2p_gid);
         P->d_grp_te = dput(P->grp_te);
         P->d_grp_lcell = dput(P->grp_lcell);
         P->d_u_ptr = dput(P->u_ptr);
@@ -1797,7 +1850,7 @@ void fmm_destroy(Ctx &c)
 {
     FmmPlan *P = c.fmm;
     if (!P) return;
-    void *ptrs[] = {P->d_cen, P->d_leaf_sb, P->d_leaf_se, P->d_grp_tb, P->d_grp_te, P->d_grp_lcell, P->d_u_ptr,
+    void *ptrs[] = {P->d_cen, P->d_m2p_gid, P->d_leaf_sb, P->d_leaf_se, P->d_grp_tb, P->d_grp_te, P->d_grp_lcell, P->d_u_ptr,
                     P->d_u_far, P->d_u_cell, P->d_u_mask, P->d_tgt_oct, P->d_tgt_node, P->d_tgt_pos,
                     P->d_Tm2m, P->d_Tl2l, P->d_Tm2l, P->d_m2l_part, P->d_m2l_tile_cell, P->d_m2l_tile_chunk,
                     P->d_m2l_chunk_code, P->d_m2l_chunk_src, P->d_m2m_child, P->d_l2l_par, P->d_l2l_child, P->d_M, P->d_L, P->d_leaf_cell, P->d_m2m_cells,
@@ -1872,16 +1925,34 @@ static void run_fmm(Ctx &c)
         Prof pr(c, "fmm_m2p");
         const int ng = (int)P.grp_tb.size();
         if (ng > 0) {
-            auto go = [&](auto kern, double ms) {
-                kern<<<(ng + 3) / 4, 256, 0, c.stream>>>(ng, P.d_grp_tb, P.d_grp_te, P.d_grp_lcell, P.d_u_ptr, P.d_u_far, P.d_u_cell,
-                                               P.d_u_mask, P.d_tgt_oct, P.d_tgt_pos, P.d_cen, P.d_M, P.d_L, P.d_pot,
-                                               P.d_tgt_node, c.rec[1 - c.cur], ms);
+            // groups of more than M2P_HALF_MAX targets with one cell per row, then the others with
+            // one cell per half row
+            auto go = [&](auto k1, auto k2, double ms) {
+                const int n1 = P.m2p_n1, n2 = ng - P.m2p_n1;
+                if (n1 > 0)
+                    k1<<<(n1 + 3) / 4, 256, 0, c.stream>>>(n1, P.d_m2p_gid, P.d_grp_tb, P.d_grp_te, P.d_grp_lcell,
+                                                          P.d_u_ptr, P.d_u_far, P.d_u_cell, P.d_u_mask, P.d_tgt_oct,
+                                                          P.d_tgt_pos, P.d_cen, P.d_M, P.d_L, P.d_pot, P.d_tgt_node,
+                                                          c.rec[1 - c.cur], ms);
+                if (n2 > 0)
+                    k2<<<(n2 + 3) / 4, 256, 0, c.stream>>>(n2, P.d_m2p_gid + n1, P.d_grp_tb, P.d_grp_te,
+                                                          P.d_grp_lcell, P.d_u_ptr, P.d_u_far, P.d_u_cell, P.d_u_mask,
+                                                          P.d_tgt_oct, P.d_tgt_pos, P.d_cen, P.d_M, P.d_L, P.d_pot,
+                                                          P.d_tgt_node, c.rec[1 - c.cur], ms);
             };
+            constexpr int PL = fmm_far_degree(PP);
             const bool lowf = P.pf != PP;
-            if (P.mixed)
-                go(lowf ? k_fmm_m2p<PP, fmm_far_degree(PP), float> : k_fmm_m2p<PP, PP, float>, P.mscale);
-            else
-                go(lowf ? k_fmm_m2p<PP, fmm_far_degree(PP), double> : k_fmm_m2p<PP, PP, double>, 1.0);
+            if (P.mixed) {
+                if (lowf)
+                    go(k_fmm_m2p<PP, PL, 1, float>, k_fmm_m2p<PP, PL, 2, float>, P.mscale);
+                else
+                    go(k_fmm_m2p<PP, PP, 1, float>, k_fmm_m2p<PP, PP, 2, float>, P.mscale);
+            } else {
+                if (lowf)
+                    go(k_fmm_m2p<PP, PL, 1, double>, k_fmm_m2p<PP, PL, 2, double>, 1.0);
+                else
+                    go(k_fmm_m2p<PP, PP, 1, double>, k_fmm_m2p<PP, PP, 2, double>, 1.0);
+            }
         }
     }
 }
