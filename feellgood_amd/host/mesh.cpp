@@ -1,7 +1,7 @@
 /*
  * mesh.cpp -- host-side mesh input and preprocessing for the LLG engine.
  *
- * Reads ASCII gmsh files without the gmsh library, builds the structured synthetic meshes the
+ * Reads gmsh files (MSH 2.2 and 4.1, ASCII and binary) without the gmsh library, builds the structured synthetic meshes the
  * benchmarks use, and turns a raw mesh into the preprocessed form that Mesh::mesh's constructor
  * produces in FeeLLGood (src/mesh.h:38-135): coordinates scaled, tetrahedra positively oriented
  * (src/tetra.cpp:410-424), every boundary/interface triangle present with a signed Ms jump
@@ -70,30 +70,87 @@ struct fgh_mesh {
 
 namespace {
 
-bool read_msh22(std::istream &in, fgh_mesh *m)
+/* Values of a gmsh file section: whitespace-separated text (ASCII files) or native little-endian
+ * binary (binary files: int 4 B, size_t 8 B, double 8 B, as gmsh writes them). */
+struct Src {
+    std::istream &in;
+    bool bin;
+    template <class T> T raw()
+    {
+        T v{};
+        in.read(reinterpret_cast<char *>(&v), sizeof v);
+        return v;
+    }
+    long long i32() /* int */
+    {
+        if (bin) return raw<int32_t>();
+        long long v = 0;
+        in >> v;
+        return v;
+    }
+    long long u64() /* size_t (MSH 4.1 binary counts and tags) */
+    {
+        if (bin) return (long long)raw<uint64_t>();
+        long long v = 0;
+        in >> v;
+        return v;
+    }
+    double f64()
+    {
+        if (bin) return raw<double>();
+        double v = 0;
+        in >> v;
+        return v;
+    }
+    /* binary payloads start right after the newline that ends the section's text line */
+    void payload()
+    {
+        if (bin) in.get();
+    }
+};
+
+/* In a binary file, skip a section this reader does not use: up to its $End line. */
+void skip_section(std::istream &in, const std::string &name)
 {
+    const std::string end = "$End" + name.substr(1);
+    size_t k = 0;
+    char ch;
+    while (k < end.size() && in.get(ch)) k = ch == end[k] ? k + 1 : (ch == end[0] ? 1 : 0);
+}
+
+void read_physical_names(std::istream &in, fgh_mesh *m)
+{
+    int n;
+    in >> n;
+    for (int i = 0; i < n; i++) {
+        int dim, tag;
+        std::string name;
+        in >> dim >> tag;
+        std::getline(in, name);
+        auto a = name.find('"'), b = name.rfind('"');
+        if (a != std::string::npos && b > a) name = name.substr(a + 1, b - a - 1);
+        m->phys_names[{dim, tag}] = name;
+    }
+}
+
+int element_nodes(int type) { return type == 4 ? 4 : type == 2 ? 3 : type == 1 ? 2 : type == 15 ? 1 : -1; }
+
+bool read_msh22(std::istream &in, fgh_mesh *m, bool bin)
+{
+    Src src{in, bin};
     std::string tok;
     while (in >> tok) {
         if (tok == "$PhysicalNames") {
-            int n;
-            in >> n;
-            for (int i = 0; i < n; i++) {
-                int dim, tag;
-                std::string name;
-                in >> dim >> tag;
-                std::getline(in, name);
-                auto a = name.find('"'), b = name.rfind('"');
-                if (a != std::string::npos && b > a) name = name.substr(a + 1, b - a - 1);
-                m->phys_names[{dim, tag}] = name;
-            }
+            read_physical_names(in, m);
         } else if (tok == "$Nodes") {
             long n;
-            in >> n;
+            in >> n;  // the count line is text in both forms
+            src.payload();
             m->pos.assign(3 * n, 0.0);
             for (long i = 0; i < n; i++) {
-                long tag;
-                double x, y, z;
-                in >> tag >> x >> y >> z;
+                const long long tag = src.i32();
+                const double x = src.f64(), y = src.f64(), z = src.f64();
+                if (in.fail()) break;  // reported below
                 if (tag < 1 || tag > n) {
                     g_err = "node tag out of range: " + std::to_string(tag);
                     return false;
@@ -105,20 +162,9 @@ bool read_msh22(std::istream &in, fgh_mesh *m)
         } else if (tok == "$Elements") {
             long n;
             in >> n;
-            for (long i = 0; i < n; i++) {
-                long id;
-                int type, ntags;
-                in >> id >> type >> ntags;
-                std::vector<int> tags(ntags);
-                for (int k = 0; k < ntags; k++) in >> tags[k];
-                int nn = type == 4 ? 4 : type == 2 ? 3 : type == 1 ? 2 : type == 15 ? 1 : -1;
-                if (nn < 0) {
-                    g_err = "unsupported element type " + std::to_string(type);
-                    return false;
-                }
-                std::vector<int> nodes(nn);
-                for (int k = 0; k < nn; k++) in >> nodes[k];
-                if (ntags < 1 || tags[0] == 0) continue; /* not in a physical group */
+            src.payload();
+            auto add = [&](int type, const std::vector<int> &tags, const int *nodes) {
+                if (tags.empty() || tags[0] == 0) return; /* not in a physical group */
                 if (type == 4) {
                     for (int k = 0; k < 4; k++) m->tets.push_back(nodes[k] - 1);
                     m->tet_phys.push_back(tags[0]);
@@ -126,74 +172,106 @@ bool read_msh22(std::istream &in, fgh_mesh *m)
                     for (int k = 0; k < 3; k++) m->tris.push_back(nodes[k] - 1);
                     m->tri_phys.push_back(tags[0]);
                 }
+            };
+            long done = 0;
+            while (done < n) {
+                // text: one element per line (id type ntags tags nodes); binary: blocks of
+                // elements of one type (type, count, ntags header, then id tags nodes each)
+                long id = 0, follow = 1;
+                int type, ntags;
+                if (bin) {
+                    type = (int)src.i32();
+                    follow = (long)src.i32();
+                    ntags = (int)src.i32();
+                } else {
+                    in >> id >> type >> ntags;
+                }
+                const int nn = element_nodes(type);
+                if (nn < 0 || ntags < 0 || follow < 1 || done + follow > n) {
+                    g_err = nn < 0 ? "unsupported element type " + std::to_string(type) : "malformed $Elements section";
+                    return false;
+                }
+                for (long e = 0; e < follow; e++) {
+                    if (bin) id = (long)src.i32();
+                    std::vector<int> tags(ntags);
+                    for (int k = 0; k < ntags; k++) tags[k] = (int)src.i32();
+                    int nodes[4];
+                    for (int k = 0; k < nn; k++) nodes[k] = (int)src.i32();
+                    add(type, tags, nodes);
+                }
+                done += follow;
             }
+        } else if (bin && tok.size() > 1 && tok[0] == '$' && tok.rfind("$End", 0) != 0) {
+            skip_section(in, tok);
+        }
+        if (in.fail()) {
+            g_err = "truncated or malformed gmsh file (" + tok + ")";
+            return false;
         }
     }
     return true;
 }
 
-bool read_msh41(std::istream &in, fgh_mesh *m)
+bool read_msh41(std::istream &in, fgh_mesh *m, bool bin)
 {
+    Src src{in, bin};
     std::string tok;
     std::map<std::pair<int, int>, std::vector<int>> ent_phys; /* (dim, entity) -> physical tags */
     while (in >> tok) {
         if (tok == "$PhysicalNames") {
-            int n;
-            in >> n;
-            for (int i = 0; i < n; i++) {
-                int dim, tag;
-                std::string name;
-                in >> dim >> tag;
-                std::getline(in, name);
-                auto a = name.find('"'), b = name.rfind('"');
-                if (a != std::string::npos && b > a) name = name.substr(a + 1, b - a - 1);
-                m->phys_names[{dim, tag}] = name;
-            }
+            read_physical_names(in, m);  // text in both forms
         } else if (tok == "$Entities") {
-            long np, nc, ns, nv;
-            in >> np >> nc >> ns >> nv;
-            for (long i = 0; i < np; i++) {
-                int tag, nph;
-                double x, y, z;
-                in >> tag >> x >> y >> z >> nph;
-                std::vector<int> ph(nph);
-                for (int k = 0; k < nph; k++) in >> ph[k];
-                ent_phys[{0, tag}] = ph;
-            }
-            long counts[3] = {nc, ns, nv};
-            for (int dim = 1; dim <= 3; dim++)
-                for (long i = 0; i < counts[dim - 1]; i++) {
-                    int tag, nph, nb;
-                    double bb[6];
-                    in >> tag;
-                    for (double &v : bb) in >> v;
-                    in >> nph;
+            src.payload();
+            long long counts[4];
+            for (long long &c : counts) c = src.u64();
+            for (int dim = 0; dim <= 3; dim++)
+                for (long long i = 0; i < counts[dim]; i++) {
+                    const int tag = (int)src.i32();
+                    for (int k = 0; k < (dim == 0 ? 3 : 6); k++) src.f64();  // point, or bounding box
+                    const long long nph = src.u64();
+                    if (nph < 0 || nph > (1 << 20)) {
+                        g_err = "malformed $Entities section";
+                        return false;
+                    }
                     std::vector<int> ph(nph);
-                    for (int k = 0; k < nph; k++) in >> ph[k];
-                    in >> nb;
-                    for (int k = 0; k < nb; k++) {
-                        int t;
-                        in >> t;
+                    for (auto &t : ph) t = (int)src.i32();
+                    if (dim > 0) {
+                        const long long nb = src.u64();
+                        if (nb < 0 || nb > (1 << 20)) {
+                            g_err = "malformed $Entities section";
+                            return false;
+                        }
+                        for (long long k = 0; k < nb; k++) src.i32();
                     }
                     ent_phys[{dim, tag}] = ph;
                 }
         } else if (tok == "$Nodes") {
-            long nblocks, nnodes, mintag, maxtag;
-            in >> nblocks >> nnodes >> mintag >> maxtag;
+            src.payload();
+            const long long nblocks = src.u64(), nnodes = src.u64();
+            src.u64();
+            src.u64();  // min and max tag
+            if (nnodes < 0 || nblocks < 0) {
+                g_err = "malformed $Nodes section";
+                return false;
+            }
             m->pos.assign(3 * nnodes, 0.0);
-            for (long b = 0; b < nblocks; b++) {
-                int edim, etag, param;
-                long nb;
-                in >> edim >> etag >> param >> nb;
+            for (long long b = 0; b < nblocks; b++) {
+                src.i32();  // entity dimension
+                src.i32();  // entity tag
+                const long long param = src.i32(), nb = src.u64();
                 if (param) {
                     g_err = "parametric nodes are not supported";
                     return false;
                 }
-                std::vector<long> tags(nb);
-                for (long k = 0; k < nb; k++) in >> tags[k];
-                for (long k = 0; k < nb; k++) {
-                    double x, y, z;
-                    in >> x >> y >> z;
+                if (nb < 0 || nb > nnodes) {
+                    g_err = "malformed $Nodes section";
+                    return false;
+                }
+                std::vector<long long> tags(nb);
+                for (auto &t : tags) t = src.u64();
+                for (long long k = 0; k < nb; k++) {
+                    const double x = src.f64(), y = src.f64(), z = src.f64();
+                    if (in.fail()) break;  // reported below
                     if (tags[k] < 1 || tags[k] > nnodes) {
                         g_err = "node tag out of range: " + std::to_string(tags[k]);
                         return false;
@@ -204,23 +282,28 @@ bool read_msh41(std::istream &in, fgh_mesh *m)
                 }
             }
         } else if (tok == "$Elements") {
-            long nblocks, nelem, mintag, maxtag;
-            in >> nblocks >> nelem >> mintag >> maxtag;
-            for (long b = 0; b < nblocks; b++) {
-                int edim, etag, type;
-                long nb;
-                in >> edim >> etag >> type >> nb;
-                int nn = type == 4 ? 4 : type == 2 ? 3 : type == 1 ? 2 : type == 15 ? 1 : -1;
+            src.payload();
+            const long long nblocks = src.u64();
+            src.u64();  // number of elements
+            src.u64();
+            src.u64();  // min and max tag
+            for (long long b = 0; b < nblocks; b++) {
+                const int edim = (int)src.i32(), etag = (int)src.i32(), type = (int)src.i32();
+                const long long nb = src.u64();
+                const int nn = element_nodes(type);
                 if (nn < 0) {
                     g_err = "unsupported element type " + std::to_string(type);
                     return false;
                 }
+                if (nb < 0) {
+                    g_err = "malformed $Elements section";
+                    return false;
+                }
                 const std::vector<int> &ph = ent_phys[{edim, etag}];
-                for (long k = 0; k < nb; k++) {
-                    long id;
-                    in >> id;
+                for (long long k = 0; k < nb; k++) {
+                    src.u64();  // element tag
                     int nodes[4];
-                    for (int q = 0; q < nn; q++) in >> nodes[q];
+                    for (int q = 0; q < nn; q++) nodes[q] = (int)src.u64();
                     for (int p : ph) {
                         if (type == 4) {
                             for (int q = 0; q < 4; q++) m->tets.push_back(nodes[q] - 1);
@@ -232,6 +315,12 @@ bool read_msh41(std::istream &in, fgh_mesh *m)
                     }
                 }
             }
+        } else if (bin && tok.size() > 1 && tok[0] == '$' && tok.rfind("$End", 0) != 0) {
+            skip_section(in, tok);
+        }
+        if (in.fail()) {
+            g_err = "truncated or malformed gmsh file (" + tok + ")";
+            return false;
         }
     }
     return true;
@@ -254,7 +343,7 @@ const char *fgh_last_error(void) { return g_err.c_str(); }
 
 fgh_mesh *fgh_read_msh(const char *path, char *err, int errlen)
 {
-    std::ifstream in(path);
+    std::ifstream in(path, std::ios::binary);
     auto fail = [&](const std::string &msg) -> fgh_mesh * {
         g_err = msg;
         if (err && errlen > 0) {
@@ -270,11 +359,20 @@ fgh_mesh *fgh_read_msh(const char *path, char *err, int errlen)
     std::string version;
     int filetype, dsize;
     in >> version >> filetype >> dsize;
-    if (filetype != 0) return fail("binary gmsh files are not supported");
+    if (filetype != 0 && filetype != 1) return fail("malformed $MeshFormat");
+    const bool bin = filetype == 1;
+    if (bin) {
+        // a binary file follows the format line with the int 1 in the writer's byte order
+        int32_t one = 0;
+        in.get();
+        in.read(reinterpret_cast<char *>(&one), sizeof one);
+        if (one != 1) return fail("binary gmsh file of the other byte order");
+        if (dsize != 8) return fail("binary gmsh file with data size " + std::to_string(dsize));
+    }
     auto *m = new fgh_mesh();
     bool ok;
-    if (version.rfind("2.", 0) == 0) ok = read_msh22(in, m);
-    else if (version.rfind("4.", 0) == 0) ok = read_msh41(in, m);
+    if (version.rfind("2.", 0) == 0) ok = read_msh22(in, m, bin);
+    else if (version.rfind("4.", 0) == 0) ok = read_msh41(in, m, bin);
     else {
         delete m;
         return fail("unsupported gmsh format " + version);
@@ -746,7 +844,7 @@ int fgh_bbox(const fgh_mesh *m, double l[3], double c[3])
 int fgh_read_sol(const fgh_mesh *m, const char *path, double *u, double *phi, double *t)
 {
     if (!m || !m->prepared) return FG_ERR_STATE;
-    std::ifstream in(path);
+    std::ifstream in(path, std::ios::binary);
     if (!in) {
         g_err = std::string("cannot open file ") + path;
         return FG_ERR_ARG;

@@ -4,7 +4,7 @@
  *
  * Reference interfaces replaced:
  *   fgh_read_msh        Mesh::mesh::readMesh / readNodes / readTetraedrons / readTriangles
- *                       (src/read.cpp:12-197), gmsh-free, ASCII MSH 2.2 and 4.1
+ *                       (src/read.cpp:12-197), gmsh-free, MSH 2.2 and 4.1, ASCII and binary
  *   fgh_prepare         Mesh::mesh ctor: scaling (src/read.cpp:75-90), Tet orientation
  *                       (src/tetra.cpp:410-424), controlTriangles (src/mesh.cpp:121-245),
  *                       sortNodes (src/mesh.cpp:334-367)
@@ -24,7 +24,8 @@ extern "C" {
 
 typedef struct fgh_mesh fgh_mesh;
 
-/* Read an ASCII gmsh file (format 2.2 or 4.1). Returns NULL and fills err on failure. */
+/* Read a gmsh file (format 2.2 or 4.1, ASCII or little-endian binary). Returns NULL and fills
+ * err on failure (including a truncated file). */
 fgh_mesh *fgh_read_msh(const char *path, char *err, int errlen);
 /* Structured cuboid [x0,x0+lx]x[y0,y0+ly]x[z0,z0+lz] of nx*ny*nz cells, each split into 6
  * tetrahedra; interior node coordinates jittered by U(-jitter, jitter) times the cell size,

@@ -22,6 +22,125 @@ def test_read_ellipsoid_counts(golden_dir):
     assert m2.n_tris == 274 and set(m2.arrays()["tri_region"]) == {0}  # the created region, index n_surf
 
 
+def _parse_msh41_ascii(path):
+    """The sections of an ASCII MSH 4.1 file as Python data (test helper for the binary writers)."""
+    lines = open(path).read().split("\n")
+    sec, i = {}, 0
+    while i < len(lines):
+        t = lines[i].strip()
+        if t.startswith("$") and not t.startswith("$End"):
+            j = lines.index("$End" + t[1:], i)
+            sec[t[1:]] = lines[i + 1:j]
+            i = j
+        i += 1
+    toks = " ".join(sec["Entities"]).split()
+    pos = 0
+
+    def nxt(f=int):
+        nonlocal pos
+        pos += 1
+        return f(toks[pos - 1])
+    counts = [nxt() for _ in range(4)]
+    ents = []
+    for dim in range(4):
+        for _ in range(counts[dim]):
+            tag = nxt()
+            box = [nxt(float) for _ in range(3 if dim == 0 else 6)]
+            ph = [nxt() for _ in range(nxt())]
+            bnd = [nxt() for _ in range(nxt())] if dim > 0 else []
+            ents.append((dim, tag, box, ph, bnd))
+    toks, pos = " ".join(sec["Nodes"]).split(), 0
+    nblocks, nnodes, nmin, nmax = (nxt() for _ in range(4))
+    nblk = []
+    for _ in range(nblocks):
+        edim, etag, par, nb = (nxt() for _ in range(4))
+        tags = [nxt() for _ in range(nb)]
+        xyz = [nxt(float) for _ in range(3 * nb)]
+        nblk.append((edim, etag, tags, xyz))
+    toks, pos = " ".join(sec["Elements"]).split(), 0
+    eblocks, nelem, emin, emax = (nxt() for _ in range(4))
+    nn = {15: 1, 1: 2, 2: 3, 4: 4}
+    eblk = []
+    for _ in range(eblocks):
+        edim, etag, typ, nb = (nxt() for _ in range(4))
+        els = [[nxt() for _ in range(1 + nn[typ])] for _ in range(nb)]
+        eblk.append((edim, etag, typ, els))
+    return dict(phys=sec["PhysicalNames"], ents=ents, counts=counts, nodes=(nnodes, nmin, nmax, nblk),
+                elems=(nelem, emin, emax, eblk))
+
+
+def _write_msh41_binary(d, path):
+    import struct
+    i, q, f = (lambda v: struct.pack("<i", v)), (lambda v: struct.pack("<Q", v)), (lambda v: struct.pack("<d", v))
+    out = [b"$MeshFormat\n4.1 1 8\n", i(1), b"\n$EndMeshFormat\n"]
+    out += [b"$PhysicalNames\n", ("\n".join(d["phys"]) + "\n").encode(), b"$EndPhysicalNames\n"]
+    out += [b"$Entities\n"] + [q(c) for c in d["counts"]]
+    for dim, tag, box, ph, bnd in d["ents"]:
+        out += [i(tag)] + [f(v) for v in box] + [q(len(ph))] + [i(v) for v in ph]
+        if dim > 0:
+            out += [q(len(bnd))] + [i(v) for v in bnd]
+    out += [b"\n$EndEntities\n"]
+    # a section the reader does not use, holding bytes that spell section names
+    out += [b"$NodeData\n", b"\x00$EndNodes$Nodes\x07" * 3, b"\n$EndNodeData\n"]
+    nnodes, nmin, nmax, nblk = d["nodes"]
+    out += [b"$Nodes\n", q(len(nblk)), q(nnodes), q(nmin), q(nmax)]
+    for edim, etag, tags, xyz in nblk:
+        out += [i(edim), i(etag), i(0), q(len(tags))] + [q(t) for t in tags] + [f(v) for v in xyz]
+    out += [b"\n$EndNodes\n"]
+    nelem, emin, emax, eblk = d["elems"]
+    out += [b"$Elements\n", q(len(eblk)), q(nelem), q(emin), q(emax)]
+    for edim, etag, typ, els in eblk:
+        out += [i(edim), i(etag), i(typ), q(len(els))] + [q(v) for e in els for v in e]
+    out += [b"\n$EndElements\n"]
+    open(path, "wb").write(b"".join(out))
+
+
+def _write_msh22_binary(d, path):
+    """Element blocks of one type each, tags (physical, elementary): the MSH 2.2 binary layout."""
+    import struct
+    i, f = (lambda v: struct.pack("<i", v)), (lambda v: struct.pack("<d", v))
+    ph = {(dim, tag): (p[0] if p else 0) for dim, tag, _, p, _ in d["ents"]}
+    out = [b"$MeshFormat\n2.2 1 8\n", i(1), b"\n$EndMeshFormat\n"]
+    out += [b"$PhysicalNames\n", ("\n".join(d["phys"]) + "\n").encode(), b"$EndPhysicalNames\n"]
+    nnodes, _, _, nblk = d["nodes"]
+    out += [("$Nodes\n%d\n" % nnodes).encode()]
+    for _, _, tags, xyz in nblk:
+        for k, t in enumerate(tags):
+            out += [i(t)] + [f(v) for v in xyz[3 * k:3 * k + 3]]
+    out += [b"\n$EndNodes\n"]
+    nelem, _, _, eblk = d["elems"]
+    out += [("$Elements\n%d\n" % nelem).encode()]
+    for edim, etag, typ, els in eblk:
+        out += [i(typ), i(len(els)), i(2)]
+        for e in els:
+            out += [i(e[0]), i(ph[(edim, etag)]), i(etag)] + [i(v) for v in e[1:]]
+    out += [b"\n$EndElements\n"]
+    open(path, "wb").write(b"".join(out))
+
+
+@pytest.mark.parametrize("fmt", ["4.1", "2.2"])
+def test_read_binary_msh(golden_dir, tmp_path, fmt):
+    """Binary gmsh files (the reference reads any file the gmsh library opens, src/read.cpp:12-60):
+    the golden ellipsoid rewritten as binary MSH 4.1 and 2.2 reads as the ASCII original, array
+    for array; truncated files are reported, not read short."""
+    src = os.path.join(golden_dir, "ellipsoid.msh")
+    d = _parse_msh41_ascii(src)
+    path = tmp_path / f"ellipsoid_{fmt}.msh"
+    (_write_msh41_binary if fmt == "4.1" else _write_msh22_binary)(d, path)
+    a, b = HostMesh.read(src), HostMesh.read(path)
+    assert b.raw_counts() == a.raw_counts() == (167, 499, 274)
+    for m in (a, b):
+        m.prepare({"ellipsoid_volume": dict(K=3e5, uk=(0, 1, 0))}, {"ellipsoid_surface": {}})
+    x, y = a.arrays(), b.arrays()
+    for k in x:
+        assert np.array_equal(x[k], y[k]), k
+    data = open(path, "rb").read()
+    cut = tmp_path / "cut.msh"
+    cut.write_bytes(data[:data.index(b"$EndNodes") - 40])
+    with pytest.raises(RuntimeError, match="truncated"):
+        HostMesh.read(cut)
+
+
 def _outward_ok(m):
     a = m.arrays()
     pos, tets, tris, dMs = a["pos"], a["tets"], a["tris"], a["dMs"]
