@@ -314,15 +314,6 @@ __host__ __device__ inline void m2p_eval2(const double2 *Mu, const double2 *Mv, 
     }
 }
 
-// Unrolled forms for a compile-time order P (device kernels). The integer factors of the
-// recurrences are carried as running doubles (stepped by the inline constants 1 and 2) instead of
-// per-term literals: gfx950's three-operand FP64 instructions take no literal operands, so
-// literals would each occupy a scalar register pair.
-// Real-type helpers of the templated evaluators (FP64, or FP32 for the mixed-precision pass).
-template <class R> struct vec2_of;
-template <> struct vec2_of<double> { using type = double2; };
-template <> struct vec2_of<float> { using type = float2; };
-
 // Reciprocal and reciprocal square root for the device evaluators: the hardware approximations
 // refined by Newton steps (two in FP64, each doubling the correct bits, to full precision for the
 // O(1) arguments here, in place of the IEEE-exact division sequences), used as is in FP32.
@@ -344,92 +335,6 @@ __device__ inline double fast_rsqrt(double x)
 }
 __device__ inline float fast_inv(float x) { return __builtin_amdgcn_rcpf(x); }
 __device__ inline float fast_rsqrt(float x) { return __builtin_amdgcn_rsqf(x); }
-
-// M2P of two expansions at one target in the real type R: su += sum M_u,n^m conj(I_n^m(x)), same
-// for sv, x in cell widths. The irregular harmonics are streamed column by column (diagonal
-// I_m^m, then I_{m+1}^m and the three-term recurrence in n) with the recurrence factors scaled by
-// 1/r^2 and carried as running values. The m = 0 column is real (M_n^0 and I_n^0 are real for
-// real charges), so only its real parts are formed; each product is its own fused multiply-add
-// into the running sum.
-template <int P, class R>
-__device__ inline void m2p_eval2_r(const typename vec2_of<R>::type *__restrict__ Mu,
-                                   const typename vec2_of<R>::type *__restrict__ Mv, R x, R y, R z, R &su, R &sv)
-{
-    const R r2 = x * x + y * y + z * z;
-    const R ir2 = fast_inv(r2), zr = z * ir2, zr2 = zr + zr, ir22 = ir2 + ir2;
-    // Factors of the three-term recurrence in n shared by every column m: tz[n] = (2n - 1) z / r^2
-    // and un[n] = (n - 1)^2 / r^2; column m takes un[n] - m^2 / r^2 = (n + m - 1)(n - m - 1) / r^2.
-    R tz[P + 1], un[P + 1];
-    tz[1] = zr;
-    un[1] = R(0);
-    {
-        R st = ir2;
-#pragma unroll
-        for (int n = 2; n <= P; n++) {
-            tz[n] = tz[n - 1] + zr2;
-            un[n] = un[n - 1] + st;
-            st += ir22;
-        }
-    }
-    R dre = fast_rsqrt(r2), dim = R(0);
-    R au = Mu[0].x * dre, av = Mv[0].x * dre;
-    if (P >= 1) {
-        R p2 = dre, p1 = zr * dre;
-        au = fma(Mu[fidx(1, 0)].x, p1, au);
-        av = fma(Mv[fidx(1, 0)].x, p1, av);
-#pragma unroll
-        for (int n = 2; n <= P; n++) {
-            const R c = tz[n] * p1 - un[n] * p2;
-            au = fma(Mu[fidx(n, 0)].x, c, au);
-            av = fma(Mv[fidx(n, 0)].x, c, av);
-            p2 = p1;
-            p1 = c;
-        }
-    }
-    R hu = R(0), hv = R(0);
-    R fd = -ir2, vm = R(0), dv = ir2;  // fd = (1 - 2m) / r^2, vm = m^2 / r^2
-#pragma unroll
-    for (int m = 1; m <= P; m++) {
-        const R nre = fd * (x * dre - y * dim), nim = fd * (x * dim + y * dre);
-        fd -= ir22;
-        vm += dv;
-        dv += ir22;
-        dre = nre;
-        dim = nim;
-        auto a = Mu[fidx(m, m)];
-        auto b = Mv[fidx(m, m)];
-        hu = fma(a.x, dre, hu);
-        hu = fma(a.y, dim, hu);
-        hv = fma(b.x, dre, hv);
-        hv = fma(b.y, dim, hv);
-        if (m + 1 <= P) {
-            R p2re = dre, p2im = dim, p1re = tz[m + 1] * dre, p1im = tz[m + 1] * dim;
-            a = Mu[fidx(m + 1, m)];
-            b = Mv[fidx(m + 1, m)];
-            hu = fma(a.x, p1re, hu);
-            hu = fma(a.y, p1im, hu);
-            hv = fma(b.x, p1re, hv);
-            hv = fma(b.y, p1im, hv);
-#pragma unroll
-            for (int n = m + 2; n <= P; n++) {
-                const R bb = un[n] - vm;
-                const R cre = tz[n] * p1re - bb * p2re, cim = tz[n] * p1im - bb * p2im;
-                a = Mu[fidx(n, m)];
-                b = Mv[fidx(n, m)];
-                hu = fma(a.x, cre, hu);
-                hu = fma(a.y, cim, hu);
-                hv = fma(b.x, cre, hv);
-                hv = fma(b.y, cim, hv);
-                p2re = p1re;
-                p2im = p1im;
-                p1re = cre;
-                p1im = cim;
-            }
-        }
-    }
-    su += au + (hu + hu);
-    sv += av + (hv + hv);
-}
 
 // L2P of two expansions at once: pu += sum L_u R, pv += sum L_v R. Not unrolled (it runs once per
 // target); the recurrence divisors are formed from loop counters.
