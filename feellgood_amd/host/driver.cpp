@@ -5,9 +5,10 @@
 // Follows src/main.cpp:103-232 (options, random seed), src/settings.cpp:314-616 (settings and
 // defaults from src/default-settings.yml), src/time_integration.cpp:11-38,136-254 (TimeStepper
 // and the loop with its retry semantics) and src/save.cpp:13-186 (.evol columns, .sol files).
-// Not supported, reported as errors: expressions in initial_magnetization or Bext (only constant
-// vectors and .sol files), region-qualified evol columns, Thiele_length, recentering and spin
-// accumulation.
+// initial_magnetization and Bext take the reference's three expressions (of x, y, z in metres and
+// of t in seconds, src/settings.cpp:454-543) through fgh::Expr. Not supported, reported as errors:
+// JavaScript function definitions and space-dependent fields in those keys, region-qualified evol
+// columns, Thiele_length, recentering and spin accumulation.
 #include <unistd.h>
 
 #include <algorithm>
@@ -30,6 +31,7 @@
 
 #include "../../include/fgmi.h"
 #include "../../include/fgmi_host.h"
+#include "expr.h"
 #include "settings.h"
 
 namespace {
@@ -54,14 +56,44 @@ struct Config {
     std::vector<fg_surface_region> surf;
     std::string init_sol;
     double m0[3] = {0, 0, 1};
+    std::vector<fgh::Expr> m0_expr;  // three expressions of x, y, z (empty: m0 or init_sol)
+    std::vector<std::string> m0_text;
     bool has_t0 = false;
     double t0 = 0;
     double Bext[3] = {0, 0, 0};
+    std::vector<fgh::Expr> B_expr;  // three expressions of t (empty: Bext)
+    std::vector<std::string> B_text;
     long max_iter = 700;
     double tol = 1e-6;
     double du_max = 0.02, dt_min = 1e-16, dt_max = 5e-13;
     int demag = FG_DEMAG_FMM, fmm_order = 8, fmm_leaf = 64, precision = FG_PREC_FP64, device = 0;
 };
+
+// Three expressions of a settings key (numbers are expressions too). Returns whether any depends
+// on the variables; if none does, their values are in val.
+bool compile3(const Node &n, const std::vector<std::string> &vars, const std::string &where, double val[3],
+              std::vector<fgh::Expr> &ex, std::vector<std::string> &text)
+{
+    if (n.kind != Node::Seq || n.items.size() != 3) throw SettingsError(where + " should have three components");
+    ex.clear();
+    text.clear();
+    bool var = false;
+    for (int k = 0; k < 3; k++) {
+        const Node &it = n.items[k];
+        if (it.kind != Node::Scalar || it.is_null())
+            throw SettingsError(where + ": component " + std::to_string(k + 1) + " is not an expression");
+        try {
+            ex.push_back(fgh::Expr::compile(it.text, vars));
+        } catch (const fgh::ExprError &e) {
+            throw SettingsError(where + ": " + e.what());
+        }
+        text.push_back(it.text);
+        var = var || ex.back().uses_variables();
+    }
+    if (!var)
+        for (int k = 0; k < 3; k++) val[k] = ex[k].eval(nullptr);
+    return var;
+}
 
 void normalize(double v[3], const std::string &where)
 {
@@ -169,14 +201,13 @@ Config read_config(const Node &root)
             c.surf.push_back(r);
         }
     if (auto n = root.get("initial_magnetization"); n && !n->is_null()) {
-        if (n->kind == Node::Scalar) c.init_sol = n->text;
-        else {
-            try {
-                fgh::as_vec3(*n, c.m0, "initial_magnetization");
-            } catch (const SettingsError &) {
-                throw SettingsError("initial_magnetization: only a .sol file name or three numbers are supported "
-                                    "(expressions of x, y, z are not)");
-            }
+        if (n->kind == Node::Scalar) {
+            if (n->text.find("function") != std::string::npos && n->text.find('{') != std::string::npos)
+                throw SettingsError("initial_magnetization: JavaScript functions are not supported (give a .sol "
+                                    "file or three expressions of x, y, z)");
+            c.init_sol = n->text;
+        } else if (!compile3(*n, {"x", "y", "z"}, "initial_magnetization", c.m0, c.m0_expr, c.m0_text)) {
+            c.m0_expr.clear();  // constants: c.m0
         }
     }
     if (auto n = root.get("initial_time"); n && !n->is_null()) {
@@ -184,11 +215,10 @@ Config read_config(const Node &root)
         c.t0 = fgh::as_double(*n, "initial_time");
     }
     if (auto n = root.get("Bext"); n && !n->is_null()) {
-        try {
-            fgh::as_vec3(*n, c.Bext, "Bext");
-        } catch (const SettingsError &) {
-            throw SettingsError("Bext: only three numbers are supported (expressions of t are not)");
-        }
+        if (n->kind != Node::Seq)
+            throw SettingsError("Bext: only three expressions of t are supported (JavaScript functions and "
+                                "space-dependent fields are not)");
+        if (!compile3(*n, {"t"}, "Bext", c.Bext, c.B_expr, c.B_text)) c.B_expr.clear();
     }
     if (auto r = root.get("recentering"); r && r->kind == Node::Map)
         if (auto n = r->get("enable"); n && !n->is_null() && fgh::as_bool(*n, "recentering.enable"))
@@ -279,6 +309,18 @@ struct Engine {
     }
 };
 
+// Settings::getMagnetization (src/settings.h:262-268): the three expressions at the node position
+// (metres), normalised.
+void initial_from_expressions(const Config &c, const fg_mesh &view, double *u)
+{
+    for (int i = 0; i < view.n_nodes; i++) {
+        double v[3];
+        for (int k = 0; k < 3; k++) v[k] = c.m0_expr[k].eval(view.pos + 3 * (size_t)i);
+        normalize(v, "initial_magnetization at node " + std::to_string(i));
+        for (int k = 0; k < 3; k++) u[3 * (size_t)i + k] = v[k];
+    }
+}
+
 int run(const Config &c, unsigned seed, bool verbose, bool check_only)
 {
     char err[512] = {0};
@@ -305,9 +347,23 @@ int run(const Config &c, unsigned seed, bool verbose, bool check_only)
         for (size_t k = 0; k < c.surf.size(); k++)
             std::cout << "surface region:    " << c.surf_names[k] << " Ks " << c.surf[k].Ks << " suppress_charges "
                       << c.surf[k].suppress_charges << "\n";
-        std::cout << "Bext:              " << c.Bext[0] << " " << c.Bext[1] << " " << c.Bext[2] << "\n";
-        std::cout << "initial:           " << (c.init_sol.empty() ? "constant" : c.init_sol) << " " << c.m0[0] << " "
-                  << c.m0[1] << " " << c.m0[2] << "\n";
+        if (c.B_expr.empty())
+            std::cout << "Bext:              " << c.Bext[0] << " " << c.Bext[1] << " " << c.Bext[2] << "\n";
+        else
+            std::cout << "Bext(t):           " << c.B_text[0] << " | " << c.B_text[1] << " | " << c.B_text[2] << "\n";
+        if (!c.m0_expr.empty()) {
+            // the normalised initial magnetization of the nodes, averaged
+            std::vector<double> u0(3 * (size_t)N);
+            initial_from_expressions(c, view, u0.data());
+            double a[3] = {0, 0, 0};
+            for (int i = 0; i < N; i++)
+                for (int k = 0; k < 3; k++) a[k] += u0[3 * (size_t)i + k] / N;
+            std::cout << "initial:           expressions " << c.m0_text[0] << " | " << c.m0_text[1] << " | "
+                      << c.m0_text[2] << " mean " << a[0] << " " << a[1] << " " << a[2] << "\n";
+        } else {
+            std::cout << "initial:           " << (c.init_sol.empty() ? "constant" : c.init_sol) << " " << c.m0[0]
+                      << " " << c.m0[1] << " " << c.m0[2] << "\n";
+        }
         std::cout << "time:              final " << c.final_time << " evol " << c.evol_dt << " dt " << c.dt_min << " "
                   << c.dt_max << " du " << c.du_max << "\n";
         std::cout << "solver:            tol " << c.tol << " max_iter " << c.max_iter << "\n";
@@ -325,6 +381,8 @@ int run(const Config &c, unsigned seed, bool verbose, bool check_only)
         if (fgh_read_sol(mesh, c.init_sol.c_str(), u.data(), phi.data(), &ts) != FG_OK)
             throw std::runtime_error(std::string("initial magnetization: ") + fgh_last_error());
         if (!c.has_t0) t = ts;
+    } else if (!c.m0_expr.empty()) {
+        initial_from_expressions(c, view, u.data());
     } else {
         double m0[3] = {c.m0[0], c.m0[1], c.m0[2]};
         normalize(m0, "initial_magnetization");
@@ -354,7 +412,12 @@ int run(const Config &c, unsigned seed, bool verbose, bool check_only)
     e.check(fg_set_state(e.ctx, FG_BOTH, u.data(), nullptr, phi.data(), nullptr), "fg_set_state");
     e.check(fg_base_projection(e.ctx, next_angle()), "fg_base_projection");  // LinAlgebra ctor
 
-    const double Hext[3] = {c.Bext[0] / MU0, c.Bext[1] / MU0, c.Bext[2] / MU0};
+    // Settings::getField (src/settings.h:275-283): H = B(t) / mu0
+    double Hext[3];
+    auto field_at = [&](double tt) {
+        for (int k = 0; k < 3; k++) Hext[k] = (c.B_expr.empty() ? c.Bext[k] : c.B_expr[k].eval(&tt)) / MU0;
+    };
+    field_at(t);
     // timing (src/time_integration.h:6-59)
     const double DTMIN = c.dt_min, DTMAX = c.dt_max, TAUR = 100. * DTMAX;
     double dt = 0, prefactor = 1;
@@ -413,6 +476,7 @@ int run(const Config &c, unsigned seed, bool verbose, bool check_only)
                 goto bailout;
             }
             e.check(fg_base_projection(e.ctx, next_angle()), "fg_base_projection");
+            field_at(t);  // the field at the step's start, also used by its energies (src/time_integration.cpp:194-231)
             e.check(fg_prepare_elements(e.ctx, Hext, dt, prefactor), "fg_prepare_elements");
             fg_solve_info info;
             const int rc = fg_solve(e.ctx, dt, &info);
@@ -441,6 +505,7 @@ int run(const Config &c, unsigned seed, bool verbose, bool check_only)
             e.check(fg_max_angle(e.ctx, &a), "fg_max_angle");
             max_angle = std::max(max_angle, a);
         }
+        field_at(t);  // the saved applied field is the one at the output time (src/save.cpp:22-24)
         {  // Fem::saver (src/save.cpp:13-162)
             double mu[3] = {0, 0, 0}, mv[3] = {0, 0, 0}, ang = 0;
             e.check(fg_average(e.ctx, 0, mu), "fg_average");

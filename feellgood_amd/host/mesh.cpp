@@ -8,6 +8,7 @@
  * (src/mesh.cpp:121-245), nodes sorted along the longest axis (src/mesh.cpp:334-367).
  */
 #include "../../include/fgmi_host.h"
+#include "expr.h"
 
 #include <algorithm>
 #include <array>
@@ -340,6 +341,32 @@ inline double mixed_product(const double *p0, const double *p1, const double *p2
 extern "C" {
 
 const char *fgh_last_error(void) { return g_err.c_str(); }
+
+int fgh_eval_expression(const char *expr, const char *vars, int64_t n, const double *vals, double *out)
+{
+    if (!expr || !vars || n < 0 || (n > 0 && !out)) {
+        g_err = "fgh_eval_expression: bad arguments";
+        return FG_ERR_ARG;
+    }
+    std::vector<std::string> names;
+    for (const char *p = vars; *p;) {
+        const char *e = std::strchr(p, ',');
+        names.emplace_back(p, e ? (size_t)(e - p) : std::strlen(p));
+        p = e ? e + 1 : p + std::strlen(p);
+    }
+    if (n > 0 && !names.empty() && !vals) {
+        g_err = "fgh_eval_expression: bad arguments";
+        return FG_ERR_ARG;
+    }
+    try {
+        const fgh::Expr x = fgh::Expr::compile(expr, names);
+        for (int64_t i = 0; i < n; i++) out[i] = x.eval(vals + (size_t)i * names.size());
+    } catch (const fgh::ExprError &e) {
+        g_err = e.what();
+        return FG_ERR_ARG;
+    }
+    return FG_OK;
+}
 
 fgh_mesh *fgh_read_msh(const char *path, char *err, int errlen)
 {

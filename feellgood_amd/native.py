@@ -117,6 +117,7 @@ HOST_SYMBOLS = {
     "fgh_node_index": (_i32, [C.c_void_p]),
     "fgh_bbox": (C.c_int, [C.c_void_p, _d, _d]),
     "fgh_last_error": (C.c_char_p, []),
+    "fgh_eval_expression": (C.c_int, [C.c_char_p, C.c_char_p, C.c_int64, _d, _d]),
     "fgh_read_sol": (C.c_int, [C.c_void_p, C.c_char_p, _d, _d, _d]),
     "fgh_write_sol": (C.c_int, [C.c_void_p, C.c_char_p, C.c_char_p, C.c_int, _d, _d]),
 }

@@ -48,6 +48,14 @@ void fgh_free(fgh_mesh *m);
 
 /* Raw counts of the file/generator content. */
 int fgh_raw_counts(const fgh_mesh *m, int64_t *n_nodes, int64_t *n_tets, int64_t *n_tris);
+/* Evaluate an expression of the settings file at n points: vars is a comma-separated list of
+ * variable names ("t", or "x,y,z"), vals[n * nvars] their values, out[n] the results. The language
+ * is the arithmetic subset of the JavaScript FeeLLGood evaluates for initial_magnetization and Bext
+ * (src/expression_parser.cpp:9-33: Math members without the prefix, cosh/sinh/tanh, ...), as
+ * fgmi_run uses it. Returns FG_OK, or FG_ERR_ARG with the reason (position, unknown name, syntax)
+ * in fgh_last_error(). No counterpart in the reference (it runs the expressions in Duktape). */
+int fgh_eval_expression(const char *expr, const char *vars, int64_t n, const double *vals, double *out);
+
 /* Write the raw mesh as ASCII MSH 2.2. */
 int fgh_write_msh22(const fgh_mesh *m, const char *path);
 

@@ -65,6 +65,61 @@ def test_json_settings_on_stdin(golden_dir, tmp_path):
     assert d["outputs"][0][4:] == ["t", "<Mx>", "<My>", "<Mz>", "E_ex", "E_demag", "E_zeeman", "E_tot"]
 
 
+def test_expression_settings(golden_dir, tmp_path):
+    """initial_magnetization as three expressions of x, y, z (metres) and Bext as three expressions
+    of t (examples/example5.py:163, examples/example2.py:69-71): accepted, the nodes' normalised
+    initial magnetization is the numpy evaluation at the prepared node positions, and the forms the
+    reference evaluates as JavaScript functions or space-time fields are reported."""
+    from feellgood_amd.mesh import HostMesh
+    s = full_test_settings(golden_dir, tmp_path)
+    s["initial_magnetization"] = ["-y", "x", "2e-9"]
+    s["Bext"] = ["0.01*cos(2*PI*5e9*t)", "Math.sin(2*Math.PI*5e9*t) * 0.01", 0]
+    rc, out, err = check(["-"], json.dumps(s))
+    assert rc == 0, err
+    d = parsed(out)
+    assert d["Bext(t)"][0] == ["0.01*cos(2*PI*5e9*t)", "|", "Math.sin(2*Math.PI*5e9*t)", "*", "0.01", "|", "0"]
+    ini = d["initial"][0]
+    assert ini[:2] == ["expressions", "-y"]
+    hm = HostMesh.read(os.path.join(golden_dir, "ellipsoid.msh"))
+    hm.prepare({"ellipsoid_volume": {}})
+    p = hm.arrays()["pos"]
+    u = np.stack([-p[:, 1], p[:, 0], np.full(len(p), 2e-9)], axis=1)
+    u /= np.linalg.norm(u, axis=1, keepdims=True)
+    assert np.allclose([float(x) for x in ini[-3:]], u.mean(axis=0), rtol=1e-8, atol=1e-12)
+    # constant expressions are constants
+    s["Bext"] = ["1", "2*0", "-1"]
+    rc, out, _ = check(["-"], json.dumps(s))
+    assert rc == 0 and [float(x) for x in parsed(out)["Bext"][0]] == [1, 0, -1]
+    for key, val, msg in (("Bext", "function(t) { return [0, 0, t]; }", "JavaScript functions"),
+                          ("Bext", {"space": "function(x,y,z) {}", "time": "function(t) {}"}, "space-dependent"),
+                          ("Bext", ["cos(t", "0", "0"], "expected"),
+                          ("initial_magnetization", ["x", "y", "w"], "unknown name 'w'"),
+                          ("initial_magnetization", "function(x,y,z) { return [x,y,z]; }", "JavaScript functions")):
+        bad = dict(s, **{key: val})
+        rc, out, err = check(["-"], json.dumps(bad))
+        assert rc != 0 and msg in err + out, (key, val, err + out)
+
+
+@pytest.mark.gpu
+def test_time_expression_field_on_gpu(golden_dir, tmp_path):
+    """Bext given as expressions of t that stay equal to the golden's constants ("1 + 0*t", ...)
+    goes through the per-step evaluation and reproduces the constant-field run exactly; a field
+    rotating at 5 GHz runs to the end and changes the trajectory."""
+    def run(bext, sub):
+        d = tmp_path / sub
+        s = full_test_settings(golden_dir, d, "direct")
+        s["Bext"] = bext
+        r = subprocess.run([DRIVER, "--seed", "2", "-"], input=json.dumps(s), capture_output=True, text=True,
+                           timeout=300)
+        assert r.returncode == 0, r.stdout + r.stderr
+        return last_evol_row(d / "full_test.evol")
+    ref, n = run([1, 0, -1], "const")
+    same, n2 = run(["1 + 0*t", "0*t", "-1 + 0*t"], "expr")
+    assert n == n2 == 21 and same == ref
+    rot, n3 = run(["cos(2*PI*5e9*t)", "sin(2*PI*5e9*t)", "-1"], "rot")
+    assert n3 == 21 and all(np.isfinite(rot)) and np.linalg.norm(np.array(rot[1:4]) - ref[1:4]) > 1e-3
+
+
 YAML = """# a settings file in block style
 outputs:
   directory: {out}/   # trailing slash is dropped

@@ -274,3 +274,46 @@ def test_cylinder_generator_valid():
     rho = np.hypot(pos[:, 0] - 1e-9, pos[:, 1] + 2e-9)
     assert r - h <= rho.max() * 1e9 <= r + h / np.sqrt(2)
     assert pos[:, 2].min() == pytest.approx(5e-9) and pos[:, 2].max() == pytest.approx(65e-9)
+
+
+def _ev(expr, names="t", vals=(0.0,)):
+    import ctypes as C
+    from feellgood_amd import native as nv
+    v = np.ascontiguousarray(vals, dtype=np.float64)
+    n = len(v) // max(1, len(names.split(",")))
+    out = np.zeros(max(n, 1))
+    rc = nv.host_lib().fgh_eval_expression(expr.encode(), names.encode(), n, nv.dptr(v), nv.dptr(out))
+    if rc != 0:
+        raise ValueError(nv.host_lib().fgh_last_error().decode())
+    return out[:n]
+
+
+def test_settings_expressions():
+    """The arithmetic subset of the JavaScript FeeLLGood evaluates for Bext(t) and
+    initial_magnetization(x, y, z) (src/expression_parser.cpp:9-33: Math members usable without the
+    prefix, cosh/sinh/tanh helpers): values against Python's math, JavaScript's operator semantics,
+    and errors that name the problem."""
+    import math
+    t = np.array([0.0, 1e-10, 3.3e-10, 1e-9])
+    A, w = 0.01, 2 * math.pi * 5e9
+    got = _ev(f"{A}*cos({w}*t)", "t", t)
+    assert np.allclose(got, A * np.cos(w * t), rtol=1e-15, atol=0)
+    assert np.allclose(_ev("Math.sin(2*Math.PI*1e9*t) + exp(-t/1e-10)", "t", t),
+                       np.sin(2 * np.pi * 1e9 * t) + np.exp(-t / 1e-10), rtol=1e-14)
+    xyz = np.array([[1e-9, 2e-9, -3e-9], [0.5, -0.25, 2.0]])
+    assert np.allclose(_ev("-y", "x,y,z", xyz.ravel()), -xyz[:, 1])
+    assert np.allclose(_ev("x*x + y**2 - z % 1e-9", "x,y,z", xyz.ravel()),
+                       xyz[:, 0] ** 2 + xyz[:, 1] ** 2 - np.fmod(xyz[:, 2], 1e-9), rtol=1e-14)
+    c = lambda e: float(_ev(e)[0])
+    assert c("2 ** 3 ** 2") == 512.0              # right-associative
+    assert c("round(2.5)") == 3.0 and c("round(-2.5)") == -2.0  # halves towards +infinity
+    assert c("0 || 7") == 7.0 and c("3 && 4") == 4.0 and c("0 && 4") == 0.0
+    assert c("1 < 2 ? 10 : 20") == 10.0 and c("(1 >= 2) + 1") == 1.0 and c("!0") == 1.0
+    assert c("min(3, -1, 2)") == -1.0 and c("max()") == -math.inf and c("hypot(3, 4)") == 5.0
+    assert c("atan2(1, 1)") == math.atan2(1, 1) and c("sign(-0.3)") == -1.0
+    assert abs(c("cosh(0.7) - (exp(0.7) + exp(-0.7)) / 2")) < 1e-15
+    assert c("Math.E") == math.e and c("SQRT1_2") == math.sqrt(0.5) and math.isnan(c("NaN"))
+    for bad, msg in (("sin(t", "expected"), ("foo(t)", "unknown name 'foo'"), ("2 +* 3", "unexpected"),
+                     ("atan2(1)", "takes 2"), ("x", "unknown name 'x'")):
+        with pytest.raises(ValueError, match=msg):
+            _ev(bad)
