@@ -173,8 +173,8 @@ def test_yaml_block_settings(golden_dir, tmp_path):
 
 
 @pytest.mark.parametrize("patch,msg", [
-    ({"Bext": ["0", "0", "0.1*t"]}, "Bext: only three numbers"),
-    ({"initial_magnetization": ["x", "y", "0"]}, "initial_magnetization: only"),
+    ({"Bext": "function(t) { return [0, 0, t]; }"}, "Bext: only three expressions of t"),
+    ({"initial_magnetization": ["x", "y"]}, "initial_magnetization should have three components"),
     ({"outputs": {"evol_columns": ["t", "nonsense"]}}, "unsupported column 'nonsense'"),
     ({"recentering": {"enable": True}}, "recentering is not supported"),
     ({"time_integration": {"max(dt)": "soon"}}, "time_integration.max(dt) should be a number"),
