@@ -6,9 +6,10 @@
 // defaults from src/default-settings.yml), src/time_integration.cpp:11-38,136-254 (TimeStepper
 // and the loop with its retry semantics) and src/save.cpp:13-186 (.evol columns, .sol files).
 // initial_magnetization and Bext take the reference's three expressions (of x, y, z in metres and
-// of t in seconds, src/settings.cpp:454-543) through fgh::Expr. Not supported, reported as errors:
-// JavaScript function definitions and space-dependent fields in those keys, region-qualified evol
-// columns, Thiele_length, recentering and spin accumulation.
+// of t in seconds, src/settings.cpp:454-543), or a function definition whose body returns them,
+// through fgh::Expr. Not supported, reported as errors: function bodies beyond one return
+// statement, functions of the regions, space-dependent fields, region-qualified evol columns,
+// Thiele_length, recentering and spin accumulation.
 #include <unistd.h>
 
 #include <algorithm>
@@ -88,6 +89,68 @@ bool compile3(const Node &n, const std::vector<std::string> &vars, const std::st
             throw SettingsError(where + ": " + e.what());
         }
         text.push_back(it.text);
+        var = var || ex.back().uses_variables();
+    }
+    if (!var)
+        for (int k = 0; k < 3; k++) val[k] = ex[k].eval(nullptr);
+    return var;
+}
+
+// A JavaScript function definition in a settings key, "function(a, b, ...) { return [e1, e2, e3]; }"
+// (ExpressionParser::set_function, src/expression_parser.cpp:60-71), limited to a body that is one
+// return statement of a three-element array: compiled as three expressions of its parameters,
+// whose number must be nparams. Returns whether any component depends on them (else the values
+// are in val).
+bool compile_function(const std::string &src, size_t nparams, const std::string &where, double val[3],
+                      std::vector<fgh::Expr> &ex, std::vector<std::string> &text)
+{
+    auto bad = [&](const std::string &why) {
+        return SettingsError(where + ": " + why + " (supported: function(...) { return [e1, e2, e3]; })");
+    };
+    auto trim = [](std::string t) {
+        const size_t a = t.find_first_not_of(" \t\r\n"), b = t.find_last_not_of(" \t\r\n");
+        return a == std::string::npos ? std::string() : t.substr(a, b - a + 1);
+    };
+    const std::string f = trim(src);
+    if (f.rfind("function", 0) != 0) throw bad("not a function definition");
+    const size_t lp = f.find('('), rp = f.find(')', lp == std::string::npos ? 0 : lp);
+    const size_t lb = f.find('{', rp == std::string::npos ? 0 : rp), rb = f.rfind('}');
+    if (lp == std::string::npos || rp == std::string::npos || lb == std::string::npos || rb == std::string::npos ||
+        rb < lb || !trim(f.substr(rb + 1)).empty())
+        throw bad("malformed function");
+    std::vector<std::string> params;
+    std::stringstream ps(f.substr(lp + 1, rp - lp - 1));
+    for (std::string q; std::getline(ps, q, ',');)
+        if (!trim(q).empty()) params.push_back(trim(q));
+    if (params.size() != nparams)
+        throw bad("the function should take " + std::to_string(nparams) + " parameter(s)");
+    std::string body = trim(f.substr(lb + 1, rb - lb - 1));
+    if (body.rfind("return", 0) != 0) throw bad("the body should be one return statement");
+    body = trim(body.substr(6));
+    if (!body.empty() && body.back() == ';') body = trim(body.substr(0, body.size() - 1));
+    if (body.size() < 2 || body.front() != '[' || body.back() != ']') throw bad("the function should return [x, y, z]");
+    // the three top-level components of the array
+    std::vector<std::string> comp(1);
+    int depth = 0;
+    for (size_t k = 1; k + 1 < body.size(); k++) {
+        const char ch = body[k];
+        depth += (ch == '(' || ch == '[') - (ch == ')' || ch == ']');
+        if (ch == ',' && depth == 0)
+            comp.emplace_back();
+        else
+            comp.back() += ch;
+    }
+    if (comp.size() != 3) throw bad("the function should return three components");
+    ex.clear();
+    text.clear();
+    bool var = false;
+    for (auto &cpt : comp) {
+        try {
+            ex.push_back(fgh::Expr::compile(trim(cpt), params));
+        } catch (const fgh::ExprError &e) {
+            throw SettingsError(where + ": " + e.what());
+        }
+        text.push_back(trim(cpt));
         var = var || ex.back().uses_variables();
     }
     if (!var)
@@ -202,10 +265,13 @@ Config read_config(const Node &root)
         }
     if (auto n = root.get("initial_magnetization"); n && !n->is_null()) {
         if (n->kind == Node::Scalar) {
-            if (n->text.find("function") != std::string::npos && n->text.find('{') != std::string::npos)
-                throw SettingsError("initial_magnetization: JavaScript functions are not supported (give a .sol "
-                                    "file or three expressions of x, y, z)");
-            c.init_sol = n->text;
+            // src/settings.cpp:457-466: a function definition, else a .sol file name
+            if (n->text.find("function") != std::string::npos && n->text.find('{') != std::string::npos) {
+                if (!compile_function(n->text, 3, "initial_magnetization", c.m0, c.m0_expr, c.m0_text))
+                    c.m0_expr.clear();
+            } else {
+                c.init_sol = n->text;
+            }
         } else if (!compile3(*n, {"x", "y", "z"}, "initial_magnetization", c.m0, c.m0_expr, c.m0_text)) {
             c.m0_expr.clear();  // constants: c.m0
         }
@@ -215,10 +281,13 @@ Config read_config(const Node &root)
         c.t0 = fgh::as_double(*n, "initial_time");
     }
     if (auto n = root.get("Bext"); n && !n->is_null()) {
-        if (n->kind != Node::Seq)
-            throw SettingsError("Bext: only three expressions of t are supported (JavaScript functions and "
-                                "space-dependent fields are not)");
-        if (!compile3(*n, {"t"}, "Bext", c.Bext, c.B_expr, c.B_text)) c.B_expr.clear();
+        if (n->kind == Node::Scalar) {
+            if (!compile_function(n->text, 1, "Bext", c.Bext, c.B_expr, c.B_text)) c.B_expr.clear();
+        } else if (n->kind == Node::Seq) {
+            if (!compile3(*n, {"t"}, "Bext", c.Bext, c.B_expr, c.B_text)) c.B_expr.clear();
+        } else {
+            throw SettingsError("Bext: space-dependent fields (space and time functions) are not supported");
+        }
     }
     if (auto r = root.get("recentering"); r && r->kind == Node::Map)
         if (auto n = r->get("enable"); n && !n->is_null() && fgh::as_bool(*n, "recentering.enable"))

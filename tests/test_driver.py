@@ -67,9 +67,10 @@ def test_json_settings_on_stdin(golden_dir, tmp_path):
 
 def test_expression_settings(golden_dir, tmp_path):
     """initial_magnetization as three expressions of x, y, z (metres) and Bext as three expressions
-    of t (examples/example5.py:163, examples/example2.py:69-71): accepted, the nodes' normalised
-    initial magnetization is the numpy evaluation at the prepared node positions, and the forms the
-    reference evaluates as JavaScript functions or space-time fields are reported."""
+    of t (examples/example5.py:163, examples/example2.py:69-71), or as function definitions that
+    return them: accepted, the nodes' normalised initial magnetization is the numpy evaluation at
+    the prepared node positions, and the forms outside the supported subset (other function
+    bodies, functions of the regions, space-time fields) are reported."""
     from feellgood_amd.mesh import HostMesh
     s = full_test_settings(golden_dir, tmp_path)
     s["initial_magnetization"] = ["-y", "x", "2e-9"]
@@ -90,11 +91,20 @@ def test_expression_settings(golden_dir, tmp_path):
     s["Bext"] = ["1", "2*0", "-1"]
     rc, out, _ = check(["-"], json.dumps(s))
     assert rc == 0 and [float(x) for x in parsed(out)["Bext"][0]] == [1, 0, -1]
-    for key, val, msg in (("Bext", "function(t) { return [0, 0, t]; }", "JavaScript functions"),
+    # the function form (src/settings.cpp:457-466,501-506), parameters named by the function
+    s["Bext"] = "function(time) { return [0.01*cos(2*PI*5e9*time), Math.max(0, time), 0]; }"
+    s["initial_magnetization"] = "function(a, b, c) {\n  return [-b, a, 2e-9];\n}"
+    rc, out, err = check(["-"], json.dumps(s))
+    assert rc == 0, err
+    d2 = parsed(out)
+    assert d2["Bext(t)"][0][0] == "0.01*cos(2*PI*5e9*time)" and d2["initial"][0][-3:] == ini[-3:]
+    for key, val, msg in (("Bext", "function(t, x) { return [0, 0, t]; }", "1 parameter"),
+                          ("Bext", "function(t) { return t; }", "return [x, y, z]"),
                           ("Bext", {"space": "function(x,y,z) {}", "time": "function(t) {}"}, "space-dependent"),
                           ("Bext", ["cos(t", "0", "0"], "expected"),
                           ("initial_magnetization", ["x", "y", "w"], "unknown name 'w'"),
-                          ("initial_magnetization", "function(x,y,z) { return [x,y,z]; }", "JavaScript functions")):
+                          ("initial_magnetization", "function(x,y,z,regions) { return [x,y,z]; }",
+                           "3 parameter")):
         bad = dict(s, **{key: val})
         rc, out, err = check(["-"], json.dumps(bad))
         assert rc != 0 and msg in err + out, (key, val, err + out)
@@ -173,7 +183,7 @@ def test_yaml_block_settings(golden_dir, tmp_path):
 
 
 @pytest.mark.parametrize("patch,msg", [
-    ({"Bext": "function(t) { return [0, 0, t]; }"}, "Bext: only three expressions of t"),
+    ({"Bext": "function(t) { var a = t; return [a, 0, 0]; }"}, "the body should be one return statement"),
     ({"initial_magnetization": ["x", "y"]}, "initial_magnetization should have three components"),
     ({"outputs": {"evol_columns": ["t", "nonsense"]}}, "unsupported column 'nonsense'"),
     ({"recentering": {"enable": True}}, "recentering is not supported"),
