@@ -140,6 +140,7 @@ void build_plan(Ctx &c, const fg_mesh *m)
         for (int p = 0; p < 5; p++) g[12 + p] = detJ * TET_PDS[p];
         mtets.insert(mtets.end(), q, q + 4);
         tet_is_mag[t] = 1;
+        c.h_tet_mesh.push_back(t);
         mreg.push_back(rg);
         da_w.insert(da_w.end(), g, g + 17);
         for (int a = 0; a < 4; a++) magn[q[a]] = 1;
@@ -483,7 +484,8 @@ void free_all(Ctx &c)
                     c.rt, c.p, c.ph, c.v, c.s, c.sh, c.t, c.w, c.ctl, c.part, c.epart, c.res, c.slice_ptr, c.slice_w,
                     c.scol, c.sval, c.tet_nodes, c.geo, c.tet_reg, c.regions, c.elem_out, c.inc_ptr, c.inc_idx,
                     c.trik_nodes, c.trik_geo, c.tri_nodes, c.tri_geo, c.tri_corr, c.cinc_ptr, c.cinc_idx, c.corr,
-                    c.src_pos, c.src_q, c.src_pos_f, c.src_rank, c.d_inst_node_off, c.inst_bn2, c.d_node_inst, c.d_inst_src};
+                    c.src_pos, c.src_q, c.src_pos_f, c.src_rank, c.d_inst_node_off, c.inst_bn2, c.d_node_inst, c.d_inst_src,
+                    c.hsp};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     if (c.h_ctl) (void)hipHostFree(c.h_ctl);
@@ -796,6 +798,55 @@ int fg_prepare_elements(fg_ctx *x, const double Hext[3], double dt, double prefa
     });
 }
 
+int fg_set_space_field(fg_ctx *x, const double *h)
+{
+    return guarded(x, [&]() {
+        Ctx &c = x->c;
+        if (!h) {
+            if (c.hsp) HIPCHK(hipFree(c.hsp));
+            c.hsp = nullptr;
+            return FG_OK;
+        }
+        // the magnetic tetrahedra's entries, component-major for coalesced reads: [3 g + k][Tm]
+        std::vector<double> soa((size_t)15 * c.Tm);
+        for (int t = 0; t < c.Tm; t++) {
+            const double *src = h + (size_t)15 * c.h_tet_mesh[t];
+            for (int q = 0; q < 15; q++) soa[(size_t)q * c.Tm + t] = src[q];
+        }
+        if (!c.hsp) c.hsp = dalloc<double>(soa.size());
+        HIPCHK(hipMemcpyAsync(c.hsp, soa.data(), soa.size() * sizeof(double), hipMemcpyHostToDevice, c.stream));
+        HIPCHK(hipStreamSynchronize(c.stream));
+        return FG_OK;
+    });
+}
+
+int fg_gauss_points(const fg_mesh *m, double *pg)
+{
+    if (!m || !pg || !m->pos || !m->tet_nodes || m->n_tets < 0) return FG_ERR_ARG;
+    for (int64_t t = 0; t < m->n_tets; t++)
+        for (int g = 0; g < 5; g++)
+            for (int k = 0; k < 3; k++) {
+                double x = 0;
+                for (int a = 0; a < 4; a++) x += tet_a(a, g) * m->pos[3 * (size_t)m->tet_nodes[4 * t + a] + k];
+                pg[15 * t + 3 * g + k] = x;
+            }
+    return FG_OK;
+}
+
+int fg_prepare_elements_amp(fg_ctx *x, double amp, double dt, double prefactor)
+{
+    return guarded(x, [&]() {
+        if (!x->c.hsp || !(dt > 0)) return x->c.hsp ? FG_ERR_ARG : FG_ERR_STATE;
+        x->dt = dt;
+        x->prefactor = prefactor;
+        for (int k = 0; k < 3; k++) x->Hext[k] = 0;
+        const double zero[3] = {0, 0, 0};
+        launch_element(x->c, zero, dt, x->c.hsp, amp);
+        x->elements_ready = true;
+        return FG_OK;
+    });
+}
+
 int fg_solve(fg_ctx *x, double dt, fg_solve_info *info)
 {
     return guarded(x, [&]() {
@@ -862,6 +913,24 @@ int fg_energy(fg_ctx *x, const double Hext[3], double E[4])
         Ctx &c = x->c;
         if (!Hext || !E) return FG_ERR_ARG;
         launch_energy(c, Hext);
+        HIPCHK(hipMemcpyAsync(c.h_part, c.res, 8 * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+        HIPCHK(hipStreamSynchronize(c.stream));
+        E[FG_E_EXCHANGE] = c.h_part[0];
+        E[FG_E_ANISOTROPY] = c.h_part[1] + c.h_part[4];
+        E[FG_E_DEMAG] = c.h_part[2] + c.h_part[5];
+        E[FG_E_ZEEMAN] = c.h_part[3];
+        return FG_OK;
+    });
+}
+
+int fg_energy_amp(fg_ctx *x, double amp, double E[4])
+{
+    return guarded(x, [&]() {
+        Ctx &c = x->c;
+        if (!E) return FG_ERR_ARG;
+        if (!c.hsp) return FG_ERR_STATE;
+        const double zero[3] = {0, 0, 0};
+        launch_energy(c, zero, c.hsp, amp);
         HIPCHK(hipMemcpyAsync(c.h_part, c.res, 8 * sizeof(double), hipMemcpyDeviceToHost, c.stream));
         HIPCHK(hipStreamSynchronize(c.stream));
         E[FG_E_EXCHANGE] = c.h_part[0];

@@ -28,6 +28,8 @@ struct Ctx {
     // sizes
     int N = 0;         // nodes
     int Tm = 0;        // magnetic tetrahedra
+    std::vector<int32_t> h_tet_mesh;  // mesh index of each magnetic tetrahedron
+    double *hsp = nullptr;            // space-dependent applied field [15][Tm] (fg_set_space_field)
     int Fm = 0;        // charged magnetic triangles (magTri)
     int Fk = 0;        // triangles with surface anisotropy in the RHS
     int64_t S = 0;     // demag sources = 5 Tm + 4 Fm
@@ -146,7 +148,9 @@ struct Prof {
 };
 
 // ---- launchers (kernels.hip) ----
-void launch_element(Ctx &c, const double Hext[3], double dt);
+// hsp: optional applied field per magnetic tet and Gauss point ([15][Tm], A/m), scaled by amp
+// and used in place of Hext
+void launch_element(Ctx &c, const double Hext[3], double dt, const double *hsp = nullptr, double amp = 0);
 void launch_basis(Ctx &c, double r);
 void launch_node_assemble(Ctx &c, double r, double c_stencil, double dt);
 void launch_r0(Ctx &c, double c_stencil);
@@ -155,7 +159,7 @@ void launch_update(Ctx &c, double dt);
 void launch_spmv_debug(Ctx &c, double c_stencil, const double2 *xin, double2 *yout);
 void launch_charges(Ctx &c);
 void launch_direct_p2p(Ctx &c);
-void launch_energy(Ctx &c, const double Hext[3]);
+void launch_energy(Ctx &c, const double Hext[3], const double *hsp = nullptr, double amp = 0);
 void launch_max_angle(Ctx &c);
 void launch_average(Ctx &c, int which);
 void launch_evolution(Ctx &c);

@@ -97,8 +97,8 @@ __global__ __launch_bounds__(BLOCK) void k_element(int Tm, const int4 *__restric
                                                    const int32_t *__restrict__ treg,
                                                    const DevRegion *__restrict__ regs,
                                                    const NodeRec *__restrict__ cur, double Hx,
-                                                   double Hy, double Hz, double dt,
-                                                   double4 *__restrict__ out)
+                                                   double Hy, double Hz, const double *__restrict__ hsp,
+                                                   double amp, double dt, double4 *__restrict__ out)
 {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= Tm) return;
@@ -131,7 +131,8 @@ __global__ __launch_bounds__(BLOCK) void k_element(int Tm, const int4 *__restric
         Hv = Hv - phv[a] * g;
     }
     const double exch = -R.Abis * (dot(dU[0], dU[0]) + dot(dU[1], dU[1]) + dot(dU[2], dU[2]));
-    const d3 H0 = Hd + mk(Hx, Hy, Hz);
+    // applied field: uniform, or amp times the space field at each Gauss point (hsp, [15][Tm])
+    const d3 Hu = Hd + mk(Hx, Hy, Hz);
     const double sdt = THETA * dt;  // s_dt / gamma0
     const d3 uk = mk(R.uk[0], R.uk[1], R.uk[2]);
     const d3 ex = mk(R.ex[0], R.ex[1], R.ex[2]), ey = mk(R.ey[0], R.ey[1], R.ey[2]),
@@ -152,6 +153,9 @@ __global__ __launch_bounds__(BLOCK) void k_element(int Tm, const int4 *__restric
             U = U + tet_a(a, g) * u[a];
             V = V + tet_a(a, g) * v[a];
         }
+        const d3 H0 = hsp ? Hd + amp * mk(hsp[(size_t)(3 * g) * Tm + t], hsp[(size_t)(3 * g + 1) * Tm + t],
+                                          hsp[(size_t)(3 * g + 2) * Tm + t])
+                          : Hu;
         double uH = exch;
         d3 Han = mk(0, 0, 0);
         if (R.hasK) {  // Tet::calc_aniso_uniax (src/tetra.cpp:171-181)
@@ -891,7 +895,8 @@ __global__ __launch_bounds__(BLOCK) void k_energy_tet(int Tm, const int4 *__rest
                                                       const int32_t *__restrict__ treg,
                                                       const DevRegion *__restrict__ regs,
                                                       const NodeRec *__restrict__ nxt, double Hx, double Hy,
-                                                      double Hz, double *__restrict__ part)
+                                                      double Hz, const double *__restrict__ hsp, double amp,
+                                                      double *__restrict__ part)
 {
     double e[4] = {0, 0, 0, 0};
     for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < Tm; t += gridDim.x * blockDim.x) {
@@ -932,7 +937,10 @@ __global__ __launch_bounds__(BLOCK) void k_energy_tet(int Tm, const int4 *__rest
                              a2 = dot(U, mk(R.ez[0], R.ez[1], R.ez[2]));
                 an += w[g] * R.K3 * ((a0 * a1) * (a0 * a1) + (a1 * a2) * (a1 * a2) + (a2 * a0) * (a2 * a0));
             }
-            ze += w[g] * dot(U, H);
+            const d3 Hg = hsp ? amp * mk(hsp[(size_t)(3 * g) * Tm + t], hsp[(size_t)(3 * g + 1) * Tm + t],
+                                         hsp[(size_t)(3 * g + 2) * Tm + t])
+                              : H;
+            ze += w[g] * dot(U, Hg);
         }
         e[0] += R.Ae * ex;
         e[1] += an;
@@ -1070,13 +1078,13 @@ __global__ __launch_bounds__(BLOCK) void k_evolution(int N, const NodeRec *__res
 // ------------------------------------------------------------------ launchers
 static inline int grid1(int64_t n) { return (int)((n + BLOCK - 1) / BLOCK); }
 
-void launch_element(Ctx &c, const double Hext[3], double dt)
+void launch_element(Ctx &c, const double Hext[3], double dt, const double *hsp, double amp)
 {
     const NodeRec *cur = c.rec[c.cur];
     if (c.Tm > 0) {
         Prof pr(c, "element");
         k_element<<<grid1(c.Tm), BLOCK, 0, c.stream>>>(c.Tm, c.tet_nodes, c.geo, c.tet_reg, c.regions, cur, Hext[0],
-                                                       Hext[1], Hext[2], dt, c.elem_out);
+                                                       Hext[1], Hext[2], hsp, amp, dt, c.elem_out);
     }
     if (c.Fk > 0) {
         Prof pr(c, "tri_aniso");
@@ -1181,14 +1189,14 @@ void launch_direct_p2p(Ctx &c)
                                                      c.rec[1 - c.cur], c.d_node_inst, c.d_inst_src);
 }
 
-void launch_energy(Ctx &c, const double Hext[3])
+void launch_energy(Ctx &c, const double Hext[3], const double *hsp, double amp)
 {
     const NodeRec *nxt = c.rec[1 - c.cur];
     const int nbt = c.blocks_for(c.Tm), nbf = c.blocks_for(c.Fm);
     {
         Prof pr(c, "energy");
         k_energy_tet<<<nbt, BLOCK, 0, c.stream>>>(c.Tm, c.tet_nodes, c.geo, c.tet_reg, c.regions, nxt, Hext[0], Hext[1],
-                                                  Hext[2], c.epart);
+                                                  Hext[2], hsp, amp, c.epart);
         if (c.Fm > 0)
             k_energy_tri<<<nbf, BLOCK, 0, c.stream>>>(c.Fm, c.tri_nodes, c.tri_geo, c.tri_geo + (size_t)12 * c.Fm, nxt,
                                                       c.epart);

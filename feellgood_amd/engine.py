@@ -103,6 +103,22 @@ class Engine:
         H = np.ascontiguousarray(Hext, dtype=np.float64)
         self._chk(self.lib.fg_prepare_elements(self._h, nv.dptr(H), dt, prefactor), "fg_prepare_elements")
 
+    def set_space_field(self, h):
+        """Space-dependent applied field, [n_tets, 5, 3] A/m at the Gauss points, or None."""
+        if h is None:
+            self._chk(self.lib.fg_set_space_field(self._h, None), "fg_set_space_field")
+            return
+        hh = np.ascontiguousarray(h, dtype=np.float64).reshape(-1)
+        self._chk(self.lib.fg_set_space_field(self._h, nv.dptr(hh)), "fg_set_space_field")
+
+    def prepare_elements_amp(self, amp, dt, prefactor):
+        self._chk(self.lib.fg_prepare_elements_amp(self._h, amp, dt, prefactor), "fg_prepare_elements_amp")
+
+    def energy_amp(self, amp):
+        E = np.zeros(4)
+        self._chk(self.lib.fg_energy_amp(self._h, amp, nv.dptr(E)), "fg_energy_amp")
+        return E
+
     def solve(self, dt):
         """Returns (failed, SolveInfo) like LinAlgebra::solve's bool (true = failure)."""
         info = nv.SolveInfo()

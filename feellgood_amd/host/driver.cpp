@@ -7,9 +7,10 @@
 // and the loop with its retry semantics) and src/save.cpp:13-186 (.evol columns, .sol files).
 // initial_magnetization and Bext take the reference's three expressions (of x, y, z in metres and
 // of t in seconds, src/settings.cpp:454-543), or a function definition whose body returns them,
-// through fgh::Expr. Not supported, reported as errors: function bodies beyond one return
-// statement, functions of the regions, space-dependent fields, region-qualified evol columns,
-// Thiele_length, recentering and spin accumulation.
+// and Bext as space and time functions (H = time(t) / mu0 * space(x) at the Gauss points), through
+// fgh::Expr. Not supported, reported as errors: function bodies beyond one return statement,
+// functions of the regions, region-qualified evol columns, Thiele_length, recentering and spin
+// accumulation.
 #include <unistd.h>
 
 #include <algorithm>
@@ -64,6 +65,10 @@ struct Config {
     double Bext[3] = {0, 0, 0};
     std::vector<fgh::Expr> B_expr;  // three expressions of t (empty: Bext)
     std::vector<std::string> B_text;
+    // R4toR3 field H = time(t) / mu0 * space(x) (src/settings.cpp:517-537): space of x, y, z at the
+    // tetrahedra's Gauss points, time of t
+    std::vector<fgh::Expr> Bs_expr, Bt_expr;
+    std::vector<std::string> Bs_text, Bt_text;
     long max_iter = 700;
     double tol = 1e-6;
     double du_max = 0.02, dt_min = 1e-16, dt_max = 5e-13;
@@ -98,11 +103,11 @@ bool compile3(const Node &n, const std::vector<std::string> &vars, const std::st
 
 // A JavaScript function definition in a settings key, "function(a, b, ...) { return [e1, e2, e3]; }"
 // (ExpressionParser::set_function, src/expression_parser.cpp:60-71), limited to a body that is one
-// return statement of a three-element array: compiled as three expressions of its parameters,
-// whose number must be nparams. Returns whether any component depends on them (else the values
-// are in val).
+// return statement of a three-element array (ncomp 3) or of one expression (ncomp 1): compiled as
+// expressions of its parameters, whose number must be nparams. Returns whether any component
+// depends on them (else the values are in val).
 bool compile_function(const std::string &src, size_t nparams, const std::string &where, double val[3],
-                      std::vector<fgh::Expr> &ex, std::vector<std::string> &text)
+                      std::vector<fgh::Expr> &ex, std::vector<std::string> &text, size_t ncomp = 3)
 {
     auto bad = [&](const std::string &why) {
         return SettingsError(where + ": " + why + " (supported: function(...) { return [e1, e2, e3]; })");
@@ -128,19 +133,24 @@ bool compile_function(const std::string &src, size_t nparams, const std::string 
     if (body.rfind("return", 0) != 0) throw bad("the body should be one return statement");
     body = trim(body.substr(6));
     if (!body.empty() && body.back() == ';') body = trim(body.substr(0, body.size() - 1));
-    if (body.size() < 2 || body.front() != '[' || body.back() != ']') throw bad("the function should return [x, y, z]");
-    // the three top-level components of the array
     std::vector<std::string> comp(1);
-    int depth = 0;
-    for (size_t k = 1; k + 1 < body.size(); k++) {
-        const char ch = body[k];
-        depth += (ch == '(' || ch == '[') - (ch == ')' || ch == ']');
-        if (ch == ',' && depth == 0)
-            comp.emplace_back();
-        else
-            comp.back() += ch;
+    if (ncomp == 1) {
+        comp[0] = body;
+    } else {
+        if (body.size() < 2 || body.front() != '[' || body.back() != ']')
+            throw bad("the function should return [x, y, z]");
+        // the three top-level components of the array
+        int depth = 0;
+        for (size_t k = 1; k + 1 < body.size(); k++) {
+            const char ch = body[k];
+            depth += (ch == '(' || ch == '[') - (ch == ')' || ch == ']');
+            if (ch == ',' && depth == 0)
+                comp.emplace_back();
+            else
+                comp.back() += ch;
+        }
+        if (comp.size() != 3) throw bad("the function should return three components");
     }
-    if (comp.size() != 3) throw bad("the function should return three components");
     ex.clear();
     text.clear();
     bool var = false;
@@ -154,7 +164,7 @@ bool compile_function(const std::string &src, size_t nparams, const std::string 
         var = var || ex.back().uses_variables();
     }
     if (!var)
-        for (int k = 0; k < 3; k++) val[k] = ex[k].eval(nullptr);
+        for (size_t k = 0; k < ncomp; k++) val[k] = ex[k].eval(nullptr);
     return var;
 }
 
@@ -286,7 +296,12 @@ Config read_config(const Node &root)
         } else if (n->kind == Node::Seq) {
             if (!compile3(*n, {"t"}, "Bext", c.Bext, c.B_expr, c.B_text)) c.B_expr.clear();
         } else {
-            throw SettingsError("Bext: space-dependent fields (space and time functions) are not supported");
+            const Node *sp = n->get("space"), *tm = n->get("time");
+            if (!sp || !tm || sp->kind != Node::Scalar || tm->kind != Node::Scalar)
+                throw SettingsError("Bext should be a function, three expressions, or space and time functions");
+            double dummy[3];
+            compile_function(sp->text, 3, "Bext.space", dummy, c.Bs_expr, c.Bs_text);
+            compile_function(tm->text, 1, "Bext.time", dummy, c.Bt_expr, c.Bt_text, 1);
         }
     }
     if (auto r = root.get("recentering"); r && r->kind == Node::Map)
@@ -416,7 +431,10 @@ int run(const Config &c, unsigned seed, bool verbose, bool check_only)
         for (size_t k = 0; k < c.surf.size(); k++)
             std::cout << "surface region:    " << c.surf_names[k] << " Ks " << c.surf[k].Ks << " suppress_charges "
                       << c.surf[k].suppress_charges << "\n";
-        if (c.B_expr.empty())
+        if (!c.Bs_expr.empty())
+            std::cout << "Bext(x,t):         space " << c.Bs_text[0] << " | " << c.Bs_text[1] << " | " << c.Bs_text[2]
+                      << " time " << c.Bt_text[0] << "\n";
+        else if (c.B_expr.empty())
             std::cout << "Bext:              " << c.Bext[0] << " " << c.Bext[1] << " " << c.Bext[2] << "\n";
         else
             std::cout << "Bext(t):           " << c.B_text[0] << " | " << c.B_text[1] << " | " << c.B_text[2] << "\n";
@@ -481,11 +499,26 @@ int run(const Config &c, unsigned seed, bool verbose, bool check_only)
     e.check(fg_set_state(e.ctx, FG_BOTH, u.data(), nullptr, phi.data(), nullptr), "fg_set_state");
     e.check(fg_base_projection(e.ctx, next_angle()), "fg_base_projection");  // LinAlgebra ctor
 
-    // Settings::getField (src/settings.h:275-283): H = B(t) / mu0
-    double Hext[3];
+    // Settings::getField (src/settings.h:275-283): H = B(t) / mu0; with a space-dependent field
+    // (getFieldSpace, getFieldTime, :309-317) the space part at every Gauss point goes to the
+    // engine once (src/mesh.cpp:377-389) and each step passes the amplitude time(t) / mu0
+    const bool spacef = !c.Bs_expr.empty();
+    double Hext[3], amp = 0;
     auto field_at = [&](double tt) {
-        for (int k = 0; k < 3; k++) Hext[k] = (c.B_expr.empty() ? c.Bext[k] : c.B_expr[k].eval(&tt)) / MU0;
+        if (spacef) {
+            amp = c.Bt_expr[0].eval(&tt) / MU0;
+            for (int k = 0; k < 3; k++) Hext[k] = NAN;  // no uniform field to report
+        } else {
+            for (int k = 0; k < 3; k++) Hext[k] = (c.B_expr.empty() ? c.Bext[k] : c.B_expr[k].eval(&tt)) / MU0;
+        }
     };
+    if (spacef) {
+        std::vector<double> pg(15 * (size_t)view.n_tets), h(pg.size());
+        e.check(fg_gauss_points(&view, pg.data()), "fg_gauss_points");
+        for (size_t q = 0; q < pg.size(); q += 3)
+            for (int k = 0; k < 3; k++) h[q + k] = c.Bs_expr[k].eval(&pg[q]);
+        e.check(fg_set_space_field(e.ctx, h.data()), "fg_set_space_field");
+    }
     field_at(t);
     // timing (src/time_integration.h:6-59)
     const double DTMIN = c.dt_min, DTMAX = c.dt_max, TAUR = 100. * DTMAX;
@@ -500,7 +533,7 @@ int run(const Config &c, unsigned seed, bool verbose, bool check_only)
     double vmax = 0;
     auto compute_all = [&]() {  // src/fem.h:203-224
         e.check(fg_calc_demag(e.ctx), "fg_calc_demag");
-        e.check(fg_energy(e.ctx, Hext, E), "fg_energy");
+        e.check(spacef ? fg_energy_amp(e.ctx, amp, E) : fg_energy(e.ctx, Hext, E), "fg_energy");
         e.check(fg_evolution(e.ctx), "fg_evolution");
     };
     compute_all();
@@ -546,7 +579,9 @@ int run(const Config &c, unsigned seed, bool verbose, bool check_only)
             }
             e.check(fg_base_projection(e.ctx, next_angle()), "fg_base_projection");
             field_at(t);  // the field at the step's start, also used by its energies (src/time_integration.cpp:194-231)
-            e.check(fg_prepare_elements(e.ctx, Hext, dt, prefactor), "fg_prepare_elements");
+            e.check(spacef ? fg_prepare_elements_amp(e.ctx, amp, dt, prefactor)
+                           : fg_prepare_elements(e.ctx, Hext, dt, prefactor),
+                    "fg_prepare_elements");
             fg_solve_info info;
             const int rc = fg_solve(e.ctx, dt, &info);
             e.check(rc, "fg_solve");

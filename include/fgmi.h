@@ -178,6 +178,21 @@ int fg_solve(fg_ctx *ctx, double dt, fg_solve_info *info);
 int fg_calc_demag(fg_ctx *ctx);
 /* Energies of the NEXT state for a uniform applied field Hext [A/m]; E[4] in joules. */
 int fg_energy(fg_ctx *ctx, const double Hext[3], double E[4]);
+/* Space-dependent applied field (the reference's R4toR3 Bext: H(x, t) = A(t) h(x), evaluated at the
+ * tetrahedra's Gauss points, src/mesh.cpp:377-389): h[15 * t + 3 g + k] is component k of h at
+ * Gauss point g of mesh tetrahedron t, in A/m, for every tetrahedron of the fg_mesh (non-magnetic
+ * ones are ignored); NULL removes it. The Gauss point positions come from fgh_gauss_points. */
+int fg_set_space_field(fg_ctx *ctx, const double *h);
+/* Positions of the 5 Gauss points of every tetrahedron of a mesh (the engine's rule, src/tetra.h:50-60,
+ * node order as in the mesh): pg[15 * t + 3 * g + k]. Needs no context. */
+int fg_gauss_points(const fg_mesh *mesh, double *pg);
+/* LinAlgebra::prepareElements(double A_Hext, const timing&) (src/linear_algebra.cpp:54-75): the
+ * element pass with the applied field amp * h at each Gauss point. FG_ERR_STATE without a space
+ * field. */
+int fg_prepare_elements_amp(fg_ctx *ctx, double amp, double dt, double prefactor);
+/* Fem::energy with an R4toR3 field (src/energy.cpp:11-16, src/tetra.cpp:381-391): the Zeeman
+ * term from amp * h at the Gauss points. FG_ERR_STATE without a space field. */
+int fg_energy_amp(fg_ctx *ctx, double amp, double E[4]);
 /* NEXT -> CURRENT. */
 int fg_evolution(fg_ctx *ctx);
 /* acos(min over magnetic edges of u_i . u_j) on the NEXT state. */

@@ -55,6 +55,9 @@ SYMBOLS = {
     "orc_get_sources": (None, [C.c_void_p, _d, _d, _d]),
     "orc_get_corrections": (None, [C.c_void_p, _d, _d]),
     "orc_energy": (None, [C.c_void_p, _d, _d]),
+    "orc_set_space_field": (None, [C.c_void_p, _d]),
+    "orc_prepare_elements_amp": (None, [C.c_void_p, C.c_double, C.c_double, C.c_double]),
+    "orc_energy_amp": (None, [C.c_void_p, C.c_double, _d]),
     "orc_evolution": (None, [C.c_void_p]),
     "orc_max_angle": (C.c_double, [C.c_void_p]),
     "orc_average": (None, [C.c_void_p, C.c_int, _d]),
@@ -163,6 +166,17 @@ class Problem:
         H = np.ascontiguousarray(Hext, dtype=np.float64)
         lib().orc_prepare_elements(self._h, d(H), dt, prefactor)
 
+    def set_space_field(self, h):
+        """Space-dependent applied field, [n_tets, 5, 3] A/m at the Gauss points, or None."""
+        if h is None:
+            lib().orc_set_space_field(self._h, None)
+        else:
+            self._hsp = np.ascontiguousarray(h, dtype=np.float64).reshape(-1)
+            lib().orc_set_space_field(self._h, d(self._hsp))
+
+    def prepare_elements_amp(self, amp, dt, prefactor):
+        lib().orc_prepare_elements_amp(self._h, amp, dt, prefactor)
+
     def solve(self, dt):
         info = SolveInfo()
         rc = lib().orc_solve(self._h, dt, C.byref(info))
@@ -210,6 +224,11 @@ class Problem:
         H = np.ascontiguousarray(Hext, dtype=np.float64)
         E = np.zeros(4)
         lib().orc_energy(self._h, d(H), d(E))
+        return E
+
+    def energy_amp(self, amp):
+        E = np.zeros(4)
+        lib().orc_energy_amp(self._h, amp, d(E))
         return E
 
     def evolution(self):

@@ -215,10 +215,12 @@ void tet_lumping(const double da[4][3], const double w[TNPI], const V3 u_nod[4],
     }
 }
 
-/* Tet::integrales (src/tetra.cpp:210-307) for a uniform Hext, no spin transfer, no drift. */
+/* Tet::integrales (src/tetra.cpp:210-307) with the applied field at each Gauss point (calc_Hext:
+ * a uniform field in every column, src/linear_algebra.cpp:30-35, or A(t) times the space field,
+ * :63-68), no spin transfer, no drift. */
 void tet_integrales(const double da[4][3], const double w[TNPI], const V3 u[4], const V3 v[4],
                     const double phi[4], const double phiv[4], const V3 ep[4], const V3 eq[4],
-                    const fg_volume_region &prm, const V3 &Hext, double dt, double prefactor,
+                    const fg_volume_region &prm, const V3 Hext[TNPI], double dt, double prefactor,
                     double Kp[8][8], double Lp[8])
 {
     const double alpha = prm.alpha;
@@ -281,8 +283,7 @@ void tet_integrales(const double da[4][3], const double w[TNPI], const V3 u[4], 
             uHeff[g] += -K3bis * dot(uk_u, uuu);
         }
     }
-    V3 Heff = Hd + Hext; /* Hst = 0 */
-    for (int g = 0; g < TNPI; g++) uHeff[g] += dot(U[g], Heff);
+    for (int g = 0; g < TNPI; g++) uHeff[g] += dot(U[g], Hd + Hext[g]); /* Hst = 0 */
     double aeff[TNPI];
     for (int g = 0; g < TNPI; g++) aeff[g] = alpha_eff(dt, alpha, uHeff[g]);
 
@@ -321,7 +322,7 @@ void tet_integrales(const double da[4][3], const double w[TNPI], const V3 u[4], 
     V3 BE[4] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
     const double sdt_g = s_dt / GAMMA0;
     for (int g = 0; g < TNPI; g++) {
-        V3 H = (Hd + Hext) + (Han[g] + sdt_g * Hv);
+        V3 H = (Hd + Hext[g]) + (Han[g] + sdt_g * Hv);
         const double wg = w[g];
         for (int i = 0; i < 4; i++) {
             const double ai_w = wg * ta[i][g];
@@ -381,6 +382,7 @@ struct orc_problem {
     std::vector<std::pair<int, int>> edges;
     std::vector<char> magNode;
     std::vector<int> magTet, magTri;
+    std::vector<double> hsp; /* space-dependent applied field [n_tets][5][3] (orc_set_space_field) */
     std::vector<int> lvd;
     /* element matrices (src/element.h:61-65) */
     std::vector<double> Kp;  /* 64 per tet */
@@ -561,10 +563,11 @@ void orc_get_basis(const orc_problem *p, double *ep, double *eq)
     }
 }
 
-/* LinAlgebra::prepareElements (src/linear_algebra.cpp:26-52) */
-void orc_prepare_elements(orc_problem *p, const double Hext_[3], double dt, double prefactor)
+/* LinAlgebra::prepareElements (src/linear_algebra.cpp:26-52; with a space field, the overload of
+ * :54-75: amp times the field at each Gauss point) */
+static void prepare_elements(orc_problem *p, const double Hext_[3], const double *hsp, double amp, double dt,
+                             double prefactor)
 {
-    V3 Hext = v3(Hext_);
     const int T = (int)p->tet.size();
 #pragma omp parallel for schedule(static)
     for (int k = 0; k < T; k++) {
@@ -581,8 +584,13 @@ void orc_prepare_elements(orc_problem *p, const double Hext_[3], double dt, doub
             ep[i] = n.ep;
             eq[i] = n.eq;
         }
+        V3 H[TNPI];
+        for (int g = 0; g < TNPI; g++)
+            H[g] = hsp ? amp * V3{hsp[15 * (size_t)k + 3 * g], hsp[15 * (size_t)k + 3 * g + 1],
+                                  hsp[15 * (size_t)k + 3 * g + 2]}
+                       : v3(Hext_);
         double Kp[8][8], Lp[8];
-        tet_integrales(t.da, t.w, u, v, phi, phiv, ep, eq, p->vol[t.reg], Hext, dt, prefactor, Kp, Lp);
+        tet_integrales(t.da, t.w, u, v, phi, phiv, ep, eq, p->vol[t.reg], H, dt, prefactor, Kp, Lp);
         std::memcpy(&p->Kp[64 * k], Kp, sizeof(Kp));
         std::memcpy(&p->Lp[8 * k], Lp, sizeof(Lp));
     }
@@ -607,6 +615,25 @@ void orc_prepare_elements(orc_problem *p, const double Hext_[3], double dt, doub
             p->LpT[6 * k + i] = dot(n.eq, BE[i]);
         }
     }
+}
+
+void orc_prepare_elements(orc_problem *p, const double Hext[3], double dt, double prefactor)
+{
+    prepare_elements(p, Hext, nullptr, 0.0, dt, prefactor);
+}
+
+void orc_prepare_elements_amp(orc_problem *p, double amp, double dt, double prefactor)
+{
+    const double zero[3] = {0, 0, 0};
+    prepare_elements(p, zero, p->hsp.empty() ? nullptr : p->hsp.data(), amp, dt, prefactor);
+}
+
+void orc_set_space_field(orc_problem *p, const double *h)
+{
+    if (h)
+        p->hsp.assign(h, h + 15 * p->tet.size());
+    else
+        p->hsp.clear();
 }
 
 } // extern "C"
@@ -977,7 +1004,7 @@ void orc_get_corrections(const orc_problem *p, double *cu, double *cv)
 
 /* Fem::energy (src/energy.cpp:5-68) for a uniform field; element energies src/tetra.cpp:309-391,
  * src/triangle.cpp:38-43,80-85 */
-void orc_energy(orc_problem *p, const double Hext_[3], double E[4])
+static void energy(orc_problem *p, const double Hext_[3], const double *hsp, double amp, double E[4])
 {
     V3 Hext = v3(Hext_);
     for (int k = 0; k < 4; k++) E[k] = 0;
@@ -1027,7 +1054,12 @@ void orc_energy(orc_problem *p, const double Hext_[3], double E[4])
             }
             E[FG_E_ANISOTROPY] += prm.K3 * c;
         }
-        for (int g = 0; g < TNPI; g++) ze += t.w[g] * dot(U[g], Hext);
+        for (int g = 0; g < TNPI; g++) {
+            const V3 Hg = hsp ? amp * V3{hsp[15 * (size_t)k + 3 * g], hsp[15 * (size_t)k + 3 * g + 1],
+                                         hsp[15 * (size_t)k + 3 * g + 2]}
+                              : Hext; /* Tet::zeemanEnergy, src/tetra.cpp:374-391 */
+            ze += t.w[g] * dot(U[g], Hg);
+        }
         E[FG_E_ZEEMAN] += -MU0 * prm.Ms * ze;
     }
     for (int k : p->magTri) {
@@ -1054,6 +1086,14 @@ void orc_energy(orc_problem *p, const double Hext_[3], double E[4])
         for (int g = 0; g < FNPI; g++) d += (dot(U[g], f.n) * ph[g]) * f.w[g];
         E[FG_E_DEMAG] += 0.5 * MU0 * f.dMs * d;
     }
+}
+
+void orc_energy(orc_problem *p, const double Hext[3], double E[4]) { energy(p, Hext, nullptr, 0.0, E); }
+
+void orc_energy_amp(orc_problem *p, double amp, double E[4])
+{
+    const double zero[3] = {0, 0, 0};
+    energy(p, zero, p->hsp.empty() ? nullptr : p->hsp.data(), amp, E);
 }
 
 /* Fem::evolution -> Node::evolution (src/node.h:107) */
@@ -1146,7 +1186,9 @@ void orc_tet_integrales(const double *da_, const double *w, const double *u_, co
         eq[i] = v3(eq_ + 3 * i);
     }
     double Kp[8][8];
-    tet_integrales(da, w, u, v, phi, phiv, ep, eq, *prm, v3(Hext), dt, prefactor, Kp, Lp);
+    V3 H[TNPI];
+    for (int g = 0; g < TNPI; g++) H[g] = v3(Hext);
+    tet_integrales(da, w, u, v, phi, phiv, ep, eq, *prm, H, dt, prefactor, Kp, Lp);
     std::memcpy(Kp_, Kp, sizeof(Kp));
 }
 

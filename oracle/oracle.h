@@ -62,6 +62,12 @@ int64_t orc_n_sources(const orc_problem *p);
 void orc_get_sources(const orc_problem *p, double *xyz, double *q_u, double *q_v);
 void orc_get_corrections(const orc_problem *p, double *corr_u, double *corr_v);
 void orc_energy(orc_problem *p, const double Hext[3], double E[4]);
+/* Space-dependent applied field (R4toR3): h[15 t + 3 g + k] at Gauss point g of tet t, A/m; NULL
+ * removes it. The _amp forms use amp * h in place of the uniform field
+ * (src/linear_algebra.cpp:54-75, src/energy.cpp:11-16, src/tetra.cpp:381-391). */
+void orc_set_space_field(orc_problem *p, const double *h);
+void orc_prepare_elements_amp(orc_problem *p, double amp, double dt, double prefactor);
+void orc_energy_amp(orc_problem *p, double amp, double E[4]);
 void orc_evolution(orc_problem *p);
 double orc_max_angle(const orc_problem *p);
 void orc_average(const orc_problem *p, int which, double out[3]);

@@ -70,7 +70,8 @@ def test_expression_settings(golden_dir, tmp_path):
     of t (examples/example5.py:163, examples/example2.py:69-71), or as function definitions that
     return them: accepted, the nodes' normalised initial magnetization is the numpy evaluation at
     the prepared node positions, and the forms outside the supported subset (other function
-    bodies, functions of the regions, space-time fields) are reported."""
+    bodies, functions of the regions, incomplete space-time fields) are reported; a space and time
+    field is accepted."""
     from feellgood_amd.mesh import HostMesh
     s = full_test_settings(golden_dir, tmp_path)
     s["initial_magnetization"] = ["-y", "x", "2e-9"]
@@ -98,9 +99,19 @@ def test_expression_settings(golden_dir, tmp_path):
     assert rc == 0, err
     d2 = parsed(out)
     assert d2["Bext(t)"][0][0] == "0.01*cos(2*PI*5e9*time)" and d2["initial"][0][-3:] == ini[-3:]
+    # a space-dependent field, H = time(t) / mu0 * space(x, y, z) (src/settings.cpp:517-537)
+    s["Bext"] = {"space": "function(x, y, z) { return [x*1e9, 0, -1]; }",
+                 "time": "function(t) { return Math.cos(2*PI*5e9*t); }"}
+    rc, out, err = check(["-"], json.dumps(s))
+    assert rc == 0, err
+    assert parsed(out)["Bext(x,t)"][0] == ["space", "x*1e9", "|", "0", "|", "-1", "time", "Math.cos(2*PI*5e9*t)"]
     for key, val, msg in (("Bext", "function(t, x) { return [0, 0, t]; }", "1 parameter"),
                           ("Bext", "function(t) { return t; }", "return [x, y, z]"),
-                          ("Bext", {"space": "function(x,y,z) {}", "time": "function(t) {}"}, "space-dependent"),
+                          ("Bext", {"space": "function(x,y,z) { return [x, y, z]; }"}, "space and time functions"),
+                          ("Bext", {"space": "function(x,y) { return [x, y, 0]; }", "time": "function(t) { return t; }"},
+                           "3 parameter"),
+                          ("Bext", {"space": "function(x,y,z) { return [x, y, z]; }", "time": "function(t) {}"},
+                           "one return statement"),
                           ("Bext", ["cos(t", "0", "0"], "expected"),
                           ("initial_magnetization", ["x", "y", "w"], "unknown name 'w'"),
                           ("initial_magnetization", "function(x,y,z,regions) { return [x,y,z]; }",
@@ -128,6 +139,15 @@ def test_time_expression_field_on_gpu(golden_dir, tmp_path):
     assert n == n2 == 21 and same == ref
     rot, n3 = run(["cos(2*PI*5e9*t)", "sin(2*PI*5e9*t)", "-1"], "rot")
     assert n3 == 21 and all(np.isfinite(rot)) and np.linalg.norm(np.array(rot[1:4]) - ref[1:4]) > 1e-3
+    # the space and time form (R4toR3): a constant space part times a time part of 1 is the same
+    # field at every Gauss point, so the run is the constant-field one bit for bit
+    st, n4 = run({"space": "function(x, y, z) { return [1 + 0*x, 0*y, -1]; }", "time": "function(t) { return 1; }"},
+                 "space")
+    assert n4 == 21 and st == ref
+    # a field that varies across the ellipsoid changes the trajectory
+    grad, n5 = run({"space": "function(x, y, z) { return [1 + 2e7*x, 0, -1]; }", "time": "function(t) { return 1; }"},
+                   "grad")
+    assert n5 == 21 and all(np.isfinite(grad)) and grad != ref
 
 
 YAML = """# a settings file in block style

@@ -165,6 +165,45 @@ def test_solve_and_update(small):
     assert i1.v_max == pytest.approx(i0.v_max, rel=1e-8)
 
 
+def test_space_field_step_and_energy(small):
+    """The space-dependent applied field (R4toR3, src/linear_algebra.cpp:54-75, src/energy.cpp:11-16):
+    a field varying over the Gauss points of fg_gauss_points, scaled by an amplitude, in the
+    element pass, one solve and the Zeeman energy, engine against oracle (the copper region
+    checks the mesh-to-magnetic-tet mapping of fg_set_space_field)."""
+    import ctypes as C
+    u, v = state(small)
+    p, g = pair(small, u, v)
+    T = small.n_tets
+    pg = np.zeros((T, 5, 3))
+    assert nv.hip_lib().fg_gauss_points(C.byref(small.mesh), nv.dptr(pg)) == nv.FG_OK
+    a = small.arrays()
+    cen = a["pos"][a["tets"]].mean(axis=1)
+    assert np.allclose((pg * np.array([-2 / 15] + [3 / 40] * 4)[None, :, None]).sum(axis=1) / (1 / 6), cen,
+                       rtol=0, atol=1e-20)  # the 5-point rule integrates x exactly
+    L = np.ptp(a["pos"], axis=0).max()
+    h = np.stack([3e4 * np.sin(pg[..., 0] / L * 3), -2e4 * np.cos(pg[..., 1] / L * 2),
+                  1e4 + 5e3 * pg[..., 2] / L], axis=-1)
+    dt, pref = 2e-14, 1.046052
+    for o in (p, g):
+        o.set_space_field(h)
+        o.base_projection(0.42)
+        o.prepare_elements_amp(0.7, dt, pref)
+        o.solve(dt)
+    rp0, c0, v0, b0 = p.system()
+    rp1, c1, v1, b1 = g.system()
+    assert rel(v1, v0) < 1e-12 and rel(b1, b0) < 1e-12
+    un0, vn0, _, _ = p.get_state(nv.FG_NEXT)
+    un1, vn1, _, _ = g.get_state(nv.FG_NEXT)
+    assert rel(vn1, vn0) < 1e-8 and np.abs(un1 - un0).max() <= 1e-12
+    for o in (p, g):
+        o.evolution()
+    E0, E1 = p.energy_amp(-1.3), g.energy_amp(-1.3)
+    assert np.allclose(E1, E0, rtol=1e-8, atol=1e-8 * np.abs(E0).max()) and E0[3] != 0
+    g.set_space_field(None)
+    with pytest.raises(RuntimeError):
+        g.prepare_elements_amp(0.7, dt, pref)
+
+
 def test_energy_average_max_angle(small):
     u, v = state(small)
     p, g = pair(small, u, v)

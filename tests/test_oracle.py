@@ -411,6 +411,55 @@ def test_solve_invariants_and_basis_independence():
     assert np.allclose(vs[0], vs[1], rtol=0, atol=1e-9 * np.abs(vs[0]).max())
 
 
+def test_space_field_reduces_to_the_uniform_field():
+    """The R4toR3 path (src/linear_algebra.cpp:54-75, src/energy.cpp:11-16): a space field equal to
+    the uniform field H at every Gauss point with amplitude 1 gives the uniform path's system,
+    solution and energies exactly; amplitude and field scale together (amp * h = (2 amp) * h / 2)."""
+    m = small_cube()
+    u = random_state(m.n_nodes)
+    H = np.array([2e4, -1e4, 5e3])
+    dt, pref = 1e-13, 1.046052
+    T = m.n_tets
+
+    def run(setup):
+        p = oracle.Problem(m.mesh, tol=1e-12, max_iter=2000)
+        p.set_state(2, u=u)
+        p.calc_demag()
+        p.evolution()
+        p.base_projection(0.3)
+        E = setup(p)
+        rc, _ = p.solve(dt)
+        assert rc == 0
+        return p.system()[2:], p.solution(), E
+
+    def uniform(p):
+        p.prepare_elements(H, dt, pref)
+        return p.energy(H)
+
+    def space(amp, scale):
+        def f(p):
+            p.set_space_field(np.tile(H * scale, (T, 5, 1)))
+            p.prepare_elements_amp(amp, dt, pref)
+            return p.energy_amp(amp)
+        return f
+    (v0, b0), x0, E0 = run(uniform)
+    for amp, scale in ((1.0, 1.0), (0.5, 2.0)):
+        (v1, b1), x1, E1 = run(space(amp, scale))
+        assert np.array_equal(v1, v0) and np.array_equal(b1, b0) and np.array_equal(x1, x0)
+        assert np.array_equal(E1, E0)
+    # a field that varies in space changes the right-hand side only through the Zeeman terms
+    p = oracle.Problem(m.mesh, tol=1e-12, max_iter=2000)
+    p.set_state(2, u=u)
+    p.calc_demag()
+    p.evolution()
+    p.base_projection(0.3)
+    h = np.random.default_rng(4).normal(size=(T, 5, 3)) * 1e4
+    p.set_space_field(h)
+    p.prepare_elements_amp(0.0, dt, pref)
+    E = p.energy_amp(0.0)
+    assert E[3] == 0.0
+
+
 def test_charges_sum_to_zero_divergence_theorem():
     """For a P1 field the volume charges and the surface charges cancel exactly (divergence
     theorem per element, with the 4-point triangle rule exact for linear u.n)."""
