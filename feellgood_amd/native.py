@@ -17,6 +17,8 @@ HOST_LIB = os.path.join(LIB_DIR, "libfgmi_host.so")
 
 FG_OK = 0
 FG_SOLVE_FAILED = 1
+FG_ERR_ARG, FG_ERR_DEVICE, FG_ERR_MESH, FG_ERR_STATE, FG_ERR_RANGE = -1, -2, -3, -4, -5
+FG_ITER_UNDEFINED, FG_ITER_CONVERGED, FG_ITER_OVERFLOW, FG_ITER_CANNOT_CONVERGE = -1, 0, 1, 2
 FG_CURRENT, FG_NEXT, FG_BOTH = 0, 1, 2
 FG_DEMAG_DIRECT, FG_DEMAG_FMM = 0, 1
 FG_PREC_FP64, FG_PREC_MIXED = 0, 1
@@ -82,6 +84,8 @@ HIP_SYMBOLS = {
     "fg_prepare_elements_amp": (C.c_int, [C.c_void_p, C.c_double, C.c_double, C.c_double]),
     "fg_energy_amp": (C.c_int, [C.c_void_p, C.c_double, _d]),
     "fg_gauss_points": (C.c_int, [C.c_void_p, _d]),
+    "fg_electrostatic_potential": (C.c_int, [C.c_void_p, _d, _d, _d, C.c_double, C.c_int32, C.c_int32, _d,
+                                             C.POINTER(SolveInfo)]),
     "fg_evolution": (C.c_int, [C.c_void_p]),
     "fg_max_angle": (C.c_int, [C.c_void_p, _d]),
     "fg_average": (C.c_int, [C.c_void_p, C.c_int, _d]),

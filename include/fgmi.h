@@ -181,7 +181,7 @@ int fg_energy(fg_ctx *ctx, const double Hext[3], double E[4]);
 /* Space-dependent applied field (the reference's R4toR3 Bext: H(x, t) = A(t) h(x), evaluated at the
  * tetrahedra's Gauss points, src/mesh.cpp:377-389): h[15 * t + 3 g + k] is component k of h at
  * Gauss point g of mesh tetrahedron t, in A/m, for every tetrahedron of the fg_mesh (non-magnetic
- * ones are ignored); NULL removes it. The Gauss point positions come from fgh_gauss_points. */
+ * ones are ignored); NULL removes it. The Gauss point positions come from fg_gauss_points. */
 int fg_set_space_field(fg_ctx *ctx, const double *h);
 /* Positions of the 5 Gauss points of every tetrahedron of a mesh (the engine's rule, src/tetra.h:50-60,
  * node order as in the mesh): pg[15 * t + 3 * g + k]. Needs no context. */
@@ -193,6 +193,19 @@ int fg_prepare_elements_amp(fg_ctx *ctx, double amp, double dt, double prefactor
 /* Fem::energy with an R4toR3 field (src/energy.cpp:11-16, src/tetra.cpp:381-391): the Zeeman
  * term from amp * h at the Gauss points. FG_ERR_STATE without a space field. */
 int fg_energy_amp(fg_ctx *ctx, double amp, double E[4]);
+
+/* The electrostatic potential of the current-injection problem on every node of a mesh
+ * (electrostatSolver::checkBoundaryConditions, solve, src/electrostatSolver.cpp:10-28,96-136,
+ * built by spinAcc::spinAcc, src/spinAccumulationSolver.cpp:21-30): conductivity sigma per volume
+ * region (the tetrahedra's Tetra::prm::sigma), current density jn and potential V per surface
+ * region (Triangle::prm::jn, V; NaN where the region sets none), algebra::cg_dir (src/algebra/cg.h:
+ * 67-121) from V = 0 with tolerance tol and max_iter iterations on HIP device `device` (made
+ * current). Returns FG_ERR_ARG unless exactly one surface region has a finite jn and one a finite V,
+ * FG_ERR_MESH on an out-of-range index, FG_ERR_DEVICE on a HIP error, else FG_OK with the potential
+ * in V_out[n_nodes] and the solver's status, iterations, residual and ||b|| in *info (the reference
+ * exits unless the status is FG_ITER_CONVERGED). Needs no context. */
+int fg_electrostatic_potential(const fg_mesh *mesh, const double *sigma, const double *jn, const double *V,
+                               double tol, int32_t max_iter, int32_t device, double *V_out, fg_solve_info *info);
 /* NEXT -> CURRENT. */
 int fg_evolution(fg_ctx *ctx);
 /* acos(min over magnetic edges of u_i . u_j) on the NEXT state. */

@@ -43,6 +43,10 @@ SYMBOLS = {
     "orc_get_basis": (None, [C.c_void_p, _d, _d]),
     "orc_prepare_elements": (None, [C.c_void_p, _d, C.c_double, C.c_double]),
     "orc_solve": (C.c_int, [C.c_void_p, C.c_double, C.POINTER(SolveInfo)]),
+    "orc_cg_dir": (C.c_int, [C.c_int64, _i64, _i32, _d, _d, _d, _d, C.c_int, _i32, C.c_double, C.c_int,
+                             C.POINTER(SolveInfo)]),
+    "orc_electrostatic_potential": (C.c_int, [C.POINTER(nv.Mesh), _d, _d, _d, C.c_double, C.c_int, _d,
+                                              C.POINTER(SolveInfo)]),
     "orc_bicg_dir": (C.c_int, [C.c_int64, _i64, _i32, _d, _d, _d, C.c_int, _i32, C.c_double, C.c_int,
                                C.POINTER(SolveInfo)]),
     "orc_system_size": (None, [C.c_void_p, _i64, _i64]),
@@ -125,6 +129,38 @@ def bicg_dir(rowptr, col, val, rhs, x0, mask=(), tol=1e-6, max_iter=700):
     lib().orc_bicg_dir(len(rhs), rowptr.ctypes.data_as(_i64), col.ctypes.data_as(_i32), d(val), d(rhs),
                        d(x), len(m), m.ctypes.data_as(_i32), tol, max_iter, C.byref(info))
     return x, info
+
+
+def cg_dir(rowptr, col, val, rhs, xd, x0, mask=(), tol=1e-6, max_iter=700):
+    """algebra::cg_dir on a CSR matrix with Dirichlet values xd on the masked indices; returns
+    (x, SolveInfo)."""
+    rowptr = np.ascontiguousarray(rowptr, dtype=np.int64)
+    col = np.ascontiguousarray(col, dtype=np.int32)
+    val = np.ascontiguousarray(val, dtype=np.float64)
+    rhs = np.ascontiguousarray(rhs, dtype=np.float64)
+    xd = np.ascontiguousarray(xd, dtype=np.float64)
+    x = np.array(x0, dtype=np.float64)
+    m = np.ascontiguousarray(mask, dtype=np.int32)
+    info = SolveInfo()
+    lib().orc_cg_dir(len(rhs), rowptr.ctypes.data_as(_i64), col.ctypes.data_as(_i32), d(val), d(rhs), d(xd),
+                     d(x), len(m), m.ctypes.data_as(_i32), tol, max_iter, C.byref(info))
+    return x, info
+
+
+def electrostatic_potential(mesh, sigma, jn, V, tol=1e-8, max_iter=1000):
+    """The electrostatic potential on the nodes of an fg_mesh view (sigma per volume region, jn and
+    V per surface region, NaN for none); returns (V, SolveInfo). Raises ValueError on bad boundary
+    conditions."""
+    sigma = np.ascontiguousarray(sigma, dtype=np.float64)
+    jn = np.ascontiguousarray(jn, dtype=np.float64)
+    V = np.ascontiguousarray(V, dtype=np.float64)
+    out = np.zeros(mesh.n_nodes)
+    info = SolveInfo()
+    rc = lib().orc_electrostatic_potential(C.byref(mesh), d(sigma), d(jn), d(V), tol, max_iter, d(out),
+                                           C.byref(info))
+    if rc < 0:
+        raise ValueError("incorrect boundary conditions for the potential solver")
+    return out, info
 
 
 class Problem:

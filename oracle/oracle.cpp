@@ -22,6 +22,7 @@
 #include <random>
 #include <string>
 #include <utility>
+#include <map>
 #include <vector>
 
 namespace {
@@ -754,6 +755,54 @@ void bicg_dir(Iter &iter, const CSR &A, std::vector<double> &x, const std::vecto
         iter.inc();
     }
 }
+
+/* algebra::cg_dir (src/algebra/cg.h:67-121): conjugate gradient with the diagonal preconditioner,
+   Dirichlet values xd on the masked indices ld */
+void cg_dir(Iter &iter, const CSR &A, std::vector<double> &x, const std::vector<double> &rhs,
+            const std::vector<double> &xd, const std::vector<int> &ld)
+{
+    const size_t DIM = x.size();
+    std::vector<double> p(DIM), q(DIM), r(DIM), z(DIM), dp(DIM), b(rhs), vt(DIM);
+    for (size_t i = 0; i < DIM; i++) {
+        double c = 0;
+        for (int64_t k = A.rowptr[i]; k < A.rowptr[i + 1]; k++)
+            if (A.col[k] == (int)i) c = A.val[k];
+        dp[i] = 1.0 / c;
+    }
+    spmv(A, xd, z);
+    for (size_t i = 0; i < DIM; i++) b[i] -= z[i];
+    mask(ld, b);
+    mask(ld, dp);
+    iter.rhsn = vnorm(b);
+    r = b;
+    spmv(A, x, vt);
+    for (size_t i = 0; i < DIM; i++) r[i] -= vt[i];
+    mask(ld, r);
+    for (size_t i = 0; i < DIM; i++) z[i] = dp[i] * r[i];
+    double rho = vdot(z, r), rho_1 = 0;
+    p = z;
+    while (!iter.converged(vnorm(r)) && iter.status != FG_ITER_OVERFLOW && iter.status != FG_ITER_CANNOT_CONVERGE) {
+        if (iter.nit > 0) {
+            for (size_t i = 0; i < DIM; i++) z[i] = dp[i] * r[i];
+            rho = vdot(z, r);
+            for (size_t i = 0; i < DIM; i++) p[i] *= rho / rho_1;
+            for (size_t i = 0; i < DIM; i++) p[i] += z[i];
+        }
+        spmv(A, p, q);
+        mask(ld, q);
+        const double qp = vdot(q, p);
+        if (qp == 0.0) {
+            iter.status = FG_ITER_CANNOT_CONVERGE;
+            break;
+        }
+        const double a = rho / qp;
+        for (size_t i = 0; i < DIM; i++) x[i] += a * p[i];
+        for (size_t i = 0; i < DIM; i++) r[i] += (-a) * q[i];
+        rho_1 = rho;
+        iter.inc();
+    }
+    for (size_t i = 0; i < DIM; i++) x[i] += xd[i];
+}
 } // namespace
 
 extern "C" {
@@ -831,6 +880,111 @@ int orc_bicg_dir(int64_t n, const int64_t *rowptr, const int32_t *col, const dou
     Iter iter(tol, max_iter);
     bicg_dir(iter, A, X, b, ld);
     std::copy(X.begin(), X.end(), x);
+    if (info) {
+        info->status = iter.status;
+        info->iterations = iter.nit;
+        info->residual = iter.res;
+        info->rhs_norm = iter.rhsn;
+        info->v_max = 0;
+    }
+    return iter.status;
+}
+
+/* algebra::cg_dir on a caller-supplied CSR matrix, Dirichlet values xd[mask_idx] */
+int orc_cg_dir(int64_t n, const int64_t *rowptr, const int32_t *col, const double *val, const double *rhs,
+               const double *xd, double *x, int n_mask, const int32_t *mask_idx, double tol, int max_iter,
+               orc_solve_info *info)
+{
+    CSR A{n, rowptr, col, val};
+    std::vector<double> X(x, x + n), b(rhs, rhs + n), XD(xd, xd + n);
+    std::vector<int> ld(mask_idx, mask_idx + n_mask);
+    Iter iter(tol, max_iter);
+    cg_dir(iter, A, X, b, XD, ld);
+    std::copy(X.begin(), X.end(), x);
+    if (info) {
+        info->status = iter.status;
+        info->iterations = iter.nit;
+        info->residual = iter.res;
+        info->rhs_norm = iter.rhsn;
+        info->v_max = 0;
+    }
+    return iter.status;
+}
+
+/* electrostatSolver::solve (src/electrostatSolver.cpp:96-136) after checkBoundaryConditions
+   (:10-28): K = sum over the tetrahedra of sigma * sum_g w_g da_i . da_j (integrales, :42-65),
+   L = - sum over the triangles of jn * a_ig w_g (:67-76), V fixed on the nodes of the triangles
+   with a potential, cg_dir from V = 0. Returns -1 on bad boundary conditions, else the status. */
+int orc_electrostatic_potential(const fg_mesh *m, const double *sigma, const double *jn, const double *Vbc,
+                                double tol, int max_iter, double *V, orc_solve_info *info)
+{
+    int nJ = 0, nV = 0;
+    for (int k = 0; k < m->n_surf_regions; k++) {
+        nJ += std::isfinite(jn[k]);
+        nV += std::isfinite(Vbc[k]);
+    }
+    if (nJ != 1 || nV != 1) return -1;
+    const int N = m->n_nodes;
+    std::vector<std::map<int, double>> rows(N);
+    for (int t = 0; t < m->n_tets; t++) {
+        V3 P[4];
+        int ind[4];
+        for (int a = 0; a < 4; a++) {
+            ind[a] = m->tet_nodes[4 * t + a];
+            P[a] = {m->pos[3 * ind[a]], m->pos[3 * ind[a] + 1], m->pos[3 * ind[a] + 2]};
+        }
+        double da[4][3], w[TNPI];
+        if (tet_geometry(P, ind, da, w, nullptr)) return -1;
+        const double sg = sigma[m->tet_region[t]];
+        double AE[4][4] = {};
+        for (int g = 0; g < TNPI; g++) {
+            const double sw = sg * w[g];
+            for (int i = 0; i < 4; i++)
+                for (int j = 0; j < 4; j++)
+                    AE[i][j] += sw * (da[i][0] * da[j][0] + da[i][1] * da[j][1] + da[i][2] * da[j][2]);
+        }
+        for (int i = 0; i < 4; i++)
+            for (int j = 0; j < 4; j++) rows[ind[i]][ind[j]] += AE[i][j];
+    }
+    std::vector<double> L(N, 0.0), Vd(N, 0.0);
+    std::vector<int> ld;
+    for (int f = 0; f < m->n_tris; f++) {
+        const int rg = m->tri_region[f];
+        const int *q = m->tri_nodes + 3 * f;
+        V3 P[3];
+        for (int a = 0; a < 3; a++) P[a] = {m->pos[3 * q[a]], m->pos[3 * q[a] + 1], m->pos[3 * q[a] + 2]};
+        const double surf = 0.5 * norm(cross(P[1] - P[0], P[2] - P[0]));
+        double J = rg >= 0 && rg < m->n_surf_regions ? jn[rg] : NAN;
+        if (!std::isfinite(J)) J = 0.0;
+        double BE[3] = {0, 0, 0};
+        for (int g = 0; g < FNPI; g++) {
+            const double Jw = J * (2.0 * surf * fpds[g]);
+            for (int i = 0; i < 3; i++) BE[i] -= fa[i][g] * Jw;
+        }
+        for (int i = 0; i < 3; i++) L[q[i]] += BE[i];
+        if (rg >= 0 && rg < m->n_surf_regions && !std::isnan(Vbc[rg]))
+            for (int i = 0; i < 3; i++) {
+                Vd[q[i]] = Vbc[rg];
+                ld.push_back(q[i]);
+            }
+    }
+    std::sort(ld.begin(), ld.end());
+    ld.erase(std::unique(ld.begin(), ld.end()), ld.end());
+    std::vector<int64_t> rp(N + 1, 0);
+    std::vector<int32_t> cl;
+    std::vector<double> vl;
+    for (int i = 0; i < N; i++) {
+        for (const auto &e : rows[i]) {
+            cl.push_back(e.first);
+            vl.push_back(e.second);
+        }
+        rp[i + 1] = (int64_t)cl.size();
+    }
+    CSR A{N, rp.data(), cl.data(), vl.data()};
+    std::vector<double> X(N, 0.0);
+    Iter iter(tol, max_iter);
+    cg_dir(iter, A, X, L, Vd, ld);
+    std::copy(X.begin(), X.end(), V);
     if (info) {
         info->status = iter.status;
         info->iterations = iter.nit;

@@ -48,6 +48,14 @@ int orc_solve(orc_problem *p, double dt, orc_solve_info *info);
 int orc_bicg_dir(int64_t n, const int64_t *rowptr, const int32_t *col, const double *val,
                  const double *rhs, double *x, int n_mask, const int32_t *mask_idx, double tol,
                  int max_iter, orc_solve_info *info);
+/* algebra::cg_dir on an arbitrary CSR matrix with Dirichlet values xd on mask_idx; returns the status. */
+int orc_cg_dir(int64_t n, const int64_t *rowptr, const int32_t *col, const double *val, const double *rhs,
+               const double *xd, double *x, int n_mask, const int32_t *mask_idx, double tol, int max_iter,
+               orc_solve_info *info);
+/* The electrostatic potential V on the nodes (electrostatSolver::solve): sigma per volume region,
+   jn and V per surface region (NaN: none). Returns -1 on bad boundary conditions, else the status. */
+int orc_electrostatic_potential(const fg_mesh *m, const double *sigma, const double *jn, const double *Vbc,
+                                double tol, int max_iter, double *V, orc_solve_info *info);
 void orc_system_size(const orc_problem *p, int64_t *rows, int64_t *nnz);
 void orc_get_system(const orc_problem *p, int64_t *rowptr, int32_t *col, double *val,
                     double *rhs);

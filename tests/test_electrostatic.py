@@ -1,0 +1,153 @@
+"""The electrostatic potential of the current-injection problem (electrostatSolver,
+src/electrostatSolver.cpp:10-136, algebra::cg_dir, src/algebra/cg.h:67-121).
+
+The oracle's cg_dir is pinned by the reference's own known-answer test (unit-tests/ut_algebra.cpp:
+192-242, the linear solution of a 1-D Laplacian with Dirichlet ends); its assembly and boundary
+terms by the closed form of a bar: V fixed on one end face, a current density jn entering the other,
+conductivities sigma1 and sigma2 in the two halves, so that V is piecewise linear with slopes
+-jn / sigma, which P1 elements reproduce exactly. The device path is checked against the oracle on a
+jittered bar and against the closed form."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import oracle
+from feellgood_amd import native as nv
+from feellgood_amd.mesh import HostMesh
+
+
+def bar(jitter=0.0, cells=(8, 3, 3), lengths=(40.0, 10.0, 12.0), seed=3):
+    """A cuboid split at mid-length into volume regions 0 and 1, with surface region 0 on the
+    x = min face, 1 on the x = max face and 2 elsewhere. Returns (fg_mesh view, owner, pos)."""
+    hm = HostMesh.cuboid(lengths, cells, jitter=jitter, seed=seed)
+    hm.prepare({"volume": {}})
+    a = hm.arrays()
+    pos = np.ascontiguousarray(a["pos"])
+    tets = np.ascontiguousarray(a["tets"])
+    tris = np.ascontiguousarray(a["tris"])
+    x0, x1 = pos[:, 0].min(), pos[:, 0].max()
+    eps = 1e-6 * (x1 - x0)
+    treg = np.ascontiguousarray((pos[tets].mean(axis=1)[:, 0] > 0.5 * (x0 + x1)).astype(np.int32))
+    tx = pos[tris][:, :, 0]
+    freg = np.full(len(tris), 2, np.int32)
+    freg[np.all(np.abs(tx - x0) < eps, axis=1)] = 0
+    freg[np.all(np.abs(tx - x1) < eps, axis=1)] = 1
+    dms = np.zeros(len(tris))
+    vol = (nv.VolumeRegion * 2)()
+    surf = (nv.SurfaceRegion * 3)()
+    m = nv.Mesh()
+    m.n_nodes, m.n_tets, m.n_tris = len(pos), len(tets), len(tris)
+    m.n_vol_regions, m.n_surf_regions = 2, 3
+    m.pos = pos.ctypes.data_as(C.POINTER(C.c_double))
+    m.tet_nodes = tets.ctypes.data_as(C.POINTER(C.c_int32))
+    m.tet_region = treg.ctypes.data_as(C.POINTER(C.c_int32))
+    m.tri_nodes = tris.ctypes.data_as(C.POINTER(C.c_int32))
+    m.tri_region = freg.ctypes.data_as(C.POINTER(C.c_int32))
+    m.tri_dMs = dms.ctypes.data_as(C.POINTER(C.c_double))
+    m.vol = C.cast(vol, C.POINTER(nv.VolumeRegion))
+    m.surf = C.cast(surf, C.POINTER(nv.SurfaceRegion))
+    owner = (hm, pos, tets, tris, treg, freg, dms, vol, surf)
+    return m, owner, pos
+
+
+def closed_form(pos, V0, jn, s1, s2):
+    """V(x) = V0 - jn (x - x0) / sigma1 up to mid-length, then slope -jn / sigma2."""
+    x = pos[:, 0]
+    x0, x1 = x.min(), x.max()
+    xm = 0.5 * (x0 + x1)
+    return np.where(x <= xm, V0 - jn * (x - x0) / s1, V0 - jn * (xm - x0) / s1 - jn * (x - xm) / s2)
+
+
+NAN = float("nan")
+BC = dict(sigma=[3e6, 1.2e6], jn=[NAN, 2e11, NAN], V=[0.25, NAN, NAN])
+
+
+def test_cg_dir_laplacian_known_answer():
+    """unit-tests/ut_algebra.cpp:192-242: a 1-D Laplacian on 1001 nodes, V = 0 and 1 at the ends,
+    no source; the solution is linear, within the solver's tolerance 1e-6."""
+    n = 1001
+    rows, cols, vals = [0, 0, n - 1, n - 1], [0, 1, n - 2, n - 1], [1.0, -1.0, -1.0, 1.0]
+    for i in range(1, n - 1):
+        rows += [i, i, i]
+        cols += [i - 1, i, i + 1]
+        vals += [-1.0, 2.0, -1.0]
+    order = np.lexsort((cols, rows))
+    rows, cols, vals = np.array(rows)[order], np.array(cols)[order], np.array(vals)[order]
+    rowptr = np.searchsorted(rows, np.arange(n + 1))
+    xd = np.zeros(n)
+    xd[n - 1] = 1.0
+    x, info = oracle.cg_dir(rowptr, cols, vals, np.zeros(n), xd, np.zeros(n), mask=[0, n - 1], tol=1e-6,
+                            max_iter=5000)
+    assert info.status == nv.FG_ITER_CONVERGED
+    i = np.arange(0, n, 50)
+    assert np.all(np.abs(x[i] - i / (n - 1)) < 1e-6)
+    assert x[0] == 0.0 and x[n - 1] == 1.0  # the Dirichlet values are restored exactly
+
+
+@pytest.mark.parametrize("jitter,two", [(0.0, True), (0.25, False)])
+def test_oracle_potential_closed_form(jitter, two):
+    """Piecewise-linear potential of the bar (two conductivities on a regular mesh, one on a
+    jittered mesh, where the linear solution is still exact for P1 elements)."""
+    m, _own, pos = bar(jitter)
+    sig = BC["sigma"] if two else [BC["sigma"][0]] * 2
+    V, info = oracle.electrostatic_potential(m, sig, BC["jn"], BC["V"], tol=1e-13, max_iter=5000)
+    assert info.status == nv.FG_ITER_CONVERGED
+    ref = closed_form(pos, BC["V"][0], BC["jn"][1], *sig)
+    assert np.abs(V - ref).max() <= 1e-9 * np.ptp(ref)
+    assert np.ptp(ref) > 1e-4
+
+
+def test_boundary_condition_errors():
+    """checkBoundaryConditions (src/electrostatSolver.cpp:10-28): exactly one surface with a current
+    density and one with a potential, in the oracle and at the C-ABI (which checks them before any
+    device work)."""
+    m, _own, _ = bar()
+    V = np.zeros(m.n_nodes)
+    info = nv.SolveInfo()
+    for jn, Vb in (([NAN, NAN, NAN], BC["V"]), (BC["jn"], [NAN, NAN, NAN]), ([1e11, 2e11, NAN], BC["V"]),
+                   (BC["jn"], [0.0, NAN, 1.0])):
+        with pytest.raises(ValueError):
+            oracle.electrostatic_potential(m, BC["sigma"], jn, Vb)
+        rc = nv.hip_lib().fg_electrostatic_potential(
+            C.byref(m), nv.dptr(np.array(BC["sigma"])), nv.dptr(np.array(jn)), nv.dptr(np.array(Vb)), 1e-8, 1000, 0,
+            nv.dptr(V), C.byref(info))
+        assert rc == nv.FG_ERR_ARG
+
+
+def device_potential(m, sigma, jn, Vb, tol=1e-8, max_iter=1000):
+    V = np.zeros(m.n_nodes)
+    info = nv.SolveInfo()
+    rc = nv.hip_lib().fg_electrostatic_potential(C.byref(m), nv.dptr(np.array(sigma, dtype=np.float64)),
+                                                 nv.dptr(np.array(jn, dtype=np.float64)),
+                                                 nv.dptr(np.array(Vb, dtype=np.float64)), tol, max_iter, 0,
+                                                 nv.dptr(V), C.byref(info))
+    assert rc == nv.FG_OK
+    return V, info
+
+
+@pytest.mark.gpu
+def test_device_potential_matches_oracle():
+    """Device assembly and cg_dir against the oracle on a jittered bar with two conductivities, at
+    the reference's tolerance 1e-8 and 1000 iterations (src/spinAccumulationSolver.cpp:21-22): the
+    same iteration count (within one, the dot products' summation orders differ) and the same
+    potential to the solver's tolerance."""
+    m, _own, _ = bar(0.3, cells=(12, 4, 5))
+    V0, i0 = oracle.electrostatic_potential(m, BC["sigma"], BC["jn"], BC["V"], tol=1e-8, max_iter=1000)
+    V1, i1 = device_potential(m, BC["sigma"], BC["jn"], BC["V"])
+    assert i0.status == i1.status == nv.FG_ITER_CONVERGED
+    assert abs(i1.iterations - i0.iterations) <= 1
+    assert i1.rhs_norm == pytest.approx(i0.rhs_norm, rel=1e-12)
+    assert np.abs(V1 - V0).max() <= 1e-7 * np.ptp(V0)
+
+
+@pytest.mark.gpu
+def test_device_potential_closed_form():
+    m, _own, pos = bar(0.0, cells=(16, 3, 3))
+    V, info = device_potential(m, BC["sigma"], BC["jn"], BC["V"], tol=1e-13, max_iter=5000)
+    assert info.status == nv.FG_ITER_CONVERGED
+    ref = closed_form(pos, BC["V"][0], BC["jn"][1], *BC["sigma"])
+    assert np.abs(V - ref).max() <= 1e-9 * np.ptp(ref)
+    # an iteration limit that is reached is reported, not an error
+    V2, info2 = device_potential(m, BC["sigma"], BC["jn"], BC["V"], tol=1e-13, max_iter=3)
+    assert info2.status == nv.FG_ITER_OVERFLOW and info2.iterations == 3
