@@ -1,0 +1,890 @@
+// transport.hip -- the current-driven side problems of spin-transfer runs: the electrostatic
+// potential of the current injection (FeeLLGood's electrostatSolver, src/electrostatSolver.cpp:
+// 10-136) and the spin accumulation of the spin-diffusion problem (spinAcc,
+// src/spinAccumulationSolver.cpp:41-287). Each assembles its finite-element matrix on the device
+// (every entry summed from its tetrahedra in mesh order, from a list the host builds once), takes
+// the surface terms on the host (surface-sized work), and runs its Krylov solver on the device with
+// the scalars of each iteration read back by the host: conjugate gradient with the diagonal
+// preconditioner and Dirichlet masking (algebra::cg_dir, src/algebra/cg.h:67-121) for the
+// potential, BiCGSTAB with Dirichlet values (algebra::bicg_dir, src/algebra/bicg.h:82-158) for the
+// spin accumulation.
+//
+// Both are solved at set-up (src/spinAccumulationSolver.cpp:21-36), so the kernels favour
+// determinism over speed: every dot product is a fixed tree in each block with the block partials
+// summed in order on the host.
+#include <algorithm>
+#include <cmath>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "../../include/fgmi.h"
+#include "common.h"
+
+using namespace fg;
+
+namespace {
+
+#define ESCHK(expr)                                                                        \
+    do {                                                                                   \
+        hipError_t e_ = (expr);                                                            \
+        if (e_ != hipSuccess) throw std::string(#expr) + ": " + hipGetErrorString(e_);    \
+    } while (0)
+
+constexpr int ESB = 256;  // threads per block
+
+// The shape-function gradients da and det J of tetrahedron t, with the nodes reordered for a positive
+// Jacobian as Tetra::Tet does (src/tetra.h:280-303); perm[k] is the caller's local position of
+// the k-th reordered node, id[] the reordered node indices.
+__device__ inline double tet_grad(const int4 q, const double *__restrict__ pos, double da[4][3], int perm[4], int id[4])
+{
+    id[0] = q.x;
+    id[1] = q.y;
+    id[2] = q.z;
+    id[3] = q.w;
+    d3 P[4];
+#pragma unroll
+    for (int a = 0; a < 4; a++) P[a] = mk(pos[3 * (size_t)id[a]], pos[3 * (size_t)id[a] + 1], pos[3 * (size_t)id[a] + 2]);
+    d3 e1 = P[1] - P[0], e2 = P[2] - P[0], e3 = P[3] - P[0];
+#pragma unroll
+    for (int a = 0; a < 4; a++) perm[a] = a;
+    if (dot(e1, cross(e2, e3)) < 0) {
+        const d3 s = e2;
+        e2 = e3;
+        e3 = s;
+        perm[2] = 3;
+        perm[3] = 2;
+        const int k = id[2];
+        id[2] = id[3];
+        id[3] = k;
+    }
+    const double J[3][3] = {{e1.x, e2.x, e3.x}, {e1.y, e2.y, e3.y}, {e1.z, e2.z, e3.z}};
+    const double detJ = J[0][0] * (J[1][1] * J[2][2] - J[1][2] * J[2][1]) -
+                        J[0][1] * (J[1][0] * J[2][2] - J[1][2] * J[2][0]) +
+                        J[0][2] * (J[1][0] * J[2][1] - J[1][1] * J[2][0]);
+    double cf[3][3];
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+            const int i1 = (i + 1) % 3, i2 = (i + 2) % 3, j1 = (j + 1) % 3, j2 = (j + 2) % 3;
+            cf[i][j] = J[i1][j1] * J[i2][j2] - J[i1][j2] * J[i2][j1];
+        }
+    const double inv = 1.0 / (cf[0][0] * J[0][0] + cf[1][0] * J[1][0] + cf[2][0] * J[2][0]);
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        const double j0 = cf[k][0] * inv, j1 = cf[k][1] * inv, j2 = cf[k][2] * inv;  // column k of J^-1
+        da[0][k] = -j0 - j1 - j2;
+        da[1][k] = j0;
+        da[2][k] = j1;
+        da[3][k] = j2;
+    }
+    return detJ;
+}
+
+// K_e[i][j] = sigma sum_g w_g da_i . da_j (electrostatSolver::integrales, src/electrostatSolver.cpp:42-65),
+// stored at the caller's local positions, [16][T].
+__global__ __launch_bounds__(ESB) void k_es_element(int T, const int4 *__restrict__ tn, const double *__restrict__ pos,
+                                                    const double *__restrict__ tsig, double *__restrict__ ke)
+{
+    const int t = blockIdx.x * ESB + threadIdx.x;
+    if (t >= T) return;
+    double da[4][3];
+    int perm[4], id[4];
+    const double detJ = tet_grad(tn[t], pos, da, perm, id);
+    const double sg = tsig[t];
+    double AE[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int j = 0; j < 4; j++) AE[i][j] = 0.0;
+#pragma unroll
+    for (int g = 0; g < NPI_TET; g++) {
+        const double sw = sg * (detJ * TET_PDS[g]);
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+#pragma unroll
+            for (int j = 0; j < 4; j++)
+                AE[i][j] += sw * (da[i][0] * da[j][0] + da[i][1] * da[j][1] + da[i][2] * da[j][2]);
+    }
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int j = 0; j < 4; j++) ke[(size_t)(perm[i] * 4 + perm[j]) * T + t] = AE[i][j];
+}
+
+// each CSR entry is the sum of its tetrahedra's K_e entries in mesh order (solver::buildMat,
+// src/solver.h:110-131)
+__global__ __launch_bounds__(ESB) void k_es_gather(int64_t nnz, const int64_t *__restrict__ cptr,
+                                                   const int64_t *__restrict__ cslot, const double *__restrict__ ke,
+                                                   double *__restrict__ val)
+{
+    const int64_t k = (int64_t)blockIdx.x * ESB + threadIdx.x;
+    if (k >= nnz) return;
+    double s = 0.0;
+    for (int64_t c = cptr[k]; c < cptr[k + 1]; c++) s += ke[cslot[c]];
+    val[k] = s;
+}
+
+// sum of the block's values, a fixed tree; thread 0 writes the block's partial
+__device__ inline void block_sum(double v, double *sh, double *out)
+{
+    sh[threadIdx.x] = v;
+    __syncthreads();
+#pragma unroll
+    for (int s = ESB / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *out = sh[0];
+}
+
+// D = 1 / diag(K), zero on the Dirichlet nodes (build_diag_precond, src/algebra/sparseMat.h:173-183,
+// then applyMask)
+__global__ __launch_bounds__(ESB) void k_es_diag(int n, const int64_t *__restrict__ rp, const int32_t *__restrict__ col,
+                                                 const double *__restrict__ val, const uint8_t *__restrict__ dir,
+                                                 double *__restrict__ D)
+{
+    const int i = blockIdx.x * ESB + threadIdx.x;
+    if (i >= n) return;
+    double c = 0.0;
+    for (int64_t k = rp[i]; k < rp[i + 1]; k++)
+        if (col[k] == i) c = val[k];
+    D[i] = dir[i] ? 0.0 : 1.0 / c;
+}
+
+// y = K x, zero on the Dirichlet nodes when mask is set; partial sums of y . w (w may be null)
+__global__ __launch_bounds__(ESB) void k_es_spmv(int n, const int64_t *__restrict__ rp, const int32_t *__restrict__ col,
+                                                 const double *__restrict__ val, const double *__restrict__ x,
+                                                 const uint8_t *__restrict__ dir, int mask, double *__restrict__ y,
+                                                 const double *__restrict__ w, double *__restrict__ part)
+{
+    __shared__ double sh[ESB];
+    const int i = blockIdx.x * ESB + threadIdx.x;
+    double yw = 0.0;
+    if (i < n) {
+        double s = 0.0;
+        for (int64_t k = rp[i]; k < rp[i + 1]; k++) s += val[k] * x[col[k]];
+        if (mask && dir[i]) s = 0.0;
+        y[i] = s;
+        if (w) yw = s * w[i];
+    }
+    if (part) block_sum(yw, sh, part + blockIdx.x);
+}
+
+// b = rhs - K xd, masked; r = b; z = D r; p = z; partials of r . r and z . r
+__global__ __launch_bounds__(ESB) void k_es_init(int n, const double *__restrict__ rhs, const double *__restrict__ kxd,
+                                                 const uint8_t *__restrict__ dir, const double *__restrict__ D,
+                                                 double *__restrict__ r, double *__restrict__ z, double *__restrict__ p,
+                                                 double *__restrict__ prr, double *__restrict__ pzr)
+{
+    __shared__ double sh[ESB];
+    const int i = blockIdx.x * ESB + threadIdx.x;
+    double rr = 0.0, zr = 0.0;
+    if (i < n) {
+        double b = rhs[i];
+        b -= kxd[i];
+        if (dir[i]) b = 0.0;
+        const double zi = D[i] * b;
+        r[i] = b;
+        z[i] = zi;
+        p[i] = zi;
+        rr = b * b;
+        zr = zi * b;
+    }
+    block_sum(rr, sh, prr + blockIdx.x);
+    block_sum(zr, sh, pzr + blockIdx.x);
+}
+
+// p = p (rho / rho_1) + z
+__global__ __launch_bounds__(ESB) void k_es_dir(int n, double ratio, const double *__restrict__ z, double *__restrict__ p)
+{
+    const int i = blockIdx.x * ESB + threadIdx.x;
+    if (i >= n) return;
+    double v = p[i];
+    v *= ratio;
+    v += z[i];
+    p[i] = v;
+}
+
+// x += a p; r += (-a) q; z = D r; partials of r . r and z . r
+__global__ __launch_bounds__(ESB) void k_es_update(int n, double a, const double *__restrict__ p,
+                                                   const double *__restrict__ q, const double *__restrict__ D,
+                                                   double *__restrict__ x, double *__restrict__ r, double *__restrict__ z,
+                                                   double *__restrict__ prr, double *__restrict__ pzr)
+{
+    __shared__ double sh[ESB];
+    const int i = blockIdx.x * ESB + threadIdx.x;
+    double rr = 0.0, zr = 0.0;
+    if (i < n) {
+        x[i] += a * p[i];
+        const double ri = r[i] + (-a) * q[i];
+        r[i] = ri;
+        const double zi = D[i] * ri;
+        z[i] = zi;
+        rr = ri * ri;
+        zr = zi * ri;
+    }
+    block_sum(rr, sh, prr + blockIdx.x);
+    block_sum(zr, sh, pzr + blockIdx.x);
+}
+
+__global__ __launch_bounds__(ESB) void k_es_add(int n, const double *__restrict__ xd, double *__restrict__ x)
+{
+    const int i = blockIdx.x * ESB + threadIdx.x;
+    if (i < n) x[i] += xd[i];
+}
+
+struct DevBufs {
+    std::vector<void *> ptrs;
+    template <class T>
+    T *alloc(size_t n)
+    {
+        void *p = nullptr;
+        ESCHK(hipMalloc(&p, std::max<size_t>(n, 1) * sizeof(T)));
+        ptrs.push_back(p);
+        return (T *)p;
+    }
+    template <class T>
+    T *upload(const std::vector<T> &h)
+    {
+        T *d = alloc<T>(h.size());
+        if (!h.empty()) ESCHK(hipMemcpy(d, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice));
+        return d;
+    }
+    ~DevBufs()
+    {
+        for (void *p : ptrs) (void)hipFree(p);
+    }
+};
+
+double host_sum(const std::vector<double> &h)
+{
+    double s = 0.0;
+    for (double v : h) s += v;
+    return s;
+}
+
+
+// ---- spin accumulation ----
+
+// Per tetrahedron (spinAcc::integrales, src/spinAccumulationSolver.cpp:216-287), at the caller's
+// local positions: rows 0-15 the diffusion block A_ij = (D0 sum w) da_i . da_j + [i == j] D0 / lsf^2
+// a_w,i (Tet::calcDiagBlock, src/tetra.cpp:133-140); rows 16-27 the s-d exchange of node i,
+// D0 / lsd^2 a_w,i u_i (X, Y, Z); rows 28-39 the source cst0 sum_g w_g (grad V . da_i) u_i; the
+// last two zero outside magnetic regions. prm = (D0, D0 / lsf^2, D0 / lsd^2, cst0).
+__global__ __launch_bounds__(ESB) void k_sa_element(int T, const int4 *__restrict__ tn, const double *__restrict__ pos,
+                                                    const double4 *__restrict__ prm, const uint8_t *__restrict__ mag,
+                                                    const double *__restrict__ V, const double *__restrict__ u,
+                                                    double *__restrict__ ke)
+{
+    const int t = blockIdx.x * ESB + threadIdx.x;
+    if (t >= T) return;
+    double da[4][3];
+    int perm[4], id[4];
+    const double detJ = tet_grad(tn[t], pos, da, perm, id);
+    const double4 pr = prm[t];
+    double w[NPI_TET], wsum = 0.0, aw[4];
+#pragma unroll
+    for (int g = 0; g < NPI_TET; g++) {
+        w[g] = detJ * TET_PDS[g];
+        wsum += w[g];
+    }
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        aw[i] = 0.0;
+#pragma unroll
+        for (int g = 0; g < NPI_TET; g++) aw[i] += tet_a(i, g) * w[g];
+    }
+    const double c = pr.x * wsum;
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            double v = (da[i][0] * da[j][0] + da[i][1] * da[j][1] + da[i][2] * da[j][2]) * c;
+            if (i == j) v += pr.y * aw[i];
+            ke[(size_t)(perm[i] * 4 + perm[j]) * T + t] = v;
+        }
+    const bool mg = mag[t];
+    double gV[3] = {0.0, 0.0, 0.0};
+    if (mg)
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            const double vi = V[id[i]];
+#pragma unroll
+            for (int k = 0; k < 3; k++) gV[k] += vi * da[i][k];
+        }
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        const double ui[3] = {u[3 * (size_t)id[i]], u[3 * (size_t)id[i] + 1], u[3 * (size_t)id[i] + 2]};
+        double be[3] = {0.0, 0.0, 0.0};
+        if (mg)
+#pragma unroll
+            for (int g = 0; g < NPI_TET; g++) {
+                const double cw = pr.w * w[g];
+                const double tmp = (cw * gV[0]) * da[i][0] + (cw * gV[1]) * da[i][1] + (cw * gV[2]) * da[i][2];
+#pragma unroll
+                for (int k = 0; k < 3; k++) be[k] += tmp * ui[k];
+            }
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            ke[(size_t)(16 + 3 * perm[i] + k) * T + t] = mg ? pr.z * (aw[i] * ui[k]) : 0.0;
+            ke[(size_t)(28 + 3 * perm[i] + k) * T + t] = be[k];
+        }
+    }
+}
+
+// per node, in mesh order of its tetrahedra: the s-d exchange sums S (X, Y, Z) and the volume
+// right-hand side; slot = local position * T + tetrahedron
+__global__ __launch_bounds__(ESB) void k_sa_node(int N, int T, const int64_t *__restrict__ nptr,
+                                                 const int64_t *__restrict__ nslot, const double *__restrict__ ke,
+                                                 double *__restrict__ S, double *__restrict__ L)
+{
+    const int i = blockIdx.x * ESB + threadIdx.x;
+    if (i >= N) return;
+    double sk[3] = {0.0, 0.0, 0.0}, b[3] = {0.0, 0.0, 0.0};
+    for (int64_t c = nptr[i]; c < nptr[i + 1]; c++) {
+        const int64_t l = nslot[c] / T, t = nslot[c] % T;
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            sk[k] += ke[(size_t)(16 + 3 * l + k) * T + t];
+            b[k] += ke[(size_t)(28 + 3 * l + k) * T + t];
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        S[3 * (size_t)i + k] = sk[k];
+        L[3 * (size_t)i + k] = b[k];
+    }
+}
+
+// entry (d, e) of node i's own 3x3 block: the diffusion diagonal, and the antisymmetric s-d
+// exchange (rows y, z of X; x, z of Y; x, y of Z, with the signs of spinAcc::integrales)
+__device__ inline double self_block(int d, int e, double a, const double *S)
+{
+    if (d == e) return a;
+    if (d == 0) return e == 1 ? S[2] : -S[1];
+    if (d == 1) return e == 0 ? -S[2] : S[0];
+    return e == 0 ? S[1] : -S[0];
+}
+
+// y = K x over the 3 N rows (row 3 i + d), zero on the Dirichlet rows when mask is set; partials of
+// y . w (w may be null) and, when pyy is set, of y . y
+__global__ __launch_bounds__(ESB) void k_sa_spmv(int n3, const int64_t *__restrict__ rp, const int32_t *__restrict__ col,
+                                                 const double *__restrict__ a, const double *__restrict__ S,
+                                                 const double *__restrict__ x, const uint8_t *__restrict__ dir, int mask,
+                                                 double *__restrict__ y, const double *__restrict__ w,
+                                                 double *__restrict__ pyw, double *__restrict__ pyy)
+{
+    __shared__ double sh[ESB];
+    const int r = blockIdx.x * ESB + threadIdx.x;
+    double yw = 0.0, yy = 0.0;
+    if (r < n3) {
+        const int i = r / 3, d = r - 3 * i;
+        double s = 0.0;
+        for (int64_t k = rp[i]; k < rp[i + 1]; k++) {
+            const int j = col[k];
+            if (j != i) {
+                s += a[k] * x[3 * (size_t)j + d];
+            } else {
+#pragma unroll
+                for (int e = 0; e < 3; e++) s += self_block(d, e, a[k], S + 3 * (size_t)i) * x[3 * (size_t)i + e];
+            }
+        }
+        if (mask && dir[i]) s = 0.0;
+        y[r] = s;
+        if (w) yw = s * w[r];
+        yy = s * s;
+    }
+    if (pyw) block_sum(yw, sh, pyw + blockIdx.x);
+    if (pyy) block_sum(yy, sh, pyy + blockIdx.x);
+}
+
+// D = 1 / diag(K), zero on the Dirichlet rows
+__global__ __launch_bounds__(ESB) void k_sa_diag(int n3, const int64_t *__restrict__ rp, const int32_t *__restrict__ col,
+                                                 const double *__restrict__ a, const uint8_t *__restrict__ dir,
+                                                 double *__restrict__ D)
+{
+    const int r = blockIdx.x * ESB + threadIdx.x;
+    if (r >= n3) return;
+    const int i = r / 3;
+    double c = 0.0;
+    for (int64_t k = rp[i]; k < rp[i + 1]; k++)
+        if (col[k] == i) c = a[k];
+    D[r] = dir[i] ? 0.0 : 1.0 / c;
+}
+
+// b = L - K xd, masked; r = rt = p = b; partials of b . b
+__global__ __launch_bounds__(ESB) void k_sa_init(int n3, const double *__restrict__ L, const double *__restrict__ kxd,
+                                                 const uint8_t *__restrict__ dir, double *__restrict__ r,
+                                                 double *__restrict__ rt, double *__restrict__ p, double *__restrict__ pbb)
+{
+    __shared__ double sh[ESB];
+    const int k = blockIdx.x * ESB + threadIdx.x;
+    double bb = 0.0;
+    if (k < n3) {
+        double b = L[k];
+        b -= kxd[k];
+        if (dir[k / 3]) b = 0.0;
+        r[k] = b;
+        rt[k] = b;
+        p[k] = b;
+        bb = b * b;
+    }
+    block_sum(bb, sh, pbb + blockIdx.x);
+}
+
+// p = (p - omega v) beta + r unless first; phat = D p
+__global__ __launch_bounds__(ESB) void k_sa_p(int n3, int first, double omega, double beta, const double *__restrict__ v,
+                                              const double *__restrict__ r, const double *__restrict__ D,
+                                              double *__restrict__ p, double *__restrict__ phat)
+{
+    const int k = blockIdx.x * ESB + threadIdx.x;
+    if (k >= n3) return;
+    double pv = p[k];
+    if (!first) {
+        pv -= v[k] * omega;
+        pv *= beta;
+        pv += r[k];
+        p[k] = pv;
+    }
+    phat[k] = D[k] * pv;
+}
+
+// s = r - alpha v; shat = D s; partials of s . s
+__global__ __launch_bounds__(ESB) void k_sa_s(int n3, double alpha, const double *__restrict__ r,
+                                              const double *__restrict__ v, const double *__restrict__ D,
+                                              double *__restrict__ sv, double *__restrict__ shat, double *__restrict__ pss)
+{
+    __shared__ double sh[ESB];
+    const int k = blockIdx.x * ESB + threadIdx.x;
+    double ss = 0.0;
+    if (k < n3) {
+        const double s = r[k] + (-alpha) * v[k];
+        sv[k] = s;
+        shat[k] = D[k] * s;
+        ss = s * s;
+    }
+    block_sum(ss, sh, pss + blockIdx.x);
+}
+
+// x += alpha phat
+__global__ __launch_bounds__(ESB) void k_sa_x1(int n3, double alpha, const double *__restrict__ phat, double *__restrict__ x)
+{
+    const int k = blockIdx.x * ESB + threadIdx.x;
+    if (k < n3) x[k] += alpha * phat[k];
+}
+
+// x += alpha phat; x += omega shat; r = s - omega t; partials of r . r and rt . r
+__global__ __launch_bounds__(ESB) void k_sa_x(int n3, double alpha, double omega, const double *__restrict__ phat,
+                                              const double *__restrict__ shat, const double *__restrict__ sv,
+                                              const double *__restrict__ tv, const double *__restrict__ rt,
+                                              double *__restrict__ x, double *__restrict__ r, double *__restrict__ prr,
+                                              double *__restrict__ prt)
+{
+    __shared__ double sh[ESB];
+    const int k = blockIdx.x * ESB + threadIdx.x;
+    double rr = 0.0, rtr = 0.0;
+    if (k < n3) {
+        double xv = x[k];
+        xv += alpha * phat[k];
+        xv += omega * shat[k];
+        x[k] = xv;
+        const double rv = sv[k] - tv[k] * omega;
+        r[k] = rv;
+        rr = rv * rv;
+        rtr = rt[k] * rv;
+    }
+    block_sum(rr, sh, prr + blockIdx.x);
+    block_sum(rtr, sh, prt + blockIdx.x);
+}
+
+// The CSR pattern of the node graph of the tetrahedra, and for each entry the list of its
+// tetrahedra's element-matrix slots ((i * 4 + j) * T + t) in mesh order.
+struct NodePattern {
+    std::vector<int64_t> rp, cptr, cslot;
+    std::vector<int32_t> col;
+};
+
+NodePattern node_pattern(const fg_mesh *m)
+{
+    const int N = m->n_nodes, T = m->n_tets;
+    std::vector<std::vector<std::pair<int32_t, int64_t>>> rows(N);
+    for (int t = 0; t < T; t++) {
+        const int32_t *q = m->tet_nodes + 4 * (size_t)t;
+        for (int i = 0; i < 4; i++)
+            for (int j = 0; j < 4; j++) rows[q[i]].push_back({q[j], (int64_t)(i * 4 + j) * T + t});
+    }
+    NodePattern P;
+    P.rp.assign(N + 1, 0);
+    P.cptr.assign(1, 0);
+    P.cslot.reserve((size_t)16 * T);
+    for (int i = 0; i < N; i++) {
+        auto &r = rows[i];
+        std::stable_sort(r.begin(), r.end(), [](const auto &a, const auto &b) { return a.first < b.first; });
+        for (size_t k = 0; k < r.size(); k++) {
+            if (k == 0 || r[k].first != r[k - 1].first) {
+                if (k > 0) P.cptr.push_back((int64_t)P.cslot.size());
+                P.col.push_back(r[k].first);
+            }
+            P.cslot.push_back(r[k].second);
+        }
+        if (!r.empty()) P.cptr.push_back((int64_t)P.cslot.size());
+        P.rp[i + 1] = (int64_t)P.col.size();
+        std::vector<std::pair<int32_t, int64_t>>().swap(r);
+    }
+    return P;
+}
+
+// tetrahedra nodes checked against the mesh; FG_OK or FG_ERR_MESH
+int check_tets(const fg_mesh *m, std::vector<int4> &tn)
+{
+    tn.resize(m->n_tets);
+    for (int t = 0; t < m->n_tets; t++) {
+        const int32_t *q = m->tet_nodes + 4 * (size_t)t;
+        for (int a = 0; a < 4; a++)
+            if (q[a] < 0 || q[a] >= m->n_nodes) return FG_ERR_MESH;
+        const int rg = m->tet_region[t];
+        if (rg < 0 || rg >= m->n_vol_regions) return FG_ERR_MESH;
+        tn[t] = make_int4(q[0], q[1], q[2], q[3]);
+    }
+    for (int64_t f = 0; f < 3 * (int64_t)m->n_tris; f++)
+        if (m->tri_nodes[f] < 0 || m->tri_nodes[f] >= m->n_nodes) return FG_ERR_MESH;
+    return FG_OK;
+}
+
+double tri_area(const fg_mesh *m, int f)
+{
+    const int32_t *q = m->tri_nodes + 3 * (size_t)f;
+    const double *p0 = m->pos + 3 * (size_t)q[0], *p1 = m->pos + 3 * (size_t)q[1], *p2 = m->pos + 3 * (size_t)q[2];
+    const d3 e1 = mk(p1[0] - p0[0], p1[1] - p0[1], p1[2] - p0[2]), e2 = mk(p2[0] - p0[0], p2[1] - p0[1], p2[2] - p0[2]);
+    const d3 c = cross(e1, e2);
+    return 0.5 * std::sqrt(dot(c, c));
+}
+}  // namespace
+
+extern "C" int fg_electrostatic_potential(const fg_mesh *m, const double *sigma, const double *jn, const double *Vbc,
+                                          double tol, int32_t max_iter, int32_t device, double *V_out,
+                                          fg_solve_info *info)
+{
+    if (!m || !sigma || !jn || !Vbc || !V_out || !m->pos || !m->tet_nodes || !m->tet_region || m->n_nodes <= 0 ||
+        m->n_tets <= 0 || (m->n_tris > 0 && (!m->tri_nodes || !m->tri_region)) || !(tol > 0) || max_iter < 1)
+        return FG_ERR_ARG;
+    // electrostatSolver::checkBoundaryConditions (src/electrostatSolver.cpp:10-28): one surface
+    // region with a current density and one with a potential
+    int nJ = 0, nV = 0;
+    for (int k = 0; k < m->n_surf_regions; k++) {
+        nJ += std::isfinite(jn[k]);
+        nV += std::isfinite(Vbc[k]);
+    }
+    if (nJ != 1 || nV != 1) return FG_ERR_ARG;
+    std::vector<int4> tn;
+    if (check_tets(m, tn) != FG_OK) return FG_ERR_MESH;
+    const int N = m->n_nodes, T = m->n_tets;
+    std::vector<double> tsig(T);
+    for (int t = 0; t < T; t++) tsig[t] = sigma[m->tet_region[t]];
+    // CSR pattern of the node graph; the entries of each (row, column) listed in mesh order
+    const NodePattern P = node_pattern(m);
+    if (P.rp[N] == 0) return FG_ERR_MESH;
+    // L = - sum over the triangles of jn a_ig w_g (src/electrostatSolver.cpp:67-76, buildVect), and
+    // the Dirichlet nodes with their potentials (:112-127)
+    std::vector<double> L(N, 0.0), Vd(N, 0.0);
+    std::vector<uint8_t> dir(N, 0);
+    for (int f = 0; f < m->n_tris; f++) {
+        const int32_t *q = m->tri_nodes + 3 * (size_t)f;
+        const int rg = m->tri_region[f];
+        const bool has = rg >= 0 && rg < m->n_surf_regions;
+        double J = has ? jn[rg] : NAN;
+        if (!std::isfinite(J)) J = 0.0;
+        const double surf = tri_area(m, f);
+        double BE[3] = {0.0, 0.0, 0.0};
+        for (int g = 0; g < NPI_TRI; g++) {
+            const double Jw = J * (2.0 * surf * TRI_PDS[g]);
+            for (int i = 0; i < 3; i++) BE[i] -= tri_a(i, g) * Jw;
+        }
+        for (int i = 0; i < 3; i++) L[q[i]] += BE[i];
+        if (has && !std::isnan(Vbc[rg]))
+            for (int i = 0; i < 3; i++) {
+                Vd[q[i]] = Vbc[rg];
+                dir[q[i]] = 1;
+            }
+    }
+    try {
+        ESCHK(hipSetDevice(device));
+        DevBufs B;
+        const int64_t nnz = P.rp[N];
+        const int nb = (N + ESB - 1) / ESB;
+        int4 *d_tn = B.upload(tn);
+        double *d_pos = B.upload(std::vector<double>(m->pos, m->pos + 3 * (size_t)N));
+        double *d_sig = B.upload(tsig);
+        double *d_ke = B.alloc<double>((size_t)16 * T);
+        int64_t *d_cptr = B.upload(P.cptr), *d_cslot = B.upload(P.cslot), *d_rp = B.upload(P.rp);
+        int32_t *d_col = B.upload(P.col);
+        double *d_val = B.alloc<double>(nnz);
+        uint8_t *d_dir = B.upload(dir);
+        double *d_L = B.upload(L), *d_Vd = B.upload(Vd);
+        double *d_D = B.alloc<double>(N), *d_x = B.alloc<double>(N), *d_r = B.alloc<double>(N),
+               *d_z = B.alloc<double>(N), *d_p = B.alloc<double>(N), *d_q = B.alloc<double>(N);
+        double *d_part = B.alloc<double>((size_t)2 * nb);
+        ESCHK(hipMemset(d_x, 0, (size_t)N * sizeof(double)));
+        k_es_element<<<(T + ESB - 1) / ESB, ESB>>>(T, d_tn, d_pos, d_sig, d_ke);
+        ESCHK(hipGetLastError());
+        k_es_gather<<<(unsigned)((nnz + ESB - 1) / ESB), ESB>>>(nnz, d_cptr, d_cslot, d_ke, d_val);
+        k_es_diag<<<nb, ESB>>>(N, d_rp, d_col, d_val, d_dir, d_D);
+        // b = L - K Vd (masked), r = b (the start is V = 0), z = D r, p = z
+        k_es_spmv<<<nb, ESB>>>(N, d_rp, d_col, d_val, d_Vd, d_dir, 0, d_q, nullptr, nullptr);
+        k_es_init<<<nb, ESB>>>(N, d_L, d_q, d_dir, d_D, d_r, d_z, d_p, d_part, d_part + nb);
+        ESCHK(hipGetLastError());
+        std::vector<double> hp(2 * (size_t)nb);
+        auto read2 = [&](double &a, double &b) {
+            ESCHK(hipMemcpy(hp.data(), d_part, hp.size() * sizeof(double), hipMemcpyDeviceToHost));
+            a = host_sum(std::vector<double>(hp.begin(), hp.begin() + nb));
+            b = host_sum(std::vector<double>(hp.begin() + nb, hp.end()));
+        };
+        double rr, zr;
+        read2(rr, zr);
+        // iteration<double> (src/algebra/iter.h:119-166)
+        const double rhsn = std::sqrt(rr);
+        double res = 0, rho = zr, rho_1 = 0;
+        int nit = 0, status = FG_ITER_UNDEFINED;
+        auto converged = [&](double nr) {
+            res = std::fabs(nr);
+            if (std::isnan(res)) {
+                status = FG_ITER_CANNOT_CONVERGE;
+                return false;
+            }
+            const bool cv = res <= rhsn * tol;
+            if (cv) status = FG_ITER_CONVERGED;
+            return cv;
+        };
+        while (!converged(std::sqrt(rr)) && status != FG_ITER_OVERFLOW && status != FG_ITER_CANNOT_CONVERGE) {
+            if (nit > 0) {
+                rho = zr;  // z = D r and z . r came with the last update
+                k_es_dir<<<nb, ESB>>>(N, rho / rho_1, d_z, d_p);
+            }
+            k_es_spmv<<<nb, ESB>>>(N, d_rp, d_col, d_val, d_p, d_dir, 1, d_q, d_p, d_part);
+            ESCHK(hipMemcpy(hp.data(), d_part, nb * sizeof(double), hipMemcpyDeviceToHost));
+            const double qp = host_sum(std::vector<double>(hp.begin(), hp.begin() + nb));
+            if (qp == 0.0) {
+                status = FG_ITER_CANNOT_CONVERGE;
+                break;
+            }
+            const double a = rho / qp;
+            k_es_update<<<nb, ESB>>>(N, a, d_p, d_q, d_D, d_x, d_r, d_z, d_part, d_part + nb);
+            read2(rr, zr);
+            rho_1 = rho;
+            if (++nit >= max_iter) status = FG_ITER_OVERFLOW;
+        }
+        k_es_add<<<nb, ESB>>>(N, d_Vd, d_x);
+        ESCHK(hipGetLastError());
+        ESCHK(hipMemcpy(V_out, d_x, (size_t)N * sizeof(double), hipMemcpyDeviceToHost));
+        if (info) {
+            *info = fg_solve_info{};
+            info->status = status;
+            info->iterations = nit;
+            info->residual = res;
+            info->rhs_norm = rhsn;
+        }
+        return FG_OK;
+    } catch (const std::string &) {
+        return FG_ERR_DEVICE;
+    }
+}
+
+extern "C" int fg_spin_accumulation(const fg_mesh *m, const fg_spin_volume *vol, const fg_spin_surface *surf,
+                                    const double *Vn, const double *u, double tol, int32_t max_iter, int32_t device,
+                                    double *s_out, fg_solve_info *info)
+{
+    if (!m || !vol || !surf || !Vn || !u || !s_out || !m->pos || !m->tet_nodes || !m->tet_region || !m->vol ||
+        m->n_nodes <= 0 || m->n_tets <= 0 || (m->n_tris > 0 && (!m->tri_nodes || !m->tri_region)) || !(tol > 0) ||
+        max_iter < 1)
+        return FG_ERR_ARG;
+    // spinAcc::checkBoundaryConditions (src/spinAccumulationSolver.cpp:41-79)
+    auto finite3 = [](const double *v) { return std::isfinite(std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2])); };
+    int nJ = 0, nS = 0;
+    for (int k = 0; k < m->n_surf_regions; k++) {
+        nJ += std::isfinite(surf[k].jn);
+        nS += finite3(surf[k].s);
+    }
+    bool ok = nJ == 1 && nS == 1;
+    for (int k = 0; k < m->n_vol_regions; k++)
+        ok = ok && std::isfinite(vol[k].N0) && vol[k].N0 != 0 && std::isfinite(vol[k].P) && vol[k].P >= 0 && vol[k].P < 1;
+    if (!ok) return FG_ERR_ARG;
+    std::vector<int4> tn;
+    if (check_tets(m, tn) != FG_OK) return FG_ERR_MESH;
+    // physical constants (src/config.h.in:33-37)
+    const double E = 1.602176634e-19, HBAR = 6.62607015e-34 / (2 * M_PI), ME = 9.1093837139e-31;
+    const double MUB = E * HBAR / (2. * ME);
+    const int N = m->n_nodes, T = m->n_tets, n3 = 3 * N;
+    std::vector<double4> prm(T);
+    std::vector<uint8_t> mag(T);
+    for (int t = 0; t < T; t++) {
+        const int rg = m->tet_region[t];
+        const fg_spin_volume &p = vol[rg];
+        const double D0 = 2.0 * p.sigma / ((E * E) * p.N0);  // spinAcc::getDiffusionCst
+        prm[t] = make_double4(D0, D0 / (p.lsf * p.lsf), D0 / (p.lsd * p.lsd), MUB * p.P * p.sigma / E);
+        mag[t] = m->vol[rg].Ms > 0;  // mesh::isMagnetic
+    }
+    const NodePattern P = node_pattern(m);
+    if (P.rp[N] == 0) return FG_ERR_MESH;
+    // each node's (local position, tetrahedron) slots in mesh order
+    std::vector<int64_t> nptr(N + 1, 0), nslot((size_t)4 * T);
+    for (int64_t k = 0; k < 4 * (int64_t)T; k++) nptr[m->tet_nodes[k] + 1]++;
+    for (int i = 0; i < N; i++) nptr[i + 1] += nptr[i];
+    {
+        std::vector<int64_t> fill(nptr.begin(), nptr.end() - 1);
+        for (int t = 0; t < T; t++)
+            for (int l = 0; l < 4; l++) nslot[fill[m->tet_nodes[4 * (size_t)t + l]]++] = (int64_t)l * T + t;
+    }
+    // Dirichlet values (spinAcc::boundaryConditions, :91-116), in triangle order
+    std::vector<double> Vd(n3, 0.0);
+    std::vector<uint8_t> dir(N, 0);
+    for (int f = 0; f < m->n_tris; f++) {
+        const int rg = m->tri_region[f];
+        if (rg < 0 || rg >= m->n_surf_regions) continue;
+        const fg_spin_surface &p = surf[rg];
+        double dv[3];
+        bool d = false;
+        if (std::isnan(std::sqrt(p.s[0] * p.s[0] + p.s[1] * p.s[1] + p.s[2] * p.s[2])) && std::isfinite(p.jn)) {
+            for (int k = 0; k < 3; k++) dv[k] = -p.jn * (MUB / E) * p.uP[k];
+            d = true;
+        } else if (finite3(p.s) && std::isnan(p.jn)) {
+            for (int k = 0; k < 3; k++) dv[k] = p.s[k];
+            d = true;
+        }
+        if (d)
+            for (int i = 0; i < 3; i++) {
+                const int q = m->tri_nodes[3 * (size_t)f + i];
+                for (int k = 0; k < 3; k++) Vd[3 * (size_t)q + k] = dv[k];
+                dir[q] = 1;
+            }
+    }
+    try {
+        ESCHK(hipSetDevice(device));
+        DevBufs B;
+        const int64_t nnz = P.rp[N];
+        const int nb = (N + ESB - 1) / ESB, nb3 = (n3 + ESB - 1) / ESB;
+        int4 *d_tn = B.upload(tn);
+        double *d_pos = B.upload(std::vector<double>(m->pos, m->pos + 3 * (size_t)N));
+        double4 *d_prm = B.upload(prm);
+        uint8_t *d_mag = B.upload(mag), *d_dir = B.upload(dir);
+        double *d_V = B.upload(std::vector<double>(Vn, Vn + N));
+        double *d_u = B.upload(std::vector<double>(u, u + (size_t)n3));
+        double *d_ke = B.alloc<double>((size_t)40 * T);
+        int64_t *d_cptr = B.upload(P.cptr), *d_cslot = B.upload(P.cslot), *d_rp = B.upload(P.rp);
+        int64_t *d_nptr = B.upload(nptr), *d_nslot = B.upload(nslot);
+        int32_t *d_col = B.upload(P.col);
+        double *d_a = B.alloc<double>(nnz), *d_S = B.alloc<double>(n3), *d_L = B.alloc<double>(n3);
+        double *d_Vd = B.upload(Vd);
+        double *d_D = B.alloc<double>(n3), *d_x = B.alloc<double>(n3), *d_r = B.alloc<double>(n3),
+               *d_rt = B.alloc<double>(n3), *d_p = B.alloc<double>(n3), *d_ph = B.alloc<double>(n3),
+               *d_v = B.alloc<double>(n3), *d_s = B.alloc<double>(n3), *d_sh = B.alloc<double>(n3),
+               *d_t = B.alloc<double>(n3);
+        double *d_part = B.alloc<double>((size_t)2 * nb3);
+        ESCHK(hipMemset(d_x, 0, (size_t)n3 * sizeof(double)));
+        k_sa_element<<<(T + ESB - 1) / ESB, ESB>>>(T, d_tn, d_pos, d_prm, d_mag, d_V, d_u, d_ke);
+        ESCHK(hipGetLastError());
+        k_es_gather<<<(unsigned)((nnz + ESB - 1) / ESB), ESB>>>(nnz, d_cptr, d_cslot, d_ke, d_a);
+        k_sa_node<<<nb, ESB>>>(N, T, d_nptr, d_nslot, d_ke, d_S, d_L);
+        ESCHK(hipGetLastError());
+        // the surface terms after the volume ones (solver::buildVect order, :173-197)
+        std::vector<double> L(n3);
+        ESCHK(hipMemcpy(L.data(), d_L, (size_t)n3 * sizeof(double), hipMemcpyDeviceToHost));
+        for (int f = 0; f < m->n_tris; f++) {
+            const int rg = m->tri_region[f];
+            if (rg < 0 || rg >= m->n_surf_regions) continue;
+            const fg_spin_surface &p = surf[rg];
+            double sv[3] = {p.s[0], p.s[1], p.s[2]};
+            if (std::isfinite(p.jn) && finite3(p.uP))
+                for (int k = 0; k < 3; k++) sv[k] = -p.jn * (MUB / E) * p.uP[k];
+            if (!finite3(sv)) continue;
+            const double surfa = tri_area(m, f);
+            double Le[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+            for (int g = 0; g < NPI_TRI; g++) {
+                const double wg = 2.0 * surfa * TRI_PDS[g];
+                for (int i = 0; i < 3; i++) {
+                    const double aiw = wg * tri_a(i, g);
+                    for (int k = 0; k < 3; k++) Le[3 * k + i] -= sv[k] * aiw;
+                }
+            }
+            for (int i = 0; i < 3; i++)
+                for (int k = 0; k < 3; k++) L[3 * (size_t)m->tri_nodes[3 * (size_t)f + i] + k] += Le[3 * k + i];
+        }
+        ESCHK(hipMemcpy(d_L, L.data(), (size_t)n3 * sizeof(double), hipMemcpyHostToDevice));
+        k_sa_diag<<<nb3, ESB>>>(n3, d_rp, d_col, d_a, d_dir, d_D);
+        k_sa_spmv<<<nb3, ESB>>>(n3, d_rp, d_col, d_a, d_S, d_Vd, d_dir, 0, d_v, nullptr, nullptr, nullptr);
+        k_sa_init<<<nb3, ESB>>>(n3, d_L, d_v, d_dir, d_r, d_rt, d_p, d_part);
+        ESCHK(hipGetLastError());
+        std::vector<double> hp(2 * (size_t)nb3);
+        auto read = [&](int k, double *out) {
+            ESCHK(hipMemcpy(hp.data(), d_part, (size_t)k * nb3 * sizeof(double), hipMemcpyDeviceToHost));
+            for (int q = 0; q < k; q++) out[q] = host_sum(std::vector<double>(hp.begin() + (size_t)q * nb3,
+                                                                              hp.begin() + (size_t)(q + 1) * nb3));
+        };
+        double rr;
+        read(1, &rr);
+        double rtr = rr;  // rt = r
+        // iteration<double> (src/algebra/iter.h:119-166) and the 6-argument bicg_dir
+        const double rhsn = std::sqrt(rr);
+        double res = 0, rho_1 = 0, rho_2 = 0, alpha = 0, beta = 0, omega = 0;
+        int nit = 0, status = FG_ITER_UNDEFINED;
+        auto converged = [&](double nr) {
+            res = std::fabs(nr);
+            if (std::isnan(res)) {
+                status = FG_ITER_CANNOT_CONVERGE;
+                return false;
+            }
+            const bool cv = res <= rhsn * tol;
+            if (cv) status = FG_ITER_CONVERGED;
+            return cv;
+        };
+        while (!converged(std::sqrt(rr))) {
+            rho_1 = rtr;
+            if (nit > 0) {
+                if (rho_2 == 0 || omega == 0) {
+                    status = FG_ITER_CANNOT_CONVERGE;
+                    break;
+                }
+                beta = (rho_1 / rho_2) * (alpha / omega);
+            }
+            k_sa_p<<<nb3, ESB>>>(n3, nit == 0, omega, beta, d_v, d_r, d_D, d_p, d_ph);
+            k_sa_spmv<<<nb3, ESB>>>(n3, d_rp, d_col, d_a, d_S, d_ph, d_dir, 1, d_v, d_rt, d_part, nullptr);
+            double vrt;
+            read(1, &vrt);
+            alpha = rho_1 / vrt;
+            k_sa_s<<<nb3, ESB>>>(n3, alpha, d_r, d_v, d_D, d_s, d_sh, d_part);
+            double ss;
+            read(1, &ss);
+            if (converged(std::sqrt(ss))) {
+                k_sa_x1<<<nb3, ESB>>>(n3, alpha, d_ph, d_x);
+                break;
+            } else if (status == FG_ITER_OVERFLOW || status == FG_ITER_CANNOT_CONVERGE) {
+                break;
+            }
+            k_sa_spmv<<<nb3, ESB>>>(n3, d_rp, d_col, d_a, d_S, d_sh, d_dir, 1, d_t, d_s, d_part, d_part + nb3);
+            double tt2[2];
+            read(2, tt2);
+            omega = tt2[0] / tt2[1];
+            k_sa_x<<<nb3, ESB>>>(n3, alpha, omega, d_ph, d_sh, d_s, d_t, d_rt, d_x, d_r, d_part, d_part + nb3);
+            double rr2[2];
+            read(2, rr2);
+            rr = rr2[0];
+            rtr = rr2[1];
+            rho_2 = rho_1;
+            if (++nit >= max_iter) status = FG_ITER_OVERFLOW;
+        }
+        k_es_add<<<nb3, ESB>>>(n3, d_Vd, d_x);
+        ESCHK(hipGetLastError());
+        ESCHK(hipMemcpy(s_out, d_x, (size_t)n3 * sizeof(double), hipMemcpyDeviceToHost));
+        if (info) {
+            *info = fg_solve_info{};
+            info->status = status;
+            info->iterations = nit;
+            info->residual = res;
+            info->rhs_norm = rhsn;
+        }
+        return FG_OK;
+    } catch (const std::string &) {
+        return FG_ERR_DEVICE;
+    }
+}

@@ -48,6 +48,15 @@ class Mesh(C.Structure):
                 ("vol", C.POINTER(VolumeRegion)), ("surf", C.POINTER(SurfaceRegion))]
 
 
+class SpinVolume(C.Structure):
+    _fields_ = [("sigma", C.c_double), ("N0", C.c_double), ("P", C.c_double), ("lsd", C.c_double),
+                ("lsf", C.c_double)]
+
+
+class SpinSurface(C.Structure):
+    _fields_ = [("jn", C.c_double), ("uP", C.c_double * 3), ("s", C.c_double * 3)]
+
+
 class Options(C.Structure):
     _fields_ = [("tol", C.c_double), ("max_iter", C.c_int32), ("demag_method", C.c_int32),
                 ("fmm_order", C.c_int32), ("fmm_leaf", C.c_int32), ("precision", C.c_int32),
@@ -84,6 +93,8 @@ HIP_SYMBOLS = {
     "fg_prepare_elements_amp": (C.c_int, [C.c_void_p, C.c_double, C.c_double, C.c_double]),
     "fg_energy_amp": (C.c_int, [C.c_void_p, C.c_double, _d]),
     "fg_gauss_points": (C.c_int, [C.c_void_p, _d]),
+    "fg_spin_accumulation": (C.c_int, [C.c_void_p, C.POINTER(SpinVolume), C.POINTER(SpinSurface), _d, _d,
+                                       C.c_double, C.c_int32, C.c_int32, _d, C.POINTER(SolveInfo)]),
     "fg_electrostatic_potential": (C.c_int, [C.c_void_p, _d, _d, _d, C.c_double, C.c_int32, C.c_int32, _d,
                                              C.POINTER(SolveInfo)]),
     "fg_evolution": (C.c_int, [C.c_void_p]),

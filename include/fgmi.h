@@ -194,6 +194,19 @@ int fg_prepare_elements_amp(fg_ctx *ctx, double amp, double dt, double prefactor
  * term from amp * h at the Gauss points. FG_ERR_STATE without a space field. */
 int fg_energy_amp(fg_ctx *ctx, double amp, double E[4]);
 
+/* Spin-diffusion parameters of a volume region (Tetra::prm, src/tetra.h: sigma [S/m], N0 density
+ * of states [J^-1 m^-3], P polarization rate in [0, 1), lsd and lsf [m]) and of a surface region
+ * (Triangle::prm, src/triangle.h: jn [A/m^2] with the polarization direction uP, or the spin
+ * accumulation s [A/m] on the surface; NaN where the region sets none). */
+typedef struct fg_spin_volume {
+    double sigma, N0, P, lsd, lsf;
+} fg_spin_volume;
+typedef struct fg_spin_surface {
+    double jn;
+    double uP[3];
+    double s[3];
+} fg_spin_surface;
+
 /* The electrostatic potential of the current-injection problem on every node of a mesh
  * (electrostatSolver::checkBoundaryConditions, solve, src/electrostatSolver.cpp:10-28,96-136,
  * built by spinAcc::spinAcc, src/spinAccumulationSolver.cpp:21-30): conductivity sigma per volume
@@ -206,6 +219,21 @@ int fg_energy_amp(fg_ctx *ctx, double amp, double E[4]);
  * exits unless the status is FG_ITER_CONVERGED). Needs no context. */
 int fg_electrostatic_potential(const fg_mesh *mesh, const double *sigma, const double *jn, const double *V,
                                double tol, int32_t max_iter, int32_t device, double *V_out, fg_solve_info *info);
+/* The spin accumulation of the spin-diffusion problem (spinAcc::checkBoundaryConditions,
+ * boundaryConditions, solve, integrales, src/spinAccumulationSolver.cpp:41-287): the 3 n_nodes
+ * system of the diffusion (D0 = 2 sigma / (e^2 N0), spin-flip length lsf) and, in the tetrahedra
+ * whose region has Ms > 0, the s-d exchange with the node magnetization u[3 n_nodes] (length lsd)
+ * and the source of the potential V[n_nodes] (fg_electrostatic_potential); Dirichlet values on the
+ * surfaces with s, or with jn (s = -jn mu_B / e uP). algebra::bicg_dir (6-argument overload,
+ * src/algebra/bicg.h:82-158) from s = 0, tolerance tol, max_iter iterations, on HIP device `device`
+ * (made current). Returns FG_ERR_ARG unless exactly one surface region has a finite jn and one a
+ * finite s and every volume region a finite nonzero N0 and 0 <= P < 1, FG_ERR_MESH on an
+ * out-of-range index, FG_ERR_DEVICE on a HIP error, else FG_OK with s in s_out[3 n_nodes]
+ * (interleaved x, y, z per node) and the solver's status in *info (the reference zeroes s unless
+ * the status is FG_ITER_CONVERGED). Needs no context. */
+int fg_spin_accumulation(const fg_mesh *mesh, const fg_spin_volume *vol, const fg_spin_surface *surf, const double *V,
+                         const double *u, double tol, int32_t max_iter, int32_t device, double *s_out,
+                         fg_solve_info *info);
 /* NEXT -> CURRENT. */
 int fg_evolution(fg_ctx *ctx);
 /* acos(min over magnetic edges of u_i . u_j) on the NEXT state. */

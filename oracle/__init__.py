@@ -47,6 +47,8 @@ SYMBOLS = {
                              C.POINTER(SolveInfo)]),
     "orc_electrostatic_potential": (C.c_int, [C.POINTER(nv.Mesh), _d, _d, _d, C.c_double, C.c_int, _d,
                                               C.POINTER(SolveInfo)]),
+    "orc_spin_accumulation": (C.c_int, [C.POINTER(nv.Mesh), C.POINTER(nv.SpinVolume), C.POINTER(nv.SpinSurface),
+                                        _d, _d, C.c_double, C.c_int, _d, C.POINTER(SolveInfo)]),
     "orc_bicg_dir": (C.c_int, [C.c_int64, _i64, _i32, _d, _d, _d, C.c_int, _i32, C.c_double, C.c_int,
                                C.POINTER(SolveInfo)]),
     "orc_system_size": (None, [C.c_void_p, _i64, _i64]),
@@ -161,6 +163,22 @@ def electrostatic_potential(mesh, sigma, jn, V, tol=1e-8, max_iter=1000):
     if rc < 0:
         raise ValueError("incorrect boundary conditions for the potential solver")
     return out, info
+
+
+def spin_accumulation(mesh, vol, surf, V, u, tol=1e-8, max_iter=1000):
+    """The spin accumulation (n_nodes, 3) of an fg_mesh view: vol / surf are sequences of
+    nv.SpinVolume / nv.SpinSurface per region, V the node potential, u the node magnetization;
+    returns (s, SolveInfo). Raises ValueError on bad boundary conditions."""
+    va = (nv.SpinVolume * len(vol))(*vol)
+    sa = (nv.SpinSurface * len(surf))(*surf)
+    V = np.ascontiguousarray(V, dtype=np.float64)
+    u = np.ascontiguousarray(u, dtype=np.float64).reshape(-1)
+    out = np.zeros(3 * mesh.n_nodes)
+    info = SolveInfo()
+    rc = lib().orc_spin_accumulation(C.byref(mesh), va, sa, d(V), d(u), tol, max_iter, d(out), C.byref(info))
+    if rc < 0:
+        raise ValueError("incorrect boundary conditions for the spin diffusion solver")
+    return out.reshape(-1, 3), info
 
 
 class Problem:

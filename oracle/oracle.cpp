@@ -695,7 +695,7 @@ struct Iter {
 
 /* algebra::bicg_dir, 5-argument overload (src/algebra/bicg.h:162-234) */
 void bicg_dir(Iter &iter, const CSR &A, std::vector<double> &x, const std::vector<double> &rhs,
-              const std::vector<int> &ld)
+              const std::vector<int> &ld, const std::vector<double> *xd = nullptr)
 {
     double rho_1 = 0, rho_2 = 0, alpha = 0, beta = 0, omega = 0;
     const size_t DIM = x.size();
@@ -707,6 +707,10 @@ void bicg_dir(Iter &iter, const CSR &A, std::vector<double> &x, const std::vecto
         for (int64_t k = A.rowptr[i]; k < A.rowptr[i + 1]; k++)
             if (A.col[k] == (int)i) c = A.val[k];
         dp[i] = 1.0 / c;
+    }
+    if (xd) { /* the 6-argument overload (src/algebra/bicg.h:82-158): b = rhs - A xd */
+        spmv(A, *xd, v);
+        for (size_t i = 0; i < DIM; i++) b[i] -= v[i];
     }
     mask(ld, b);
     mask(ld, dp);
@@ -754,6 +758,8 @@ void bicg_dir(Iter &iter, const CSR &A, std::vector<double> &x, const std::vecto
         rho_2 = rho_1;
         iter.inc();
     }
+    if (xd)
+        for (size_t i = 0; i < DIM; i++) x[i] += (*xd)[i];
 }
 
 /* algebra::cg_dir (src/algebra/cg.h:67-121): conjugate gradient with the diagonal preconditioner,
@@ -985,6 +991,161 @@ int orc_electrostatic_potential(const fg_mesh *m, const double *sigma, const dou
     Iter iter(tol, max_iter);
     cg_dir(iter, A, X, L, Vd, ld);
     std::copy(X.begin(), X.end(), V);
+    if (info) {
+        info->status = iter.status;
+        info->iterations = iter.nit;
+        info->residual = iter.res;
+        info->rhs_norm = iter.rhsn;
+        info->v_max = 0;
+    }
+    return iter.status;
+}
+
+/* spinAcc::checkBoundaryConditions, boundaryConditions, solve and integrales
+   (src/spinAccumulationSolver.cpp:41-287): the spin accumulation s (3 per node, interleaved) of
+   the diffusion problem with the node magnetization u and potential V. Returns -1 on bad
+   boundary conditions, else the bicg_dir status. */
+int orc_spin_accumulation(const fg_mesh *m, const fg_spin_volume *vp, const fg_spin_surface *sp, const double *Vn,
+                          const double *u, double tol, int max_iter, double *s_out, orc_solve_info *info)
+{
+    const double E = 1.602176634e-19, HBAR = 6.62607015e-34 / (2 * M_PI), ME = 9.1093837139e-31;
+    const double MUB = E * HBAR / (2. * ME); /* src/config.h.in:33-37 */
+    auto finite3 = [](const double *v) { return std::isfinite(std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2])); };
+    int nJ = 0, nS = 0;
+    for (int k = 0; k < m->n_surf_regions; k++) {
+        nJ += std::isfinite(sp[k].jn);
+        nS += finite3(sp[k].s);
+    }
+    bool ok = nJ == 1 && nS == 1;
+    for (int k = 0; k < m->n_vol_regions; k++)
+        ok = ok && std::isfinite(vp[k].N0) && vp[k].N0 != 0 && std::isfinite(vp[k].P) && vp[k].P >= 0 && vp[k].P < 1;
+    if (!ok) return -1;
+    const int N = m->n_nodes, D = 3 * N;
+    std::vector<std::map<int, double>> rows(D);
+    std::vector<double> L(D, 0.0);
+    for (int t = 0; t < m->n_tets; t++) {
+        V3 P[4];
+        int ind[4];
+        for (int a = 0; a < 4; a++) {
+            ind[a] = m->tet_nodes[4 * t + a];
+            P[a] = {m->pos[3 * ind[a]], m->pos[3 * ind[a] + 1], m->pos[3 * ind[a] + 2]};
+        }
+        double da[4][3], w[TNPI];
+        if (tet_geometry(P, ind, da, w, nullptr)) return -1;
+        const int rg = m->tet_region[t];
+        const fg_spin_volume &pr = vp[rg];
+        const double D0 = 2.0 * pr.sigma / ((E * E) * pr.N0);
+        double aw[4], wsum = 0;
+        for (int g = 0; g < TNPI; g++) wsum += w[g];
+        for (int i = 0; i < 4; i++) {
+            aw[i] = 0;
+            for (int g = 0; g < TNPI; g++) aw[i] += ta[i][g] * w[g];
+        }
+        /* calcDiagBlock (src/tetra.cpp:133-140): da da^T (D0 sum w) + diag(D0 / lsf^2 a_w) */
+        double AE[12][12] = {};
+        const double c = D0 * wsum, itsf = D0 / (pr.lsf * pr.lsf);
+        for (int i = 0; i < 4; i++)
+            for (int j = 0; j < 4; j++) {
+                double v = (da[i][0] * da[j][0] + da[i][1] * da[j][1] + da[i][2] * da[j][2]) * c;
+                if (i == j) v += itsf * aw[i];
+                for (int d = 0; d < 3; d++) AE[4 * d + i][4 * d + j] += v;
+            }
+        double BE[12] = {};
+        if (m->vol[rg].Ms > 0) {
+            const double itsd = D0 / (pr.lsd * pr.lsd);
+            for (int i = 0; i < 4; i++) {
+                const double *ui = u + 3 * ind[i];
+                const double X = itsd * (aw[i] * ui[0]), Y = itsd * (aw[i] * ui[1]), Z = itsd * (aw[i] * ui[2]);
+                AE[4 + i][8 + i] += X;
+                AE[8 + i][4 + i] -= X;
+                AE[0 + i][8 + i] -= Y;
+                AE[8 + i][0 + i] += Y;
+                AE[0 + i][4 + i] += Z;
+                AE[4 + i][0 + i] -= Z;
+            }
+            /* the source cst0 grad V . da_i m_i (:262-286) */
+            const double cst0 = MUB * pr.P * pr.sigma / E;
+            double gV[3] = {0, 0, 0};
+            for (int i = 0; i < 4; i++)
+                for (int k = 0; k < 3; k++) gV[k] += Vn[ind[i]] * da[i][k];
+            for (int g = 0; g < TNPI; g++) {
+                const double cw = cst0 * w[g];
+                const double cg[3] = {cw * gV[0], cw * gV[1], cw * gV[2]};
+                for (int i = 0; i < 4; i++) {
+                    const double *ui = u + 3 * ind[i];
+                    const double tmp = cg[0] * da[i][0] + cg[1] * da[i][1] + cg[2] * da[i][2];
+                    BE[i] += tmp * ui[0];
+                    BE[4 + i] += tmp * ui[1];
+                    BE[8 + i] += tmp * ui[2];
+                }
+            }
+        }
+        /* solver::buildMat / buildVect (src/solver.h:110-143) */
+        for (int i = 0; i < 4; i++)
+            for (int j = 0; j < 4; j++)
+                for (int di = 0; di < 3; di++)
+                    for (int dj = 0; dj < 3; dj++) rows[3 * ind[i] + di][3 * ind[j] + dj] += AE[4 * di + i][4 * dj + j];
+        for (int i = 0; i < 4; i++)
+            for (int d = 0; d < 3; d++) L[3 * ind[i] + d] += BE[4 * d + i];
+    }
+    /* boundary terms and Dirichlet values (:91-116, :173-197) */
+    std::vector<double> Vd(D, 0.0);
+    std::vector<int> ld;
+    for (int f = 0; f < m->n_tris; f++) {
+        const int rg = m->tri_region[f];
+        const int *q = m->tri_nodes + 3 * f;
+        if (rg < 0 || rg >= m->n_surf_regions) continue;
+        const fg_spin_surface &pr = sp[rg];
+        V3 P[3];
+        for (int a = 0; a < 3; a++) P[a] = {m->pos[3 * q[a]], m->pos[3 * q[a] + 1], m->pos[3 * q[a] + 2]};
+        const double surf = 0.5 * norm(cross(P[1] - P[0], P[2] - P[0]));
+        double sv[3] = {pr.s[0], pr.s[1], pr.s[2]};
+        if (std::isfinite(pr.jn) && finite3(pr.uP))
+            for (int k = 0; k < 3; k++) sv[k] = -pr.jn * (MUB / E) * pr.uP[k];
+        double Le[9] = {};
+        if (finite3(sv))
+            for (int g = 0; g < FNPI; g++) {
+                const double wg = 2.0 * surf * fpds[g];
+                for (int i = 0; i < 3; i++) {
+                    const double aiw = wg * fa[i][g];
+                    for (int k = 0; k < 3; k++) Le[3 * k + i] -= sv[k] * aiw;
+                }
+            }
+        for (int i = 0; i < 3; i++)
+            for (int k = 0; k < 3; k++) L[3 * q[i] + k] += Le[3 * k + i];
+        double dv[3];
+        bool dirichlet = false;
+        if (std::isnan(std::sqrt(pr.s[0] * pr.s[0] + pr.s[1] * pr.s[1] + pr.s[2] * pr.s[2])) && std::isfinite(pr.jn)) {
+            for (int k = 0; k < 3; k++) dv[k] = -pr.jn * (MUB / E) * pr.uP[k];
+            dirichlet = true;
+        } else if (finite3(pr.s) && std::isnan(pr.jn)) {
+            for (int k = 0; k < 3; k++) dv[k] = pr.s[k];
+            dirichlet = true;
+        }
+        if (dirichlet)
+            for (int i = 0; i < 3; i++)
+                for (int k = 0; k < 3; k++) {
+                    Vd[3 * q[i] + k] = dv[k];
+                    ld.push_back(3 * q[i] + k);
+                }
+    }
+    std::sort(ld.begin(), ld.end());
+    ld.erase(std::unique(ld.begin(), ld.end()), ld.end());
+    std::vector<int64_t> rp(D + 1, 0);
+    std::vector<int32_t> cl;
+    std::vector<double> vl;
+    for (int i = 0; i < D; i++) {
+        for (const auto &e : rows[i]) {
+            cl.push_back(e.first);
+            vl.push_back(e.second);
+        }
+        rp[i + 1] = (int64_t)cl.size();
+    }
+    CSR A{D, rp.data(), cl.data(), vl.data()};
+    std::vector<double> X(D, 0.0);
+    Iter iter(tol, max_iter);
+    bicg_dir(iter, A, X, L, ld, &Vd);
+    std::copy(X.begin(), X.end(), s_out);
     if (info) {
         info->status = iter.status;
         info->iterations = iter.nit;

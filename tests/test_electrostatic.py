@@ -17,9 +17,10 @@ from feellgood_amd import native as nv
 from feellgood_amd.mesh import HostMesh
 
 
-def bar(jitter=0.0, cells=(8, 3, 3), lengths=(40.0, 10.0, 12.0), seed=3):
-    """A cuboid split at mid-length into volume regions 0 and 1, with surface region 0 on the
-    x = min face, 1 on the x = max face and 2 elsewhere. Returns (fg_mesh view, owner, pos)."""
+def bar(jitter=0.0, cells=(8, 3, 3), lengths=(40.0, 10.0, 12.0), seed=3, ms=(0.0, 0.0)):
+    """A cuboid split at mid-length into volume regions 0 and 1 (saturation magnetizations ms),
+    with surface region 0 on the x = min face, 1 on the x = max face and 2 elsewhere. Returns
+    (fg_mesh view, owner, pos)."""
     hm = HostMesh.cuboid(lengths, cells, jitter=jitter, seed=seed)
     hm.prepare({"volume": {}})
     a = hm.arrays()
@@ -35,6 +36,7 @@ def bar(jitter=0.0, cells=(8, 3, 3), lengths=(40.0, 10.0, 12.0), seed=3):
     freg[np.all(np.abs(tx - x1) < eps, axis=1)] = 1
     dms = np.zeros(len(tris))
     vol = (nv.VolumeRegion * 2)()
+    vol[0].Ms, vol[1].Ms = ms
     surf = (nv.SurfaceRegion * 3)()
     m = nv.Mesh()
     m.n_nodes, m.n_tets, m.n_tris = len(pos), len(tets), len(tris)
