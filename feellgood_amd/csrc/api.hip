@@ -147,6 +147,10 @@ void build_plan(Ctx &c, const fg_mesh *m)
     }
     const int Tm = (int)mreg.size();
     c.Tm = Tm;
+    c.h_mtets = mtets;
+    c.h_mreg = mreg;
+    c.h_reg_Ms.resize(std::max(1, m->n_vol_regions));
+    for (int k = 0; k < m->n_vol_regions; k++) c.h_reg_Ms[k] = m->vol[k].Ms;
     if (Tm == 0) throw std::string("no magnetic tetrahedron");
     std::sort(edges.begin(), edges.end());
     edges.erase(std::unique(edges.begin(), edges.end()), edges.end());
@@ -485,7 +489,7 @@ void free_all(Ctx &c)
                     c.scol, c.sval, c.tet_nodes, c.geo, c.tet_reg, c.regions, c.elem_out, c.inc_ptr, c.inc_idx,
                     c.trik_nodes, c.trik_geo, c.tri_nodes, c.tri_geo, c.tri_corr, c.cinc_ptr, c.cinc_idx, c.corr,
                     c.src_pos, c.src_q, c.src_pos_f, c.src_rank, c.d_inst_node_off, c.inst_bn2, c.d_node_inst, c.d_inst_src,
-                    c.hsp};
+                    c.hsp, c.hst};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     if (c.h_ctl) (void)hipHostFree(c.h_ctl);
@@ -815,6 +819,40 @@ int fg_set_space_field(fg_ctx *x, const double *h)
         }
         if (!c.hsp) c.hsp = dalloc<double>(soa.size());
         HIPCHK(hipMemcpyAsync(c.hsp, soa.data(), soa.size() * sizeof(double), hipMemcpyHostToDevice, c.stream));
+        HIPCHK(hipStreamSynchronize(c.stream));
+        return FG_OK;
+    });
+}
+
+int fg_set_spin_accumulation(fg_ctx *x, const double *s, const fg_spin_volume *vol)
+{
+    return guarded(x, [&]() {
+        Ctx &c = x->c;
+        if (!s) {
+            if (c.hst) HIPCHK(hipFree(c.hst));
+            c.hst = nullptr;
+            return FG_OK;
+        }
+        if (!vol) return FG_ERR_ARG;
+        // Tetra::calc_Hst (src/tetra.cpp:93-106) with the prefactor of spinAcc::prepareExtraField
+        // (src/spinAccumulationSolver.cpp:138-149): Hst_g = D0 / (lsd^2 gamma0 Ms) sum_i s_i a_ig
+        const double E = 1.602176634e-19;
+        std::vector<double> h((size_t)15 * c.Tm);
+        for (int t = 0; t < c.Tm; t++) {
+            const int rg = c.h_mreg[t];
+            const fg_spin_volume &p = vol[rg];
+            const double D0 = 2.0 * p.sigma / ((E * E) * p.N0);
+            const double pref = D0 / ((p.lsd * p.lsd) * GAMMA0 * c.h_reg_Ms[rg]);
+            const int32_t *q = &c.h_mtets[4 * (size_t)t];
+            for (int g = 0; g < NPI_TET; g++)
+                for (int k = 0; k < 3; k++) {
+                    double v = 0.0;
+                    for (int i = 0; i < 4; i++) v += s[3 * (size_t)q[i] + k] * tet_a(i, g);
+                    h[(size_t)(3 * g + k) * c.Tm + t] = pref * v;
+                }
+        }
+        if (!c.hst) c.hst = dalloc<double>(h.size());
+        HIPCHK(hipMemcpyAsync(c.hst, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice, c.stream));
         HIPCHK(hipStreamSynchronize(c.stream));
         return FG_OK;
     });

@@ -30,6 +30,10 @@ struct Ctx {
     int Tm = 0;        // magnetic tetrahedra
     std::vector<int32_t> h_tet_mesh;  // mesh index of each magnetic tetrahedron
     double *hsp = nullptr;            // space-dependent applied field [15][Tm] (fg_set_space_field)
+    double *hst = nullptr;            // spin-accumulation field Hst [15][Tm] (fg_set_spin_accumulation)
+    std::vector<int32_t> h_mtets;     // nodes of each magnetic tetrahedron, positive orientation
+    std::vector<int32_t> h_mreg;      // region of each magnetic tetrahedron
+    std::vector<double> h_reg_Ms;     // Ms per volume region
     int Fm = 0;        // charged magnetic triangles (magTri)
     int Fk = 0;        // triangles with surface anisotropy in the RHS
     int64_t S = 0;     // demag sources = 5 Tm + 4 Fm

@@ -98,7 +98,8 @@ __global__ __launch_bounds__(BLOCK) void k_element(int Tm, const int4 *__restric
                                                    const DevRegion *__restrict__ regs,
                                                    const NodeRec *__restrict__ cur, double Hx,
                                                    double Hy, double Hz, const double *__restrict__ hsp,
-                                                   double amp, double dt, double4 *__restrict__ out)
+                                                   double amp, const double *__restrict__ hst, double dt,
+                                                   double4 *__restrict__ out)
 {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= Tm) return;
@@ -172,16 +173,24 @@ __global__ __launch_bounds__(BLOCK) void k_element(int Tm, const int4 *__restric
             Han = Han + (-R.K3bis) * (b3.x * ex + b3.y * ey + b3.z * ez + sdt * tm);
             uH += -R.K3bis * dot(bu, b3);
         }
-        uH += dot(U, H0);
+        // the spin-accumulation field (Tet::integrales, src/tetra.cpp:248-255,296-302): in the
+        // effective field of alpha_eff, in the right-hand side, and its second-order term
+        // -(Hst . U) s_dt V; zero without one
+        const d3 Hs = hst ? mk(hst[(size_t)(3 * g) * Tm + t], hst[(size_t)(3 * g + 1) * Tm + t],
+                               hst[(size_t)(3 * g + 2) * Tm + t])
+                          : mk(0.0, 0.0, 0.0);
+        uH += dot(U, H0 + Hs);
         const double ae = alpha_eff(dt, R.alpha, uH);
         const d3 H = H0 + (Han + sdt * Hv);
         const double wg = w[g];
+        const double hsu = dot(Hs, U);
 #pragma unroll
         for (int a = 0; a < 4; a++) {
             const double aw = wg * tet_a(a, g);
             contrib[a] += tet_a(a, g) * (wg * ae);
             BE[a] = BE[a] - (wg * R.Abis) * gex[a];
-            BE[a] = BE[a] + aw * H;
+            BE[a] = BE[a] + aw * (H + Hs);
+            if (hst) BE[a] = BE[a] - ((aw * hsu) * (sdt * GAMMA0)) * V;
         }
     }
 #pragma unroll
@@ -1084,7 +1093,7 @@ void launch_element(Ctx &c, const double Hext[3], double dt, const double *hsp, 
     if (c.Tm > 0) {
         Prof pr(c, "element");
         k_element<<<grid1(c.Tm), BLOCK, 0, c.stream>>>(c.Tm, c.tet_nodes, c.geo, c.tet_reg, c.regions, cur, Hext[0],
-                                                       Hext[1], Hext[2], hsp, amp, dt, c.elem_out);
+                                                       Hext[1], Hext[2], hsp, amp, c.hst, dt, c.elem_out);
     }
     if (c.Fk > 0) {
         Prof pr(c, "tri_aniso");

@@ -111,6 +111,16 @@ class Engine:
         hh = np.ascontiguousarray(h, dtype=np.float64).reshape(-1)
         self._chk(self.lib.fg_set_space_field(self._h, nv.dptr(hh)), "fg_set_space_field")
 
+    def set_spin_accumulation(self, s, vol=None):
+        """The spin-accumulation field Hst of the element pass from s [n_nodes, 3] (A/m) and the
+        regions' nv.SpinVolume parameters, or None to remove it."""
+        if s is None:
+            self._chk(self.lib.fg_set_spin_accumulation(self._h, None, None), "fg_set_spin_accumulation")
+            return
+        ss = np.ascontiguousarray(s, dtype=np.float64).reshape(-1)
+        va = (nv.SpinVolume * len(vol))(*vol)
+        self._chk(self.lib.fg_set_spin_accumulation(self._h, nv.dptr(ss), va), "fg_set_spin_accumulation")
+
     def prepare_elements_amp(self, amp, dt, prefactor):
         self._chk(self.lib.fg_prepare_elements_amp(self._h, amp, dt, prefactor), "fg_prepare_elements_amp")
 

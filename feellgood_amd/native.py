@@ -90,6 +90,7 @@ HIP_SYMBOLS = {
     "fg_calc_demag": (C.c_int, [C.c_void_p]),
     "fg_energy": (C.c_int, [C.c_void_p, _d, _d]),
     "fg_set_space_field": (C.c_int, [C.c_void_p, _d]),
+    "fg_set_spin_accumulation": (C.c_int, [C.c_void_p, _d, C.POINTER(SpinVolume)]),
     "fg_prepare_elements_amp": (C.c_int, [C.c_void_p, C.c_double, C.c_double, C.c_double]),
     "fg_energy_amp": (C.c_int, [C.c_void_p, C.c_double, _d]),
     "fg_gauss_points": (C.c_int, [C.c_void_p, _d]),

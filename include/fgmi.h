@@ -207,6 +207,15 @@ typedef struct fg_spin_surface {
     double s[3];
 } fg_spin_surface;
 
+/* The spin-accumulation field of the element pass (spinAcc::prepareExtraField and Tetra::calc_Hst,
+ * src/spinAccumulationSolver.cpp:138-149, src/tetra.cpp:93-106): Hst = D0 / (lsd^2 gamma0 Ms)
+ * sum_i s_i a_ig at the Gauss points of every magnetic tetrahedron, from the node spin
+ * accumulation s[3 n_nodes] (fg_spin_accumulation) and the regions' spin parameters. Every later
+ * fg_prepare_elements / fg_prepare_elements_amp adds Hst to the effective field of alpha_eff and
+ * to the right-hand side with its second-order term (Tet::integrales, src/tetra.cpp:248-255,
+ * 296-302) until it is replaced or removed (s = NULL). */
+int fg_set_spin_accumulation(fg_ctx *ctx, const double *s, const fg_spin_volume *vol);
+
 /* The electrostatic potential of the current-injection problem on every node of a mesh
  * (electrostatSolver::checkBoundaryConditions, solve, src/electrostatSolver.cpp:10-28,96-136,
  * built by spinAcc::spinAcc, src/spinAccumulationSolver.cpp:21-30): conductivity sigma per volume

@@ -62,6 +62,7 @@ SYMBOLS = {
     "orc_get_corrections": (None, [C.c_void_p, _d, _d]),
     "orc_energy": (None, [C.c_void_p, _d, _d]),
     "orc_set_space_field": (None, [C.c_void_p, _d]),
+    "orc_set_spin_accumulation": (None, [C.c_void_p, _d, C.POINTER(nv.SpinVolume)]),
     "orc_prepare_elements_amp": (None, [C.c_void_p, C.c_double, C.c_double, C.c_double]),
     "orc_energy_amp": (None, [C.c_void_p, C.c_double, _d]),
     "orc_evolution": (None, [C.c_void_p]),
@@ -230,6 +231,16 @@ class Problem:
 
     def prepare_elements_amp(self, amp, dt, prefactor):
         lib().orc_prepare_elements_amp(self._h, amp, dt, prefactor)
+
+    def set_spin_accumulation(self, s, vol=None):
+        """The spin-accumulation field of the element pass from s [n_nodes, 3] and the regions'
+        nv.SpinVolume parameters, or None."""
+        if s is None:
+            lib().orc_set_spin_accumulation(self._h, None, None)
+            return
+        ss = np.ascontiguousarray(s, dtype=np.float64).reshape(-1)
+        va = (nv.SpinVolume * len(vol))(*vol)
+        lib().orc_set_spin_accumulation(self._h, d(ss), va)
 
     def solve(self, dt):
         info = SolveInfo()

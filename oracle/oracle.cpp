@@ -222,7 +222,7 @@ void tet_lumping(const double da[4][3], const double w[TNPI], const V3 u_nod[4],
 void tet_integrales(const double da[4][3], const double w[TNPI], const V3 u[4], const V3 v[4],
                     const double phi[4], const double phiv[4], const V3 ep[4], const V3 eq[4],
                     const fg_volume_region &prm, const V3 Hext[TNPI], double dt, double prefactor,
-                    double Kp[8][8], double Lp[8])
+                    double Kp[8][8], double Lp[8], const V3 *Hst = nullptr)
 {
     const double alpha = prm.alpha;
     const double Ms = prm.Ms;
@@ -284,7 +284,8 @@ void tet_integrales(const double da[4][3], const double w[TNPI], const V3 u[4], 
             uHeff[g] += -K3bis * dot(uk_u, uuu);
         }
     }
-    for (int g = 0; g < TNPI; g++) uHeff[g] += dot(U[g], Hd + Hext[g]); /* Hst = 0 */
+    /* Heff = Hd + Hext + Hst (src/tetra.cpp:248-255) */
+    for (int g = 0; g < TNPI; g++) uHeff[g] += dot(U[g], Hst ? (Hd + Hext[g]) + Hst[g] : Hd + Hext[g]);
     double aeff[TNPI];
     for (int g = 0; g < TNPI; g++) aeff[g] = alpha_eff(dt, alpha, uHeff[g]);
 
@@ -329,7 +330,12 @@ void tet_integrales(const double da[4][3], const double w[TNPI], const V3 u[4], 
             const double ai_w = wg * ta[i][g];
             V3 ex = da[i][0] * dU[0] + da[i][1] * dU[1] + da[i][2] * dU[2];
             BE[i] = BE[i] - (wg * Abis) * ex;
-            BE[i] = BE[i] + ai_w * H;
+            if (Hst) { /* the Hst terms of src/tetra.cpp:296-302 */
+                BE[i] = BE[i] + ai_w * (H + Hst[g]);
+                BE[i] = BE[i] - ((ai_w * dot(Hst[g], U[g])) * s_dt) * V[g];
+            } else {
+                BE[i] = BE[i] + ai_w * H;
+            }
         }
     }
     /* Lp = Perm * P * vec(BE) (src/tetra.cpp:306) */
@@ -384,6 +390,7 @@ struct orc_problem {
     std::vector<char> magNode;
     std::vector<int> magTet, magTri;
     std::vector<double> hsp; /* space-dependent applied field [n_tets][5][3] (orc_set_space_field) */
+    std::vector<double> hst; /* spin-accumulation field [n_tets][5][3] (orc_set_spin_accumulation) */
     std::vector<int> lvd;
     /* element matrices (src/element.h:61-65) */
     std::vector<double> Kp;  /* 64 per tet */
@@ -591,7 +598,12 @@ static void prepare_elements(orc_problem *p, const double Hext_[3], const double
                                   hsp[15 * (size_t)k + 3 * g + 2]}
                        : v3(Hext_);
         double Kp[8][8], Lp[8];
-        tet_integrales(t.da, t.w, u, v, phi, phiv, ep, eq, p->vol[t.reg], H, dt, prefactor, Kp, Lp);
+        V3 Hs[TNPI];
+        const bool st = !p->hst.empty();
+        if (st)
+            for (int g = 0; g < TNPI; g++)
+                Hs[g] = {p->hst[15 * (size_t)k + 3 * g], p->hst[15 * (size_t)k + 3 * g + 1], p->hst[15 * (size_t)k + 3 * g + 2]};
+        tet_integrales(t.da, t.w, u, v, phi, phiv, ep, eq, p->vol[t.reg], H, dt, prefactor, Kp, Lp, st ? Hs : nullptr);
         std::memcpy(&p->Kp[64 * k], Kp, sizeof(Kp));
         std::memcpy(&p->Lp[8 * k], Lp, sizeof(Lp));
     }
@@ -627,6 +639,32 @@ void orc_prepare_elements_amp(orc_problem *p, double amp, double dt, double pref
 {
     const double zero[3] = {0, 0, 0};
     prepare_elements(p, zero, p->hsp.empty() ? nullptr : p->hsp.data(), amp, dt, prefactor);
+}
+
+/* Tetra::calc_Hst with spinAcc::prepareExtraField's prefactor D0 / (lsd^2 gamma0 Ms)
+   (src/tetra.cpp:93-106, src/spinAccumulationSolver.cpp:138-149) from the node spin accumulation
+   s[3 n_nodes]; NULL removes it */
+void orc_set_spin_accumulation(orc_problem *p, const double *sacc, const fg_spin_volume *vp)
+{
+    if (!sacc) {
+        p->hst.clear();
+        return;
+    }
+    const double E = 1.602176634e-19;
+    p->hst.assign(15 * p->tet.size(), 0.0);
+    for (size_t k = 0; k < p->tet.size(); k++) {
+        const TetGeo &t = p->tet[k];
+        if (!p->isMagTet(t)) continue;
+        const fg_spin_volume &pr = vp[t.reg];
+        const double D0 = 2.0 * pr.sigma / ((E * E) * pr.N0);
+        const double pref = D0 / ((pr.lsd * pr.lsd) * GAMMA0 * p->vol[t.reg].Ms);
+        for (int g = 0; g < TNPI; g++)
+            for (int c = 0; c < 3; c++) {
+                double v = 0.0;
+                for (int i = 0; i < 4; i++) v += sacc[3 * (size_t)t.ind[i] + c] * ta[i][g];
+                p->hst[15 * k + 3 * g + c] = pref * v;
+            }
+    }
 }
 
 void orc_set_space_field(orc_problem *p, const double *h)

@@ -56,6 +56,8 @@ int orc_cg_dir(int64_t n, const int64_t *rowptr, const int32_t *col, const doubl
    jn and V per surface region (NaN: none). Returns -1 on bad boundary conditions, else the status. */
 int orc_electrostatic_potential(const fg_mesh *m, const double *sigma, const double *jn, const double *Vbc,
                                 double tol, int max_iter, double *V, orc_solve_info *info);
+/* The spin-accumulation field Hst of the element pass from s[3 n_nodes]; NULL removes it. */
+void orc_set_spin_accumulation(orc_problem *p, const double *s, const fg_spin_volume *vp);
 /* The spin accumulation s[3 n_nodes] (spinAcc::solve): returns -1 on bad boundary conditions,
    else the bicg_dir status. */
 int orc_spin_accumulation(const fg_mesh *m, const fg_spin_volume *vp, const fg_spin_surface *sp, const double *V,

@@ -204,6 +204,34 @@ def test_space_field_step_and_energy(small):
         g.prepare_elements_amp(0.7, dt, pref)
 
 
+def test_spin_field_step(small):
+    """The spin-accumulation field Hst of the element pass (src/tetra.cpp:93-106,248-255,296-302)
+    from a non-uniform node spin accumulation, with v != 0 so that its second-order term is active:
+    element pass and one solve, engine against oracle; removing it restores the plain system."""
+    u, v = state(small)
+    p, g = pair(small, u, v)
+    rng = np.random.default_rng(11)
+    sacc = rng.normal(size=(small.n_nodes, 3)) * 2e5 + [1e5, 0.0, -3e5]
+    vols = [nv.SpinVolume(5e6, 1e47, 0.4, 4e-9, 10e-9)] * small.mesh.n_vol_regions
+    H = np.array([1e4, 5e3, -2e4])
+    dt, pref = 2e-14, 1.046052
+    for o in (p, g):
+        o.set_spin_accumulation(sacc, vols)
+        o.base_projection(0.42)
+        o.prepare_elements(H, dt, pref)
+        o.solve(dt)
+    rp0, c0, v0, b0 = p.system()
+    rp1, c1, v1, b1 = g.system()
+    assert rel(v1, v0) < 1e-12 and rel(b1, b0) < 1e-12
+    un0, vn0, _, _ = p.get_state(nv.FG_NEXT)
+    un1, vn1, _, _ = g.get_state(nv.FG_NEXT)
+    assert rel(vn1, vn0) < 1e-8 and np.abs(un1 - un0).max() <= 1e-12
+    g.set_spin_accumulation(None)
+    g.prepare_elements(H, dt, pref)
+    g.solve(dt)
+    assert rel(g.system()[3], b0) > 1e-9  # the field changed the right-hand side
+
+
 def test_energy_average_max_angle(small):
     u, v = state(small)
     p, g = pair(small, u, v)

@@ -580,3 +580,43 @@ def test_basis_angles_reproduce_rand_mt19937():
     assert np.all((a >= 0) & (a < 2 / math.pi))
     assert np.array_equal(a, oracle.basis_angles(2, 4))
     assert not np.array_equal(a, oracle.basis_angles(3, 4))
+
+
+def test_spin_field_reduces_to_an_applied_field():
+    """The spin-accumulation field of the element pass (src/tetra.cpp:93-106,248-255,296-302):
+    with a uniform spin accumulation c, Hst = D0 / (lsd^2 gamma0 Ms) c at every Gauss point, and
+    with v = 0 the system is the one of the applied field Hext + Hst (the second-order term
+    vanishes); with v != 0 the right-hand side differs from it by a term linear in v."""
+    from feellgood_amd import native as nv
+    m = small_cube()
+    u = random_state(m.n_nodes)
+    H = np.array([2e4, -1e4, 5e3])
+    c = np.array([3e5, 1e5, -2e5])
+    sv = nv.SpinVolume(5e6, 1e47, 0.4, 5e-9, 10e-9)
+    vols = [sv] * m.mesh.n_vol_regions
+    E = 1.602176634e-19
+    gamma0 = 1.76085962784e11 * 1.25663706127e-6
+    hst = 2 * sv.sigma / (E**2 * sv.N0) / (sv.lsd**2 * gamma0 * m.mesh.vol[0].Ms) * c
+    dt, pref = 1e-13, 1.046052
+    rng = np.random.default_rng(2)
+    vrand = np.cross(u, rng.normal(size=u.shape)) * 1e9
+
+    def system(v, spin, Hext):
+        p = oracle.Problem(m.mesh, tol=1e-12, max_iter=2000)
+        p.set_state(2, u=u, v=v)
+        p.calc_demag()
+        p.evolution()
+        p.base_projection(0.3)
+        if spin:
+            p.set_spin_accumulation(np.tile(c, (m.n_nodes, 1)), vols)
+        p.prepare_elements(Hext, dt, pref)
+        p.solve(dt)  # assembles the system
+        return p.system()[2:]
+
+    v0 = np.zeros_like(u)
+    (a1, b1), (a0, b0) = system(v0, True, H), system(v0, False, H + hst)
+    assert np.allclose(a1, a0, rtol=1e-12, atol=1e-12 * np.abs(a0).max())
+    assert np.allclose(b1, b0, rtol=1e-12, atol=1e-12 * np.abs(b0).max())
+    d = [system(k * vrand, True, H)[1] - system(k * vrand, False, H + hst)[1] for k in (1.0, 2.0)]
+    assert np.abs(d[0]).max() > 1e-6 * np.abs(b0).max()
+    assert np.allclose(d[1], 2 * d[0], rtol=1e-6, atol=1e-9 * np.abs(d[0]).max())
