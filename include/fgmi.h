@@ -21,6 +21,12 @@
  *   fg_evolution          Fem::evolution (src/fem.h:159-163)
  *   fg_max_angle          Mesh::mesh::max_angle (src/mesh.h:295-306)
  *   fg_average            Mesh::mesh::avg (src/mesh.cpp:89-106)
+ *   fg_set_space_field    Mesh::mesh::setExtSpaceField (src/mesh.cpp:377-389), with
+ *                         fg_prepare_elements_amp / fg_energy_amp for the R4toR3 field
+ *   fg_electrostatic_potential  electrostatSolver (src/electrostatSolver.cpp:10-136)
+ *   fg_spin_accumulation  spinAcc::solve (src/spinAccumulationSolver.cpp:151-287)
+ *   fg_set_spin_accumulation    spinAcc::prepareExtraField + Tetra::calc_Hst
+ *                         (src/spinAccumulationSolver.cpp:138-149, src/tetra.cpp:93-106)
  *
  * Threading: one context per host thread; calls on one context are serialised on the
  * context's HIP stream.
