@@ -147,8 +147,6 @@ void build_plan(Ctx &c, const fg_mesh *m)
     }
     const int Tm = (int)mreg.size();
     c.Tm = Tm;
-    c.h_mtets = mtets;
-    c.h_mreg = mreg;
     c.h_reg_Ms.resize(std::max(1, m->n_vol_regions));
     for (int k = 0; k < m->n_vol_regions; k++) c.h_reg_Ms[k] = m->vol[k].Ms;
     if (Tm == 0) throw std::string("no magnetic tetrahedron");
@@ -489,7 +487,7 @@ void free_all(Ctx &c)
                     c.scol, c.sval, c.tet_nodes, c.geo, c.tet_reg, c.regions, c.elem_out, c.inc_ptr, c.inc_idx,
                     c.trik_nodes, c.trik_geo, c.tri_nodes, c.tri_geo, c.tri_corr, c.cinc_ptr, c.cinc_idx, c.corr,
                     c.src_pos, c.src_q, c.src_pos_f, c.src_rank, c.d_inst_node_off, c.inst_bn2, c.d_node_inst, c.d_inst_src,
-                    c.hsp, c.hst};
+                    c.hsp, c.hst, c.spin_s, c.spin_pref};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     if (c.h_ctl) (void)hipHostFree(c.h_ctl);
@@ -829,30 +827,31 @@ int fg_set_spin_accumulation(fg_ctx *x, const double *s, const fg_spin_volume *v
     return guarded(x, [&]() {
         Ctx &c = x->c;
         if (!s) {
-            if (c.hst) HIPCHK(hipFree(c.hst));
-            c.hst = nullptr;
+            for (double **p : {&c.hst, &c.spin_s, &c.spin_pref}) {
+                if (*p) HIPCHK(hipFree(*p));
+                *p = nullptr;
+            }
             return FG_OK;
         }
         if (!vol) return FG_ERR_ARG;
-        // Tetra::calc_Hst (src/tetra.cpp:93-106) with the prefactor of spinAcc::prepareExtraField
-        // (src/spinAccumulationSolver.cpp:138-149): Hst_g = D0 / (lsd^2 gamma0 Ms) sum_i s_i a_ig
+        // the prefactor of spinAcc::prepareExtraField (src/spinAccumulationSolver.cpp:138-149),
+        // D0 / (lsd^2 gamma0 Ms) per magnetic region; Hst itself on the device
         const double E = 1.602176634e-19;
-        std::vector<double> h((size_t)15 * c.Tm);
-        for (int t = 0; t < c.Tm; t++) {
-            const int rg = c.h_mreg[t];
-            const fg_spin_volume &p = vol[rg];
-            const double D0 = 2.0 * p.sigma / ((E * E) * p.N0);
-            const double pref = D0 / ((p.lsd * p.lsd) * GAMMA0 * c.h_reg_Ms[rg]);
-            const int32_t *q = &c.h_mtets[4 * (size_t)t];
-            for (int g = 0; g < NPI_TET; g++)
-                for (int k = 0; k < 3; k++) {
-                    double v = 0.0;
-                    for (int i = 0; i < 4; i++) v += s[3 * (size_t)q[i] + k] * tet_a(i, g);
-                    h[(size_t)(3 * g + k) * c.Tm + t] = pref * v;
-                }
+        std::vector<double> pref(c.h_reg_Ms.size(), 0.0);
+        for (size_t rg = 0; rg < pref.size(); rg++)
+            if (c.h_reg_Ms[rg] > 0) {
+                const fg_spin_volume &p = vol[rg];
+                const double D0 = 2.0 * p.sigma / ((E * E) * p.N0);
+                pref[rg] = D0 / ((p.lsd * p.lsd) * GAMMA0 * c.h_reg_Ms[rg]);
+            }
+        if (!c.hst) {
+            c.hst = dalloc<double>((size_t)15 * c.Tm);
+            c.spin_s = dalloc<double>((size_t)3 * c.N);
+            c.spin_pref = dalloc<double>(pref.size());
         }
-        if (!c.hst) c.hst = dalloc<double>(h.size());
-        HIPCHK(hipMemcpyAsync(c.hst, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice, c.stream));
+        HIPCHK(hipMemcpyAsync(c.spin_s, s, (size_t)3 * c.N * sizeof(double), hipMemcpyHostToDevice, c.stream));
+        HIPCHK(hipMemcpyAsync(c.spin_pref, pref.data(), pref.size() * sizeof(double), hipMemcpyHostToDevice, c.stream));
+        launch_spin_field(c);
         HIPCHK(hipStreamSynchronize(c.stream));
         return FG_OK;
     });

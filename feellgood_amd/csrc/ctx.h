@@ -31,8 +31,8 @@ struct Ctx {
     std::vector<int32_t> h_tet_mesh;  // mesh index of each magnetic tetrahedron
     double *hsp = nullptr;            // space-dependent applied field [15][Tm] (fg_set_space_field)
     double *hst = nullptr;            // spin-accumulation field Hst [15][Tm] (fg_set_spin_accumulation)
-    std::vector<int32_t> h_mtets;     // nodes of each magnetic tetrahedron, positive orientation
-    std::vector<int32_t> h_mreg;      // region of each magnetic tetrahedron
+    double *spin_s = nullptr;         // node spin accumulation [3 N] and Hst prefactor per region
+    double *spin_pref = nullptr;
     std::vector<double> h_reg_Ms;     // Ms per volume region
     int Fm = 0;        // charged magnetic triangles (magTri)
     int Fk = 0;        // triangles with surface anisotropy in the RHS
@@ -164,6 +164,8 @@ void launch_spmv_debug(Ctx &c, double c_stencil, const double2 *xin, double2 *yo
 void launch_charges(Ctx &c);
 void launch_direct_p2p(Ctx &c);
 void launch_energy(Ctx &c, const double Hext[3], const double *hsp = nullptr, double amp = 0);
+// c.hst from the node spin accumulation c.spin_s and the per-region prefactors c.spin_pref
+void launch_spin_field(Ctx &c);
 void launch_max_angle(Ctx &c);
 void launch_average(Ctx &c, int which);
 void launch_evolution(Ctx &c);

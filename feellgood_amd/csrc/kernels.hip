@@ -1087,6 +1087,39 @@ __global__ __launch_bounds__(BLOCK) void k_evolution(int N, const NodeRec *__res
 // ------------------------------------------------------------------ launchers
 static inline int grid1(int64_t n) { return (int)((n + BLOCK - 1) / BLOCK); }
 
+// Tetra::calc_Hst (src/tetra.cpp:93-106): Hst_g = pref_region sum_i s_i a_ig at the Gauss points of
+// each magnetic tetrahedron, [15][Tm]
+__global__ __launch_bounds__(BLOCK) void k_spin_field(int Tm, const int4 *__restrict__ tn, const int32_t *__restrict__ treg,
+                                                      const double *__restrict__ pref, const double *__restrict__ s,
+                                                      double *__restrict__ hst)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= Tm) return;
+    const int4 q = tn[t];
+    const int nd[4] = {q.x, q.y, q.z, q.w};
+    const double p = pref[treg[t]];
+    double sn[4][3];
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int k = 0; k < 3; k++) sn[i][k] = s[3 * (size_t)nd[i] + k];
+#pragma unroll
+    for (int g = 0; g < 5; g++)
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            double v = 0.0;
+#pragma unroll
+            for (int i = 0; i < 4; i++) v += sn[i][k] * tet_a(i, g);
+            hst[(size_t)(3 * g + k) * Tm + t] = p * v;
+        }
+}
+
+void launch_spin_field(Ctx &c)
+{
+    if (c.Tm > 0)
+        k_spin_field<<<grid1(c.Tm), BLOCK, 0, c.stream>>>(c.Tm, c.tet_nodes, c.tet_reg, c.spin_pref, c.spin_s, c.hst);
+}
+
 void launch_element(Ctx &c, const double Hext[3], double dt, const double *hsp, double amp)
 {
     const NodeRec *cur = c.rec[c.cur];
