@@ -271,12 +271,12 @@ double host_sum(const std::vector<double> &h)
 // Per tetrahedron (spinAcc::integrales, src/spinAccumulationSolver.cpp:216-287), at the caller's
 // local positions: rows 0-15 the diffusion block A_ij = (D0 sum w) da_i . da_j + [i == j] D0 / lsf^2
 // a_w,i (Tet::calcDiagBlock, src/tetra.cpp:133-140); rows 16-27 the s-d exchange of node i,
-// D0 / lsd^2 a_w,i u_i (X, Y, Z); rows 28-39 the source cst0 sum_g w_g (grad V . da_i) u_i; the
+// D0 / lsd^2 a_w,i u_i (X, Y, Z); rows 28-39 the source cst0 sum_g w_g (grad V . da_i) usrc_i; the
 // last two zero outside magnetic regions. prm = (D0, D0 / lsf^2, D0 / lsd^2, cst0).
 __global__ __launch_bounds__(ESB) void k_sa_element(int T, const int4 *__restrict__ tn, const double *__restrict__ pos,
                                                     const double4 *__restrict__ prm, const uint8_t *__restrict__ mag,
                                                     const double *__restrict__ V, const double *__restrict__ u,
-                                                    double *__restrict__ ke)
+                                                    const double *__restrict__ usrc, double *__restrict__ ke)
 {
     const int t = blockIdx.x * ESB + threadIdx.x;
     if (t >= T) return;
@@ -317,6 +317,7 @@ __global__ __launch_bounds__(ESB) void k_sa_element(int T, const int4 *__restric
 #pragma unroll
     for (int i = 0; i < 4; i++) {
         const double ui[3] = {u[3 * (size_t)id[i]], u[3 * (size_t)id[i] + 1], u[3 * (size_t)id[i] + 2]};
+        const double us[3] = {usrc[3 * (size_t)id[i]], usrc[3 * (size_t)id[i] + 1], usrc[3 * (size_t)id[i] + 2]};
         double be[3] = {0.0, 0.0, 0.0};
         if (mg)
 #pragma unroll
@@ -324,7 +325,7 @@ __global__ __launch_bounds__(ESB) void k_sa_element(int T, const int4 *__restric
                 const double cw = pr.w * w[g];
                 const double tmp = (cw * gV[0]) * da[i][0] + (cw * gV[1]) * da[i][1] + (cw * gV[2]) * da[i][2];
 #pragma unroll
-                for (int k = 0; k < 3; k++) be[k] += tmp * ui[k];
+                for (int k = 0; k < 3; k++) be[k] += tmp * us[k];
             }
 #pragma unroll
         for (int k = 0; k < 3; k++) {
@@ -692,8 +693,8 @@ extern "C" int fg_electrostatic_potential(const fg_mesh *m, const double *sigma,
 }
 
 extern "C" int fg_spin_accumulation(const fg_mesh *m, const fg_spin_volume *vol, const fg_spin_surface *surf,
-                                    const double *Vn, const double *u, double tol, int32_t max_iter, int32_t device,
-                                    double *s_out, fg_solve_info *info)
+                                    const double *Vn, const double *u, const double *u_src, double tol,
+                                    int32_t max_iter, int32_t device, double *s_out, fg_solve_info *info)
 {
     if (!m || !vol || !surf || !Vn || !u || !s_out || !m->pos || !m->tet_nodes || !m->tet_region || !m->vol ||
         m->n_nodes <= 0 || m->n_tets <= 0 || (m->n_tris > 0 && (!m->tri_nodes || !m->tri_region)) || !(tol > 0) ||
@@ -770,6 +771,7 @@ extern "C" int fg_spin_accumulation(const fg_mesh *m, const fg_spin_volume *vol,
         uint8_t *d_mag = B.upload(mag), *d_dir = B.upload(dir);
         double *d_V = B.upload(std::vector<double>(Vn, Vn + N));
         double *d_u = B.upload(std::vector<double>(u, u + (size_t)n3));
+        double *d_us = u_src ? B.upload(std::vector<double>(u_src, u_src + (size_t)n3)) : d_u;
         double *d_ke = B.alloc<double>((size_t)40 * T);
         int64_t *d_cptr = B.upload(P.cptr), *d_cslot = B.upload(P.cslot), *d_rp = B.upload(P.rp);
         int64_t *d_nptr = B.upload(nptr), *d_nslot = B.upload(nslot);
@@ -782,7 +784,7 @@ extern "C" int fg_spin_accumulation(const fg_mesh *m, const fg_spin_volume *vol,
                *d_t = B.alloc<double>(n3);
         double *d_part = B.alloc<double>((size_t)2 * nb3);
         ESCHK(hipMemset(d_x, 0, (size_t)n3 * sizeof(double)));
-        k_sa_element<<<(T + ESB - 1) / ESB, ESB>>>(T, d_tn, d_pos, d_prm, d_mag, d_V, d_u, d_ke);
+        k_sa_element<<<(T + ESB - 1) / ESB, ESB>>>(T, d_tn, d_pos, d_prm, d_mag, d_V, d_u, d_us, d_ke);
         ESCHK(hipGetLastError());
         k_es_gather<<<(unsigned)((nnz + ESB - 1) / ESB), ESB>>>(nnz, d_cptr, d_cslot, d_ke, d_a);
         k_sa_node<<<nb, ESB>>>(N, T, d_nptr, d_nslot, d_ke, d_S, d_L);

@@ -8,9 +8,10 @@
 // initial_magnetization and Bext take the reference's three expressions (of x, y, z in metres and
 // of t in seconds, src/settings.cpp:454-543), or a function definition whose body returns them,
 // and Bext as space and time functions (H = time(t) / mu0 * space(x) at the Gauss points), through
-// fgh::Expr. Not supported, reported as errors: function bodies beyond one return statement,
-// functions of the regions, region-qualified evol columns, Thiele_length, recentering and spin
-// accumulation.
+// fgh::Expr. Spin accumulation (spin_accumulation.enable) solves the potential once and the spin
+// accumulation at every compute_all, with its field in the element pass and its columns in the
+// .sol files. Not supported, reported as errors: function bodies beyond one return statement,
+// functions of the regions, region-qualified evol columns, Thiele_length and recentering.
 #include <unistd.h>
 
 #include <algorithm>
@@ -24,6 +25,7 @@
 #include <cstring>
 #include <ctime>
 #include <fstream>
+#include <iomanip>
 #include <iostream>
 #include <random>
 #include <sstream>
@@ -56,6 +58,12 @@ struct Config {
     std::vector<std::string> vol_names, surf_names;
     std::vector<fg_volume_region> vol;
     std::vector<fg_surface_region> surf;
+    // spin accumulation (src/settings.cpp:414-418,438-449,545-550; defaults of
+    // src/default-settings.yml: zero volume parameters, null surface ones)
+    bool spin_acc = false, V_file = false;
+    std::vector<fg_spin_volume> spin_vol;
+    std::vector<fg_spin_surface> spin_surf;
+    std::vector<double> surf_V;
     std::string init_sol;
     double m0[3] = {0, 0, 1};
     std::vector<fgh::Expr> m0_expr;  // three expressions of x, y, z (empty: m0 or init_sol)
@@ -235,8 +243,14 @@ Config read_config(const Node &root)
             const std::string w = "mesh.volume_regions." + f.first;
             check_map(&f.second, w);
             fg_volume_region r = default_volume();
+            fg_spin_volume sv{0.0, 0.0, 0.0, 0.0, 0.0};
             const Node &p = f.second;
             if (p.kind == Node::Map) {
+                if (auto n = p.get("sigma")) sv.sigma = fgh::as_double(*n, w + ".sigma");
+                if (auto n = p.get("dens_state")) sv.N0 = fgh::as_double(*n, w + ".dens_state");
+                if (auto n = p.get("P")) sv.P = fgh::as_double(*n, w + ".P");
+                if (auto n = p.get("l_sd")) sv.lsd = fgh::as_double(*n, w + ".l_sd");
+                if (auto n = p.get("l_sf")) sv.lsf = fgh::as_double(*n, w + ".l_sf");
                 if (auto n = p.get("Ae")) r.Ae = fgh::as_double(*n, w + ".Ae");
                 if (auto n = p.get("Ms")) r.Ms = fgh::as_double(*n, w + ".Ms");
                 if (auto n = p.get("K")) r.K = fgh::as_double(*n, w + ".K");
@@ -253,6 +267,7 @@ Config read_config(const Node &root)
             normalize(r.ez, w + ".ez");
             c.vol_names.push_back(f.first);
             c.vol.push_back(r);
+            c.spin_vol.push_back(sv);
         }
     if (c.vol.empty()) throw SettingsError("mesh.volume_regions should name at least one region");
     const Node *sr = m->get("surface_regions");
@@ -263,8 +278,20 @@ Config read_config(const Node &root)
             check_map(&f.second, w);
             fg_surface_region r{};
             r.uk[2] = 1;
+            fg_spin_surface ss{NAN, {NAN, NAN, NAN}, {NAN, NAN, NAN}};
+            double V = NAN;
             const Node &p = f.second;
             if (p.kind == Node::Map) {
+                // jn and V may be null, read as NaN (src/settings.cpp:438-449)
+                if (auto n = p.get("jn"); n && !n->is_null()) ss.jn = fgh::as_double(*n, w + ".jn");
+                if (auto n = p.get("V"); n && !n->is_null()) V = fgh::as_double(*n, w + ".V");
+                if (!std::isnan(ss.jn) && !std::isnan(V))
+                    throw SettingsError(w + ": a surface region cannot have both no-null J and V");
+                if (auto n = p.get("uP"); n && !n->is_null()) {
+                    fgh::as_vec3(*n, ss.uP, w + ".uP");
+                    normalize(ss.uP, w + ".uP");
+                }
+                if (auto n = p.get("s"); n && !n->is_null()) fgh::as_vec3(*n, ss.s, w + ".s");
                 if (auto n = p.get("Ks")) r.Ks = fgh::as_double(*n, w + ".Ks");
                 if (auto n = p.get("uk")) fgh::as_vec3(*n, r.uk, w + ".uk");
                 if (auto n = p.get("suppress_charges")) r.suppress_charges = fgh::as_bool(*n, w + ".suppress_charges");
@@ -272,6 +299,8 @@ Config read_config(const Node &root)
             normalize(r.uk, w + ".uk");
             c.surf_names.push_back(f.first);
             c.surf.push_back(r);
+            c.spin_surf.push_back(ss);
+            c.surf_V.push_back(V);
         }
     if (auto n = root.get("initial_magnetization"); n && !n->is_null()) {
         if (n->kind == Node::Scalar) {
@@ -290,6 +319,11 @@ Config read_config(const Node &root)
         c.has_t0 = true;
         c.t0 = fgh::as_double(*n, "initial_time");
     }
+    if (auto n = root.get("spin_accumulation"); n && !n->is_null()) {
+        check_map(n, "spin_accumulation");
+        if (auto b = n->get("enable")) c.spin_acc = fgh::as_bool(*b, "spin_accumulation.enable");
+        if (auto b = n->get("V_file")) c.V_file = fgh::as_bool(*b, "spin_accumulation.V_file");
+    }
     if (auto n = root.get("Bext"); n && !n->is_null()) {
         if (n->kind == Node::Scalar) {
             if (!compile_function(n->text, 1, "Bext", c.Bext, c.B_expr, c.B_text)) c.B_expr.clear();
@@ -307,9 +341,6 @@ Config read_config(const Node &root)
     if (auto r = root.get("recentering"); r && r->kind == Node::Map)
         if (auto n = r->get("enable"); n && !n->is_null() && fgh::as_bool(*n, "recentering.enable"))
             throw SettingsError("recentering is not supported");
-    if (auto r = root.get("spin_accumulation"); r && r->kind == Node::Map)
-        if (auto n = r->get("enable"); n && !n->is_null() && fgh::as_bool(*n, "spin_accumulation.enable"))
-            throw SettingsError("spin_accumulation is not supported");
     if (auto s = root.get("finite_element_solver"); s && s->kind == Node::Map) {
         if (auto n = s->get("max(iter)"); n && !n->is_null()) c.max_iter = fgh::as_long(*n, "finite_element_solver.max(iter)");
         if (auto n = s->get("tolerance"); n && !n->is_null()) c.tol = fgh::as_double(*n, "finite_element_solver.tolerance");
@@ -431,6 +462,18 @@ int run(const Config &c, unsigned seed, bool verbose, bool check_only)
         for (size_t k = 0; k < c.surf.size(); k++)
             std::cout << "surface region:    " << c.surf_names[k] << " Ks " << c.surf[k].Ks << " suppress_charges "
                       << c.surf[k].suppress_charges << "\n";
+        if (c.spin_acc) {
+            std::cout << "spin accumulation: V_file " << c.V_file << "\n";
+            for (size_t k = 0; k < c.vol.size(); k++)
+                std::cout << "spin volume:       " << c.vol_names[k] << " sigma " << c.spin_vol[k].sigma << " N0 "
+                          << c.spin_vol[k].N0 << " P " << c.spin_vol[k].P << " lsd " << c.spin_vol[k].lsd << " lsf "
+                          << c.spin_vol[k].lsf << "\n";
+            for (size_t k = 0; k < c.surf.size(); k++)
+                std::cout << "spin surface:      " << c.surf_names[k] << " jn " << c.spin_surf[k].jn << " V "
+                          << c.surf_V[k] << " uP " << c.spin_surf[k].uP[0] << " " << c.spin_surf[k].uP[1] << " "
+                          << c.spin_surf[k].uP[2] << " s " << c.spin_surf[k].s[0] << " " << c.spin_surf[k].s[1]
+                          << " " << c.spin_surf[k].s[2] << "\n";
+        }
         if (!c.Bs_expr.empty())
             std::cout << "Bext(x,t):         space " << c.Bs_text[0] << " | " << c.Bs_text[1] << " | " << c.Bs_text[2]
                       << " time " << c.Bt_text[0] << "\n";
@@ -531,8 +574,60 @@ int run(const Config &c, unsigned seed, bool verbose, bool check_only)
     set_dt(std::sqrt(DTMIN * DTMAX));
     double E[4] = {0, 0, 0, 0};
     double vmax = 0;
+    // spin accumulation: the potential once, then s from the magnetization at every compute_all
+    // (spinAcc::spinAcc and compute, src/spinAccumulationSolver.cpp:8-36,151-163); the exchange
+    // reads the CURRENT state and the source the NEXT one, as the reference does
+    std::vector<double> Vpot, sacc, ucur, unext;
+    // per region of the mesh view: the settings' regions first, then the surfaces the preparation
+    // adds (controlTriangles), which set neither jn, V, uP nor s
+    std::vector<fg_spin_volume> svol(c.spin_vol);
+    svol.resize(std::max<size_t>(svol.size(), (size_t)view.n_vol_regions), fg_spin_volume{0.0, 0.0, 0.0, 0.0, 0.0});
+    std::vector<fg_spin_surface> ssurf(c.spin_surf);
+    std::vector<double> sV(c.surf_V);
+    ssurf.resize(std::max<size_t>(ssurf.size(), (size_t)view.n_surf_regions),
+                 fg_spin_surface{NAN, {NAN, NAN, NAN}, {NAN, NAN, NAN}});
+    sV.resize(ssurf.size(), NAN);
+    auto spin_compute = [&]() {
+        ucur.resize(3 * (size_t)N);
+        unext.resize(3 * (size_t)N);
+        e.check(fg_get_state(e.ctx, FG_CURRENT, ucur.data(), nullptr, nullptr, nullptr), "fg_get_state");
+        e.check(fg_get_state(e.ctx, FG_NEXT, unext.data(), nullptr, nullptr, nullptr), "fg_get_state");
+        fg_solve_info si;
+        e.check(fg_spin_accumulation(&view, svol.data(), ssurf.data(), Vpot.data(), ucur.data(),
+                                     unext.data(), 1e-8, 1000, c.device, sacc.data(), &si),
+                "fg_spin_accumulation");
+        const bool ok = si.status == FG_ITER_CONVERGED;
+        if (!ok) std::fill(sacc.begin(), sacc.end(), 0.0);
+        e.check(fg_set_spin_accumulation(e.ctx, sacc.data(), svol.data()), "fg_set_spin_accumulation");
+        return ok;
+    };
+    if (c.spin_acc) {
+        std::vector<double> sig, jn;
+        for (auto &p : svol) sig.push_back(p.sigma);
+        for (auto &p : ssurf) jn.push_back(p.jn);
+        Vpot.resize(N);
+        sacc.resize(3 * (size_t)N);
+        fg_solve_info si;
+        const int rc = fg_electrostatic_potential(&view, sig.data(), jn.data(), sV.data(), 1e-8, 1000, c.device,
+                                                  Vpot.data(), &si);
+        if (rc == FG_ERR_ARG) throw std::runtime_error("incorrect boundary conditions for potential V solver");
+        e.check(rc, "fg_electrostatic_potential");
+        if (si.status != FG_ITER_CONVERGED) throw std::runtime_error("electrostatic solver did not converge");
+        if (c.V_file) {  // electrostatSolver::save (src/electrostatSolver.cpp:138-156)
+            mkdir(c.out_dir.c_str(), 0777);
+            const std::string vname = c.out_dir + "/" + c.basename + "_V.sol";
+            std::ofstream vf(vname);
+            if (!vf) throw std::runtime_error("cannot open " + vname);
+            vf << "## real-world time: " << date_string() << "\n## columns: index\tV\n" << std::scientific
+               << std::setprecision(8);
+            const int32_t *ni = fgh_node_index(mesh);
+            for (int i = 0; i < N; i++) vf << i << '\t' << Vpot[ni[i]] << "\n";
+        }
+        if (!spin_compute()) throw std::runtime_error("spin diffusion solver (first try) failed");
+    }
     auto compute_all = [&]() {  // src/fem.h:203-224
         e.check(fg_calc_demag(e.ctx), "fg_calc_demag");
+        if (c.spin_acc && !spin_compute() && verbose) std::cerr << "spin diffusion solver failed.\n";
         e.check(spacef ? fg_energy_amp(e.ctx, amp, E) : fg_energy(e.ctx, Hext, E), "fg_energy");
         e.check(fg_evolution(e.ctx), "fg_evolution");
     };
@@ -644,8 +739,9 @@ int run(const Config &c, unsigned seed, bool verbose, bool check_only)
                 e.check(fg_get_state(e.ctx, FG_NEXT, u.data(), nullptr, phi.data(), nullptr), "fg_get_state");
                 std::ostringstream md;
                 md << common_metadata(c) << "## time: " << std::scientific << t << "\n"
-                   << "## columns: idx\tmx\tmy\tmz\tphi\n";
-                if (fgh_write_sol(mesh, name.c_str(), md.str().c_str(), 7, u.data(), phi.data()) != FG_OK)
+                   << "## columns: idx\tmx\tmy\tmz\tphi" << (c.spin_acc ? "\tsx\tsy\tsz" : "") << "\n";
+                if (fgh_write_sol_s(mesh, name.c_str(), md.str().c_str(), 7, u.data(), phi.data(),
+                                    c.spin_acc ? sacc.data() : nullptr) != FG_OK)
                     throw std::runtime_error(std::string("writing ") + name + ": " + fgh_last_error());
                 if (verbose) std::cout << " " << name << "\n";
             }

@@ -915,6 +915,12 @@ int fgh_read_sol(const fgh_mesh *m, const char *path, double *u, double *phi, do
 int fgh_write_sol(const fgh_mesh *m, const char *path, const char *metadata, int precision,
                   const double *u, const double *phi)
 {
+    return fgh_write_sol_s(m, path, metadata, precision, u, phi, nullptr);
+}
+
+int fgh_write_sol_s(const fgh_mesh *m, const char *path, const char *metadata, int precision,
+                    const double *u, const double *phi, const double *s)
+{
     if (!m || !m->prepared) return FG_ERR_STATE;
     FILE *f = std::fopen(path, "w");
     if (!f) {
@@ -925,8 +931,12 @@ int fgh_write_sol(const fgh_mesh *m, const char *path, const char *metadata, int
     const size_t N = m->node_index.size();
     for (size_t i = 0; i < N; i++) {
         int32_t j = m->node_index[i];
-        std::fprintf(f, "%zu\t%.*e\t%.*e\t%.*e\t%.*e\n", i, precision, u[3 * j], precision, u[3 * j + 1], precision,
+        std::fprintf(f, "%zu\t%.*e\t%.*e\t%.*e\t%.*e", i, precision, u[3 * j], precision, u[3 * j + 1], precision,
                      u[3 * j + 2], precision, phi ? phi[j] : 0.0);
+        if (s)
+            std::fprintf(f, "\t%.*e\t%.*e\t%.*e", precision, s[3 * j], precision, s[3 * j + 1], precision,
+                         s[3 * j + 2]);
+        std::fputc('\n', f);
     }
     std::fclose(f);
     return FG_OK;

@@ -94,7 +94,7 @@ HIP_SYMBOLS = {
     "fg_prepare_elements_amp": (C.c_int, [C.c_void_p, C.c_double, C.c_double, C.c_double]),
     "fg_energy_amp": (C.c_int, [C.c_void_p, C.c_double, _d]),
     "fg_gauss_points": (C.c_int, [C.c_void_p, _d]),
-    "fg_spin_accumulation": (C.c_int, [C.c_void_p, C.POINTER(SpinVolume), C.POINTER(SpinSurface), _d, _d,
+    "fg_spin_accumulation": (C.c_int, [C.c_void_p, C.POINTER(SpinVolume), C.POINTER(SpinSurface), _d, _d, _d,
                                        C.c_double, C.c_int32, C.c_int32, _d, C.POINTER(SolveInfo)]),
     "fg_electrostatic_potential": (C.c_int, [C.c_void_p, _d, _d, _d, C.c_double, C.c_int32, C.c_int32, _d,
                                              C.POINTER(SolveInfo)]),
@@ -140,6 +140,7 @@ HOST_SYMBOLS = {
     "fgh_eval_expression": (C.c_int, [C.c_char_p, C.c_char_p, C.c_int64, _d, _d]),
     "fgh_read_sol": (C.c_int, [C.c_void_p, C.c_char_p, _d, _d, _d]),
     "fgh_write_sol": (C.c_int, [C.c_void_p, C.c_char_p, C.c_char_p, C.c_int, _d, _d]),
+    "fgh_write_sol_s": (C.c_int, [C.c_void_p, C.c_char_p, C.c_char_p, C.c_int, _d, _d, _d]),
 }
 
 _libs = {}

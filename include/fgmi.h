@@ -238,7 +238,10 @@ int fg_electrostatic_potential(const fg_mesh *mesh, const double *sigma, const d
  * boundaryConditions, solve, integrales, src/spinAccumulationSolver.cpp:41-287): the 3 n_nodes
  * system of the diffusion (D0 = 2 sigma / (e^2 N0), spin-flip length lsf) and, in the tetrahedra
  * whose region has Ms > 0, the s-d exchange with the node magnetization u[3 n_nodes] (length lsd)
- * and the source of the potential V[n_nodes] (fg_electrostatic_potential); Dirichlet values on the
+ * and the source of the potential V[n_nodes] (fg_electrostatic_potential) along u_src[3 n_nodes]
+ * (NULL: u). The reference reads the exchange from the CURRENT state and the source from the NEXT
+ * one (Tet::calcOffDiagBlock, src/tetra.cpp:142-148; mesh::getNode_u, src/mesh.h:156), which differ
+ * when compute_all runs between a solve and the evolution (src/fem.h:203-224). Dirichlet values on the
  * surfaces with s, or with jn (s = -jn mu_B / e uP). algebra::bicg_dir (6-argument overload,
  * src/algebra/bicg.h:82-158) from s = 0, tolerance tol, max_iter iterations, on HIP device `device`
  * (made current). Returns FG_ERR_ARG unless exactly one surface region has a finite jn and one a
@@ -247,8 +250,8 @@ int fg_electrostatic_potential(const fg_mesh *mesh, const double *sigma, const d
  * (interleaved x, y, z per node) and the solver's status in *info (the reference zeroes s unless
  * the status is FG_ITER_CONVERGED). Needs no context. */
 int fg_spin_accumulation(const fg_mesh *mesh, const fg_spin_volume *vol, const fg_spin_surface *surf, const double *V,
-                         const double *u, double tol, int32_t max_iter, int32_t device, double *s_out,
-                         fg_solve_info *info);
+                         const double *u, const double *u_src, double tol, int32_t max_iter, int32_t device,
+                         double *s_out, fg_solve_info *info);
 /* NEXT -> CURRENT. */
 int fg_evolution(fg_ctx *ctx);
 /* acos(min over magnetic edges of u_i . u_j) on the NEXT state. */

@@ -80,6 +80,10 @@ const char *fgh_last_error(void);
 
 /* .sol files: u and phi per node in file order, header "## time: t". */
 int fgh_read_sol(const fgh_mesh *m, const char *path, double *u, double *phi, double *t);
+/* fgh_write_sol with the spin accumulation s[3 * n_nodes] in three more columns (sx sy sz) when s
+ * is not NULL (Mesh::mesh::savesol with spin_acc, src/save.cpp:164-186). */
+int fgh_write_sol_s(const fgh_mesh *m, const char *path, const char *metadata, int precision,
+                    const double *u, const double *phi, const double *s);
 int fgh_write_sol(const fgh_mesh *m, const char *path, const char *metadata, int precision,
                   const double *u, const double *phi);
 

@@ -48,7 +48,7 @@ SYMBOLS = {
     "orc_electrostatic_potential": (C.c_int, [C.POINTER(nv.Mesh), _d, _d, _d, C.c_double, C.c_int, _d,
                                               C.POINTER(SolveInfo)]),
     "orc_spin_accumulation": (C.c_int, [C.POINTER(nv.Mesh), C.POINTER(nv.SpinVolume), C.POINTER(nv.SpinSurface),
-                                        _d, _d, C.c_double, C.c_int, _d, C.POINTER(SolveInfo)]),
+                                        _d, _d, _d, C.c_double, C.c_int, _d, C.POINTER(SolveInfo)]),
     "orc_bicg_dir": (C.c_int, [C.c_int64, _i64, _i32, _d, _d, _d, C.c_int, _i32, C.c_double, C.c_int,
                                C.POINTER(SolveInfo)]),
     "orc_system_size": (None, [C.c_void_p, _i64, _i64]),
@@ -166,7 +166,7 @@ def electrostatic_potential(mesh, sigma, jn, V, tol=1e-8, max_iter=1000):
     return out, info
 
 
-def spin_accumulation(mesh, vol, surf, V, u, tol=1e-8, max_iter=1000):
+def spin_accumulation(mesh, vol, surf, V, u, tol=1e-8, max_iter=1000, u_src=None):
     """The spin accumulation (n_nodes, 3) of an fg_mesh view: vol / surf are sequences of
     nv.SpinVolume / nv.SpinSurface per region, V the node potential, u the node magnetization;
     returns (s, SolveInfo). Raises ValueError on bad boundary conditions."""
@@ -176,7 +176,9 @@ def spin_accumulation(mesh, vol, surf, V, u, tol=1e-8, max_iter=1000):
     u = np.ascontiguousarray(u, dtype=np.float64).reshape(-1)
     out = np.zeros(3 * mesh.n_nodes)
     info = SolveInfo()
-    rc = lib().orc_spin_accumulation(C.byref(mesh), va, sa, d(V), d(u), tol, max_iter, d(out), C.byref(info))
+    us = None if u_src is None else np.ascontiguousarray(u_src, dtype=np.float64).reshape(-1)
+    rc = lib().orc_spin_accumulation(C.byref(mesh), va, sa, d(V), d(u), None if us is None else d(us), tol, max_iter,
+                                     d(out), C.byref(info))
     if rc < 0:
         raise ValueError("incorrect boundary conditions for the spin diffusion solver")
     return out.reshape(-1, 3), info

@@ -1044,8 +1044,10 @@ int orc_electrostatic_potential(const fg_mesh *m, const double *sigma, const dou
    the diffusion problem with the node magnetization u and potential V. Returns -1 on bad
    boundary conditions, else the bicg_dir status. */
 int orc_spin_accumulation(const fg_mesh *m, const fg_spin_volume *vp, const fg_spin_surface *sp, const double *Vn,
-                          const double *u, double tol, int max_iter, double *s_out, orc_solve_info *info)
+                          const double *u, const double *u_src, double tol, int max_iter, double *s_out,
+                          orc_solve_info *info)
 {
+    if (!u_src) u_src = u; /* the source's magnetization (mesh::getNode_u, NEXT) */
     const double E = 1.602176634e-19, HBAR = 6.62607015e-34 / (2 * M_PI), ME = 9.1093837139e-31;
     const double MUB = E * HBAR / (2. * ME); /* src/config.h.in:33-37 */
     auto finite3 = [](const double *v) { return std::isfinite(std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2])); };
@@ -1110,7 +1112,7 @@ int orc_spin_accumulation(const fg_mesh *m, const fg_spin_volume *vp, const fg_s
                 const double cw = cst0 * w[g];
                 const double cg[3] = {cw * gV[0], cw * gV[1], cw * gV[2]};
                 for (int i = 0; i < 4; i++) {
-                    const double *ui = u + 3 * ind[i];
+                    const double *ui = u_src + 3 * ind[i];
                     const double tmp = cg[0] * da[i][0] + cg[1] * da[i][1] + cg[2] * da[i][2];
                     BE[i] += tmp * ui[0];
                     BE[4 + i] += tmp * ui[1];

@@ -61,7 +61,8 @@ void orc_set_spin_accumulation(orc_problem *p, const double *s, const fg_spin_vo
 /* The spin accumulation s[3 n_nodes] (spinAcc::solve): returns -1 on bad boundary conditions,
    else the bicg_dir status. */
 int orc_spin_accumulation(const fg_mesh *m, const fg_spin_volume *vp, const fg_spin_surface *sp, const double *V,
-                          const double *u, double tol, int max_iter, double *s_out, orc_solve_info *info);
+                          const double *u, const double *u_src, double tol, int max_iter, double *s_out,
+                          orc_solve_info *info);
 void orc_system_size(const orc_problem *p, int64_t *rows, int64_t *nnz);
 void orc_get_system(const orc_problem *p, int64_t *rowptr, int32_t *col, double *val,
                     double *rhs);

@@ -272,3 +272,96 @@ def test_full_test_golden_on_gpu(golden_dir, tmp_path, demag):
     tol = 1e-8 if demag == "direct" else 1e-6
     assert np.linalg.norm(m - evol[-1][1:4]) < tol, (m, evol[-1][1:4])
     assert abs(e_tot - evol[-1][8]) < tol * abs(evol[-1][8])
+
+
+def bar_msh(path, cells=(10, 2, 2), lengths=(40.0, 10.0, 12.0)):
+    """A cuboid bar as gmsh 2.2 with surface regions left (x = 0), right (x = max) and side, and
+    one volume region bar; node positions in nm. Returns the positions."""
+    from feellgood_amd.mesh import HostMesh
+    hm = HostMesh.cuboid(lengths, cells)
+    hm.prepare({"volume": {}})
+    a = hm.arrays()
+    pos = a["pos"] / 1e-9
+    x0, x1 = pos[:, 0].min(), pos[:, 0].max()
+    tx = pos[a["tris"]][:, :, 0]
+    tag = np.where(np.all(np.abs(tx - x0) < 1e-6, axis=1), 1, np.where(np.all(np.abs(tx - x1) < 1e-6, axis=1), 2, 3))
+    with open(path, "w") as f:
+        f.write('$MeshFormat\n2.2 0 8\n$EndMeshFormat\n$PhysicalNames\n4\n2 1 "left"\n2 2 "right"\n2 3 "side"\n'
+                '3 4 "bar"\n$EndPhysicalNames\n$Nodes\n%d\n' % len(pos))
+        for i, p in enumerate(pos):
+            f.write("%d %.17g %.17g %.17g\n" % (i + 1, *p))
+        f.write("$EndNodes\n$Elements\n%d\n" % (len(a["tris"]) + len(a["tets"])))
+        k = 1
+        for t, g in zip(a["tris"], tag):
+            f.write("%d 2 2 %d %d %d %d %d\n" % (k, g, g, *(t + 1)))
+            k += 1
+        for t in a["tets"]:
+            f.write("%d 4 2 4 4 %d %d %d %d\n" % (k, *(t + 1)))
+            k += 1
+        f.write("$EndElements\n")
+    return pos
+
+
+def spin_settings(msh, outdir):
+    return {
+        "outputs": {"directory": str(outdir), "file_basename": "spin", "evol_time_step": 1e-12,
+                    "final_time": 4e-12, "mag_config_every": 2},
+        "mesh": {"filename": str(msh), "length_unit": 1e-9,
+                 "volume_regions": {"bar": {"Ms": 8e5, "sigma": 5e6, "dens_state": 1e47, "P": 0.4,
+                                            "l_sd": 4e-9, "l_sf": 10e-9}},
+                 "surface_regions": {"left": {"V": 0, "s": [0, 0, 0]},
+                                     "right": {"jn": 1e6, "uP": [0, 0, 2]}, "side": {}}},
+        "initial_magnetization": [1, 0, 0],
+        "Bext": [0, 0, 0],
+        "time_integration": {"min(dt)": 1e-16, "max(dt)": 1e-12, "max(du)": 0.1},
+        "spin_accumulation": {"enable": True, "V_file": True},
+        "gpu": {"demag": "direct"},
+    }
+
+
+def test_spin_accumulation_settings(tmp_path):
+    """The spin-accumulation keys (src/settings.cpp:414-418,438-449,545-550): region parameters,
+    null jn / V read as NaN, uP normalised, and a surface with both jn and V rejected."""
+    bar_msh(tmp_path / "bar.msh")
+    s = spin_settings(tmp_path / "bar.msh", tmp_path)
+    rc, out, err = check(["-"], json.dumps(s))
+    assert rc == 0, err
+    d = parsed(out)
+    assert d["spin accumulation"][0] == ["V_file", "1"]
+    assert [float(x) for x in d["spin volume"][0][2::2]] == [5e6, 1e47, 0.4, 4e-9, 10e-9]
+    sf = {r[0]: r for r in d["spin surface"]}
+    assert sf["left"][2] == "nan" and float(sf["left"][4]) == 0.0 and [float(x) for x in sf["right"][6:9]] == [0, 0, 1]
+    bad = json.loads(json.dumps(s))
+    bad["mesh"]["surface_regions"]["right"]["V"] = 1.0
+    rc, out, err = check(["-"], json.dumps(bad))
+    assert rc != 0 and "both no-null J and V" in err + out
+
+
+@pytest.mark.gpu
+def test_spin_accumulation_run_on_gpu(tmp_path):
+    """A short spin-transfer run of the bar: the potential file holds the closed form
+    V = -jn x / sigma (V = 0 on the left, jn entering on the right), the .sol files carry the three
+    spin-accumulation columns, and the current's spin torque moves the magnetization."""
+    pos = bar_msh(tmp_path / "bar.msh")
+    s = spin_settings(tmp_path / "bar.msh", tmp_path)
+    r = subprocess.run([DRIVER, "--seed", "2", "-"], input=json.dumps(s), capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    rows = [ln.split() for ln in open(tmp_path / "spin_V.sol") if not ln.startswith("#")]
+    V = np.array([float(x[1]) for x in rows])
+    x = (pos[:, 0] - pos[:, 0].min()) * 1e-9
+    # the V file is in node-file order; the bar's positions here are already in that order
+    ref = -1e6 * x / 5e6
+    assert np.abs(V - ref).max() <= 1e-6 * np.abs(ref).max()
+    sol = [ln.split() for ln in open(tmp_path / "spin_iter2.sol") if not ln.startswith("#")]
+    assert len(sol[0]) == 8
+    sz = np.array([[float(v) for v in ln[5:8]] for ln in sol])
+    assert np.abs(sz).max() > 0 and np.all(np.isfinite(sz))
+    row, n = last_evol_row(tmp_path / "spin.evol")
+    off = json.loads(json.dumps(s))
+    off["spin_accumulation"]["enable"] = False
+    off["outputs"]["directory"] = str(tmp_path / "off")
+    r = subprocess.run([DRIVER, "--seed", "2", "-"], input=json.dumps(off), capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    row0, n0 = last_evol_row(tmp_path / "off" / "spin.evol")
+    assert n == n0 == 5 and np.all(np.isfinite(row))
+    assert np.abs(np.array(row[1:4]) - row0[1:4]).max() > 1e-9

@@ -114,18 +114,19 @@ def test_boundary_condition_errors():
             oracle.spin_accumulation(m, list(vols), list(surfs), V, u)
         va = (nv.SpinVolume * 2)(*vols)
         sa = (nv.SpinSurface * 3)(*surfs)
-        rc = nv.hip_lib().fg_spin_accumulation(C.byref(m), va, sa, nv.dptr(V), nv.dptr(u), 1e-8, 1000, 0,
+        rc = nv.hip_lib().fg_spin_accumulation(C.byref(m), va, sa, nv.dptr(V), nv.dptr(u), None, 1e-8, 1000, 0,
                                                nv.dptr(out), C.byref(info))
         assert rc == nv.FG_ERR_ARG
 
 
-def device_spin(m, vols, surfs, V, u, tol=1e-8, max_iter=1000):
+def device_spin(m, vols, surfs, V, u, tol=1e-8, max_iter=1000, u_src=None):
     va = (nv.SpinVolume * len(vols))(*vols)
     sa = (nv.SpinSurface * len(surfs))(*surfs)
     out = np.zeros(3 * m.n_nodes)
     info = nv.SolveInfo()
     rc = nv.hip_lib().fg_spin_accumulation(C.byref(m), va, sa, nv.dptr(np.ascontiguousarray(V, dtype=np.float64)),
-                                           nv.dptr(np.ascontiguousarray(u, dtype=np.float64).reshape(-1)), tol,
+                                           nv.dptr(np.ascontiguousarray(u, dtype=np.float64).reshape(-1)),
+                                           None if u_src is None else nv.dptr(np.ascontiguousarray(u_src).reshape(-1)), tol,
                                            max_iter, 0, nv.dptr(out), C.byref(info))
     assert rc == nv.FG_OK
     return out.reshape(-1, 3), info
@@ -143,8 +144,10 @@ def test_device_spin_accumulation_matches_oracle():
     x = (pos[:, 0] - pos[:, 0].min()) / np.ptp(pos[:, 0])
     V = -0.02 * x + 0.004 * x**2 + 1e-3 * np.sin(pos[:, 1] * 3e8)
     vols = [vol(sigma=4e6, N0=2e47, lsf=8e-9), vol(sigma=6e6, N0=1e47, P=0.5, lsf=12e-9, lsd=4e-9)]
-    s0, i0 = oracle.spin_accumulation(m, vols, SURF, V, u, tol=1e-8, max_iter=1000)
-    s1, i1 = device_spin(m, vols, SURF, V, u)
+    u2 = u + 0.2 * rng.normal(size=u.shape)
+    u2 /= np.linalg.norm(u2, axis=1, keepdims=True)
+    s0, i0 = oracle.spin_accumulation(m, vols, SURF, V, u, tol=1e-8, max_iter=1000, u_src=u2)
+    s1, i1 = device_spin(m, vols, SURF, V, u, u_src=u2)
     assert i0.status == i1.status == nv.FG_ITER_CONVERGED
     assert abs(i1.iterations - i0.iterations) <= 2, (i0.iterations, i1.iterations)
     assert i1.rhs_norm == pytest.approx(i0.rhs_norm, rel=1e-10)
