@@ -54,7 +54,7 @@ def build_hip():
     os.replace(tmp, os.path.join(LIB, "libfgmi.so"))
 
 
-def build_host():
+def build_host_lib():
     os.makedirs(LIB, exist_ok=True)
     srcs = sorted(glob.glob(os.path.join(HERE, "host", "*.cpp")))
     drv_only = ("driver.cpp", "settings.cpp")
@@ -62,19 +62,27 @@ def build_host():
     tmp = os.path.join(LIB, "libfgmi_host.so.tmp")
     _run(["g++"] + HOST_FLAGS + ["-shared", "-o", tmp] + lib_srcs)
     os.replace(tmp, os.path.join(LIB, "libfgmi_host.so"))
+
+
+def build_driver():
+    """Link the drop-in time loop against both libraries (libfgmi.so must exist)."""
     drv = os.path.join(HERE, "host", "driver.cpp")
-    if os.path.exists(drv):
-        _run(["g++"] + HOST_FLAGS + ["-o", os.path.join(LIB, "fgmi_run"), drv,
-                                     os.path.join(HERE, "host", "settings.cpp"),
-                                     "-L" + LIB, "-lfgmi_host", "-lfgmi", "-Wl,-rpath,$ORIGIN",
-                                     "-Wl,-rpath-link," + "/opt/rocm/lib"])
+    _run(["g++"] + HOST_FLAGS + ["-o", os.path.join(LIB, "fgmi_run"), drv,
+                                 os.path.join(HERE, "host", "settings.cpp"),
+                                 "-L" + LIB, "-lfgmi_host", "-lfgmi", "-Wl,-rpath,$ORIGIN",
+                                 "-Wl,-rpath-link," + "/opt/rocm/lib"])
+
+
+def build_host():
+    build_host_lib()
+    if os.path.exists(os.path.join(LIB, "libfgmi.so")):
+        build_driver()
 
 
 def build_all():
-    build_host()
+    build_host_lib()
     build_hip()
-    if os.path.exists(os.path.join(HERE, "host", "driver.cpp")):
-        build_host()  # link the driver against the fresh libfgmi.so
+    build_driver()
 
 
 if __name__ == "__main__":

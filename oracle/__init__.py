@@ -74,6 +74,14 @@ SYMBOLS = {
     "orc_tet_integrales": (None, [_d, _d, _d, _d, _d, _d, _d, _d, C.POINTER(nv.VolumeRegion), _d,
                                   C.c_double, C.c_double, _d, _d]),
     "orc_tri_potential": (C.c_double, [_d, _d, C.c_double, C.c_int]),
+    "orc_aniso_uniax": (None, [_d, _d, _d, C.c_double, C.c_double, _d, _d]),
+    "orc_aniso_cub": (None, [_d, _d, _d, _d, _d, C.c_double, C.c_double, _d, _d]),
+    "orc_tet_energy": (None, [_d, _d, _d, _d, C.POINTER(nv.VolumeRegion), _d, _d]),
+    "orc_tri_energy": (None, [_d, _d, _d, C.c_double, C.c_double, _d, _d]),
+    "orc_tet_charges": (None, [_d, _d, _d, C.c_double, _d]),
+    "orc_tri_charges": (None, [_d, _d, C.c_double, _d]),
+    "orc_tri_integrales": (None, [_d, _d, _d, _d, C.c_double, C.c_double, _d, _d]),
+    "orc_gauss_tables": (None, [_d, _d, _d, _d]),
     "orc_set_basis": (None, [_d, C.c_double, _d, _d]),
     "orc_mt_uniform": (None, [C.c_uint32, C.c_int, _d]),
     "orc_basis_angles": (None, [C.c_uint32, C.c_int, _d]),

@@ -219,6 +219,38 @@ void tet_lumping(const double da[4][3], const double w[TNPI], const V3 u_nod[4],
 /* Tet::integrales (src/tetra.cpp:210-307) with the applied field at each Gauss point (calc_Hext:
  * a uniform field in every column, src/linear_algebra.cpp:30-35, or A(t) times the space field,
  * :63-68), no spin transfer, no drift. */
+/* Tet::calc_aniso_uniax (src/tetra.cpp:171-181): H_an,g += Kbis (uk . (U_g + s_dt V_g)) uk and
+   the uHeff contribution Kbis (uk . U_g)^2, added to uHeff[g]. */
+void aniso_uniax(const V3 U[TNPI], const V3 V[TNPI], V3 uk, double Kbis, double s_dt, V3 Han[TNPI],
+                 double uHeff[TNPI])
+{
+    for (int g = 0; g < TNPI; g++) {
+        Han[g] = Han[g] + (Kbis * dot(uk, U[g] + s_dt * V[g])) * uk;
+        uHeff[g] += Kbis * sq(dot(U[g], uk));
+    }
+}
+
+/* Tet::calc_aniso_cub (src/tetra.cpp:183-208): with beta = (ex.U, ey.U, ez.U) and
+   beta' = (ex.V, ey.V, ez.V), H_an,g += -K3bis [sum_a beta_a (1 - beta_a^2) e_a
+   + s_dt (ex o beta') o (1 - 3 beta^2)] (o: component-wise; the second term takes the components
+   of ex only, as the reference writes it) and uHeff[g] += -K3bis sum_a beta_a^2 (1 - beta_a^2). */
+void aniso_cub(const V3 U[TNPI], const V3 V[TNPI], V3 ex, V3 ey, V3 ez, double K3bis, double s_dt,
+               V3 Han[TNPI], double uHeff[TNPI])
+{
+    for (int g = 0; g < TNPI; g++) {
+        V3 uk_u{dot(ex, U[g]), dot(ey, U[g]), dot(ez, U[g])};
+        V3 uk_v{dot(ex, V[g]), dot(ey, V[g]), dot(ez, V[g])};
+        V3 uuu{uk_u.x * (1.0 - uk_u.x * uk_u.x), uk_u.y * (1.0 - uk_u.y * uk_u.y),
+               uk_u.z * (1.0 - uk_u.z * uk_u.z)};
+        V3 tmp{uk_v.x * ex.x, uk_v.y * ex.y, uk_v.z * ex.z};
+        V3 one3{1 - 3 * uk_u.x * uk_u.x, 1 - 3 * uk_u.y * uk_u.y, 1 - 3 * uk_u.z * uk_u.z};
+        V3 t2{tmp.x * one3.x, tmp.y * one3.y, tmp.z * one3.z};
+        V3 h = uuu.x * ex + uuu.y * ey + uuu.z * ez + s_dt * t2;
+        Han[g] = Han[g] + (-K3bis) * h;
+        uHeff[g] += -K3bis * dot(uk_u, uuu);
+    }
+}
+
 void tet_integrales(const double da[4][3], const double w[TNPI], const V3 u[4], const V3 v[4],
                     const double phi[4], const double phiv[4], const V3 ep[4], const V3 eq[4],
                     const fg_volume_region &prm, const V3 Hext[TNPI], double dt, double prefactor,
@@ -258,32 +290,10 @@ void tet_integrales(const double da[4][3], const double w[TNPI], const V3 u[4], 
         uHeff[g] = exch;
         Han[g] = {0, 0, 0};
     }
-    if (prm.K != 0) { /* calc_aniso_uniax (src/tetra.cpp:171-181), called with s_dt/gamma0 */
-        double Kbis = 2.0 * prm.K / (MU0 * Ms);
-        V3 uk = v3(prm.uk);
-        double sdt = s_dt / GAMMA0;
-        for (int g = 0; g < TNPI; g++) {
-            Han[g] = Han[g] + (Kbis * dot(uk, U[g] + sdt * V[g])) * uk;
-            uHeff[g] += Kbis * sq(dot(U[g], uk));
-        }
-    }
-    if (prm.K3 != 0) { /* calc_aniso_cub (src/tetra.cpp:183-208) */
-        double K3bis = 2.0 * prm.K3 / (MU0 * Ms);
-        V3 ex = v3(prm.ex), ey = v3(prm.ey), ez = v3(prm.ez);
-        double sdt = s_dt / GAMMA0;
-        for (int g = 0; g < TNPI; g++) {
-            V3 uk_u{dot(ex, U[g]), dot(ey, U[g]), dot(ez, U[g])};
-            V3 uk_v{dot(ex, V[g]), dot(ey, V[g]), dot(ez, V[g])};
-            V3 uuu{uk_u.x * (1.0 - uk_u.x * uk_u.x), uk_u.y * (1.0 - uk_u.y * uk_u.y),
-                   uk_u.z * (1.0 - uk_u.z * uk_u.z)};
-            V3 tmp{uk_v.x * ex.x, uk_v.y * ex.y, uk_v.z * ex.z};
-            V3 one3{1 - 3 * uk_u.x * uk_u.x, 1 - 3 * uk_u.y * uk_u.y, 1 - 3 * uk_u.z * uk_u.z};
-            V3 t2{tmp.x * one3.x, tmp.y * one3.y, tmp.z * one3.z};
-            V3 h = uuu.x * ex + uuu.y * ey + uuu.z * ez + sdt * t2;
-            Han[g] = Han[g] + (-K3bis) * h;
-            uHeff[g] += -K3bis * dot(uk_u, uuu);
-        }
-    }
+    if (prm.K != 0) /* calc_aniso_uniax called with s_dt/gamma0 (src/tetra.cpp:236-241) */
+        aniso_uniax(U, V, v3(prm.uk), 2.0 * prm.K / (MU0 * Ms), s_dt / GAMMA0, Han, uHeff);
+    if (prm.K3 != 0) /* calc_aniso_cub (src/tetra.cpp:242-246) */
+        aniso_cub(U, V, v3(prm.ex), v3(prm.ey), v3(prm.ez), 2.0 * prm.K3 / (MU0 * Ms), s_dt / GAMMA0, Han, uHeff);
     /* Heff = Hd + Hext + Hst (src/tetra.cpp:248-255) */
     for (int g = 0; g < TNPI; g++) uHeff[g] += dot(U[g], Hst ? (Hd + Hext[g]) + Hst[g] : Hd + Hext[g]);
     double aeff[TNPI];
@@ -376,6 +386,123 @@ double tri_potential(const V3 P[3], const double s[3], double surf, double dMs, 
     return 0.5 * dMs * pot;
 }
 
+/* Tri geometry (src/triangle.h:113-127): area, unit normal of (P1 - P0) x (P2 - P0), weights
+   w_g = 2 area pds_g. */
+void tri_geometry(const V3 P[3], TriGeo &f)
+{
+    V3 nv = cross(P[1] - P[0], P[2] - P[0]);
+    f.surf = 0.5 * norm(nv);
+    f.n = normalized(nv);
+    for (int j = 0; j < FNPI; j++) f.w[j] = 2.0 * f.surf * fpds[j];
+}
+
+/* Tri::integrales (src/triangle.cpp:6-36): surface (Neel) anisotropy, Kbis = 2 Ks / dMs,
+   BE_i = sum_g w_g Kbis (uk . U_g) a_ig uk, Lp = [eq_i . BE_i ; ep_i . BE_i] (Perm after P). */
+void tri_integrales(const TriGeo &f, const V3 u[3], const V3 ep[3], const V3 eq[3], double Ks, V3 uk,
+                    double Lp[6])
+{
+    double Kbis = 2.0 * Ks / f.dMs;
+    V3 BE[3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+    for (int g = 0; g < FNPI; g++) {
+        V3 U{0, 0, 0};
+        for (int i = 0; i < 3; i++) U = U + fa[i][g] * u[i];
+        double pre = f.w[g] * Kbis * dot(uk, U);
+        for (int i = 0; i < 3; i++) BE[i] = BE[i] + (pre * fa[i][g]) * uk;
+    }
+    for (int i = 0; i < 3; i++) {
+        Lp[3 + i] = dot(ep[i], BE[i]);
+        Lp[i] = dot(eq[i], BE[i]);
+    }
+}
+
+/* Tri::charges (src/triangle.cpp:45-59): q_g = dMs w_g (U_g . n), zero when dMs = 0. */
+void tri_charges(const TriGeo &f, const V3 u[3], double q[FNPI])
+{
+    for (int g = 0; g < FNPI; g++) {
+        V3 U{0, 0, 0};
+        for (int i = 0; i < 3; i++) U = U + fa[i][g] * u[i];
+        q[g] = (f.dMs != 0.0) ? f.dMs * (f.w[g] * dot(U, f.n)) : 0.0;
+    }
+}
+
+/* Tri::anisotropyEnergy and Tri::demagEnergy (src/triangle.cpp:38-43,80-85): e[0] = -Ks sum_g
+   w_g (U_g . uk)^2, e[1] = 1/2 mu0 dMs sum_g (U_g . n) phi_g w_g. */
+void tri_energy_terms(const TriGeo &f, const V3 u[3], const double phi[3], double Ks, V3 uk, double e[2])
+{
+    V3 U[FNPI];
+    double ph[FNPI];
+    for (int g = 0; g < FNPI; g++) {
+        V3 s{0, 0, 0};
+        double sp = 0;
+        for (int i = 0; i < 3; i++) {
+            s = s + fa[i][g] * u[i];
+            sp += phi[i] * fa[i][g];
+        }
+        U[g] = s;
+        ph[g] = sp;
+    }
+    double a = 0;
+    for (int g = 0; g < FNPI; g++) a += f.w[g] * sq(dot(U[g], uk));
+    e[0] = -Ks * a;
+    double d = 0;
+    for (int g = 0; g < FNPI; g++) d += (dot(U[g], f.n) * ph[g]) * f.w[g];
+    e[1] = 0.5 * MU0 * f.dMs * d;
+}
+
+/* Tet::charges (src/tetra.cpp:347-359): q_g = -Ms w_g div(u), div u = sum_i da_i . u_i. */
+void tet_charges(const double da[4][3], const double w[TNPI], const V3 vn[4], double Ms, double q[TNPI])
+{
+    double dud_sum = 0;
+    for (int i = 0; i < 4; i++) dud_sum += vn[i].x * da[i][0];
+    double sy = 0, sz = 0;
+    for (int i = 0; i < 4; i++) sy += vn[i].y * da[i][1];
+    for (int i = 0; i < 4; i++) sz += vn[i].z * da[i][2];
+    dud_sum = dud_sum + sy + sz;
+    for (int g = 0; g < TNPI; g++) q[g] = -Ms * w[g] * dud_sum;
+}
+
+/* The energies of one tetrahedron (src/tetra.cpp:309-391): e[0] exchange A sum_g w_g |grad U|^2,
+   e[1] uniaxial -K sum_g w_g (uk . U_g)^2, e[2] cubic K3 sum_g w_g (b0^2 b1^2 + b1^2 b2^2 +
+   b2^2 b0^2), e[3] demag -1/2 mu0 Ms sum_g w_g div(U) phi_g, e[4] Zeeman -mu0 Ms sum_g w_g U_g . H_g. */
+void tet_energy_terms(const double da[4][3], const double w[TNPI], const V3 u[4], const double phi[4],
+                      const fg_volume_region &prm, const V3 Hg[TNPI], double e[5])
+{
+    V3 U[TNPI], dU[3];
+    double ph[TNPI];
+    for (int g = 0; g < TNPI; g++) {
+        V3 s{0, 0, 0};
+        double sp = 0;
+        for (int i = 0; i < 4; i++) {
+            s = s + ta[i][g] * u[i];
+            sp += phi[i] * ta[i][g];
+        }
+        U[g] = s;
+        ph[g] = sp;
+    }
+    for (int d = 0; d < 3; d++) {
+        V3 s{0, 0, 0};
+        for (int i = 0; i < 4; i++) s = s + da[i][d] * u[i];
+        dU[d] = s;
+    }
+    double dens = dot(dU[0], dU[0]) + dot(dU[1], dU[1]) + dot(dU[2], dU[2]);
+    double ex = 0, dm = 0, an = 0, ze = 0, c = 0;
+    for (int g = 0; g < TNPI; g++) ex += w[g] * dens;
+    e[0] = prm.Ae * ex;
+    double divu = dU[0].x + dU[1].y + dU[2].z;
+    for (int g = 0; g < TNPI; g++) dm += w[g] * (divu * ph[g]);
+    e[3] = -0.5 * MU0 * prm.Ms * dm;
+    V3 uk = v3(prm.uk);
+    for (int g = 0; g < TNPI; g++) an += w[g] * sq(dot(uk, U[g]));
+    e[1] = -prm.K * an;
+    for (int g = 0; g < TNPI; g++) {
+        double a0 = dot(U[g], v3(prm.ex)), a1 = dot(U[g], v3(prm.ey)), a2 = dot(U[g], v3(prm.ez));
+        c += w[g] * (sq(a0 * a1) + sq(a1 * a2) + sq(a2 * a0));
+    }
+    e[2] = prm.K3 * c;
+    for (int g = 0; g < TNPI; g++) ze += w[g] * dot(U[g], Hg[g]);
+    e[4] = -MU0 * prm.Ms * ze;
+}
+
 } // namespace
 
 /* ====================================================================================== */
@@ -457,10 +584,8 @@ orc_problem *orc_create(const fg_mesh *m, double tol, int max_iter)
         for (int i = 0; i < 3; i++) f.ind[i] = m->tri_nodes[3 * k + i];
         f.reg = m->tri_region[k];
         f.dMs = m->tri_dMs[k];
-        V3 nv = cross(p->node[f.ind[1]].p - p->node[f.ind[0]].p, p->node[f.ind[2]].p - p->node[f.ind[0]].p);
-        f.surf = 0.5 * norm(nv);
-        f.n = normalized(nv);
-        for (int j = 0; j < FNPI; j++) f.w[j] = 2.0 * f.surf * fpds[j];
+        const V3 P[3] = {p->node[f.ind[0]].p, p->node[f.ind[1]].p, p->node[f.ind[2]].p};
+        tri_geometry(P, f);
     }
     /* edges (src/mesh.h:100-114) */
     p->edges.reserve(6 * p->tet.size());
@@ -613,20 +738,14 @@ static void prepare_elements(orc_problem *p, const double Hext_[3], const double
         bool mag = p->magNode[f.ind[0]] && p->magNode[f.ind[1]] && p->magNode[f.ind[2]];
         if (!mag || p->surf.empty() || p->surf[f.reg].Ks == 0) continue;
         const fg_surface_region &sp = p->surf[f.reg];
-        double Kbis = 2.0 * sp.Ks / f.dMs;
-        V3 uk = v3(sp.uk);
-        V3 BE[3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
-        for (int g = 0; g < FNPI; g++) {
-            V3 U{0, 0, 0};
-            for (int i = 0; i < 3; i++) U = U + fa[i][g] * p->node[f.ind[i]].d[0].u;
-            double pre = f.w[g] * Kbis * dot(uk, U);
-            for (int i = 0; i < 3; i++) BE[i] = BE[i] + (pre * fa[i][g]) * uk;
-        }
+        V3 u[3], ep[3], eq[3];
         for (int i = 0; i < 3; i++) {
             const Node &n = p->node[f.ind[i]];
-            p->LpT[6 * k + 3 + i] = dot(n.ep, BE[i]);
-            p->LpT[6 * k + i] = dot(n.eq, BE[i]);
+            u[i] = n.d[0].u;
+            ep[i] = n.ep;
+            eq[i] = n.eq;
         }
+        tri_integrales(f, u, ep, eq, sp.Ks, v3(sp.uk), &p->LpT[6 * k]);
     }
 }
 
@@ -1241,20 +1360,15 @@ void demag_charges(orc_problem *p, bool of_v)
             vn[i] = of_v ? n.d[1].v : n.d[1].u;
             P[i] = n.p;
         }
-        double dud_sum = 0;
-        for (int i = 0; i < 4; i++) dud_sum += vn[i].x * t.da[i][0];
-        double sy = 0, sz = 0;
-        for (int i = 0; i < 4; i++) sy += vn[i].y * t.da[i][1];
-        for (int i = 0; i < 4; i++) sz += vn[i].z * t.da[i][2];
-        dud_sum = dud_sum + sy + sz;
-        const double Ms = p->vol[t.reg].Ms;
+        double qt[TNPI];
+        tet_charges(t.da, t.w, vn, p->vol[t.reg].Ms, qt);
         for (int g = 0; g < TNPI; g++) {
             V3 G{0, 0, 0};
             for (int i = 0; i < 4; i++) G = G + ta[i][g] * P[i];
             p->src_xyz[3 * ns] = G.x;
             p->src_xyz[3 * ns + 1] = G.y;
             p->src_xyz[3 * ns + 2] = G.z;
-            q[ns] = -Ms * t.w[g] * dud_sum;
+            q[ns] = qt[g];
             ns++;
         }
     }
@@ -1269,13 +1383,10 @@ void demag_charges(orc_problem *p, bool of_v)
         }
         double qg[FNPI];
         V3 G[FNPI];
+        tri_charges(f, vn, qg);
         for (int g = 0; g < FNPI; g++) {
-            V3 U{0, 0, 0}, Gp{0, 0, 0};
-            for (int i = 0; i < 3; i++) {
-                U = U + fa[i][g] * vn[i];
-                Gp = Gp + fa[i][g] * P[i];
-            }
-            qg[g] = (f.dMs != 0.0) ? f.dMs * (f.w[g] * dot(U, f.n)) : 0.0;
+            V3 Gp{0, 0, 0};
+            for (int i = 0; i < 3; i++) Gp = Gp + fa[i][g] * P[i];
             G[g] = Gp;
             p->src_xyz[3 * ns] = Gp.x;
             p->src_xyz[3 * ns + 1] = Gp.y;
@@ -1372,74 +1483,32 @@ static void energy(orc_problem *p, const double Hext_[3], const double *hsp, dou
             u[i] = p->node[t.ind[i]].d[1].u;
             phi[i] = p->node[t.ind[i]].d[1].phi;
         }
-        V3 U[TNPI], dU[3];
-        double ph[TNPI];
-        for (int g = 0; g < TNPI; g++) {
-            V3 s{0, 0, 0};
-            double sp = 0;
-            for (int i = 0; i < 4; i++) {
-                s = s + ta[i][g] * u[i];
-                sp += phi[i] * ta[i][g];
-            }
-            U[g] = s;
-            ph[g] = sp;
-        }
-        for (int d = 0; d < 3; d++) {
-            V3 s{0, 0, 0};
-            for (int i = 0; i < 4; i++) s = s + t.da[i][d] * u[i];
-            dU[d] = s;
-        }
-        double dens = dot(dU[0], dU[0]) + dot(dU[1], dU[1]) + dot(dU[2], dU[2]);
-        double ex = 0, dm = 0, an = 0, ze = 0;
-        for (int g = 0; g < TNPI; g++) ex += t.w[g] * dens;
-        E[FG_E_EXCHANGE] += prm.Ae * ex;
-        double divu = dU[0].x + dU[1].y + dU[2].z;
-        for (int g = 0; g < TNPI; g++) dm += t.w[g] * (divu * ph[g]);
-        E[FG_E_DEMAG] += -0.5 * MU0 * prm.Ms * dm;
-        if (prm.K != 0.0) {
-            V3 uk = v3(prm.uk);
-            for (int g = 0; g < TNPI; g++) an += t.w[g] * sq(dot(uk, U[g]));
-            E[FG_E_ANISOTROPY] += -prm.K * an;
-        }
-        if (prm.K3 != 0.0) {
-            double c = 0;
-            for (int g = 0; g < TNPI; g++) {
-                double a0 = dot(U[g], v3(prm.ex)), a1 = dot(U[g], v3(prm.ey)), a2 = dot(U[g], v3(prm.ez));
-                c += t.w[g] * (sq(a0 * a1) + sq(a1 * a2) + sq(a2 * a0));
-            }
-            E[FG_E_ANISOTROPY] += prm.K3 * c;
-        }
-        for (int g = 0; g < TNPI; g++) {
-            const V3 Hg = hsp ? amp * V3{hsp[15 * (size_t)k + 3 * g], hsp[15 * (size_t)k + 3 * g + 1],
-                                         hsp[15 * (size_t)k + 3 * g + 2]}
-                              : Hext; /* Tet::zeemanEnergy, src/tetra.cpp:374-391 */
-            ze += t.w[g] * dot(U[g], Hg);
-        }
-        E[FG_E_ZEEMAN] += -MU0 * prm.Ms * ze;
+        V3 Hg[TNPI];
+        for (int g = 0; g < TNPI; g++) /* Tet::zeemanEnergy, src/tetra.cpp:374-391 */
+            Hg[g] = hsp ? amp * V3{hsp[15 * (size_t)k + 3 * g], hsp[15 * (size_t)k + 3 * g + 1],
+                                   hsp[15 * (size_t)k + 3 * g + 2]}
+                        : Hext;
+        double e[5];
+        tet_energy_terms(t.da, t.w, u, phi, prm, Hg, e);
+        E[FG_E_EXCHANGE] += e[0];
+        E[FG_E_DEMAG] += e[3];
+        if (prm.K != 0.0) E[FG_E_ANISOTROPY] += e[1];
+        if (prm.K3 != 0.0) E[FG_E_ANISOTROPY] += e[2];
+        E[FG_E_ZEEMAN] += e[4];
     }
     for (int k : p->magTri) {
         const TriGeo &f = p->tri[k];
-        V3 U[FNPI];
-        double ph[FNPI];
-        for (int g = 0; g < FNPI; g++) {
-            V3 s{0, 0, 0};
-            double sp = 0;
-            for (int i = 0; i < 3; i++) {
-                s = s + fa[i][g] * p->node[f.ind[i]].d[1].u;
-                sp += p->node[f.ind[i]].d[1].phi * fa[i][g];
-            }
-            U[g] = s;
-            ph[g] = sp;
+        V3 u[3];
+        double phi[3];
+        for (int i = 0; i < 3; i++) {
+            u[i] = p->node[f.ind[i]].d[1].u;
+            phi[i] = p->node[f.ind[i]].d[1].phi;
         }
-        if (!p->surf.empty() && p->surf[f.reg].Ks != 0.0) {
-            V3 uk = v3(p->surf[f.reg].uk);
-            double a = 0;
-            for (int g = 0; g < FNPI; g++) a += f.w[g] * sq(dot(U[g], uk));
-            E[FG_E_ANISOTROPY] += -p->surf[f.reg].Ks * a;
-        }
-        double d = 0;
-        for (int g = 0; g < FNPI; g++) d += (dot(U[g], f.n) * ph[g]) * f.w[g];
-        E[FG_E_DEMAG] += 0.5 * MU0 * f.dMs * d;
+        const bool ks = !p->surf.empty() && p->surf[f.reg].Ks != 0.0;
+        double e[2];
+        tri_energy_terms(f, u, phi, ks ? p->surf[f.reg].Ks : 0.0, ks ? v3(p->surf[f.reg].uk) : V3{0, 0, 0}, e);
+        if (ks) E[FG_E_ANISOTROPY] += e[0];
+        E[FG_E_DEMAG] += e[1];
     }
 }
 
@@ -1545,6 +1614,122 @@ void orc_tet_integrales(const double *da_, const double *w, const double *u_, co
     for (int g = 0; g < TNPI; g++) H[g] = v3(Hext);
     tet_integrales(da, w, u, v, phi, phiv, ep, eq, *prm, H, dt, prefactor, Kp, Lp);
     std::memcpy(Kp_, Kp, sizeof(Kp));
+}
+
+/* Tet::calc_aniso_uniax / calc_aniso_cub on Gauss-point values U, V [5][3]: H_aniso [5][3] and the
+   uHeff contributions [5] (zeroed first). */
+void orc_aniso_uniax(const double *U_, const double *V_, const double *uk, double Kbis, double s_dt, double *H,
+                     double *contrib)
+{
+    V3 U[TNPI], V[TNPI], Han[TNPI];
+    for (int g = 0; g < TNPI; g++) {
+        U[g] = v3(U_ + 3 * g);
+        V[g] = v3(V_ + 3 * g);
+        Han[g] = {0, 0, 0};
+        contrib[g] = 0.0;
+    }
+    aniso_uniax(U, V, v3(uk), Kbis, s_dt, Han, contrib);
+    for (int g = 0; g < TNPI; g++) {
+        H[3 * g] = Han[g].x;
+        H[3 * g + 1] = Han[g].y;
+        H[3 * g + 2] = Han[g].z;
+    }
+}
+
+void orc_aniso_cub(const double *U_, const double *V_, const double *ex, const double *ey, const double *ez,
+                   double K3bis, double s_dt, double *H, double *contrib)
+{
+    V3 U[TNPI], V[TNPI], Han[TNPI];
+    for (int g = 0; g < TNPI; g++) {
+        U[g] = v3(U_ + 3 * g);
+        V[g] = v3(V_ + 3 * g);
+        Han[g] = {0, 0, 0};
+        contrib[g] = 0.0;
+    }
+    aniso_cub(U, V, v3(ex), v3(ey), v3(ez), K3bis, s_dt, Han, contrib);
+    for (int g = 0; g < TNPI; g++) {
+        H[3 * g] = Han[g].x;
+        H[3 * g + 1] = Han[g].y;
+        H[3 * g + 2] = Han[g].z;
+    }
+}
+
+/* Per-element energies: orc_tet_energy e[5] (exchange, uniaxial, cubic, demag, Zeeman with the field
+   Hg [5][3] at the Gauss points) from node values u [4][3], phi [4]; orc_tri_energy e[2] (surface
+   anisotropy, demag) of the triangle of positions pos [3][3] with node values u [3][3], phi [3]. */
+void orc_tet_energy(const double *da_, const double *w, const double *u_, const double *phi,
+                    const fg_volume_region *prm, const double *Hg_, double *e)
+{
+    double da[4][3];
+    V3 u[4], Hg[TNPI];
+    for (int i = 0; i < 4; i++) {
+        for (int k = 0; k < 3; k++) da[i][k] = da_[3 * i + k];
+        u[i] = v3(u_ + 3 * i);
+    }
+    for (int g = 0; g < TNPI; g++) Hg[g] = v3(Hg_ + 3 * g);
+    tet_energy_terms(da, w, u, phi, *prm, Hg, e);
+}
+
+void orc_tri_energy(const double *pos, const double *u_, const double *phi, double dMs, double Ks, const double *uk,
+                    double *e)
+{
+    TriGeo f;
+    V3 P[3] = {v3(pos), v3(pos + 3), v3(pos + 6)};
+    tri_geometry(P, f);
+    f.dMs = dMs;
+    V3 u[3] = {v3(u_), v3(u_ + 3), v3(u_ + 6)};
+    tri_energy_terms(f, u, phi, Ks, v3(uk), e);
+}
+
+/* Per-element charges: orc_tet_charges q[5] (Tet::charges) from da [4][3], w [5], u [4][3]; orc_tri_charges
+   q[4] (Tri::charges) of the triangle pos [3][3] with node values u [3][3]. */
+void orc_tet_charges(const double *da_, const double *w, const double *u_, double Ms, double *q)
+{
+    double da[4][3];
+    V3 u[4];
+    for (int i = 0; i < 4; i++) {
+        for (int k = 0; k < 3; k++) da[i][k] = da_[3 * i + k];
+        u[i] = v3(u_ + 3 * i);
+    }
+    tet_charges(da, w, u, Ms, q);
+}
+
+void orc_tri_charges(const double *pos, const double *u_, double dMs, double *q)
+{
+    TriGeo f;
+    V3 P[3] = {v3(pos), v3(pos + 3), v3(pos + 6)};
+    tri_geometry(P, f);
+    f.dMs = dMs;
+    V3 u[3] = {v3(u_), v3(u_ + 3), v3(u_ + 6)};
+    tri_charges(f, u, q);
+}
+
+/* Tri::integrales (surface anisotropy) of the triangle pos [3][3]: Lp[6] from u, ep, eq [3][3]. */
+void orc_tri_integrales(const double *pos, const double *u_, const double *ep_, const double *eq_, double dMs,
+                        double Ks, const double *uk, double *Lp)
+{
+    TriGeo f;
+    V3 P[3] = {v3(pos), v3(pos + 3), v3(pos + 6)};
+    tri_geometry(P, f);
+    f.dMs = dMs;
+    V3 u[3], ep[3], eq[3];
+    for (int i = 0; i < 3; i++) {
+        u[i] = v3(u_ + 3 * i);
+        ep[i] = v3(ep_ + 3 * i);
+        eq[i] = v3(eq_ + 3 * i);
+    }
+    tri_integrales(f, u, ep, eq, Ks, v3(uk), Lp);
+}
+
+/* Gauss tables: a[4][5] of the tetrahedron and a[3][4] of the triangle, with their weights. */
+void orc_gauss_tables(double *ta_, double *tpds_, double *fa_, double *fpds_)
+{
+    for (int i = 0; i < TN; i++)
+        for (int g = 0; g < TNPI; g++) ta_[TNPI * i + g] = ta[i][g];
+    for (int g = 0; g < TNPI; g++) tpds_[g] = tpds[g];
+    for (int i = 0; i < FN; i++)
+        for (int g = 0; g < FNPI; g++) fa_[FNPI * i + g] = fa[i][g];
+    for (int g = 0; g < FNPI; g++) fpds_[g] = fpds[g];
 }
 
 double orc_tri_potential(const double *pos, const double *s, double dMs, int i)

@@ -102,6 +102,26 @@ void orc_tet_integrales(const double *da, const double *w, const double *u, cons
                         const double *phi, const double *phiv, const double *ep,
                         const double *eq, const fg_volume_region *prm, const double *Hext,
                         double dt, double prefactor, double *Kp, double *Lp);
+/* Tet::calc_aniso_uniax / calc_aniso_cub on Gauss-point values U[15], V[15] ([5][3]): H[15] and
+ * the uHeff contributions contrib[5]. */
+void orc_aniso_uniax(const double *U, const double *V, const double *uk, double Kbis, double s_dt,
+                     double *H, double *contrib);
+void orc_aniso_cub(const double *U, const double *V, const double *ex, const double *ey, const double *ez,
+                   double K3bis, double s_dt, double *H, double *contrib);
+/* Energies of one tetrahedron, e[5] = exchange, uniaxial, cubic, demag, Zeeman (field Hg[15] at the
+ * Gauss points), and of one triangle, e[2] = surface anisotropy, demag. */
+void orc_tet_energy(const double *da, const double *w, const double *u, const double *phi,
+                    const fg_volume_region *prm, const double *Hg, double *e);
+void orc_tri_energy(const double *pos, const double *u, const double *phi, double dMs, double Ks,
+                    const double *uk, double *e);
+/* Charges of one tetrahedron (q[5]) and one triangle (q[4]). */
+void orc_tet_charges(const double *da, const double *w, const double *u, double Ms, double *q);
+void orc_tri_charges(const double *pos, const double *u, double dMs, double *q);
+/* Tri::integrales of one triangle: Lp[6]. */
+void orc_tri_integrales(const double *pos, const double *u, const double *ep, const double *eq, double dMs,
+                        double Ks, const double *uk, double *Lp);
+/* Gauss tables a[4][5], pds[5] (tetrahedron) and a[3][4], pds[4] (triangle). */
+void orc_gauss_tables(double *ta, double *tpds, double *fa, double *fpds);
 /* Tri::potential at local vertex i: pos[9], s[3] = getter(node).n values, surf, dMs. */
 double orc_tri_potential(const double *pos, const double *s, double dMs, int i);
 /* Node::setBasis on one node. */

@@ -156,13 +156,13 @@ def test_solve_and_update(small):
     (rc0, i0), (f1, i1) = out
     assert rc0 == 0 and not f1
     assert i1.status == 0 and abs(i1.iterations - i0.iterations) <= 1
-    assert i1.rhs_norm == pytest.approx(i0.rhs_norm, rel=1e-12)
+    assert i1.rhs_norm == pytest.approx(i0.rhs_norm, rel=1e-12, abs=0)
     un0, vn0, _, _ = p.get_state(nv.FG_NEXT)
     un1, vn1, _, _ = g.get_state(nv.FG_NEXT)
     assert rel(vn1, vn0) < 1e-8
     assert np.abs(un1 - un0).max() <= 1e-6  # per-step |dm| bound
     assert np.abs(un1 - un0).max() <= 1e-12
-    assert i1.v_max == pytest.approx(i0.v_max, rel=1e-8)
+    assert i1.v_max == pytest.approx(i0.v_max, rel=1e-8, abs=0)
 
 
 def test_space_field_step_and_energy(small):
@@ -242,7 +242,7 @@ def test_energy_average_max_angle(small):
     assert np.allclose(E1, E0, rtol=1e-8, atol=1e-8 * np.abs(E0).max())
     assert np.allclose(g.average(0), p.average(0), rtol=1e-12, atol=1e-14)
     assert np.allclose(g.average(1), p.average(1), rtol=1e-12, atol=1e-14 * np.abs(v).max())
-    assert g.max_angle() == pytest.approx(p.max_angle(), rel=1e-12)
+    assert g.max_angle() == pytest.approx(p.max_angle(), rel=1e-12, abs=0)
 
 
 def test_trajectory_20_steps():

@@ -273,7 +273,7 @@ def test_cylinder_generator_valid():
     # (1, -2), ends at z0 and z0 + L
     rho = np.hypot(pos[:, 0] - 1e-9, pos[:, 1] + 2e-9)
     assert r - h <= rho.max() * 1e9 <= r + h / np.sqrt(2)
-    assert pos[:, 2].min() == pytest.approx(5e-9) and pos[:, 2].max() == pytest.approx(65e-9)
+    assert pos[:, 2].min() == pytest.approx(5e-9, rel=1e-12, abs=0) and pos[:, 2].max() == pytest.approx(65e-9, rel=1e-12, abs=0)
 
 
 def _ev(expr, names="t", vals=(0.0,)):

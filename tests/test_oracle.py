@@ -57,7 +57,7 @@ def test_tet_inner_tables_unit_tet():
     dadu = np.array([[-1., -1., -1.], [1, 0, 0], [0, 1, 0], [0, 0, 1]])
     assert np.array_equal(da, dadu)
     assert np.array_equal(w, PDS_TET)
-    assert w.sum() == pytest.approx(1 / 6, rel=5e-16)
+    assert w.sum() == pytest.approx(1 / 6, rel=5e-16, abs=0)
 
 
 def test_tet_geometry_random_tets_against_numpy():
@@ -75,7 +75,7 @@ def test_tet_geometry_random_tets_against_numpy():
         expect = np.vstack([-Ji.sum(axis=0), Ji])
         assert np.allclose(da, expect, rtol=1e-12, atol=1e-12 * np.abs(expect).max())
         vol = abs(np.dot(P[1] - P[0], np.cross(P[2] - P[0], P[3] - P[0]))) / 6
-        assert w.sum() == pytest.approx(vol, rel=1e-13)
+        assert w.sum() == pytest.approx(vol, rel=1e-13, abs=0)
         # sum of shape-function gradients vanishes (partition of unity)
         assert np.allclose(da.sum(axis=0), 0, atol=1e-12 * np.abs(da).max())
     assert geometry(np.zeros((4, 3)))[0] == -1  # singular
@@ -96,7 +96,7 @@ def test_calc_alpha_eff_ut_time_int():
             ref = alpha + rdt / 2 * (M if h > M else h)
         else:
             ref = alpha / (1 + rdt / (2 * alpha) * M) if h < -M else alpha / (1 - rdt / (2 * alpha) * h)
-        assert out[0] == pytest.approx(ref, rel=5e-16)
+        assert out[0] == pytest.approx(ref, rel=5e-16, abs=0)
 
 
 def test_tet_lumping_legacy_loops():
@@ -295,8 +295,8 @@ def test_tri_potential_legacy_closed_form_and_quadrature():
         dMs = dr()
         for i in range(3):
             got = _potential(P, s, dMs, i)
-            assert got == pytest.approx(_legacy_potential(P, s, dMs, i), rel=1e-12)
-            assert got == pytest.approx(dMs * _quad_potential(P, s, i), rel=1e-9)
+            assert got == pytest.approx(_legacy_potential(P, s, dMs, i), rel=1e-12, abs=0)
+            assert got == pytest.approx(dMs * _quad_potential(P, s, i), rel=1e-9, abs=0)
 
 
 # ---------------------------------------------------------------- Krylov
@@ -508,7 +508,7 @@ def test_uniform_sphere_demag_field():
         V = abs(np.linalg.det(J)) / 6
         Hz += -(phi[t] @ da[:, 2]) * V
         vol += V
-    assert Hz / vol == pytest.approx(-8e5 / 3, rel=0.03)
+    assert Hz / vol == pytest.approx(-8e5 / 3, rel=0.03, abs=0)
 
 
 def test_energy_closed_forms():
@@ -523,13 +523,13 @@ def test_energy_closed_forms():
     H = np.array([1e4, 2e4, -3e4])
     E = p.energy(H)
     V = (20e-9) ** 3
-    assert E[3] == pytest.approx(-MU0 * 8e5 * V * (mz[0] @ H), rel=1e-12)
+    assert E[3] == pytest.approx(-MU0 * 8e5 * V * (mz[0] @ H), rel=1e-12, abs=0)
     assert abs(E[0]) < 1e-30
     lin = np.zeros((N, 3))
     lin[:, 0] = pos[:, 1] * 1e7  # u_x = 1e7 * y  -> |grad u|^2 = 1e14
     p.set_state(2, u=lin)
     E = p.energy(np.zeros(3))
-    assert E[0] == pytest.approx(1.3e-11 * 1e14 * V, rel=1e-12)
+    assert E[0] == pytest.approx(1.3e-11 * 1e14 * V, rel=1e-12, abs=0)
 
 
 def test_demag_energy_volume_equals_surface_form():
@@ -555,7 +555,7 @@ def test_demag_energy_volume_equals_surface_form():
         Hd = -(phi[t] @ da)
         ubar = u[t].mean(axis=0)  # P1 field integrates to the vertex mean
         tot += -0.5 * MU0 * 8e5 * V * (ubar @ Hd)
-    assert E[2] == pytest.approx(tot, rel=1e-10)
+    assert E[2] == pytest.approx(tot, rel=1e-10, abs=0)
 
 
 # ---------------------------------------------------------------- the reference's golden
