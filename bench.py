@@ -30,50 +30,13 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
+from feellgood_amd.workloads import (MU0, build_dots, build_mesh, cube_workload, dots_workload, field,  # noqa: E402
+                                     film_workload, initial_state, prefactor, wire_workload, workload_text)
+
 METRIC = "LLG steps/s at 2M tets: learned+mixed vs plain FP64 GPU at |dm|<=1e-6"
-MU0 = 1.25663706127e-6
 HBM_PEAK_GBS = 8000.0
 FP64_VALU_TFLOPS = 78.6
 FP32_VALU_TFLOPS = 157.3
-
-
-def film_workload():
-    return dict(name="film", lengths=(1000.0, 500.0, 20.0), cells=(333, 167, 6), jitter=0.1, seed=2,
-                Ms=8e5, Ae=1.3e-11, alpha=0.02, B=0.025, angle=170.0, dt=5e-14, tol=1e-10)
-
-
-def cube_workload():
-    return dict(name="cube", lengths=(100.0, 100.0, 100.0), cells=(20, 20, 20), jitter=0.0, seed=1,
-                Ms=8e5, Ae=1.3e-11, alpha=0.5, B=0.0, angle=0.0, dt=1e-14, tol=1e-10)
-
-
-def wire_workload():
-    return dict(name="wire", radius=50.0, length=4000.0, h=2.66, seed=5, Ms=8e5, Ae=1.3e-11, alpha=0.05,
-                dt=2e-14, tol=1e-10, reversed=200.0, ramp_T_per_s=-0.06 / 1e-10)
-
-
-def dots_workload():
-    return dict(name="dots", count=256, seed=4, h=4.4, dt=1e-14, tol=1e-10, alpha=0.1, B=0.0, angle=0.0)
-
-
-def build_dots(w):
-    """256 ellipsoidal nanodots: semi-axes U[10,60] x U[10,60] x U[3,20] nm (seed 4), a grid of
-    spacing h split into 6 tetrahedra per cell with the boundary projected on the ellipsoid;
-    per dot Ms U[6e5, 1.4e6] A/m, A = 1.3e-11 J/m, uniaxial K U[0, 5e4] J/m^3 along a random axis,
-    alpha = 0.1, random unit initial m."""
-    from feellgood_amd.mesh import HostMesh
-    rng = np.random.default_rng(w["seed"])
-    meshes, us = [], []
-    for k in range(w["count"]):
-        a, b, c = rng.uniform(10, 60), rng.uniform(10, 60), rng.uniform(3, 20)
-        m = HostMesh.ellipsoid((0.0, 0.0, 0.0), (a, b, c), w["h"])
-        uk = rng.normal(size=3)
-        m.prepare({"volume": dict(Ms=rng.uniform(6e5, 1.4e6), Ae=1.3e-11, alpha=w["alpha"], K=rng.uniform(0, 5e4),
-                                  uk=tuple(uk / np.linalg.norm(uk)))})
-        u = rng.normal(size=(m.n_nodes, 3))
-        meshes.append(m)
-        us.append(u / np.linalg.norm(u, axis=1, keepdims=True))
-    return meshes, np.concatenate(us)
 
 
 def bench_dots(args):
@@ -137,61 +100,6 @@ def bench_dots(args):
            "tet_steps_per_s": round(T * 1e3 / ms, 1),
            "bicgstab_iterations_per_step": round(float(np.mean(iters)), 2)}
     print(json.dumps(out), flush=True)
-
-
-def workload_text(w, m):
-    if w["name"] == "wire":
-        return (f"wire r {w['radius']:g} nm, L {w['length']:g} nm, h {w['h']:g} nm ({m.n_tets} tets, {m.n_nodes} "
-                f"nodes, {m.n_tris} surface triangles), +z with a reversed {w['reversed']:g} nm end, field ramp "
-                f"{w['ramp_T_per_s'] * 1e-10:g} T per 0.1 ns along z, dt {w['dt']:g} s, tol {w['tol']:g}")
-    return (f"{w['name']} {w['lengths'][0]:g}x{w['lengths'][1]:g}x{w['lengths'][2]:g} nm, "
-            f"{w['cells'][0]}x{w['cells'][1]}x{w['cells'][2]} cells x 6 tets "
-            f"({m.n_tets} tets, {m.n_nodes} nodes, {m.n_tris} surface triangles), "
-            f"jitter {w['jitter']}, dt {w['dt']:g} s, tol {w['tol']:g}")
-
-
-def build_mesh(w):
-    from feellgood_amd.mesh import HostMesh
-    if w["name"] == "wire":
-        m = HostMesh.cylinder((0.0, 0.0), 0.0, w["radius"], w["length"], w["h"])
-        m.prepare({"volume": dict(Ms=w["Ms"], Ae=w["Ae"], alpha=w["alpha"])})
-        return m
-    m = HostMesh.cuboid(w["lengths"], w["cells"], jitter=w["jitter"], seed=w["seed"])
-    m.prepare({"volume": dict(Ms=w["Ms"], Ae=w["Ae"], alpha=w["alpha"])})
-    return m
-
-
-def initial_state(m, w):
-    pos = m.arrays()["pos"]
-    if w["name"] == "wire":
-        # +z, a reversed domain over the first `reversed` nm, a seeded +/-0.02 perturbation
-        rng = np.random.default_rng(w["seed"])
-        u = np.zeros((m.n_nodes, 3))
-        u[:, 2] = np.where(pos[:, 2] < w["reversed"] * 1e-9, -1.0, 1.0)
-        u += rng.uniform(-0.02, 0.02, size=u.shape)
-        return u / np.linalg.norm(u, axis=1, keepdims=True)
-    if w["name"] == "film":
-        L = w["lengths"][0] * 1e-9
-        ph = 0.3 * np.sin(np.pi * pos[:, 0] / L)
-        u = np.stack([np.cos(ph), np.sin(ph), np.zeros_like(ph)], axis=1)
-    else:
-        rng = np.random.default_rng(w["seed"])
-        u = np.array([1.0, 1.0, 0.0]) / math.sqrt(2) + rng.uniform(-0.05, 0.05, size=(m.n_nodes, 3))
-        u /= np.linalg.norm(u, axis=1, keepdims=True)
-    return u
-
-
-def field(w, t=0.0):
-    if "ramp_T_per_s" in w:
-        return np.array([0.0, 0.0, w["ramp_T_per_s"] * t]) / MU0
-    a = math.radians(w["angle"])
-    return np.array([math.cos(a), math.sin(a), 0.0]) * w["B"] / MU0
-
-
-def prefactor(dt):
-    # timing::set_dt with dt_min = dt_max = dt (src/time_integration.h:37-42)
-    tt = dt / (100.0 * dt)
-    return 1.0 + tt * abs(math.log(tt))
 
 
 # ------------------------------------------------------------------ byte / flop models

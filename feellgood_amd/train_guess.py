@@ -72,13 +72,13 @@ def main(argv=None):
     ap.add_argument("--demag", default="fmm")
     args = ap.parse_args(argv)
 
-    import bench
+    from feellgood_amd import workloads as W
     from feellgood_amd.engine import Engine, options
-    w = bench.film_workload() if args.workload == "film" else bench.cube_workload()
-    m = bench.build_mesh(w)
-    u0 = bench.initial_state(m, w)
+    w = W.film_workload() if args.workload == "film" else W.cube_workload()
+    m = W.build_mesh(w)
+    u0 = W.initial_state(m, w)
     dt = w["dt"]
-    pref = bench.prefactor(dt)
+    pref = W.prefactor(dt)
     g = Engine(m.mesh, options(tol=w["tol"], max_iter=10000, demag=args.demag))
     g.guess_init(seed=args.seed)
     rng = np.random.default_rng(args.seed)

@@ -1,0 +1,280 @@
+"""The benchmark configurations at full size on the GPU.
+
+* cube (48,000 tets, 200 steps, FMM order 8): the FMM engine against the exact (direct-sum) engine
+  in lock-step at every step, and the first 10 steps of the direct engine against the oracle.
+* film (2,001,996 tets), wire (10,142,976 tets): properties that hold at any size -- |u| = 1 and
+  v . u = 0 on every node, an independent FP64 residual |b - A x| <= tol |b| of the assembled
+  system, the FMM potentials at 256 sampled nodes against the oracle's exact sums -- and the
+  per-step |dm| of the mixed-precision (and learned-guess) engine against the plain FP64 one; on
+  the film also the per-step |dm| and the demag energy of the FMM engines against the exact
+  engine.
+* dots (256 ellipsoidal dots, 1,197,228 tets in one batched context): three instances against the
+  oracle on their own meshes, and the batched FMM against the batched exact sum.
+
+Bounds: |dm| <= 1e-6 per step; FP64 parity of the same algorithm 1e-8 on u; the order-8 FMM's
+potential error <= 1e-3 of the largest sampled potential (its error on the test films is 6e-4).
+"""
+import math
+import time
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+import oracle
+from feellgood_amd import native as nv
+from feellgood_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+FMM_BOUND = 1e-3  # order 8
+
+
+def log(msg):
+    print(f"[configs] {msg}", flush=True)
+
+
+def engine(mesh, **kw):
+    from feellgood_amd.engine import Engine
+    kw.setdefault("max_iter", 10000)
+    return Engine(mesh, **kw)
+
+
+def start(o, u0, H):
+    """compute_all at t0 (src/time_integration.cpp:136): demag, energy, evolution."""
+    o.set_state(nv.FG_BOTH, u=u0)
+    o.calc_demag()
+    E = o.energy(H)
+    o.evolution()
+    return E
+
+
+def step(o, r, H, dt, pref):
+    o.base_projection(r)
+    o.prepare_elements(H, dt, pref)
+    rc = o.solve(dt)
+    failed = rc[0] if isinstance(rc[0], bool) else rc[0] != 0
+    assert not failed, "solver failure"
+    o.calc_demag()
+    E = o.energy(H)
+    o.evolution()
+    return rc[1], E
+
+
+def follow(o, ref):
+    """Put engine o at the reference engine's CURRENT u, v with its own demag potentials."""
+    u, v, _, _ = ref.get_state(nv.FG_CURRENT)
+    o.set_state(nv.FG_BOTH, u=u, v=v)
+    o.calc_demag()
+    o.evolution()
+
+
+def dm(a, b):
+    return float(np.abs(a.get_state(nv.FG_CURRENT)[0] - b.get_state(nv.FG_CURRENT)[0]).max())
+
+
+def check_properties(g, tol, what):
+    """|u| = 1, v . u = 0 on NEXT and the independent FP64 residual of the last solve."""
+    u, v, _, _ = g.get_state(nv.FG_NEXT)
+    assert np.abs(np.linalg.norm(u, axis=1) - 1).max() <= 1e-12, what
+    vn = np.linalg.norm(v, axis=1).max()
+    assert np.abs(np.sum(u * v, axis=1)).max() <= 1e-10 * vn, what
+    rp, col, val, rhs = g.system()
+    A = sp.csr_matrix((val, col, rp), shape=(len(rhs), len(rhs)))
+    x = g.solution()
+    res = np.linalg.norm(rhs - A @ x)
+    bn = np.linalg.norm(rhs)
+    log(f"{what}: |b - A x| / |b| = {res / bn:.3e} (tol {tol:g})")
+    assert res <= 1.5 * tol * bn, (what, res / bn)
+
+
+def check_samples(g, p, n, what, bound=FMM_BOUND):
+    """FMM potentials phi, phi_v of the NEXT state at n sampled nodes against the oracle's exact
+    sums on the same state."""
+    u, v, phi, phiv = g.get_state(nv.FG_NEXT)
+    p.set_state(nv.FG_NEXT, u=u, v=v)
+    idx = np.linspace(0, len(phi) - 1, n).astype(np.int32)
+    e_phi, e_phiv = p.demag_sample(idx)
+    err = np.abs(phi[idx] - e_phi).max() / np.abs(e_phi).max()
+    errv = np.abs(phiv[idx] - e_phiv).max() / max(np.abs(e_phiv).max(), 1e-300)
+    log(f"{what}: FMM potential error at {n} nodes: phi {err:.2e}, phi_v {errv:.2e} of the largest")
+    assert err <= bound and errv <= bound, (what, err, errv)
+
+
+# ---------------------------------------------------------------- cube, 48,000 tets
+@pytest.fixture(scope="module")
+def cube():
+    w = W.cube_workload()
+    m = W.build_mesh(w)
+    assert (m.n_nodes, m.n_tets) == (9261, 48000)
+    return w, m, W.initial_state(m, w)
+
+
+def test_cube_fmm_lockstep_200_steps_against_exact(cube):
+    """The order-8 FMM engine follows the exact engine for the whole 200-step run: at every step it
+    starts from the exact run's state and its step differs by |dm| <= 1e-6; the free-running FMM
+    trajectory is reported."""
+    w, m, u0 = cube
+    H = W.field(w)
+    dt, pref = w["dt"], W.prefactor(w["dt"])
+    d = engine(m.mesh, tol=w["tol"], demag="direct")
+    f = engine(m.mesh, tol=w["tol"], demag="fmm", fmm_order=8)
+    free = engine(m.mesh, tol=w["tol"], demag="fmm", fmm_order=8)
+    for o in (d, f, free):
+        start(o, u0, H)
+    angles = oracle.basis_angles(w["seed"], w["steps"])
+    worst, t0 = 0.0, time.perf_counter()
+    for k in range(w["steps"]):
+        follow(f, d)
+        for o in (d, f, free):
+            step(o, angles[k], H, dt, pref)
+        e = dm(f, d)
+        worst = max(worst, e)
+        assert e <= 1e-6, (k, e)
+    log(f"cube: 200 lock-step steps, max per-step |dm| FMM vs exact {worst:.3e}; free-running after 200 "
+        f"steps {dm(free, d):.3e} ({time.perf_counter() - t0:.1f} s)")
+
+
+def test_cube_first_10_steps_against_oracle(cube):
+    """The exact-demag engine and the oracle run the first 10 steps side by side (same basis
+    angles): |dm| <= 1e-8 and energies within 1e-8 relative at every step."""
+    w, m, u0 = cube
+    H = W.field(w)
+    dt, pref = w["dt"], W.prefactor(w["dt"])
+    g = engine(m.mesh, tol=w["tol"], demag="direct")
+    p = oracle.Problem(m.mesh, tol=w["tol"], max_iter=10000)
+    E0 = start(p, u0, H)
+    E1 = start(g, u0, H)
+    assert np.allclose(E1, E0, rtol=1e-8, atol=1e-8 * np.abs(E0).max())
+    angles = oracle.basis_angles(w["seed"], 10)
+    for k in range(10):
+        i0, E0 = step(p, angles[k], H, dt, pref)
+        i1, E1 = step(g, angles[k], H, dt, pref)
+        e = dm(g, p)
+        assert e <= 1e-8, (k, e)
+        assert np.allclose(E1, E0, rtol=1e-8, atol=1e-8 * np.abs(E0).max()), (k, E0, E1)
+        assert abs(i1.iterations - i0.iterations) <= 1
+    log(f"cube: 10 steps against the oracle, last |dm| {e:.2e}, energies {E1}")
+
+
+# ---------------------------------------------------------------- film, 2,001,996 tets
+def test_film_full_size():
+    """Plain FP64 FMM, mixed-precision FMM with the committed network's guess, and the exact engine
+    on the 2M-tet film: two steps in lock-step behind the exact engine; properties, sampled
+    potentials, per-step |dm| <= 1e-6 of both FMM engines against exact, the demag energy of the FMM
+    within 1e-3 of exact, and the learned+mixed step against the plain FP64 one."""
+    w = W.film_workload()
+    t0 = time.perf_counter()
+    m = W.build_mesh(w)
+    u0 = W.initial_state(m, w)
+    H = W.field(w)
+    dt, pref = w["dt"], W.prefactor(w["dt"])
+    d = engine(m.mesh, tol=w["tol"], demag="direct")
+    f = engine(m.mesh, tol=w["tol"], demag="fmm", fmm_order=8)
+    x = engine(m.mesh, tol=w["tol"], demag="fmm", fmm_order=8, precision="mixed")
+    params = np.load(W.guess_weights_path("film"), allow_pickle=False)
+    x.guess_set(params)
+    x.guess_enable(True)
+    Ed = start(d, u0, H)
+    start(f, u0, H)
+    start(x, u0, H)
+    log(f"film: {m.n_tets} tets, engines ready in {time.perf_counter() - t0:.1f} s")
+    p = oracle.Problem(m.mesh, tol=w["tol"], max_iter=10000)
+    rng = np.random.default_rng(w["seed"])
+    for k in range(2):
+        r = (2 / math.pi) * rng.uniform()
+        follow(f, d)
+        follow(x, d)
+        _, Ed = step(d, r, H, dt, pref)
+        _, Ef = step(f, r, H, dt, pref)
+        _, Ex = step(x, r, H, dt, pref)
+        ef, ex = dm(f, d), dm(x, d)
+        eE = abs(Ef[2] - Ed[2]) / abs(Ed[2])
+        exE = abs(Ex[2] - Ed[2]) / abs(Ed[2])
+        log(f"film step {k}: per-step |dm| against exact: FP64 FMM {ef:.3e}, learned+mixed {ex:.3e}; "
+            f"E_demag relative error FP64 FMM {eE:.2e}, learned+mixed {exE:.2e}; learned+mixed against "
+            f"FP64 FMM {dm(x, f):.3e}")
+        assert ef <= 1e-6 and ex <= 1e-6
+        assert eE <= 1e-3 and exE <= 1e-3
+    check_properties(f, w["tol"], "film FP64 FMM")
+    check_properties(x, w["tol"], "film learned+mixed")
+    # the potentials of the last step at 256 sampled nodes against the exact sums
+    check_samples(f, p, 256, "film FP64 FMM")
+    check_samples(x, p, 256, "film learned+mixed")
+    log(f"film: done in {time.perf_counter() - t0:.1f} s")
+
+
+# ---------------------------------------------------------------- wire, 10 M tets
+def test_wire_full_size():
+    """The 10M-tet nanowire: one step of the plain FP64 FMM engine and of the mixed-precision one
+    from the same state; properties, sampled potentials against the exact sums, and the mixed
+    step against the plain one."""
+    w = W.wire_workload()
+    t0 = time.perf_counter()
+    m = W.build_mesh(w)
+    u0 = W.initial_state(m, w)
+    assert m.n_tets > 9_000_000
+    dt, pref = w["dt"], W.prefactor(w["dt"])
+    H = W.field(w, 500 * dt)  # the ramp after 500 steps (3 mT)
+    f = engine(m.mesh, tol=w["tol"], demag="fmm", fmm_order=8)
+    x = engine(m.mesh, tol=w["tol"], demag="fmm", fmm_order=8, precision="mixed")
+    start(f, u0, H)
+    start(x, u0, H)
+    log(f"wire: {m.n_tets} tets, engines ready in {time.perf_counter() - t0:.1f} s")
+    r = 0.37
+    for k in range(2):
+        follow(x, f)
+        step(f, r, H, dt, pref)
+        step(x, r, H, dt, pref)
+        e = dm(x, f)
+        log(f"wire step {k}: per-step |dm| mixed against FP64 FMM {e:.3e}")
+        assert e <= 1e-6
+    check_properties(f, w["tol"], "wire FP64 FMM")
+    check_properties(x, w["tol"], "wire mixed")
+    f.close()
+    p = oracle.Problem(m.mesh, tol=w["tol"], max_iter=10000)
+    check_samples(x, p, 256, "wire mixed")
+    log(f"wire: done in {time.perf_counter() - t0:.1f} s")
+
+
+# ---------------------------------------------------------------- dots, 256 in one context
+def test_dots_batch_full_size():
+    """256 ellipsoidal dots (1.2 M tets) in one batched context: one step of the exact-demag batch
+    against the oracle on three instances' own meshes (|dm| <= 1e-8), and the batched FMM (mixed
+    precision) against the batched exact sum, per step |dm| <= 1e-6."""
+    w = W.dots_workload()
+    t0 = time.perf_counter()
+    meshes, u0 = W.build_dots(w)
+    T = sum(mm.n_tets for mm in meshes)
+    assert 1_100_000 < T < 1_300_000
+    from feellgood_amd.engine import Engine, options
+    d = Engine.batch([mm.mesh for mm in meshes], options(tol=w["tol"], max_iter=10000, demag="direct"))
+    f = Engine.batch([mm.mesh for mm in meshes], options(tol=w["tol"], max_iter=10000, demag="fmm",
+                                                         precision="mixed"))
+    no, _ = d.layout()
+    H = W.field(w)
+    dt, pref = w["dt"], W.prefactor(w["dt"])
+    start(d, u0, H)
+    start(f, u0, H)
+    log(f"dots: {len(meshes)} dots, {T} tets, engines ready in {time.perf_counter() - t0:.1f} s")
+    r = 0.21
+    for k in range(2):
+        follow(f, d)
+        step(d, r, H, dt, pref)
+        step(f, r, H, dt, pref)
+        e = dm(f, d)
+        log(f"dots step {k}: per-step |dm| batched FMM (mixed) against batched exact {e:.3e}")
+        assert e <= 1e-6
+    check_properties(d, w["tol"], "dots exact")
+    ud = d.get_state(nv.FG_CURRENT)[0]
+    sizes = [mm.n_tets for mm in meshes]
+    for k in (0, int(np.argmax(sizes)), len(meshes) - 1):
+        mm = meshes[k]
+        p = oracle.Problem(mm.mesh, tol=w["tol"], max_iter=10000)
+        start(p, u0[no[k]:no[k + 1]], H)
+        for _ in range(2):
+            step(p, r, H, dt, pref)
+        e = float(np.abs(p.get_state(nv.FG_CURRENT)[0] - ud[no[k]:no[k + 1]]).max())
+        log(f"dots: instance {k} ({mm.n_tets} tets) against the oracle after 2 steps: |dm| {e:.2e}")
+        assert e <= 1e-8, (k, e)
+    log(f"dots: done in {time.perf_counter() - t0:.1f} s")

@@ -232,6 +232,45 @@ def test_spin_field_step(small):
     assert rel(g.system()[3], b0) > 1e-9  # the field changed the right-hand side
 
 
+def test_surface_anisotropy_step_and_energy():
+    """Surface (Neel) anisotropy Ks != 0 on the boundary triangles (Tri::integrales and
+    Tri::anisotropyEnergy, src/triangle.cpp:6-43) together with uniaxial K and cubic K3 in the
+    volume (src/tetra.cpp:171-208): element pass, assembled system and right-hand side, one solve
+    and the energies, engine against oracle. Ks changes the right-hand side and the anisotropy
+    energy, so the comparison covers the surface term."""
+    m = HostMesh.cuboid((25.0, 20.0, 15.0), (5, 4, 3), jitter=0.15, seed=6)
+    prm = dict(alpha=0.3, Ae=1.3e-11, Ms=8e5, K=4e4, uk=(1, 2, 0.5), K3=-1e4)
+    m.prepare({"volume": prm}, {"surface": dict(Ks=2e-3, uk=(0.3, -0.2, 1.0))})
+    u, v = state(m, seed=8)
+    p, g = pair(m, u, v)
+    H = np.array([2e4, -1e4, 3e4])
+    dt, pref = 2e-14, 1.046052
+    for o in (p, g):
+        o.base_projection(0.29)
+        o.prepare_elements(H, dt, pref)
+        o.solve(dt)
+    rp0, c0, v0, b0 = p.system()
+    rp1, c1, v1, b1 = g.system()
+    assert np.array_equal(rp0, rp1) and np.array_equal(c0, c1)
+    assert rel(v1, v0) < 1e-12 and rel(b1, b0) < 1e-12
+    un0, vn0, _, _ = p.get_state(nv.FG_NEXT)
+    un1, vn1, _, _ = g.get_state(nv.FG_NEXT)
+    assert rel(vn1, vn0) < 1e-8 and np.abs(un1 - un0).max() <= 1e-12
+    for o in (p, g):
+        o.calc_demag()
+    E0, E1 = p.energy(H), g.energy(H)
+    assert np.allclose(E1, E0, rtol=1e-8, atol=1e-8 * np.abs(E0).max())
+    m0 = HostMesh.cuboid((25.0, 20.0, 15.0), (5, 4, 3), jitter=0.15, seed=6)
+    m0.prepare({"volume": prm})
+    p0, _ = pair(m0, u, v)
+    p0.base_projection(0.29)
+    p0.prepare_elements(H, dt, pref)
+    p0.solve(dt)
+    assert rel(p0.system()[3], b0) > 1e-9  # the surface term is 1e5 times the 1e-12 tolerance above
+    p0.calc_demag()
+    assert abs(p0.energy(H)[1] - E0[1]) > 1e-3 * abs(E0[1])
+
+
 def test_energy_average_max_angle(small):
     u, v = state(small)
     p, g = pair(small, u, v)
