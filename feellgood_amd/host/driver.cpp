@@ -22,6 +22,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
+#include <csignal>
 #include <cstring>
 #include <ctime>
 #include <fstream>
@@ -39,6 +40,43 @@
 #include "settings.h"
 
 namespace {
+
+// SIGINT and SIGTERM set this flag; the time loop saves the state and terminates at the start of
+// its next step attempt (src/main.cpp:15-18,256-271; src/time_integration.cpp:99-134)
+volatile sig_atomic_t received_signal = 0;
+void signal_handler(int sig) { received_signal = sig; }
+
+// The user's defaults file, read before the settings (src/main.cpp:20-34,226):
+// $XDG_CONFIG_HOME/feellgood/default.yml, or $HOME/.config/feellgood/default.yml
+std::string user_defaults_filename()
+{
+    const char *x = getenv("XDG_CONFIG_HOME");
+    std::string home = (x && *x) ? x : "";
+    if (home.empty()) {
+        const char *h = getenv("HOME");
+        home = std::string(h ? h : "") + "/.config";
+    }
+    return home + "/feellgood/default.yml";
+}
+
+// Settings over defaults: maps merge key by key (the later file's value wins, nested maps merged
+// the same way), anything else is replaced -- the effect of reading the two files one after the
+// other into the same Settings (src/settings.cpp:596-616).
+fgh::Node merge_settings(fgh::Node base, const fgh::Node &over)
+{
+    if (base.kind != fgh::Node::Map || over.kind != fgh::Node::Map) return over;
+    for (const auto &f : over.fields) {
+        bool found = false;
+        for (auto &b : base.fields)
+            if (b.first == f.first) {
+                b.second = merge_settings(b.second, f.second);
+                found = true;
+                break;
+            }
+        if (!found) base.fields.push_back(f);
+    }
+    return base;
+}
 
 using fgh::Node;
 using fgh::SettingsError;
@@ -436,7 +474,7 @@ void initial_from_expressions(const Config &c, const fg_mesh &view, double *u)
     }
 }
 
-int run(const Config &c, unsigned seed, bool verbose, bool check_only)
+int run(const Config &c, unsigned seed, bool verbose, bool check_only, const std::string &steps_log)
 {
     char err[512] = {0};
     fgh_mesh *mesh = fgh_read_msh(c.mesh_file.c_str(), err, sizeof err);
@@ -659,9 +697,55 @@ int run(const Config &c, unsigned seed, bool verbose, bool check_only)
     double max_angle = 0;
     e.check(fg_max_angle(e.ctx, &max_angle), "fg_max_angle");
     const auto wall0 = std::chrono::steady_clock::now();
+    // one line per step attempt: its start time, dt, timing prefactor, basis angle, applied field,
+    // outcome (0 accepted, 1 solver failure, 2 rejected on max(du)) and BiCGSTAB iterations, so
+    // that the same sequence of decisions can be replayed elsewhere
+    std::ofstream slog;
+    if (!steps_log.empty()) {
+        slog.open(steps_log);
+        if (!slog) throw std::runtime_error("cannot open " + steps_log);
+        slog << "# t dt prefactor angle Hx Hy Hz outcome iterations\n" << std::setprecision(17);
+    }
+    auto log_attempt = [&](double r, int outcome, int iters) {
+        if (slog)
+            slog << t << ' ' << dt << ' ' << prefactor << ' ' << r << ' ' << Hext[0] << ' ' << Hext[1] << ' '
+                 << Hext[2] << ' ' << outcome << ' ' << iters << '\n';
+    };
+    // state saved on SIGINT / SIGTERM (exit_if_signal_received, src/time_integration.cpp:99-134)
+    auto exit_if_signal_received = [&]() {
+        if (!received_signal) return;
+        const char *name = received_signal == SIGINT ? "SIGINT" : received_signal == SIGTERM ? "SIGTERM" : "signal";
+        std::cout << "\nReceived " << name;
+        if (c.mag_every > 0) {
+            std::cout << ": saving the magnetization configuration...\n";
+            const std::string fname = base + "_at_exit.sol";
+            e.check(fg_get_state(e.ctx, FG_NEXT, u.data(), nullptr, phi.data(), nullptr), "fg_get_state");
+            std::ostringstream md;
+            md << common_metadata(c) << "## time: " << std::scientific << t << "\n"
+               << "## columns: idx\tmx\tmy\tmz\tphi" << (c.spin_acc ? "\tsx\tsy\tsz" : "") << "\n";
+            if (fgh_write_sol_s(mesh, fname.c_str(), md.str().c_str(), 7, u.data(), phi.data(),
+                                c.spin_acc ? sacc.data() : nullptr) != FG_OK)
+                throw std::runtime_error(std::string("writing ") + fname + ": " + fgh_last_error());
+            std::cout << "Magnetization configuration saved to " << fname << "\n";
+        } else {
+            std::cout << ": magnetization configuration not saved.\n";
+        }
+        std::cout << "Terminating.\n";
+        std::cout << "steps:             " << nt << " accepted, " << bad << " rejected, " << total_iter
+                  << " BiCGSTAB iterations\n" << std::flush;
+        fout.close();
+        slog.close();
+        const int sig = received_signal;
+        fg_destroy(e.ctx);
+        fgh_free(mesh);
+        signal(sig, SIG_DFL);  // the process ends with the signal's own status
+        raise(sig);
+        exit(1);
+    };
     for (int step_nb = 0; step_nb <= step_count; step_nb++) {
         const double t_target = t_initial + step_nb * t_step;
         while (t < t_target) {
+            exit_if_signal_received();
             set_dt(stepper(t_target - t));
             const bool last_step = (dt == t_target - t);
             if (verbose)
@@ -672,7 +756,8 @@ int run(const Config &c, unsigned seed, bool verbose, bool check_only)
                 status = 1;
                 goto bailout;
             }
-            e.check(fg_base_projection(e.ctx, next_angle()), "fg_base_projection");
+            const double r_attempt = next_angle();
+            e.check(fg_base_projection(e.ctx, r_attempt), "fg_base_projection");
             field_at(t);  // the field at the step's start, also used by its energies (src/time_integration.cpp:194-231)
             e.check(spacef ? fg_prepare_elements_amp(e.ctx, amp, dt, prefactor)
                            : fg_prepare_elements(e.ctx, Hext, dt, prefactor),
@@ -683,6 +768,7 @@ int run(const Config &c, unsigned seed, bool verbose, bool check_only)
             vmax = info.v_max;
             total_iter += info.iterations;
             if (rc == FG_SOLVE_FAILED) {
+                log_attempt(r_attempt, 1, info.iterations);
                 flag++;
                 bad++;
                 set_soft_limit(dt / 2);
@@ -691,10 +777,12 @@ int run(const Config &c, unsigned seed, bool verbose, bool check_only)
             const double dumax = dt * vmax;
             set_soft_limit((c.du_max / vmax) * 0.95);
             if (dumax > c.du_max) {
+                log_attempt(r_attempt, 2, info.iterations);
                 flag++;
                 bad++;
                 continue;
             }
+            log_attempt(r_attempt, 0, info.iterations);
             compute_all();
             nt++;
             flag = 0;
@@ -769,6 +857,8 @@ void usage(const char *prog)
               << "  -v, --verbose    print every integration step\n"
               << "  --seed N         fixed random seed (default: random)\n"
               << "  --check          read the settings and the mesh, print them, run nothing\n"
+              << "  --steps-log FILE write every step attempt (t, dt, prefactor, basis angle, field,\n"
+              << "                   outcome, iterations) to FILE\n"
               << "  -h, --help       this help\n"
               << "  -V, --version    version\n";
 }
@@ -779,7 +869,7 @@ int main(int argc, char *argv[])
 {
     bool verbose = false, fixed_seed = false, check_only = false;
     unsigned seed = 0;
-    std::string file;
+    std::string file, steps_log;
     for (int k = 1; k < argc; k++) {
         const std::string a = argv[k];
         if (a == "-v" || a == "--verbose") verbose = true;
@@ -787,6 +877,8 @@ int main(int argc, char *argv[])
         else if (a == "--seed" && k + 1 < argc) {
             fixed_seed = true;
             seed = (unsigned)std::strtoul(argv[++k], nullptr, 10);
+        } else if (a == "--steps-log" && k + 1 < argc) {
+            steps_log = argv[++k];
         } else if (a == "-h" || a == "--help") {
             usage(argv[0]);
             return 0;
@@ -823,10 +915,35 @@ int main(int argc, char *argv[])
             ss << in.rdbuf();
             text = ss.str();
         }
-        Config c = read_config(fgh::parse_settings(text));
+        fgh::Node root = fgh::parse_settings(text);
+        {  // the user's defaults first, then the settings over them
+            const std::string dname = user_defaults_filename();
+            std::ifstream din(dname);
+            if (din) {
+                std::ostringstream ds;
+                ds << din.rdbuf();
+                fgh::Node defaults;
+                try {
+                    defaults = fgh::parse_settings(ds.str());
+                } catch (const std::exception &ex) {
+                    throw std::runtime_error(dname + ": " + ex.what());
+                }
+                if (!defaults.is_null()) root = merge_settings(defaults, root);
+            }
+        }
+        Config c = read_config(root);
         c.settings_name = file == "-" ? "standard input" : file;
         std::cout << "random seed:       " << seed << "\n";
-        return run(c, seed, verbose, check_only);
+        // SIGINT and SIGTERM save the state before terminating (src/main.cpp:256-271)
+        struct sigaction action;
+        action.sa_handler = signal_handler;
+        action.sa_flags = 0;
+        sigemptyset(&action.sa_mask);
+        if (sigaction(SIGINT, &action, nullptr) == -1 || sigaction(SIGTERM, &action, nullptr) == -1) {
+            perror("sigaction");
+            return 1;
+        }
+        return run(c, seed, verbose, check_only, steps_log);
     } catch (const std::exception &ex) {
         std::cerr << "error: " << ex.what() << "\n";
         return 1;
