@@ -285,27 +285,43 @@ def run_gpu(args, w, m, u0, precision, guess):
                 prof_ms_per_step=(tp1 - tp0) * 1e3 / prof_steps, prof_steps=prof_steps)
 
 
-def step_parity(args, w, m, u0, precision, guess, steps=3):
-    """Per-step |dm| of the fast mode against the plain FP64 engine: both start every step from the
-    plain engine's u, v with the same basis angle, each with the demag potentials of its own
-    precision; max over `steps` steps (untimed)."""
+def step_accuracy(args, w, m, u0, precision, guess, steps=2):
+    """Per-step accuracy, untimed. The exact engine (FP64 direct-sum demag, pinned against the oracle)
+    steps from the initial state; before every step the plain FP64 FMM engine and the measured engine
+    are put on its u, v (each with the demag potentials of its own method), then all three take the
+    step with the same basis angle. Returns the max over `steps` steps of |dm| of each FMM engine
+    against the exact one and of the measured engine against the plain one, and the relative errors
+    of their demag energies."""
     from feellgood_amd import native as nv
+    ex_args = argparse.Namespace(**vars(args))
+    ex_args.demag = "direct"
+    ex = make_engine(ex_args, w, m, "fp64", None, False)
     ref = make_engine(args, w, m, "fp64", None, False)
     fast = make_engine(args, w, m, precision, guess, False)
-    a, b = Stepper(ref, w, u0), Stepper(fast, w, u0)
-    dm = 0.0
+    a, b, c = Stepper(ex, w, u0), Stepper(ref, w, u0), Stepper(fast, w, u0)
+    out = dict(dm_plain_vs_exact=0.0, dm_vs_exact=0.0, dm_vs_plain=0.0, edemag_plain_vs_exact=0.0,
+               edemag_vs_exact=0.0)
+    H = field(w)
     for _ in range(steps):
-        u, v, _, _ = ref.get_state(nv.FG_CURRENT)
-        fast.set_state(nv.FG_BOTH, u=u, v=v)
-        fast.calc_demag()
-        fast.evolution()
+        u, v, _, _ = ex.get_state(nv.FG_CURRENT)
+        for g in (ref, fast):
+            g.set_state(nv.FG_BOTH, u=u, v=v)
+            g.calc_demag()
+            g.evolution()
         r = a.angle()
-        a.step(r)
-        b.step(r)
-        dm = max(dm, float(np.abs(ref.get_state(nv.FG_CURRENT)[0] - fast.get_state(nv.FG_CURRENT)[0]).max()))
-    ref.close()
-    fast.close()
-    return dm
+        for st in (a, b, c):
+            st.step(r)
+        un = {k: g.get_state(nv.FG_CURRENT)[0] for k, g in (("ex", ex), ("ref", ref), ("fast", fast))}
+        # fg_energy's third component: the demag energy
+        E = {k: g.energy(H)[2] for k, g in (("ex", ex), ("ref", ref), ("fast", fast))}
+        out["dm_plain_vs_exact"] = max(out["dm_plain_vs_exact"], float(np.abs(un["ref"] - un["ex"]).max()))
+        out["dm_vs_exact"] = max(out["dm_vs_exact"], float(np.abs(un["fast"] - un["ex"]).max()))
+        out["dm_vs_plain"] = max(out["dm_vs_plain"], float(np.abs(un["fast"] - un["ref"]).max()))
+        out["edemag_plain_vs_exact"] = max(out["edemag_plain_vs_exact"], abs(E["ref"] / E["ex"] - 1))
+        out["edemag_vs_exact"] = max(out["edemag_vs_exact"], abs(E["fast"] / E["ex"] - 1))
+    for g in (ex, ref, fast):
+        g.close()
+    return {k: float(f"{v:.3e}") for k, v in out.items()}
 
 
 def measured_traffic(kernel, args):
@@ -369,42 +385,56 @@ def roofline(stats, sizes):
     return rl, per_kernel
 
 
-def cpu_baseline(w, m, u0, n_targets=256):
-    """The FP64 oracle on the host cores: one full step of element pass, assembly, BiCGSTAB, update
-    and energies on the whole mesh, plus the exact (direct-sum) demag of u and v evaluated for
-    n_targets nodes and scaled to all nodes."""
+def cpu_model():
+    try:
+        for ln in open("/proc/cpuinfo"):
+            if ln.startswith("model name"):
+                return ln.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(w, m, u0, fmm_order=8, fmm_leaf=64):
+    """CPU restatement of FeeLLGood's algorithm (TBB path unavailable): the FP64 oracle with OpenMP on
+    the host threads, one full step of the reference loop body on the whole mesh -- element pass,
+    assembly, BiCGSTAB, update, the demag of u and v by the oracle's octree FMM (same order and leaf
+    size as the GPU's), energies, evolution and max_angle -- after the compute_all of the start."""
     import oracle
     from feellgood_amd import native as nv
-    cores = os.cpu_count() or 1
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     t0 = time.perf_counter()
     p = oracle.Problem(m.mesh, tol=w["tol"], max_iter=10000)
-    p.set_state(nv.FG_BOTH, u=u0)
-    t_create = time.perf_counter() - t0
+    p.set_demag(fmm_order, fmm_leaf)
     H = field(w)
+    p.set_state(nv.FG_BOTH, u=u0)
+    p.calc_demag()
+    p.energy(H)
+    p.evolution()
+    t_setup = time.perf_counter() - t0
     dt = w["dt"]
+    part = {}
     t1 = time.perf_counter()
     p.base_projection(0.3)
     p.prepare_elements(H, dt, prefactor(dt))
     rc, info = p.solve(dt)
+    t2 = time.perf_counter()
+    p.calc_demag()
+    t3 = time.perf_counter()
     p.energy(H)
     p.evolution()
     p.max_angle()
-    t_step = time.perf_counter() - t1
-    targets = np.arange(0, m.n_nodes, max(1, m.n_nodes // n_targets), dtype=np.int32)[:n_targets]
-    t2 = time.perf_counter()
-    p.demag_sample(targets)
-    t_sample = time.perf_counter() - t2
-    S = 5 * m.n_tets + 4 * m.n_tris
-    t_demag = t_sample * m.n_nodes / len(targets)
-    per_step = t_step + t_demag
-    return {"value": round(1.0 / per_step, 8), "unit": "steps/s", "cores": cores, "kind": "port",
-            "sample": (f"FP64 oracle with OpenMP on {cores} host threads: one full step of element pass, "
-                       f"assembly, BiCGSTAB ({info.iterations} iterations), update, energies and max_angle "
-                       f"on the whole mesh ({t_step:.2f} s) plus the direct-sum demag of u and v (charges "
-                       f"for all {S} sources, potentials at {len(targets)} of {m.n_nodes} nodes in "
-                       f"{t_sample:.2f} s, scaled to all nodes: {t_demag:.1f} s); setup {t_create:.1f} s "
-                       f"excluded"),
-            "seconds_per_step": round(per_step, 3)}
+    t4 = time.perf_counter()
+    part = dict(elements_and_solve=t2 - t1, demag_fmm=t3 - t2, energy_evolution_max_angle=t4 - t3)
+    per_step = t4 - t1
+    return {"value": round(1.0 / per_step, 6), "unit": "steps/s", "cores": threads, "kind": "port",
+            "label": "CPU restatement of FeeLLGood's algorithm (TBB path unavailable)",
+            "cpu": cpu_model(), "nproc": os.cpu_count(),
+            "sample": (f"one full step of the FP64 oracle on {threads} OpenMP threads ({cpu_model()}): element "
+                       f"pass, assembly, BiCGSTAB ({info.iterations} iterations), update, the demag of u and v "
+                       f"by the oracle's octree FMM (order {fmm_order}, leaves of {fmm_leaf} sources), "
+                       f"energies, evolution and max_angle on the whole mesh; start-up {t_setup:.1f} s excluded"),
+            "seconds_per_step": round(per_step, 3), "seconds": {k: round(v, 3) for k, v in part.items()}}
 
 
 def default_guess_weights(w):
@@ -489,8 +519,9 @@ def main():
         plain = dict(ms=(time.perf_counter() - t1) * 1e3 / args.steps, iters=list(st.iters))
         g.close()
         print(f"[bench] plain fp64: {plain['ms']:.2f} ms/step, iterations {plain['iters']}", file=sys.stderr, flush=True)
-        dm = step_parity(args, w, m, u0, precision, guess)
-        print(f"[bench] per-step |dm| against plain fp64: {dm:.3e}", file=sys.stderr, flush=True)
+        acc = step_accuracy(args, w, m, u0, precision, guess)
+        dm = acc["dm_vs_plain"]
+        print(f"[bench] per-step accuracy: {acc}", file=sys.stderr, flush=True)
     sizes = dict(N=m.n_nodes, T=m.n_tets, F=m.n_tris, S=5 * m.n_tets + 4 * m.n_tris)
     # stencil entries N + 2E from the engine's own count
     from feellgood_amd.engine import Engine, options
@@ -542,6 +573,7 @@ def main():
                              "bicgstab_iterations_per_step": round(float(np.mean(plain["iters"])), 2)}
         out["speedup_vs_plain_fp64"] = round(plain["ms"] / res["ms_per_step"], 4)
         out["max_step_dm_vs_plain_fp64"] = dm
+        out["accuracy_vs_exact_demag"] = acc
     if not args.no_cpu:
         try:
             out["cpu_baseline"] = cpu_baseline(w, m, u0)

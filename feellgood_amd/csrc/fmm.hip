@@ -69,12 +69,12 @@ struct FmmPlan {
     // M2M and L2L as real linear operators per child octant (xlate_layout packing), and their
     // launch schedules: M2M parents per level with their children by octant (-1: none); L2L
     // parents per level with the children that receive a local expansion
-    double *d_Tm2m = nullptr, *d_Tl2l = nullptr;
+    void *d_Tm2m = nullptr, *d_Tl2l = nullptr;  // FP64, or FP32 in the mixed-precision mode
     // M2L on the matrix cores: the operator of every offset code (fragment order), the target
     // cells in tiles of 64 (one octant class per tile), each tile's offset codes in chunks of up
     // to M2L_CHUNK with the source cell of every (offset, target) (-1: none), and the partial sums
     // of every chunk, reduced per target in chunk order
-    double *d_Tm2l = nullptr, *d_m2l_part = nullptr;
+    void *d_Tm2l = nullptr, *d_m2l_part = nullptr;  // FP64, or FP32 in the mixed-precision mode
     std::vector<int32_t> m2l_tile_cell, m2l_tile_chunk, m2l_chunk_code, m2l_chunk_src;
     int32_t *d_m2l_tile_cell = nullptr, *d_m2l_tile_chunk = nullptr, *d_m2l_chunk_code = nullptr,
             *d_m2l_chunk_src = nullptr;
@@ -91,7 +91,7 @@ struct FmmPlan {
     double2 *d_pot = nullptr;  // [T] near-field + correction potential
     float *d_nf_valf = nullptr;  // FP32 coefficients (mixed precision; d_nf_val is then released)
     bool mixed = false;
-    double mscale = 1.0;         // multipoles times mscale (a power of two) in the FP32 pass
+    double mscale = 1.0;         // expansions times mscale (a power of two) in the FP32 mode
     int64_t m2p_pairs = 0, m2l_pairs = 0, lc_cols = 0;
     int64_t m2p_slots = 0, m2p_far_pairs = 0;  // lane slots the M2P launch mapping executes (useful pairs / slots = SIMD efficiency)
     // M2P launches: the groups of more than M2P_HALF_MAX targets (one cell per row), then the
@@ -102,7 +102,9 @@ struct FmmPlan {
     int64_t nf_nnz = 0;
     int32_t *d_tgt_node = nullptr;
     double4 *d_tgt_pos = nullptr;
-    double2 *d_M = nullptr, *d_L = nullptr;  // [ncell][2][nc]
+    // multipole and local expansions [ncell][2][nc]: double2, or float2 times mscale in the
+    // mixed-precision mode (every translation then runs in FP32 on the matrix cores)
+    void *d_M = nullptr, *d_L = nullptr;
     std::vector<int32_t> src_cells_lev_off, tgt_cells_lev_off;  // offsets into the lists
     std::vector<int32_t> src_cells, tgt_cells;
     // Launch schedule in global cell ids (a device plan may hold several independent trees, one
@@ -112,6 +114,10 @@ struct FmmPlan {
     std::vector<double> cell_iw;  // 1 / cell width
     std::vector<int32_t> m2m_lev_off, m2m_cells, m2l_cells, l2l_lev_off, l2l_cells;
     int32_t *d_leaf_cell = nullptr, *d_m2m_cells = nullptr;
+    // P2M source records in tree order (k_fmm_p2m): double4 / float4 positions with the weight
+    // w0, codes, and the two further weights of each triangle Gauss point (double2 / float2)
+    void *d_p2m_pos = nullptr, *d_p2m_tw = nullptr;
+    int32_t *d_p2m_code = nullptr;
     int ncell() const { return (int)cen.size(); }
 };
 
@@ -144,6 +150,10 @@ namespace fg {
 // truncation error of a cell three widths away at this degree stays below that of the cells two
 // widths away at degree p (the film's FMM error grows by at most 5 % over one degree everywhere).
 constexpr int fmm_far_degree(int p) { return p <= 6 ? p - 2 : p == 8 ? 5 : 6; }
+// Degree of the far field of the second charge set (charges of v) in the mixed-precision mode: its
+// potential enters the step only through theta dt H_d(v) (src/tetra.cpp:289), a term some 1e-4 of
+// H_d(u) at the benchmark steps, so its multipole evaluation is cut to this degree.
+constexpr int fmm_v_degree(int p) { return p <= 4 ? p : 4; }
 
 // Host construction of the plan from source positions (original order) and target nodes.
 std::string fmm_build_host(FmmPlan &P, const std::vector<double4> &src, const std::vector<int32_t> &tnode,
@@ -755,62 +765,151 @@ __device__ __forceinline__ double sum8_dpp(double v)
     v += dpp_mov<0x141>(v);  // row_half_mirror
     return v;
 }
+template <int CTRL>
+__device__ __forceinline__ float dpp_mov(float v)
+{
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xf, 0xf, true));
+}
+__device__ __forceinline__ float sum8_dpp(float v)
+{
+    v += dpp_mov<0xB1>(v);
+    v += dpp_mov<0x4E>(v);
+    v += dpp_mov<0x141>(v);
+    return v;
+}
 
-// P2M: wavefronts over 64 / G source leaves; G lanes per leaf, each a strided subset of the leaf's
-// sources, the next source's position and charge loaded while the current one is accumulated.
-// Launch H accumulates the coefficient columns of half H (a balanced split,
-// p2m_half_mask), so each lane holds about half of one expansion; the lanes' sums are combined
-// with cross-lane sums. blockIdx.y = charge set.
-template <int P, int G, unsigned MASK>
-__device__ __forceinline__ void p2m_half(int li, int lane, int set, bool act, int cell, const double4 *__restrict__ cen,
-                                         const int32_t *__restrict__ sb, const int32_t *__restrict__ se,
-                                         const double4 *__restrict__ spos, const double2 *__restrict__ q,
-                                         double2 *__restrict__ M)
+// P2M from the element values (the charges are linear in them, as in the near field): source s of
+// a leaf carries its position and weight w0 (pos[s]) and code[s], a tetrahedron t >= 0, whose charges
+// are w0 qe[t], or ~k for Gauss point k = 4 f + g of triangle f, whose charges are
+// w0 qe[e] + tw[k].x qe[e + 1] + tw[k].y qe[e + 2] with e = Tm + 3 f (the vertex values u_a . n and
+// weights dMs w_g lambda_a(g)). FP64 (S4 = double4, positions absolute, F = double) or FP32 (S4 =
+// float4, positions relative to the leaf centre in cell widths and weights times mscale,
+// F = float). G lanes per leaf, each a strided subset of its sources, the next source loaded
+// while the current one is accumulated; both charge sets at once, columns of MASK only (FP64: two
+// launches, one per half of the columns; FP32: one launch), combined by cross-lane sums.
+template <class S4> struct P2mSrc;
+template <> struct P2mSrc<double4> { typedef double2 W2; };
+template <> struct P2mSrc<float4> { typedef float2 W2; };
+
+template <int P, int G, unsigned MASK, class F, class S4>
+__global__ __launch_bounds__(64) void k_fmm_p2m(int nleaf, int Tm, const int32_t *__restrict__ leaf_cell,
+                                                const double4 *__restrict__ cen, const int32_t *__restrict__ sb,
+                                                const int32_t *__restrict__ se, const S4 *__restrict__ pos,
+                                                const int32_t *__restrict__ code,
+                                                const typename P2mSrc<S4>::W2 *__restrict__ tw,
+                                                const double2 *__restrict__ qe, F *__restrict__ M)
 {
     constexpr int NCo = fcount(P), NH = p2m_half_count<P>(MASK);
-    double acc[2 * NH];
+    constexpr bool REL = std::is_same<S4, float4>::value;  // positions already relative, in cell widths
+    const int lane = threadIdx.x & (G - 1);
+    const int li = (blockIdx.x * 64 + threadIdx.x) / G;
+    const bool act = li < nleaf;
+    const int cell = act ? leaf_cell[li] : 0;
+    F au[2 * NH], av[2 * NH];
 #pragma unroll
-    for (int k = 0; k < 2 * NH; k++) acc[k] = 0.0;
+    for (int k = 0; k < 2 * NH; k++) au[k] = av[k] = F(0);
     const double4 c = cen[cell];
     const double iw = c.w;
     const int kb = act ? sb[li] + lane : 0, ke = act ? se[li] : 0;
-    double4 x = kb < ke ? spos[kb] : make_double4(0, 0, 0, 0);
-    double qn = kb < ke ? (set ? q[kb].y : q[kb].x) : 0.0;
+    S4 x = kb < ke ? pos[kb] : S4{0, 0, 0, 0};
+    int cd = kb < ke ? code[kb] : 0;
     for (int k = kb; k < ke; k += G) {
-        const double4 xc = x;
-        const double qc = qn;
-        if (k + G < ke) {  // next source's position and charge, loaded under this one's work
-            x = spos[k + G];
-            qn = set ? q[k + G].y : q[k + G].x;
+        const S4 xc = x;
+        const int cc = cd;
+        if (k + G < ke) {  // the next source, loaded under this one's work
+            x = pos[k + G];
+            cd = code[k + G];
         }
-        p2m_accumulate_cols<P, MASK>((xc.x - c.x) * iw, (xc.y - c.y) * iw, (xc.z - c.z) * iw, qc, acc);
+        F qu, qv;
+        if (cc >= 0) {
+            const double2 e = qe[cc];
+            qu = xc.w * (F)e.x;
+            qv = xc.w * (F)e.y;
+        } else {
+            const int gk = ~cc, e0 = Tm + 3 * (gk >> 2);
+            const auto w12 = tw[gk];
+            const double2 a = qe[e0], b = qe[e0 + 1], d = qe[e0 + 2];
+            qu = xc.w * (F)a.x + w12.x * (F)b.x + w12.y * (F)d.x;
+            qv = xc.w * (F)a.y + w12.x * (F)b.y + w12.y * (F)d.y;
+        }
+        F rx, ry, rz;
+        if constexpr (REL) {
+            rx = xc.x;
+            ry = xc.y;
+            rz = xc.z;
+        } else {
+            rx = (xc.x - c.x) * iw;
+            ry = (xc.y - c.y) * iw;
+            rz = (xc.z - c.z) * iw;
+        }
+        p2m_accumulate2_cols<P, MASK, F>(rx, ry, rz, qu, qv, au, av);
     }
     static_assert(G == 8, "the cross-lane sum spans 8 lanes");
 #pragma unroll
-    for (int k = 0; k < 2 * NH; k++) acc[k] = sum8_dpp(acc[k]);
+    for (int k = 0; k < 2 * NH; k++) {
+        au[k] = sum8_dpp(au[k]);
+        av[k] = sum8_dpp(av[k]);
+    }
     if (act && lane == 0) {
-        double2 *dst = M + ((size_t)cell * 2 + set) * NCo;
+        F *du = M + (size_t)cell * 4 * NCo, *dv = du + 2 * NCo;
 #pragma unroll
         for (int m = 0; m <= P; m++) {
             if (!((MASK >> m) & 1)) continue;
             const int o = p2m_col_off<P>(MASK, m);
 #pragma unroll
-            for (int n = m; n <= P; n++)
-                dst[fidx(n, m)] = make_double2(acc[2 * (o + n - m)] * p2m_scale(n, m), acc[2 * (o + n - m) + 1] * p2m_scale(n, m));
+            for (int n = m; n <= P; n++) {
+                const F sc = (F)p2m_scale(n, m);
+                const int i = 2 * (o + n - m);
+                du[2 * fidx(n, m)] = au[i] * sc;
+                du[2 * fidx(n, m) + 1] = au[i + 1] * sc;
+                dv[2 * fidx(n, m)] = av[i] * sc;
+                dv[2 * fidx(n, m) + 1] = av[i + 1] * sc;
+            }
         }
     }
 }
 
-template <int P, int G, int H>
-__global__ __launch_bounds__(64) void k_fmm_p2m(int nleaf, const int32_t *__restrict__ leaf_cell,
-                                                const double4 *__restrict__ cen, const int32_t *__restrict__ sb,
-                                                const int32_t *__restrict__ se, const double4 *__restrict__ spos,
-                                                const double2 *__restrict__ q, double2 *__restrict__ M)
+// The P2M source records from the sources in tree order: position (and, in FP32, relative to the
+// leaf centre in cell widths), weight w0, code and the triangle weights (src_w: the near field's
+// per-source weights, (w_g, 0, 0) for a tetrahedron point and dMs w_g lambda_a(g) for a triangle
+// point). One thread per leaf.
+template <class S4>
+__global__ void k_p2m_records(int nleaf, int Tm, int64_t S5, const int32_t *__restrict__ leaf_cell,
+                              const double4 *__restrict__ cen, const int32_t *__restrict__ sb,
+                              const int32_t *__restrict__ se, const double4 *__restrict__ spos,
+                              const double4 *__restrict__ src_w, double wscale, S4 *__restrict__ pos)
 {
-    const int lane = threadIdx.x & (G - 1);
-    const int li = (blockIdx.x * 64 + threadIdx.x) / G;
-    const bool act = li < nleaf;
-    p2m_half<P, G, p2m_half_mask<P>(H)>(li, lane, blockIdx.y, act, act ? leaf_cell[li] : 0, cen, sb, se, spos, q, M);
+    const int li = blockIdx.x * blockDim.x + threadIdx.x;
+    if (li >= nleaf) return;
+    const double4 c = cen[leaf_cell[li]];
+    for (int k = sb[li]; k < se[li]; k++) {
+        const double4 y = spos[k];
+        const double w = src_w[k].x * wscale;
+        if constexpr (std::is_same<S4, float4>::value)
+            pos[k] = make_float4((float)((y.x - c.x) * c.w), (float)((y.y - c.y) * c.w), (float)((y.z - c.z) * c.w),
+                                 (float)w);
+        else
+            pos[k] = make_double4(y.x, y.y, y.z, w);
+    }
+}
+
+// Codes (tetrahedron index, or ~(4 f + g) for Gauss point g of triangle f) and triangle weights,
+// one thread per source in the original order (tetrahedron points first, then triangle points).
+template <class W2>
+__global__ void k_p2m_codes(int Tm, int Fm, const int64_t *__restrict__ rank, const double4 *__restrict__ src_w,
+                            double wscale, int32_t *__restrict__ code, W2 *__restrict__ tw)
+{
+    const int64_t s = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (s < (int64_t)5 * Tm) {
+        code[rank[s]] = (int32_t)(s / 5);
+    } else if (s < (int64_t)5 * Tm + (int64_t)4 * Fm) {
+        const int k = (int)(s - (int64_t)5 * Tm);
+        const int64_t r = rank[s];
+        code[r] = ~k;
+        const double4 w = src_w[r];
+        tw[k].x = w.y * wscale;
+        tw[k].y = w.z * wscale;
+    }
 }
 
 // Translations between expansions of degree P in real form: NR = 2 fcount(P) reals, real j being
@@ -823,6 +922,20 @@ __global__ __launch_bounds__(64) void k_fmm_p2m(int nleaf, const int32_t *__rest
 // output degree n to input degrees <= n and L2L to degrees >= n, so each row tile runs only the
 // k-steps [kb, ke) its degrees need.
 typedef double f64x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+// The 16x16x4 matrix-core products in FP64 and in FP32 take the same A and B fragments; the four
+// accumulators of lane l hold column l % 16 of D, rows l / 16 + 4 t (FP64) or 4 (l / 16) + t (FP32).
+template <class F> struct Mfma;
+template <> struct Mfma<double> {
+    typedef f64x4 V;
+    __device__ static V mma(double a, double b, V c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
+    __device__ static constexpr int row(int kq, int t) { return kq + 4 * t; }
+};
+template <> struct Mfma<float> {
+    typedef f32x4 V;
+    __device__ static V mma(float a, float b, V c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
+    __device__ static constexpr int row(int kq, int t) { return 4 * kq + t; }
+};
 template <int P> struct XL {
     static constexpr int NCo = fcount(P), NR = 2 * NCo, NT = (NR + 15) / 16, NKS = (NR + 3) / 4;
     static constexpr int frag_len() { return 8 * NT * NKS * 64; }
@@ -845,11 +958,14 @@ template <int P> struct XL {
 // takes as long as one block). M2M (K = 0): parent
 // = sum over octants of T_o child_o, the four partial tiles summed through LDS and written to the
 // parent. L2L (K = 1): child_o += T_o parent for each octant with a receiving child.
-template <int P, int K, int R>
+template <int P, int K, int R, class F>
 __device__ __forceinline__ void xlate_tile(int n, const int32_t *__restrict__ cells, const int32_t *__restrict__ child,
-                                           const double *__restrict__ Top, double *__restrict__ X, f64x4 (*red)[64])
+                                           const F *__restrict__ Top, F *__restrict__ X,
+                                           typename Mfma<F>::V (*red)[64])
 {
     using L = XL<P>;
+    using MF = Mfma<F>;
+    using V = typename MF::V;
     constexpr int NR = L::NR, NT = L::NT, NKS = L::NKS, kb = L::kb(K, R), ke = L::ke(K, R);
     const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
     const int s = (blockIdx.x / NT) & 1, ct = blockIdx.x / (2 * NT);
@@ -857,42 +973,42 @@ __device__ __forceinline__ void xlate_tile(int n, const int32_t *__restrict__ ce
     const int pi = ct * 16 + col;
     const bool act = pi < n;
     const int pc = act ? cells[pi] : 0;
-    const double *A = Top + (size_t)R * NKS * 64 + l;
-    double pv[NKS];  // L2L: the parent's fragments, the same for every octant
+    const F *A = Top + (size_t)R * NKS * 64 + l;
+    F pv[NKS];  // L2L: the parent's fragments, the same for every octant
     if (K == 1) {
-        const double *xs = X + ((size_t)pc * 2 + s) * NR;
+        const F *xs = X + ((size_t)pc * 2 + s) * NR;
 #pragma unroll
         for (int ks = 0; ks < NKS; ks++) {
             if (ks < kb || ks >= ke) continue;
             const int k = 4 * ks + kq;
-            pv[ks] = act && k < NR ? xs[k] : 0.0;
+            pv[ks] = act && k < NR ? xs[k] : F(0);
         }
     }
-    f64x4 acc = {0.0, 0.0, 0.0, 0.0};
+    V acc = {F(0), F(0), F(0), F(0)};
     for (int oo = 0; oo < 2; oo++) {
         const int o = 2 * w + oo;
         const int ch = act ? child[(size_t)pi * 8 + o] : -1;
-        const double *Ao = A + (size_t)o * NT * NKS * 64;
-        double av[NKS], bv[NKS];
-        const double *xs = X + ((size_t)(ch < 0 ? 0 : ch) * 2 + s) * NR;
+        const F *Ao = A + (size_t)o * NT * NKS * 64;
+        F av[NKS], bv[NKS];
+        const F *xs = X + ((size_t)(ch < 0 ? 0 : ch) * 2 + s) * NR;
 #pragma unroll
         for (int ks = 0; ks < NKS; ks++) {
             if (ks < kb || ks >= ke) continue;
             const int k = 4 * ks + kq;
             av[ks] = Ao[ks * 64];
-            bv[ks] = K == 1 ? pv[ks] : (ch >= 0 && k < NR ? xs[k] : 0.0);
+            bv[ks] = K == 1 ? pv[ks] : (ch >= 0 && k < NR ? xs[k] : F(0));
         }
-        if (K == 1) acc = f64x4{0.0, 0.0, 0.0, 0.0};
+        if (K == 1) acc = V{F(0), F(0), F(0), F(0)};
 #pragma unroll
         for (int ks = 0; ks < NKS; ks++) {
             if (ks < kb || ks >= ke) continue;
-            acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av[ks], bv[ks], acc, 0, 0, 0);
+            acc = MF::mma(av[ks], bv[ks], acc);
         }
         if (K == 1 && ch >= 0) {
-            double *d = X + ((size_t)ch * 2 + s) * NR;
+            F *d = X + ((size_t)ch * 2 + s) * NR;
 #pragma unroll
             for (int t = 0; t < 4; t++) {
-                const int r = 16 * R + kq + 4 * t;
+                const int r = 16 * R + MF::row(kq, t);
                 if (r < NR) d[r] += acc[t];
             }
         }
@@ -903,38 +1019,38 @@ __device__ __forceinline__ void xlate_tile(int n, const int32_t *__restrict__ ce
         if (w == 0 && act) {
 #pragma unroll
             for (int q = 0; q < 3; q++) acc += red[q][l];
-            double *d = X + ((size_t)pc * 2 + s) * NR;
+            F *d = X + ((size_t)pc * 2 + s) * NR;
 #pragma unroll
             for (int t = 0; t < 4; t++) {
-                const int r = 16 * R + kq + 4 * t;
+                const int r = 16 * R + MF::row(kq, t);
                 if (r < NR) d[r] = acc[t];
             }
         }
     }
 }
 
-template <int P, int K, int R>
+template <int P, int K, int R, class F>
 __device__ __forceinline__ void xlate_tile_dispatch(int r, int n, const int32_t *__restrict__ cells,
-                                                    const int32_t *__restrict__ child, const double *__restrict__ Top,
-                                                    double *__restrict__ X, f64x4 (*red)[64])
+                                                    const int32_t *__restrict__ child, const F *__restrict__ Top,
+                                                    F *__restrict__ X, typename Mfma<F>::V (*red)[64])
 {
     if constexpr (R < XL<P>::NT) {
         if (r == R)
-            xlate_tile<P, K, R>(n, cells, child, Top, X, red);
+            xlate_tile<P, K, R, F>(n, cells, child, Top, X, red);
         else
-            xlate_tile_dispatch<P, K, R + 1>(r, n, cells, child, Top, X, red);
+            xlate_tile_dispatch<P, K, R + 1, F>(r, n, cells, child, Top, X, red);
     }
 }
 
-template <int P, int K>
+template <int P, int K, class F>
 __global__ __launch_bounds__(256) void k_fmm_xlate(int n, const int32_t *__restrict__ cells,
-                                                   const int32_t *__restrict__ child, const double *__restrict__ Top,
-                                                   double *__restrict__ X)
+                                                   const int32_t *__restrict__ child, const F *__restrict__ Top,
+                                                   F *__restrict__ X)
 {
     constexpr int NT = XL<P>::NT;
-    __shared__ f64x4 red[3][64];  // partial tiles of wavefronts 1..3 (M2M)
+    __shared__ typename Mfma<F>::V red[3][64];  // partial tiles of wavefronts 1..3 (M2M)
     const int R = blockIdx.x % NT;
-    xlate_tile_dispatch<P, K, 0>(R, n, cells, child, Top, X, red);
+    xlate_tile_dispatch<P, K, 0, F>(R, n, cells, child, Top, X, red);
 }
 
 // M2L on the matrix cores. The translation from a source cell to a target cell at offset d (in
@@ -947,67 +1063,69 @@ __global__ __launch_bounds__(256) void k_fmm_xlate(int n, const int32_t *__restr
 // none). The chunk's partial sums go to a buffer; k_fmm_m2l_reduce adds a tile's chunks in order
 // and writes L = (1/W) sum (deterministic).
 constexpr int M2L_CHUNK = 4;
-template <int P>
+template <int P, class F>
 __global__ __launch_bounds__(256) void k_fmm_m2l_mfma(const int32_t *__restrict__ tile_chunk, int ntiles,
                                                       const int32_t *__restrict__ chunk_code,
                                                       const int32_t *__restrict__ chunk_src,
-                                                      const double *__restrict__ Tm2l, const double *__restrict__ X,
-                                                      double *__restrict__ part)
+                                                      const F *__restrict__ Tm2l, const F *__restrict__ X,
+                                                      F *__restrict__ part)
 {
     using L = XL<P>;
+    using MF = Mfma<F>;
+    using V = typename MF::V;
     constexpr int NR = L::NR, NT = L::NT, NKS = L::NKS, OL = NT * NKS * 64;
-    __shared__ double sA[OL];
+    __shared__ F sA[OL];
     const int ch = blockIdx.x, tid = threadIdx.x, w = tid >> 6, l = tid & 63;
     const int col = l & 15, kq = l >> 4;
-    f64x4 acc[NT][2];
+    V acc[NT][2];
 #pragma unroll
-    for (int R = 0; R < NT; R++) acc[R][0] = acc[R][1] = f64x4{0.0, 0.0, 0.0, 0.0};
+    for (int R = 0; R < NT; R++) acc[R][0] = acc[R][1] = V{F(0), F(0), F(0), F(0)};
     for (int i = 0; i < M2L_CHUNK; i++) {
         const int code = chunk_code[ch * M2L_CHUNK + i];
         if (code < 0) break;  // uniform: the chunk's offsets are packed first
         __syncthreads();
-        const double *A = Tm2l + (size_t)code * OL;
+        const F *A = Tm2l + (size_t)code * OL;
         for (int e = tid; e < OL; e += 256) sA[e] = A[e];
         __syncthreads();
         const int src = chunk_src[((size_t)ch * M2L_CHUNK + i) * 64 + 16 * w + col];
-        double bv[2][NKS];
+        F bv[2][NKS];
 #pragma unroll
         for (int s = 0; s < 2; s++) {
-            const double *xs = X + ((size_t)(src < 0 ? 0 : src) * 2 + s) * NR;
+            const F *xs = X + ((size_t)(src < 0 ? 0 : src) * 2 + s) * NR;
 #pragma unroll
             for (int ks = 0; ks < NKS; ks++) {
                 const int k = 4 * ks + kq;
-                bv[s][ks] = src >= 0 && k < NR ? xs[k] : 0.0;
+                bv[s][ks] = src >= 0 && k < NR ? xs[k] : F(0);
             }
         }
 #pragma unroll
         for (int R = 0; R < NT; R++)
 #pragma unroll
             for (int ks = 0; ks < NKS; ks++) {
-                const double a = sA[(R * NKS + ks) * 64 + l];
-                acc[R][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, bv[0][ks], acc[R][0], 0, 0, 0);
-                acc[R][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, bv[1][ks], acc[R][1], 0, 0, 0);
+                const F a = sA[(R * NKS + ks) * 64 + l];
+                acc[R][0] = MF::mma(a, bv[0][ks], acc[R][0]);
+                acc[R][1] = MF::mma(a, bv[1][ks], acc[R][1]);
             }
     }
     // partial sums of this chunk: [chunk][target slot][set][NR]
-    double *d = part + ((size_t)ch * 64 + 16 * w + col) * 2 * NR;
+    F *d = part + ((size_t)ch * 64 + 16 * w + col) * 2 * NR;
 #pragma unroll
     for (int s = 0; s < 2; s++)
 #pragma unroll
         for (int R = 0; R < NT; R++)
 #pragma unroll
             for (int t = 0; t < 4; t++) {
-                const int r = 16 * R + kq + 4 * t;
+                const int r = 16 * R + MF::row(kq, t);
                 if (r < NR) d[s * NR + r] = acc[R][s][t];
             }
 }
 
 // L of every M2L target: (1/W) times the sum of its tile's chunk partials, in chunk order.
-template <int P>
+template <int P, class F>
 __global__ __launch_bounds__(256) void k_fmm_m2l_reduce(int ntiles, const int32_t *__restrict__ tile_cell,
                                                         const int32_t *__restrict__ tile_chunk,
                                                         const double4 *__restrict__ cen,
-                                                        const double *__restrict__ part, double *__restrict__ Lx)
+                                                        const F *__restrict__ part, F *__restrict__ Lx)
 {
     constexpr int NR = 2 * fcount(P);
     const int64_t id = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
@@ -1019,7 +1137,7 @@ __global__ __launch_bounds__(256) void k_fmm_m2l_reduce(int ntiles, const int32_
     if (cell < 0) return;
     double sum = 0.0;
     for (int c = tile_chunk[tile]; c < tile_chunk[tile + 1]; c++) sum += part[((size_t)c * 64 + slot) * 2 * NR + sr];
-    Lx[(size_t)cell * 2 * NR + sr] = sum * cen[cell].w;
+    Lx[(size_t)cell * 2 * NR + sr] = (F)(sum * cen[cell].w);
 }
 
 __device__ inline double wsum64(double v)
@@ -1201,14 +1319,14 @@ __device__ __forceinline__ void static_for(F &&f)
     }
 }
 
-// M2P of the two expansions of a row's cell, degree Q, at one target per lane (x in cell widths):
-// su = sum M_u,n^m conj(I_n^m(x)), sv likewise. Coefficient double k of a set (k = 2 fidx(n, m),
-// + 1 for the imaginary part) sits where fmac_coef<H, k> reads it. The irregular
-// harmonics are streamed column by column (diagonal I_m^m, then I_{m+1}^m and the three-term
-// recurrence in n), the recurrence factors of each degree shared by all columns; the m = 0 column
-// is real (M_n^0 and I_n^0 are real for real charges).
-template <int Q, int H, class R, int NQ>
-__device__ __forceinline__ void m2p_rows(const R (&cu)[NQ], const R (&cv)[NQ], R x, R y, R z, R &su, R &sv)
+// M2P of the two expansions of a row's cell at one target per lane (x in cell widths), the first
+// to degree Q, the second to degree QV <= Q: su = sum M_u,n^m conj(I_n^m(x)), sv likewise.
+// Coefficient k of a set (k = 2 fidx(n, m), + 1 for the imaginary part) sits where
+// fmac_coef<H, k> reads it. The irregular harmonics are streamed column by column (diagonal
+// I_m^m, then I_{m+1}^m and the three-term recurrence in n), the recurrence factors of each degree
+// shared by all columns; the m = 0 column is real (M_n^0 and I_n^0 are real for real charges).
+template <int Q, int QV, int H, class R, int NQ, int NQV>
+__device__ __forceinline__ void m2p_rows(const R (&cu)[NQ], const R (&cv)[NQV], R x, R y, R z, R &su, R &sv)
 {
     const R r2 = x * x + y * y + z * z;
     const R ir2 = fast_inv(r2), zr = z * ir2, zr2 = zr + zr, ir22 = ir2 + ir2;
@@ -1234,12 +1352,12 @@ __device__ __forceinline__ void m2p_rows(const R (&cu)[NQ], const R (&cv)[NQ], R
         constexpr int k1 = 2 * fidx(1, 0);
         R p2 = dre, p1 = zr * dre;
         fmac_coef<H, k1>(au, cu, p1);
-        fmac_coef<H, k1>(av, cv, p1);
+        if constexpr (QV >= 1) fmac_coef<H, k1>(av, cv, p1);
         static_for<2, Q + 1>([&](auto nc) {
             constexpr int n = decltype(nc)::value, k = 2 * fidx(n, 0);
             const R c = tz[n] * p1 - un[n] * p2;
             fmac_coef<H, k>(au, cu, c);
-            fmac_coef<H, k>(av, cv, c);
+            if constexpr (n <= QV) fmac_coef<H, k>(av, cv, c);
             p2 = p1;
             p1 = c;
         });
@@ -1256,23 +1374,29 @@ __device__ __forceinline__ void m2p_rows(const R (&cu)[NQ], const R (&cv)[NQ], R
         dim = nim;
         fmac_coef<H, kd>(hu, cu, dre);
         fmac_coef<H, kd + 1>(hu, cu, dim);
-        fmac_coef<H, kd>(hv, cv, dre);
-        fmac_coef<H, kd + 1>(hv, cv, dim);
+        if constexpr (m <= QV) {
+            fmac_coef<H, kd>(hv, cv, dre);
+            fmac_coef<H, kd + 1>(hv, cv, dim);
+        }
         if constexpr (m + 1 <= Q) {
             constexpr int k1 = 2 * fidx(m + 1, m);
             R p2re = dre, p2im = dim, p1re = tz[m + 1] * dre, p1im = tz[m + 1] * dim;
             fmac_coef<H, k1>(hu, cu, p1re);
             fmac_coef<H, k1 + 1>(hu, cu, p1im);
-            fmac_coef<H, k1>(hv, cv, p1re);
-            fmac_coef<H, k1 + 1>(hv, cv, p1im);
+            if constexpr (m + 1 <= QV) {
+                fmac_coef<H, k1>(hv, cv, p1re);
+                fmac_coef<H, k1 + 1>(hv, cv, p1im);
+            }
             static_for<m + 2, Q + 1>([&](auto nc) {
                 constexpr int n = decltype(nc)::value, k = 2 * fidx(n, m);
                 const R bb = un[n] - vm;
                 const R cre = tz[n] * p1re - bb * p2re, cim = tz[n] * p1im - bb * p2im;
                 fmac_coef<H, k>(hu, cu, cre);
                 fmac_coef<H, k + 1>(hu, cu, cim);
-                fmac_coef<H, k>(hv, cv, cre);
-                fmac_coef<H, k + 1>(hv, cv, cim);
+                if constexpr (n <= QV) {
+                    fmac_coef<H, k>(hv, cv, cre);
+                    fmac_coef<H, k + 1>(hv, cv, cim);
+                }
                 p2re = p1re;
                 p2im = p1im;
                 p1re = cre;
@@ -1287,10 +1411,11 @@ __device__ __forceinline__ void m2p_rows(const R (&cu)[NQ], const R (&cv)[NQ], R
 // The source cell of a row (H = 1) or half row (H = 2) in a step: both expansions in
 // row-broadcast layout (degree Q of order-P storage), its centre and 1/W, and whether it applies
 // to this lane's target.
-template <int P, int Q, int H>
+template <int P, int Q, int QV, int H, class R>
 struct M2pCell {
     static constexpr int LPC = 16 / H, NK = 2 * fcount(Q), NQ = (NK + LPC - 1) / LPC;
-    double raw[2][NQ];
+    static constexpr int NKV = 2 * fcount(QV), NQV = (NKV + LPC - 1) / LPC;
+    R raw[NQ], rawv[NQV];
     double4 c;
     bool on;
 };
@@ -1298,14 +1423,14 @@ struct M2pCell {
 // List entries [b, e) of a group, evaluated to degree Q <= P (expansions stored to P): step s puts
 // entry b + s cps + slot in each cell slot (other wavefronts on the SIMD hide the loads' latency).
 // Every lane runs every step (invalid lanes evaluate at a dummy offset and add 0).
-template <int P, int Q, int H, class R>
+template <int P, int Q, int QV, int H, class R>
 __device__ __forceinline__ void m2p_range(int b, int e, int cps, int slot, bool tv, int oc, double4 y,
                                           const int32_t *__restrict__ u_cell, const uint8_t *__restrict__ u_mask,
-                                          const double4 *__restrict__ cen, const double *__restrict__ Md,
-                                          double mscale, double &pu, double &pv)
+                                          const double4 *__restrict__ cen, const R *__restrict__ Md, double &pu,
+                                          double &pv)
 {
-    using C = M2pCell<P, Q, H>;
-    constexpr int NCo = fcount(P), LPC = C::LPC, NK = C::NK, NQ = C::NQ;
+    using C = M2pCell<P, Q, QV, H, R>;
+    constexpr int NCo = fcount(P), LPC = C::LPC, NK = C::NK, NQ = C::NQ, NKV = C::NKV, NQV = C::NQV;
     const int j = threadIdx.x & (LPC - 1);  // lane within the cell's lanes
     const int nst = (e - b + cps - 1) / cps;  // uniform over the wavefront
     auto fetch = [&](int s, C &cl) {
@@ -1315,32 +1440,32 @@ __device__ __forceinline__ void m2p_range(int b, int e, int cps, int slot, bool 
         const int cell = u_cell[ec];
         cl.c = cen[cell];
         cl.on = tv && ev && ((u_mask[ec] >> oc) & 1);
-        const double *src = Md + (size_t)cell * 4 * NCo;
+        const R *src = Md + (size_t)cell * 4 * NCo;
 #pragma unroll
         for (int q = 0; q < NQ; q++) {
             // lanes past the last coefficient repeat it; no DPP read takes them
-            const int k = min(LPC * q + j, NK - 1);
-            cl.raw[0][q] = src[k];
-            cl.raw[1][q] = src[2 * NCo + k];
+            cl.raw[q] = src[min(LPC * q + j, NK - 1)];
         }
+#pragma unroll
+        for (int q = 0; q < NQV; q++) cl.rawv[q] = src[2 * NCo + min(LPC * q + j, NKV - 1)];
     };
     auto eval = [&](C &cl) {
-        R cf[2][NQ];
+        R cf[NQ], cfv[NQV];
 #pragma unroll
-        for (int st = 0; st < 2; st++)
+        for (int q = 0; q < NQ; q++) {
+            cf[q] = cl.raw[q];
+            row_fence(cf[q]);
+        }
 #pragma unroll
-            for (int q = 0; q < NQ; q++) {
-                if constexpr (sizeof(R) == 8)
-                    cf[st][q] = cl.raw[st][q];
-                else
-                    cf[st][q] = (R)(cl.raw[st][q] * mscale);
-                row_fence(cf[st][q]);
-            }
+        for (int q = 0; q < NQV; q++) {
+            cfv[q] = cl.rawv[q];
+            row_fence(cfv[q]);
+        }
         const double4 c = cl.c;
         const R x = cl.on ? (R)((y.x - c.x) * c.w) : R(2), yy = cl.on ? (R)((y.y - c.y) * c.w) : R(0),
                 z = cl.on ? (R)((y.z - c.z) * c.w) : R(0);
         R su, sv;
-        m2p_rows<Q, H, R, NQ>(cf[0], cf[1], x, yy, z, su, sv);
+        m2p_rows<Q, QV, H, R, NQ, NQV>(cf, cfv, x, yy, z, su, sv);
         pu += cl.on ? (double)su * c.w : 0.0;
         pv += cl.on ? (double)sv * c.w : 0.0;
     };
@@ -1355,7 +1480,7 @@ __device__ __forceinline__ void m2p_range(int b, int e, int cps, int slot, bool 
 // targets; each half row holds its own cell, lanes 8 h + i of row r evaluate cell slot 2 r + h for
 // target i, so a step covers 8 cells (each coefficient product then takes two masked broadcasts).
 // gid lists the groups of the launch.
-template <int P, int PF, int H, class R>
+template <int P, int PF, int PV, int H, class R>
 __global__ __launch_bounds__(256) void k_fmm_m2p(int ng, const int32_t *__restrict__ gid,
                                                  const int32_t *__restrict__ grp_tb,
                                                  const int32_t *__restrict__ grp_te,
@@ -1364,10 +1489,10 @@ __global__ __launch_bounds__(256) void k_fmm_m2p(int ng, const int32_t *__restri
                                                  const int32_t *__restrict__ u_cell,
                                                  const uint8_t *__restrict__ u_mask,
                                                  const uint8_t *__restrict__ tgt_oct, const double4 *__restrict__ tpos,
-                                                 const double4 *__restrict__ cen,
-                                                 const double2 *__restrict__ M, const double2 *__restrict__ Lx,
-                                                 const double2 *__restrict__ pot, const int32_t *__restrict__ tgt_node,
-                                                 NodeRec *__restrict__ nxt, double mscale)
+                                                 const double4 *__restrict__ cen, const R *__restrict__ M,
+                                                 const R *__restrict__ Lx, const double2 *__restrict__ pot,
+                                                 const int32_t *__restrict__ tgt_node, NodeRec *__restrict__ nxt,
+                                                 double mscale)
 {
     constexpr int NCo = fcount(P), CW = 2 * NCo;
     const int wv = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
@@ -1376,7 +1501,6 @@ __global__ __launch_bounds__(256) void k_fmm_m2p(int ng, const int32_t *__restri
     const int g = gid[gi];
     const int lane = threadIdx.x & 63, row = lane >> 4, j = lane & 15;
     const int tb = grp_tb[g], te = grp_te[g], ub = u_ptr[g], ue = u_ptr[g + 1], uf = u_far[g], lc = grp_lcell[g];
-    const double *Md = (const double *)M;
     for (int t0 = tb; t0 < te; t0 += 64) {
         const int nt = min(64, te - t0);
         // H = 1: nr target rows, cps = 4 / nr cells per step; H = 2: one target row of 8 lanes
@@ -1390,10 +1514,10 @@ __global__ __launch_bounds__(256) void k_fmm_m2p(int ng, const int32_t *__restri
         const int oc = tv ? tgt_oct[t] : 0;
         double pu = 0, pv = 0;
         if constexpr (PF == P) {
-            m2p_range<P, P, H, R>(ub, ue, cps, slot, tv, oc, y, u_cell, u_mask, cen, Md, mscale, pu, pv);
+            m2p_range<P, P, PV, H, R>(ub, ue, cps, slot, tv, oc, y, u_cell, u_mask, cen, M, pu, pv);
         } else {
-            m2p_range<P, P, H, R>(ub, uf, cps, slot, tv, oc, y, u_cell, u_mask, cen, Md, mscale, pu, pv);
-            m2p_range<P, PF, H, R>(uf, ue, cps, slot, tv, oc, y, u_cell, u_mask, cen, Md, mscale, pu, pv);
+            m2p_range<P, P, PV, H, R>(ub, uf, cps, slot, tv, oc, y, u_cell, u_mask, cen, M, pu, pv);
+            m2p_range<P, PF, (PV < PF ? PV : PF), H, R>(uf, ue, cps, slot, tv, oc, y, u_cell, u_mask, cen, M, pu, pv);
         }
         // each target's cell slots, in slot order (all lanes take part in the shuffles)
         double su = 0, sv = 0;
@@ -1403,14 +1527,15 @@ __global__ __launch_bounds__(256) void k_fmm_m2p(int ng, const int32_t *__restri
             sv += __shfl(pv, src, 64);
         }
         if (slot == 0 && tv) {
-            su /= mscale;
-            sv /= mscale;
+            // the local expansion of the group's ancestor (scaled like the multipoles)
             if (lc >= 0) {
                 const double4 c = cen[lc];
                 const double iw = c.w;
-                l2p_eval2_t<P>(Lx + (size_t)lc * CW, Lx + (size_t)lc * CW + NCo, (y.x - c.x) * iw, (y.y - c.y) * iw,
-                               (y.z - c.z) * iw, su, sv);
+                l2p_eval2_t<P>(Lx + (size_t)lc * 2 * CW, Lx + (size_t)lc * 2 * CW + 2 * NCo, (y.x - c.x) * iw,
+                               (y.y - c.y) * iw, (y.z - c.z) * iw, su, sv);
             }
+            su /= mscale;
+            sv /= mscale;
             const double2 pn = pot[t];
             NodeRec *r = nxt + tgt_node[t];
             r->phi = (pn.x + su) / (4 * M_PI);
@@ -1676,10 +1801,20 @@ int fmm_create(Ctx &c)
         P->d_tgt_pos = dput(P->tgt_pos);
         P->d_leaf_cell = dput(P->leaf_cell);
         P->d_m2m_cells = dput(P->m2m_cells);
-        P->d_Tm2m = dput(xlate_pack(*P, 0));
-        P->d_Tl2l = dput(xlate_pack(*P, 1));
+        P->mixed = c.opt.precision == FG_PREC_MIXED;
+        // FP32 mode: charges of order charge_unit times the unit-width regular harmonics; the
+        // expansions are scaled to order one by a power of two (exact in both directions)
+        P->mscale = P->mixed ? std::ldexp(1.0, -(int)std::lround(std::log2(c.charge_unit))) : 1.0;
+        // translation operators in the precision of the expansions
+        auto put_op = [&](const std::vector<double> &v) -> void * {
+            if (!P->mixed) return dput(v);
+            return dput(std::vector<float>(v.begin(), v.end()));
+        };
+        const size_t fsz = P->mixed ? sizeof(float) : sizeof(double);
+        P->d_Tm2m = put_op(xlate_pack(*P, 0));
+        P->d_Tl2l = put_op(xlate_pack(*P, 1));
         xlate_schedules(*P);
-        P->d_Tm2l = dput(m2l_pack(*P));
+        P->d_Tm2l = put_op(m2l_pack(*P));
         m2l_schedule(*P);
         P->d_m2l_tile_cell = dput(P->m2l_tile_cell);
         P->d_m2l_tile_chunk = dput(P->m2l_tile_chunk);
@@ -1688,22 +1823,18 @@ int fmm_create(Ctx &c)
         {
             void *pp = nullptr;
             const size_t np = std::max<size_t>(1, (size_t)P->m2l_nchunks * 64 * 2 * 2 * P->nc);
-            if (hipMalloc(&pp, np * sizeof(double)) != hipSuccess) throw std::string("hipMalloc M2L partials");
-            P->d_m2l_part = (double *)pp;
+            if (hipMalloc(&pp, np * fsz) != hipSuccess) throw std::string("hipMalloc M2L partials");
+            P->d_m2l_part = pp;
         }
         P->d_m2m_child = dput(P->m2m_child);
         P->d_l2l_par = dput(P->l2l_par);
         P->d_l2l_child = dput(P->l2l_child);
         const size_t nexp = (size_t)P->ncell() * 2 * P->nc;
         void *d = nullptr;
-        if (hipMalloc(&d, nexp * sizeof(double2)) != hipSuccess) throw std::string("hipMalloc M");
-        P->d_M = (double2 *)d;
-        if (hipMalloc(&d, nexp * sizeof(double2)) != hipSuccess) throw std::string("hipMalloc L");
-        P->d_L = (double2 *)d;
-        P->mixed = c.opt.precision == FG_PREC_MIXED;
-        // FP32 pass: charges of order charge_unit times the unit-width regular harmonics; scale
-        // them to order one by a power of two (exact in both directions)
-        P->mscale = P->mixed ? std::ldexp(1.0, -(int)std::lround(std::log2(c.charge_unit))) : 1.0;
+        if (hipMalloc(&d, nexp * 2 * fsz) != hipSuccess) throw std::string("hipMalloc M");
+        P->d_M = d;
+        if (hipMalloc(&d, nexp * 2 * fsz) != hipSuccess) throw std::string("hipMalloc L");
+        P->d_L = d;
         fmm_build_near(c, *P);
         // interaction counts (benchmark work models)
         for (size_t g = 0; g < P->grp_tb.size(); g++)
@@ -1831,6 +1962,31 @@ void fmm_build_near(Ctx &c, FmmPlan &P)
         k_nf_build<<<(unsigned)(((int64_t)T * 64 + 255) / 256), 256, 0, c.stream>>>(
             T, d_tgt_tl, P.d_tgt_pos, P.d_lc_ptr, d_lc_base, d_lc_nk, c.src_rank, c.src_pos, wsrc, P.d_nf_ptr,
             P.d_nf_val);
+    {
+        // P2M source records, from the same per-source weights
+        const int NL = (int)P.leaf_cell.size();
+        const size_t s4 = P.mixed ? sizeof(float4) : sizeof(double4), w2 = P.mixed ? sizeof(float2) : sizeof(double2);
+        if (hipMalloc(&P.d_p2m_pos, std::max<size_t>(1, (size_t)S) * s4) != hipSuccess ||
+            hipMalloc(&P.d_p2m_tw, std::max<size_t>(1, 4 * (size_t)Fm) * w2) != hipSuccess ||
+            hipMalloc((void **)&P.d_p2m_code, std::max<size_t>(1, (size_t)S) * sizeof(int32_t)) != hipSuccess)
+            throw std::string("hipMalloc P2M records");
+        const unsigned nbs = (unsigned)((S + 255) / 256), nbl = (unsigned)((NL + 255) / 256);
+        if (P.mixed) {
+            k_p2m_codes<float2><<<nbs, 256, 0, c.stream>>>(Tm, Fm, c.src_rank, wsrc, P.mscale, P.d_p2m_code,
+                                                          (float2 *)P.d_p2m_tw);
+            if (NL > 0)
+                k_p2m_records<float4><<<nbl, 256, 0, c.stream>>>(NL, Tm, S, P.d_leaf_cell, P.d_cen, P.d_leaf_sb,
+                                                                P.d_leaf_se, c.src_pos, wsrc, P.mscale,
+                                                                (float4 *)P.d_p2m_pos);
+        } else {
+            k_p2m_codes<double2><<<nbs, 256, 0, c.stream>>>(Tm, Fm, c.src_rank, wsrc, 1.0, P.d_p2m_code,
+                                                           (double2 *)P.d_p2m_tw);
+            if (NL > 0)
+                k_p2m_records<double4><<<nbl, 256, 0, c.stream>>>(NL, Tm, S, P.d_leaf_cell, P.d_cen, P.d_leaf_sb,
+                                                                 P.d_leaf_se, c.src_pos, wsrc, 1.0,
+                                                                 (double4 *)P.d_p2m_pos);
+        }
+    }
     if (P.mixed && P.nf_nnz > 0) {
         if (hipMalloc(&d, P.nf_nnz * sizeof(float)) != hipSuccess) throw std::string("hipMalloc near-field values");
         P.d_nf_valf = (float *)d;
@@ -1854,7 +2010,7 @@ void fmm_destroy(Ctx &c)
                     P->d_u_far, P->d_u_cell, P->d_u_mask, P->d_tgt_oct, P->d_tgt_node, P->d_tgt_pos,
                     P->d_Tm2m, P->d_Tl2l, P->d_Tm2l, P->d_m2l_part, P->d_m2l_tile_cell, P->d_m2l_tile_chunk,
                     P->d_m2l_chunk_code, P->d_m2l_chunk_src, P->d_m2m_child, P->d_l2l_par, P->d_l2l_child, P->d_M, P->d_L, P->d_leaf_cell, P->d_m2m_cells,
-                    P->d_nf_ptr, P->d_nf_val,
+                    P->d_nf_ptr, P->d_nf_val, P->d_p2m_pos, P->d_p2m_tw, P->d_p2m_code,
                     P->d_nf_valf, P->d_tl_tb, P->d_tl_te, P->d_lc_ptr, P->d_lc_col, P->d_pot, c.qe};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
@@ -1863,11 +2019,14 @@ void fmm_destroy(Ctx &c)
     c.fmm = nullptr;
 }
 
-template <int PP>
+// One FMM pass in the expansions' precision F: FP64, or FP32 (expansions times mscale, FP32
+// translations on the matrix cores and FP32 evaluation) in the mixed-precision mode.
+template <int PP, class F>
 static void run_fmm(Ctx &c)
 {
     FmmPlan &P = *c.fmm;
     const int NL = (int)P.leaf_cell.size();
+    F *M = (F *)P.d_M, *Lx = (F *)P.d_L;
     {
         Prof pr(c, "fmm_near");
         const unsigned nb = (unsigned)(((int64_t)P.ntl * 64 + 255) / 256);
@@ -1881,12 +2040,20 @@ static void run_fmm(Ctx &c)
     {
         Prof pr(c, "fmm_p2m");
         constexpr int G = 8;
-        if (NL > 0)
-        {
-            k_fmm_p2m<PP, G, 0><<<dim3((NL * G + 63) / 64, 2), 64, 0, c.stream>>>(NL, P.d_leaf_cell, P.d_cen, P.d_leaf_sb,
-                                                                                  P.d_leaf_se, c.src_pos, c.src_q, P.d_M);
-            k_fmm_p2m<PP, G, 1><<<dim3((NL * G + 63) / 64, 2), 64, 0, c.stream>>>(NL, P.d_leaf_cell, P.d_cen, P.d_leaf_sb,
-                                                                                  P.d_leaf_se, c.src_pos, c.src_q, P.d_M);
+        const unsigned nb = (unsigned)((NL * G + 63) / 64);
+        if (NL > 0) {
+            if constexpr (std::is_same<F, float>::value) {
+                k_fmm_p2m<PP, G, (1u << (PP + 1)) - 1, float, float4><<<nb, 64, 0, c.stream>>>(
+                    NL, c.Tm, P.d_leaf_cell, P.d_cen, P.d_leaf_sb, P.d_leaf_se, (const float4 *)P.d_p2m_pos,
+                    P.d_p2m_code, (const float2 *)P.d_p2m_tw, c.qe, M);
+            } else {
+                k_fmm_p2m<PP, G, p2m_half_mask<PP>(0), double, double4><<<nb, 64, 0, c.stream>>>(
+                    NL, c.Tm, P.d_leaf_cell, P.d_cen, P.d_leaf_sb, P.d_leaf_se, (const double4 *)P.d_p2m_pos,
+                    P.d_p2m_code, (const double2 *)P.d_p2m_tw, c.qe, M);
+                k_fmm_p2m<PP, G, p2m_half_mask<PP>(1), double, double4><<<nb, 64, 0, c.stream>>>(
+                    NL, c.Tm, P.d_leaf_cell, P.d_cen, P.d_leaf_sb, P.d_leaf_se, (const double4 *)P.d_p2m_pos,
+                    P.d_p2m_code, (const double2 *)P.d_p2m_tw, c.qe, M);
+            }
         }
     }
     {
@@ -1894,21 +2061,21 @@ static void run_fmm(Ctx &c)
         for (int l = P.Lmax - 1; l >= 2; l--) {
             const int n = P.m2m_lev_off[l + 1] - P.m2m_lev_off[l];
             if (n == 0) continue;
-            k_fmm_xlate<PP, 0><<<(n + 15) / 16 * 2 * XL<PP>::NT, 256, 0, c.stream>>>(
-                n, P.d_m2m_cells + P.m2m_lev_off[l], P.d_m2m_child + (size_t)8 * P.m2m_lev_off[l], P.d_Tm2m,
-                (double *)P.d_M);
+            k_fmm_xlate<PP, 0, F><<<(n + 15) / 16 * 2 * XL<PP>::NT, 256, 0, c.stream>>>(
+                n, P.d_m2m_cells + P.m2m_lev_off[l], P.d_m2m_child + (size_t)8 * P.m2m_lev_off[l],
+                (const F *)P.d_Tm2m, M);
         }
     }
     {
         Prof pr(c, "fmm_m2l");
         if (P.m2l_nchunks > 0)
-            k_fmm_m2l_mfma<PP><<<P.m2l_nchunks, 256, 0, c.stream>>>(P.d_m2l_tile_chunk, P.m2l_ntiles, P.d_m2l_chunk_code,
-                                                                   P.d_m2l_chunk_src, P.d_Tm2l, (const double *)P.d_M,
-                                                                   P.d_m2l_part);
+            k_fmm_m2l_mfma<PP, F><<<P.m2l_nchunks, 256, 0, c.stream>>>(P.d_m2l_tile_chunk, P.m2l_ntiles,
+                                                                      P.d_m2l_chunk_code, P.d_m2l_chunk_src,
+                                                                      (const F *)P.d_Tm2l, M, (F *)P.d_m2l_part);
         if (P.m2l_ntiles > 0) {
             const int64_t n = (int64_t)P.m2l_ntiles * 64 * 2 * 2 * P.nc;
-            k_fmm_m2l_reduce<PP><<<(int)((n + 255) / 256), 256, 0, c.stream>>>(
-                P.m2l_ntiles, P.d_m2l_tile_cell, P.d_m2l_tile_chunk, P.d_cen, P.d_m2l_part, (double *)P.d_L);
+            k_fmm_m2l_reduce<PP, F><<<(int)((n + 255) / 256), 256, 0, c.stream>>>(
+                P.m2l_ntiles, P.d_m2l_tile_cell, P.d_m2l_tile_chunk, P.d_cen, (const F *)P.d_m2l_part, Lx);
         }
     }
     {
@@ -1916,9 +2083,9 @@ static void run_fmm(Ctx &c)
         for (int l = 3; l <= P.Lmax; l++) {
             const int n = P.l2l_par_off[l + 1] - P.l2l_par_off[l];
             if (n == 0) continue;
-            k_fmm_xlate<PP, 1><<<(n + 15) / 16 * 2 * XL<PP>::NT, 256, 0, c.stream>>>(
-                n, P.d_l2l_par + P.l2l_par_off[l], P.d_l2l_child + (size_t)8 * P.l2l_par_off[l], P.d_Tl2l,
-                (double *)P.d_L);
+            k_fmm_xlate<PP, 1, F><<<(n + 15) / 16 * 2 * XL<PP>::NT, 256, 0, c.stream>>>(
+                n, P.d_l2l_par + P.l2l_par_off[l], P.d_l2l_child + (size_t)8 * P.l2l_par_off[l],
+                (const F *)P.d_Tl2l, Lx);
         }
     }
     {
@@ -1927,32 +2094,26 @@ static void run_fmm(Ctx &c)
         if (ng > 0) {
             // groups of more than M2P_HALF_MAX targets with one cell per row, then the others with
             // one cell per half row
-            auto go = [&](auto k1, auto k2, double ms) {
+            auto go = [&](auto k1, auto k2) {
                 const int n1 = P.m2p_n1, n2 = ng - P.m2p_n1;
                 if (n1 > 0)
                     k1<<<(n1 + 3) / 4, 256, 0, c.stream>>>(n1, P.d_m2p_gid, P.d_grp_tb, P.d_grp_te, P.d_grp_lcell,
                                                           P.d_u_ptr, P.d_u_far, P.d_u_cell, P.d_u_mask, P.d_tgt_oct,
-                                                          P.d_tgt_pos, P.d_cen, P.d_M, P.d_L, P.d_pot, P.d_tgt_node,
-                                                          c.rec[1 - c.cur], ms);
+                                                          P.d_tgt_pos, P.d_cen, M, Lx, P.d_pot, P.d_tgt_node,
+                                                          c.rec[1 - c.cur], P.mscale);
                 if (n2 > 0)
                     k2<<<(n2 + 3) / 4, 256, 0, c.stream>>>(n2, P.d_m2p_gid + n1, P.d_grp_tb, P.d_grp_te,
                                                           P.d_grp_lcell, P.d_u_ptr, P.d_u_far, P.d_u_cell, P.d_u_mask,
-                                                          P.d_tgt_oct, P.d_tgt_pos, P.d_cen, P.d_M, P.d_L, P.d_pot,
-                                                          P.d_tgt_node, c.rec[1 - c.cur], ms);
+                                                          P.d_tgt_oct, P.d_tgt_pos, P.d_cen, M, Lx, P.d_pot,
+                                                          P.d_tgt_node, c.rec[1 - c.cur], P.mscale);
             };
             constexpr int PL = fmm_far_degree(PP);
-            const bool lowf = P.pf != PP;
-            if (P.mixed) {
-                if (lowf)
-                    go(k_fmm_m2p<PP, PL, 1, float>, k_fmm_m2p<PP, PL, 2, float>, P.mscale);
-                else
-                    go(k_fmm_m2p<PP, PP, 1, float>, k_fmm_m2p<PP, PP, 2, float>, P.mscale);
-            } else {
-                if (lowf)
-                    go(k_fmm_m2p<PP, PL, 1, double>, k_fmm_m2p<PP, PL, 2, double>, 1.0);
-                else
-                    go(k_fmm_m2p<PP, PP, 1, double>, k_fmm_m2p<PP, PP, 2, double>, 1.0);
-            }
+            // the second expansion (charges of v) to degree fmm_v_degree in the FP32 mode
+            constexpr int PV = std::is_same<F, float>::value ? fmm_v_degree(PP) : PP;
+            if (P.pf != PP)
+                go(k_fmm_m2p<PP, PL, PV, 1, F>, k_fmm_m2p<PP, PL, PV, 2, F>);
+            else
+                go(k_fmm_m2p<PP, PP, PV, 1, F>, k_fmm_m2p<PP, PP, PV, 2, F>);
         }
     }
 }
@@ -1981,11 +2142,12 @@ void fmm_info(const Ctx &c, int64_t info[16])
 
 void launch_fmm(Ctx &c)
 {
+    const bool f32 = c.fmm->mixed;
     switch (c.fmm->p) {
-    case 4: run_fmm<4>(c); break;
-    case 6: run_fmm<6>(c); break;
-    case 8: run_fmm<8>(c); break;
-    case 10: run_fmm<10>(c); break;
+    case 4: f32 ? run_fmm<4, float>(c) : run_fmm<4, double>(c); break;
+    case 6: f32 ? run_fmm<6, float>(c) : run_fmm<6, double>(c); break;
+    case 8: f32 ? run_fmm<8, float>(c) : run_fmm<8, double>(c); break;
+    case 10: f32 ? run_fmm<10, float>(c) : run_fmm<10, double>(c); break;
     default: break;
     }
 }

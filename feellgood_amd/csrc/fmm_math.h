@@ -230,6 +230,58 @@ __host__ __device__ inline void p2m_accumulate_cols(double x, double y, double z
     }
 }
 
+// P2M accumulation of one source into both expansions at once, columns of MASK only: the unit
+// harmonics Q_n^m of the source (the recurrences of p2m_accumulate_cols started from Q_0^0 = 1)
+// times its charges qu (into au) and qv (into av). T is the arithmetic type (FP64, or FP32 in the
+// mixed-precision mode).
+template <int P, unsigned MASK, class T>
+__host__ __device__ inline void p2m_accumulate2_cols(T x, T y, T z, T qu, T qv, T (&au)[2 * p2m_half_count<P>(MASK)],
+                                                     T (&av)[2 * p2m_half_count<P>(MASK)])
+{
+    const T r2 = x * x + y * y + z * z, z2 = z + z, r22 = r2 + r2;
+    T dre = T(1), dim = T(0);
+    T tm = T(1);  // -(2m - 1) once stepped
+    auto put = [&](int o, T re, T im) {
+        au[o] += re * qu;
+        au[o + 1] += im * qu;
+        av[o] += re * qv;
+        av[o + 1] += im * qv;
+    };
+#pragma unroll
+    for (int m = 0; m <= P; m++) {
+        if ((MASK >> m) == 0) break;
+        if (m > 0) {
+            tm -= T(2);
+            const T nre = tm * (x * dre - y * dim), nim = tm * (x * dim + y * dre);
+            dre = nre;
+            dim = nim;
+        }
+        if (!((MASK >> m) & 1)) continue;
+        const int o = 2 * p2m_col_off<P>(MASK, m);
+        put(o, dre, dim);
+        if (m + 1 <= P) {
+            const T a1 = (m == 0 ? T(1) : T(2) - tm) * z;  // (2m+1) z
+            T p2re = dre, p2im = dim;
+            T p1re = a1 * dre, p1im = a1 * dim;
+            put(o + 2, p1re, p1im);
+            T az = a1 + z2;
+            T kr2 = T(2 * m + 1) * r2, dk = T(2 * m + 3) * r2;
+#pragma unroll
+            for (int n = m + 2; n <= P; n++) {
+                const T cre = az * p1re - kr2 * p2re, cim = az * p1im - kr2 * p2im;
+                put(o + 2 * (n - m), cre, cim);
+                p2re = p1re;
+                p2im = p1im;
+                p1re = cre;
+                p1im = cim;
+                az += z2;
+                kr2 += dk;
+                dk += r22;
+            }
+        }
+    }
+}
+
 // Factorial scale of p2m_accumulate_cols' sums: coefficient (n, m) times 1 / ((n+m)! (n-m)!).
 __host__ __device__ constexpr double p2m_scale(int n, int m)
 {
@@ -336,14 +388,17 @@ __device__ inline double fast_rsqrt(double x)
 __device__ inline float fast_inv(float x) { return __builtin_amdgcn_rcpf(x); }
 __device__ inline float fast_rsqrt(float x) { return __builtin_amdgcn_rsqf(x); }
 
-// L2P of two expansions at once: pu += sum L_u R, pv += sum L_v R. Not unrolled (it runs once per
-// target); the recurrence divisors are formed from loop counters.
-template <int P>
-__host__ __device__ inline void l2p_eval2_t(const double2 *__restrict__ Lu, const double2 *__restrict__ Lv, double x,
-                                            double y, double z, double &pu, double &pv)
+// L2P of two expansions at once: pu += sum L_u R, pv += sum L_v R. The expansions are T arrays of
+// interleaved real and imaginary parts (FP64, or FP32 in the mixed-precision mode); the sums are
+// FP64. Not unrolled (it runs once per target); the recurrence divisors are formed from loop
+// counters.
+template <int P, class T>
+__host__ __device__ inline void l2p_eval2_t(const T *__restrict__ Lu, const T *__restrict__ Lv, double x, double y,
+                                            double z, double &pu, double &pv)
 {
     const double r2 = x * x + y * y + z * z;
     double dre = 1.0, dim = 0.0;
+    auto at = [](const T *L, int k) { return make_double2((double)L[2 * k], (double)L[2 * k + 1]); };
 #pragma unroll 1
     for (int m = 0; m <= P; m++) {
         if (m > 0) {
@@ -352,12 +407,12 @@ __host__ __device__ inline void l2p_eval2_t(const double2 *__restrict__ Lu, cons
             dre = nre;
             dim = nim;
         }
-        double2 a = Lu[fidx(m, m)], b = Lv[fidx(m, m)];
+        double2 a = at(Lu, fidx(m, m)), b = at(Lv, fidx(m, m));
         double au = a.x * dre - a.y * dim, av = b.x * dre - b.y * dim;
         if (m + 1 <= P) {
             double p2re = dre, p2im = dim, p1re = z * dre, p1im = z * dim;
-            a = Lu[fidx(m + 1, m)];
-            b = Lv[fidx(m + 1, m)];
+            a = at(Lu, fidx(m + 1, m));
+            b = at(Lv, fidx(m + 1, m));
             au += a.x * p1re - a.y * p1im;
             av += b.x * p1re - b.y * p1im;
 #pragma unroll 1
@@ -365,8 +420,8 @@ __host__ __device__ inline void l2p_eval2_t(const double2 *__restrict__ Lu, cons
                 const double inv = 1.0 / ((double)(n + m) * (double)(n - m));
                 const double aa = (2.0 * n - 1.0) * z;
                 const double cre = (aa * p1re - r2 * p2re) * inv, cim = (aa * p1im - r2 * p2im) * inv;
-                a = Lu[fidx(n, m)];
-                b = Lv[fidx(n, m)];
+                a = at(Lu, fidx(n, m));
+                b = at(Lv, fidx(n, m));
                 au += a.x * cre - a.y * cim;
                 av += b.x * cre - b.y * cim;
                 p2re = p1re;

@@ -734,11 +734,12 @@ __global__ __launch_bounds__(BLOCK) void k_charges_tet(int Tm, const int4 *__res
     du = du + duy + duz;
     dv = dv + dvy + dvz;
     const double Ms = regs[treg[t]].Ms;
+    if (q)  // per-source charges for the direct sum (the FMM takes the element values)
 #pragma unroll
-    for (int g = 0; g < 5; g++) {
-        const double wg = geo[(size_t)(12 + g) * Tm + t];
-        q[rank[(size_t)5 * t + g]] = make_double2(-Ms * wg * du, -Ms * wg * dv);
-    }
+        for (int g = 0; g < 5; g++) {
+            const double wg = geo[(size_t)(12 + g) * Tm + t];
+            q[rank[(size_t)5 * t + g]] = make_double2(-Ms * wg * du, -Ms * wg * dv);
+        }
     if (qe) qe[t] = make_double2(-Ms * du, -Ms * dv);
 }
 
@@ -794,7 +795,7 @@ __global__ __launch_bounds__(BLOCK) void k_charges_tri(int Fm, int64_t src_off, 
         G[g] = tri_a(0, g) * P[0] + tri_a(1, g) * P[1] + tri_a(2, g) * P[2];
         qu[g] = dMs != 0.0 ? dMs * (wg * dot(U, n)) : 0.0;
         qv[g] = dMs != 0.0 ? dMs * (wg * dot(V, n)) : 0.0;
-        q[rank[src_off + (int64_t)4 * f + g]] = make_double2(qu[g], qv[g]);
+        if (q) q[rank[src_off + (int64_t)4 * f + g]] = make_double2(qu[g], qv[g]);
     }
     const double su[3] = {dot(u[0], n), dot(u[1], n), dot(u[2], n)};
     const double sv[3] = {dot(v[0], n), dot(v[1], n), dot(v[2], n)};
@@ -1210,15 +1211,16 @@ void launch_spmv_debug(Ctx &c, double cst, const double2 *xin, double2 *yout)
 void launch_charges(Ctx &c)
 {
     const NodeRec *nxt = c.rec[1 - c.cur];
+    double2 *q = c.fmm ? nullptr : c.src_q;  // the FMM's P2M and near field take the element values
     if (c.Tm > 0) {
         Prof pr(c, "charges_tet");
         k_charges_tet<<<grid1(c.Tm), BLOCK, 0, c.stream>>>(c.Tm, c.tet_nodes, c.geo, c.tet_reg, c.regions, nxt,
-                                                           c.src_rank, c.src_q, c.qe);
+                                                           c.src_rank, q, c.qe);
     }
     if (c.Fm > 0) {
         Prof pr(c, "charges_tri");
         k_charges_tri<<<grid1(c.Fm), BLOCK, 0, c.stream>>>(c.Fm, (int64_t)5 * c.Tm, c.tri_nodes, c.tri_geo, c.pos4,
-                                                           nxt, c.src_rank, c.src_q, c.tri_corr, c.qe, c.Tm);
+                                                           nxt, c.src_rank, q, c.tri_corr, c.qe, c.Tm);
     }
     Prof pr(c, "corr_gather");
     k_corr_gather<<<c.blocks_for(c.N), BLOCK, 0, c.stream>>>(c.N, c.cinc_ptr, c.cinc_idx, c.tri_corr, c.corr);

@@ -56,6 +56,8 @@ SYMBOLS = {
     "orc_get_solution": (None, [C.c_void_p, _d]),
     "orc_get_tet_element": (None, [C.c_void_p, C.c_int, _d, _d]),
     "orc_calc_demag": (None, [C.c_void_p]),
+    "orc_set_demag": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
+    "orc_fmm_potentials": (C.c_int, [_d, C.c_int64, _d, _d, _d, C.c_int64, C.c_int, C.c_int, _d, _d]),
     "orc_demag_sample": (None, [C.c_void_p, _i32, C.c_int, _d, _d]),
     "orc_n_sources": (C.c_int64, [C.c_void_p]),
     "orc_get_sources": (None, [C.c_void_p, _d, _d, _d]),
@@ -120,6 +122,17 @@ def mt_uniform(seed, n):
     out = np.zeros(n)
     lib().orc_mt_uniform(seed, n, d(out))
     return out
+
+
+def fmm_potentials(xyz, qu, qv, tgt, order=8, leaf=64):
+    """The CPU FMM: sum_s q_s / |x_t - y_s| for both charge sets at the targets; (pu, pv, leaf level)."""
+    xyz = np.ascontiguousarray(xyz, dtype=np.float64).reshape(-1)
+    tgt = np.ascontiguousarray(tgt, dtype=np.float64).reshape(-1)
+    qu = np.ascontiguousarray(qu, dtype=np.float64)
+    qv = np.ascontiguousarray(qv, dtype=np.float64)
+    pu = np.zeros(len(tgt) // 3); pv = np.zeros(len(tgt) // 3)
+    L = lib().orc_fmm_potentials(d(xyz), len(qu), d(qu), d(qv), d(tgt), len(pu), order, leaf, d(pu), d(pv))
+    return pu, pv, L
 
 
 def basis_angles(seed, n):
@@ -277,6 +290,11 @@ class Problem:
 
     def calc_demag(self):
         lib().orc_calc_demag(self._h)
+
+    def set_demag(self, order=0, leaf=64):
+        """Demag method of calc_demag: order 0 = exact direct sums (the default), else the CPU FMM."""
+        if lib().orc_set_demag(self._h, order, leaf) != 0:
+            raise ValueError("bad FMM order or leaf size")
 
     def demag_sample(self, targets):
         t = np.ascontiguousarray(targets, dtype=np.int32)

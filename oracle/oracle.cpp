@@ -11,6 +11,7 @@
  * invariants, and end to end against ci-tests/full_test.py's golden values.
  */
 #include "oracle.h"
+#include "fmm_cpu.h"
 
 #include <algorithm>
 #include <cfloat>
@@ -537,6 +538,7 @@ struct orc_problem {
     std::vector<double> src_qu, src_qv;
     std::vector<double> corr_u, corr_v;
     double totalMagVol = 0;
+    int fmm_order = 0, fmm_leaf = 64; /* orc_set_demag: 0 = exact direct sums */
 
     bool isMagTet(const TetGeo &t) const { return vol[t.reg].Ms > 0; }
     double &K(int i, int j)
@@ -1429,15 +1431,63 @@ void demag_one(orc_problem *p, bool of_v)
         else p->node[i].d[1].phi = val;
     }
 }
+
+/* Both potentials by the CPU FMM (fmm_cpu.cpp) at the magnetic nodes, plus the corrections */
+void demag_fmm(orc_problem *p)
+{
+    demag_charges(p, false);
+    demag_charges(p, true);
+    std::vector<double> txyz;
+    std::vector<int> tn;
+    for (int i = 0; i < p->N; i++)
+        if (p->magNode[i]) {
+            tn.push_back(i);
+            txyz.push_back(p->node[i].p.x);
+            txyz.push_back(p->node[i].p.y);
+            txyz.push_back(p->node[i].p.z);
+        }
+    std::vector<double> pu, pv;
+    orc_fmm::potentials(p->src_xyz, p->src_qu, p->src_qv, txyz, p->fmm_order, p->fmm_leaf, pu, pv);
+    for (size_t k = 0; k < tn.size(); k++) {
+        p->node[tn[k]].d[1].phi = (pu[k] + p->corr_u[tn[k]]) / (4 * M_PI);
+        p->node[tn[k]].d[1].phiv = (pv[k] + p->corr_v[tn[k]]) / (4 * M_PI);
+    }
+}
 } // namespace
 
 extern "C" {
 
-/* fmm::calc_demag (src/fmm_demag.h:96-101) */
+/* fmm::calc_demag (src/fmm_demag.h:96-101): exact direct sums, or the CPU FMM after orc_set_demag */
 void orc_calc_demag(orc_problem *p)
 {
+    if (p->fmm_order > 0) {
+        demag_fmm(p);
+        return;
+    }
     demag_one(p, false);
     demag_one(p, true);
+}
+
+/* Demag method of orc_calc_demag: order 0 = exact direct sums (the default); order > 0 = the CPU
+ * FMM of that order with leaves of `leaf` sources on average (the CPU-baseline timing). */
+int orc_set_demag(orc_problem *p, int order, int leaf)
+{
+    if (order < 0 || order > 12 || leaf < 1) return -1;
+    p->fmm_order = order;
+    p->fmm_leaf = leaf;
+    return 0;
+}
+
+/* The CPU FMM on raw sources and targets (tests): potentials without corrections or 1/(4 pi);
+ * returns the leaf level. */
+int orc_fmm_potentials(const double *sxyz, int64_t S, const double *qu, const double *qv, const double *txyz,
+                       int64_t T, int order, int leaf, double *pu, double *pv)
+{
+    std::vector<double> a(sxyz, sxyz + 3 * S), b(qu, qu + S), c(qv, qv + S), t(txyz, txyz + 3 * T), ou, ov;
+    const int L = orc_fmm::potentials(a, b, c, t, order, leaf, ou, ov);
+    std::copy(ou.begin(), ou.end(), pu);
+    std::copy(ov.begin(), ov.end(), pv);
+    return L;
 }
 
 /* The charges of both potentials and the exact potentials at n given targets only (CPU-baseline

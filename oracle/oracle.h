@@ -70,6 +70,13 @@ void orc_get_solution(const orc_problem *p, double *x);
 /* Element matrices of tet k after orc_prepare_elements: Kp[64] row-major, Lp[8]. */
 void orc_get_tet_element(const orc_problem *p, int k, double *Kp, double *Lp);
 void orc_calc_demag(orc_problem *p);
+/* Demag method of orc_calc_demag: order 0 = exact direct sums (default); order > 0 = the CPU FMM
+ * (fmm_cpu.cpp) of that order, leaves of `leaf` sources on average. Returns 0, or -1 on bad values. */
+int orc_set_demag(orc_problem *p, int order, int leaf);
+/* The CPU FMM on raw sources and targets: sum_s q_s / |x_t - y_s| for both charge sets (no
+ * corrections, no 1/(4 pi)); returns the leaf level. */
+int orc_fmm_potentials(const double *sxyz, int64_t S, const double *qu, const double *qv, const double *txyz,
+                       int64_t T, int order, int leaf, double *pu, double *pv);
 /* Charges of u and v NEXT and the exact potentials at n targets only (state untouched). */
 void orc_demag_sample(orc_problem *p, const int32_t *targets, int n, double *phi, double *phiv);
 /* Charges of the last demag call: sources [n_src] (tets then tris), corrections [n_nodes]. */

@@ -46,6 +46,36 @@ def fmm_ref(xyz, qu, qv, tgt, order, leaf, far=0):
     return pu, pv, L.value
 
 
+def test_oracle_fmm_converges_with_order():
+    """The CPU oracle's FMM (the CPU baseline's demag: a classic octree FMM with M2L on every level
+    and L2P) against the exact direct sums: the error falls with the order, by at least 3x per two
+    orders, and stays below 1e-4 of the largest potential at order 8 (5e-4 for the random v charges)."""
+    m = film()
+    p, _, _ = charged(m)
+    xyz, qu, qv = p.sources()
+    cu, cv = p.corrections()
+    _, _, phi, phiv = p.get_state(nv.FG_NEXT)
+    tgt = m.arrays()["pos"]
+    exact = (phi * 4 * np.pi - cu, phiv * 4 * np.pi - cv)
+    errs = []
+    for order in (4, 6, 8):
+        pu, pv, L = oracle.fmm_potentials(xyz, qu, qv, tgt, order, 64)
+        assert L >= 3
+        errs.append([np.abs(a - b).max() / np.abs(b).max() for a, b in ((pu, exact[0]), (pv, exact[1]))])
+    for k in range(2):
+        assert errs[1][k] * 3 < errs[0][k] and errs[2][k] * 3 < errs[1][k], errs
+    assert errs[2][0] < 1e-4 and errs[2][1] < 5e-4, errs
+    # the problem-level switch: orc_calc_demag with the FMM against the exact potentials
+    u, v, _, _ = p.get_state(nv.FG_NEXT)
+    q = oracle.Problem(m.mesh)
+    q.set_demag(8, 64)
+    q.set_state(nv.FG_BOTH, u=u, v=v)
+    q.calc_demag()
+    _, _, fphi, fphiv = q.get_state(nv.FG_NEXT)
+    assert np.abs(fphi - phi).max() <= 2e-4 * np.abs(phi).max()
+    assert np.abs(fphiv - phiv).max() <= 1e-3 * np.abs(phiv).max()
+
+
 def test_host_fmm_converges_with_order():
     m = film()
     p, _, _ = charged(m)
@@ -158,12 +188,13 @@ def test_step_with_fmm_against_exact_oracle(order):
 
 @pytest.mark.gpu
 def test_mixed_precision_fmm_step():
-    """FP32 near-field coefficients and FP32 multipole-to-target evaluation: potentials within 1e-5
-    relative of the FP64 FMM, and one LLG step within |dm| <= 1e-6 of the exact FP64 oracle."""
+    """The mixed-precision FMM (FP32 expansions, translations and near-field coefficients; the far
+    field of the charges of v to degree 4): phi within 1e-5 relative of the FP64 FMM, phi_v within
+    3e-3 (it enters the step only as theta dt H_d(v)), and one LLG step from a state with a nonzero
+    previous v within |dm| <= 1e-6 of the exact FP64 oracle and within 1e-7 of the FP64 FMM step."""
     from feellgood_amd.engine import Engine
     m = film(24, 12, 3)
     p, u, v = charged(m)
-    _, _, phi0, phiv0 = p.get_state(nv.FG_NEXT)
     pots = {}
     for prec in ("fp64", "mixed"):
         g = Engine(m.mesh, demag="fmm", fmm_order=8, fmm_leaf=64, precision=prec)
@@ -171,21 +202,24 @@ def test_mixed_precision_fmm_step():
         g.calc_demag()
         pots[prec] = g.get_state(nv.FG_NEXT)[2:]
         g.close()
-    for k in range(2):
-        ref = pots["fp64"][k]
-        assert np.abs(pots["mixed"][k] - ref).max() <= 1e-5 * np.abs(ref).max()
-    v0 = np.zeros_like(u)
+    err = [np.abs(pots["mixed"][k] - pots["fp64"][k]).max() / np.abs(pots["fp64"][k]).max() for k in range(2)]
+    print(f"mixed against FP64 FMM: phi {err[0]:.2e}, phi_v {err[1]:.2e} of the largest", flush=True)
+    assert err[0] <= 1e-5 and err[1] <= 3e-3, err
+    v0 = v  # random, so its charges are as large as those of the film's smooth 1e11 rad/s velocities
     H = np.array([-0.025, 0.0044, 0.0]) / nv.MU0
     dt = 5e-14
     pref = 1 + 0.01 * abs(np.log(0.01))
     o = oracle.Problem(m.mesh, tol=1e-10, max_iter=1000)
+    f = Engine(m.mesh, tol=1e-10, max_iter=1000, demag="fmm", fmm_order=8, fmm_leaf=64)
     g = Engine(m.mesh, tol=1e-10, max_iter=1000, demag="fmm", fmm_order=8, fmm_leaf=64, precision="mixed")
-    for e in (o, g):
+    for e in (o, f, g):
         e.set_state(nv.FG_BOTH, u=u, v=v0)
         e.calc_demag()
         e.evolution()
         e.base_projection(0.3)
         e.prepare_elements(H, dt, pref)
         e.solve(dt)
-    dm = np.abs(g.get_state(nv.FG_NEXT)[0] - o.get_state(nv.FG_NEXT)[0]).max()
-    assert dm <= 1e-6, dm
+    un = {k: e.get_state(nv.FG_NEXT)[0] for k, e in (("o", o), ("f", f), ("g", g))}
+    dm, dmf = np.abs(un["g"] - un["o"]).max(), np.abs(un["g"] - un["f"]).max()
+    print(f"step |dm|: mixed against exact {dm:.2e}, against FP64 FMM {dmf:.2e}", flush=True)
+    assert dm <= 1e-6 and dmf <= 1e-7, (dm, dmf)
