@@ -14,6 +14,7 @@
 // summed in order on the host.
 #include <algorithm>
 #include <cmath>
+#include <mutex>
 #include <string>
 #include <utility>
 #include <vector>
@@ -235,6 +236,12 @@ __global__ __launch_bounds__(ESB) void k_es_add(int n, const double *__restrict_
     if (i < n) x[i] += xd[i];
 }
 
+__global__ __launch_bounds__(ESB) void k_es_zero(int n, double *__restrict__ x)
+{
+    const int i = blockIdx.x * ESB + threadIdx.x;
+    if (i < n) x[i] = 0.0;
+}
+
 struct DevBufs {
     std::vector<void *> ptrs;
     template <class T>
@@ -255,6 +262,31 @@ struct DevBufs {
     ~DevBufs()
     {
         for (void *p : ptrs) (void)hipFree(p);
+    }
+};
+
+// Device slices carved from one buffer the caller does not allocate (the static scratch array
+// below): 256-byte aligned, never freed. A slice that would pass the end throws ScratchFull before
+// it is handed out, so nothing is written beyond the buffer.
+struct ScratchFull {};
+struct ArenaBufs {
+    char *base;
+    size_t cap, off = 0;
+    template <class T>
+    T *alloc(size_t n)
+    {
+        const size_t bytes = (std::max<size_t>(n, 1) * sizeof(T) + 255) & ~(size_t)255;
+        if (bytes > cap - off) throw ScratchFull{};
+        T *p = (T *)(base + off);
+        off += bytes;
+        return p;
+    }
+    template <class T>
+    T *upload(const std::vector<T> &h)
+    {
+        T *d = alloc<T>(h.size());
+        if (!h.empty()) ESCHK(hipMemcpy(d, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice));
+        return d;
     }
 };
 
@@ -562,35 +594,41 @@ double tri_area(const fg_mesh *m, int f)
     const d3 c = cross(e1, e2);
     return 0.5 * std::sqrt(dot(c, c));
 }
-}  // namespace
 
-extern "C" int fg_electrostatic_potential(const fg_mesh *m, const double *sigma, const double *jn, const double *Vbc,
-                                          double tol, int32_t max_iter, int32_t device, double *V_out,
-                                          fg_solve_info *info)
+// Host side of the potential's set-up, shared by both entry points: the arguments,
+// electrostatSolver::checkBoundaryConditions (src/electrostatSolver.cpp:10-28: one surface region
+// with a current density and one with a potential), the node pattern and the surface terms.
+struct EsSetup {
+    std::vector<int4> tn;
+    std::vector<double> tsig, L, Vd;
+    std::vector<uint8_t> dir;
+    NodePattern P;
+};
+
+int es_setup(const fg_mesh *m, const double *sigma, const double *jn, const double *Vbc, double tol, int32_t max_iter,
+             const double *V_out, EsSetup &S)
 {
     if (!m || !sigma || !jn || !Vbc || !V_out || !m->pos || !m->tet_nodes || !m->tet_region || m->n_nodes <= 0 ||
         m->n_tets <= 0 || (m->n_tris > 0 && (!m->tri_nodes || !m->tri_region)) || !(tol > 0) || max_iter < 1)
         return FG_ERR_ARG;
-    // electrostatSolver::checkBoundaryConditions (src/electrostatSolver.cpp:10-28): one surface
-    // region with a current density and one with a potential
     int nJ = 0, nV = 0;
     for (int k = 0; k < m->n_surf_regions; k++) {
         nJ += std::isfinite(jn[k]);
         nV += std::isfinite(Vbc[k]);
     }
     if (nJ != 1 || nV != 1) return FG_ERR_ARG;
-    std::vector<int4> tn;
-    if (check_tets(m, tn) != FG_OK) return FG_ERR_MESH;
+    if (check_tets(m, S.tn) != FG_OK) return FG_ERR_MESH;
     const int N = m->n_nodes, T = m->n_tets;
-    std::vector<double> tsig(T);
-    for (int t = 0; t < T; t++) tsig[t] = sigma[m->tet_region[t]];
+    S.tsig.resize(T);
+    for (int t = 0; t < T; t++) S.tsig[t] = sigma[m->tet_region[t]];
     // CSR pattern of the node graph; the entries of each (row, column) listed in mesh order
-    const NodePattern P = node_pattern(m);
-    if (P.rp[N] == 0) return FG_ERR_MESH;
+    S.P = node_pattern(m);
+    if (S.P.rp[N] == 0) return FG_ERR_MESH;
     // L = - sum over the triangles of jn a_ig w_g (src/electrostatSolver.cpp:67-76, buildVect), and
     // the Dirichlet nodes with their potentials (:112-127)
-    std::vector<double> L(N, 0.0), Vd(N, 0.0);
-    std::vector<uint8_t> dir(N, 0);
+    S.L.assign(N, 0.0);
+    S.Vd.assign(N, 0.0);
+    S.dir.assign(N, 0);
     for (int f = 0; f < m->n_tris; f++) {
         const int32_t *q = m->tri_nodes + 3 * (size_t)f;
         const int rg = m->tri_region[f];
@@ -603,90 +641,153 @@ extern "C" int fg_electrostatic_potential(const fg_mesh *m, const double *sigma,
             const double Jw = J * (2.0 * surf * TRI_PDS[g]);
             for (int i = 0; i < 3; i++) BE[i] -= tri_a(i, g) * Jw;
         }
-        for (int i = 0; i < 3; i++) L[q[i]] += BE[i];
+        for (int i = 0; i < 3; i++) S.L[q[i]] += BE[i];
         if (has && !std::isnan(Vbc[rg]))
             for (int i = 0; i < 3; i++) {
-                Vd[q[i]] = Vbc[rg];
-                dir[q[i]] = 1;
+                S.Vd[q[i]] = Vbc[rg];
+                S.dir[q[i]] = 1;
             }
     }
+    return FG_OK;
+}
+
+// Device side: assembly and algebra::cg_dir on the null stream of the current device, with every
+// buffer taken from B (all of them before the first kernel). Throws std::string on a HIP error.
+template <class Bufs>
+void es_solve(Bufs &B, const fg_mesh *m, const EsSetup &S, double tol, int32_t max_iter, double *V_out,
+              fg_solve_info *info)
+{
+    const int N = m->n_nodes, T = m->n_tets;
+    const NodePattern &P = S.P;
+    const int64_t nnz = P.rp[N];
+    const int nb = (N + ESB - 1) / ESB;
+    int4 *d_tn = B.upload(S.tn);
+    double *d_pos = B.upload(std::vector<double>(m->pos, m->pos + 3 * (size_t)N));
+    double *d_sig = B.upload(S.tsig);
+    double *d_ke = B.template alloc<double>((size_t)16 * T);
+    int64_t *d_cptr = B.upload(P.cptr), *d_cslot = B.upload(P.cslot), *d_rp = B.upload(P.rp);
+    int32_t *d_col = B.upload(P.col);
+    double *d_val = B.template alloc<double>(nnz);
+    uint8_t *d_dir = B.upload(S.dir);
+    double *d_L = B.upload(S.L), *d_Vd = B.upload(S.Vd);
+    double *d_D = B.template alloc<double>(N), *d_x = B.template alloc<double>(N), *d_r = B.template alloc<double>(N),
+           *d_z = B.template alloc<double>(N), *d_p = B.template alloc<double>(N), *d_q = B.template alloc<double>(N);
+    double *d_part = B.template alloc<double>((size_t)2 * nb);
+    k_es_zero<<<nb, ESB>>>(N, d_x);
+    k_es_element<<<(T + ESB - 1) / ESB, ESB>>>(T, d_tn, d_pos, d_sig, d_ke);
+    ESCHK(hipGetLastError());
+    k_es_gather<<<(unsigned)((nnz + ESB - 1) / ESB), ESB>>>(nnz, d_cptr, d_cslot, d_ke, d_val);
+    k_es_diag<<<nb, ESB>>>(N, d_rp, d_col, d_val, d_dir, d_D);
+    // b = L - K Vd (masked), r = b (the start is V = 0), z = D r, p = z
+    k_es_spmv<<<nb, ESB>>>(N, d_rp, d_col, d_val, d_Vd, d_dir, 0, d_q, nullptr, nullptr);
+    k_es_init<<<nb, ESB>>>(N, d_L, d_q, d_dir, d_D, d_r, d_z, d_p, d_part, d_part + nb);
+    ESCHK(hipGetLastError());
+    std::vector<double> hp(2 * (size_t)nb);
+    auto read2 = [&](double &a, double &b) {
+        ESCHK(hipMemcpy(hp.data(), d_part, hp.size() * sizeof(double), hipMemcpyDeviceToHost));
+        a = host_sum(std::vector<double>(hp.begin(), hp.begin() + nb));
+        b = host_sum(std::vector<double>(hp.begin() + nb, hp.end()));
+    };
+    double rr, zr;
+    read2(rr, zr);
+    // iteration<double> (src/algebra/iter.h:119-166)
+    const double rhsn = std::sqrt(rr);
+    double res = 0, rho = zr, rho_1 = 0;
+    int nit = 0, status = FG_ITER_UNDEFINED;
+    auto converged = [&](double nr) {
+        res = std::fabs(nr);
+        if (std::isnan(res)) {
+            status = FG_ITER_CANNOT_CONVERGE;
+            return false;
+        }
+        const bool cv = res <= rhsn * tol;
+        if (cv) status = FG_ITER_CONVERGED;
+        return cv;
+    };
+    while (!converged(std::sqrt(rr)) && status != FG_ITER_OVERFLOW && status != FG_ITER_CANNOT_CONVERGE) {
+        if (nit > 0) {
+            rho = zr;  // z = D r and z . r came with the last update
+            k_es_dir<<<nb, ESB>>>(N, rho / rho_1, d_z, d_p);
+        }
+        k_es_spmv<<<nb, ESB>>>(N, d_rp, d_col, d_val, d_p, d_dir, 1, d_q, d_p, d_part);
+        ESCHK(hipMemcpy(hp.data(), d_part, nb * sizeof(double), hipMemcpyDeviceToHost));
+        const double qp = host_sum(std::vector<double>(hp.begin(), hp.begin() + nb));
+        if (qp == 0.0) {
+            status = FG_ITER_CANNOT_CONVERGE;
+            break;
+        }
+        const double a = rho / qp;
+        k_es_update<<<nb, ESB>>>(N, a, d_p, d_q, d_D, d_x, d_r, d_z, d_part, d_part + nb);
+        read2(rr, zr);
+        rho_1 = rho;
+        if (++nit >= max_iter) status = FG_ITER_OVERFLOW;
+    }
+    k_es_add<<<nb, ESB>>>(N, d_Vd, d_x);
+    ESCHK(hipGetLastError());
+    ESCHK(hipMemcpy(V_out, d_x, (size_t)N * sizeof(double), hipMemcpyDeviceToHost));
+    if (info) {
+        *info = fg_solve_info{};
+        info->status = status;
+        info->iterations = nit;
+        info->residual = res;
+        info->rhs_norm = rhsn;
+    }
+}
+
+// The scratch of fg_electrostatic_potential_noalloc: one copy per GPU, held until the process ends.
+// Host code uses the device address hipGetSymbolAddress returns for the current device, never the
+// array's name.
+constexpr size_t ES_SCRATCH_BYTES = FG_ES_SCRATCH_BYTES;
+static __device__ __align__(256) char scratch[ES_SCRATCH_BYTES];
+constexpr int ES_MAX_DEVICES = 64;
+std::mutex es_scratch_mu;
+void *es_scratch_ptr[ES_MAX_DEVICES] = {};
+
+// The scratch's address on `device`, taken once per device with that device current.
+void *es_scratch(int device)
+{
+    std::lock_guard<std::mutex> lk(es_scratch_mu);
+    if (device < 0 || device >= ES_MAX_DEVICES) throw std::string("device index out of range");
+    if (!es_scratch_ptr[device]) {
+        void *p = nullptr;
+        ESCHK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        es_scratch_ptr[device] = p;
+    }
+    return es_scratch_ptr[device];
+}
+}  // namespace
+
+extern "C" int fg_electrostatic_potential(const fg_mesh *m, const double *sigma, const double *jn, const double *Vbc,
+                                          double tol, int32_t max_iter, int32_t device, double *V_out,
+                                          fg_solve_info *info)
+{
+    EsSetup S;
+    const int rc = es_setup(m, sigma, jn, Vbc, tol, max_iter, V_out, S);
+    if (rc != FG_OK) return rc;
     try {
         ESCHK(hipSetDevice(device));
         DevBufs B;
-        const int64_t nnz = P.rp[N];
-        const int nb = (N + ESB - 1) / ESB;
-        int4 *d_tn = B.upload(tn);
-        double *d_pos = B.upload(std::vector<double>(m->pos, m->pos + 3 * (size_t)N));
-        double *d_sig = B.upload(tsig);
-        double *d_ke = B.alloc<double>((size_t)16 * T);
-        int64_t *d_cptr = B.upload(P.cptr), *d_cslot = B.upload(P.cslot), *d_rp = B.upload(P.rp);
-        int32_t *d_col = B.upload(P.col);
-        double *d_val = B.alloc<double>(nnz);
-        uint8_t *d_dir = B.upload(dir);
-        double *d_L = B.upload(L), *d_Vd = B.upload(Vd);
-        double *d_D = B.alloc<double>(N), *d_x = B.alloc<double>(N), *d_r = B.alloc<double>(N),
-               *d_z = B.alloc<double>(N), *d_p = B.alloc<double>(N), *d_q = B.alloc<double>(N);
-        double *d_part = B.alloc<double>((size_t)2 * nb);
-        ESCHK(hipMemset(d_x, 0, (size_t)N * sizeof(double)));
-        k_es_element<<<(T + ESB - 1) / ESB, ESB>>>(T, d_tn, d_pos, d_sig, d_ke);
-        ESCHK(hipGetLastError());
-        k_es_gather<<<(unsigned)((nnz + ESB - 1) / ESB), ESB>>>(nnz, d_cptr, d_cslot, d_ke, d_val);
-        k_es_diag<<<nb, ESB>>>(N, d_rp, d_col, d_val, d_dir, d_D);
-        // b = L - K Vd (masked), r = b (the start is V = 0), z = D r, p = z
-        k_es_spmv<<<nb, ESB>>>(N, d_rp, d_col, d_val, d_Vd, d_dir, 0, d_q, nullptr, nullptr);
-        k_es_init<<<nb, ESB>>>(N, d_L, d_q, d_dir, d_D, d_r, d_z, d_p, d_part, d_part + nb);
-        ESCHK(hipGetLastError());
-        std::vector<double> hp(2 * (size_t)nb);
-        auto read2 = [&](double &a, double &b) {
-            ESCHK(hipMemcpy(hp.data(), d_part, hp.size() * sizeof(double), hipMemcpyDeviceToHost));
-            a = host_sum(std::vector<double>(hp.begin(), hp.begin() + nb));
-            b = host_sum(std::vector<double>(hp.begin() + nb, hp.end()));
-        };
-        double rr, zr;
-        read2(rr, zr);
-        // iteration<double> (src/algebra/iter.h:119-166)
-        const double rhsn = std::sqrt(rr);
-        double res = 0, rho = zr, rho_1 = 0;
-        int nit = 0, status = FG_ITER_UNDEFINED;
-        auto converged = [&](double nr) {
-            res = std::fabs(nr);
-            if (std::isnan(res)) {
-                status = FG_ITER_CANNOT_CONVERGE;
-                return false;
-            }
-            const bool cv = res <= rhsn * tol;
-            if (cv) status = FG_ITER_CONVERGED;
-            return cv;
-        };
-        while (!converged(std::sqrt(rr)) && status != FG_ITER_OVERFLOW && status != FG_ITER_CANNOT_CONVERGE) {
-            if (nit > 0) {
-                rho = zr;  // z = D r and z . r came with the last update
-                k_es_dir<<<nb, ESB>>>(N, rho / rho_1, d_z, d_p);
-            }
-            k_es_spmv<<<nb, ESB>>>(N, d_rp, d_col, d_val, d_p, d_dir, 1, d_q, d_p, d_part);
-            ESCHK(hipMemcpy(hp.data(), d_part, nb * sizeof(double), hipMemcpyDeviceToHost));
-            const double qp = host_sum(std::vector<double>(hp.begin(), hp.begin() + nb));
-            if (qp == 0.0) {
-                status = FG_ITER_CANNOT_CONVERGE;
-                break;
-            }
-            const double a = rho / qp;
-            k_es_update<<<nb, ESB>>>(N, a, d_p, d_q, d_D, d_x, d_r, d_z, d_part, d_part + nb);
-            read2(rr, zr);
-            rho_1 = rho;
-            if (++nit >= max_iter) status = FG_ITER_OVERFLOW;
-        }
-        k_es_add<<<nb, ESB>>>(N, d_Vd, d_x);
-        ESCHK(hipGetLastError());
-        ESCHK(hipMemcpy(V_out, d_x, (size_t)N * sizeof(double), hipMemcpyDeviceToHost));
-        if (info) {
-            *info = fg_solve_info{};
-            info->status = status;
-            info->iterations = nit;
-            info->residual = res;
-            info->rhs_norm = rhsn;
-        }
+        es_solve(B, m, S, tol, max_iter, V_out, info);
         return FG_OK;
+    } catch (const std::string &) {
+        return FG_ERR_DEVICE;
+    }
+}
+
+extern "C" int fg_electrostatic_potential_noalloc(const fg_mesh *m, const double *sigma, const double *jn,
+                                                  const double *Vbc, double tol, int32_t max_iter, int32_t device,
+                                                  double *V_out, fg_solve_info *info)
+{
+    EsSetup S;
+    const int rc = es_setup(m, sigma, jn, Vbc, tol, max_iter, V_out, S);
+    if (rc != FG_OK) return rc;
+    try {
+        ESCHK(hipSetDevice(device));
+        ArenaBufs B{(char *)es_scratch(device), ES_SCRATCH_BYTES};
+        es_solve(B, m, S, tol, max_iter, V_out, info);
+        return FG_OK;
+    } catch (const ScratchFull &) {
+        return FG_ERR_RANGE;
     } catch (const std::string &) {
         return FG_ERR_DEVICE;
     }

@@ -98,6 +98,8 @@ HIP_SYMBOLS = {
                                        C.c_double, C.c_int32, C.c_int32, _d, C.POINTER(SolveInfo)]),
     "fg_electrostatic_potential": (C.c_int, [C.c_void_p, _d, _d, _d, C.c_double, C.c_int32, C.c_int32, _d,
                                              C.POINTER(SolveInfo)]),
+    "fg_electrostatic_potential_noalloc": (C.c_int, [C.c_void_p, _d, _d, _d, C.c_double, C.c_int32, C.c_int32, _d,
+                                                     C.POINTER(SolveInfo)]),
     "fg_evolution": (C.c_int, [C.c_void_p]),
     "fg_max_angle": (C.c_int, [C.c_void_p, _d]),
     "fg_average": (C.c_int, [C.c_void_p, C.c_int, _d]),

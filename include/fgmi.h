@@ -234,6 +234,17 @@ int fg_set_spin_accumulation(fg_ctx *ctx, const double *s, const fg_spin_volume 
  * exits unless the status is FG_ITER_CONVERGED). Needs no context. */
 int fg_electrostatic_potential(const fg_mesh *mesh, const double *sigma, const double *jn, const double *V,
                                double tol, int32_t max_iter, int32_t device, double *V_out, fg_solve_info *info);
+/* fg_electrostatic_potential without any device allocation. No counterpart in the reference: it is
+ * for callers that cannot allocate device memory (code under HIP graph capture, serving loops,
+ * harnesses that forbid hipMalloc). Same arguments, kernels, results and return codes; every device
+ * buffer of the set-up and the solve comes from one static scratch array of FG_ES_SCRATCH_BYTES per
+ * GPU, held until the process ends. A mesh whose set-up needs more (about 1.8 kB per node of a
+ * tetrahedral mesh, i.e. some 35,000 nodes) returns FG_ERR_RANGE before any kernel runs. Calls on
+ * one GPU share that array: calls that overlap on one GPU race. */
+#define FG_ES_SCRATCH_BYTES ((size_t)64 << 20)
+int fg_electrostatic_potential_noalloc(const fg_mesh *mesh, const double *sigma, const double *jn, const double *V,
+                                      double tol, int32_t max_iter, int32_t device, double *V_out,
+                                      fg_solve_info *info);
 /* The spin accumulation of the spin-diffusion problem (spinAcc::checkBoundaryConditions,
  * boundaryConditions, solve, integrales, src/spinAccumulationSolver.cpp:41-287): the 3 n_nodes
  * system of the diffusion (D0 = 2 sigma / (e^2 N0), spin-flip length lsf) and, in the tetrahedra
