@@ -822,6 +822,27 @@ int fg_set_space_field(fg_ctx *x, const double *h)
     });
 }
 
+// The spin-accumulation buffers of a context and the prefactor of spinAcc::prepareExtraField
+// (src/spinAccumulationSolver.cpp:138-149), D0 / (lsd^2 gamma0 Ms) per magnetic region, into pref
+// (uploaded on the context's stream; the caller synchronises before pref goes)
+static void spin_field_setup(Ctx &c, const fg_spin_volume *vol, std::vector<double> &pref)
+{
+    const double E = 1.602176634e-19;
+    pref.assign(c.h_reg_Ms.size(), 0.0);
+    for (size_t rg = 0; rg < pref.size(); rg++)
+        if (c.h_reg_Ms[rg] > 0) {
+            const fg_spin_volume &p = vol[rg];
+            const double D0 = 2.0 * p.sigma / ((E * E) * p.N0);
+            pref[rg] = D0 / ((p.lsd * p.lsd) * GAMMA0 * c.h_reg_Ms[rg]);
+        }
+    if (!c.hst) {
+        c.hst = dalloc<double>((size_t)15 * c.Tm);
+        c.spin_s = dalloc<double>((size_t)3 * c.N);
+        c.spin_pref = dalloc<double>(pref.size());
+    }
+    HIPCHK(hipMemcpyAsync(c.spin_pref, pref.data(), pref.size() * sizeof(double), hipMemcpyHostToDevice, c.stream));
+}
+
 int fg_set_spin_accumulation(fg_ctx *x, const double *s, const fg_spin_volume *vol)
 {
     return guarded(x, [&]() {
@@ -834,24 +855,30 @@ int fg_set_spin_accumulation(fg_ctx *x, const double *s, const fg_spin_volume *v
             return FG_OK;
         }
         if (!vol) return FG_ERR_ARG;
-        // the prefactor of spinAcc::prepareExtraField (src/spinAccumulationSolver.cpp:138-149),
-        // D0 / (lsd^2 gamma0 Ms) per magnetic region; Hst itself on the device
-        const double E = 1.602176634e-19;
-        std::vector<double> pref(c.h_reg_Ms.size(), 0.0);
-        for (size_t rg = 0; rg < pref.size(); rg++)
-            if (c.h_reg_Ms[rg] > 0) {
-                const fg_spin_volume &p = vol[rg];
-                const double D0 = 2.0 * p.sigma / ((E * E) * p.N0);
-                pref[rg] = D0 / ((p.lsd * p.lsd) * GAMMA0 * c.h_reg_Ms[rg]);
-            }
-        if (!c.hst) {
-            c.hst = dalloc<double>((size_t)15 * c.Tm);
-            c.spin_s = dalloc<double>((size_t)3 * c.N);
-            c.spin_pref = dalloc<double>(pref.size());
-        }
+        std::vector<double> pref;
+        spin_field_setup(c, vol, pref);
         HIPCHK(hipMemcpyAsync(c.spin_s, s, (size_t)3 * c.N * sizeof(double), hipMemcpyHostToDevice, c.stream));
-        HIPCHK(hipMemcpyAsync(c.spin_pref, pref.data(), pref.size() * sizeof(double), hipMemcpyHostToDevice, c.stream));
+        launch_spin_field(c);  // Hst itself on the device
+        HIPCHK(hipStreamSynchronize(c.stream));
+        return FG_OK;
+    });
+}
+
+int fg_spin_solve_ctx(fg_spin *sp, fg_ctx *x, int32_t accumulate, double *s_out, fg_solve_info *info)
+{
+    return guarded(x, [&]() {
+        Ctx &c = x->c;
+        if (!sp) return FG_ERR_ARG;
+        int nreg = 0, nn = 0, dev = 0;
+        const fg_spin_volume *vol = spin_volumes(sp, &nreg, &nn, &dev);
+        if (nn != c.N || dev != c.device || nreg < (int)c.h_reg_Ms.size()) return FG_ERR_ARG;
+        std::vector<double> pref;
+        spin_field_setup(c, vol, pref);
+        const int rc = spin_solve_nodes(sp, c.rec[c.cur], c.rec[1 - c.cur], c.stream, accumulate, c.spin_s, info);
+        if (rc != FG_OK) return rc;
         launch_spin_field(c);
+        if (s_out)
+            HIPCHK(hipMemcpyAsync(s_out, c.spin_s, (size_t)3 * c.N * sizeof(double), hipMemcpyDeviceToHost, c.stream));
         HIPCHK(hipStreamSynchronize(c.stream));
         return FG_OK;
     });

@@ -189,4 +189,12 @@ void fmm_destroy(Ctx &c);
 void launch_fmm(Ctx &c);
 void fmm_info(const Ctx &c, int64_t info[16]);
 
+// ---- persistent spin-diffusion solver (transport.hip) ----
+// its volume regions, node count and device
+const fg_spin_volume *spin_volumes(const fg_spin *sp, int *n_regions, int *n_nodes, int *device);
+// one solve from the node states cur (s-d exchange) and nxt (source) on stream st; s, zero unless
+// converged, copied to s_dev[3 N] on the stream
+int spin_solve_nodes(fg_spin *sp, const NodeRec *cur, const NodeRec *nxt, hipStream_t st, int accumulate,
+                     double *s_dev, fg_solve_info *info);
+
 }  // namespace fg

@@ -367,15 +367,37 @@ __global__ __launch_bounds__(ESB) void k_sa_element(int T, const int4 *__restric
     }
 }
 
+// the assembly of one call added to the kept system (acc) or replacing it: matrix values in the
+// pattern's order, each entry's tetrahedra in mesh order after the kept value, as solver::buildMat
+// adds into K (src/solver.h:110-129)
+__global__ __launch_bounds__(ESB) void k_sp_gather(int64_t nnz, const int64_t *__restrict__ cptr,
+                                                   const int64_t *__restrict__ cslot, const double *__restrict__ ke,
+                                                   int acc, double *__restrict__ val)
+{
+    const int64_t k = (int64_t)blockIdx.x * ESB + threadIdx.x;
+    if (k >= nnz) return;
+    double s = acc ? val[k] : 0.0;
+    for (int64_t c = cptr[k]; c < cptr[k + 1]; c++) s += ke[cslot[c]];
+    val[k] = s;
+}
+
 // per node, in mesh order of its tetrahedra: the s-d exchange sums S (X, Y, Z) and the volume
-// right-hand side; slot = local position * T + tetrahedron
+// right-hand side after the kept values (acc), then the surface right-hand side Ls of the
+// triangles (solver::buildVect, src/solver.h:131-142, volume terms first, :173-197);
+// slot = local position * T + tetrahedron
 __global__ __launch_bounds__(ESB) void k_sa_node(int N, int T, const int64_t *__restrict__ nptr,
                                                  const int64_t *__restrict__ nslot, const double *__restrict__ ke,
-                                                 double *__restrict__ S, double *__restrict__ L)
+                                                 const double *__restrict__ Ls, int acc, double *__restrict__ S,
+                                                 double *__restrict__ L)
 {
     const int i = blockIdx.x * ESB + threadIdx.x;
     if (i >= N) return;
-    double sk[3] = {0.0, 0.0, 0.0}, b[3] = {0.0, 0.0, 0.0};
+    double sk[3], b[3];
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        sk[k] = acc ? S[3 * (size_t)i + k] : 0.0;
+        b[k] = acc ? L[3 * (size_t)i + k] : 0.0;
+    }
     for (int64_t c = nptr[i]; c < nptr[i + 1]; c++) {
         const int64_t l = nslot[c] / T, t = nslot[c] % T;
 #pragma unroll
@@ -387,9 +409,46 @@ __global__ __launch_bounds__(ESB) void k_sa_node(int N, int T, const int64_t *__
 #pragma unroll
     for (int k = 0; k < 3; k++) {
         S[3 * (size_t)i + k] = sk[k];
-        L[3 * (size_t)i + k] = b[k];
+        L[3 * (size_t)i + k] = b[k] + Ls[3 * (size_t)i + k];
     }
 }
+
+// the node magnetizations of an engine state: u from CURRENT (the s-d exchange) and u_src from
+// NEXT (the source)
+__global__ __launch_bounds__(ESB) void k_sp_rec_u(int N, const NodeRec *__restrict__ cur,
+                                                  const NodeRec *__restrict__ nxt, double *__restrict__ u,
+                                                  double *__restrict__ us)
+{
+    const int i = blockIdx.x * ESB + threadIdx.x;
+    if (i >= N) return;
+    const NodeRec a = cur[i], b = nxt[i];
+    u[3 * (size_t)i] = a.ux;
+    u[3 * (size_t)i + 1] = a.uy;
+    u[3 * (size_t)i + 2] = a.uz;
+    us[3 * (size_t)i] = b.ux;
+    us[3 * (size_t)i + 1] = b.uy;
+    us[3 * (size_t)i + 2] = b.uz;
+}
+
+// Scalars of algebra::bicg_dir and algebra::iteration (src/algebra/bicg.h:82-158,
+// src/algebra/iter.h:119-178) in device memory, updated by one-block kernels between the vector
+// phases so that chunks of iterations run with no host read. stop: the loop has ended; fin = 1:
+// its last half step (x += alpha phat) is due, 2: taken.
+struct SpinCtl {
+    double rho1, rho2, alpha, omega, beta, rhsn, res, tol;
+    int nit, max_iter, status, stop, fin, pad[3];
+};
+
+__global__ void k_sp_reset(SpinCtl *c, double tol, int max_iter)
+{
+    if (threadIdx.x) return;
+    *c = SpinCtl{};
+    c->tol = tol;
+    c->max_iter = max_iter;
+    c->status = FG_ITER_UNDEFINED;
+}
+
+enum SpinPhase { SP_INIT, SP_ALPHA, SP_S, SP_OMEGA, SP_LOOP };
 
 // entry (d, e) of node i's own 3x3 block: the diffusion diagonal, and the antisymmetric s-d
 // exchange (rows y, z of X; x, z of Y; x, y of Z, with the signs of spinAcc::integrales)
@@ -402,14 +461,16 @@ __device__ inline double self_block(int d, int e, double a, const double *S)
 }
 
 // y = K x over the 3 N rows (row 3 i + d), zero on the Dirichlet rows when mask is set; partials of
-// y . w (w may be null) and, when pyy is set, of y . y
+// y . w (w may be null) and, when pyy is set, of y . y. Nothing once the solver has stopped.
 __global__ __launch_bounds__(ESB) void k_sa_spmv(int n3, const int64_t *__restrict__ rp, const int32_t *__restrict__ col,
                                                  const double *__restrict__ a, const double *__restrict__ S,
                                                  const double *__restrict__ x, const uint8_t *__restrict__ dir, int mask,
                                                  double *__restrict__ y, const double *__restrict__ w,
-                                                 double *__restrict__ pyw, double *__restrict__ pyy)
+                                                 double *__restrict__ pyw, double *__restrict__ pyy,
+                                                 const SpinCtl *__restrict__ ctl)
 {
     __shared__ double sh[ESB];
+    if (ctl && ctl->stop) return;
     const int r = blockIdx.x * ESB + threadIdx.x;
     double yw = 0.0, yy = 0.0;
     if (r < n3) {
@@ -447,10 +508,11 @@ __global__ __launch_bounds__(ESB) void k_sa_diag(int n3, const int64_t *__restri
     D[r] = dir[i] ? 0.0 : 1.0 / c;
 }
 
-// b = L - K xd, masked; r = rt = p = b; partials of b . b
+// b = L - K xd, masked; r = rt = p = b; x = 0; partials of b . b
 __global__ __launch_bounds__(ESB) void k_sa_init(int n3, const double *__restrict__ L, const double *__restrict__ kxd,
                                                  const uint8_t *__restrict__ dir, double *__restrict__ r,
-                                                 double *__restrict__ rt, double *__restrict__ p, double *__restrict__ pbb)
+                                                 double *__restrict__ rt, double *__restrict__ p, double *__restrict__ x,
+                                                 double *__restrict__ pbb)
 {
     __shared__ double sh[ESB];
     const int k = blockIdx.x * ESB + threadIdx.x;
@@ -462,22 +524,103 @@ __global__ __launch_bounds__(ESB) void k_sa_init(int n3, const double *__restric
         r[k] = b;
         rt[k] = b;
         p[k] = b;
+        x[k] = 0.0;
         bb = b * b;
     }
     block_sum(bb, sh, pbb + blockIdx.x);
 }
 
-// p = (p - omega v) beta + r unless first; phat = D p
-__global__ __launch_bounds__(ESB) void k_sa_p(int n3, int first, double omega, double beta, const double *__restrict__ v,
+// The scalar step that follows a reduction: the partials of one (two for SP_OMEGA and SP_LOOP)
+// dot products, nb each, summed in a fixed order per thread and a fixed tree; thread 0 then takes
+// the step of bicg_dir that needs them.
+__global__ __launch_bounds__(ESB) void k_sp_ctl(int nb, const double *__restrict__ part, SpinCtl *__restrict__ c,
+                                                int phase)
+{
+    __shared__ double sh[2][ESB];
+    if (c->stop && !(phase == SP_LOOP && c->fin == 1)) return;
+    const int nsum = (phase == SP_OMEGA || phase == SP_LOOP) ? 2 : 1;
+    for (int q = 0; q < nsum; q++) {
+        double s = 0.0;
+        for (int b = threadIdx.x; b < nb; b += ESB) s += part[(size_t)q * nb + b];
+        sh[q][threadIdx.x] = s;
+    }
+    __syncthreads();
+    for (int w = ESB / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w)
+            for (int q = 0; q < nsum; q++) sh[q][threadIdx.x] += sh[q][threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x) return;
+    const double a0 = sh[0][0], a1 = nsum > 1 ? sh[1][0] : 0.0;
+    // iteration::converged (src/algebra/iter.h:145-166): NaN sets CANNOT_CONVERGE without ending
+    // the loop, as in the reference
+    auto converged = [c](double nr) {
+        c->res = fabs(nr);
+        if (isnan(c->res)) {
+            c->status = FG_ITER_CANNOT_CONVERGE;
+            return false;
+        }
+        const bool cv = c->res <= c->rhsn * c->tol;
+        if (cv) c->status = FG_ITER_CONVERGED;
+        return cv;
+    };
+    switch (phase) {
+    case SP_INIT:  // ||b||, the loop's first test, rho1 = rt . r with rt = r = b
+        c->rhsn = sqrt(a0);
+        if (converged(sqrt(a0))) {
+            c->stop = 1;
+            return;
+        }
+        c->rho1 = a0;
+        break;
+    case SP_ALPHA:  // alpha = rho1 / (v . rt)
+        c->alpha = c->rho1 / a0;
+        break;
+    case SP_S:  // the test on ||s||: the last half step, or a break on overflow / breakdown
+        if (converged(sqrt(a0))) {
+            c->stop = 1;
+            c->fin = 1;
+        } else if (c->status == FG_ITER_OVERFLOW || c->status == FG_ITER_CANNOT_CONVERGE) {
+            c->stop = 1;
+        }
+        break;
+    case SP_OMEGA:  // omega = (t . s) / (t . t)
+        c->omega = a0 / a1;
+        break;
+    case SP_LOOP:  // rho2 = rho1, ++iter, the loop's test on ||r||, rho1 = rt . r, beta
+        if (c->stop) {
+            c->fin = 2;
+            return;
+        }
+        c->rho2 = c->rho1;
+        if (++c->nit >= c->max_iter) c->status = FG_ITER_OVERFLOW;
+        if (converged(sqrt(a0))) {
+            c->stop = 1;
+            return;
+        }
+        c->rho1 = a1;
+        if (c->rho2 == 0 || c->omega == 0) {
+            c->status = FG_ITER_CANNOT_CONVERGE;
+            c->stop = 1;
+            return;
+        }
+        c->beta = (c->rho1 / c->rho2) * (c->alpha / c->omega);
+        break;
+    }
+}
+
+// p = (p - omega v) beta + r after the first iteration; phat = D p
+__global__ __launch_bounds__(ESB) void k_sp_p(int n3, const SpinCtl *__restrict__ c, const double *__restrict__ v,
                                               const double *__restrict__ r, const double *__restrict__ D,
                                               double *__restrict__ p, double *__restrict__ phat)
 {
+    if (c->stop) return;
     const int k = blockIdx.x * ESB + threadIdx.x;
     if (k >= n3) return;
     double pv = p[k];
-    if (!first) {
-        pv -= v[k] * omega;
-        pv *= beta;
+    if (c->nit > 0) {
+        pv -= v[k] * c->omega;
+        pv *= c->beta;
         pv += r[k];
         p[k] = pv;
     }
@@ -485,15 +628,16 @@ __global__ __launch_bounds__(ESB) void k_sa_p(int n3, int first, double omega, d
 }
 
 // s = r - alpha v; shat = D s; partials of s . s
-__global__ __launch_bounds__(ESB) void k_sa_s(int n3, double alpha, const double *__restrict__ r,
+__global__ __launch_bounds__(ESB) void k_sp_s(int n3, const SpinCtl *__restrict__ c, const double *__restrict__ r,
                                               const double *__restrict__ v, const double *__restrict__ D,
                                               double *__restrict__ sv, double *__restrict__ shat, double *__restrict__ pss)
 {
     __shared__ double sh[ESB];
+    if (c->stop) return;
     const int k = blockIdx.x * ESB + threadIdx.x;
     double ss = 0.0;
     if (k < n3) {
-        const double s = r[k] + (-alpha) * v[k];
+        const double s = r[k] + (-c->alpha) * v[k];
         sv[k] = s;
         shat[k] = D[k] * s;
         ss = s * s;
@@ -501,15 +645,9 @@ __global__ __launch_bounds__(ESB) void k_sa_s(int n3, double alpha, const double
     block_sum(ss, sh, pss + blockIdx.x);
 }
 
-// x += alpha phat
-__global__ __launch_bounds__(ESB) void k_sa_x1(int n3, double alpha, const double *__restrict__ phat, double *__restrict__ x)
-{
-    const int k = blockIdx.x * ESB + threadIdx.x;
-    if (k < n3) x[k] += alpha * phat[k];
-}
-
-// x += alpha phat; x += omega shat; r = s - omega t; partials of r . r and rt . r
-__global__ __launch_bounds__(ESB) void k_sa_x(int n3, double alpha, double omega, const double *__restrict__ phat,
+// x += alpha phat; x += omega shat; r = s - omega t; partials of r . r and rt . r. Once stopped:
+// the last half step x += alpha phat if it is due.
+__global__ __launch_bounds__(ESB) void k_sp_x(int n3, const SpinCtl *__restrict__ c, const double *__restrict__ phat,
                                               const double *__restrict__ shat, const double *__restrict__ sv,
                                               const double *__restrict__ tv, const double *__restrict__ rt,
                                               double *__restrict__ x, double *__restrict__ r, double *__restrict__ prr,
@@ -517,6 +655,11 @@ __global__ __launch_bounds__(ESB) void k_sa_x(int n3, double alpha, double omega
 {
     __shared__ double sh[ESB];
     const int k = blockIdx.x * ESB + threadIdx.x;
+    if (c->stop) {
+        if (c->fin == 1 && k < n3) x[k] += c->alpha * phat[k];
+        return;
+    }
+    const double alpha = c->alpha, omega = c->omega;
     double rr = 0.0, rtr = 0.0;
     if (k < n3) {
         double xv = x[k];
@@ -530,6 +673,16 @@ __global__ __launch_bounds__(ESB) void k_sa_x(int n3, double alpha, double omega
     }
     block_sum(rr, sh, prr + blockIdx.x);
     block_sum(rtr, sh, prt + blockIdx.x);
+}
+
+// x += xd (the Dirichlet values); zero everywhere when zero_failed is set and the solver did not
+// converge (spinAcc::compute, src/spinAccumulationSolver.cpp:151-163)
+__global__ __launch_bounds__(ESB) void k_sp_finish(int n3, const SpinCtl *__restrict__ c, int zero_failed,
+                                                   const double *__restrict__ xd, double *__restrict__ x)
+{
+    const int k = blockIdx.x * ESB + threadIdx.x;
+    if (k >= n3) return;
+    x[k] = (zero_failed && c->status != FG_ITER_CONVERGED) ? 0.0 : x[k] + xd[k];
 }
 
 // The CSR pattern of the node graph of the tetrahedra, and for each entry the list of its
@@ -793,13 +946,46 @@ extern "C" int fg_electrostatic_potential_noalloc(const fg_mesh *m, const double
     }
 }
 
-extern "C" int fg_spin_accumulation(const fg_mesh *m, const fg_spin_volume *vol, const fg_spin_surface *surf,
-                                    const double *Vn, const double *u, const double *u_src, double tol,
-                                    int32_t max_iter, int32_t device, double *s_out, fg_solve_info *info)
+// ---- the persistent spin-diffusion solver ----
+
+// spinAcc (src/spinAccumulationSolver.h): what does not change between solves, built once (the
+// parameters, the node pattern, the Dirichlet values, the surface right-hand side, the device
+// buffers), and the system K, L that the solves with accumulate set add into.
+struct fg_spin {
+    int device = 0, N = 0, T = 0, n3 = 0, nb = 0, nb3 = 0;
+    int64_t nnz = 0;
+    double tol = 0;
+    int32_t max_iter = 0;
+    int assemblies = 0;  // assemblies summed in K and L
+    std::vector<fg_spin_volume> vol;
+    hipStream_t stream = nullptr;
+    DevBufs B;
+    int4 *tn = nullptr;
+    double4 *prm = nullptr;
+    uint8_t *mag = nullptr, *dir = nullptr;
+    int64_t *cptr = nullptr, *cslot = nullptr, *rp = nullptr, *nptr = nullptr, *nslot = nullptr;
+    int32_t *col = nullptr;
+    double *pos = nullptr, *V = nullptr, *u = nullptr, *us = nullptr, *ke = nullptr, *a = nullptr, *S = nullptr,
+           *L = nullptr, *Ls = nullptr, *Vd = nullptr, *D = nullptr, *x = nullptr, *r = nullptr, *rt = nullptr,
+           *p = nullptr, *ph = nullptr, *v = nullptr, *s = nullptr, *sh = nullptr, *t = nullptr, *part = nullptr;
+    SpinCtl *ctl = nullptr, *h_ctl = nullptr;
+    ~fg_spin()
+    {
+        if (h_ctl) (void)hipHostFree(h_ctl);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+namespace {
+
+// spinAcc::spinAcc up to the first solve (src/spinAccumulationSolver.cpp:8-36,41-116): the
+// boundary-condition check, the per-tetrahedron parameters, the node pattern, the Dirichlet values
+// and the surface terms of solve (:173-197, the same in every call), then the device buffers.
+int spin_setup(const fg_mesh *m, const fg_spin_volume *vol, const fg_spin_surface *surf, const double *Vn,
+               double tol, int32_t max_iter, int32_t device, fg_spin &sp)
 {
-    if (!m || !vol || !surf || !Vn || !u || !s_out || !m->pos || !m->tet_nodes || !m->tet_region || !m->vol ||
-        m->n_nodes <= 0 || m->n_tets <= 0 || (m->n_tris > 0 && (!m->tri_nodes || !m->tri_region)) || !(tol > 0) ||
-        max_iter < 1)
+    if (!m || !vol || !surf || !Vn || !m->pos || !m->tet_nodes || !m->tet_region || !m->vol || m->n_nodes <= 0 ||
+        m->n_tets <= 0 || (m->n_tris > 0 && (!m->tri_nodes || !m->tri_region)) || !(tol > 0) || max_iter < 1)
         return FG_ERR_ARG;
     // spinAcc::checkBoundaryConditions (src/spinAccumulationSolver.cpp:41-79)
     auto finite3 = [](const double *v) { return std::isfinite(std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2])); };
@@ -838,13 +1024,15 @@ extern "C" int fg_spin_accumulation(const fg_mesh *m, const fg_spin_volume *vol,
         for (int t = 0; t < T; t++)
             for (int l = 0; l < 4; l++) nslot[fill[m->tet_nodes[4 * (size_t)t + l]]++] = (int64_t)l * T + t;
     }
-    // Dirichlet values (spinAcc::boundaryConditions, :91-116), in triangle order
-    std::vector<double> Vd(n3, 0.0);
+    // Dirichlet values (spinAcc::boundaryConditions, :91-116) and the surface right-hand side
+    // (solve, :173-197), both in triangle order
+    std::vector<double> Vd(n3, 0.0), Ls(n3, 0.0);
     std::vector<uint8_t> dir(N, 0);
     for (int f = 0; f < m->n_tris; f++) {
         const int rg = m->tri_region[f];
         if (rg < 0 || rg >= m->n_surf_regions) continue;
         const fg_spin_surface &p = surf[rg];
+        const int32_t *q = m->tri_nodes + 3 * (size_t)f;
         double dv[3];
         bool d = false;
         if (std::isnan(std::sqrt(p.s[0] * p.s[0] + p.s[1] * p.s[1] + p.s[2] * p.s[2])) && std::isfinite(p.jn)) {
@@ -856,138 +1044,202 @@ extern "C" int fg_spin_accumulation(const fg_mesh *m, const fg_spin_volume *vol,
         }
         if (d)
             for (int i = 0; i < 3; i++) {
-                const int q = m->tri_nodes[3 * (size_t)f + i];
-                for (int k = 0; k < 3; k++) Vd[3 * (size_t)q + k] = dv[k];
-                dir[q] = 1;
+                for (int k = 0; k < 3; k++) Vd[3 * (size_t)q[i] + k] = dv[k];
+                dir[q[i]] = 1;
             }
+        double sv[3] = {p.s[0], p.s[1], p.s[2]};
+        if (std::isfinite(p.jn) && finite3(p.uP))
+            for (int k = 0; k < 3; k++) sv[k] = -p.jn * (MUB / E) * p.uP[k];
+        if (!finite3(sv)) continue;
+        const double surfa = tri_area(m, f);
+        double Le[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+        for (int g = 0; g < NPI_TRI; g++) {
+            const double wg = 2.0 * surfa * TRI_PDS[g];
+            for (int i = 0; i < 3; i++) {
+                const double aiw = wg * tri_a(i, g);
+                for (int k = 0; k < 3; k++) Le[3 * k + i] -= sv[k] * aiw;
+            }
+        }
+        for (int i = 0; i < 3; i++)
+            for (int k = 0; k < 3; k++) Ls[3 * (size_t)q[i] + k] += Le[3 * k + i];
     }
+    sp.device = device;
+    sp.N = N;
+    sp.T = T;
+    sp.n3 = n3;
+    sp.nb = (N + ESB - 1) / ESB;
+    sp.nb3 = (n3 + ESB - 1) / ESB;
+    sp.nnz = P.rp[N];
+    sp.tol = tol;
+    sp.max_iter = max_iter;
+    sp.vol.assign(vol, vol + m->n_vol_regions);
+    ESCHK(hipSetDevice(device));
+    ESCHK(hipStreamCreateWithFlags(&sp.stream, hipStreamNonBlocking));
+    DevBufs &B = sp.B;
+    sp.tn = B.upload(tn);
+    sp.pos = B.upload(std::vector<double>(m->pos, m->pos + 3 * (size_t)N));
+    sp.prm = B.upload(prm);
+    sp.mag = B.upload(mag);
+    sp.dir = B.upload(dir);
+    sp.V = B.upload(std::vector<double>(Vn, Vn + N));
+    sp.u = B.alloc<double>(n3);
+    sp.us = B.alloc<double>(n3);
+    sp.ke = B.alloc<double>((size_t)40 * T);
+    sp.cptr = B.upload(P.cptr);
+    sp.cslot = B.upload(P.cslot);
+    sp.rp = B.upload(P.rp);
+    sp.col = B.upload(P.col);
+    sp.nptr = B.upload(nptr);
+    sp.nslot = B.upload(nslot);
+    sp.a = B.alloc<double>(sp.nnz);
+    sp.Vd = B.upload(Vd);
+    sp.Ls = B.upload(Ls);
+    for (double **q : {&sp.S, &sp.L, &sp.D, &sp.x, &sp.r, &sp.rt, &sp.p, &sp.ph, &sp.v, &sp.s, &sp.sh, &sp.t})
+        *q = B.alloc<double>(n3);
+    sp.part = B.alloc<double>((size_t)2 * sp.nb3);
+    sp.ctl = B.alloc<SpinCtl>(1);
+    ESCHK(hipHostMalloc((void **)&sp.h_ctl, sizeof(SpinCtl), hipHostMallocDefault));
+    return FG_OK;
+}
+
+// spinAcc::solve (src/spinAccumulationSolver.cpp:165-209) at the magnetizations in sp.u (s-d
+// exchange) and sp.us (source), on stream st: the assembly, added to the kept system when acc is
+// set, then bicg_dir from s = 0 with its scalars on the device; s in sp.x.
+void spin_run(fg_spin &sp, hipStream_t st, int acc, int zero_failed, fg_solve_info *info)
+{
+    const int T = sp.T, N = sp.N, n3 = sp.n3, nb3 = sp.nb3;
+    acc = acc && sp.assemblies > 0;
+    k_sp_reset<<<1, 64, 0, st>>>(sp.ctl, sp.tol, sp.max_iter);
+    k_sa_element<<<(T + ESB - 1) / ESB, ESB, 0, st>>>(T, sp.tn, sp.pos, sp.prm, sp.mag, sp.V, sp.u, sp.us, sp.ke);
+    k_sp_gather<<<(unsigned)((sp.nnz + ESB - 1) / ESB), ESB, 0, st>>>(sp.nnz, sp.cptr, sp.cslot, sp.ke, acc, sp.a);
+    k_sa_node<<<sp.nb, ESB, 0, st>>>(N, T, sp.nptr, sp.nslot, sp.ke, sp.Ls, acc, sp.S, sp.L);
+    sp.assemblies = acc ? sp.assemblies + 1 : 1;
+    k_sa_diag<<<nb3, ESB, 0, st>>>(n3, sp.rp, sp.col, sp.a, sp.dir, sp.D);
+    k_sa_spmv<<<nb3, ESB, 0, st>>>(n3, sp.rp, sp.col, sp.a, sp.S, sp.Vd, sp.dir, 0, sp.v, nullptr, nullptr, nullptr,
+                                   nullptr);
+    k_sa_init<<<nb3, ESB, 0, st>>>(n3, sp.L, sp.v, sp.dir, sp.r, sp.rt, sp.p, sp.x, sp.part);
+    k_sp_ctl<<<1, ESB, 0, st>>>(nb3, sp.part, sp.ctl, SP_INIT);
+    ESCHK(hipGetLastError());
+    // chunks of iterations between host reads of the control block; bicg_dir ends by itself after
+    // at most max_iter + 1 iterations (the overflow status breaks the one after the max_iter-th)
+    constexpr int CHUNK = 16;
+    const int max_chunks = sp.max_iter / CHUNK + 2;
+    double *pv = sp.part, *pt = sp.part + nb3;
+    for (int ch = 0;; ch++) {
+        ESCHK(hipMemcpyAsync(sp.h_ctl, sp.ctl, sizeof(SpinCtl), hipMemcpyDeviceToHost, st));
+        ESCHK(hipStreamSynchronize(st));
+        if (sp.h_ctl->stop) break;
+        if (ch >= max_chunks) throw std::string("spin solver: bicg_dir did not end");
+        for (int it = 0; it < CHUNK; it++) {
+            k_sp_p<<<nb3, ESB, 0, st>>>(n3, sp.ctl, sp.v, sp.r, sp.D, sp.p, sp.ph);
+            k_sa_spmv<<<nb3, ESB, 0, st>>>(n3, sp.rp, sp.col, sp.a, sp.S, sp.ph, sp.dir, 1, sp.v, sp.rt, pv, nullptr,
+                                           sp.ctl);
+            k_sp_ctl<<<1, ESB, 0, st>>>(nb3, sp.part, sp.ctl, SP_ALPHA);
+            k_sp_s<<<nb3, ESB, 0, st>>>(n3, sp.ctl, sp.r, sp.v, sp.D, sp.s, sp.sh, pv);
+            k_sp_ctl<<<1, ESB, 0, st>>>(nb3, sp.part, sp.ctl, SP_S);
+            k_sa_spmv<<<nb3, ESB, 0, st>>>(n3, sp.rp, sp.col, sp.a, sp.S, sp.sh, sp.dir, 1, sp.t, sp.s, pv, pt, sp.ctl);
+            k_sp_ctl<<<1, ESB, 0, st>>>(nb3, sp.part, sp.ctl, SP_OMEGA);
+            k_sp_x<<<nb3, ESB, 0, st>>>(n3, sp.ctl, sp.ph, sp.sh, sp.s, sp.t, sp.rt, sp.x, sp.r, pv, pt);
+            k_sp_ctl<<<1, ESB, 0, st>>>(nb3, sp.part, sp.ctl, SP_LOOP);
+        }
+        ESCHK(hipGetLastError());
+    }
+    k_sp_finish<<<nb3, ESB, 0, st>>>(n3, sp.ctl, zero_failed, sp.Vd, sp.x);
+    ESCHK(hipGetLastError());
+    if (info) {
+        const SpinCtl &h = *sp.h_ctl;
+        *info = fg_solve_info{};
+        info->status = h.status;
+        info->iterations = h.nit;
+        info->residual = h.res;
+        info->rhs_norm = h.rhsn;
+    }
+}
+
+}  // namespace
+
+namespace fg {
+
+const fg_spin_volume *spin_volumes(const fg_spin *sp, int *n_regions, int *n_nodes, int *device)
+{
+    *n_regions = (int)sp->vol.size();
+    *n_nodes = sp->N;
+    *device = sp->device;
+    return sp->vol.data();
+}
+
+int spin_solve_nodes(fg_spin *sp, const NodeRec *cur, const NodeRec *nxt, hipStream_t st, int accumulate,
+                     double *s_dev, fg_solve_info *info)
+{
     try {
-        ESCHK(hipSetDevice(device));
-        DevBufs B;
-        const int64_t nnz = P.rp[N];
-        const int nb = (N + ESB - 1) / ESB, nb3 = (n3 + ESB - 1) / ESB;
-        int4 *d_tn = B.upload(tn);
-        double *d_pos = B.upload(std::vector<double>(m->pos, m->pos + 3 * (size_t)N));
-        double4 *d_prm = B.upload(prm);
-        uint8_t *d_mag = B.upload(mag), *d_dir = B.upload(dir);
-        double *d_V = B.upload(std::vector<double>(Vn, Vn + N));
-        double *d_u = B.upload(std::vector<double>(u, u + (size_t)n3));
-        double *d_us = u_src ? B.upload(std::vector<double>(u_src, u_src + (size_t)n3)) : d_u;
-        double *d_ke = B.alloc<double>((size_t)40 * T);
-        int64_t *d_cptr = B.upload(P.cptr), *d_cslot = B.upload(P.cslot), *d_rp = B.upload(P.rp);
-        int64_t *d_nptr = B.upload(nptr), *d_nslot = B.upload(nslot);
-        int32_t *d_col = B.upload(P.col);
-        double *d_a = B.alloc<double>(nnz), *d_S = B.alloc<double>(n3), *d_L = B.alloc<double>(n3);
-        double *d_Vd = B.upload(Vd);
-        double *d_D = B.alloc<double>(n3), *d_x = B.alloc<double>(n3), *d_r = B.alloc<double>(n3),
-               *d_rt = B.alloc<double>(n3), *d_p = B.alloc<double>(n3), *d_ph = B.alloc<double>(n3),
-               *d_v = B.alloc<double>(n3), *d_s = B.alloc<double>(n3), *d_sh = B.alloc<double>(n3),
-               *d_t = B.alloc<double>(n3);
-        double *d_part = B.alloc<double>((size_t)2 * nb3);
-        ESCHK(hipMemset(d_x, 0, (size_t)n3 * sizeof(double)));
-        k_sa_element<<<(T + ESB - 1) / ESB, ESB>>>(T, d_tn, d_pos, d_prm, d_mag, d_V, d_u, d_us, d_ke);
-        ESCHK(hipGetLastError());
-        k_es_gather<<<(unsigned)((nnz + ESB - 1) / ESB), ESB>>>(nnz, d_cptr, d_cslot, d_ke, d_a);
-        k_sa_node<<<nb, ESB>>>(N, T, d_nptr, d_nslot, d_ke, d_S, d_L);
-        ESCHK(hipGetLastError());
-        // the surface terms after the volume ones (solver::buildVect order, :173-197)
-        std::vector<double> L(n3);
-        ESCHK(hipMemcpy(L.data(), d_L, (size_t)n3 * sizeof(double), hipMemcpyDeviceToHost));
-        for (int f = 0; f < m->n_tris; f++) {
-            const int rg = m->tri_region[f];
-            if (rg < 0 || rg >= m->n_surf_regions) continue;
-            const fg_spin_surface &p = surf[rg];
-            double sv[3] = {p.s[0], p.s[1], p.s[2]};
-            if (std::isfinite(p.jn) && finite3(p.uP))
-                for (int k = 0; k < 3; k++) sv[k] = -p.jn * (MUB / E) * p.uP[k];
-            if (!finite3(sv)) continue;
-            const double surfa = tri_area(m, f);
-            double Le[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-            for (int g = 0; g < NPI_TRI; g++) {
-                const double wg = 2.0 * surfa * TRI_PDS[g];
-                for (int i = 0; i < 3; i++) {
-                    const double aiw = wg * tri_a(i, g);
-                    for (int k = 0; k < 3; k++) Le[3 * k + i] -= sv[k] * aiw;
-                }
-            }
-            for (int i = 0; i < 3; i++)
-                for (int k = 0; k < 3; k++) L[3 * (size_t)m->tri_nodes[3 * (size_t)f + i] + k] += Le[3 * k + i];
-        }
-        ESCHK(hipMemcpy(d_L, L.data(), (size_t)n3 * sizeof(double), hipMemcpyHostToDevice));
-        k_sa_diag<<<nb3, ESB>>>(n3, d_rp, d_col, d_a, d_dir, d_D);
-        k_sa_spmv<<<nb3, ESB>>>(n3, d_rp, d_col, d_a, d_S, d_Vd, d_dir, 0, d_v, nullptr, nullptr, nullptr);
-        k_sa_init<<<nb3, ESB>>>(n3, d_L, d_v, d_dir, d_r, d_rt, d_p, d_part);
-        ESCHK(hipGetLastError());
-        std::vector<double> hp(2 * (size_t)nb3);
-        auto read = [&](int k, double *out) {
-            ESCHK(hipMemcpy(hp.data(), d_part, (size_t)k * nb3 * sizeof(double), hipMemcpyDeviceToHost));
-            for (int q = 0; q < k; q++) out[q] = host_sum(std::vector<double>(hp.begin() + (size_t)q * nb3,
-                                                                              hp.begin() + (size_t)(q + 1) * nb3));
-        };
-        double rr;
-        read(1, &rr);
-        double rtr = rr;  // rt = r
-        // iteration<double> (src/algebra/iter.h:119-166) and the 6-argument bicg_dir
-        const double rhsn = std::sqrt(rr);
-        double res = 0, rho_1 = 0, rho_2 = 0, alpha = 0, beta = 0, omega = 0;
-        int nit = 0, status = FG_ITER_UNDEFINED;
-        auto converged = [&](double nr) {
-            res = std::fabs(nr);
-            if (std::isnan(res)) {
-                status = FG_ITER_CANNOT_CONVERGE;
-                return false;
-            }
-            const bool cv = res <= rhsn * tol;
-            if (cv) status = FG_ITER_CONVERGED;
-            return cv;
-        };
-        while (!converged(std::sqrt(rr))) {
-            rho_1 = rtr;
-            if (nit > 0) {
-                if (rho_2 == 0 || omega == 0) {
-                    status = FG_ITER_CANNOT_CONVERGE;
-                    break;
-                }
-                beta = (rho_1 / rho_2) * (alpha / omega);
-            }
-            k_sa_p<<<nb3, ESB>>>(n3, nit == 0, omega, beta, d_v, d_r, d_D, d_p, d_ph);
-            k_sa_spmv<<<nb3, ESB>>>(n3, d_rp, d_col, d_a, d_S, d_ph, d_dir, 1, d_v, d_rt, d_part, nullptr);
-            double vrt;
-            read(1, &vrt);
-            alpha = rho_1 / vrt;
-            k_sa_s<<<nb3, ESB>>>(n3, alpha, d_r, d_v, d_D, d_s, d_sh, d_part);
-            double ss;
-            read(1, &ss);
-            if (converged(std::sqrt(ss))) {
-                k_sa_x1<<<nb3, ESB>>>(n3, alpha, d_ph, d_x);
-                break;
-            } else if (status == FG_ITER_OVERFLOW || status == FG_ITER_CANNOT_CONVERGE) {
-                break;
-            }
-            k_sa_spmv<<<nb3, ESB>>>(n3, d_rp, d_col, d_a, d_S, d_sh, d_dir, 1, d_t, d_s, d_part, d_part + nb3);
-            double tt2[2];
-            read(2, tt2);
-            omega = tt2[0] / tt2[1];
-            k_sa_x<<<nb3, ESB>>>(n3, alpha, omega, d_ph, d_sh, d_s, d_t, d_rt, d_x, d_r, d_part, d_part + nb3);
-            double rr2[2];
-            read(2, rr2);
-            rr = rr2[0];
-            rtr = rr2[1];
-            rho_2 = rho_1;
-            if (++nit >= max_iter) status = FG_ITER_OVERFLOW;
-        }
-        k_es_add<<<nb3, ESB>>>(n3, d_Vd, d_x);
-        ESCHK(hipGetLastError());
-        ESCHK(hipMemcpy(s_out, d_x, (size_t)n3 * sizeof(double), hipMemcpyDeviceToHost));
-        if (info) {
-            *info = fg_solve_info{};
-            info->status = status;
-            info->iterations = nit;
-            info->residual = res;
-            info->rhs_norm = rhsn;
-        }
+        ESCHK(hipSetDevice(sp->device));
+        k_sp_rec_u<<<sp->nb, ESB, 0, st>>>(sp->N, cur, nxt, sp->u, sp->us);
+        spin_run(*sp, st, accumulate, 1, info);
+        ESCHK(hipMemcpyAsync(s_dev, sp->x, (size_t)sp->n3 * sizeof(double), hipMemcpyDeviceToDevice, st));
         return FG_OK;
     } catch (const std::string &) {
         return FG_ERR_DEVICE;
     }
+}
+
+}  // namespace fg
+
+extern "C" int fg_spin_create(const fg_mesh *m, const fg_spin_volume *vol, const fg_spin_surface *surf,
+                              const double *Vn, double tol, int32_t max_iter, int32_t device, fg_spin **out)
+{
+    if (!out) return FG_ERR_ARG;
+    *out = nullptr;
+    fg_spin *sp = new fg_spin;
+    try {
+        const int rc = spin_setup(m, vol, surf, Vn, tol, max_iter, device, *sp);
+        if (rc != FG_OK) {
+            delete sp;
+            return rc;
+        }
+    } catch (const std::string &) {
+        delete sp;
+        return FG_ERR_DEVICE;
+    }
+    *out = sp;
+    return FG_OK;
+}
+
+extern "C" int fg_spin_solve(fg_spin *sp, const double *u, const double *u_src, int32_t accumulate, double *s_out,
+                             fg_solve_info *info)
+{
+    if (!sp || !u || !s_out) return FG_ERR_ARG;
+    try {
+        ESCHK(hipSetDevice(sp->device));
+        const size_t bytes = (size_t)sp->n3 * sizeof(double);
+        ESCHK(hipMemcpyAsync(sp->u, u, bytes, hipMemcpyHostToDevice, sp->stream));
+        ESCHK(hipMemcpyAsync(sp->us, u_src ? u_src : u, bytes, hipMemcpyHostToDevice, sp->stream));
+        spin_run(*sp, sp->stream, accumulate, 0, info);
+        ESCHK(hipMemcpyAsync(s_out, sp->x, bytes, hipMemcpyDeviceToHost, sp->stream));
+        ESCHK(hipStreamSynchronize(sp->stream));
+        return FG_OK;
+    } catch (const std::string &) {
+        return FG_ERR_DEVICE;
+    }
+}
+
+extern "C" int fg_spin_destroy(fg_spin *sp)
+{
+    if (!sp) return FG_ERR_ARG;
+    (void)hipSetDevice(sp->device);
+    (void)hipStreamSynchronize(sp->stream);
+    delete sp;
+    return FG_OK;
+}
+
+extern "C" int fg_spin_accumulation(const fg_mesh *m, const fg_spin_volume *vol, const fg_spin_surface *surf,
+                                    const double *Vn, const double *u, const double *u_src, double tol,
+                                    int32_t max_iter, int32_t device, double *s_out, fg_solve_info *info)
+{
+    if (!u || !s_out) return FG_ERR_ARG;
+    fg_spin *sp = nullptr;
+    int rc = fg_spin_create(m, vol, surf, Vn, tol, max_iter, device, &sp);
+    if (rc != FG_OK) return rc;
+    rc = fg_spin_solve(sp, u, u_src, 0, s_out, info);
+    fg_spin_destroy(sp);
+    return rc;
 }

@@ -121,6 +121,15 @@ class Engine:
         va = (nv.SpinVolume * len(vol))(*vol)
         self._chk(self.lib.fg_set_spin_accumulation(self._h, nv.dptr(ss), va), "fg_set_spin_accumulation")
 
+    def spin_solve(self, spin, accumulate=True, want_s=True):
+        """One solve of a SpinSolver at this engine's state (exchange from CURRENT, source from NEXT),
+        installed as the spin-accumulation field; returns (s [n_nodes, 3] or None, SolveInfo)."""
+        info = nv.SolveInfo()
+        s = np.zeros(3 * self.N) if want_s else None
+        self._chk(self.lib.fg_spin_solve_ctx(spin._h, self._h, int(accumulate), None if s is None else nv.dptr(s),
+                                             C.byref(info)), "fg_spin_solve_ctx")
+        return (None if s is None else s.reshape(-1, 3)), info
+
     def prepare_elements_amp(self, amp, dt, prefactor):
         self._chk(self.lib.fg_prepare_elements_amp(self._h, amp, dt, prefactor), "fg_prepare_elements_amp")
 
@@ -248,3 +257,42 @@ class Engine:
         E = self.energy(Hext)
         self.evolution()
         return False, info, E
+
+
+class SpinSolver:
+    """The persistent spin-diffusion solver of a run (fg_spin_*, spinAcc in the reference):
+    built once from the mesh view, the regions' nv.SpinVolume / nv.SpinSurface parameters and the
+    node potential V; every solve with accumulate=True adds its assembly to the system of the
+    earlier ones, as the reference's solver does."""
+
+    def __init__(self, mesh, vol, surf, V, tol=1e-8, max_iter=1000, device=0):
+        self.lib = nv.hip_lib()
+        self.N = mesh.n_nodes
+        va = (nv.SpinVolume * len(vol))(*vol)
+        sa = (nv.SpinSurface * len(surf))(*surf)
+        VV = np.ascontiguousarray(V, dtype=np.float64)
+        h = C.c_void_p()
+        rc = self.lib.fg_spin_create(C.byref(mesh), va, sa, nv.dptr(VV), tol, max_iter, device, C.byref(h))
+        if rc != nv.FG_OK:
+            raise EngineError(f"fg_spin_create: {rc}")
+        self._h = h
+
+    def solve(self, u, u_src=None, accumulate=True):
+        """s [n_nodes, 3] and SolveInfo at the magnetizations u (s-d exchange) and u_src (source)."""
+        uu = np.ascontiguousarray(u, dtype=np.float64).reshape(-1)
+        us = None if u_src is None else np.ascontiguousarray(u_src, dtype=np.float64).reshape(-1)
+        s = np.zeros(3 * self.N)
+        info = nv.SolveInfo()
+        rc = self.lib.fg_spin_solve(self._h, nv.dptr(uu), None if us is None else nv.dptr(us), int(accumulate),
+                                    nv.dptr(s), C.byref(info))
+        if rc != nv.FG_OK:
+            raise EngineError(f"fg_spin_solve: {rc}")
+        return s.reshape(-1, 3), info
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self.lib.fg_spin_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()

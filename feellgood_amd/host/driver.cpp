@@ -28,6 +28,7 @@
 #include <fstream>
 #include <iomanip>
 #include <iostream>
+#include <memory>
 #include <random>
 #include <sstream>
 #include <string>
@@ -615,7 +616,7 @@ int run(const Config &c, unsigned seed, bool verbose, bool check_only, const std
     // spin accumulation: the potential once, then s from the magnetization at every compute_all
     // (spinAcc::spinAcc and compute, src/spinAccumulationSolver.cpp:8-36,151-163); the exchange
     // reads the CURRENT state and the source the NEXT one, as the reference does
-    std::vector<double> Vpot, sacc, ucur, unext;
+    std::vector<double> Vpot, sacc;
     // per region of the mesh view: the settings' regions first, then the surfaces the preparation
     // adds (controlTriangles), which set neither jn, V, uP nor s
     std::vector<fg_spin_volume> svol(c.spin_vol);
@@ -625,19 +626,14 @@ int run(const Config &c, unsigned seed, bool verbose, bool check_only, const std
     ssurf.resize(std::max<size_t>(ssurf.size(), (size_t)view.n_surf_regions),
                  fg_spin_surface{NAN, {NAN, NAN, NAN}, {NAN, NAN, NAN}});
     sV.resize(ssurf.size(), NAN);
+    // the persistent solver (spinAcc, built once): every call adds its assembly to the system of
+    // the earlier ones, as the reference's solver does (src/solver.h:110-142), and installs s, or
+    // zero when it did not converge, as the field of the next element pass
+    std::unique_ptr<fg_spin, int (*)(fg_spin *)> spin(nullptr, fg_spin_destroy);
     auto spin_compute = [&]() {
-        ucur.resize(3 * (size_t)N);
-        unext.resize(3 * (size_t)N);
-        e.check(fg_get_state(e.ctx, FG_CURRENT, ucur.data(), nullptr, nullptr, nullptr), "fg_get_state");
-        e.check(fg_get_state(e.ctx, FG_NEXT, unext.data(), nullptr, nullptr, nullptr), "fg_get_state");
         fg_solve_info si;
-        e.check(fg_spin_accumulation(&view, svol.data(), ssurf.data(), Vpot.data(), ucur.data(),
-                                     unext.data(), 1e-8, 1000, c.device, sacc.data(), &si),
-                "fg_spin_accumulation");
-        const bool ok = si.status == FG_ITER_CONVERGED;
-        if (!ok) std::fill(sacc.begin(), sacc.end(), 0.0);
-        e.check(fg_set_spin_accumulation(e.ctx, sacc.data(), svol.data()), "fg_set_spin_accumulation");
-        return ok;
+        e.check(fg_spin_solve_ctx(spin.get(), e.ctx, 1, sacc.data(), &si), "fg_spin_solve_ctx");
+        return si.status == FG_ITER_CONVERGED;
     };
     if (c.spin_acc) {
         std::vector<double> sig, jn;
@@ -651,9 +647,9 @@ int run(const Config &c, unsigned seed, bool verbose, bool check_only, const std
         if (rc == FG_ERR_ARG) throw std::runtime_error("incorrect boundary conditions for potential V solver");
         e.check(rc, "fg_electrostatic_potential");
         if (si.status != FG_ITER_CONVERGED) throw std::runtime_error("electrostatic solver did not converge");
-        if (c.V_file) {  // electrostatSolver::save (src/electrostatSolver.cpp:138-156)
-            mkdir(c.out_dir.c_str(), 0777);
-            const std::string vname = c.out_dir + "/" + c.basename + "_V.sol";
+        if (c.V_file) {  // electrostatSolver::save (src/electrostatSolver.cpp:138-156), in the working
+                         // directory as the reference names it (src/spinAccumulationSolver.cpp:26-27)
+            const std::string vname = c.basename + "_V.sol";
             std::ofstream vf(vname);
             if (!vf) throw std::runtime_error("cannot open " + vname);
             vf << "## real-world time: " << date_string() << "\n## columns: index\tV\n" << std::scientific
@@ -661,6 +657,11 @@ int run(const Config &c, unsigned seed, bool verbose, bool check_only, const std
             const int32_t *ni = fgh_node_index(mesh);
             for (int i = 0; i < N; i++) vf << i << '\t' << Vpot[ni[i]] << "\n";
         }
+        fg_spin *sp = nullptr;
+        const int src = fg_spin_create(&view, svol.data(), ssurf.data(), Vpot.data(), 1e-8, 1000, c.device, &sp);
+        if (src == FG_ERR_ARG) throw std::runtime_error("incorrect boundary conditions for spin diffusion solver");
+        e.check(src, "fg_spin_create");
+        spin.reset(sp);
         if (!spin_compute()) throw std::runtime_error("spin diffusion solver (first try) failed");
     }
     auto compute_all = [&]() {  // src/fem.h:203-224

@@ -96,6 +96,11 @@ HIP_SYMBOLS = {
     "fg_gauss_points": (C.c_int, [C.c_void_p, _d]),
     "fg_spin_accumulation": (C.c_int, [C.c_void_p, C.POINTER(SpinVolume), C.POINTER(SpinSurface), _d, _d, _d,
                                        C.c_double, C.c_int32, C.c_int32, _d, C.POINTER(SolveInfo)]),
+    "fg_spin_create": (C.c_int, [C.c_void_p, C.POINTER(SpinVolume), C.POINTER(SpinSurface), _d, C.c_double,
+                                 C.c_int32, C.c_int32, C.POINTER(C.c_void_p)]),
+    "fg_spin_solve": (C.c_int, [C.c_void_p, _d, _d, C.c_int32, _d, C.POINTER(SolveInfo)]),
+    "fg_spin_solve_ctx": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, _d, C.POINTER(SolveInfo)]),
+    "fg_spin_destroy": (C.c_int, [C.c_void_p]),
     "fg_electrostatic_potential": (C.c_int, [C.c_void_p, _d, _d, _d, C.c_double, C.c_int32, C.c_int32, _d,
                                              C.POINTER(SolveInfo)]),
     "fg_electrostatic_potential_noalloc": (C.c_int, [C.c_void_p, _d, _d, _d, C.c_double, C.c_int32, C.c_int32, _d,

@@ -25,6 +25,9 @@
  *                         fg_prepare_elements_amp / fg_energy_amp for the R4toR3 field
  *   fg_electrostatic_potential  electrostatSolver (src/electrostatSolver.cpp:10-136)
  *   fg_spin_accumulation  spinAcc::solve (src/spinAccumulationSolver.cpp:151-287)
+ *   fg_spin_create / fg_spin_solve / fg_spin_solve_ctx / fg_spin_destroy
+ *                         spinAcc::spinAcc, compute, solve: the persistent solver of a run,
+ *                         with the reference's accumulation of K and L_rhs (src/solver.h:110-142)
  *   fg_set_spin_accumulation    spinAcc::prepareExtraField + Tetra::calc_Hst
  *                         (src/spinAccumulationSolver.cpp:138-149, src/tetra.cpp:93-106)
  *
@@ -263,6 +266,33 @@ int fg_electrostatic_potential_noalloc(const fg_mesh *mesh, const double *sigma,
 int fg_spin_accumulation(const fg_mesh *mesh, const fg_spin_volume *vol, const fg_spin_surface *surf, const double *V,
                          const double *u, const double *u_src, double tol, int32_t max_iter, int32_t device,
                          double *s_out, fg_solve_info *info);
+/* The persistent spin-diffusion solver of a run (spinAcc, src/spinAccumulationSolver.h; built by
+ * spinAcc::spinAcc, src/spinAccumulationSolver.cpp:8-36, and called by every compute_all,
+ * src/fem.h:203-224). fg_spin_create checks the boundary conditions and builds the node pattern,
+ * the Dirichlet values, the surface right-hand side and every device buffer once, on HIP device
+ * `device`, from the mesh, the regions' spin parameters and the node potential V[n_nodes]; return
+ * codes as fg_spin_accumulation's, *out NULL on error. */
+typedef struct fg_spin fg_spin;
+int fg_spin_create(const fg_mesh *mesh, const fg_spin_volume *vol, const fg_spin_surface *surf, const double *V,
+                   double tol, int32_t max_iter, int32_t device, fg_spin **out);
+/* One solve (spinAcc::solve, src/spinAccumulationSolver.cpp:165-209) at the node magnetizations
+ * u[3 n_nodes] (s-d exchange) and u_src[3 n_nodes] (source; NULL: u), from s = 0, with the
+ * BiCGSTAB scalars kept on the device (one host read per 16 iterations). With accumulate nonzero
+ * the new element and surface terms are ADDED to the system of the earlier solves, as in the
+ * reference, whose solver::buildMat / buildVect add into K and L_rhs (src/solver.h:110-142) and
+ * whose spinAcc never clears them, so that its k-th solve is that of the sum of k assemblies; with
+ * accumulate zero they replace it. s_out[3 n_nodes] and *info as fg_spin_accumulation's (s is
+ * returned whatever the status). */
+int fg_spin_solve(fg_spin *sp, const double *u, const double *u_src, int32_t accumulate, double *s_out,
+                  fg_solve_info *info);
+/* fg_spin_solve at an engine's own state, as compute_all does (src/fem.h:203-224): the s-d
+ * exchange from the CURRENT state and the source from the NEXT one, read on the device and
+ * solved on the context's stream; s then becomes the context's spin-accumulation field (as
+ * fg_set_spin_accumulation with the solver's regions), or zero when the solver did not converge
+ * (spinAcc::compute, src/spinAccumulationSolver.cpp:151-163). The solver must be built on the
+ * context's mesh, on its device (FG_ERR_ARG otherwise). s_out[3 n_nodes] (may be NULL) receives s. */
+int fg_spin_solve_ctx(fg_spin *sp, fg_ctx *ctx, int32_t accumulate, double *s_out, fg_solve_info *info);
+int fg_spin_destroy(fg_spin *sp);
 /* NEXT -> CURRENT. */
 int fg_evolution(fg_ctx *ctx);
 /* acos(min over magnetic edges of u_i . u_j) on the NEXT state. */

@@ -65,6 +65,8 @@ SYMBOLS = {
     "orc_energy": (None, [C.c_void_p, _d, _d]),
     "orc_set_space_field": (None, [C.c_void_p, _d]),
     "orc_set_spin_accumulation": (None, [C.c_void_p, _d, C.POINTER(nv.SpinVolume)]),
+    "orc_spin_accumulation_hist": (C.c_int, [C.POINTER(nv.Mesh), C.POINTER(nv.SpinVolume), C.POINTER(nv.SpinSurface),
+                                             _d, C.c_int, _d, _d, C.c_double, C.c_int, _d, C.POINTER(SolveInfo)]),
     "orc_prepare_elements_amp": (None, [C.c_void_p, C.c_double, C.c_double, C.c_double]),
     "orc_energy_amp": (None, [C.c_void_p, C.c_double, _d]),
     "orc_evolution": (None, [C.c_void_p]),
@@ -200,6 +202,25 @@ def spin_accumulation(mesh, vol, surf, V, u, tol=1e-8, max_iter=1000, u_src=None
     us = None if u_src is None else np.ascontiguousarray(u_src, dtype=np.float64).reshape(-1)
     rc = lib().orc_spin_accumulation(C.byref(mesh), va, sa, d(V), d(u), None if us is None else d(us), tol, max_iter,
                                      d(out), C.byref(info))
+    if rc < 0:
+        raise ValueError("incorrect boundary conditions for the spin diffusion solver")
+    return out.reshape(-1, 3), info
+
+
+def spin_accumulation_hist(mesh, vol, surf, V, uh, ush=None, tol=1e-8, max_iter=1000):
+    """The k-th solve of a spin-transfer run, k = len(uh): the system summed over the assemblies at
+    the magnetizations uh[j] (s-d exchange) and ush[j] (source; None: uh), as the reference's
+    solver accumulates K and L_rhs over the calls of spinAcc::solve. Returns (s, SolveInfo)."""
+    va = (nv.SpinVolume * len(vol))(*vol)
+    sa = (nv.SpinSurface * len(surf))(*surf)
+    V = np.ascontiguousarray(V, dtype=np.float64)
+    uh = np.ascontiguousarray(uh, dtype=np.float64).reshape(-1)
+    k = uh.size // (3 * mesh.n_nodes)
+    us = None if ush is None else np.ascontiguousarray(ush, dtype=np.float64).reshape(-1)
+    out = np.zeros(3 * mesh.n_nodes)
+    info = SolveInfo()
+    rc = lib().orc_spin_accumulation_hist(C.byref(mesh), va, sa, d(V), k, d(uh), None if us is None else d(us), tol,
+                                          max_iter, d(out), C.byref(info))
     if rc < 0:
         raise ValueError("incorrect boundary conditions for the spin diffusion solver")
     return out.reshape(-1, 3), info

@@ -1168,7 +1168,18 @@ int orc_spin_accumulation(const fg_mesh *m, const fg_spin_volume *vp, const fg_s
                           const double *u, const double *u_src, double tol, int max_iter, double *s_out,
                           orc_solve_info *info)
 {
-    if (!u_src) u_src = u; /* the source's magnetization (mesh::getNode_u, NEXT) */
+    return orc_spin_accumulation_hist(m, vp, sp, Vn, 1, u, u_src, tol, max_iter, s_out, info);
+}
+
+/* The k-th solve of a run: solver::buildMat / buildVect add into K and L_rhs (src/solver.h:110-142)
+ * and spinAcc::solve never clears them (src/spinAccumulationSolver.cpp:165-197), so the system is
+ * the sum of the assemblies of the k calls, each its element terms then its surface terms, in the
+ * reference's order; uh / ush hold the k magnetizations (ush NULL: uh). */
+int orc_spin_accumulation_hist(const fg_mesh *m, const fg_spin_volume *vp, const fg_spin_surface *sp,
+                               const double *Vn, int nhist, const double *uh, const double *ush, double tol,
+                               int max_iter, double *s_out, orc_solve_info *info)
+{
+    if (nhist < 1) return -1;
     const double E = 1.602176634e-19, HBAR = 6.62607015e-34 / (2 * M_PI), ME = 9.1093837139e-31;
     const double MUB = E * HBAR / (2. * ME); /* src/config.h.in:33-37 */
     auto finite3 = [](const double *v) { return std::isfinite(std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2])); };
@@ -1184,111 +1195,115 @@ int orc_spin_accumulation(const fg_mesh *m, const fg_spin_volume *vp, const fg_s
     const int N = m->n_nodes, D = 3 * N;
     std::vector<std::map<int, double>> rows(D);
     std::vector<double> L(D, 0.0);
-    for (int t = 0; t < m->n_tets; t++) {
-        V3 P[4];
-        int ind[4];
-        for (int a = 0; a < 4; a++) {
-            ind[a] = m->tet_nodes[4 * t + a];
-            P[a] = {m->pos[3 * ind[a]], m->pos[3 * ind[a] + 1], m->pos[3 * ind[a] + 2]};
-        }
-        double da[4][3], w[TNPI];
-        if (tet_geometry(P, ind, da, w, nullptr)) return -1;
-        const int rg = m->tet_region[t];
-        const fg_spin_volume &pr = vp[rg];
-        const double D0 = 2.0 * pr.sigma / ((E * E) * pr.N0);
-        double aw[4], wsum = 0;
-        for (int g = 0; g < TNPI; g++) wsum += w[g];
-        for (int i = 0; i < 4; i++) {
-            aw[i] = 0;
-            for (int g = 0; g < TNPI; g++) aw[i] += ta[i][g] * w[g];
-        }
-        /* calcDiagBlock (src/tetra.cpp:133-140): da da^T (D0 sum w) + diag(D0 / lsf^2 a_w) */
-        double AE[12][12] = {};
-        const double c = D0 * wsum, itsf = D0 / (pr.lsf * pr.lsf);
-        for (int i = 0; i < 4; i++)
-            for (int j = 0; j < 4; j++) {
-                double v = (da[i][0] * da[j][0] + da[i][1] * da[j][1] + da[i][2] * da[j][2]) * c;
-                if (i == j) v += itsf * aw[i];
-                for (int d = 0; d < 3; d++) AE[4 * d + i][4 * d + j] += v;
-            }
-        double BE[12] = {};
-        if (m->vol[rg].Ms > 0) {
-            const double itsd = D0 / (pr.lsd * pr.lsd);
-            for (int i = 0; i < 4; i++) {
-                const double *ui = u + 3 * ind[i];
-                const double X = itsd * (aw[i] * ui[0]), Y = itsd * (aw[i] * ui[1]), Z = itsd * (aw[i] * ui[2]);
-                AE[4 + i][8 + i] += X;
-                AE[8 + i][4 + i] -= X;
-                AE[0 + i][8 + i] -= Y;
-                AE[8 + i][0 + i] += Y;
-                AE[0 + i][4 + i] += Z;
-                AE[4 + i][0 + i] -= Z;
-            }
-            /* the source cst0 grad V . da_i m_i (:262-286) */
-            const double cst0 = MUB * pr.P * pr.sigma / E;
-            double gV[3] = {0, 0, 0};
-            for (int i = 0; i < 4; i++)
-                for (int k = 0; k < 3; k++) gV[k] += Vn[ind[i]] * da[i][k];
-            for (int g = 0; g < TNPI; g++) {
-                const double cw = cst0 * w[g];
-                const double cg[3] = {cw * gV[0], cw * gV[1], cw * gV[2]};
-                for (int i = 0; i < 4; i++) {
-                    const double *ui = u_src + 3 * ind[i];
-                    const double tmp = cg[0] * da[i][0] + cg[1] * da[i][1] + cg[2] * da[i][2];
-                    BE[i] += tmp * ui[0];
-                    BE[4 + i] += tmp * ui[1];
-                    BE[8 + i] += tmp * ui[2];
-                }
-            }
-        }
-        /* solver::buildMat / buildVect (src/solver.h:110-143) */
-        for (int i = 0; i < 4; i++)
-            for (int j = 0; j < 4; j++)
-                for (int di = 0; di < 3; di++)
-                    for (int dj = 0; dj < 3; dj++) rows[3 * ind[i] + di][3 * ind[j] + dj] += AE[4 * di + i][4 * dj + j];
-        for (int i = 0; i < 4; i++)
-            for (int d = 0; d < 3; d++) L[3 * ind[i] + d] += BE[4 * d + i];
-    }
-    /* boundary terms and Dirichlet values (:91-116, :173-197) */
     std::vector<double> Vd(D, 0.0);
     std::vector<int> ld;
-    for (int f = 0; f < m->n_tris; f++) {
-        const int rg = m->tri_region[f];
-        const int *q = m->tri_nodes + 3 * f;
-        if (rg < 0 || rg >= m->n_surf_regions) continue;
-        const fg_spin_surface &pr = sp[rg];
-        V3 P[3];
-        for (int a = 0; a < 3; a++) P[a] = {m->pos[3 * q[a]], m->pos[3 * q[a] + 1], m->pos[3 * q[a] + 2]};
-        const double surf = 0.5 * norm(cross(P[1] - P[0], P[2] - P[0]));
-        double sv[3] = {pr.s[0], pr.s[1], pr.s[2]};
-        if (std::isfinite(pr.jn) && finite3(pr.uP))
-            for (int k = 0; k < 3; k++) sv[k] = -pr.jn * (MUB / E) * pr.uP[k];
-        double Le[9] = {};
-        if (finite3(sv))
-            for (int g = 0; g < FNPI; g++) {
-                const double wg = 2.0 * surf * fpds[g];
-                for (int i = 0; i < 3; i++) {
-                    const double aiw = wg * fa[i][g];
-                    for (int k = 0; k < 3; k++) Le[3 * k + i] -= sv[k] * aiw;
+    for (int h = 0; h < nhist; h++) {
+        /* the source's magnetization (mesh::getNode_u, NEXT) */
+        const double *u = uh + (size_t)h * D, *u_src = (ush ? ush : uh) + (size_t)h * D;
+        for (int t = 0; t < m->n_tets; t++) {
+            V3 P[4];
+            int ind[4];
+            for (int a = 0; a < 4; a++) {
+                ind[a] = m->tet_nodes[4 * t + a];
+                P[a] = {m->pos[3 * ind[a]], m->pos[3 * ind[a] + 1], m->pos[3 * ind[a] + 2]};
+            }
+            double da[4][3], w[TNPI];
+            if (tet_geometry(P, ind, da, w, nullptr)) return -1;
+            const int rg = m->tet_region[t];
+            const fg_spin_volume &pr = vp[rg];
+            const double D0 = 2.0 * pr.sigma / ((E * E) * pr.N0);
+            double aw[4], wsum = 0;
+            for (int g = 0; g < TNPI; g++) wsum += w[g];
+            for (int i = 0; i < 4; i++) {
+                aw[i] = 0;
+                for (int g = 0; g < TNPI; g++) aw[i] += ta[i][g] * w[g];
+            }
+            /* calcDiagBlock (src/tetra.cpp:133-140): da da^T (D0 sum w) + diag(D0 / lsf^2 a_w) */
+            double AE[12][12] = {};
+            const double c = D0 * wsum, itsf = D0 / (pr.lsf * pr.lsf);
+            for (int i = 0; i < 4; i++)
+                for (int j = 0; j < 4; j++) {
+                    double v = (da[i][0] * da[j][0] + da[i][1] * da[j][1] + da[i][2] * da[j][2]) * c;
+                    if (i == j) v += itsf * aw[i];
+                    for (int d = 0; d < 3; d++) AE[4 * d + i][4 * d + j] += v;
+                }
+            double BE[12] = {};
+            if (m->vol[rg].Ms > 0) {
+                const double itsd = D0 / (pr.lsd * pr.lsd);
+                for (int i = 0; i < 4; i++) {
+                    const double *ui = u + 3 * ind[i];
+                    const double X = itsd * (aw[i] * ui[0]), Y = itsd * (aw[i] * ui[1]), Z = itsd * (aw[i] * ui[2]);
+                    AE[4 + i][8 + i] += X;
+                    AE[8 + i][4 + i] -= X;
+                    AE[0 + i][8 + i] -= Y;
+                    AE[8 + i][0 + i] += Y;
+                    AE[0 + i][4 + i] += Z;
+                    AE[4 + i][0 + i] -= Z;
+                }
+                /* the source cst0 grad V . da_i m_i (:262-286) */
+                const double cst0 = MUB * pr.P * pr.sigma / E;
+                double gV[3] = {0, 0, 0};
+                for (int i = 0; i < 4; i++)
+                    for (int k = 0; k < 3; k++) gV[k] += Vn[ind[i]] * da[i][k];
+                for (int g = 0; g < TNPI; g++) {
+                    const double cw = cst0 * w[g];
+                    const double cg[3] = {cw * gV[0], cw * gV[1], cw * gV[2]};
+                    for (int i = 0; i < 4; i++) {
+                        const double *ui = u_src + 3 * ind[i];
+                        const double tmp = cg[0] * da[i][0] + cg[1] * da[i][1] + cg[2] * da[i][2];
+                        BE[i] += tmp * ui[0];
+                        BE[4 + i] += tmp * ui[1];
+                        BE[8 + i] += tmp * ui[2];
+                    }
                 }
             }
-        for (int i = 0; i < 3; i++)
-            for (int k = 0; k < 3; k++) L[3 * q[i] + k] += Le[3 * k + i];
-        double dv[3];
-        bool dirichlet = false;
-        if (std::isnan(std::sqrt(pr.s[0] * pr.s[0] + pr.s[1] * pr.s[1] + pr.s[2] * pr.s[2])) && std::isfinite(pr.jn)) {
-            for (int k = 0; k < 3; k++) dv[k] = -pr.jn * (MUB / E) * pr.uP[k];
-            dirichlet = true;
-        } else if (finite3(pr.s) && std::isnan(pr.jn)) {
-            for (int k = 0; k < 3; k++) dv[k] = pr.s[k];
-            dirichlet = true;
+            /* solver::buildMat / buildVect (src/solver.h:110-143) */
+            for (int i = 0; i < 4; i++)
+                for (int j = 0; j < 4; j++)
+                    for (int di = 0; di < 3; di++)
+                        for (int dj = 0; dj < 3; dj++) rows[3 * ind[i] + di][3 * ind[j] + dj] += AE[4 * di + i][4 * dj + j];
+            for (int i = 0; i < 4; i++)
+                for (int d = 0; d < 3; d++) L[3 * ind[i] + d] += BE[4 * d + i];
         }
-        if (dirichlet)
-            for (int i = 0; i < 3; i++)
-                for (int k = 0; k < 3; k++) {
-                    Vd[3 * q[i] + k] = dv[k];
-                    ld.push_back(3 * q[i] + k);
+        /* boundary terms and Dirichlet values (:91-116, :173-197) */
+        for (int f = 0; f < m->n_tris; f++) {
+            const int rg = m->tri_region[f];
+            const int *q = m->tri_nodes + 3 * f;
+            if (rg < 0 || rg >= m->n_surf_regions) continue;
+            const fg_spin_surface &pr = sp[rg];
+            V3 P[3];
+            for (int a = 0; a < 3; a++) P[a] = {m->pos[3 * q[a]], m->pos[3 * q[a] + 1], m->pos[3 * q[a] + 2]};
+            const double surf = 0.5 * norm(cross(P[1] - P[0], P[2] - P[0]));
+            double sv[3] = {pr.s[0], pr.s[1], pr.s[2]};
+            if (std::isfinite(pr.jn) && finite3(pr.uP))
+                for (int k = 0; k < 3; k++) sv[k] = -pr.jn * (MUB / E) * pr.uP[k];
+            double Le[9] = {};
+            if (finite3(sv))
+                for (int g = 0; g < FNPI; g++) {
+                    const double wg = 2.0 * surf * fpds[g];
+                    for (int i = 0; i < 3; i++) {
+                        const double aiw = wg * fa[i][g];
+                        for (int k = 0; k < 3; k++) Le[3 * k + i] -= sv[k] * aiw;
+                    }
                 }
+            for (int i = 0; i < 3; i++)
+                for (int k = 0; k < 3; k++) L[3 * q[i] + k] += Le[3 * k + i];
+            double dv[3];
+            bool dirichlet = false;
+            if (std::isnan(std::sqrt(pr.s[0] * pr.s[0] + pr.s[1] * pr.s[1] + pr.s[2] * pr.s[2])) && std::isfinite(pr.jn)) {
+                for (int k = 0; k < 3; k++) dv[k] = -pr.jn * (MUB / E) * pr.uP[k];
+                dirichlet = true;
+            } else if (finite3(pr.s) && std::isnan(pr.jn)) {
+                for (int k = 0; k < 3; k++) dv[k] = pr.s[k];
+                dirichlet = true;
+            }
+            if (dirichlet && h == 0)
+                for (int i = 0; i < 3; i++)
+                    for (int k = 0; k < 3; k++) {
+                        Vd[3 * q[i] + k] = dv[k];
+                        ld.push_back(3 * q[i] + k);
+                    }
+        }
     }
     std::sort(ld.begin(), ld.end());
     ld.erase(std::unique(ld.begin(), ld.end()), ld.end());

@@ -63,6 +63,12 @@ void orc_set_spin_accumulation(orc_problem *p, const double *s, const fg_spin_vo
 int orc_spin_accumulation(const fg_mesh *m, const fg_spin_volume *vp, const fg_spin_surface *sp, const double *V,
                           const double *u, const double *u_src, double tol, int max_iter, double *s_out,
                           orc_solve_info *info);
+/* The k = nhist-th solve of a run (the reference's solver never clears K and L_rhs between the
+   calls of spinAcc::solve): the system summed over the magnetizations uh[nhist][3 n_nodes] and
+   sources ush (NULL: uh). */
+int orc_spin_accumulation_hist(const fg_mesh *m, const fg_spin_volume *vp, const fg_spin_surface *sp,
+                               const double *V, int nhist, const double *uh, const double *ush, double tol,
+                               int max_iter, double *s_out, orc_solve_info *info);
 void orc_system_size(const orc_problem *p, int64_t *rows, int64_t *nnz);
 void orc_get_system(const orc_problem *p, int64_t *rowptr, int32_t *col, double *val,
                     double *rhs);

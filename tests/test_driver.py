@@ -440,10 +440,12 @@ def test_spin_accumulation_settings(tmp_path):
 def test_spin_accumulation_run_on_gpu(tmp_path):
     """A short spin-transfer run of the bar: the potential file holds the closed form
     V = -jn x / sigma (V = 0 on the left, jn entering on the right), the .sol files carry the three
-    spin-accumulation columns, and the current's spin torque moves the magnetization."""
+    spin-accumulation columns, and the current's spin torque moves the magnetization. The potential
+    file goes to the working directory, as the reference writes it."""
     pos = bar_msh(tmp_path / "bar.msh")
-    s = spin_settings(tmp_path / "bar.msh", tmp_path)
-    r = subprocess.run([DRIVER, "--seed", "2", "-"], input=json.dumps(s), capture_output=True, text=True, timeout=300)
+    s = spin_settings(tmp_path / "bar.msh", tmp_path / "out")
+    r = subprocess.run([DRIVER, "--seed", "2", "-"], input=json.dumps(s), capture_output=True, text=True, timeout=300,
+                       cwd=tmp_path)
     assert r.returncode == 0, r.stdout + r.stderr
     rows = [ln.split() for ln in open(tmp_path / "spin_V.sol") if not ln.startswith("#")]
     V = np.array([float(x[1]) for x in rows])
@@ -451,11 +453,12 @@ def test_spin_accumulation_run_on_gpu(tmp_path):
     # the V file is in node-file order; the bar's positions here are already in that order
     ref = -1e6 * x / 5e6
     assert np.abs(V - ref).max() <= 1e-6 * np.abs(ref).max()
-    sol = [ln.split() for ln in open(tmp_path / "spin_iter2.sol") if not ln.startswith("#")]
+    assert not (tmp_path / "out" / "spin_V.sol").exists()
+    sol = [ln.split() for ln in open(tmp_path / "out" / "spin_iter2.sol") if not ln.startswith("#")]
     assert len(sol[0]) == 8
     sz = np.array([[float(v) for v in ln[5:8]] for ln in sol])
     assert np.abs(sz).max() > 0 and np.all(np.isfinite(sz))
-    row, n = last_evol_row(tmp_path / "spin.evol")
+    row, n = last_evol_row(tmp_path / "out" / "spin.evol")
     off = json.loads(json.dumps(s))
     off["spin_accumulation"]["enable"] = False
     off["outputs"]["directory"] = str(tmp_path / "off")
