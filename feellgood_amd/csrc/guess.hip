@@ -6,10 +6,12 @@
 // costs iterations, never accuracy.
 //
 // Network, per magnetic node i (weights stored in BF16, arithmetic in FP32):
-//   features f_i = [u_i, s_h h_i, s_v v_i]  with h_i = F_t,i / m_i the tangent part of the
-//                  assembled right-hand side over the lumped mass, v_i the previous v (9 values)
+//   features f_i = [u_i, s_h h_i, s_v v_i, s_d d_i]  with h_i = F_t,i / m_i the tangent part of the
+//                  assembled right-hand side over the lumped mass, v_i the previous v and d_i the
+//                  second difference v_i - 2 v_i,prev + v_i,prev2 of the last three solutions
+//                  (zero unless they came at one fixed step) (12 values)
 //   aggregation a_i = mean of f_j over the magnetic mesh-edge neighbours j of i
-//   x = [f_i, a_i] (18) -> z1 = W1 x + b1 -> h1 = GELU(z1) (64) -> z2 = W2 h1 + b2 -> h2 = GELU(z2)
+//   x = [f_i, a_i] (24) -> z1 = W1 x + b1 -> h1 = GELU(z1) (64) -> z2 = W2 h1 + b2 -> h2 = GELU(z2)
 //   (64) -> o = W3 h2 + b3 (3) -> v_pred = P_u (v_base,i + s_out o), P_u the tangent projection
 //   guess x0_i = (v_pred . ep_i, v_pred . eq_i) / gamma0
 // The base the network corrects is the linear extrapolation of the last two solutions,
@@ -19,8 +21,10 @@
 // GELU is the tanh form 0.5 z (1 + tanh(sqrt(2/pi) (z + 0.044715 z^3))).
 // Training: loss = sum_i |v_pred,i - v*_i|^2 / (n_mag s_out^2) against the converged solution v*
 // of the same solve; Adam on FP32 master weights; the forward pass uses their BF16 rounding.
-// The scales are the root mean squares of h, v* and v* - P_u v_base over the first training
+// The scales are the root mean squares of h, v*, v* - P_u v_base and d over the first training
 // batch, so the base has loss 1 on it; W3 and b3 start at zero (the network starts as the base).
+// The second difference is what the linear extrapolation leaves out at a fixed step (the
+// quadratic extrapolation is v_base + d), the term a trained network can add to the base.
 // The reference guess (P_u v_i) is scored on the same batch as loss_ref.
 #include <cmath>
 #include <cstring>
@@ -34,14 +38,15 @@ namespace fg {
 
 namespace {
 
-constexpr int NF = 9, NX = 18, NH = 64, NO = 3;
-constexpr int FS = 12;  // feature row: the NF features, then the base v the network corrects
+constexpr int NF = 12, NX = 24, NH = 64, NO = 3;
+constexpr int FS = 16;  // feature row: the NF features, then the base v the network corrects, 1 pad
 constexpr int OFF_W1 = 0, OFF_B1 = OFF_W1 + NH * NX, OFF_W2 = OFF_B1 + NH, OFF_B2 = OFF_W2 + NH * NH,
               OFF_W3 = OFF_B2 + NH, OFF_B3 = OFF_W3 + NO * NH, NTRAIN = OFF_B3 + NO;
-constexpr int OFF_S = NTRAIN, NPARAM = NTRAIN + 3;
+constexpr int OFF_S = NTRAIN, NSC = 4, NPARAM = NTRAIN + NSC;  // scales s_h, s_v, s_out, s_d
 static_assert(NPARAM == FG_GUESS_NPARAM, "parameter layout and header disagree");
 constexpr int GB = 128;  // threads per block of the per-node kernels
-constexpr int NXP = 20;  // inputs padded to whole MFMA k-steps of 4
+constexpr int NXP = 24;  // inputs padded to whole MFMA k-steps of 4
+constexpr int KS1 = NXP / 4;  // k-steps of the first layer
 
 // GELU in its tanh form, written as 0.5 (1 + tanh(a)) = 1 / (1 + exp(-2a)) with
 // a = sqrt(2/pi) (z + 0.044715 z^3), evaluated by the hardware exp2 and reciprocal (about one ulp
@@ -75,7 +80,9 @@ __host__ __device__ inline float bf16_round(float x)
 
 struct GuessNet {
     int N = 0, nmag = 0, nblk = 0;
-    bool enabled = false, have_batch = false, scaled = false;
+    // scaled: the scales of h, v* and the output set; scaled_d: that of the second difference,
+    // which the first training batches (fewer than three earlier solutions) cannot give
+    bool enabled = false, have_batch = false, scaled = false, scaled_d = false;
     int adam_t = 0;
     float *params = nullptr;                 // FP32 master + scales [NPARAM]
     uint16_t *wbf = nullptr;                 // BF16 copy of the trainable block [NTRAIN]
@@ -87,10 +94,10 @@ struct GuessNet {
     double *lpart = nullptr;                 // per-block loss partials (2 per block)
     double *h_l = nullptr;                   // pinned
     float *xq = nullptr;                     // [N][NXP] network inputs of the inference pass
-    float *vhist = nullptr;                  // [N][3] previous v of the last recorded solve
+    float *vhist = nullptr;                  // [N][6] previous v of the last two recorded solves
     float *ninv = nullptr;                   // [N] 1 / number of magnetic mesh neighbours
     int nhist = 0;                           // assemblies recorded since the last state reset
-    double dt_hist = 0;                      // step of the last recorded solve
+    double dt_hist = 0, dt_hist2 = 0;        // steps of the last two recorded solves
 };
 
 namespace {
@@ -122,11 +129,12 @@ __global__ __launch_bounds__(256) void k_guess_features(int N, const NodeRec *__
                                                         const double4 *__restrict__ eq4,
                                                         const double *__restrict__ mass,
                                                         const int32_t *__restrict__ mag, float *__restrict__ feat,
-                                                        float *__restrict__ vhist, float rho, int have_hist)
+                                                        float *__restrict__ vhist, float rho, int have_hist,
+                                                        int have_d)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= N) return;
-    float f[NF] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    float f[NF] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
     if (mag[i]) {
         const NodeRec n = cur[i], v = vsrc[i];
         const double2 bi = b[i];
@@ -142,21 +150,26 @@ __global__ __launch_bounds__(256) void k_guess_features(int N, const NodeRec *__
         f[7] = (float)v.vy;
         f[8] = (float)v.vz;
     }
-    // extrapolated base from this solve's previous v and the one before it; remember this one
+    // extrapolated base from this solve's previous v and the one before it, the second difference
+    // with the one before that; shift the history
     float vb[3];
     for (int k = 0; k < 3; k++) {
-        const float v1 = f[6 + k], v2 = have_hist ? vhist[(size_t)i * 3 + k] : v1;
+        const float v1 = f[6 + k], v2 = have_hist ? vhist[(size_t)i * 6 + k] : v1;
+        const float v3 = have_d ? vhist[(size_t)i * 6 + 3 + k] : v2;
         vb[k] = v1 + rho * (v1 - v2);
-        vhist[(size_t)i * 3 + k] = v1;
+        f[9 + k] = have_d ? (v1 - v2) - (v2 - v3) : 0.f;
+        vhist[(size_t)i * 6 + 3 + k] = v2;
+        vhist[(size_t)i * 6 + k] = v1;
     }
     float4 *dst = reinterpret_cast<float4 *>(feat + (size_t)i * FS);
     dst[0] = make_float4(f[0], f[1], f[2], f[3]);
     dst[1] = make_float4(f[4], f[5], f[6], f[7]);
-    dst[2] = make_float4(f[8], vb[0], vb[1], vb[2]);
+    dst[2] = make_float4(f[8], f[9], f[10], f[11]);
+    dst[3] = make_float4(vb[0], vb[1], vb[2], 0.f);
 }
 
 struct SWeights {
-    float W1[NH * NX], b1[NH], W2[NH * NH], b2[NH], W3[NO * NH], b3[NO], s[3];
+    float W1[NH * NX], b1[NH], W2[NH * NH], b2[NH], W3[NO * NH], b3[NO], s[NSC];
 };
 
 __device__ inline void load_weights(SWeights &S, const float *__restrict__ p, const uint16_t *__restrict__ wbf)
@@ -169,7 +182,7 @@ __device__ inline void load_weights(SWeights &S, const float *__restrict__ p, co
         S.b2[k] = p[OFF_B2 + k];
     }
     if (threadIdx.x < NO) S.b3[threadIdx.x] = p[OFF_B3 + threadIdx.x];
-    if (threadIdx.x < 3) S.s[threadIdx.x] = p[OFF_S + threadIdx.x];
+    if (threadIdx.x < NSC) S.s[threadIdx.x] = p[OFF_S + threadIdx.x];
     __syncthreads();
 }
 
@@ -180,9 +193,9 @@ __device__ inline void gather_input(int i, const float *__restrict__ feat, const
                                     const int64_t *__restrict__ sptr, const int32_t *__restrict__ swid,
                                     const int32_t *__restrict__ scol, const float *s, float x[NX])
 {
-    const float sc[NF] = {1, 1, 1, s[0], s[0], s[0], s[1], s[1], s[1]};
+    const float sc[NF] = {1, 1, 1, s[0], s[0], s[0], s[1], s[1], s[1], s[3], s[3], s[3]};
     for (int k = 0; k < NF; k++) x[k] = feat[(size_t)i * FS + k] * sc[k];
-    float a[NF] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    float a[NF] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
     const int sl = i >> 6, lane = i & 63;
     const int64_t off = sptr[sl];
     const int width = swid[sl];
@@ -201,6 +214,9 @@ __device__ inline void gather_input(int i, const float *__restrict__ feat, const
         a[6] += f1.z;
         a[7] += f1.w;
         a[8] += f2.x;
+        a[9] += f2.y;
+        a[10] += f2.z;
+        a[11] += f2.w;
     }
     const float ic = ninv[i];
     for (int k = 0; k < NF; k++) x[NF + k] = a[k] * ic * sc[k];
@@ -234,13 +250,13 @@ __global__ __launch_bounds__(256) void k_guess_input(int N, const float *__restr
     if (i >= N) return;
     float x[NX] = {};
     if (mag[i]) {
-        const float s[3] = {p[OFF_S], p[OFF_S + 1], p[OFF_S + 2]};
+        const float s[NSC] = {p[OFF_S], p[OFF_S + 1], p[OFF_S + 2], p[OFF_S + 3]};
         gather_input(i, feat, ninv, sptr, swid, scol, s, x);
     }
+    static_assert(NX == NXP, "the inputs fill whole k-steps");
     float4 *dst = reinterpret_cast<float4 *>(xq + (size_t)i * NXP);
 #pragma unroll
-    for (int k = 0; k < 4; k++) dst[k] = make_float4(x[4 * k], x[4 * k + 1], x[4 * k + 2], x[4 * k + 3]);
-    dst[4] = make_float4(x[16], x[17], 0.f, 0.f);
+    for (int k = 0; k < NXP / 4; k++) dst[k] = make_float4(x[4 * k], x[4 * k + 1], x[4 * k + 2], x[4 * k + 3]);
 }
 
 // Inference on the matrix cores. Each wavefront takes 64 nodes as four 16-node column tiles and
@@ -253,10 +269,10 @@ __global__ __launch_bounds__(256) void k_guess_input(int N, const float *__restr
 // The three outputs (rows 0..2, lanes 0..15) go through LDS to one node per lane for v_pred.
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 struct MfmaNet {
-    float4 W2f[16 * 64];  // [(R2 * 4 + R) * 64 + l], j = 0..3
-    float W1f[20 * 64];   // [(R * 5 + s) * 64 + l]
-    float4 W3f[4 * 64];   // [R * 64 + l], j = 0..3
-    float b1[NH], b2[NH], b3[4], s[3];
+    float4 W2f[16 * 64];    // [(R2 * 4 + R) * 64 + l], j = 0..3
+    float W1f[4 * KS1 * 64];  // [(R * KS1 + s) * 64 + l]
+    float4 W3f[4 * 64];     // [R * 64 + l], j = 0..3
+    float b1[NH], b2[NH], b3[4], s[NSC];
     float o[256 * NO];
 };
 
@@ -274,8 +290,8 @@ __global__ __launch_bounds__(256) void k_guess_infer(int N, const float *__restr
         const uint16_t *w = wbf + OFF_W2 + row * NH + col;
         S.W2f[e] = make_float4(bf16_to_float(w[0]), bf16_to_float(w[1]), bf16_to_float(w[2]), bf16_to_float(w[3]));
     }
-    for (int e = tid; e < 20 * 64; e += 256) {
-        const int f = e >> 6, q = e & 63, R = f / 5, s = f - 5 * R, k = 4 * s + (q >> 4);
+    for (int e = tid; e < 4 * KS1 * 64; e += 256) {
+        const int f = e >> 6, q = e & 63, R = f / KS1, s = f - KS1 * R, k = 4 * s + (q >> 4);
         S.W1f[e] = k < NX ? bf16_to_float(wbf[OFF_W1 + (16 * R + (q & 15)) * NX + k]) : 0.f;
     }
     for (int e = tid; e < 4 * 64; e += 256) {
@@ -290,17 +306,17 @@ __global__ __launch_bounds__(256) void k_guess_infer(int N, const float *__restr
         S.b2[tid] = p[OFF_B2 + tid];
     }
     if (tid < 4) S.b3[tid] = tid < NO ? p[OFF_B3 + tid] : 0.f;
-    if (tid < 3) S.s[tid] = p[OFF_S + tid];
+    if (tid < NSC) S.s[tid] = p[OFF_S + tid];
     __syncthreads();
     for (int base = blockIdx.x * 256; base < N; base += gridDim.x * 256) {
         const int wbase = base + 64 * wv;
         // B operands of layer 1: inputs 4s + g4 of node wbase + 16T + ln
-        float xb[4][5];
+        float xb[4][KS1];
 #pragma unroll
         for (int T = 0; T < 4; T++) {
             const int node = wbase + 16 * T + ln;
 #pragma unroll
-            for (int s = 0; s < 5; s++) xb[T][s] = node < N ? xq[(size_t)node * NXP + 4 * s + g4] : 0.f;
+            for (int s = 0; s < KS1; s++) xb[T][s] = node < N ? xq[(size_t)node * NXP + 4 * s + g4] : 0.f;
         }
         f32x4 h1[4][4];
 #pragma unroll
@@ -311,8 +327,8 @@ __global__ __launch_bounds__(256) void k_guess_infer(int N, const float *__restr
 #pragma unroll
             for (int T = 0; T < 4; T++) h1[R][T] = a0;
 #pragma unroll
-            for (int s = 0; s < 5; s++) {
-                const float a = S.W1f[(R * 5 + s) * 64 + l];
+            for (int s = 0; s < KS1; s++) {
+                const float a = S.W1f[(R * KS1 + s) * 64 + l];
 #pragma unroll
                 for (int T = 0; T < 4; T++) h1[R][T] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, xb[T][s], h1[R][T], 0, 0, 0);
             }
@@ -583,13 +599,14 @@ __global__ void k_guess_adam(float *__restrict__ p, const float *__restrict__ g,
     }
 }
 
-// sums for the scales: |h|^2, |v*|^2, |v* - P_u v_base|^2 over magnetic nodes
+// sums for the scales: |h|^2, |v*|^2, |v* - P_u v_base|^2, |d|^2 over magnetic nodes
+constexpr int NMOM = 4;
 __global__ __launch_bounds__(256) void k_guess_moments(int N, const float *__restrict__ feat,
                                                        const int32_t *__restrict__ mag,
                                                        const NodeRec *__restrict__ nxt, double *__restrict__ part)
 {
-    __shared__ double red[3][256];
-    double s[3] = {0, 0, 0};
+    __shared__ double red[NMOM][256];
+    double s[NMOM] = {0, 0, 0, 0};
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x) {
         if (!mag[i]) continue;
         const float *f = feat + (size_t)i * FS;
@@ -602,17 +619,18 @@ __global__ __launch_bounds__(256) void k_guess_moments(int N, const float *__res
             s[1] += vs[k] * vs[k];
             const double d = vs[k] - (vb[k] - ur * f[k]);
             s[2] += d * d;
+            s[3] += (double)f[9 + k] * f[9 + k];
         }
     }
-    for (int k = 0; k < 3; k++) red[k][threadIdx.x] = s[k];
+    for (int k = 0; k < NMOM; k++) red[k][threadIdx.x] = s[k];
     __syncthreads();
     for (int h = 128; h > 0; h >>= 1) {
         if ((int)threadIdx.x < h)
-            for (int k = 0; k < 3; k++) red[k][threadIdx.x] += red[k][threadIdx.x + h];
+            for (int k = 0; k < NMOM; k++) red[k][threadIdx.x] += red[k][threadIdx.x + h];
         __syncthreads();
     }
     if (threadIdx.x == 0)
-        for (int k = 0; k < 3; k++) part[3 * blockIdx.x + k] = red[k][0];
+        for (int k = 0; k < NMOM; k++) part[NMOM * blockIdx.x + k] = red[k][0];
 }
 
 template <class T>
@@ -657,11 +675,11 @@ void guess_init(Ctx &c, uint64_t seed)
         galloc(&g->adam_v, NTRAIN);
         galloc(&g->grad, NTRAIN);
         galloc(&g->feat, (size_t)c.N * FS);
-        galloc(&g->vhist, (size_t)c.N * 3);
+        galloc(&g->vhist, (size_t)c.N * 6);
         galloc(&g->ninv, (size_t)c.N);
         k_guess_ncount<<<(c.N + 255) / 256, 256, 0, c.stream>>>(c.N, c.mag, c.slice_ptr, c.slice_w, c.scol, g->ninv);
-        galloc(&g->lpart, 2 * ((c.N + GB - 1) / GB) + 3 * 1024);
-        const size_t nl = std::max<size_t>(3 * 1024, 2 * (size_t)((c.N + GB - 1) / GB));
+        galloc(&g->lpart, 2 * ((c.N + GB - 1) / GB) + NMOM * 1024);
+        const size_t nl = std::max<size_t>(NMOM * 1024, 2 * (size_t)((c.N + GB - 1) / GB));
         if (hipHostMalloc((void **)&g->h_l, nl * sizeof(double), hipHostMallocDefault) != hipSuccess)
             throw std::string("hipHostMalloc (guess)");
     }
@@ -672,7 +690,7 @@ void guess_init(Ctx &c, uint64_t seed)
     for (int k = 0; k < NH * NX; k++) p[OFF_W1 + k] = nd(gen) / std::sqrt((float)NX);
     for (int k = 0; k < NH * NH; k++) p[OFF_W2 + k] = nd(gen) / std::sqrt((float)NH);
     guess_set(c, p.data());
-    g->scaled = false;
+    g->scaled = g->scaled_d = false;
 }
 
 void guess_set(Ctx &c, const float *params)
@@ -684,6 +702,7 @@ void guess_set(Ctx &c, const float *params)
     (void)hipMemset(g->adam_v, 0, NTRAIN * sizeof(float));
     g->adam_t = 0;
     g->scaled = params[OFF_S] > 0 && params[OFF_S + 1] > 0 && params[OFF_S + 2] > 0;
+    g->scaled_d = params[OFF_S + 3] > 0;
     k_guess_round<<<32, 256, 0, c.stream>>>(g->params, g->wbf);
     sync_or_throw(c, "guess set");
 }
@@ -708,9 +727,14 @@ void guess_on_assembly(Ctx &c, double dt)
         // from the third: extrapolate only between two solutions
         const bool ext = g->nhist >= 2 && g->dt_hist > 0;
         const float rho = ext ? (float)(dt / g->dt_hist) : 0.f;
+        // the second difference from the fourth assembly on, when the last three solutions came at
+        // this same step
+        const bool dif = g->nhist >= 3 && dt == g->dt_hist && dt == g->dt_hist2;
         k_guess_features<<<(c.N + 255) / 256, 256, 0, c.stream>>>(c.N, c.rec[c.cur], vsrc, c.b, c.ep4, c.eq4, c.mass,
-                                                                  c.mag, g->feat, g->vhist, rho, ext ? 1 : 0);
+                                                                  c.mag, g->feat, g->vhist, rho, ext ? 1 : 0,
+                                                                  dif ? 1 : 0);
         g->nhist++;
+        g->dt_hist2 = g->dt_hist;
         g->dt_hist = dt;
     }
     g->have_batch = false;  // the solution of this solve is not known yet
@@ -755,20 +779,30 @@ static void compute_scales(Ctx &c)
     GuessNet *g = c.guess;
     const int nb = 1024;
     k_guess_moments<<<nb, 256, 0, c.stream>>>(c.N, g->feat, c.mag, c.rec[1 - c.cur], g->lpart);
-    if (hipMemcpyAsync(g->h_l, g->lpart, 3 * nb * sizeof(double), hipMemcpyDeviceToHost, c.stream) != hipSuccess)
+    if (hipMemcpyAsync(g->h_l, g->lpart, NMOM * nb * sizeof(double), hipMemcpyDeviceToHost, c.stream) != hipSuccess)
         throw std::string("hipMemcpy (guess moments)");
     sync_or_throw(c, "guess moments");
-    double s[3] = {0, 0, 0};
+    double s[NMOM] = {0, 0, 0, 0};
     for (int b = 0; b < nb; b++)
-        for (int k = 0; k < 3; k++) s[k] += g->h_l[3 * b + k];
+        for (int k = 0; k < NMOM; k++) s[k] += g->h_l[NMOM * b + k];
     const double n = std::max(1, g->nmag);
-    float sc[3];
-    sc[0] = s[0] > 0 ? (float)(1.0 / std::sqrt(s[0] / n)) : 1.f;  // 1 / rms|h|
-    sc[1] = s[1] > 0 ? (float)(1.0 / std::sqrt(s[1] / n)) : 1.f;  // 1 / rms|v*|
-    sc[2] = s[2] > 0 ? (float)std::sqrt(s[2] / n) : 1.f;          // rms|v* - P v_base|
-    if (hipMemcpy(g->params + OFF_S, sc, 3 * sizeof(float), hipMemcpyHostToDevice) != hipSuccess)
-        throw std::string("hipMemcpy (guess scales)");
-    g->scaled = true;
+    if (!g->scaled) {
+        float sc[3];
+        sc[0] = s[0] > 0 ? (float)(1.0 / std::sqrt(s[0] / n)) : 1.f;  // 1 / rms|h|
+        sc[1] = s[1] > 0 ? (float)(1.0 / std::sqrt(s[1] / n)) : 1.f;  // 1 / rms|v*|
+        sc[2] = s[2] > 0 ? (float)std::sqrt(s[2] / n) : 1.f;          // rms|v* - P v_base|
+        if (hipMemcpy(g->params + OFF_S, sc, 3 * sizeof(float), hipMemcpyHostToDevice) != hipSuccess)
+            throw std::string("hipMemcpy (guess scales)");
+        g->scaled = true;
+    }
+    // 1 / rms|d| from the first batch that has the second difference; until then the scale stays
+    // zero and so does the network's d input
+    if (!g->scaled_d && s[3] > 0) {
+        const float sd = (float)(1.0 / std::sqrt(s[3] / n));
+        if (hipMemcpy(g->params + OFF_S + 3, &sd, sizeof(float), hipMemcpyHostToDevice) != hipSuccess)
+            throw std::string("hipMemcpy (guess scales)");
+        g->scaled_d = true;
+    }
 }
 
 // loss of the network and of the reference guess on the last solve, gradient into g->grad
@@ -814,7 +848,7 @@ void guess_gradient(Ctx &c, float *grad, double *loss, double *loss_ref)
 {
     GuessNet *g = c.guess;
     if (!g->have_batch) throw std::string("no solved step to train on");
-    if (!g->scaled) compute_scales(c);
+    if (!g->scaled || !g->scaled_d) compute_scales(c);
     loss_and_gradient(c, loss, loss_ref);
     if (grad && hipMemcpy(grad, g->grad, NTRAIN * sizeof(float), hipMemcpyDeviceToHost) != hipSuccess)
         throw std::string("hipMemcpy (guess gradient)");
@@ -824,7 +858,7 @@ void guess_train(Ctx &c, float lr, double *loss, double *loss_ref)
 {
     GuessNet *g = c.guess;
     if (!g->have_batch) throw std::string("no solved step to train on");
-    if (!g->scaled) compute_scales(c);
+    if (!g->scaled || !g->scaled_d) compute_scales(c);
     loss_and_gradient(c, loss, loss_ref);
     g->adam_t++;
     const float c1 = 1.0f / (1.0f - std::pow(0.9f, (float)g->adam_t));

@@ -231,15 +231,15 @@ class Engine:
         return a.value, b.value
 
     def guess_gradient(self):
-        g = np.zeros(nv.FG_GUESS_NPARAM - 3, dtype=np.float32)
+        g = np.zeros(nv.FG_GUESS_NPARAM - nv.FG_GUESS_NSCALE, dtype=np.float32)
         a, b = C.c_double(), C.c_double()
         self._chk(self.lib.fg_guess_gradient(self._h, nv.fptr(g), C.byref(a), C.byref(b)), "fg_guess_gradient")
         return g, a.value, b.value
 
     def guess_features(self):
-        f = np.zeros(12 * self.N, dtype=np.float32)
+        f = np.zeros(nv.FG_GUESS_FEATURE_ROW * self.N, dtype=np.float32)
         self._chk(self.lib.fg_guess_features(self._h, nv.fptr(f)), "fg_guess_features")
-        return f.reshape(self.N, 12)
+        return f.reshape(self.N, nv.FG_GUESS_FEATURE_ROW)
 
     def reset_kernel_stats(self):
         self.lib.fg_reset_kernel_stats(self._h)

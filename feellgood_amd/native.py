@@ -73,7 +73,11 @@ _d = C.POINTER(C.c_double)
 _i32 = C.POINTER(C.c_int32)
 _i64 = C.POINTER(C.c_int64)
 _f32 = C.POINTER(C.c_float)
-FG_GUESS_NPARAM = 5574
+FG_GUESS_NPARAM = 5959
+FG_GUESS_NSCALE = 4        # the scales s_h, s_v, s_out, s_d at the end of the parameter block
+FG_GUESS_FEATURE_ROW = 16  # fg_guess_features: u, h, v, d (12 features), the base (3), one pad
+# offsets in the parameter block: W1[64][24], b1[64], W2[64][64], b2[64], W3[3][64], b3[3], scales
+GUESS_OFFSETS = dict(W1=0, b1=1536, W2=1600, b2=5696, W3=5760, b3=5952, scales=5955)
 
 # name -> (restype, argtypes); every symbol of include/fgmi.h
 HIP_SYMBOLS = {

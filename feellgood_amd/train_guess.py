@@ -35,8 +35,10 @@ def field_direction(rng, bmax=0.05):
     return d / np.linalg.norm(d) * b / MU0
 
 
-def run_direction(g, u0, H, dt, pref, steps, rng, train=None, inner=4, lr=2e-3):
-    """Steps from u0 under H; returns the iteration counts and the training losses."""
+def run_direction(g, u0, H, dt, pref, steps, rng, train=None, inner=4, lr=2e-3, losses_out=None):
+    """Steps from u0 under H; returns the iteration counts and the training losses (the
+    network's loss over the reference guess's). losses_out, when given, receives the loss of the
+    current parameters on each solve from the fourth on (the second difference is there)."""
     from feellgood_amd import native as nv
     g.set_state(nv.FG_BOTH, u=u0)
     g.calc_demag()
@@ -53,6 +55,8 @@ def run_direction(g, u0, H, dt, pref, steps, rng, train=None, inner=4, lr=2e-3):
             for _ in range(inner):
                 loss, ref = g.guess_train(lr)
             losses.append(loss / ref)
+        if losses_out is not None and k >= 3:
+            losses_out.append(g.guess_gradient()[1])
         g.calc_demag()
         g.evolution()
     return its, losses
@@ -69,9 +73,11 @@ def main(argv=None):
     ap.add_argument("--lr", type=float, default=2e-3)
     ap.add_argument("--seed", type=int, default=3)
     ap.add_argument("--out", default=None)
+    ap.add_argument("--report", default=None, help="write the summary as JSON")
     ap.add_argument("--demag", default="fmm")
     args = ap.parse_args(argv)
 
+    from feellgood_amd import native as nv
     from feellgood_amd import workloads as W
     from feellgood_amd.engine import Engine, options
     w = W.film_workload() if args.workload == "film" else W.cube_workload()
@@ -96,23 +102,54 @@ def main(argv=None):
     t_train = time.perf_counter() - t0
     trained = g.guess_get()
     base_only = trained.copy()
-    base_only[5376:5571] = 0  # W3, b3: the untrained head, v_pred = P_u v_base
-    res = {}
+    o = nv.GUESS_OFFSETS
+    base_only[o["W3"]:o["scales"]] = 0  # W3, b3: the untrained head, v_pred = P_u v_base
+    # held-out directions: the same states and basis angles with each guess; iterations per step,
+    # steps per second (guess and solve included) and the final magnetization against the
+    # reference guess's run (all solves meet the same tolerance)
+    res, rate, final = {}, {}, {}
     for name, params, on in (("reference", trained, False), ("base", base_only, True), ("learned", trained, True)):
         g.guess_set(params)
         g.guess_enable(on)
-        tot = []
+        tot, ends = [], []
+        g.synchronize()
+        t1 = time.perf_counter()
         for d, H in enumerate(test_fields):
             its, _ = run_direction(g, u0, H, dt, pref, args.steps, np.random.default_rng(777 + d))
             tot += its
+            ends.append(g.get_state(nv.FG_CURRENT)[0])
+        g.synchronize()
+        rate[name] = len(tot) / (time.perf_counter() - t1)
         res[name] = float(np.mean(tot))
+        final[name] = ends
+    # the guess error of the base and of the trained network on the same held-out solves (the
+    # loss Σ|v_pred - v*|² / (n s_out²) of each parameter set, in a run without the timing)
+    held_loss = {}
+    for name, params in (("base", base_only), ("learned", trained)):
+        g.guess_set(params)
+        g.guess_enable(True)
+        ls = []
+        for d, H in enumerate(test_fields):
+            run_direction(g, u0, H, dt, pref, args.steps, np.random.default_rng(777 + d), losses_out=ls)
+        held_loss[name] = float(np.mean(ls))
+    dm_end = {k: float(max(np.abs(a - b).max() for a, b in zip(final[k], final["reference"])))
+              for k in ("base", "learned")}
     g.guess_set(trained)
     g.guess_enable(False)
-    out = {"workload": args.workload, "train_directions": args.directions, "steps": args.steps,
-           "epochs": args.epochs, "train_seconds": round(t_train, 1), "final_loss_ratio": hist[-1],
+    out = {"workload": args.workload, "train_directions": args.directions, "held_out_directions": args.held_out,
+           "steps": args.steps, "epochs": args.epochs, "train_steps": args.directions * args.steps * args.epochs,
+           "train_seconds": round(t_train, 1), "final_loss_ratio": hist[-1],
            "iterations_per_step_reference": res["reference"], "iterations_per_step_base": res["base"],
-           "iterations_per_step_learned": res["learned"]}
+           "iterations_per_step_learned": res["learned"],
+           "steps_per_s_reference": round(rate["reference"], 2), "steps_per_s_base": round(rate["base"], 2),
+           "steps_per_s_learned": round(rate["learned"], 2),
+           "final_dm_vs_reference_guess": dm_end,
+           "held_out_loss_base": held_loss["base"], "held_out_loss_learned": held_loss["learned"],
+           "held_out_loss_ratio_learned_over_base": held_loss["learned"] / held_loss["base"]}
     print(json.dumps(out), flush=True)
+    if args.report:
+        with open(args.report, "w") as f:
+            json.dump(out, f, indent=1)
     if args.out:
         np.save(args.out, g.guess_get())
     g.close()

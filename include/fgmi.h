@@ -329,9 +329,11 @@ int fg_fmm_reference(const double *src_xyz, int64_t S, const double *tgt_xyz, in
  * of its linear extrapolation v + (dt / dt_prev)(v - v_prev) over the last two solutions (the
  * previous v alone until two solves have followed fg_set_state). BiCGSTAB then solves to the same
  * tolerance, so the guess changes iteration counts, not results beyond the tolerance.
- * Parameter block (FP32): W1[64][18], b1[64], W2[64][64], b2[64], W3[3][64], b3[3], then the
- * scales s_h, s_v, s_out (set from the first training batch when zero). */
-#define FG_GUESS_NPARAM 5574
+ * Its inputs are the node's u, tangent right-hand side over the lumped mass, previous v and the
+ * second difference of the last three solutions, and their means over the mesh neighbours.
+ * Parameter block (FP32): W1[64][24], b1[64], W2[64][64], b2[64], W3[3][64], b3[3], then the
+ * scales s_h, s_v, s_out, s_d (set from the first training batch when zero). */
+#define FG_GUESS_NPARAM 5959
 /* Allocate the network with random weights (W3 = 0: it starts as the extrapolation). Every
  * later fg_solve records the features of its assembly, so plain solves provide training data. */
 int fg_guess_init(fg_ctx *ctx, uint64_t seed);
@@ -345,10 +347,11 @@ int fg_guess_enable(fg_ctx *ctx, int on);
  * guess's loss on the same batch (both normalised by the extrapolation's error on the first
  * training batch, so that an untrained network scores about 1 there). */
 int fg_guess_train(fg_ctx *ctx, float lr, double *loss, double *loss_ref);
-/* Gradient of the loss on the last solve at the current parameters, [FG_GUESS_NPARAM - 3]. */
+/* Gradient of the loss on the last solve at the current parameters, [FG_GUESS_NPARAM - 4]. */
 int fg_guess_gradient(fg_ctx *ctx, float *grad, double *loss, double *loss_ref);
-/* Raw features of the last solve, [12 * n_nodes]: u, the tangent RHS over the mass, previous v,
- * and the extrapolated base the network corrects. */
+/* Raw features of the last solve, [16 * n_nodes]: u, the tangent RHS over the mass, previous v,
+ * the second difference of the last three solutions, the extrapolated base the network corrects,
+ * and one unused value. */
 int fg_guess_features(fg_ctx *ctx, float *feat);
 
 /* Octree FMM plan of an FG_DEMAG_FMM context (FG_ERR_STATE otherwise): info[0] leaf level,

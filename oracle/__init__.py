@@ -378,15 +378,15 @@ def run(mesh, u0, final_time, evol_time_step, dt_min, dt_max, du_max, Bext, seed
 
 
 def guess_eval(params, feat, adj_ptr, adj_idx, mag, v_target=None, round_bf16=True, want_grad=True, v_base=None):
-    """The learned-guess network on the CPU: (v_pred [n,3], loss, loss_ref, grad). feat is [n, 9]
-    (u, h, previous v) or the engine's [n, 12] export, whose last three columns are the base the
-    network corrects (v_base, else the previous v)."""
+    """The learned-guess network on the CPU: (v_pred [n,3], loss, loss_ref, grad). feat is [n, 12]
+    (u, h, previous v, second difference) or the engine's [n, 16] export, whose columns 12..14 are
+    the base the network corrects (v_base, else the previous v)."""
     n = len(mag)
     params = np.ascontiguousarray(params, dtype=np.float32)
     feat = np.asarray(feat, dtype=np.float32).reshape(n, -1)
-    if feat.shape[1] == 12 and v_base is None:
-        v_base = feat[:, 9:]
-    feat = np.ascontiguousarray(feat[:, :9]).reshape(-1)
+    if feat.shape[1] == nv.FG_GUESS_FEATURE_ROW and v_base is None:
+        v_base = feat[:, 12:15]
+    feat = np.ascontiguousarray(feat[:, :12]).reshape(-1)
     vb = None if v_base is None else np.ascontiguousarray(v_base, dtype=np.float32).reshape(-1)
     fp = C.POINTER(C.c_float)
     adj_ptr = np.ascontiguousarray(adj_ptr, dtype=np.int64)
@@ -394,7 +394,7 @@ def guess_eval(params, feat, adj_ptr, adj_idx, mag, v_target=None, round_bf16=Tr
     mag = np.ascontiguousarray(mag, dtype=np.int32)
     vt = None if v_target is None else np.ascontiguousarray(v_target, dtype=np.float64).reshape(-1)
     vp = np.zeros(3 * n)
-    grad = np.zeros(len(params) - 3) if want_grad else None
+    grad = np.zeros(len(params) - nv.FG_GUESS_NSCALE) if want_grad else None
     loss, loss_ref = C.c_double(), C.c_double()
     lib().orc_guess_eval(params.ctypes.data_as(C.POINTER(C.c_float)), 1 if round_bf16 else 0, n,
                          feat.ctypes.data_as(fp), None if vb is None else vb.ctypes.data_as(fp),

@@ -1932,14 +1932,14 @@ bailout:
 
 /* ---------------- learned initial guess: the network of the engine's guess, restated ----------------
  * Same parameter layout and formulas as the engine (documented in include/fgmi.h): features
- * f = [u, s_h h, s_v v], neighbour mean over the given adjacency (magnetic neighbours only), input
- * [f, mean], 18 -> 64 -> 64 -> 3 with the tanh GELU, v_pred = P_u (v_base + s_out o) with the
+ * f = [u, s_h h, s_v v, s_d d], neighbour mean over the given adjacency (magnetic neighbours only),
+ * input [f, mean], 24 -> 64 -> 64 -> 3 with the tanh GELU, v_pred = P_u (v_base + s_out o) with the
  * given base per node (the previous v when v_base is NULL), loss sum |v_pred - v*|^2 /
  * (n_mag s_out^2); loss_ref scores the reference guess P_u v the same way. Arithmetic in double; with round_bf16 the weight
  * matrices are first rounded to BF16 (round to nearest even) as the engine's forward pass uses
  * them. Gradients are with respect to the (rounded) weights and the biases. */
 namespace {
-const int GNF = 9, GNX = 18, GNH = 64, GNO = 3;
+const int GNF = 12, GNX = 24, GNH = 64, GNO = 3;
 const int G_W1 = 0, G_B1 = G_W1 + GNH * GNX, G_W2 = G_B1 + GNH, G_B2 = G_W2 + GNH * GNH, G_W3 = G_B2 + GNH,
           G_B3 = G_W3 + GNO * GNH, G_NTRAIN = G_B3 + GNO, G_S = G_NTRAIN;
 double g_bf16(float x)
@@ -1970,13 +1970,13 @@ void orc_guess_eval(const float *params, int round_bf16, int n, const float *fea
         const bool is_w = (k >= G_W1 && k < G_B1) || (k >= G_W2 && k < G_B2) || (k >= G_W3 && k < G_B3);
         W[k] = (round_bf16 && is_w) ? g_bf16(params[k]) : (double)params[k];
     }
-    const double sh = params[G_S], sv = params[G_S + 1], so = params[G_S + 2];
+    const double sh = params[G_S], sv = params[G_S + 1], so = params[G_S + 2], sd = params[G_S + 3];
     int nmag = 0;
     for (int i = 0; i < n; i++) nmag += mag[i] != 0;
     const double inv_norm = 1.0 / (std::max(1, nmag) * so * so);
     if (grad) std::fill(grad, grad + G_NTRAIN, 0.0);
     double L = 0, Lr = 0;
-    const double sc[GNF] = {1, 1, 1, sh, sh, sh, sv, sv, sv};
+    const double sc[GNF] = {1, 1, 1, sh, sh, sh, sv, sv, sv, sd, sd, sd};
     for (int i = 0; i < n; i++) {
         if (!mag[i]) {
             if (v_pred) v_pred[3 * i] = v_pred[3 * i + 1] = v_pred[3 * i + 2] = 0;

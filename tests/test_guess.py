@@ -4,9 +4,10 @@ CPU: the oracle's restatement of the network, pinned by central finite differenc
 (gradient), by the closed form of a zero output head (the reference guess, loss = loss_ref) and
 by the tangency of its prediction.
 GPU: the engine's features against numpy recomputations from the assembled system, its forward
-prediction and gradient against the oracle on the same batch, and training: the loss falls below
-the reference guess's and the trained guess saves iterations on held-out applied fields while the
-solve still meets its tolerance.
+prediction and gradient against the oracle on the same batch, and training: on held-out applied
+fields the trained network's guess error falls well below its untrained base's (the linear
+extrapolation) and it takes no more iterations than the base, while the solve still meets its
+tolerance.
 """
 import math
 
@@ -18,7 +19,9 @@ from feellgood_amd import native as nv
 from feellgood_amd.mesh import HostMesh
 
 NP = nv.FG_GUESS_NPARAM
-W1, B1, W2, B2, W3, B3 = 0, 1152, 1216, 5312, 5376, 5568
+_O = nv.GUESS_OFFSETS
+W1, B1, W2, B2, W3, B3 = _O["W1"], _O["b1"], _O["W2"], _O["b2"], _O["W3"], _O["b3"]
+NS = nv.FG_GUESS_NSCALE
 
 
 def random_problem(n=40, seed=0):
@@ -27,7 +30,8 @@ def random_problem(n=40, seed=0):
     u /= np.linalg.norm(u, axis=1, keepdims=True)
     h = rng.normal(size=(n, 3)) * 3e4
     v = rng.normal(size=(n, 3)) * 2e9
-    feat = np.concatenate([u, h, v], axis=1).astype(np.float32)
+    d2 = rng.normal(size=(n, 3)) * 1e7
+    feat = np.concatenate([u, h, v, d2], axis=1).astype(np.float32)
     nb = [set() for _ in range(n)]
     for i in range(n):
         for j in ((i + 1) % n, (i + 7) % n):
@@ -43,7 +47,7 @@ def random_problem(n=40, seed=0):
     target = v + rng.normal(size=(n, 3)) * 1e8
     target -= np.sum(target * u, axis=1, keepdims=True) * u
     p = (rng.normal(size=NP) * 0.2).astype(np.float32)
-    p[-3:] = [1 / 3e4, 1 / 2e9, 1e8]
+    p[-NS:] = [1 / 3e4, 1 / 2e9, 1e8, 1 / 1e7]
     return p, feat, ptr, np.array(idx, dtype=np.int32), mag, target
 
 
@@ -69,15 +73,16 @@ def test_oracle_zero_head_is_the_reference_guess():
     p, feat, ptr, idx, mag, tgt = random_problem(seed=2)
     p[W3:B3 + 3] = 0
     vp, L, Lr, _ = oracle.guess_eval(p, feat, ptr, idx, mag, tgt)
-    u, v = feat[:, :3].astype(np.float64), feat[:, 6:].astype(np.float64)
+    u, v = feat[:, :3].astype(np.float64), feat[:, 6:9].astype(np.float64)
     ref = v - np.sum(v * u, axis=1, keepdims=True) * u
     ok = mag.astype(bool)
     assert np.allclose(vp[ok], ref[ok], rtol=0, atol=1e-6 * np.abs(ref).max())
     assert L == pytest.approx(Lr, rel=1e-12)
-    # with a base (the engine's 12-column export), the zero head predicts the projected base
+    # with a base (the engine's 16-column export), the zero head predicts the projected base
     rng = np.random.default_rng(4)
     base = (v + rng.normal(size=v.shape) * 3e8).astype(np.float32)
-    vb, L2, Lr2, _ = oracle.guess_eval(p, np.concatenate([feat, base], axis=1), ptr, idx, mag, tgt)
+    row = np.concatenate([feat, base, np.zeros((len(feat), 1), np.float32)], axis=1)
+    vb, L2, Lr2, _ = oracle.guess_eval(p, row, ptr, idx, mag, tgt)
     b64 = base.astype(np.float64)
     refb = b64 - np.sum(b64 * u, axis=1, keepdims=True) * u
     assert np.allclose(vb[ok], refb[ok], rtol=0, atol=1e-6 * np.abs(refb).max())
@@ -147,7 +152,7 @@ def test_features_forward_and_gradient_match_oracle():
     g.guess_init(seed=7)
     dt, pref, H = 2e-14, 1 + 0.01 * abs(math.log(0.01)), field(0.4)
     v_hist = []
-    for k in range(3):
+    for k in range(4):
         u_before = g.get_state(nv.FG_CURRENT)[0]
         v_before = g.get_state(nv.FG_NEXT)[1]
         v_hist.append(v_before)
@@ -155,7 +160,7 @@ def test_features_forward_and_gradient_match_oracle():
         g.prepare_elements(H, dt, pref)
         failed, info = g.solve(dt)
         assert not failed
-        if k == 2:
+        if k == 3:
             ep, eq = g.basis()
             _, _, _, rhs = g.system()
             break
@@ -168,15 +173,17 @@ def test_features_forward_and_gradient_match_oracle():
     assert np.allclose(feat[:, :3], u_before, atol=1e-6)
     assert np.allclose(feat[:, 6:9], v_before, rtol=1e-6, atol=1e-6 * np.abs(v_before).max())
     assert np.allclose(feat[:, 3:6], h, rtol=1e-5, atol=1e-5 * np.abs(h).max())
-    # the base: linear extrapolation of the last two solutions (same dt); at the third assembly
-    # after set_state both previous v's come from solves
-    base = 2 * v_hist[2] - v_hist[1]
-    assert np.allclose(feat[:, 9:], base, rtol=1e-5, atol=1e-5 * np.abs(base).max())
+    # the second difference of the last three solutions (all at the same dt: from the fourth
+    # assembly after set_state on), and the base: the linear extrapolation of the last two
+    d2 = v_hist[3] - 2 * v_hist[2] + v_hist[1]
+    assert np.allclose(feat[:, 9:12], d2, rtol=1e-4, atol=1e-4 * np.abs(d2).max())
+    base = 2 * v_hist[3] - v_hist[2]
+    assert np.allclose(feat[:, 12:15], base, rtol=1e-5, atol=1e-5 * np.abs(base).max())
     # gradient on this batch (scales are set from it on the first call)
     grad, loss, loss_ref = g.guess_gradient()
     p = g.guess_get()
     # scales set from this batch: the untrained network (the base) scores 1
-    assert p[-3:].min() > 0 and loss == pytest.approx(1.0, rel=1e-4)
+    assert p[-NS:].min() > 0 and loss == pytest.approx(1.0, rel=1e-4)
     vstar = g.get_state(nv.FG_NEXT)[1]
     ptr, idx = adjacency(m)
     mag = np.ones(m.n_nodes, dtype=np.int32)
@@ -237,14 +244,19 @@ def test_inference_pass_matches_oracle():
     proj = np.sum(vp * ep, axis=1, keepdims=True) * ep + np.sum(vp * eq, axis=1, keepdims=True) * eq
     v = g.get_state(nv.FG_NEXT)[1]
     # the network's own contribution must be visible against its base
-    prev = feat[:, 9:].astype(np.float64)
+    prev = feat[:, 12:15].astype(np.float64)
     assert np.abs(vp - prev).max() > 1e-3 * np.abs(prev).max()
     assert np.allclose(v, proj, rtol=0, atol=2e-5 * np.abs(proj).max())
     g.close()
 
 
 @pytest.mark.gpu
-def test_training_beats_the_reference_guess_on_held_out_fields():
+def test_training_beats_the_untrained_base_on_held_out_fields():
+    """Trained on plain solves of five field directions, the network against its own untrained base
+    (W3 = b3 = 0: the linear extrapolation of the last two solutions) and against the reference
+    guess (previous v) on two held-out directions, from the same states and basis angles: its loss
+    (the guess's error against the converged solution) falls below 0.7 of the base's, and it takes
+    no more BiCGSTAB iterations than the base, which takes no more than the reference guess."""
     from feellgood_amd.engine import Engine
     m = cube(10)
     dt, pref = 2e-14, 1 + 0.01 * abs(math.log(0.01))
@@ -252,43 +264,46 @@ def test_training_beats_the_reference_guess_on_held_out_fields():
     g.guess_init(seed=11)
     u0 = initial_u(m)
     rng = np.random.default_rng(12)
-    losses = []
     for ep in range(6):
         for theta in (0.0, 1.3, 2.6, 3.9, 5.2):
             g.set_state(nv.FG_BOTH, u=u0)
             g.calc_demag()
             g.evolution()
             H = field(theta + 0.1 * ep)
-            for k in range(8):
+            for k in range(10):
                 g.base_projection(rng.uniform(0, 2 / math.pi))
                 g.prepare_elements(H, dt, pref)
                 failed, _ = g.solve(dt)
                 assert not failed
                 if k >= 2:
                     for _ in range(4):
-                        loss, loss_ref = g.guess_train(2e-3)
-                    losses.append(loss / loss_ref)
+                        g.guess_train(2e-3)
                 g.calc_demag()
                 g.evolution()
-    assert np.mean(losses[-20:]) < 0.8, losses[-20:]
-    # held-out fields: iterations with and without the network, same angles and states
-    its = {}
-    for on in (False, True):
+    trained = g.guess_get()
+    base = trained.copy()
+    base[W3:B3 + 3] = 0
+    its, loss = {}, {}
+    for name, params, on in (("reference", trained, False), ("base", base, True), ("trained", trained, True)):
+        g.guess_set(params)
         g.guess_enable(on)
         rng = np.random.default_rng(99)
-        tot = 0
+        tot, ls = 0, []
         for theta in (0.65, 3.25):
             g.set_state(nv.FG_BOTH, u=u0)
             g.calc_demag()
             g.evolution()
-            for k in range(8):
+            for k in range(10):
                 g.base_projection(rng.uniform(0, 2 / math.pi))
                 g.prepare_elements(field(theta), dt, pref)
                 failed, info = g.solve(dt)
                 assert not failed and info.residual <= 1e-10
                 tot += info.iterations
+                if k >= 3:  # the second difference is there from the fourth solve on
+                    ls.append(g.guess_gradient()[1])
                 g.calc_demag()
                 g.evolution()
-        its[on] = tot
-    # the network may not lose iterations; the gain is measured at full size by the benchmark
-    assert its[True] <= its[False], its
+        its[name], loss[name] = tot, float(np.mean(ls))
+    print(f"held-out: iterations {its}, loss trained / base {loss['trained'] / loss['base']:.3f}", flush=True)
+    assert loss["trained"] < 0.7 * loss["base"], loss
+    assert its["trained"] <= its["base"] <= its["reference"], its
