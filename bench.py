@@ -53,16 +53,29 @@ def bench_dots(args):
           file=sys.stderr, flush=True)
     prec = "fp64" if args.mode == "plain" else "mixed"
 
+    block = 0
+
     def make(leaf, profile=False):
         return Engine.batch([m.mesh for m in meshes], options(tol=w["tol"], max_iter=10000, demag="fmm",
                                                               fmm_order=args.fmm_order, fmm_leaf=leaf,
                                                               fmm_far_order=args.fmm_far_order,
-                                                              precision=prec, profile=profile))
+                                                              precision=prec, profile=profile,
+                                                              batch_solver=args.batch_solver, batch_block=block))
     leaf, tune = tune_fmm_leaf(make, u0)
     print(f"[bench] autotuned fmm_leaf {leaf}: {tune}", file=sys.stderr, flush=True)
-    g = make(leaf)
     H = np.zeros(3)
     dt, pref = w["dt"], prefactor(w["dt"])
+    tune_block = None
+    if args.batch_solver == "persistent":
+        from feellgood_amd.autotune import tune_batch_block
+
+        def make_blk(b):
+            nonlocal block
+            block = b
+            return make(leaf)
+        block, tune_block = tune_batch_block(make_blk, u0, H, dt, pref)
+        print(f"[bench] autotuned batch block {block}: {tune_block}", file=sys.stderr, flush=True)
+    g = make(leaf)
     rng = np.random.default_rng(w["seed"])
     g.set_state(nv.FG_BOTH, u=u0)
     g.calc_demag()
@@ -96,7 +109,9 @@ def bench_dots(args):
            "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
            "dtype": "f64" if prec == "fp64" else "f32+f64", "data": "synthetic",
            "config": {"workload": f"{len(meshes)} ellipsoidal dots, {T} tets, h {w['h']} nm, dt {w['dt']:g} s",
-                      "mode": prec, "fmm_leaf": leaf, "autotune_ms_per_demag": tune},
+                      "mode": prec, "fmm_leaf": leaf, "autotune_ms_per_demag": tune,
+                      "batch_solver": args.batch_solver, "batch_block": block if tune_block else None,
+                      "autotune_ms_per_solve": tune_block},
            "tet_steps_per_s": round(T * 1e3 / ms, 1),
            "bicgstab_iterations_per_step": round(float(np.mean(iters)), 2)}
     print(json.dumps(out), flush=True)
@@ -113,6 +128,15 @@ def m2p_flops(p):
     for m in range(1, p + 1):
         f += 18 + (15 if m < p else 0) + 17 * max(0, p - m - 1)
     return f + 14
+
+
+V_DEGREE = 4  # degree of the far field of the charges of v in the mixed-precision mode
+
+
+def m2p_flops_v(p, pv):
+    """m2p_flops with the second expansion (charges of v) taken to degree pv <= p only: its fused
+    multiply-adds of degrees above pv (one per real m = 0 term, two per complex term) drop out."""
+    return m2p_flops(p) - sum(2 + 4 * n for n in range(pv + 1, p + 1))
 
 
 FAR_DEGREE = {4: 2, 6: 4, 8: 5, 10: 6}  # lower degree of the far list entries, per order
@@ -161,10 +185,10 @@ def m2m_flops(p):
 
 
 def p2m_flops(p):
-    """FP64 operations of accumulating one source into both expansions (the regular recurrence is
-    evaluated once per charge set)."""
+    """Operations of accumulating one source into both expansions: its charges from the element
+    values, the regular recurrence once, and four fused multiply-adds per complex coefficient."""
     rec = 5 + 7 * p + 2 * p + 9 * sum(max(0, p - m - 1) for m in range(p + 1))
-    return 2 * (rec + 2 * (p + 1) * (p + 2) // 2)
+    return 4 + rec + 8 * (p + 1) * (p + 2) // 2
 
 
 def kernel_model(name, n):
@@ -190,24 +214,31 @@ def kernel_model(name, n):
     f = n.get("fmm")
     if f:
         p, Tg = f["order"], f["targets"]
+        mixed = n.get("precision") == "mixed"
+        pf = far_degree(p, n.get("far_order"), n.get("precision"))
+        pv = V_DEGREE if mixed else p
         models.update({
-            # coefficient rows + shared columns + element values read once + per-target I/O
-            "fmm_near": ("hbm", 8 * f["near_coefs"] + 4 * f["near_columns"] + 16 * (T + 3 * F)
+            # coefficient rows (FP32 in the mixed mode) + shared columns + element values read once
+            # + per-target I/O
+            "fmm_near": ("hbm", (4 if mixed else 8) * f["near_coefs"] + 4 * f["near_columns"] + 16 * (T + 3 * F)
                          + (8 + 16 + 16 + 4) * Tg + 12 * f["target_leaves"]),
-            "fmm_m2p": ("valu32" if n.get("precision") == "mixed" else "valu64",
-                        m2p_flops(p) * (f["m2p_pairs"] - f["m2p_far_pairs"])
-                        + m2p_flops(far_degree(p, n.get("far_order"), n.get("precision"))) * f["m2p_far_pairs"]),
-            "fmm_m2l": ("mfma64", m2l_flops(p) * f["m2l_pairs"]),
-            "fmm_m2m": ("mfma64", m2m_flops(p) * f["m2m_cells"]),
-            "fmm_p2m": ("valu64", p2m_flops(p) * f["sources"]),
+            "fmm_m2p": ("valu32" if mixed else "valu64",
+                        m2p_flops_v(p, min(p, pv)) * (f["m2p_pairs"] - f["m2p_far_pairs"])
+                        + m2p_flops_v(pf, min(pf, pv)) * f["m2p_far_pairs"]),
+            "fmm_m2l": ("mfma32" if mixed else "mfma64", m2l_flops(p) * f["m2l_pairs"]),
+            "fmm_m2m": ("mfma32" if mixed else "mfma64", m2m_flops(p) * f["m2m_cells"]),
+            "fmm_p2m": ("valu32" if mixed else "valu64", p2m_flops(p) * f["sources"]),
         })
     return models.get(name)
 
 
 def make_engine(args, w, m, precision, guess, profile):
     from feellgood_amd.engine import Engine, options
+    # the Krylov mode applies to the measured mixed modes; the plain FP64 engine keeps FP64 bicg_dir
+    krylov = getattr(args, "krylov", "fp64") if precision == "mixed" else "fp64"
     opts = options(tol=w["tol"], max_iter=10000, demag=args.demag, fmm_order=args.fmm_order,
-                   fmm_leaf=args.fmm_leaf, precision=precision, profile=profile, fmm_far_order=args.fmm_far_order)
+                   fmm_leaf=args.fmm_leaf, precision=precision, profile=profile, fmm_far_order=args.fmm_far_order,
+                   krylov=krylov, krylov_inner_tol=getattr(args, "krylov_inner_tol", 0.0))
     g = Engine(m.mesh, opts)
     if guess is not None:
         g.guess_set(guess)
@@ -360,7 +391,7 @@ def roofline(stats, sizes):
                 entry["GBps"] = round(work / avg_s / 1e9, 1)
                 entry["frac_hbm"] = round(work / avg_s / 1e9 / HBM_PEAK_GBS, 4)
             else:
-                pk = FP32_VALU_TFLOPS if bound == "valu32" else FP64_VALU_TFLOPS
+                pk = FP32_VALU_TFLOPS if bound.endswith("32") else FP64_VALU_TFLOPS
                 entry["TFLOPs"] = round(work / avg_s / 1e12, 2)
                 entry["frac_valu"] = round(work / avg_s / 1e12 / pk, 4)
         per_kernel[name] = entry
@@ -377,11 +408,11 @@ def roofline(stats, sizes):
               "avg_launch_us": round(avg_s * 1e6, 3)}
     else:
         achieved = work / avg_s / 1e12
-        pk = FP32_VALU_TFLOPS if bound == "valu32" else FP64_VALU_TFLOPS
-        rl = {"kernel": name, "bound": "valu", "achieved": round(achieved, 2), "peak": pk,
-              "unit": "TFLOP/s", "frac": round(achieved / pk, 4), "traffic": None,
+        pk = FP32_VALU_TFLOPS if bound.endswith("32") else FP64_VALU_TFLOPS
+        rl = {"kernel": name, "bound": "mfma" if bound.startswith("mfma") else "valu", "achieved": round(achieved, 2),
+              "peak": pk, "unit": "TFLOP/s", "frac": round(achieved / pk, 4), "traffic": None,
               "flops_per_launch": work, "avg_launch_us": round(avg_s * 1e6, 3),
-              "precision": "fp32" if bound == "valu32" else "fp64"}
+              "precision": "fp32" if bound.endswith("32") else "fp64"}
     return rl, per_kernel
 
 
@@ -459,7 +490,15 @@ def main():
                     help="degree of the far source cells (0: the order in FP64, the lower far degree in mixed)")
     ap.add_argument("--guess", default=None, help="network parameters (.npy); default: the committed ones")
     ap.add_argument("--prof-steps", type=int, default=3)
+    ap.add_argument("--krylov", default="fp64", choices=["fp64", "mixed"],
+                    help="the mixed modes' Krylov solve: FP64 bicg_dir, or FP32 bicg_dir inside FP64 refinement")
+    ap.add_argument("--krylov-inner-tol", type=float, default=0.0,
+                    help="relative target of each FP32 solve of --krylov mixed (0: the library's default)")
+    ap.add_argument("--batch-solver", default="persistent", choices=["union", "persistent"],
+                    help="--dots: one bicg_dir over the union, or one launch with a block per dot")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-compare", action="store_true",
+                    help="skip the plain FP64 run and the accuracy check (profiling runs of one mode)")
     ap.add_argument("--small", action="store_true", help="the 48k-tet cube instead of the film")
     ap.add_argument("--dots", action="store_true", help="the batched nanodots instead of the film")
     ap.add_argument("--wire", action="store_true",
@@ -498,14 +537,14 @@ def main():
             # no trained weights for this workload: the untrained head, i.e. the extrapolated base
             from feellgood_amd import native as nv
             guess = np.zeros(nv.FG_GUESS_NPARAM, dtype=np.float32)
-            guess[-3:] = 1.0
+            guess[-nv.FG_GUESS_NSCALE:] = 1.0
             guess_src = "untrained head: the extrapolation of the last two solutions"
     args.precision = precision
     res = run_gpu(args, w, m, u0, precision, guess)
     print(f"[bench] {args.mode}: {res['ms_per_step']:.2f} ms/step, iterations {res['iters']}", file=sys.stderr,
           flush=True)
     plain = None
-    if args.mode != "plain":
+    if args.mode != "plain" and not args.no_compare:
         g = make_engine(args, w, m, "fp64", None, False)
         st = Stepper(g, w, u0)
         for _ in range(args.warmup):
@@ -557,6 +596,8 @@ def main():
                    "initial_guess": (f"learned ({guess_src})" if guess is not None else
                                      "previous v (reference buildInitGuess)"),
                    "demag": args.demag, "fmm_order": args.fmm_order if args.demag == "fmm" else None,
+                   "krylov": args.krylov if precision == "mixed" else "fp64",
+                   "krylov_inner_tol": args.krylov_inner_tol if precision == "mixed" and args.krylov == "mixed" else None,
                    "parallelism": "single GPU"},
         "bicgstab_iterations_per_step": (round(float(np.mean(res["iters"])), 2) if res["iters"] else None),
         "solver_failures": res["fails"],

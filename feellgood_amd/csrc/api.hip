@@ -412,6 +412,21 @@ void build_plan(Ctx &c, const fg_mesh *m)
     upload(c.scol, scol, st);
     c.sval = dalloc<double>(c.nnzS);
     upload(c.sval, sval, st);
+    if (c.opt.krylov == FG_KRYLOV_MIXED) {
+        std::vector<float> svf(sval.begin(), sval.end());
+        c.svalf = dalloc<float>(c.nnzS);
+        upload(c.svalf, svf, st);
+        for (float4 **q : {&c.ep4f, &c.eq4f, &c.Dgf, &c.wf}) *q = dalloc<float4>(N);
+        for (float2 **q : {&c.dprecf, &c.rf, &c.rtf, &c.pf, &c.phf, &c.vf, &c.sf, &c.shf, &c.tf, &c.df}) {
+            *q = dalloc<float2>(N);
+            HIPCHK(hipMemsetAsync(*q, 0, sizeof(float2) * N, st));
+        }
+        c.ic = dalloc<KrylovCtl>(1);
+        HIPCHK(hipHostMalloc((void **)&c.h_ic, sizeof(KrylovCtl), hipHostMallocDefault));
+        std::memset(c.h_ic, 0, sizeof(KrylovCtl));
+        HIPCHK(hipMemcpyAsync(c.ic, c.h_ic, sizeof(KrylovCtl), hipMemcpyHostToDevice, st));
+        HIPCHK(hipStreamSynchronize(st));
+    }
     std::vector<int4> tn(Tm);
     for (int t = 0; t < Tm; t++) tn[t] = make_int4(mtets[4 * t], mtets[4 * t + 1], mtets[4 * t + 2], mtets[4 * t + 3]);
     c.tet_nodes = dalloc<int4>(Tm);
@@ -437,8 +452,14 @@ void build_plan(Ctx &c, const fg_mesh *m)
     c.elem_out = dalloc<double4>((size_t)4 * Tm + (size_t)3 * Fk);
     c.inc_ptr = dalloc<int32_t>(N + 1);
     upload(c.inc_ptr, incp, st);
-    c.inc_idx = dalloc<int32_t>(inci.size());
-    upload(c.inc_idx, inci, st);
+    {
+        // the slot of each element output entry in node order (inverse of the gather list)
+        std::vector<int32_t> epos(inci.size());
+        for (size_t k = 0; k < inci.size(); k++) epos[inci[k]] = (int32_t)k;
+        c.epos = dalloc<int32_t>(epos.size());
+        upload(c.epos, epos, st);
+        HIPCHK(hipStreamSynchronize(st));  // the host vector goes at the end of this block
+    }
     c.trik_nodes = dalloc<int4>(trikn.size());
     upload(c.trik_nodes, trikn, st);
     c.trik_geo = dalloc<double>(tkgeo.size());
@@ -476,6 +497,19 @@ void build_plan(Ctx &c, const fg_mesh *m)
         upload(c.d_node_inst, ni, st);
         c.d_inst_src = dalloc<longlong4>(K);
         upload(c.d_inst_src, rs, st);
+        // instances by decreasing size (the persistent solve starts the long ones first)
+        std::vector<int32_t> order(K);
+        for (size_t k = 0; k < K; k++) order[k] = (int32_t)k;
+        std::stable_sort(order.begin(), order.end(), [&](int32_t a, int32_t b) {
+            return c.inst_node_off[a + 1] - c.inst_node_off[a] > c.inst_node_off[b + 1] - c.inst_node_off[b];
+        });
+        c.d_inst_order = dalloc<int32_t>(K);
+        upload(c.d_inst_order, order, st);
+        c.d_inst_stat = dalloc<int2>(K);
+        c.d_inst_resn = dalloc<double2>(K);
+        HIPCHK(hipHostMalloc((void **)&c.h_inst_stat, K * sizeof(int2), hipHostMallocDefault));
+        HIPCHK(hipHostMalloc((void **)&c.h_inst_resn, K * sizeof(double2), hipHostMallocDefault));
+        HIPCHK(hipStreamSynchronize(st));  // order goes at the end of this block
     }
     HIPCHK(hipStreamSynchronize(st));
 }
@@ -484,13 +518,18 @@ void free_all(Ctx &c)
 {
     void *ptrs[] = {c.rec[0], c.rec[1], c.pos4, c.ep4, c.eq4, c.mag, c.mass, c.sdiag, c.Dg, c.dprec, c.b, c.x, c.r,
                     c.rt, c.p, c.ph, c.v, c.s, c.sh, c.t, c.w, c.ctl, c.part, c.epart, c.res, c.slice_ptr, c.slice_w,
-                    c.scol, c.sval, c.tet_nodes, c.geo, c.tet_reg, c.regions, c.elem_out, c.inc_ptr, c.inc_idx,
+                    c.scol, c.sval, c.tet_nodes, c.geo, c.tet_reg, c.regions, c.elem_out, c.inc_ptr, c.epos,
                     c.trik_nodes, c.trik_geo, c.tri_nodes, c.tri_geo, c.tri_corr, c.cinc_ptr, c.cinc_idx, c.corr,
                     c.src_pos, c.src_q, c.src_pos_f, c.src_rank, c.d_inst_node_off, c.inst_bn2, c.d_node_inst, c.d_inst_src,
-                    c.hsp, c.hst, c.spin_s, c.spin_pref};
+                    c.hsp, c.hst, c.spin_s, c.spin_pref, c.svalf, c.ep4f, c.eq4f, c.Dgf, c.wf, c.dprecf, c.rf,
+                    c.rtf, c.pf, c.phf, c.vf, c.sf, c.shf, c.tf, c.df, c.ic, c.d_inst_order, c.d_inst_stat,
+                    c.d_inst_resn};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     if (c.h_ctl) (void)hipHostFree(c.h_ctl);
+    if (c.h_ic) (void)hipHostFree(c.h_ic);
+    if (c.h_inst_stat) (void)hipHostFree(c.h_inst_stat);
+    if (c.h_inst_resn) (void)hipHostFree(c.h_inst_resn);
     if (c.h_part) (void)hipHostFree(c.h_part);
     for (auto &pnd : c.pending) {
         (void)hipEventDestroy(pnd.a);
@@ -595,7 +634,13 @@ int create_impl(const fg_mesh *mesh, const fg_options *opt, fg_ctx **out, const 
     Ctx &c = x->c;
     if (opt) c.opt = *opt;
     else fg_default_options(&c.opt);
-    if (c.opt.max_iter <= 0 || !(c.opt.tol > 0)) {
+    if (c.opt.max_iter <= 0 || !(c.opt.tol > 0) ||
+        (c.opt.krylov != FG_KRYLOV_FP64 && c.opt.krylov != FG_KRYLOV_MIXED) || !(c.opt.krylov_inner_tol >= 0) ||
+        c.opt.krylov_inner_tol >= 1 ||
+        (c.opt.batch_solver != FG_BATCH_UNION && c.opt.batch_solver != FG_BATCH_PERSISTENT) ||
+        (c.opt.batch_solver == FG_BATCH_PERSISTENT && c.opt.krylov == FG_KRYLOV_MIXED) ||
+        (c.opt.batch_block != 0 && c.opt.batch_block != 256 && c.opt.batch_block != 512 &&
+         c.opt.batch_block != 1024)) {
         g_create_err = "fg_create: invalid options";
         delete x;
         return FG_ERR_ARG;
@@ -911,6 +956,96 @@ int fg_prepare_elements_amp(fg_ctx *x, double amp, double dt, double prefactor)
     });
 }
 
+// After the solve (either Krylov mode): the update of NEXT, the guess's bookkeeping and v_max.
+static int finish_solve(fg_ctx *x, double dt, fg_solve_info *info)
+{
+    Ctx &c = x->c;
+    launch_update(c, dt);
+    guess_on_solved(c);
+    HIPCHK(hipMemcpyAsync(c.h_ctl, c.ctl, sizeof(KrylovCtl), hipMemcpyDeviceToHost, c.stream));
+    HIPCHK(hipStreamSynchronize(c.stream));
+    if (info) info->v_max = GAMMA0 * std::sqrt(c.h_ctl->vmax2);
+    if (c.opt.profile) c.flush_stats();
+    return FG_OK;
+}
+
+// FG_KRYLOV_MIXED: FP64 residual, FP32 bicg_dir on the correction, x += d, until the FP64 residual
+// meets tol ||b||. The FP32 loop runs in batches between host reads of its control block, the
+// first batch one more than the last solve's FP32 count; the outer test follows each correction.
+static int solve_mixed(fg_ctx *x, double dt, double cst, fg_solve_info *info)
+{
+    Ctx &c = x->c;
+    // the FP32 loop's relative target (above its rounding floor) and its iterations per correction
+    const double TOL_IN = c.opt.krylov_inner_tol > 0 ? c.opt.krylov_inner_tol : 3e-7;
+    constexpr int MAX_INNER = 20;
+    constexpr int MAX_REF = 8;  // corrections
+    launch_mixed_start(c);
+    launch_mixed_residual(c, cst, TOL_IN, MAX_INNER);
+    bool done = false;
+    for (int ref = 0; ref < MAX_REF && !done; ref++) {
+        int launched = 0, batch = std::max(2, std::min(x->last_nit + 1, MAX_INNER));
+        while (true) {
+            for (int k = 0; k < batch && launched <= MAX_INNER; k++, launched++) launch_inner_iteration(c, cst);
+            HIPCHK(hipMemcpyAsync(c.h_ic, c.ic, sizeof(KrylovCtl), hipMemcpyDeviceToHost, c.stream));
+            HIPCHK(hipStreamSynchronize(c.stream));
+            if (c.h_ic->done || launched > MAX_INNER) break;
+            batch = 4;
+        }
+        launch_refine(c);
+        launch_mixed_residual(c, cst, TOL_IN, MAX_INNER);
+        HIPCHK(hipMemcpyAsync(c.h_ctl, c.ctl, sizeof(KrylovCtl), hipMemcpyDeviceToHost, c.stream));
+        HIPCHK(hipStreamSynchronize(c.stream));
+        done = c.h_ctl->done != 0;
+    }
+    const KrylovCtl k = *c.h_ctl;
+    x->last_nit = std::max(2, k.refinements > 0 ? k.nit / k.refinements : k.nit);
+    const bool failed = !done || k.status != FG_ITER_CONVERGED;
+    if (info) {
+        std::memset(info, 0, sizeof(*info));
+        info->status = done ? k.status : FG_ITER_OVERFLOW;
+        info->iterations = k.nit;
+        info->residual = k.res;
+        info->rhs_norm = k.rhsn;
+        info->refinements = k.refinements;
+        info->inner_iterations = k.nit;
+    }
+    if (failed) return FG_SOLVE_FAILED;
+    return finish_solve(x, dt, info);
+}
+
+// FG_BATCH_PERSISTENT: one launch solves every instance to its own criterion; the batch has failed
+// if any instance has. Reported: the worst status, the most iterations, the largest residual and
+// the smallest nonzero ||b_k||.
+static int solve_batch(fg_ctx *x, double dt, double cst, fg_solve_info *info)
+{
+    Ctx &c = x->c;
+    const int K = (int)c.inst_node_off.size() - 1;
+    launch_batch_solve(c, cst, c.opt.batch_block);
+    HIPCHK(hipMemcpyAsync(c.h_inst_stat, c.d_inst_stat, K * sizeof(int2), hipMemcpyDeviceToHost, c.stream));
+    HIPCHK(hipMemcpyAsync(c.h_inst_resn, c.d_inst_resn, K * sizeof(double2), hipMemcpyDeviceToHost, c.stream));
+    HIPCHK(hipStreamSynchronize(c.stream));
+    int status = FG_ITER_CONVERGED, nit = 0;
+    double res = 0, rhsn = -1;
+    for (int k = 0; k < K; k++) {
+        const int2 st = c.h_inst_stat[k];
+        const double2 rn = c.h_inst_resn[k];
+        if (st.x != FG_ITER_CONVERGED && status == FG_ITER_CONVERGED) status = st.x;
+        nit = std::max(nit, st.y);
+        res = std::max(res, rn.x);
+        if (rn.y > 0 && (rhsn < 0 || rn.y < rhsn)) rhsn = rn.y;
+    }
+    x->last_nit = std::max(2, nit);
+    if (info) {
+        std::memset(info, 0, sizeof(*info));
+        info->status = status;
+        info->iterations = nit;
+        info->residual = res;
+        info->rhs_norm = std::max(rhsn, 0.0);
+    }
+    if (status != FG_ITER_CONVERGED) return FG_SOLVE_FAILED;
+    return finish_solve(x, dt, info);
+}
+
 int fg_solve(fg_ctx *x, double dt, fg_solve_info *info)
 {
     return guarded(x, [&]() {
@@ -921,6 +1056,9 @@ int fg_solve(fg_ctx *x, double dt, fg_solve_info *info)
         launch_node_assemble(c, x->r, cst, dt);
         guess_on_assembly(c, dt);
         x->basis_valid = true;
+        if (c.opt.krylov == FG_KRYLOV_MIXED) return solve_mixed(x, dt, cst, info);
+        if (c.opt.batch_solver == FG_BATCH_PERSISTENT && c.inst_node_off.size() > 2)
+            return solve_batch(x, dt, cst, info);
         launch_r0(c, cst);
         // BiCGSTAB: batches of iterations, the device control block says when to stop. Iteration
         // n's convergence is tested at the start of iteration n + 1, so the first batch is the
@@ -948,13 +1086,7 @@ int fg_solve(fg_ctx *x, double dt, fg_solve_info *info)
             info->v_max = 0;
         }
         if (failed) return FG_SOLVE_FAILED;
-        launch_update(c, dt);
-        guess_on_solved(c);
-        HIPCHK(hipMemcpyAsync(c.h_ctl, c.ctl, sizeof(KrylovCtl), hipMemcpyDeviceToHost, c.stream));
-        HIPCHK(hipStreamSynchronize(c.stream));
-        if (info) info->v_max = GAMMA0 * std::sqrt(c.h_ctl->vmax2);
-        if (c.opt.profile) c.flush_stats();
-        return FG_OK;
+        return finish_solve(x, dt, info);
     });
 }
 

@@ -63,6 +63,7 @@ struct KrylovCtl {
     int status;      // FG_ITER_*
     int done;
     int max_iter;
+    int refinements;  // FP64 refinement sweeps of the mixed-precision solve
 };
 
 // small vector helpers

@@ -53,6 +53,11 @@ struct Ctx {
     double *inst_bn2 = nullptr;          // batch only: ||b_k||^2 per instance
     int32_t *d_node_inst = nullptr;      // batch only: instance of each node
     longlong4 *d_inst_src = nullptr;     // batch only: tetrahedron and triangle source ranges
+    int32_t *d_inst_order = nullptr;     // batch only: instances by decreasing node count
+    int2 *d_inst_stat = nullptr;         // batch only (FG_BATCH_PERSISTENT): status, iterations
+    double2 *d_inst_resn = nullptr;      //   and (residual, ||b_k||) per instance, pinned mirrors
+    int2 *h_inst_stat = nullptr;
+    double2 *h_inst_resn = nullptr;
     double charge_unit = 1.0;            // typical |charge| (Ms_max w^(2/3)), scales the FP32 FMM pass
     std::vector<int64_t> h_rowptr;       // stencil CSR (unpadded), for fg_get_system
     std::vector<int32_t> h_col;
@@ -74,6 +79,14 @@ struct Ctx {
     double4 *w = nullptr;     // lifted 3-D vector fed to the stencil
     KrylovCtl *ctl = nullptr;
     KrylovCtl *h_ctl = nullptr;  // pinned host mirror
+    // mixed-precision Krylov (FG_KRYLOV_MIXED): FP32 stencil values, per-step basis, diagonal
+    // blocks and preconditioner, the FP32 loop's vectors and its control block
+    float *svalf = nullptr;
+    float4 *ep4f = nullptr, *eq4f = nullptr, *Dgf = nullptr, *wf = nullptr;
+    float2 *dprecf = nullptr, *rf = nullptr, *rtf = nullptr, *pf = nullptr, *phf = nullptr, *vf = nullptr,
+           *sf = nullptr, *shf = nullptr, *tf = nullptr, *df = nullptr;
+    KrylovCtl *ic = nullptr;
+    KrylovCtl *h_ic = nullptr;
     double *part = nullptr;      // [8][MAX_PART] partial sums
     double *h_part = nullptr;    // pinned host mirror of small results
 
@@ -88,8 +101,9 @@ struct Ctx {
     double *geo = nullptr;    // SoA [17][Tm]: da (12, row-major 4x3), w (5)
     int32_t *tet_reg = nullptr;
     DevRegion *regions = nullptr;
-    double4 *elem_out = nullptr;  // [4 Tm + 3 Fk] (D contribution, BE xyz)
-    int32_t *inc_ptr = nullptr, *inc_idx = nullptr;  // node -> elem_out entries
+    double4 *elem_out = nullptr;  // [4 Tm + 3 Fk] (D contribution, BE xyz), in node order
+    int32_t *inc_ptr = nullptr;   // node i's contributions: elem_out[inc_ptr[i] .. inc_ptr[i+1])
+    int32_t *epos = nullptr;      // element output entry (a Tm + t, 4 Tm + a Fk + f) -> its slot
 
     // surface anisotropy triangles
     int4 *trik_nodes = nullptr;
@@ -159,6 +173,14 @@ void launch_basis(Ctx &c, double r);
 void launch_node_assemble(Ctx &c, double r, double c_stencil, double dt);
 void launch_r0(Ctx &c, double c_stencil);
 void launch_iteration(Ctx &c, double c_stencil);
+// mixed-precision Krylov: FP32 copies and count reset; FP64 residual and outer test; one FP32
+// iteration; x += d
+void launch_mixed_start(Ctx &c);
+void launch_mixed_residual(Ctx &c, double c_stencil, double tol_in, int max_inner);
+void launch_inner_iteration(Ctx &c, double c_stencil);
+void launch_refine(Ctx &c);
+// FG_BATCH_PERSISTENT: every instance's bicg_dir in one launch (one block per instance)
+void launch_batch_solve(Ctx &c, double c_stencil, int block);
 void launch_update(Ctx &c, double dt);
 void launch_spmv_debug(Ctx &c, double c_stencil, const double2 *xin, double2 *yout);
 void launch_charges(Ctx &c);

@@ -968,7 +968,9 @@ __device__ __forceinline__ void xlate_tile(int n, const int32_t *__restrict__ ce
     using V = typename MF::V;
     constexpr int NR = L::NR, NT = L::NT, NKS = L::NKS, kb = L::kb(K, R), ke = L::ke(K, R);
     const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
-    const int s = (blockIdx.x / NT) & 1, ct = blockIdx.x / (2 * NT);
+    // the row tiles and charge sets of a column tile on one XCD: they read the same children
+    const int b = xcd_block(blockIdx.x, gridDim.x);
+    const int s = (b / NT) & 1, ct = b / (2 * NT);
     const int col = l & 15, kq = l >> 4;
     const int pi = ct * 16 + col;
     const bool act = pi < n;
@@ -1049,7 +1051,7 @@ __global__ __launch_bounds__(256) void k_fmm_xlate(int n, const int32_t *__restr
 {
     constexpr int NT = XL<P>::NT;
     __shared__ typename Mfma<F>::V red[3][64];  // partial tiles of wavefronts 1..3 (M2M)
-    const int R = blockIdx.x % NT;
+    const int R = xcd_block(blockIdx.x, gridDim.x) % NT;
     xlate_tile_dispatch<P, K, 0, F>(R, n, cells, child, Top, X, red);
 }
 

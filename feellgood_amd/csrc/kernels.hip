@@ -4,7 +4,9 @@
 //   node state     NodeRec[N] x 2 (CURRENT / NEXT; evolution copies NEXT)      64 B per node
 //   basis          ep4[N], eq4[N] (double4)
 //   tet geometry   SoA geo[17][Tm] (da 4x3 row-major, w[5]); nodes int4[Tm]
-//   element output elem_out[4 Tm (+3 Fk)] double4 = (alpha_eff-weighted mass, BE x, y, z)
+//   element output elem_out[4 Tm (+3 Fk)] double4 = (alpha_eff-weighted mass, BE x, y, z), stored in
+//                  node order (each node's contributions contiguous, in mesh order of their
+//                  elements): the element pass scatters its writes so the node pass reads runs
 //   stiffness S'   SELL-64 (col int32, val f64), S'_ij = sum_t Abis_t sum_g w_g (da_a . da_b)
 //   Krylov vectors double2[N] (the two tangent unknowns of a node side by side)
 //
@@ -99,7 +101,7 @@ __global__ __launch_bounds__(BLOCK) void k_element(int Tm, const int4 *__restric
                                                    const NodeRec *__restrict__ cur, double Hx,
                                                    double Hy, double Hz, const double *__restrict__ hsp,
                                                    double amp, const double *__restrict__ hst, double dt,
-                                                   double4 *__restrict__ out)
+                                                   const int32_t *__restrict__ epos, double4 *__restrict__ out)
 {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= Tm) return;
@@ -194,14 +196,14 @@ __global__ __launch_bounds__(BLOCK) void k_element(int Tm, const int4 *__restric
         }
     }
 #pragma unroll
-    for (int a = 0; a < 4; a++) out[(size_t)a * Tm + t] = make_double4(contrib[a], BE[a].x, BE[a].y, BE[a].z);
+    for (int a = 0; a < 4; a++) out[epos[(size_t)a * Tm + t]] = make_double4(contrib[a], BE[a].x, BE[a].y, BE[a].z);
 }
 
 // Tri::integrales (src/triangle.cpp:6-36): surface anisotropy right-hand side, 3-D form.
 __global__ __launch_bounds__(BLOCK) void k_tri_aniso(int Fk, const int4 *__restrict__ tn,
                                                      const double *__restrict__ tg,
                                                      const NodeRec *__restrict__ cur,
-                                                     double4 *__restrict__ out)
+                                                     const int32_t *__restrict__ epos, double4 *__restrict__ out)
 {
     const int f = blockIdx.x * blockDim.x + threadIdx.x;
     if (f >= Fk) return;
@@ -220,7 +222,7 @@ __global__ __launch_bounds__(BLOCK) void k_tri_aniso(int Fk, const int4 *__restr
         const double pre = tg[(size_t)g * Fk + f] * Kbis * dot(uk, U);
         for (int a = 0; a < 3; a++) BE[a] = BE[a] + (pre * tri_a(a, g)) * uk;
     }
-    for (int a = 0; a < 3; a++) out[(size_t)a * Fk + f] = make_double4(0.0, BE[a].x, BE[a].y, BE[a].z);
+    for (int a = 0; a < 3; a++) out[epos[(size_t)a * Fk + f]] = make_double4(0.0, BE[a].x, BE[a].y, BE[a].z);
 }
 
 // Node::setBasis (src/node.h:73-102)
@@ -260,7 +262,7 @@ __global__ __launch_bounds__(BLOCK) void k_basis(int N, double cr, double sr, co
 __global__ __launch_bounds__(BLOCK) void k_node_assemble(
     int N, double cr, double sr, double cst, const NodeRec *__restrict__ cur,
     const NodeRec *__restrict__ vsrc, const int32_t *__restrict__ mag,
-    const int32_t *__restrict__ inc_ptr, const int32_t *__restrict__ inc_idx,
+    const int32_t *__restrict__ inc_ptr,
     const double4 *__restrict__ eout, const double *__restrict__ mass,
     const double *__restrict__ sdiag, double4 *__restrict__ ep4, double4 *__restrict__ eq4,
     double4 *__restrict__ Dg, double2 *__restrict__ dprec, double2 *__restrict__ b,
@@ -285,7 +287,7 @@ __global__ __launch_bounds__(BLOCK) void k_node_assemble(
         double D = 0;
         d3 F = mk(0, 0, 0);
         for (int k = inc_ptr[i]; k < inc_ptr[i + 1]; k++) {
-            const double4 e = eout[inc_idx[k]];
+            const double4 e = eout[k];
             D += e.x;
             F = F + mk(e.y, e.z, e.w);
         }
@@ -625,6 +627,547 @@ __global__ __launch_bounds__(BLOCK) void k_iter_x(int N, int nb, KrylovCtl *ctl,
         const int nit = ctl->nit + 1;
         ctl->nit = nit;
         if (nit >= ctl->max_iter) ctl->status = 1;
+    }
+}
+
+// ------------------------------------------------------------------ batched instances, one launch
+// FG_BATCH_PERSISTENT: one block per instance runs that instance's whole bicg_dir
+// (src/algebra/bicg.h:162-234) in a single launch for the batch, with block-level reductions (a
+// fixed tree), its own criterion ||r_k|| <= tol ||b_k|| and no host read or grid-wide step between
+// iterations: the stencil never crosses instances, so a block needs only its own rows and their
+// lifted vectors. The phases and scalars are those of the grid kernels above. Blocks take the
+// instances largest first (order[]), so the long solves start early.
+
+// sums of NV values over the block (fixed tree), the same in every thread
+template <int NV, int BS>
+__device__ inline void bsum(double (&v)[NV], double (*sh)[BS / 64])
+{
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < NV; k++) {
+        const double r = wave_sum(v[k]);
+        if (lane == 0) sh[k][wid] = r;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < NV; k++) {
+        double s = sh[k][0];
+        for (int q = 1; q < BS / 64; q++) s += sh[k][q];
+        v[k] = s;
+    }
+    __syncthreads();  // sh is free again
+}
+
+template <int BS>
+__global__ __launch_bounds__(BS) void k_bicg_batch(const int32_t *__restrict__ order,
+                                                   const int64_t *__restrict__ off, double cst, double tol,
+                                                   int max_iter, STENCIL_ARGS, const double2 *__restrict__ dprec,
+                                                   const double2 *__restrict__ b, double2 *__restrict__ x,
+                                                   double2 *__restrict__ r, double2 *__restrict__ rt,
+                                                   double2 *__restrict__ p, double2 *__restrict__ ph,
+                                                   double2 *__restrict__ v, double2 *__restrict__ s,
+                                                   double2 *__restrict__ sh, double2 *__restrict__ t,
+                                                   double4 *__restrict__ w, int2 *__restrict__ stat,
+                                                   double2 *__restrict__ resn)
+{
+    __shared__ double red[2][BS / 64];
+    const int k = order[blockIdx.x];
+    const int n0 = (int)off[k], n1 = (int)off[k + 1];
+    // r = b - A x0 (w = lift(x0) from the assembly or the guess), rt = p = r
+    double a2[2] = {0, 0};
+    for (int i = n0 + threadIdx.x; i < n1; i += BS) {
+        double2 ri = make_double2(0, 0);
+        const double2 bi = b[i];
+        if (mag[i]) {
+            const double2 y = stencil_row(i, cst, sptr, swid, scol, sval, w, ep4, eq4, Dg, x[i]);
+            ri = make_double2(bi.x - y.x, bi.y - y.y);
+        }
+        r[i] = ri;
+        rt[i] = ri;
+        p[i] = ri;
+        a2[0] += ri.x * ri.x + ri.y * ri.y;
+        a2[1] += bi.x * bi.x + bi.y * bi.y;
+    }
+    bsum<2, BS>(a2, red);
+    const double rhsn = sqrt(fabs(a2[1]));
+    double res = sqrt(fabs(a2[0])), rho1 = a2[0], rho2 = 0, alpha = 0, omega = 0, beta = 0;
+    int nit = 0, status = -1;
+    while (true) {
+        // loop head: the test on ||r||, then rho and beta (src/algebra/bicg.h:180-199)
+        if (isnan(res)) {
+            status = FG_ITER_CANNOT_CONVERGE;
+            break;
+        }
+        if (res <= rhsn * tol) {
+            status = FG_ITER_CONVERGED;
+            break;
+        }
+        if (nit > 0) {
+            if (rho2 == 0 || omega == 0) {
+                status = FG_ITER_CANNOT_CONVERGE;
+                break;
+            }
+            beta = (rho1 / rho2) * (alpha / omega);
+        }
+        for (int i = n0 + threadIdx.x; i < n1; i += BS) {
+            double2 pi;
+            const double2 ri = r[i];
+            if (nit > 0) {
+                const double2 vi = v[i];
+                pi = p[i];
+                pi.x = (pi.x - omega * vi.x) * beta + ri.x;
+                pi.y = (pi.y - omega * vi.y) * beta + ri.y;
+            } else {
+                pi = ri;
+            }
+            p[i] = pi;
+            const double2 d = dprec[i];
+            const double2 h = make_double2(d.x * pi.x, d.y * pi.y);
+            ph[i] = h;
+            const double4 e = ep4[i], q = eq4[i];
+            w[i] = make_double4(h.x * e.x + h.y * q.x, h.x * e.y + h.y * q.y, h.x * e.z + h.y * q.z, 0.0);
+        }
+        __syncthreads();  // the instance's w
+        double a1[1] = {0};
+        for (int i = n0 + threadIdx.x; i < n1; i += BS) {
+            double2 y = make_double2(0, 0);
+            if (mag[i]) y = stencil_row(i, cst, sptr, swid, scol, sval, w, ep4, eq4, Dg, ph[i]);
+            v[i] = y;
+            const double2 q = rt[i];
+            a1[0] += y.x * q.x + y.y * q.y;
+        }
+        bsum<1, BS>(a1, red);  // every thread has read w before it is rewritten below
+        alpha = rho1 / a1[0];
+        double a3[1] = {0};
+        for (int i = n0 + threadIdx.x; i < n1; i += BS) {
+            const double2 ri = r[i], vi = v[i];
+            const double2 si = make_double2(ri.x + (-alpha) * vi.x, ri.y + (-alpha) * vi.y);
+            s[i] = si;
+            a3[0] += si.x * si.x + si.y * si.y;
+            const double2 d = dprec[i];
+            const double2 h = make_double2(d.x * si.x, d.y * si.y);
+            sh[i] = h;
+            const double4 e = ep4[i], q = eq4[i];
+            w[i] = make_double4(h.x * e.x + h.y * q.x, h.x * e.y + h.y * q.y, h.x * e.z + h.y * q.z, 0.0);
+        }
+        bsum<1, BS>(a3, red);  // also publishes the instance's w
+        // the test on ||s|| (src/algebra/bicg.h:206-212)
+        const double rs = sqrt(fabs(a3[0]));
+        if (isnan(rs)) {
+            status = FG_ITER_CANNOT_CONVERGE;
+            res = rs;
+            break;
+        }
+        if (rs <= rhsn * tol) {
+            for (int i = n0 + threadIdx.x; i < n1; i += BS) {
+                double2 xi = x[i];
+                const double2 h = ph[i];
+                xi.x += alpha * h.x;
+                xi.y += alpha * h.y;
+                x[i] = xi;
+            }
+            status = FG_ITER_CONVERGED;
+            res = rs;
+            break;
+        }
+        if (status == FG_ITER_OVERFLOW || status == FG_ITER_CANNOT_CONVERGE) {
+            res = rs;
+            break;
+        }
+        double a4[2] = {0, 0};
+        for (int i = n0 + threadIdx.x; i < n1; i += BS) {
+            double2 y = make_double2(0, 0);
+            if (mag[i]) y = stencil_row(i, cst, sptr, swid, scol, sval, w, ep4, eq4, Dg, sh[i]);
+            t[i] = y;
+            const double2 si = s[i];
+            a4[0] += y.x * si.x + y.y * si.y;
+            a4[1] += y.x * y.x + y.y * y.y;
+        }
+        bsum<2, BS>(a4, red);
+        omega = a4[0] / a4[1];
+        double a5[2] = {0, 0};
+        for (int i = n0 + threadIdx.x; i < n1; i += BS) {
+            double2 xi = x[i];
+            const double2 h = ph[i], hs = sh[i], si = s[i], ti = t[i], q = rt[i];
+            xi.x += alpha * h.x;
+            xi.y += alpha * h.y;
+            xi.x += omega * hs.x;
+            xi.y += omega * hs.y;
+            x[i] = xi;
+            const double2 ri = make_double2(si.x - omega * ti.x, si.y - omega * ti.y);
+            r[i] = ri;
+            a5[0] += ri.x * ri.x + ri.y * ri.y;
+            a5[1] += q.x * ri.x + q.y * ri.y;
+        }
+        bsum<2, BS>(a5, red);
+        res = sqrt(fabs(a5[0]));
+        rho2 = rho1;
+        rho1 = a5[1];
+        if (++nit >= max_iter) status = FG_ITER_OVERFLOW;
+    }
+    if (threadIdx.x == 0) {
+        stat[k] = make_int2(status, nit);
+        resn[k] = make_double2(res, rhsn);
+    }
+}
+
+// ------------------------------------------------------------------ mixed-precision Krylov
+// FG_KRYLOV_MIXED: BiCGSTAB on A d = r in FP32 (stencil values, basis, diagonal blocks,
+// preconditioner and vectors in FP32, dot products summed in FP64) inside FP64 iterative
+// refinement: r = b - A x in FP64, the FP32 solve from d = 0 to a loose tolerance, x += d, until
+// the FP64 residual meets tol ||b||. The FP32 loop has its own control block (ic) with the FP64
+// loop's scalars and stopping rules (src/algebra/bicg.h:162-234); the outer block (ctl) keeps
+// ||b||, the tolerance, the status and the counts.
+
+__device__ inline float2 stencil_row_f(int i, float cst, const int64_t *__restrict__ sptr,
+                                       const int32_t *__restrict__ swid, const int32_t *__restrict__ scol,
+                                       const float *__restrict__ sval, const float4 *__restrict__ w,
+                                       const float4 *__restrict__ ep4, const float4 *__restrict__ eq4,
+                                       const float4 *__restrict__ Dg, float2 xi)
+{
+    const int sl = i >> 6, lane = i & 63;
+    const int64_t off = sptr[sl];
+    const int width = swid[sl];
+    float zx = 0, zy = 0, zz = 0;
+    for (int k = 0; k < width; k++) {
+        const int64_t e = off + (int64_t)k * SELL_C + lane;
+        const int j = scol[e];
+        const float sv = sval[e];
+        const float4 wj = w[j];
+        zx += sv * wj.x;
+        zy += sv * wj.y;
+        zz += sv * wj.z;
+    }
+    const float4 p = ep4[i], q = eq4[i], g = Dg[i];
+    float2 y;
+    y.x = cst * (q.x * zx + q.y * zy + q.z * zz) + (g.x * xi.x + g.y * xi.y);
+    y.y = cst * (p.x * zx + p.y * zy + p.z * zz) + (g.z * xi.x + g.w * xi.y);
+    return y;
+}
+
+#define STENCIL_ARGS_F                                                                        \
+    const int64_t *__restrict__ sptr, const int32_t *__restrict__ swid,                       \
+        const int32_t *__restrict__ scol, const float *__restrict__ sval,                     \
+        const float4 *__restrict__ ep4, const float4 *__restrict__ eq4,                       \
+        const float4 *__restrict__ Dg, const int32_t *__restrict__ mag
+#define STENCIL_PASS_F c.slice_ptr, c.slice_w, c.scol, c.svalf, c.ep4f, c.eq4f, c.Dgf, c.mag
+
+__device__ inline float4 lift_f(float2 h, float4 e, float4 q)
+{
+    return make_float4(h.x * e.x + h.y * q.x, h.x * e.y + h.y * q.y, h.x * e.z + h.y * q.z, 0.f);
+}
+
+// FP32 copies of the per-step basis, diagonal blocks and preconditioner
+__global__ __launch_bounds__(BLOCK) void k_to_f32(int N, const double4 *__restrict__ ep4,
+                                                  const double4 *__restrict__ eq4, const double4 *__restrict__ Dg,
+                                                  const double2 *__restrict__ dprec, float4 *__restrict__ ep4f,
+                                                  float4 *__restrict__ eq4f, float4 *__restrict__ Dgf,
+                                                  float2 *__restrict__ dprecf)
+{
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x) {
+        const double4 e = ep4[i], q = eq4[i], g = Dg[i];
+        const double2 d = dprec[i];
+        ep4f[i] = make_float4((float)e.x, (float)e.y, (float)e.z, 0.f);
+        eq4f[i] = make_float4((float)q.x, (float)q.y, (float)q.z, 0.f);
+        Dgf[i] = make_float4((float)g.x, (float)g.y, (float)g.z, (float)g.w);
+        dprecf[i] = make_float2((float)d.x, (float)d.y);
+    }
+}
+
+// the outer loop's start (||b||, or the batch's criterion, is set by the assembly): counts and
+// status reset
+__global__ void k_mixed_init(KrylovCtl *ctl)
+{
+    if (threadIdx.x == 0) {
+        ctl->nit = 0;
+        ctl->status = -1;
+        ctl->done = 0;
+        ctl->refinements = 0;
+        ctl->res = 0;
+    }
+}
+
+// FP64 residual r = b - A x (masked, with w = lift(x)); r in FP32 as the inner right-hand side,
+// rt = p = r, d = 0; partials: slot 0 the FP64 ||r||^2, slots 1 and 2 the FP32 vector's (the inner
+// loop's first ||r|| and rt . r)
+__global__ __launch_bounds__(BLOCK) void k_resid(int N, double cst, STENCIL_ARGS, const KrylovCtl *ctl,
+                                                 const double4 *__restrict__ w, const double2 *__restrict__ x,
+                                                 const double2 *__restrict__ b, float2 *__restrict__ rf,
+                                                 float2 *__restrict__ rtf, float2 *__restrict__ pf,
+                                                 float2 *__restrict__ df, double *__restrict__ part)
+{
+    if (ctl->done) return;
+    double rr = 0, ff = 0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x) {
+        double2 ri = make_double2(0, 0);
+        if (mag[i]) {
+            const double2 y = stencil_row(i, cst, sptr, swid, scol, sval, w, ep4, eq4, Dg, x[i]);
+            const double2 bi = b[i];
+            ri = make_double2(bi.x - y.x, bi.y - y.y);
+        }
+        rr += ri.x * ri.x + ri.y * ri.y;
+        const float2 f = make_float2((float)ri.x, (float)ri.y);
+        rf[i] = f;
+        rtf[i] = f;
+        pf[i] = f;
+        df[i] = make_float2(0.f, 0.f);
+        ff += (double)f.x * f.x + (double)f.y * f.y;
+    }
+    const double vals[3] = {rr, ff, ff};
+    const int slots[3] = {0, 1, 2};
+    block_store<3>(vals, slots, part);
+}
+
+// The outer test on the FP64 residual (one block): converged, NaN or out of iterations ends both
+// loops; otherwise the inner loop starts from d = 0 with ||r|| as its scale and tolerance
+// tol_in, or what the outer tolerance still asks if that is looser.
+__global__ void k_outer(int nb, KrylovCtl *ctl, KrylovCtl *ic, double tol_in, int max_inner,
+                        const double *__restrict__ part)
+{
+    if (ctl->done) return;
+    const int slots[1] = {0};
+    double v[1];
+    block_load<1>(part, slots, nb, v);
+    if (threadIdx.x) return;
+    const double res = sqrt(fabs(v[0]));
+    ctl->res = res;
+    int st = -1;
+    if (isnan(res)) st = FG_ITER_CANNOT_CONVERGE;
+    else if (res <= ctl->rhsn * ctl->tol) st = FG_ITER_CONVERGED;
+    else if (ctl->nit >= ctl->max_iter) st = FG_ITER_OVERFLOW;
+    if (st >= 0) {
+        ctl->status = st;
+        ctl->done = 1;
+        ic->done = 1;
+        return;
+    }
+    ic->rhsn = res;
+    ic->tol = fmax(tol_in, 0.5 * ctl->rhsn * ctl->tol / res);
+    ic->max_iter = min(max_inner, ctl->max_iter - ctl->nit);
+    ic->nit = 0;
+    ic->status = -1;
+    ic->done = 0;
+    ic->rho1 = ic->rho2 = ic->alpha = ic->omega = ic->beta = 0.0;
+    ctl->refinements++;
+}
+
+// x += d, w = lift(x) (FP64); the inner iterations added to the outer count
+__global__ __launch_bounds__(BLOCK) void k_refine(int N, KrylovCtl *ctl, const KrylovCtl *ic,
+                                                  const double4 *__restrict__ ep4, const double4 *__restrict__ eq4,
+                                                  const float2 *__restrict__ df, double2 *__restrict__ x,
+                                                  double4 *__restrict__ w)
+{
+    if (ctl->done) return;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x) {
+        double2 xi = x[i];
+        const float2 d = df[i];
+        xi.x += (double)d.x;
+        xi.y += (double)d.y;
+        x[i] = xi;
+        const double4 e = ep4[i], q = eq4[i];
+        w[i] = make_double4(xi.x * e.x + xi.y * q.x, xi.x * e.y + xi.y * q.y, xi.x * e.z + xi.y * q.z, 0.0);
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) ctl->nit += ic->nit;
+}
+
+// The FP32 loop: the same five phases as the FP64 one, on ic
+__global__ __launch_bounds__(BLOCK) void kf_iter_p(int N, int nb, KrylovCtl *ic, const float4 *__restrict__ ep4,
+                                                   const float4 *__restrict__ eq4, const float2 *__restrict__ dprec,
+                                                   const float2 *__restrict__ r, const float2 *__restrict__ v,
+                                                   float2 *__restrict__ p, float2 *__restrict__ ph,
+                                                   float4 *__restrict__ w, const double *__restrict__ part)
+{
+    if (ic->done) return;
+    const int slots[2] = {1, 2};
+    double red[2];
+    block_load<2>(part, slots, nb, red);
+    const double res = sqrt(fabs(red[0]));
+    const double rho1 = red[1];
+    const int nit = ic->nit;
+    int status = ic->status, done = 0;
+    if (isnan(res)) {
+        status = 2;
+        done = 1;
+    } else if (res <= ic->rhsn * ic->tol) {
+        status = 0;
+        done = 1;
+    }
+    double beta = 0;
+    const double omega = ic->omega;
+    if (!done && nit > 0) {
+        if (ic->rho2 == 0 || omega == 0) {
+            status = 2;
+            done = 1;
+        } else {
+            beta = (rho1 / ic->rho2) * (ic->alpha / omega);
+        }
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        ic->res = res;
+        ic->status = status;
+        ic->rho1 = rho1;
+        ic->beta = beta;
+        if (done) ic->done = 1;
+    }
+    if (done) return;
+    const float om = (float)omega, be = (float)beta;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x) {
+        float2 pi;
+        const float2 ri = r[i];
+        if (nit > 0) {
+            const float2 vi = v[i];
+            pi = p[i];
+            pi.x = (pi.x - om * vi.x) * be + ri.x;
+            pi.y = (pi.y - om * vi.y) * be + ri.y;
+        } else {
+            pi = ri;
+        }
+        p[i] = pi;
+        const float2 d = dprec[i];
+        const float2 h = make_float2(d.x * pi.x, d.y * pi.y);
+        ph[i] = h;
+        w[i] = lift_f(h, ep4[i], eq4[i]);
+    }
+}
+
+__global__ __launch_bounds__(BLOCK) void kf_spmv_v(int N, float cst, STENCIL_ARGS_F, const KrylovCtl *ic,
+                                                   const float4 *__restrict__ w, const float2 *__restrict__ ph,
+                                                   const float2 *__restrict__ rt, float2 *__restrict__ v,
+                                                   double *__restrict__ part)
+{
+    if (ic->done) return;
+    double vr = 0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x) {
+        float2 y = make_float2(0, 0);
+        if (mag[i]) y = stencil_row_f(i, cst, sptr, swid, scol, sval, w, ep4, eq4, Dg, ph[i]);
+        v[i] = y;
+        const float2 q = rt[i];
+        vr += (double)y.x * q.x + (double)y.y * q.y;
+    }
+    const double vals[1] = {vr};
+    const int slots[1] = {3};
+    block_store<1>(vals, slots, part);
+}
+
+__global__ __launch_bounds__(BLOCK) void kf_iter_s(int N, int nb, KrylovCtl *ic, const float4 *__restrict__ ep4,
+                                                   const float4 *__restrict__ eq4, const float2 *__restrict__ dprec,
+                                                   const float2 *__restrict__ r, const float2 *__restrict__ v,
+                                                   float2 *__restrict__ s, float2 *__restrict__ sh,
+                                                   float4 *__restrict__ w, double *__restrict__ part)
+{
+    if (ic->done) return;
+    const int slots[1] = {3};
+    double red[1];
+    block_load<1>(part, slots, nb, red);
+    const double alpha = ic->rho1 / red[0];
+    if (blockIdx.x == 0 && threadIdx.x == 0) ic->alpha = alpha;
+    const float al = (float)alpha;
+    double ss = 0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x) {
+        const float2 ri = r[i], vi = v[i];
+        const float2 si = make_float2(ri.x + (-al) * vi.x, ri.y + (-al) * vi.y);
+        s[i] = si;
+        ss += (double)si.x * si.x + (double)si.y * si.y;
+        const float2 d = dprec[i];
+        const float2 h = make_float2(d.x * si.x, d.y * si.y);
+        sh[i] = h;
+        w[i] = lift_f(h, ep4[i], eq4[i]);
+    }
+    const double vals[1] = {ss};
+    const int slots2[1] = {4};
+    block_store<1>(vals, slots2, part);
+}
+
+__global__ __launch_bounds__(BLOCK) void kf_spmv_t(int N, int nb, float cst, STENCIL_ARGS_F, KrylovCtl *ic,
+                                                   const float4 *__restrict__ w, const float2 *__restrict__ sh,
+                                                   const float2 *__restrict__ s, const float2 *__restrict__ ph,
+                                                   float2 *__restrict__ x, float2 *__restrict__ t,
+                                                   double *__restrict__ part)
+{
+    if (ic->done) return;
+    const int slots[1] = {4};
+    double red[1];
+    block_load<1>(part, slots, nb, red);
+    const double res = sqrt(fabs(red[0]));
+    int status = ic->status, done = 0;
+    bool conv = false;
+    if (isnan(res)) {
+        status = 2;
+        done = 1;
+    } else if (res <= ic->rhsn * ic->tol) {
+        status = 0;
+        done = 1;
+        conv = true;
+    } else if (status == 1 || status == 2) {
+        done = 1;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0 && done) {
+        ic->res = res;
+        ic->status = status;
+        ic->done = 1;
+    }
+    if (done) {
+        if (conv) {
+            const float al = (float)ic->alpha;
+            for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x) {
+                float2 xi = x[i];
+                const float2 h = ph[i];
+                xi.x += al * h.x;
+                xi.y += al * h.y;
+                x[i] = xi;
+            }
+        }
+        return;
+    }
+    double ts = 0, tt = 0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x) {
+        float2 y = make_float2(0, 0);
+        if (mag[i]) y = stencil_row_f(i, cst, sptr, swid, scol, sval, w, ep4, eq4, Dg, sh[i]);
+        t[i] = y;
+        const float2 si = s[i];
+        ts += (double)y.x * si.x + (double)y.y * si.y;
+        tt += (double)y.x * y.x + (double)y.y * y.y;
+    }
+    const double vals[2] = {ts, tt};
+    const int slots2[2] = {5, 6};
+    block_store<2>(vals, slots2, part);
+}
+
+__global__ __launch_bounds__(BLOCK) void kf_iter_x(int N, int nb, KrylovCtl *ic, const float2 *__restrict__ ph,
+                                                   const float2 *__restrict__ sh, const float2 *__restrict__ s,
+                                                   const float2 *__restrict__ t, const float2 *__restrict__ rt,
+                                                   float2 *__restrict__ x, float2 *__restrict__ r,
+                                                   double *__restrict__ part)
+{
+    if (ic->done) return;
+    const int slots[2] = {5, 6};
+    double red[2];
+    block_load<2>(part, slots, nb, red);
+    const double omega = red[0] / red[1];
+    const float al = (float)ic->alpha, om = (float)omega;
+    double rr = 0, rtr = 0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x) {
+        float2 xi = x[i];
+        const float2 h = ph[i], hs = sh[i], si = s[i], ti = t[i], q = rt[i];
+        xi.x += al * h.x;
+        xi.y += al * h.y;
+        xi.x += om * hs.x;
+        xi.y += om * hs.y;
+        x[i] = xi;
+        const float2 ri = make_float2(si.x - om * ti.x, si.y - om * ti.y);
+        r[i] = ri;
+        rr += (double)ri.x * ri.x + (double)ri.y * ri.y;
+        rtr += (double)q.x * ri.x + (double)q.y * ri.y;
+    }
+    const double vals[2] = {rr, rtr};
+    const int slots2[2] = {1, 2};
+    block_store<2>(vals, slots2, part);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        ic->omega = omega;
+        ic->rho2 = ic->rho1;
+        const int nit = ic->nit + 1;
+        ic->nit = nit;
+        if (nit >= ic->max_iter) ic->status = 1;
     }
 }
 
@@ -1127,12 +1670,12 @@ void launch_element(Ctx &c, const double Hext[3], double dt, const double *hsp, 
     if (c.Tm > 0) {
         Prof pr(c, "element");
         k_element<<<grid1(c.Tm), BLOCK, 0, c.stream>>>(c.Tm, c.tet_nodes, c.geo, c.tet_reg, c.regions, cur, Hext[0],
-                                                       Hext[1], Hext[2], hsp, amp, c.hst, dt, c.elem_out);
+                                                       Hext[1], Hext[2], hsp, amp, c.hst, dt, c.epos, c.elem_out);
     }
     if (c.Fk > 0) {
         Prof pr(c, "tri_aniso");
         k_tri_aniso<<<grid1(c.Fk), BLOCK, 0, c.stream>>>(c.Fk, c.trik_nodes, c.trik_geo, cur,
-                                                         c.elem_out + (size_t)4 * c.Tm);
+                                                         c.epos + (size_t)4 * c.Tm, c.elem_out);
     }
 }
 
@@ -1143,7 +1686,7 @@ void launch_node_assemble(Ctx &c, double r, double cst, double /*dt*/)
     {
         Prof pr(c, "node_assemble");
         k_node_assemble<<<nb, BLOCK, 0, c.stream>>>(c.N, cos(r), sin(r), cst, c.rec[c.cur], vsrc, c.mag, c.inc_ptr,
-                                                    c.inc_idx, c.elem_out, c.mass, c.sdiag, c.ep4, c.eq4, c.Dg,
+                                                    c.elem_out, c.mass, c.sdiag, c.ep4, c.eq4, c.Dg,
                                                     c.dprec, c.b, c.x, c.w, c.part);
     }
     k_set_rhsn<<<1, 64, 0, c.stream>>>(nb, c.ctl, c.part);
@@ -1189,6 +1732,75 @@ void launch_iteration(Ctx &c, double cst)
         Prof pr(c, "iter_x");
         k_iter_x<<<nb, BLOCK, 0, c.stream>>>(c.N, nb, c.ctl, c.ph, c.sh, c.s, c.t, c.rt, c.x, c.r, c.part);
     }
+}
+
+void launch_mixed_start(Ctx &c)
+{
+    const int nb = c.blocks_for(c.N);
+    {
+        Prof pr(c, "to_f32");
+        k_to_f32<<<nb, BLOCK, 0, c.stream>>>(c.N, c.ep4, c.eq4, c.Dg, c.dprec, c.ep4f, c.eq4f, c.Dgf, c.dprecf);
+    }
+    k_mixed_init<<<1, 64, 0, c.stream>>>(c.ctl);
+}
+
+void launch_mixed_residual(Ctx &c, double cst, double tol_in, int max_inner)
+{
+    const int nb = c.blocks_for(c.N);
+    {
+        Prof pr(c, "spmv_r0");
+        k_resid<<<nb, BLOCK, 0, c.stream>>>(c.N, cst, STENCIL_PASS, c.ctl, c.w, c.x, c.b, c.rf, c.rtf, c.pf, c.df,
+                                            c.part);
+    }
+    k_outer<<<1, 64, 0, c.stream>>>(nb, c.ctl, c.ic, tol_in, max_inner, c.part);
+}
+
+void launch_inner_iteration(Ctx &c, double cst)
+{
+    const int nb = c.blocks_for(c.N);
+    const float cf = (float)cst;
+    {
+        Prof pr(c, "iter_p");
+        kf_iter_p<<<nb, BLOCK, 0, c.stream>>>(c.N, nb, c.ic, c.ep4f, c.eq4f, c.dprecf, c.rf, c.vf, c.pf, c.phf, c.wf,
+                                              c.part);
+    }
+    {
+        Prof pr(c, "spmv");
+        kf_spmv_v<<<nb, BLOCK, 0, c.stream>>>(c.N, cf, STENCIL_PASS_F, c.ic, c.wf, c.phf, c.rtf, c.vf, c.part);
+    }
+    {
+        Prof pr(c, "iter_s");
+        kf_iter_s<<<nb, BLOCK, 0, c.stream>>>(c.N, nb, c.ic, c.ep4f, c.eq4f, c.dprecf, c.rf, c.vf, c.sf, c.shf, c.wf,
+                                              c.part);
+    }
+    {
+        Prof pr(c, "spmv");
+        kf_spmv_t<<<nb, BLOCK, 0, c.stream>>>(c.N, nb, cf, STENCIL_PASS_F, c.ic, c.wf, c.shf, c.sf, c.phf, c.df, c.tf,
+                                              c.part);
+    }
+    {
+        Prof pr(c, "iter_x");
+        kf_iter_x<<<nb, BLOCK, 0, c.stream>>>(c.N, nb, c.ic, c.phf, c.shf, c.sf, c.tf, c.rtf, c.df, c.rf, c.part);
+    }
+}
+
+void launch_batch_solve(Ctx &c, double cst, int block)
+{
+    const int K = (int)c.inst_node_off.size() - 1;
+    Prof pr(c, "bicg_batch");
+#define FG_BATCH_ARGS                                                                                         \
+    c.d_inst_order, c.d_inst_node_off, cst, c.opt.tol, c.opt.max_iter, STENCIL_PASS, c.dprec, c.b, c.x, c.r, c.rt, \
+        c.p, c.ph, c.v, c.s, c.sh, c.t, c.w, c.d_inst_stat, c.d_inst_resn
+    if (block == 1024) k_bicg_batch<1024><<<K, 1024, 0, c.stream>>>(FG_BATCH_ARGS);
+    else if (block == 512) k_bicg_batch<512><<<K, 512, 0, c.stream>>>(FG_BATCH_ARGS);
+    else k_bicg_batch<256><<<K, 256, 0, c.stream>>>(FG_BATCH_ARGS);
+#undef FG_BATCH_ARGS
+}
+
+void launch_refine(Ctx &c)
+{
+    Prof pr(c, "refine");
+    k_refine<<<c.blocks_for(c.N), BLOCK, 0, c.stream>>>(c.N, c.ctl, c.ic, c.ep4, c.eq4, c.df, c.x, c.w);
 }
 
 void launch_update(Ctx &c, double dt)

@@ -15,7 +15,8 @@ class EngineError(RuntimeError):
 
 
 def options(tol=1e-6, max_iter=700, demag="direct", fmm_order=8, fmm_leaf=64, precision="fp64",
-            device=0, profile=False, fmm_far_order=0):
+            device=0, profile=False, fmm_far_order=0, krylov="fp64", krylov_inner_tol=0.0, batch_solver="union",
+            batch_block=0):
     o = nv.Options()
     nv.hip_lib().fg_default_options(C.byref(o))
     o.tol = tol
@@ -27,6 +28,10 @@ def options(tol=1e-6, max_iter=700, demag="direct", fmm_order=8, fmm_leaf=64, pr
     o.device = device
     o.profile = 1 if profile else 0
     o.fmm_far_order = fmm_far_order
+    o.krylov = {"fp64": nv.FG_KRYLOV_FP64, "mixed": nv.FG_KRYLOV_MIXED}[krylov]
+    o.krylov_inner_tol = krylov_inner_tol
+    o.batch_solver = {"union": nv.FG_BATCH_UNION, "persistent": nv.FG_BATCH_PERSISTENT}[batch_solver]
+    o.batch_block = batch_block
     return o
 
 

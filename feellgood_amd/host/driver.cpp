@@ -120,6 +120,7 @@ struct Config {
     double tol = 1e-6;
     double du_max = 0.02, dt_min = 1e-16, dt_max = 5e-13;
     int demag = FG_DEMAG_FMM, fmm_order = 8, fmm_leaf = 64, precision = FG_PREC_FP64, device = 0;
+    int krylov = FG_KRYLOV_FP64;
 };
 
 // Three expressions of a settings key (numbers are expressions too). Returns whether any depends
@@ -405,6 +406,12 @@ Config read_config(const Node &root)
             else if (v == "mixed") c.precision = FG_PREC_MIXED;
             else throw SettingsError("gpu.precision should be fp64 or mixed");
         }
+        if (auto n = s->get("krylov"); n && !n->is_null()) {
+            const std::string v = fgh::as_string(*n, "gpu.krylov");
+            if (v == "fp64") c.krylov = FG_KRYLOV_FP64;
+            else if (v == "mixed") c.krylov = FG_KRYLOV_MIXED;
+            else throw SettingsError("gpu.krylov should be fp64 or mixed");
+        }
         if (auto n = s->get("device"); n && !n->is_null()) c.device = (int)fgh::as_long(*n, "gpu.device");
     }
     for (auto &col : c.cols) {
@@ -568,6 +575,7 @@ int run(const Config &c, unsigned seed, bool verbose, bool check_only, const std
     opt.fmm_order = c.fmm_order;
     opt.fmm_leaf = c.fmm_leaf;
     opt.precision = c.precision;
+    opt.krylov = c.krylov;
     opt.device = c.device;
     if (fg_create(&view, &opt, &e.ctx) != FG_OK) throw std::runtime_error(std::string("fg_create: ") + fg_last_error(nullptr));
 

@@ -22,6 +22,8 @@ FG_ITER_UNDEFINED, FG_ITER_CONVERGED, FG_ITER_OVERFLOW, FG_ITER_CANNOT_CONVERGE 
 FG_CURRENT, FG_NEXT, FG_BOTH = 0, 1, 2
 FG_DEMAG_DIRECT, FG_DEMAG_FMM = 0, 1
 FG_PREC_FP64, FG_PREC_MIXED = 0, 1
+FG_KRYLOV_FP64, FG_KRYLOV_MIXED = 0, 1
+FG_BATCH_UNION, FG_BATCH_PERSISTENT = 0, 1
 ITER_STATUS = {-1: "UNDEFINED", 0: "CONVERGED", 1: "ITER_OVERFLOW", 2: "CANNOT_CONVERGE"}
 
 MU0 = 1.25663706127e-6
@@ -60,7 +62,9 @@ class SpinSurface(C.Structure):
 class Options(C.Structure):
     _fields_ = [("tol", C.c_double), ("max_iter", C.c_int32), ("demag_method", C.c_int32),
                 ("fmm_order", C.c_int32), ("fmm_leaf", C.c_int32), ("precision", C.c_int32),
-                ("device", C.c_int32), ("profile", C.c_int32), ("fmm_far_order", C.c_int32)]
+                ("device", C.c_int32), ("profile", C.c_int32), ("fmm_far_order", C.c_int32),
+                ("krylov", C.c_int32), ("krylov_inner_tol", C.c_double), ("batch_solver", C.c_int32),
+                ("batch_block", C.c_int32)]
 
 
 class SolveInfo(C.Structure):

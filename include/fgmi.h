@@ -75,8 +75,25 @@ extern "C" {
 
 /* ---- precision modes ---- */
 #define FG_PREC_FP64 0        /* plain FP64 everywhere */
-#define FG_PREC_MIXED 1       /* FMM demag: FP32 near-field coefficients and FP32 multipole-to-target
-                                 evaluation (FP64 sums); everything else FP64 */
+#define FG_PREC_MIXED 1       /* FMM demag in FP32: near-field coefficients, expansions, the
+                                 translations on the FP32 matrix cores, the multipole-to-target
+                                 evaluation, the far field of the v charges to degree 4 (FP64
+                                 potentials and sums); everything else FP64 */
+
+/* ---- Krylov solve (fg_solve) ---- */
+#define FG_KRYLOV_FP64 0      /* bicg_dir in FP64 (src/algebra/bicg.h:162-234) */
+#define FG_KRYLOV_MIXED 1     /* FP64 iterative refinement around an FP32 bicg_dir: r = b - A x in
+                                 FP64, the FP32 solve of A d = r from d = 0 (FP32 stencil values,
+                                 basis, blocks, preconditioner and vectors, FP64 dot products) to
+                                 the relative target krylov_inner_tol, or what tol still asks if
+                                 that is looser, x += d, until the FP64 residual meets tol ||b|| */
+
+/* ---- Krylov solve of a batch (fg_create_batch) ---- */
+#define FG_BATCH_UNION 0      /* one bicg_dir over the union, ||r|| <= tol min_k ||b_k|| */
+#define FG_BATCH_PERSISTENT 1 /* one launch: a block of batch_block threads per instance runs that
+                                 instance's bicg_dir to ||r_k|| <= tol ||b_k|| with block-level
+                                 reductions and no host read between iterations (FP64; not with
+                                 FG_KRYLOV_MIXED) */
 
 /* Material of a volume region (Tetra::prm, src/tetra.h:226-252). Ms <= 0: non-magnetic. */
 typedef struct fg_volume_region {
@@ -136,6 +153,10 @@ typedef struct fg_options {
                               degree 2, 4, 5, 6 for fmm_order 4, 6, 8, 10 (FMM error up by under
                               5 %); 0 = default: fmm_order in FP64, the lower degree in
                               FG_PREC_MIXED */
+    int32_t krylov;        /* FG_KRYLOV_FP64 (default) or FG_KRYLOV_MIXED */
+    double krylov_inner_tol; /* FG_KRYLOV_MIXED: relative target of each FP32 solve (0 = 3e-7) */
+    int32_t batch_solver;  /* batches: FG_BATCH_UNION (default) or FG_BATCH_PERSISTENT */
+    int32_t batch_block;   /* FG_BATCH_PERSISTENT: threads per instance, 256, 512 or 1024 (0 = 256) */
 } fg_options;
 
 /* Result of fg_solve (iteration<T> state, src/algebra/iter.h:36-179, and LinAlgebra::v_max). */
@@ -145,8 +166,9 @@ typedef struct fg_solve_info {
     double residual;       /* last monitored residual norm */
     double rhs_norm;       /* ||b|| */
     double v_max;          /* gamma0 * max_i |(x_2i, x_2i+1)| [1/s] */
-    int32_t refinements;   /* FP64 refinement sweeps (mixed precision), else 0 */
-    int32_t inner_iterations; /* FP32 inner iterations (mixed precision), else 0 */
+    int32_t refinements;   /* FG_KRYLOV_MIXED: FP32 solves, each followed by an FP64 correction;
+                              else 0 */
+    int32_t inner_iterations; /* FG_KRYLOV_MIXED: FP32 iterations in all (= iterations); else 0 */
 } fg_solve_info;
 
 typedef struct fg_ctx fg_ctx;

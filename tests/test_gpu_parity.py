@@ -381,3 +381,68 @@ def test_ellipsoid_fixture_step(golden_dir):
     un0 = p.get_state(nv.FG_NEXT)[0]
     un1 = g.get_state(nv.FG_NEXT)[0]
     assert np.abs(un1 - un0).max() <= 1e-6
+
+
+def test_mixed_krylov_solve_and_trajectory():
+    """FG_KRYLOV_MIXED (FP32 bicg_dir inside FP64 refinement) on the 5-cell cube: one solve against
+    the oracle's FP64 bicg_dir on the same system (both to tol 1e-12: v within 1e-8, u within 1e-12),
+    the accepted solution's FP64 residual recomputed from the assembled system at most tol ||b||,
+    the refinement counts reported, and a 20-step lock-step run with |dm| <= 1e-6 and energies
+    within 1e-8 of the oracle."""
+    from feellgood_amd.engine import Engine
+    m = cube()
+    u, v = state(m)
+    p = oracle.Problem(m.mesh, tol=1e-12, max_iter=2000)
+    g = Engine(m.mesh, tol=1e-12, max_iter=2000, krylov="mixed")
+    for o in (p, g):
+        o.set_state(nv.FG_BOTH, u=u, v=v)
+        o.calc_demag()
+        o.evolution()
+    H = np.array([3e4, -2e4, 1e4])
+    dt, pref = 2e-14, 1.046052
+    for o in (p, g):
+        o.base_projection(0.13)
+        o.prepare_elements(H, dt, pref)
+    rc0, i0 = p.solve(dt)
+    f1, i1 = g.solve(dt)
+    assert rc0 == 0 and not f1 and i1.status == nv.FG_ITER_CONVERGED
+    assert i1.refinements >= 1 and i1.inner_iterations == i1.iterations > 0
+    print(f"mixed Krylov: {i1.refinements} corrections, {i1.inner_iterations} FP32 iterations "
+          f"(FP64 bicg_dir: {i0.iterations})", flush=True)
+    rp, col, val, b = g.system()
+    x = g.solution()
+    Ax = np.array([val[rp[i]:rp[i + 1]] @ x[col[rp[i]:rp[i + 1]]] for i in range(len(b))])
+    assert np.linalg.norm(b - Ax) <= 1e-12 * np.linalg.norm(b) * 1.01
+    un0, vn0, _, _ = p.get_state(nv.FG_NEXT)
+    un1, vn1, _, _ = g.get_state(nv.FG_NEXT)
+    assert rel(vn1, vn0) < 1e-8 and np.abs(un1 - un0).max() <= 1e-12
+    g.close()
+
+    m = cube(n=4, alpha=0.5, K=0.0, K3=0.0)
+    u, v = state(m, seed=5)
+    v[:] = 0
+    p = oracle.Problem(m.mesh, tol=1e-10, max_iter=2000)
+    g = Engine(m.mesh, tol=1e-10, max_iter=2000, krylov="mixed")
+    for o in (p, g):
+        o.set_state(nv.FG_BOTH, u=u, v=v)
+        o.calc_demag()
+        o.evolution()
+    H = np.array([0.0, 0.0, 2e4])
+    dt = 1e-14
+    pref = 1 + 0.01 * abs(np.log(0.01))
+    angles = oracle.basis_angles(1, 20)
+    for k in range(20):
+        for o in (p, g):
+            o.base_projection(angles[k])
+            o.prepare_elements(H, dt, pref)
+            o.solve(dt)
+            o.calc_demag()
+        E0 = p.energy(H)
+        E1 = g.energy(H)
+        for o in (p, g):
+            o.evolution()
+        u0 = p.get_state(nv.FG_CURRENT)[0]
+        u1 = g.get_state(nv.FG_CURRENT)[0]
+        assert np.abs(u1 - u0).max() <= 1e-6, k
+        assert np.allclose(E1, E0, rtol=1e-8, atol=1e-8 * np.abs(E0).max()), k
+    g.close()
