@@ -39,6 +39,14 @@ FP64_VALU_TFLOPS = 78.6
 FP32_VALU_TFLOPS = 157.3
 
 
+def iters_text(its):
+    """The per-step BiCGSTAB iteration counts, listed when few, else as counts of each value."""
+    if len(its) <= 20:
+        return str(list(its))
+    vals, cnt = np.unique(np.asarray(its), return_counts=True)
+    return "{" + ", ".join(f"{v}: {c} steps" for v, c in zip(vals, cnt)) + "}"
+
+
 def bench_dots(args):
     """Batched nanodots in one context: dot-steps/s and tet-steps/s, mixed precision FMM demag with
     the leaf size picked by the autotuner."""
@@ -479,7 +487,8 @@ def default_guess_weights(w):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--steps", type=int, default=2000,
+                    help="timed steps (default: the film's whole 2,000-step run, about 10 s of GPU time per mode)")
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--mode", default="learned+mixed", choices=["learned+mixed", "mixed", "plain"],
                     help="the measured configuration; the plain FP64 engine is timed beside it")
@@ -541,8 +550,8 @@ def main():
             guess_src = "untrained head: the extrapolation of the last two solutions"
     args.precision = precision
     res = run_gpu(args, w, m, u0, precision, guess)
-    print(f"[bench] {args.mode}: {res['ms_per_step']:.2f} ms/step, iterations {res['iters']}", file=sys.stderr,
-          flush=True)
+    print(f"[bench] {args.mode}: {res['ms_per_step']:.2f} ms/step, iterations {iters_text(res['iters'])}",
+          file=sys.stderr, flush=True)
     plain = None
     if args.mode != "plain" and not args.no_compare:
         g = make_engine(args, w, m, "fp64", None, False)
@@ -557,7 +566,8 @@ def main():
         g.synchronize()
         plain = dict(ms=(time.perf_counter() - t1) * 1e3 / args.steps, iters=list(st.iters))
         g.close()
-        print(f"[bench] plain fp64: {plain['ms']:.2f} ms/step, iterations {plain['iters']}", file=sys.stderr, flush=True)
+        print(f"[bench] plain fp64: {plain['ms']:.2f} ms/step, iterations {iters_text(plain['iters'])}", file=sys.stderr,
+              flush=True)
         acc = step_accuracy(args, w, m, u0, precision, guess)
         dm = acc["dm_vs_plain"]
         print(f"[bench] per-step accuracy: {acc}", file=sys.stderr, flush=True)

@@ -536,6 +536,12 @@ void free_all(Ctx &c)
         (void)hipEventDestroy(pnd.b);
     }
     for (auto e : c.event_pool) (void)hipEventDestroy(e);
+    if (c.ev_fork) (void)hipEventDestroy(c.ev_fork);
+    if (c.ev_mid) (void)hipEventDestroy(c.ev_mid);
+    if (c.ev_join) (void)hipEventDestroy(c.ev_join);
+    if (c.aux) (void)hipStreamDestroy(c.aux);
+    c.ev_fork = c.ev_mid = c.ev_join = nullptr;
+    c.aux = nullptr;
 }
 
 double stencil_coef(const fg_ctx *x) { return x->prefactor * THETA * x->dt * GAMMA0; }
@@ -654,6 +660,10 @@ int create_impl(const fg_mesh *mesh, const fg_options *opt, fg_ctx **out, const 
     try {
         HIPCHK(hipSetDevice(c.device));
         HIPCHK(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
+        HIPCHK(hipStreamCreateWithFlags(&c.aux, hipStreamNonBlocking));
+        HIPCHK(hipEventCreateWithFlags(&c.ev_fork, hipEventDisableTiming));
+        HIPCHK(hipEventCreateWithFlags(&c.ev_mid, hipEventDisableTiming));
+        HIPCHK(hipEventCreateWithFlags(&c.ev_join, hipEventDisableTiming));
         build_plan(c, mesh);
         if (c.opt.demag_method == FG_DEMAG_FMM) {
             int rc = fmm_create(c);

@@ -22,6 +22,10 @@ struct KernelStat {
 struct Ctx {
     int device = 0;
     hipStream_t stream = nullptr;
+    // a second stream (default priority) on which the FMM near field runs beside the upward pass,
+    // joined before the far-field evaluation reads its potentials; fork / join events
+    hipStream_t aux = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_mid = nullptr, ev_join = nullptr;
     fg_options opt{};
     std::string err;
 

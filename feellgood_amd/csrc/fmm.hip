@@ -89,6 +89,8 @@ struct FmmPlan {
     int64_t *d_lc_ptr = nullptr;  // [ntl+1] shared column list of each such leaf
     int32_t *d_lc_col = nullptr;
     double2 *d_pot = nullptr;  // [T] near-field + correction potential
+    double2 *d_far = nullptr;  // [T] multipole (far) part of the potential, M2P's output
+    int32_t *d_tgt_lc = nullptr;  // [T] local-expansion cell of each target's group (-1: none)
     float *d_nf_valf = nullptr;  // FP32 coefficients (mixed precision; d_nf_val is then released)
     bool mixed = false;
     double mscale = 1.0;         // expansions times mscale (a power of two) in the FP32 mode
@@ -1486,23 +1488,19 @@ template <int P, int PF, int PV, int H, class R>
 __global__ __launch_bounds__(256) void k_fmm_m2p(int ng, const int32_t *__restrict__ gid,
                                                  const int32_t *__restrict__ grp_tb,
                                                  const int32_t *__restrict__ grp_te,
-                                                 const int32_t *__restrict__ grp_lcell,
                                                  const int32_t *__restrict__ u_ptr, const int32_t *__restrict__ u_far,
                                                  const int32_t *__restrict__ u_cell,
                                                  const uint8_t *__restrict__ u_mask,
                                                  const uint8_t *__restrict__ tgt_oct, const double4 *__restrict__ tpos,
                                                  const double4 *__restrict__ cen, const R *__restrict__ M,
-                                                 const R *__restrict__ Lx, const double2 *__restrict__ pot,
-                                                 const int32_t *__restrict__ tgt_node, NodeRec *__restrict__ nxt,
-                                                 double mscale)
+                                                 double2 *__restrict__ far)
 {
-    constexpr int NCo = fcount(P), CW = 2 * NCo;
     const int wv = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
     const int gi = xcd_block(blockIdx.x, gridDim.x) * 4 + wv;
     if (gi >= ng) return;  // whole wavefront
     const int g = gid[gi];
     const int lane = threadIdx.x & 63, row = lane >> 4, j = lane & 15;
-    const int tb = grp_tb[g], te = grp_te[g], ub = u_ptr[g], ue = u_ptr[g + 1], uf = u_far[g], lc = grp_lcell[g];
+    const int tb = grp_tb[g], te = grp_te[g], ub = u_ptr[g], ue = u_ptr[g + 1], uf = u_far[g];
     for (int t0 = tb; t0 < te; t0 += 64) {
         const int nt = min(64, te - t0);
         // H = 1: nr target rows, cps = 4 / nr cells per step; H = 2: one target row of 8 lanes
@@ -1528,22 +1526,38 @@ __global__ __launch_bounds__(256) void k_fmm_m2p(int ng, const int32_t *__restri
             su += __shfl(pu, src, 64);
             sv += __shfl(pv, src, 64);
         }
-        if (slot == 0 && tv) {
-            // the local expansion of the group's ancestor (scaled like the multipoles)
-            if (lc >= 0) {
-                const double4 c = cen[lc];
-                const double iw = c.w;
-                l2p_eval2_t<P>(Lx + (size_t)lc * 2 * CW, Lx + (size_t)lc * 2 * CW + 2 * NCo, (y.x - c.x) * iw,
-                               (y.y - c.y) * iw, (y.z - c.z) * iw, su, sv);
-            }
-            su /= mscale;
-            sv /= mscale;
-            const double2 pn = pot[t];
-            NodeRec *r = nxt + tgt_node[t];
-            r->phi = (pn.x + su) / (4 * M_PI);
-            r->phiv = (pn.y + sv) / (4 * M_PI);
-        }
+        if (slot == 0 && tv) far[t] = make_double2(su, sv);
     }
+}
+
+// Per target: the multipole part (M2P), the local expansion of the group's ancestor (scaled like
+// the multipoles), the scale undone, plus the near field and corrections, into the node's
+// potentials (1/4 pi, src/fmm_demag.h:184-224).
+template <int P, class R>
+__global__ __launch_bounds__(256) void k_fmm_finish(int T, const int32_t *__restrict__ tgt_lc,
+                                                    const double4 *__restrict__ tpos, const double4 *__restrict__ cen,
+                                                    const R *__restrict__ Lx, const double2 *__restrict__ far,
+                                                    const double2 *__restrict__ pot, const int32_t *__restrict__ tgt_node,
+                                                    NodeRec *__restrict__ nxt, double mscale)
+{
+    constexpr int NCo = fcount(P), CW = 2 * NCo;
+    const int t = blockIdx.x * 256 + threadIdx.x;
+    if (t >= T) return;
+    const double2 f = far[t];
+    double su = f.x, sv = f.y;
+    const int lc = tgt_lc[t];
+    if (lc >= 0) {
+        const double4 c = cen[lc], y = tpos[t];
+        const double iw = c.w;
+        l2p_eval2_t<P>(Lx + (size_t)lc * 2 * CW, Lx + (size_t)lc * 2 * CW + 2 * NCo, (y.x - c.x) * iw,
+                       (y.y - c.y) * iw, (y.z - c.z) * iw, su, sv);
+    }
+    su /= mscale;
+    sv /= mscale;
+    const double2 pn = pot[t];
+    NodeRec *r = nxt + tgt_node[t];
+    r->phi = (pn.x + su) / (4 * M_PI);
+    r->phiv = (pn.y + sv) / (4 * M_PI);
 }
 
 void fmm_build_near(Ctx &c, FmmPlan &P);
@@ -1801,6 +1815,13 @@ int fmm_create(Ctx &c)
         P->d_tgt_oct = dput(P->tgt_oct);
         P->d_tgt_node = dput(P->tgt_node);
         P->d_tgt_pos = dput(P->tgt_pos);
+        {
+            std::vector<int32_t> tlc(P->tgt_node.size(), -1);
+            for (size_t g = 0; g < P->grp_tb.size(); g++)
+                for (int t = P->grp_tb[g]; t < P->grp_te[g]; t++) tlc[t] = P->grp_lcell[g];
+            P->d_tgt_lc = dput(tlc);
+            P->d_far = dput(std::vector<double2>(std::max<size_t>(1, P->tgt_node.size()), make_double2(0, 0)));
+        }
         P->d_leaf_cell = dput(P->leaf_cell);
         P->d_m2m_cells = dput(P->m2m_cells);
         P->mixed = c.opt.precision == FG_PREC_MIXED;
@@ -2009,7 +2030,7 @@ void fmm_destroy(Ctx &c)
     FmmPlan *P = c.fmm;
     if (!P) return;
     void *ptrs[] = {P->d_cen, P->d_m2p_gid, P->d_leaf_sb, P->d_leaf_se, P->d_grp_tb, P->d_grp_te, P->d_grp_lcell, P->d_u_ptr,
-                    P->d_u_far, P->d_u_cell, P->d_u_mask, P->d_tgt_oct, P->d_tgt_node, P->d_tgt_pos,
+                    P->d_u_far, P->d_u_cell, P->d_u_mask, P->d_tgt_oct, P->d_tgt_node, P->d_tgt_pos, P->d_tgt_lc, P->d_far,
                     P->d_Tm2m, P->d_Tl2l, P->d_Tm2l, P->d_m2l_part, P->d_m2l_tile_cell, P->d_m2l_tile_chunk,
                     P->d_m2l_chunk_code, P->d_m2l_chunk_src, P->d_m2m_child, P->d_l2l_par, P->d_l2l_child, P->d_M, P->d_L, P->d_leaf_cell, P->d_m2m_cells,
                     P->d_nf_ptr, P->d_nf_val, P->d_p2m_pos, P->d_p2m_tw, P->d_p2m_code,
@@ -2029,16 +2050,27 @@ static void run_fmm(Ctx &c)
     FmmPlan &P = *c.fmm;
     const int NL = (int)P.leaf_cell.size();
     F *M = (F *)P.d_M, *Lx = (F *)P.d_L;
+    // The near field reads the element values and corrections and writes potentials that only the
+    // final sum reads: it runs on the second stream beside P2M and M2M (memory latency against VALU
+    // and matrix-core work), and M2L and L2L follow it there beside M2P. Profiled runs keep one
+    // stream, so each kernel's time is its own.
+    const bool fork = !c.opt.profile && c.aux;
+    hipStream_t ns = fork ? c.aux : c.stream;
+    if (fork) {
+        if (hipEventRecord(c.ev_fork, c.stream) != hipSuccess || hipStreamWaitEvent(c.aux, c.ev_fork, 0) != hipSuccess)
+            throw std::string("hipEventRecord / hipStreamWaitEvent (near-field fork)");
+    }
     {
         Prof pr(c, "fmm_near");
         const unsigned nb = (unsigned)(((int64_t)P.ntl * 64 + 255) / 256);
         if (P.ntl > 0 && P.d_nf_valf)
-            k_nf_apply<float><<<nb, 256, 0, c.stream>>>(P.ntl, P.d_tl_tb, P.d_tl_te, P.d_lc_ptr, P.d_lc_col, P.d_nf_ptr,
-                                                        P.d_nf_valf, c.qe, P.d_tgt_node, c.corr, P.d_pot);
+            k_nf_apply<float><<<nb, 256, 0, ns>>>(P.ntl, P.d_tl_tb, P.d_tl_te, P.d_lc_ptr, P.d_lc_col, P.d_nf_ptr,
+                                                  P.d_nf_valf, c.qe, P.d_tgt_node, c.corr, P.d_pot);
         else if (P.ntl > 0)
-            k_nf_apply<double><<<nb, 256, 0, c.stream>>>(P.ntl, P.d_tl_tb, P.d_tl_te, P.d_lc_ptr, P.d_lc_col,
-                                                         P.d_nf_ptr, P.d_nf_val, c.qe, P.d_tgt_node, c.corr, P.d_pot);
+            k_nf_apply<double><<<nb, 256, 0, ns>>>(P.ntl, P.d_tl_tb, P.d_tl_te, P.d_lc_ptr, P.d_lc_col, P.d_nf_ptr,
+                                                   P.d_nf_val, c.qe, P.d_tgt_node, c.corr, P.d_pot);
     }
+
     {
         Prof pr(c, "fmm_p2m");
         constexpr int G = 8;
@@ -2068,15 +2100,21 @@ static void run_fmm(Ctx &c)
                 (const F *)P.d_Tm2m, M);
         }
     }
+    // the downward pass (M2L, L2L) on the second stream once the multipoles are complete, beside
+    // M2P, which reads only the multipoles (matrix cores and latency against VALU issue)
+    if (fork) {
+        if (hipEventRecord(c.ev_mid, c.stream) != hipSuccess || hipStreamWaitEvent(c.aux, c.ev_mid, 0) != hipSuccess)
+            throw std::string("hipEventRecord / hipStreamWaitEvent (downward-pass fork)");
+    }
     {
         Prof pr(c, "fmm_m2l");
         if (P.m2l_nchunks > 0)
-            k_fmm_m2l_mfma<PP, F><<<P.m2l_nchunks, 256, 0, c.stream>>>(P.d_m2l_tile_chunk, P.m2l_ntiles,
-                                                                      P.d_m2l_chunk_code, P.d_m2l_chunk_src,
-                                                                      (const F *)P.d_Tm2l, M, (F *)P.d_m2l_part);
+            k_fmm_m2l_mfma<PP, F><<<P.m2l_nchunks, 256, 0, ns>>>(P.d_m2l_tile_chunk, P.m2l_ntiles,
+                                                                P.d_m2l_chunk_code, P.d_m2l_chunk_src,
+                                                                (const F *)P.d_Tm2l, M, (F *)P.d_m2l_part);
         if (P.m2l_ntiles > 0) {
             const int64_t n = (int64_t)P.m2l_ntiles * 64 * 2 * 2 * P.nc;
-            k_fmm_m2l_reduce<PP, F><<<(int)((n + 255) / 256), 256, 0, c.stream>>>(
+            k_fmm_m2l_reduce<PP, F><<<(int)((n + 255) / 256), 256, 0, ns>>>(
                 P.m2l_ntiles, P.d_m2l_tile_cell, P.d_m2l_tile_chunk, P.d_cen, (const F *)P.d_m2l_part, Lx);
         }
     }
@@ -2085,11 +2123,12 @@ static void run_fmm(Ctx &c)
         for (int l = 3; l <= P.Lmax; l++) {
             const int n = P.l2l_par_off[l + 1] - P.l2l_par_off[l];
             if (n == 0) continue;
-            k_fmm_xlate<PP, 1, F><<<(n + 15) / 16 * 2 * XL<PP>::NT, 256, 0, c.stream>>>(
+            k_fmm_xlate<PP, 1, F><<<(n + 15) / 16 * 2 * XL<PP>::NT, 256, 0, ns>>>(
                 n, P.d_l2l_par + P.l2l_par_off[l], P.d_l2l_child + (size_t)8 * P.l2l_par_off[l],
                 (const F *)P.d_Tl2l, Lx);
         }
     }
+    if (fork && hipEventRecord(c.ev_join, c.aux) != hipSuccess) throw std::string("hipEventRecord (join)");
     {
         Prof pr(c, "fmm_m2p");
         const int ng = (int)P.grp_tb.size();
@@ -2099,15 +2138,13 @@ static void run_fmm(Ctx &c)
             auto go = [&](auto k1, auto k2) {
                 const int n1 = P.m2p_n1, n2 = ng - P.m2p_n1;
                 if (n1 > 0)
-                    k1<<<(n1 + 3) / 4, 256, 0, c.stream>>>(n1, P.d_m2p_gid, P.d_grp_tb, P.d_grp_te, P.d_grp_lcell,
-                                                          P.d_u_ptr, P.d_u_far, P.d_u_cell, P.d_u_mask, P.d_tgt_oct,
-                                                          P.d_tgt_pos, P.d_cen, M, Lx, P.d_pot, P.d_tgt_node,
-                                                          c.rec[1 - c.cur], P.mscale);
+                    k1<<<(n1 + 3) / 4, 256, 0, c.stream>>>(n1, P.d_m2p_gid, P.d_grp_tb, P.d_grp_te, P.d_u_ptr,
+                                                          P.d_u_far, P.d_u_cell, P.d_u_mask, P.d_tgt_oct, P.d_tgt_pos,
+                                                          P.d_cen, M, P.d_far);
                 if (n2 > 0)
-                    k2<<<(n2 + 3) / 4, 256, 0, c.stream>>>(n2, P.d_m2p_gid + n1, P.d_grp_tb, P.d_grp_te,
-                                                          P.d_grp_lcell, P.d_u_ptr, P.d_u_far, P.d_u_cell, P.d_u_mask,
-                                                          P.d_tgt_oct, P.d_tgt_pos, P.d_cen, M, Lx, P.d_pot,
-                                                          P.d_tgt_node, c.rec[1 - c.cur], P.mscale);
+                    k2<<<(n2 + 3) / 4, 256, 0, c.stream>>>(n2, P.d_m2p_gid + n1, P.d_grp_tb, P.d_grp_te, P.d_u_ptr,
+                                                          P.d_u_far, P.d_u_cell, P.d_u_mask, P.d_tgt_oct, P.d_tgt_pos,
+                                                          P.d_cen, M, P.d_far);
             };
             constexpr int PL = fmm_far_degree(PP);
             // the second expansion (charges of v) to degree fmm_v_degree in the FP32 mode
@@ -2117,6 +2154,17 @@ static void run_fmm(Ctx &c)
             else
                 go(k_fmm_m2p<PP, PP, PV, 1, F>, k_fmm_m2p<PP, PP, PV, 2, F>);
         }
+    }
+    // join: the near field, M2L and L2L are in
+    if (fork && hipStreamWaitEvent(c.stream, c.ev_join, 0) != hipSuccess)
+        throw std::string("hipStreamWaitEvent (join)");
+    {
+        Prof pr(c, "fmm_finish");
+        const int T = (int)P.tgt_node.size();
+        if (T > 0)
+            k_fmm_finish<PP, F><<<(T + 255) / 256, 256, 0, c.stream>>>(T, P.d_tgt_lc, P.d_tgt_pos, P.d_cen, Lx, P.d_far,
+                                                                        P.d_pot, P.d_tgt_node, c.rec[1 - c.cur],
+                                                                        P.mscale);
     }
 }
 
