@@ -747,7 +747,8 @@ void guess_on_assembly(Ctx &c, double dt)
                                                                    g->params, g->xq);
         }
         Prof pr(c, "guess_forward");
-        const int nb = std::min((c.N + 255) / 256, 2048);
+        // two blocks per CU, each staging the weights once and looping over node tiles
+        const int nb = std::min((c.N + 255) / 256, 512);
         k_guess_infer<<<nb, 256, 0, c.stream>>>(c.N, g->xq, g->feat, c.mag, g->params, g->wbf, c.ep4, c.eq4, c.x, c.w);
     }
 }
