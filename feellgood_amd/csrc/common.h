@@ -17,6 +17,17 @@ constexpr int BLOCK = 256;        // threads per block of the streaming kernels
 constexpr int MAX_PART = 1024;    // blocks of a grid-stride streaming kernel (partial sums)
 constexpr int SELL_C = 64;        // rows per SELL slice = one wavefront
 
+// Logical block index with the dispatcher's round-robin over the 8 XCDs undone: consecutive
+// logical blocks (neighbouring nodes and elements, which share gathered data) run on the same XCD
+// and its L2. A bijection of [0, nb).
+__device__ inline int xcd_block(int b, int nb)
+{
+    const int q = nb >> 3, r = nb & 7, x = b & 7;
+    return x * q + min(x, r) + (b >> 3);
+}
+// this block's logical index
+__device__ inline int lblock() { return xcd_block((int)blockIdx.x, (int)gridDim.x); }
+
 // shape functions at the Gauss points (src/tetra.h:190-220, src/triangle.h:42-64)
 __host__ __device__ inline double tet_a(int i, int g)
 {
@@ -64,6 +75,9 @@ struct KrylovCtl {
     int done;
     int max_iter;
     int refinements;  // FP64 refinement sweeps of the mixed-precision solve
+    // set by the t = A s phase when ||s|| meets the criterion, turned into done by the next phase:
+    // a block of that launch can start after block 0 has written, so it may not test done itself
+    int stop;
 };
 
 // small vector helpers

@@ -741,13 +741,6 @@ static T *dput(const std::vector<T> &h)
     return (T *)d;
 }
 
-// Logical block index with the dispatcher's round-robin over the 8 XCDs undone: consecutive
-// logical blocks (neighbouring cells, which share source data) run on the same XCD and its L2.
-__device__ inline int xcd_block(int b, int nb)
-{
-    const int q = nb >> 3, r = nb & 7, x = b & 7;
-    return x * q + min(x, r) + (b >> 3);
-}
 
 // Sum over each aligned group of 8 lanes, left in every lane of the group: data-parallel-primitive
 // moves (swap within pairs, swap pairs within quads, mirror within the half row) of the two
@@ -1055,6 +1048,84 @@ __global__ __launch_bounds__(256) void k_fmm_xlate(int n, const int32_t *__restr
     __shared__ typename Mfma<F>::V red[3][64];  // partial tiles of wavefronts 1..3 (M2M)
     const int R = xcd_block(blockIdx.x, gridDim.x) % NT;
     xlate_tile_dispatch<P, K, 0, F>(R, n, cells, child, Top, X, red);
+}
+
+// Row tiles R.. of the M2M below: acc[R] += T_o[R rows] child_o for the wavefront's two octants,
+// over the k-steps of the input degrees that row tile's output degrees take.
+template <int P, int R, class F>
+__device__ __forceinline__ void m2m_rows(const F *__restrict__ A, const F (&bv)[2][XL<P>::NKS],
+                                         typename Mfma<F>::V (&acc)[XL<P>::NT])
+{
+    if constexpr (R < XL<P>::NT) {
+        using L = XL<P>;
+        constexpr int NKS = L::NKS, ke = L::ke(0, R);
+        F av[2][ke];
+#pragma unroll
+        for (int oo = 0; oo < 2; oo++)
+#pragma unroll
+            for (int ks = 0; ks < ke; ks++) av[oo][ks] = A[((size_t)oo * L::NT + R) * NKS * 64 + ks * 64];
+#pragma unroll
+        for (int oo = 0; oo < 2; oo++)
+#pragma unroll
+            for (int ks = 0; ks < ke; ks++) acc[R] = Mfma<F>::mma(av[oo][ks], bv[oo][ks], acc[R]);
+        m2m_rows<P, R + 1, F>(A, bv, acc);
+    }
+}
+
+constexpr int M2M_ALL_MIN = 2048;  // parents of a level from which k_fmm_m2m_all runs it
+
+// M2M for the levels with many parents. One block per (16 parents, charge set) takes every row
+// tile, so each child expansion is read once and every wavefront has a row tile's worth of
+// matrix-core work per load: wavefront w loads the B fragments of octants 2w and 2w + 1 (column =
+// parent, k-step = four input reals), then applies those octants' operators tile by tile. The four
+// partial results of each row tile go through LDS; wavefront w sums and writes tiles w, w + 4.
+template <int P, class F>
+__global__ __launch_bounds__(256) void k_fmm_m2m_all(int n, const int32_t *__restrict__ cells,
+                                                     const int32_t *__restrict__ child, const F *__restrict__ Top,
+                                                     F *__restrict__ X)
+{
+    using L = XL<P>;
+    using MF = Mfma<F>;
+    using V = typename MF::V;
+    constexpr int NR = L::NR, NT = L::NT, NKS = L::NKS;
+    __shared__ V red[NT][4][64];
+    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+    const int b = xcd_block(blockIdx.x, gridDim.x);  // the two charge sets of a column tile together
+    const int s = b & 1, ct = b >> 1;
+    const int col = l & 15, kq = l >> 4;
+    const int pi = ct * 16 + col;
+    const bool act = pi < n;
+    F bv[2][NKS];
+#pragma unroll
+    for (int oo = 0; oo < 2; oo++) {
+        const int ch = act ? child[(size_t)pi * 8 + 2 * w + oo] : -1;
+        const F *xs = X + ((size_t)(ch < 0 ? 0 : ch) * 2 + s) * NR;
+#pragma unroll
+        for (int ks = 0; ks < NKS; ks++) {
+            const int k = 4 * ks + kq;
+            bv[oo][ks] = ch >= 0 && k < NR ? xs[k] : F(0);
+        }
+    }
+    V acc[NT];
+#pragma unroll
+    for (int R = 0; R < NT; R++) acc[R] = V{F(0), F(0), F(0), F(0)};
+    m2m_rows<P, 0, F>(Top + (size_t)2 * w * NT * NKS * 64 + l, bv, acc);
+#pragma unroll
+    for (int R = 0; R < NT; R++) red[R][w][l] = acc[R];
+    __syncthreads();
+    if (!act) return;
+    const int pc = cells[pi];
+    F *d = X + ((size_t)pc * 2 + s) * NR;
+    for (int R = w; R < NT; R += 4) {
+        V a = red[R][0][l];
+#pragma unroll
+        for (int q = 1; q < 4; q++) a += red[R][q][l];
+#pragma unroll
+        for (int t = 0; t < 4; t++) {
+            const int r = 16 * R + MF::row(kq, t);
+            if (r < NR) d[r] = a[t];
+        }
+    }
 }
 
 // M2L on the matrix cores. The translation from a source cell to a target cell at offset d (in
@@ -2095,9 +2166,16 @@ static void run_fmm(Ctx &c)
         for (int l = P.Lmax - 1; l >= 2; l--) {
             const int n = P.m2m_lev_off[l + 1] - P.m2m_lev_off[l];
             if (n == 0) continue;
-            k_fmm_xlate<PP, 0, F><<<(n + 15) / 16 * 2 * XL<PP>::NT, 256, 0, c.stream>>>(
-                n, P.d_m2m_cells + P.m2m_lev_off[l], P.d_m2m_child + (size_t)8 * P.m2m_lev_off[l],
-                (const F *)P.d_Tm2m, M);
+            // a block per row tile where parents are few (a level then takes as long as one
+            // block), all row tiles in one block where they are many
+            if (n >= M2M_ALL_MIN)
+                k_fmm_m2m_all<PP, F><<<(n + 15) / 16 * 2, 256, 0, c.stream>>>(
+                    n, P.d_m2m_cells + P.m2m_lev_off[l], P.d_m2m_child + (size_t)8 * P.m2m_lev_off[l],
+                    (const F *)P.d_Tm2m, M);
+            else
+                k_fmm_xlate<PP, 0, F><<<(n + 15) / 16 * 2 * XL<PP>::NT, 256, 0, c.stream>>>(
+                    n, P.d_m2m_cells + P.m2m_lev_off[l], P.d_m2m_child + (size_t)8 * P.m2m_lev_off[l],
+                    (const F *)P.d_Tm2m, M);
         }
     }
     // the downward pass (M2L, L2L) on the second stream once the multipoles are complete, beside
