@@ -1074,10 +1074,15 @@ int fg_solve(fg_ctx *x, double dt, fg_solve_info *info)
         // n's convergence is tested at the start of iteration n + 1, so the first batch is the
         // previous solve's count plus one: a solve that converges like the last one needs a single
         // round trip to the host.
+        // Each batch is followed by the node update gated on the control block, so that a solve
+        // accepted in the batch is finished by the time the host reads it (no second round trip);
+        // profiled runs update after the read, so the update's statistics hold real launches only.
+        const bool gated = !c.opt.profile;
         int launched = 0;
         int batch = std::max(2, std::min(x->last_nit + 1, c.opt.max_iter));
         while (true) {
             for (int k = 0; k < batch && launched <= c.opt.max_iter; k++, launched++) launch_iteration(c, cst);
+            if (gated) launch_update(c, dt, true);
             HIPCHK(hipMemcpyAsync(c.h_ctl, c.ctl, sizeof(KrylovCtl), hipMemcpyDeviceToHost, c.stream));
             HIPCHK(hipStreamSynchronize(c.stream));
             if (c.h_ctl->done || launched > c.opt.max_iter) break;
@@ -1096,7 +1101,10 @@ int fg_solve(fg_ctx *x, double dt, fg_solve_info *info)
             info->v_max = 0;
         }
         if (failed) return FG_SOLVE_FAILED;
-        return finish_solve(x, dt, info);
+        if (!gated) return finish_solve(x, dt, info);
+        guess_on_solved(c);
+        if (info) info->v_max = GAMMA0 * std::sqrt(k.vmax2);
+        return FG_OK;
     });
 }
 
@@ -1107,8 +1115,12 @@ int fg_calc_demag(fg_ctx *x)
         launch_charges(c);
         if (c.opt.demag_method == FG_DEMAG_FMM) launch_fmm(c);
         else launch_direct_p2p(c);
-        HIPCHK(hipStreamSynchronize(c.stream));
-        if (c.opt.profile) c.flush_stats();
+        // no wait: every entry point works in order on the context's stream and waits where it
+        // reads results back
+        if (c.opt.profile) {
+            HIPCHK(hipStreamSynchronize(c.stream));
+            c.flush_stats();
+        }
         return FG_OK;
     });
 }

@@ -185,7 +185,9 @@ void launch_inner_iteration(Ctx &c, double c_stencil);
 void launch_refine(Ctx &c);
 // FG_BATCH_PERSISTENT: every instance's bicg_dir in one launch (one block per instance)
 void launch_batch_solve(Ctx &c, double c_stencil, int block);
-void launch_update(Ctx &c, double dt);
+// gated: only if the control block says the solve was accepted (launched with the last
+// iterations, before the host has read it)
+void launch_update(Ctx &c, double dt, bool gated = false);
 void launch_spmv_debug(Ctx &c, double c_stencil, const double2 *xin, double2 *yout);
 void launch_charges(Ctx &c);
 void launch_direct_p2p(Ctx &c);

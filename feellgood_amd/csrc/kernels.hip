@@ -39,9 +39,10 @@ __device__ inline double wave_min(double v)
     return v;
 }
 
-// Sum of NV per-thread values over the block, stored to part[slot[k] * MAX_PART + blockIdx.x].
+// Sum of NV per-thread values over the block, stored to part[slot[k] * MAX_PART + blk], blk being
+// the block index the kernel assigned its items by (blockIdx.x or lblock()).
 template <int NV>
-__device__ inline void block_store(const double (&v)[NV], const int (&slot)[NV], double *part)
+__device__ inline void block_store(const double (&v)[NV], const int (&slot)[NV], double *part, int blk)
 {
     __shared__ double sh[NV][BLOCK / 64];
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
@@ -57,7 +58,7 @@ __device__ inline void block_store(const double (&v)[NV], const int (&slot)[NV],
         for (int k = 0; k < NV; k++) {
             double s = sh[k][0];
             for (int q = 1; q < BLOCK / 64; q++) s += sh[k][q];
-            part[slot[k] * MAX_PART + blockIdx.x] = s;
+            part[slot[k] * MAX_PART + blk] = s;
         }
 }
 
@@ -312,7 +313,7 @@ __global__ __launch_bounds__(BLOCK) void k_node_assemble(
     }
     const double vals[1] = {bb};
     const int slots[1] = {0};
-    block_store<1>(vals, slots, part);
+    block_store<1>(vals, slots, part, blockIdx.x);
 }
 
 // ------------------------------------------------------------------ stencil SpMV
@@ -373,11 +374,11 @@ __global__ __launch_bounds__(BLOCK) void k_r0(int N, double cst, STENCIL_ARGS, c
     // rt = r, so rt . r = ||r||^2 for the first iteration
     const double vals[2] = {rr, rr};
     const int slots[2] = {1, 2};
-    block_store<2>(vals, slots, part);
+    block_store<2>(vals, slots, part, blockIdx.x);
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         ctl->nit = 0;
         ctl->status = -1;
-        ctl->done = 0;
+        ctl->done = ctl->stop = 0;
         ctl->rho1 = ctl->rho2 = ctl->alpha = ctl->omega = ctl->beta = 0.0;
     }
 }
@@ -491,7 +492,7 @@ __global__ __launch_bounds__(BLOCK) void k_spmv_v(int N, double cst, STENCIL_ARG
 {
     if (ctl->done) return;
     double vr = 0;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x) {
+    for (int i = lblock() * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x) {
         double2 y = make_double2(0, 0);
         if (mag[i]) y = stencil_row(i, cst, sptr, swid, scol, sval, w, ep4, eq4, Dg, ph[i]);
         v[i] = y;
@@ -500,7 +501,7 @@ __global__ __launch_bounds__(BLOCK) void k_spmv_v(int N, double cst, STENCIL_ARG
     }
     const double vals[1] = {vr};
     const int slots[1] = {3};
-    block_store<1>(vals, slots, part);
+    block_store<1>(vals, slots, part, lblock());
 }
 
 // (E) alpha = rho_1 / (v . rt); s = r - alpha v; s^ = D s; w = lift(s^); partial ||s||^2
@@ -531,7 +532,7 @@ __global__ __launch_bounds__(BLOCK) void k_iter_s(int N, int nb, KrylovCtl *ctl,
     }
     const double vals[1] = {ss};
     const int slots2[1] = {4};
-    block_store<1>(vals, slots2, part);
+    block_store<1>(vals, slots2, part, blockIdx.x);
 }
 
 // (F) convergence on ||s|| (x += alpha p^ when converged), else t = A s^ (masked);
@@ -562,12 +563,12 @@ __global__ __launch_bounds__(BLOCK) void k_spmv_t(int N, int nb, double cst, STE
     if (blockIdx.x == 0 && threadIdx.x == 0 && done) {
         ctl->res = res;
         ctl->status = status;
-        ctl->done = 1;
+        ctl->stop = 1;
     }
     if (done) {
         if (conv) {
             const double alpha = ctl->alpha;
-            for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x) {
+            for (int i = lblock() * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x) {
                 double2 xi = x[i];
                 const double2 h = ph[i];
                 xi.x += alpha * h.x;
@@ -578,7 +579,7 @@ __global__ __launch_bounds__(BLOCK) void k_spmv_t(int N, int nb, double cst, STE
         return;
     }
     double ts = 0, tt = 0;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x) {
+    for (int i = lblock() * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x) {
         double2 y = make_double2(0, 0);
         if (mag[i]) y = stencil_row(i, cst, sptr, swid, scol, sval, w, ep4, eq4, Dg, sh[i]);
         t[i] = y;
@@ -588,7 +589,7 @@ __global__ __launch_bounds__(BLOCK) void k_spmv_t(int N, int nb, double cst, STE
     }
     const double vals[2] = {ts, tt};
     const int slots2[2] = {5, 6};
-    block_store<2>(vals, slots2, part);
+    block_store<2>(vals, slots2, part, lblock());
 }
 
 // (G) omega = (t.s)/(t.t); x += alpha p^ + omega s^; r = s - omega t; partials ||r||^2, rt.r
@@ -599,6 +600,10 @@ __global__ __launch_bounds__(BLOCK) void k_iter_x(int N, int nb, KrylovCtl *ctl,
                                                   double *__restrict__ part)
 {
     if (ctl->done) return;
+    if (ctl->stop) {  // stopped on ||s|| by the previous phase
+        if (blockIdx.x == 0 && threadIdx.x == 0) ctl->done = 1;
+        return;
+    }
     const int slots[2] = {5, 6};
     double red[2];
     block_load<2>(part, slots, nb, red);
@@ -620,7 +625,7 @@ __global__ __launch_bounds__(BLOCK) void k_iter_x(int N, int nb, KrylovCtl *ctl,
     }
     const double vals[2] = {rr, rtr};
     const int slots2[2] = {1, 2};
-    block_store<2>(vals, slots2, part);
+    block_store<2>(vals, slots2, part, blockIdx.x);
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         ctl->omega = omega;
         ctl->rho2 = ctl->rho1;
@@ -881,7 +886,7 @@ __global__ void k_mixed_init(KrylovCtl *ctl)
     if (threadIdx.x == 0) {
         ctl->nit = 0;
         ctl->status = -1;
-        ctl->done = 0;
+        ctl->done = ctl->stop = 0;
         ctl->refinements = 0;
         ctl->res = 0;
     }
@@ -915,7 +920,7 @@ __global__ __launch_bounds__(BLOCK) void k_resid(int N, double cst, STENCIL_ARGS
     }
     const double vals[3] = {rr, ff, ff};
     const int slots[3] = {0, 1, 2};
-    block_store<3>(vals, slots, part);
+    block_store<3>(vals, slots, part, blockIdx.x);
 }
 
 // The outer test on the FP64 residual (one block): converged, NaN or out of iterations ends both
@@ -946,7 +951,7 @@ __global__ void k_outer(int nb, KrylovCtl *ctl, KrylovCtl *ic, double tol_in, in
     ic->max_iter = min(max_inner, ctl->max_iter - ctl->nit);
     ic->nit = 0;
     ic->status = -1;
-    ic->done = 0;
+    ic->done = ic->stop = 0;
     ic->rho1 = ic->rho2 = ic->alpha = ic->omega = ic->beta = 0.0;
     ctl->refinements++;
 }
@@ -1037,7 +1042,7 @@ __global__ __launch_bounds__(BLOCK) void kf_spmv_v(int N, float cst, STENCIL_ARG
 {
     if (ic->done) return;
     double vr = 0;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x) {
+    for (int i = lblock() * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x) {
         float2 y = make_float2(0, 0);
         if (mag[i]) y = stencil_row_f(i, cst, sptr, swid, scol, sval, w, ep4, eq4, Dg, ph[i]);
         v[i] = y;
@@ -1046,7 +1051,7 @@ __global__ __launch_bounds__(BLOCK) void kf_spmv_v(int N, float cst, STENCIL_ARG
     }
     const double vals[1] = {vr};
     const int slots[1] = {3};
-    block_store<1>(vals, slots, part);
+    block_store<1>(vals, slots, part, lblock());
 }
 
 __global__ __launch_bounds__(BLOCK) void kf_iter_s(int N, int nb, KrylovCtl *ic, const float4 *__restrict__ ep4,
@@ -1075,7 +1080,7 @@ __global__ __launch_bounds__(BLOCK) void kf_iter_s(int N, int nb, KrylovCtl *ic,
     }
     const double vals[1] = {ss};
     const int slots2[1] = {4};
-    block_store<1>(vals, slots2, part);
+    block_store<1>(vals, slots2, part, blockIdx.x);
 }
 
 __global__ __launch_bounds__(BLOCK) void kf_spmv_t(int N, int nb, float cst, STENCIL_ARGS_F, KrylovCtl *ic,
@@ -1104,12 +1109,12 @@ __global__ __launch_bounds__(BLOCK) void kf_spmv_t(int N, int nb, float cst, STE
     if (blockIdx.x == 0 && threadIdx.x == 0 && done) {
         ic->res = res;
         ic->status = status;
-        ic->done = 1;
+        ic->stop = 1;
     }
     if (done) {
         if (conv) {
             const float al = (float)ic->alpha;
-            for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x) {
+            for (int i = lblock() * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x) {
                 float2 xi = x[i];
                 const float2 h = ph[i];
                 xi.x += al * h.x;
@@ -1120,7 +1125,7 @@ __global__ __launch_bounds__(BLOCK) void kf_spmv_t(int N, int nb, float cst, STE
         return;
     }
     double ts = 0, tt = 0;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x) {
+    for (int i = lblock() * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x) {
         float2 y = make_float2(0, 0);
         if (mag[i]) y = stencil_row_f(i, cst, sptr, swid, scol, sval, w, ep4, eq4, Dg, sh[i]);
         t[i] = y;
@@ -1130,7 +1135,7 @@ __global__ __launch_bounds__(BLOCK) void kf_spmv_t(int N, int nb, float cst, STE
     }
     const double vals[2] = {ts, tt};
     const int slots2[2] = {5, 6};
-    block_store<2>(vals, slots2, part);
+    block_store<2>(vals, slots2, part, lblock());
 }
 
 __global__ __launch_bounds__(BLOCK) void kf_iter_x(int N, int nb, KrylovCtl *ic, const float2 *__restrict__ ph,
@@ -1140,6 +1145,10 @@ __global__ __launch_bounds__(BLOCK) void kf_iter_x(int N, int nb, KrylovCtl *ic,
                                                    double *__restrict__ part)
 {
     if (ic->done) return;
+    if (ic->stop) {  // stopped on ||s|| by the previous phase
+        if (blockIdx.x == 0 && threadIdx.x == 0) ic->done = 1;
+        return;
+    }
     const int slots[2] = {5, 6};
     double red[2];
     block_load<2>(part, slots, nb, red);
@@ -1161,7 +1170,7 @@ __global__ __launch_bounds__(BLOCK) void kf_iter_x(int N, int nb, KrylovCtl *ic,
     }
     const double vals[2] = {rr, rtr};
     const int slots2[2] = {1, 2};
-    block_store<2>(vals, slots2, part);
+    block_store<2>(vals, slots2, part, blockIdx.x);
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         ic->omega = omega;
         ic->rho2 = ic->rho1;
@@ -1196,12 +1205,20 @@ __global__ __launch_bounds__(BLOCK) void k_lift(int N, const double4 *__restrict
 // ------------------------------------------------------------------ update
 // v = gamma0 (x_p ep + x_q eq); u_next = normalize(u + dt v); partial max |x_i|^2
 // (src/solver.cpp:74-88, src/node.h:116-122)
+// The solve ended and is accepted: converged, no overflow or breakdown, final residual within tol
+// (the host's test of the same control block)
+__device__ inline bool solved(const KrylovCtl *ctl)
+{
+    return ctl->done && ctl->status != 1 && ctl->status != 2 && ctl->res <= ctl->tol;
+}
+
 __global__ __launch_bounds__(BLOCK) void k_update(int N, double dt, const int32_t *__restrict__ mag,
                                                   const double2 *__restrict__ x, const double4 *__restrict__ ep4,
                                                   const double4 *__restrict__ eq4,
                                                   const NodeRec *__restrict__ cur, NodeRec *__restrict__ next,
-                                                  double *__restrict__ part)
+                                                  double *__restrict__ part, const KrylovCtl *gate)
 {
+    if (gate && !solved(gate)) return;
     double m2 = 0;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x) {
         if (!mag[i]) continue;
@@ -1237,8 +1254,9 @@ __global__ __launch_bounds__(BLOCK) void k_update(int N, double dt, const int32_
     }
 }
 
-__global__ void k_finish_update(int nb, KrylovCtl *ctl, const double *__restrict__ part)
+__global__ void k_finish_update(int nb, KrylovCtl *ctl, const double *__restrict__ part, bool gated)
 {
+    if (gated && !solved(ctl)) return;
     if (threadIdx.x < 64) {
         double m = 0;
         for (int i = threadIdx.x; i < nb; i += 64) m = fmax(m, part[7 * MAX_PART + i]);
@@ -1362,7 +1380,7 @@ __global__ __launch_bounds__(BLOCK) void k_corr_gather(int N, const int32_t *__r
                                                        const int32_t *__restrict__ cidx,
                                                        const double2 *__restrict__ corr3, double2 *__restrict__ corr)
 {
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x) {
+    for (int i = lblock() * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x) {
         double cu = 0, cv = 0;
         for (int k = cptr[i]; k < cptr[i + 1]; k++) {
             const double2 e = corr3[cidx[k]];
@@ -1501,7 +1519,7 @@ __global__ __launch_bounds__(BLOCK) void k_energy_tet(int Tm, const int4 *__rest
         e[3] += -MU0 * R.Ms * ze;
     }
     const int slots[4] = {0, 1, 2, 3};
-    block_store<4>(e, slots, part);
+    block_store<4>(e, slots, part, blockIdx.x);
 }
 
 __global__ __launch_bounds__(BLOCK) void k_energy_tri(int Fm, const int4 *__restrict__ tn,
@@ -1510,7 +1528,7 @@ __global__ __launch_bounds__(BLOCK) void k_energy_tri(int Fm, const int4 *__rest
                                                       const NodeRec *__restrict__ nxt, double *__restrict__ part)
 {
     double e[2] = {0, 0};
-    for (int f = blockIdx.x * blockDim.x + threadIdx.x; f < Fm; f += gridDim.x * blockDim.x) {
+    for (int f = lblock() * blockDim.x + threadIdx.x; f < Fm; f += gridDim.x * blockDim.x) {
         const int4 q = tn[f];
         const int nd[3] = {q.x, q.y, q.z};
         const d3 n = mk(tg[f], tg[(size_t)Fm + f], tg[(size_t)2 * Fm + f]);
@@ -1539,7 +1557,7 @@ __global__ __launch_bounds__(BLOCK) void k_energy_tri(int Fm, const int4 *__rest
         e[1] += 0.5 * MU0 * dMs * d;
     }
     const int slots[2] = {4, 5};
-    block_store<2>(e, slots, part);
+    block_store<2>(e, slots, part, lblock());
 }
 
 // final sums of the partials: out[k] = sum of slot k, k < nslots; slots >= nfirst use nb_b blocks
@@ -1564,7 +1582,7 @@ __global__ __launch_bounds__(BLOCK) void k_max_angle(int N, const int64_t *__res
                                                      const NodeRec *__restrict__ nxt, double *__restrict__ part)
 {
     double mn = 1.0;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x) {
+    for (int i = lblock() * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x) {
         const int sl = i >> 6, lane = i & 63;
         const int64_t off = sptr[sl];
         const NodeRec a = nxt[i];
@@ -1582,7 +1600,7 @@ __global__ __launch_bounds__(BLOCK) void k_max_angle(int N, const int64_t *__res
     if (threadIdx.x == 0) {
         double s = sh[0];
         for (int q = 1; q < BLOCK / 64; q++) s = fmin(s, sh[q]);
-        part[blockIdx.x] = s;
+        part[lblock()] = s;
     }
 }
 
@@ -1619,7 +1637,7 @@ __global__ __launch_bounds__(BLOCK) void k_average(int Tm, int which, const int4
         }
     }
     const int slots[3] = {0, 1, 2};
-    block_store<3>(e, slots, part);
+    block_store<3>(e, slots, part, blockIdx.x);
 }
 
 // Node::evolution (src/node.h:107): CURRENT = NEXT
@@ -1803,14 +1821,15 @@ void launch_refine(Ctx &c)
     k_refine<<<c.blocks_for(c.N), BLOCK, 0, c.stream>>>(c.N, c.ctl, c.ic, c.ep4, c.eq4, c.df, c.x, c.w);
 }
 
-void launch_update(Ctx &c, double dt)
+void launch_update(Ctx &c, double dt, bool gated)
 {
     const int nb = c.blocks_for(c.N);
     {
         Prof pr(c, "update");
-        k_update<<<nb, BLOCK, 0, c.stream>>>(c.N, dt, c.mag, c.x, c.ep4, c.eq4, c.rec[c.cur], c.rec[1 - c.cur], c.part);
+        k_update<<<nb, BLOCK, 0, c.stream>>>(c.N, dt, c.mag, c.x, c.ep4, c.eq4, c.rec[c.cur], c.rec[1 - c.cur], c.part,
+                                             gated ? c.ctl : nullptr);
     }
-    k_finish_update<<<1, 64, 0, c.stream>>>(nb, c.ctl, c.part);
+    k_finish_update<<<1, 64, 0, c.stream>>>(nb, c.ctl, c.part, gated);
 }
 
 void launch_spmv_debug(Ctx &c, double cst, const double2 *xin, double2 *yout)
