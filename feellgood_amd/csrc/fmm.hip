@@ -2148,10 +2148,10 @@ static void run_fmm(Ctx &c)
         const unsigned nb = (unsigned)((NL * G + 63) / 64);
         if (NL > 0) {
             if constexpr (std::is_same<F, float>::value) {
-                k_fmm_p2m<PP, G, p2m_half_mask<PP>(0), float, float4><<<nb, 64, 0, c.stream>>>(
+                k_fmm_p2m<PP, G, p2m_acc_half_mask<PP>(0), float, float4><<<nb, 64, 0, c.stream>>>(
                     NL, c.Tm, P.d_leaf_cell, P.d_cen, P.d_leaf_sb, P.d_leaf_se, (const float4 *)P.d_p2m_pos,
                     P.d_p2m_code, (const float2 *)P.d_p2m_tw, c.qe, M);
-                k_fmm_p2m<PP, G, p2m_half_mask<PP>(1), float, float4><<<nb, 64, 0, c.stream>>>(
+                k_fmm_p2m<PP, G, p2m_acc_half_mask<PP>(1), float, float4><<<nb, 64, 0, c.stream>>>(
                     NL, c.Tm, P.d_leaf_cell, P.d_cen, P.d_leaf_sb, P.d_leaf_se, (const float4 *)P.d_p2m_pos,
                     P.d_p2m_code, (const float2 *)P.d_p2m_tw, c.qe, M);
             } else {

@@ -163,6 +163,36 @@ __host__ __device__ constexpr unsigned p2m_half_mask(int h)
     }
     return h == 0 ? a : ((1u << (P + 1)) - 1) & ~a;
 }
+// The same split weighted by accumulators for p2m_accumulate2_cols, whose m = 0 column is real
+// (half the accumulators of a complex column of its length).
+template <int P>
+__host__ __device__ constexpr unsigned p2m_acc_half_mask(int h)
+{
+    unsigned a = 0;
+    int na = 0, nb = 0;  // accumulated reals
+    int order[P + 1] = {}, w[P + 1] = {};
+    for (int m = 0; m <= P; m++) {
+        order[m] = m;
+        w[m] = (m == 0 ? 1 : 2) * (P + 1 - m);
+    }
+    for (int i = 0; i <= P; i++)  // columns by decreasing weight (stable)
+        for (int j = i + 1; j <= P; j++)
+            if (w[order[j]] > w[order[i]]) {
+                const int t = order[i];
+                order[i] = order[j];
+                order[j] = t;
+            }
+    for (int i = 0; i <= P; i++) {
+        const int m = order[i];
+        if (na <= nb) {
+            a |= 1u << m;
+            na += w[m];
+        } else {
+            nb += w[m];
+        }
+    }
+    return h == 0 ? a : ((1u << (P + 1)) - 1) & ~a;
+}
 template <int P>
 __host__ __device__ constexpr int p2m_col_off(unsigned mask, int m)
 {
@@ -233,26 +263,50 @@ __host__ __device__ inline void p2m_accumulate_cols(double x, double y, double z
 // P2M accumulation of one source into both expansions at once, columns of MASK only: the unit
 // harmonics Q_n^m of the source (the recurrences of p2m_accumulate_cols started from Q_0^0 = 1)
 // times its charges qu (into au) and qv (into av). T is the arithmetic type (FP64, or FP32 in the
-// mixed-precision mode).
+// mixed-precision mode). The factors (2n - 1) z are formed once per source and shared by the
+// columns, ((n - 1)^2 - m^2) r^2 is one product by an integer constant, and the m = 0 column is
+// real (its imaginary accumulators are left untouched).
 template <int P, unsigned MASK, class T>
 __host__ __device__ inline void p2m_accumulate2_cols(T x, T y, T z, T qu, T qv, T (&au)[2 * p2m_half_count<P>(MASK)],
                                                      T (&av)[2 * p2m_half_count<P>(MASK)])
 {
-    const T r2 = x * x + y * y + z * z, z2 = z + z, r22 = r2 + r2;
-    T dre = T(1), dim = T(0);
-    T tm = T(1);  // -(2m - 1) once stepped
+    const T r2 = x * x + y * y + z * z, z2 = z + z;
+    T tz[P + 1];  // tz[n] = (2n - 1) z
+    tz[0] = T(0);
+    if (P >= 1) tz[1] = z;
+#pragma unroll
+    for (int n = 2; n <= P; n++) tz[n] = tz[n - 1] + z2;
     auto put = [&](int o, T re, T im) {
         au[o] += re * qu;
         au[o + 1] += im * qu;
         av[o] += re * qv;
         av[o + 1] += im * qv;
     };
+    if (MASK & 1u) {
+        const int o = 2 * p2m_col_off<P>(MASK, 0);
+        au[o] += qu;
+        av[o] += qv;
+        if (P >= 1) {
+            T p2 = T(1), p1 = z;
+            au[o + 2] += p1 * qu;
+            av[o + 2] += p1 * qv;
 #pragma unroll
-    for (int m = 0; m <= P; m++) {
+            for (int n = 2; n <= P; n++) {
+                const T c = tz[n] * p1 - (T((n - 1) * (n - 1)) * r2) * p2;
+                au[o + 2 * n] += c * qu;
+                av[o + 2 * n] += c * qv;
+                p2 = p1;
+                p1 = c;
+            }
+        }
+    }
+    T dre = -x, dim = -y;  // Q_1^1 = -(x + i y)
+#pragma unroll
+    for (int m = 1; m <= P; m++) {
         if ((MASK >> m) == 0) break;
-        if (m > 0) {
-            tm -= T(2);
-            const T nre = tm * (x * dre - y * dim), nim = tm * (x * dim + y * dre);
+        if (m > 1) {
+            const T f = T(-(2 * m - 1));
+            const T nre = f * (x * dre - y * dim), nim = f * (x * dim + y * dre);
             dre = nre;
             dim = nim;
         }
@@ -260,23 +314,18 @@ __host__ __device__ inline void p2m_accumulate2_cols(T x, T y, T z, T qu, T qv, 
         const int o = 2 * p2m_col_off<P>(MASK, m);
         put(o, dre, dim);
         if (m + 1 <= P) {
-            const T a1 = (m == 0 ? T(1) : T(2) - tm) * z;  // (2m+1) z
             T p2re = dre, p2im = dim;
-            T p1re = a1 * dre, p1im = a1 * dim;
+            T p1re = tz[m + 1] * dre, p1im = tz[m + 1] * dim;
             put(o + 2, p1re, p1im);
-            T az = a1 + z2;
-            T kr2 = T(2 * m + 1) * r2, dk = T(2 * m + 3) * r2;
 #pragma unroll
             for (int n = m + 2; n <= P; n++) {
-                const T cre = az * p1re - kr2 * p2re, cim = az * p1im - kr2 * p2im;
+                const T kr2 = T((n - 1) * (n - 1) - m * m) * r2;
+                const T cre = tz[n] * p1re - kr2 * p2re, cim = tz[n] * p1im - kr2 * p2im;
                 put(o + 2 * (n - m), cre, cim);
                 p2re = p1re;
                 p2im = p1im;
                 p1re = cre;
                 p1im = cim;
-                az += z2;
-                kr2 += dk;
-                dk += r22;
             }
         }
     }

@@ -246,7 +246,7 @@ __global__ __launch_bounds__(256) void k_guess_input(int N, const float *__restr
                                                      const int32_t *__restrict__ scol, const float *__restrict__ p,
                                                      float *__restrict__ xq)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = lblock() * blockDim.x + threadIdx.x;  // XCD order: neighbouring rows share an L2
     if (i >= N) return;
     float x[NX] = {};
     if (mag[i]) {
