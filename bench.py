@@ -92,16 +92,10 @@ def bench_dots(args):
     iters = []
 
     def step():
-        g.base_projection((2 / math.pi) * rng.uniform())
-        g.prepare_elements(H, dt, pref)
-        failed, info = g.solve(dt)
-        if failed:
+        rc, info, _, _ = g.step((2 / math.pi) * rng.uniform(), H, dt, pref)
+        if rc != 0:
             raise RuntimeError("solver failure")
         iters.append(info.iterations)
-        g.calc_demag()
-        g.energy(H)
-        g.evolution()
-        g.max_angle()
     for _ in range(args.warmup):
         step()
     iters.clear()
@@ -273,19 +267,16 @@ class Stepper:
         return (2 / math.pi) * self.rng.uniform()
 
     def step(self, r=None):
-        g = self.g
-        g.base_projection(self.angle() if r is None else r)
+        """base_projection, prepare_elements, solve, calc_demag, energy, evolution, max_angle: one
+        fg_step call (the same kernels as the separate entry points, one host read at the end)."""
+        from feellgood_amd import native as nv
+        r = self.angle() if r is None else r
         self.H = field(self.w, self.t)  # time-dependent for the wire's ramp
-        g.prepare_elements(self.H, self.dt, self.pref)
-        failed, info = g.solve(self.dt)
-        if failed:
+        rc, info, _, _ = self.g.step(r, self.H, self.dt, self.pref)
+        if rc == nv.FG_SOLVE_FAILED:
             self.fails += 1
             return info
         self.t += self.dt
-        g.calc_demag()
-        g.energy(self.H)
-        g.evolution()
-        g.max_angle()
         self.iters.append(info.iterations)
         return info
 

@@ -1181,6 +1181,39 @@ int fg_max_angle(fg_ctx *x, double *angle)
     });
 }
 
+int fg_step(fg_ctx *x, const double Hext[3], double dt, double prefactor, double r, double dumax,
+            fg_solve_info *info, double E[4], double *max_angle)
+{
+    return guarded(x, [&]() {
+        Ctx &c = x->c;
+        if (!Hext || !E || !max_angle) return FG_ERR_ARG;
+        int rc = fg_base_projection(x, r);
+        if (rc == FG_OK) rc = fg_prepare_elements(x, Hext, dt, prefactor);
+        fg_solve_info si{};
+        if (rc == FG_OK) rc = fg_solve(x, dt, &si);
+        if (info && (rc == FG_OK || rc == FG_SOLVE_FAILED)) *info = si;
+        if (rc != FG_OK) return rc;
+        if (dumax > 0 && dt * si.v_max > dumax) return FG_STEP_REJECTED;
+        // compute_all and max_angle, queued back to back and read back once
+        launch_charges(c);
+        if (c.opt.demag_method == FG_DEMAG_FMM) launch_fmm(c);
+        else launch_direct_p2p(c);
+        launch_energy(c, Hext);
+        launch_evolution(c);
+        x->basis_valid = false;
+        launch_max_angle(c);
+        HIPCHK(hipMemcpyAsync(c.h_part, c.res, 9 * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+        HIPCHK(hipStreamSynchronize(c.stream));
+        E[FG_E_EXCHANGE] = c.h_part[0];
+        E[FG_E_ANISOTROPY] = c.h_part[1] + c.h_part[4];
+        E[FG_E_DEMAG] = c.h_part[2] + c.h_part[5];
+        E[FG_E_ZEEMAN] = c.h_part[3];
+        *max_angle = std::acos(c.h_part[8]);
+        if (c.opt.profile) c.flush_stats();
+        return FG_OK;
+    });
+}
+
 int fg_average(fg_ctx *x, int which, double out[3])
 {
     return guarded(x, [&]() {

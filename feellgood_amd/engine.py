@@ -249,19 +249,21 @@ class Engine:
     def reset_kernel_stats(self):
         self.lib.fg_reset_kernel_stats(self._h)
 
-    def step(self, r, Hext, dt, prefactor):
-        """One LLG step of the reference loop (src/time_integration.cpp:193-231) at fixed dt:
-        base_projection, prepare_elements, solve, then compute_all (demag, energy, evolution).
-        Returns (failed, SolveInfo, E)."""
-        self.base_projection(r)
-        self.prepare_elements(Hext, dt, prefactor)
-        failed, info = self.solve(dt)
-        if failed:
-            return True, info, None
-        self.calc_demag()
-        E = self.energy(Hext)
-        self.evolution()
-        return False, info, E
+    def step(self, r, Hext, dt, prefactor, dumax=0.0):
+        """One step of the reference loop body (src/time_integration.cpp:193-242) in one call
+        (fg_step): base_projection(r), prepare_elements, solve, then, unless the solve failed or
+        dt * v_max > dumax (dumax <= 0: no limit), compute_all (demag, energy, evolution) and
+        max_angle. Returns (status, SolveInfo, E, max_angle): status FG_OK, FG_SOLVE_FAILED or
+        FG_STEP_REJECTED; E and max_angle are None unless the step was taken."""
+        H = np.ascontiguousarray(Hext, dtype=np.float64)
+        info = nv.SolveInfo()
+        E = np.zeros(4)
+        a = C.c_double()
+        rc = self._chk(self.lib.fg_step(self._h, nv.dptr(H), dt, prefactor, r, dumax, C.byref(info), nv.dptr(E),
+                                        C.byref(a)), "fg_step")
+        if rc != nv.FG_OK:
+            return rc, info, None, None
+        return rc, info, E, a.value
 
 
 class SpinSolver:

@@ -17,6 +17,7 @@ HOST_LIB = os.path.join(LIB_DIR, "libfgmi_host.so")
 
 FG_OK = 0
 FG_SOLVE_FAILED = 1
+FG_STEP_REJECTED = 2
 FG_ERR_ARG, FG_ERR_DEVICE, FG_ERR_MESH, FG_ERR_STATE, FG_ERR_RANGE = -1, -2, -3, -4, -5
 FG_ITER_UNDEFINED, FG_ITER_CONVERGED, FG_ITER_OVERFLOW, FG_ITER_CANNOT_CONVERGE = -1, 0, 1, 2
 FG_CURRENT, FG_NEXT, FG_BOTH = 0, 1, 2
@@ -115,6 +116,8 @@ HIP_SYMBOLS = {
                                                      C.POINTER(SolveInfo)]),
     "fg_evolution": (C.c_int, [C.c_void_p]),
     "fg_max_angle": (C.c_int, [C.c_void_p, _d]),
+    "fg_step": (C.c_int, [C.c_void_p, _d, C.c_double, C.c_double, C.c_double, C.c_double, C.POINTER(SolveInfo), _d,
+                          _d]),
     "fg_average": (C.c_int, [C.c_void_p, C.c_int, _d]),
     "fg_get_basis": (C.c_int, [C.c_void_p, _d, _d]),
     "fg_system_size": (C.c_int, [C.c_void_p, _i64, _i64]),

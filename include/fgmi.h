@@ -20,6 +20,8 @@
  *   fg_energy             Fem::energy (src/fem.h:152, src/energy.cpp:5-68)
  *   fg_evolution          Fem::evolution (src/fem.h:159-163)
  *   fg_max_angle          Mesh::mesh::max_angle (src/mesh.h:295-306)
+ *   fg_step               the loop body of Fem::time_integration for a uniform field without spin
+ *                         accumulation (src/time_integration.cpp:193-242): the calls above in order
  *   fg_average            Mesh::mesh::avg (src/mesh.cpp:89-106)
  *   fg_set_space_field    Mesh::mesh::setExtSpaceField (src/mesh.cpp:377-389), with
  *                         fg_prepare_elements_amp / fg_energy_amp for the R4toR3 field
@@ -46,6 +48,7 @@ extern "C" {
 /* ---- status codes (int return value of every entry point) ---- */
 #define FG_OK 0
 #define FG_SOLVE_FAILED 1     /* fg_solve: the Krylov solve did not converge (reference returns true) */
+#define FG_STEP_REJECTED 2    /* fg_step: dt * v_max above dumax, the step is not taken */
 #define FG_ERR_ARG (-1)       /* invalid argument */
 #define FG_ERR_DEVICE (-2)    /* HIP runtime error */
 #define FG_ERR_MESH (-3)      /* degenerate element or inconsistent mesh */
@@ -319,6 +322,16 @@ int fg_spin_destroy(fg_spin *sp);
 int fg_evolution(fg_ctx *ctx);
 /* acos(min over magnetic edges of u_i . u_j) on the NEXT state. */
 int fg_max_angle(fg_ctx *ctx, double *angle);
+/* One step of the reference's time loop body for a uniform applied field Hext [A/m] and no spin
+ * accumulation (src/time_integration.cpp:193-242): base_projection(r), prepareElements(Hext, dt),
+ * solve and get_v_max; then, when the solve succeeded and dt * v_max <= dumax (dumax <= 0: no
+ * limit), compute_all (calc_demag, energy, evolution; src/fem.h:203-224) and max_angle. The same
+ * kernels as the separate calls, with one host read of the results at the end instead of one per
+ * call. Returns FG_OK (step taken: info, E[4] of the new state, *max_angle), FG_SOLVE_FAILED (the
+ * solve failed, nothing evolved: the reference retries with a smaller dt), FG_STEP_REJECTED
+ * (dt * v_max > dumax: NEXT holds the solution, nothing evolved) or an error. */
+int fg_step(fg_ctx *ctx, const double Hext[3], double dt, double prefactor, double r, double dumax,
+            fg_solve_info *info, double E[4], double *max_angle);
 /* Volume average of u NEXT (which = 0) or v NEXT (which = 1) over the magnetic tetrahedra. */
 int fg_average(fg_ctx *ctx, int which, double out[3]);
 

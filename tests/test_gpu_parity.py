@@ -365,6 +365,46 @@ def test_bitwise_deterministic(small):
     assert np.array_equal(outs[0][0], outs[1][0]) and np.array_equal(outs[0][1], outs[1][1])
 
 
+@pytest.mark.parametrize("demag", ["direct", "fmm"])
+def test_step_entry_point_matches_the_separate_calls(small, demag):
+    """fg_step, the loop body in one call, against base_projection, prepare_elements, solve,
+    calc_demag, energy, evolution and max_angle called one by one on a second engine: the same
+    states to the bit, the same energies, max angle and iteration counts over three steps. A
+    dumax below dt v_max rejects the step: nothing evolves (CURRENT unchanged, NEXT holds the
+    solution), as in the reference's loop (src/time_integration.cpp:215-222)."""
+    u, v = state(small)
+    H = np.array([1e4, 2e4, 0.0])
+    dt, pref = 1e-14, 1.04
+    kw = dict(tol=1e-10, demag=demag, fmm_order=8, fmm_leaf=16)
+    a, b = engine(small.mesh, **kw), engine(small.mesh, **kw)
+    for g in (a, b):
+        g.set_state(nv.FG_BOTH, u=u, v=v)
+        g.calc_demag()
+        g.evolution()
+    for r in (0.5, 0.2, 0.9):
+        rc, info, E, ang = a.step(r, H, dt, pref)
+        assert rc == nv.FG_OK
+        b.base_projection(r)
+        b.prepare_elements(H, dt, pref)
+        failed, info_b = b.solve(dt)
+        assert not failed and info.iterations == info_b.iterations and info.v_max == info_b.v_max
+        b.calc_demag()
+        E_b = b.energy(H)
+        b.evolution()
+        ang_b = b.max_angle()
+        assert np.array_equal(E, E_b) and ang == ang_b
+        for which in (nv.FG_CURRENT, nv.FG_NEXT):
+            for x, y in zip(a.get_state(which), b.get_state(which)):
+                assert np.array_equal(x, y)
+    cur = a.get_state(nv.FG_CURRENT)[0]
+    rc, info, E, ang = a.step(0.3, H, dt, pref, dumax=0.5 * dt * info.v_max)
+    assert rc == nv.FG_STEP_REJECTED and E is None and info.v_max > 0
+    assert np.array_equal(a.get_state(nv.FG_CURRENT)[0], cur)
+    assert not np.array_equal(a.get_state(nv.FG_NEXT)[0], cur)
+    a.close()
+    b.close()
+
+
 def test_ellipsoid_fixture_step(golden_dir):
     """One step of the full_test.py settings on examples/ellipsoid.msh, engine vs oracle."""
     m = HostMesh.read(os.path.join(golden_dir, "ellipsoid.msh"))
