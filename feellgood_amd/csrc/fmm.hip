@@ -70,6 +70,7 @@ struct FmmPlan {
     // launch schedules: M2M parents per level with their children by octant (-1: none); L2L
     // parents per level with the children that receive a local expansion
     void *d_Tm2m = nullptr, *d_Tl2l = nullptr;  // FP64, or FP32 in the mixed-precision mode
+    void *d_Tm2m_pair = nullptr;  // FP32 mode: M2M in the k order of k_fmm_m2m_all's pair loads
     // M2L on the matrix cores: the operator of every offset code (fragment order), the target
     // cells in tiles of 64 (one octant class per tile), each tile's offset codes in chunks of up
     // to M2L_CHUNK with the source cell of every (offset, target) (-1: none), and the partial sums
@@ -946,6 +947,12 @@ template <int P> struct XL {
     {
         return K == 0 ? (2 * fcount(deg(16 * R + 15 < NR ? 16 * R + 15 : NR - 1)) + 3) / 4 : NKS;
     }
+    // Pair order (FP32 M2M of k_fmm_m2m_all): lane group g of k-step q takes input real
+    // kpair(q, g) = 8 (q / 2) + 2 g + q % 2, so one 8-byte load per lane and 8-real block feeds two
+    // k-steps; NKS2 steps cover the NR reals, ke2(R) those M2M row tile R needs
+    static constexpr int NKS2 = 2 * ((NR + 7) / 8);
+    static constexpr int kpair(int q, int g) { return 8 * (q / 2) + 2 * g + q % 2; }
+    static constexpr int ke2(int R) { return 2 * ((ke(0, R) + 1) / 2); }
 };
 
 // One block per (16-column tile, charge set, row tile); a column is a parent cell, and wavefront
@@ -1052,13 +1059,14 @@ __global__ __launch_bounds__(256) void k_fmm_xlate(int n, const int32_t *__restr
 
 // Row tiles R.. of the M2M below: acc[R] += T_o[R rows] child_o for the wavefront's two octants,
 // over the k-steps of the input degrees that row tile's output degrees take.
-template <int P, int R, class F>
-__device__ __forceinline__ void m2m_rows(const F *__restrict__ A, const F (&bv)[2][XL<P>::NKS],
+template <int P, int R, class F, int NK>
+__device__ __forceinline__ void m2m_rows(const F *__restrict__ A, const F (&bv)[2][NK],
                                          typename Mfma<F>::V (&acc)[XL<P>::NT])
 {
     if constexpr (R < XL<P>::NT) {
         using L = XL<P>;
-        constexpr int NKS = L::NKS, ke = L::ke(0, R);
+        constexpr bool PAIR = NK == L::NKS2 && NK != L::NKS;
+        constexpr int NKS = NK, ke = PAIR ? L::ke2(R) : L::ke(0, R);
         F av[2][ke];
 #pragma unroll
         for (int oo = 0; oo < 2; oo++)
@@ -1068,7 +1076,7 @@ __device__ __forceinline__ void m2m_rows(const F *__restrict__ A, const F (&bv)[
         for (int oo = 0; oo < 2; oo++)
 #pragma unroll
             for (int ks = 0; ks < ke; ks++) acc[R] = Mfma<F>::mma(av[oo][ks], bv[oo][ks], acc[R]);
-        m2m_rows<P, R + 1, F>(A, bv, acc);
+        m2m_rows<P, R + 1, F, NK>(A, bv, acc);
     }
 }
 
@@ -1095,21 +1103,36 @@ __global__ __launch_bounds__(256) void k_fmm_m2m_all(int n, const int32_t *__res
     const int col = l & 15, kq = l >> 4;
     const int pi = ct * 16 + col;
     const bool act = pi < n;
-    F bv[2][NKS];
+    // FP32: pair order, one 8-byte load per 8-real block (NR is even and each expansion starts on
+    // an 8-byte boundary); FP64: one real per k-step
+    constexpr bool PAIR = std::is_same<F, float>::value;
+    constexpr int NK = PAIR ? L::NKS2 : NKS;
+    F bv[2][NK];
 #pragma unroll
     for (int oo = 0; oo < 2; oo++) {
         const int ch = act ? child[(size_t)pi * 8 + 2 * w + oo] : -1;
         const F *xs = X + ((size_t)(ch < 0 ? 0 : ch) * 2 + s) * NR;
+        if constexpr (PAIR) {
+            static_assert(NR % 2 == 0, "pairs never straddle the end of an expansion");
 #pragma unroll
-        for (int ks = 0; ks < NKS; ks++) {
-            const int k = 4 * ks + kq;
-            bv[oo][ks] = ch >= 0 && k < NR ? xs[k] : F(0);
+            for (int j = 0; j < NK / 2; j++) {
+                const int k = 8 * j + 2 * kq;
+                const float2 v = ch >= 0 && k < NR ? reinterpret_cast<const float2 *>(xs)[k / 2] : make_float2(0.f, 0.f);
+                bv[oo][2 * j] = v.x;
+                bv[oo][2 * j + 1] = v.y;
+            }
+        } else {
+#pragma unroll
+            for (int ks = 0; ks < NKS; ks++) {
+                const int k = 4 * ks + kq;
+                bv[oo][ks] = ch >= 0 && k < NR ? xs[k] : F(0);
+            }
         }
     }
     V acc[NT];
 #pragma unroll
     for (int R = 0; R < NT; R++) acc[R] = V{F(0), F(0), F(0), F(0)};
-    m2m_rows<P, 0, F>(Top + (size_t)2 * w * NT * NKS * 64 + l, bv, acc);
+    m2m_rows<P, 0, F, NK>(Top + (size_t)2 * w * NT * NK * 64 + l, bv, acc);
 #pragma unroll
     for (int R = 0; R < NT; R++) red[R][w][l] = acc[R];
     __syncthreads();
@@ -1655,11 +1678,12 @@ static int64_t m2p_lane_slots(const FmmPlan &P)
 // order k_fmm_xlate reads; built by applying the reference translations (m2m_coef, l2l_coef) to
 // unit expansions, so device and host translations are the same linear maps.
 template <int P>
-static std::vector<double> xlate_pack(const FmmPlan &Pl, int K)
+static std::vector<double> xlate_pack(const FmmPlan &Pl, int K, bool pair)
 {
     using L = XL<P>;
-    constexpr int NR = L::NR, NCo = L::NCo, NT = L::NT, NKS = L::NKS;
-    std::vector<double> out((size_t)L::frag_len(), 0.0), T((size_t)NR * NR);
+    constexpr int NR = L::NR, NCo = L::NCo, NT = L::NT;
+    const int NKS = pair ? L::NKS2 : L::NKS;
+    std::vector<double> out((size_t)8 * NT * NKS * 64, 0.0), T((size_t)NR * NR);
     std::vector<double2> e(NCo);
     for (int o = 0; o < 8; o++) {
         const double2 *Rd = &Pl.Rc[(size_t)o * Pl.nc];
@@ -1676,20 +1700,20 @@ static std::vector<double> xlate_pack(const FmmPlan &Pl, int K)
         for (int R = 0; R < NT; R++)
             for (int ks = 0; ks < NKS; ks++)
                 for (int l = 0; l < 64; l++) {
-                    const int r = 16 * R + (l & 15), k = 4 * ks + (l >> 4);
+                    const int r = 16 * R + (l & 15), k = pair ? L::kpair(ks, l >> 4) : 4 * ks + (l >> 4);
                     out[(((size_t)o * NT + R) * NKS + ks) * 64 + l] = r < NR && k < NR ? T[(size_t)r * NR + k] : 0.0;
                 }
     }
     return out;
 }
 
-static std::vector<double> xlate_pack(const FmmPlan &Pl, int K)
+static std::vector<double> xlate_pack(const FmmPlan &Pl, int K, bool pair = false)
 {
     switch (Pl.p) {
-    case 4: return xlate_pack<4>(Pl, K);
-    case 6: return xlate_pack<6>(Pl, K);
-    case 8: return xlate_pack<8>(Pl, K);
-    default: return xlate_pack<10>(Pl, K);
+    case 4: return xlate_pack<4>(Pl, K, pair);
+    case 6: return xlate_pack<6>(Pl, K, pair);
+    case 8: return xlate_pack<8>(Pl, K, pair);
+    default: return xlate_pack<10>(Pl, K, pair);
     }
 }
 
@@ -1906,6 +1930,7 @@ int fmm_create(Ctx &c)
         };
         const size_t fsz = P->mixed ? sizeof(float) : sizeof(double);
         P->d_Tm2m = put_op(xlate_pack(*P, 0));
+        if (P->mixed) P->d_Tm2m_pair = put_op(xlate_pack(*P, 0, true));
         P->d_Tl2l = put_op(xlate_pack(*P, 1));
         xlate_schedules(*P);
         P->d_Tm2l = put_op(m2l_pack(*P));
@@ -2102,7 +2127,7 @@ void fmm_destroy(Ctx &c)
     if (!P) return;
     void *ptrs[] = {P->d_cen, P->d_m2p_gid, P->d_leaf_sb, P->d_leaf_se, P->d_grp_tb, P->d_grp_te, P->d_grp_lcell, P->d_u_ptr,
                     P->d_u_far, P->d_u_cell, P->d_u_mask, P->d_tgt_oct, P->d_tgt_node, P->d_tgt_pos, P->d_tgt_lc, P->d_far,
-                    P->d_Tm2m, P->d_Tl2l, P->d_Tm2l, P->d_m2l_part, P->d_m2l_tile_cell, P->d_m2l_tile_chunk,
+                    P->d_Tm2m, P->d_Tm2m_pair, P->d_Tl2l, P->d_Tm2l, P->d_m2l_part, P->d_m2l_tile_cell, P->d_m2l_tile_chunk,
                     P->d_m2l_chunk_code, P->d_m2l_chunk_src, P->d_m2m_child, P->d_l2l_par, P->d_l2l_child, P->d_M, P->d_L, P->d_leaf_cell, P->d_m2m_cells,
                     P->d_nf_ptr, P->d_nf_val, P->d_p2m_pos, P->d_p2m_tw, P->d_p2m_code,
                     P->d_nf_valf, P->d_tl_tb, P->d_tl_te, P->d_lc_ptr, P->d_lc_col, P->d_pot, c.qe};
@@ -2174,7 +2199,7 @@ static void run_fmm(Ctx &c)
             if (n >= M2M_ALL_MIN)
                 k_fmm_m2m_all<PP, F><<<(n + 15) / 16 * 2, 256, 0, c.stream>>>(
                     n, P.d_m2m_cells + P.m2m_lev_off[l], P.d_m2m_child + (size_t)8 * P.m2m_lev_off[l],
-                    (const F *)P.d_Tm2m, M);
+                    (const F *)(std::is_same<F, float>::value ? P.d_Tm2m_pair : P.d_Tm2m), M);
             else
                 k_fmm_xlate<PP, 0, F><<<(n + 15) / 16 * 2 * XL<PP>::NT, 256, 0, c.stream>>>(
                     n, P.d_m2m_cells + P.m2m_lev_off[l], P.d_m2m_child + (size_t)8 * P.m2m_lev_off[l],
