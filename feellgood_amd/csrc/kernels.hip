@@ -39,6 +39,36 @@ __device__ inline double wave_min(double v)
     return v;
 }
 
+// This lane's share (p[lane], p[lane + 64], ...) of nb partials, combined by OP (a deterministic
+// order): eight independent accumulators keep eight loads in flight instead of one after another.
+template <class OP>
+__device__ inline double lane_fold(const double *__restrict__ p, int nb, double init, OP op)
+{
+    double a0 = init, a1 = init, a2 = init, a3 = init, a4 = init, a5 = init, a6 = init, a7 = init;
+    int i = threadIdx.x & 63;
+    for (; i + 7 * 64 < nb; i += 8 * 64) {
+        a0 = op(a0, p[i]);
+        a1 = op(a1, p[i + 64]);
+        a2 = op(a2, p[i + 128]);
+        a3 = op(a3, p[i + 192]);
+        a4 = op(a4, p[i + 256]);
+        a5 = op(a5, p[i + 320]);
+        a6 = op(a6, p[i + 384]);
+        a7 = op(a7, p[i + 448]);
+    }
+    for (; i < nb; i += 64) a0 = op(a0, p[i]);
+    return op(op(op(a0, a1), op(a2, a3)), op(op(a4, a5), op(a6, a7)));
+}
+struct AddOp {
+    __device__ double operator()(double a, double b) const { return a + b; }
+};
+struct MaxOp {
+    __device__ double operator()(double a, double b) const { return fmax(a, b); }
+};
+struct MinOp {
+    __device__ double operator()(double a, double b) const { return fmin(a, b); }
+};
+
 // Sum of NV per-thread values over the block, stored to part[slot[k] * MAX_PART + blk], blk being
 // the block index the kernel assigned its items by (blockIdx.x or lblock()).
 template <int NV>
@@ -70,9 +100,7 @@ __device__ inline void block_load(const double *part, const int (&slot)[NV], int
     if (threadIdx.x < 64) {
 #pragma unroll
         for (int k = 0; k < NV; k++) {
-            double s = 0;
-            for (int i = threadIdx.x; i < nb; i += 64) s += part[slot[k] * MAX_PART + i];
-            s = wave_sum(s);
+            const double s = wave_sum(lane_fold(part + slot[k] * MAX_PART, nb, 0.0, AddOp()));
             if (threadIdx.x == 0) res[k] = s;
         }
     }
@@ -1258,9 +1286,7 @@ __global__ void k_finish_update(int nb, KrylovCtl *ctl, const double *__restrict
 {
     if (gated && !solved(ctl)) return;
     if (threadIdx.x < 64) {
-        double m = 0;
-        for (int i = threadIdx.x; i < nb; i += 64) m = fmax(m, part[7 * MAX_PART + i]);
-        m = wave_max(m);
+        const double m = wave_max(lane_fold(part + 7 * MAX_PART, nb, 0.0, MaxOp()));
         if (threadIdx.x == 0) ctl->vmax2 = m;
     }
 }
@@ -1567,9 +1593,7 @@ __global__ void k_sum_finish(int nslots, int nfirst, int nb_a, int nb_b, const d
     if (threadIdx.x < 64) {
         for (int k = 0; k < nslots; k++) {
             const int nb = k < nfirst ? nb_a : nb_b;
-            double s = 0;
-            for (int i = threadIdx.x; i < nb; i += 64) s += part[k * MAX_PART + i];
-            s = wave_sum(s);
+            const double s = wave_sum(lane_fold(part + k * MAX_PART, nb, 0.0, AddOp()));
             if (threadIdx.x == 0) out[k] = s;
         }
     }
@@ -1607,9 +1631,7 @@ __global__ __launch_bounds__(BLOCK) void k_max_angle(int N, const int64_t *__res
 __global__ void k_min_finish(int nb, const double *__restrict__ part, double *__restrict__ out)
 {
     if (threadIdx.x < 64) {
-        double m = 1.0;
-        for (int i = threadIdx.x; i < nb; i += 64) m = fmin(m, part[i]);
-        m = wave_min(m);
+        const double m = wave_min(lane_fold(part, nb, 1.0, MinOp()));
         if (threadIdx.x == 0) out[0] = m;
     }
 }
