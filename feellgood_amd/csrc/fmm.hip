@@ -2147,26 +2147,12 @@ static void run_fmm(Ctx &c)
     const int NL = (int)P.leaf_cell.size();
     F *M = (F *)P.d_M, *Lx = (F *)P.d_L;
     // The near field reads the element values and corrections and writes potentials that only the
-    // final sum reads: it runs on the second stream beside P2M and M2M (memory latency against VALU
-    // and matrix-core work), and M2L and L2L follow it there beside M2P. Profiled runs keep one
-    // stream, so each kernel's time is its own.
+    // final sum reads, and M2L and L2L need the multipoles only: once M2M is done all three run on
+    // the second stream beside M2P (memory latency and matrix-core work against VALU issue). Beside
+    // P2M and M2M the near field slowed them more than it gained. Profiled runs keep one stream, so
+    // each kernel's time is its own.
     const bool fork = !c.opt.profile && c.aux;
     hipStream_t ns = fork ? c.aux : c.stream;
-    if (fork) {
-        if (hipEventRecord(c.ev_fork, c.stream) != hipSuccess || hipStreamWaitEvent(c.aux, c.ev_fork, 0) != hipSuccess)
-            throw std::string("hipEventRecord / hipStreamWaitEvent (near-field fork)");
-    }
-    {
-        Prof pr(c, "fmm_near");
-        const unsigned nb = (unsigned)(((int64_t)P.ntl * 64 + 255) / 256);
-        if (P.ntl > 0 && P.d_nf_valf)
-            k_nf_apply<float><<<nb, 256, 0, ns>>>(P.ntl, P.d_tl_tb, P.d_tl_te, P.d_lc_ptr, P.d_lc_col, P.d_nf_ptr,
-                                                  P.d_nf_valf, c.qe, P.d_tgt_node, c.corr, P.d_pot);
-        else if (P.ntl > 0)
-            k_nf_apply<double><<<nb, 256, 0, ns>>>(P.ntl, P.d_tl_tb, P.d_tl_te, P.d_lc_ptr, P.d_lc_col, P.d_nf_ptr,
-                                                   P.d_nf_val, c.qe, P.d_tgt_node, c.corr, P.d_pot);
-    }
-
     {
         Prof pr(c, "fmm_p2m");
         constexpr int G = 8;
@@ -2206,12 +2192,23 @@ static void run_fmm(Ctx &c)
                     (const F *)P.d_Tm2m, M);
         }
     }
-    // the downward pass (M2L, L2L) on the second stream once the multipoles are complete, beside
-    // M2P, which reads only the multipoles (matrix cores and latency against VALU issue)
+    // the near field and the downward pass (M2L, L2L) on the second stream once the multipoles are
+    // complete, beside M2P
     if (fork) {
         if (hipEventRecord(c.ev_mid, c.stream) != hipSuccess || hipStreamWaitEvent(c.aux, c.ev_mid, 0) != hipSuccess)
             throw std::string("hipEventRecord / hipStreamWaitEvent (downward-pass fork)");
     }
+    {
+        Prof pr(c, "fmm_near");
+        const unsigned nb = (unsigned)(((int64_t)P.ntl * 64 + 255) / 256);
+        if (P.ntl > 0 && P.d_nf_valf)
+            k_nf_apply<float><<<nb, 256, 0, ns>>>(P.ntl, P.d_tl_tb, P.d_tl_te, P.d_lc_ptr, P.d_lc_col, P.d_nf_ptr,
+                                                  P.d_nf_valf, c.qe, P.d_tgt_node, c.corr, P.d_pot);
+        else if (P.ntl > 0)
+            k_nf_apply<double><<<nb, 256, 0, ns>>>(P.ntl, P.d_tl_tb, P.d_tl_te, P.d_lc_ptr, P.d_lc_col, P.d_nf_ptr,
+                                                   P.d_nf_val, c.qe, P.d_tgt_node, c.corr, P.d_pot);
+    }
+
     {
         Prof pr(c, "fmm_m2l");
         if (P.m2l_nchunks > 0)
