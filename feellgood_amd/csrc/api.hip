@@ -536,11 +536,10 @@ void free_all(Ctx &c)
         (void)hipEventDestroy(pnd.b);
     }
     for (auto e : c.event_pool) (void)hipEventDestroy(e);
-    if (c.ev_fork) (void)hipEventDestroy(c.ev_fork);
     if (c.ev_mid) (void)hipEventDestroy(c.ev_mid);
     if (c.ev_join) (void)hipEventDestroy(c.ev_join);
     if (c.aux) (void)hipStreamDestroy(c.aux);
-    c.ev_fork = c.ev_mid = c.ev_join = nullptr;
+    c.ev_mid = c.ev_join = nullptr;
     c.aux = nullptr;
 }
 
@@ -661,7 +660,6 @@ int create_impl(const fg_mesh *mesh, const fg_options *opt, fg_ctx **out, const 
         HIPCHK(hipSetDevice(c.device));
         HIPCHK(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
         HIPCHK(hipStreamCreateWithFlags(&c.aux, hipStreamNonBlocking));
-        HIPCHK(hipEventCreateWithFlags(&c.ev_fork, hipEventDisableTiming));
         HIPCHK(hipEventCreateWithFlags(&c.ev_mid, hipEventDisableTiming));
         HIPCHK(hipEventCreateWithFlags(&c.ev_join, hipEventDisableTiming));
         build_plan(c, mesh);

@@ -22,10 +22,10 @@ struct KernelStat {
 struct Ctx {
     int device = 0;
     hipStream_t stream = nullptr;
-    // a second stream (default priority) on which the FMM near field runs beside the upward pass,
-    // joined before the far-field evaluation reads its potentials; fork / join events
+    // a second stream (default priority) on which the FMM's near field, M2L and L2L run beside M2P:
+    // ev_mid (multipoles complete) forks it, ev_join joins it before the final per-target sum
     hipStream_t aux = nullptr;
-    hipEvent_t ev_fork = nullptr, ev_mid = nullptr, ev_join = nullptr;
+    hipEvent_t ev_mid = nullptr, ev_join = nullptr;
     fg_options opt{};
     std::string err;
 
