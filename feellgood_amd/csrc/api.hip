@@ -1070,16 +1070,28 @@ int fg_solve(fg_ctx *x, double dt, fg_solve_info *info)
         launch_r0(c, cst);
         // BiCGSTAB: batches of iterations, the device control block says when to stop. Iteration
         // n's convergence is tested at the start of iteration n + 1, so the first batch is the
-        // previous solve's count plus one: a solve that converges like the last one needs a single
-        // round trip to the host.
+        // previous solve's count plus that test: a solve that converges like the last one needs a
+        // single round trip to the host.
         // Each batch is followed by the node update gated on the control block, so that a solve
         // accepted in the batch is finished by the time the host reads it (no second round trip);
         // profiled runs update after the read, so the update's statistics hold real launches only.
+        // A batch ends with the first phase of its next iteration, the test that may end the solve;
+        // the next batch starts with that iteration's other four phases.
         const bool gated = !c.opt.profile;
-        int launched = 0;
+        int launched = 0;   // iterations queued
+        bool open = false;  // the last batch ended with the first phase of iteration launched + 1
         int batch = std::max(2, std::min(x->last_nit + 1, c.opt.max_iter));
         while (true) {
-            for (int k = 0; k < batch && launched <= c.opt.max_iter; k++, launched++) launch_iteration(c, cst);
+            if (open) {
+                launch_iteration(c, cst, 2);
+                launched++;
+                open = false;
+            }
+            for (int k = 0; k + 1 < batch && launched <= c.opt.max_iter; k++, launched++) launch_iteration(c, cst);
+            if (launched <= c.opt.max_iter) {
+                launch_iteration(c, cst, 1);
+                open = true;
+            }
             if (gated) launch_update(c, dt, true);
             HIPCHK(hipMemcpyAsync(c.h_ctl, c.ctl, sizeof(KrylovCtl), hipMemcpyDeviceToHost, c.stream));
             HIPCHK(hipStreamSynchronize(c.stream));

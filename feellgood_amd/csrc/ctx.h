@@ -176,7 +176,9 @@ void launch_element(Ctx &c, const double Hext[3], double dt, const double *hsp =
 void launch_basis(Ctx &c, double r);
 void launch_node_assemble(Ctx &c, double r, double c_stencil, double dt);
 void launch_r0(Ctx &c, double c_stencil);
-void launch_iteration(Ctx &c, double c_stencil);
+// one BiCGSTAB iteration (part 0), its first phase alone (1: the convergence test of the previous
+// iteration) or the other four (2)
+void launch_iteration(Ctx &c, double c_stencil, int part = 0);
 // mixed-precision Krylov: FP32 copies and count reset; FP64 residual and outer test; one FP32
 // iteration; x += d
 void launch_mixed_start(Ctx &c);

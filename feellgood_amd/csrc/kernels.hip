@@ -1749,13 +1749,14 @@ void launch_r0(Ctx &c, double cst)
     k_r0<<<nb, BLOCK, 0, c.stream>>>(c.N, cst, STENCIL_PASS, c.w, c.x, c.b, c.r, c.rt, c.p, c.ctl, c.part);
 }
 
-void launch_iteration(Ctx &c, double cst)
+void launch_iteration(Ctx &c, double cst, int part)
 {
     const int nb = c.blocks_for(c.N);
-    {
+    if (part != 2) {
         Prof pr(c, "iter_p");
         k_iter_p<<<nb, BLOCK, 0, c.stream>>>(c.N, nb, c.ctl, c.ep4, c.eq4, c.dprec, c.r, c.v, c.p, c.ph, c.w, c.part);
     }
+    if (part == 1) return;
     {
         Prof pr(c, "spmv");
         k_spmv_v<<<nb, BLOCK, 0, c.stream>>>(c.N, cst, STENCIL_PASS, c.ctl, c.w, c.ph, c.rt, c.v, c.part);
