@@ -766,14 +766,24 @@ int run(const Config &c, unsigned seed, bool verbose, bool check_only, const std
                 goto bailout;
             }
             const double r_attempt = next_angle();
-            e.check(fg_base_projection(e.ctx, r_attempt), "fg_base_projection");
             field_at(t);  // the field at the step's start, also used by its energies (src/time_integration.cpp:194-231)
-            e.check(spacef ? fg_prepare_elements_amp(e.ctx, amp, dt, prefactor)
-                           : fg_prepare_elements(e.ctx, Hext, dt, prefactor),
-                    "fg_prepare_elements");
+            // a uniform field without spin accumulation: the loop body in one call (fg_step), one
+            // host read at its end; otherwise the calls one by one
+            const bool one_call = !spacef && !c.spin_acc && c.du_max > 0;  // fg_step's dumax test is this loop's
             fg_solve_info info;
-            const int rc = fg_solve(e.ctx, dt, &info);
-            e.check(rc, "fg_solve");
+            double E_step[4] = {0, 0, 0, 0}, a_step = 0;
+            int rc;
+            if (one_call) {
+                rc = fg_step(e.ctx, Hext, dt, prefactor, r_attempt, c.du_max, &info, E_step, &a_step);
+                e.check(rc, "fg_step");
+            } else {
+                e.check(fg_base_projection(e.ctx, r_attempt), "fg_base_projection");
+                e.check(spacef ? fg_prepare_elements_amp(e.ctx, amp, dt, prefactor)
+                               : fg_prepare_elements(e.ctx, Hext, dt, prefactor),
+                        "fg_prepare_elements");
+                rc = fg_solve(e.ctx, dt, &info);
+                e.check(rc, "fg_solve");
+            }
             vmax = info.v_max;
             total_iter += info.iterations;
             if (rc == FG_SOLVE_FAILED) {
@@ -792,13 +802,17 @@ int run(const Config &c, unsigned seed, bool verbose, bool check_only, const std
                 continue;
             }
             log_attempt(r_attempt, 0, info.iterations);
-            compute_all();
+            double a = a_step;
+            if (one_call) {
+                for (int k = 0; k < 4; k++) E[k] = E_step[k];
+            } else {
+                compute_all();
+                e.check(fg_max_angle(e.ctx, &a), "fg_max_angle");
+            }
             nt++;
             flag = 0;
             if (last_step) t = t_target;
             else t += dt;
-            double a;
-            e.check(fg_max_angle(e.ctx, &a), "fg_max_angle");
             max_angle = std::max(max_angle, a);
         }
         field_at(t);  // the saved applied field is the one at the output time (src/save.cpp:22-24)
