@@ -754,11 +754,13 @@ __device__ __forceinline__ double dpp_mov(double v)
     const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xf, 0xf, true);
     return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
 }
-__device__ __forceinline__ double sum8_dpp(double v)
+template <int G>
+__device__ __forceinline__ double sum_dpp(double v)
 {
-    v += dpp_mov<0xB1>(v);   // quad_perm [1,0,3,2]
-    v += dpp_mov<0x4E>(v);   // quad_perm [2,3,0,1]
-    v += dpp_mov<0x141>(v);  // row_half_mirror
+    static_assert(G == 4 || G == 8, "sums over aligned groups of 4 or 8 lanes");
+    v += dpp_mov<0xB1>(v);                     // quad_perm [1,0,3,2]
+    v += dpp_mov<0x4E>(v);                     // quad_perm [2,3,0,1]
+    if constexpr (G == 8) v += dpp_mov<0x141>(v);  // row_half_mirror
     return v;
 }
 template <int CTRL>
@@ -766,11 +768,13 @@ __device__ __forceinline__ float dpp_mov(float v)
 {
     return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xf, 0xf, true));
 }
-__device__ __forceinline__ float sum8_dpp(float v)
+template <int G>
+__device__ __forceinline__ float sum_dpp(float v)
 {
+    static_assert(G == 4 || G == 8, "sums over aligned groups of 4 or 8 lanes");
     v += dpp_mov<0xB1>(v);
     v += dpp_mov<0x4E>(v);
-    v += dpp_mov<0x141>(v);
+    if constexpr (G == 8) v += dpp_mov<0x141>(v);
     return v;
 }
 
@@ -781,8 +785,8 @@ __device__ __forceinline__ float sum8_dpp(float v)
 // weights dMs w_g lambda_a(g)). FP64 (S4 = double4, positions absolute, F = double) or FP32 (S4 =
 // float4, positions relative to the leaf centre in cell widths and weights times mscale,
 // F = float). G lanes per leaf, each a strided subset of its sources, the next source loaded
-// while the current one is accumulated; both charge sets at once, columns of MASK only (FP64: two
-// launches, one per half of the columns; FP32: one launch), combined by cross-lane sums.
+// while the current one is accumulated; both charge sets at once, columns of MASK only (two
+// launches, one per half of the columns), combined by cross-lane sums.
 template <class S4> struct P2mSrc;
 template <> struct P2mSrc<double4> { typedef double2 W2; };
 template <> struct P2mSrc<float4> { typedef float2 W2; };
@@ -840,11 +844,11 @@ __global__ __launch_bounds__(64) void k_fmm_p2m(int nleaf, int Tm, const int32_t
         }
         p2m_accumulate2_cols<P, MASK, F>(rx, ry, rz, qu, qv, au, av);
     }
-    static_assert(G == 8, "the cross-lane sum spans 8 lanes");
+    static_assert(G == 4 || G == 8, "the cross-lane sum spans 4 or 8 lanes");
 #pragma unroll
     for (int k = 0; k < 2 * NH; k++) {
-        au[k] = sum8_dpp(au[k]);
-        av[k] = sum8_dpp(av[k]);
+        au[k] = sum_dpp<G>(au[k]);
+        av[k] = sum_dpp<G>(av[k]);
     }
     if (act && lane == 0) {
         F *du = M + (size_t)cell * 4 * NCo, *dv = du + 2 * NCo;
@@ -2155,7 +2159,7 @@ static void run_fmm(Ctx &c)
     hipStream_t ns = fork ? c.aux : c.stream;
     {
         Prof pr(c, "fmm_p2m");
-        constexpr int G = 8;
+        constexpr int G = 8;  // lanes per leaf (4 measured slower: longer per-lane source loops)
         const unsigned nb = (unsigned)((NL * G + 63) / 64);
         if (NL > 0) {
             if constexpr (std::is_same<F, float>::value) {
