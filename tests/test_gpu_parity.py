@@ -403,6 +403,20 @@ def test_step_entry_point_matches_the_separate_calls(small, demag):
     assert not np.array_equal(a.get_state(nv.FG_NEXT)[0], cur)
     a.close()
     b.close()
+    # a solve that cannot converge in its iterations: FG_SOLVE_FAILED with the reference's
+    # ITER_OVERFLOW status, CURRENT and NEXT u untouched by the gated update and nothing evolved
+    f = engine(small.mesh, tol=1e-30, max_iter=2, demag=demag, fmm_order=8, fmm_leaf=16)
+    f.set_state(nv.FG_BOTH, u=u, v=v)
+    f.calc_demag()
+    f.evolution()
+    before = [x.copy() for x in f.get_state(nv.FG_CURRENT)]
+    nxt_u = f.get_state(nv.FG_NEXT)[0]
+    rc, info, E, ang = f.step(0.5, H, dt, pref)
+    assert rc == nv.FG_SOLVE_FAILED and E is None and info.status == nv.FG_ITER_OVERFLOW, (rc, info.status)
+    for x, y in zip(f.get_state(nv.FG_CURRENT), before):
+        assert np.array_equal(x, y)
+    assert np.array_equal(f.get_state(nv.FG_NEXT)[0], nxt_u)
+    f.close()
 
 
 def test_ellipsoid_fixture_step(golden_dir):
