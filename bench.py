@@ -506,7 +506,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this engine runs on one GPU; --gpus {args.gpus} is not supported "
-              "(the multi-GPU sharding is a design in DESIGN.md, not an implementation)", file=sys.stderr)
+              "(sharding the path across GPUs is designed, not implemented)", file=sys.stderr)
         sys.exit(2)
     if args.dots:
         bench_dots(args)
