@@ -260,21 +260,46 @@ __global__ __launch_bounds__(256) void k_guess_input(int N, const float *__restr
 }
 
 // Inference on the matrix cores. Each wavefront takes 64 nodes as four 16-node column tiles and
-// computes the layers transposed, Z^T = W X^T, with v_mfma_f32_16x16x4f32 (A: 16x4 weights, lane
-// l holding row l%16, k l/16; B: 4x16 activations, lane l holding k l/16, column l%16; D: lane l
-// holding rows 4(l/16)+j, j < 4, column l%16). Hidden layer k-steps are taken as the features
-// {16R + 4g + j : g < 4} for fixed (R, j), which are exactly the D registers (R, j) of the lanes
-// of group g = l/16: each layer's output feeds the next as its B operand without moving data.
+// computes the layers transposed, Z^T = W X^T, with v_mfma_f32_16x16x16_bf16 (A: 16x16 weights,
+// lane l holding row l%16, k 4(l/16)..4(l/16)+3; B: 16x16 activations, lane l holding those k of
+// column l%16; D: lane l holding rows 4(l/16)+j, j < 4, column l%16, FP32). A hidden layer's 16-row
+// tile R of D is therefore, lane for lane, the B operand of the next layer's k-block R: each layer
+// feeds the next without moving data. The weights are BF16 already; each FP32 activation x is
+// split into BF16 parts hi + lo (x - hi rounded again), and both parts are multiplied, so the
+// products carry x to about 16 bits (a relative 2^-17) while running at the BF16 matrix rate.
 // The weight fragments are staged in LDS in lane order (one conflict-free read per fragment).
 // The three outputs (rows 0..2, lanes 0..15) go through LDS to one node per lane for v_pred.
 typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef short bf16x4 __attribute__((ext_vector_type(4)));
+constexpr int KB1 = (NX + 15) / 16;  // 16-input k-blocks of the first layer
 struct MfmaNet {
-    float4 W2f[16 * 64];    // [(R2 * 4 + R) * 64 + l], j = 0..3
-    float W1f[4 * KS1 * 64];  // [(R * KS1 + s) * 64 + l]
-    float4 W3f[4 * 64];     // [R * 64 + l], j = 0..3
+    bf16x4 W2b[16 * 64];      // [(R2 * 4 + R) * 64 + l]: W2[16 R2 + l%16][16 R + 4(l/16) + j]
+    bf16x4 W1b[4 * KB1 * 64];  // [(R * KB1 + q) * 64 + l]: W1[16 R + l%16][16 q + 4(l/16) + j]
+    bf16x4 W3b[4 * 64];       // [R * 64 + l]: W3[l%16][16 R + 4(l/16) + j], rows past NO zero
     float b1[NH], b2[NH], b3[4], s[NSC];
     float o[256 * NO];
 };
+__device__ inline short bf16_bits(float x)
+{
+    uint32_t b;
+    __builtin_memcpy(&b, &x, 4);
+    return (short)(b >> 16);
+}
+// hi and lo BF16 parts of four activations
+__device__ inline void split_bf16(const f32x4 &x, bf16x4 &hi, bf16x4 &lo)
+{
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        const float h = bf16_round(x[j]);
+        hi[j] = bf16_bits(h);
+        lo[j] = bf16_bits(bf16_round(x[j] - h));
+    }
+}
+__device__ inline f32x4 mma2(const bf16x4 &a, const bf16x4 &bh, const bf16x4 &bl, f32x4 acc)
+{
+    acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bh, acc, 0, 0, 0);
+    return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bl, acc, 0, 0, 0);
+}
 
 __global__ __launch_bounds__(256) void k_guess_infer(int N, const float *__restrict__ xq,
                                                      const float *__restrict__ feat, const int32_t *__restrict__ mag,
@@ -288,18 +313,22 @@ __global__ __launch_bounds__(256) void k_guess_infer(int N, const float *__restr
         const int f = e >> 6, q = e & 63, R2 = f >> 2, R = f & 3;
         const int row = 16 * R2 + (q & 15), col = 16 * R + 4 * (q >> 4);
         const uint16_t *w = wbf + OFF_W2 + row * NH + col;
-        S.W2f[e] = make_float4(bf16_to_float(w[0]), bf16_to_float(w[1]), bf16_to_float(w[2]), bf16_to_float(w[3]));
+        S.W2b[e] = bf16x4{(short)w[0], (short)w[1], (short)w[2], (short)w[3]};
     }
-    for (int e = tid; e < 4 * KS1 * 64; e += 256) {
-        const int f = e >> 6, q = e & 63, R = f / KS1, s = f - KS1 * R, k = 4 * s + (q >> 4);
-        S.W1f[e] = k < NX ? bf16_to_float(wbf[OFF_W1 + (16 * R + (q & 15)) * NX + k]) : 0.f;
+    for (int e = tid; e < 4 * KB1 * 64; e += 256) {
+        const int f = e >> 6, q = e & 63, R = f / KB1, kb = f - KB1 * R, k0 = 16 * kb + 4 * (q >> 4);
+        bf16x4 v = {0, 0, 0, 0};
+#pragma unroll
+        for (int j = 0; j < 4; j++)
+            if (k0 + j < NX) v[j] = (short)wbf[OFF_W1 + (16 * R + (q & 15)) * NX + k0 + j];
+        S.W1b[e] = v;
     }
     for (int e = tid; e < 4 * 64; e += 256) {
         const int R = e >> 6, q = e & 63, o = q & 15, col = 16 * R + 4 * (q >> 4);
-        float v[4] = {0.f, 0.f, 0.f, 0.f};
+        bf16x4 v = {0, 0, 0, 0};
         if (o < NO)
-            for (int j = 0; j < 4; j++) v[j] = bf16_to_float(wbf[OFF_W3 + o * NH + col + j]);
-        S.W3f[e] = make_float4(v[0], v[1], v[2], v[3]);
+            for (int j = 0; j < 4; j++) v[j] = (short)wbf[OFF_W3 + o * NH + col + j];
+        S.W3b[e] = v;
     }
     if (tid < NH) {
         S.b1[tid] = p[OFF_B1 + tid];
@@ -310,13 +339,22 @@ __global__ __launch_bounds__(256) void k_guess_infer(int N, const float *__restr
     __syncthreads();
     for (int base = blockIdx.x * 256; base < N; base += gridDim.x * 256) {
         const int wbase = base + 64 * wv;
-        // B operands of layer 1: inputs 4s + g4 of node wbase + 16T + ln
-        float xb[4][KS1];
+        // B operands of layer 1: inputs 16 q + 4 g4 + j of node wbase + 16 T + ln, split
+        bf16x4 xh[4][KB1], xl[4][KB1];
 #pragma unroll
         for (int T = 0; T < 4; T++) {
             const int node = wbase + 16 * T + ln;
 #pragma unroll
-            for (int s = 0; s < KS1; s++) xb[T][s] = node < N ? xq[(size_t)node * NXP + 4 * s + g4] : 0.f;
+            for (int q = 0; q < KB1; q++) {
+                const int k0 = 16 * q + 4 * g4;
+                f32x4 v = {0.f, 0.f, 0.f, 0.f};
+                static_assert(NXP % 4 == 0 && NX % 4 == 0, "whole float4 groups of inputs");
+                if (node < N && k0 < NX) {
+                    const float4 t = *reinterpret_cast<const float4 *>(xq + (size_t)node * NXP + k0);
+                    v = f32x4{t.x, t.y, t.z, t.w};
+                }
+                split_bf16(v, xh[T][q], xl[T][q]);
+            }
         }
         f32x4 h1[4][4];
 #pragma unroll
@@ -327,16 +365,21 @@ __global__ __launch_bounds__(256) void k_guess_infer(int N, const float *__restr
 #pragma unroll
             for (int T = 0; T < 4; T++) h1[R][T] = a0;
 #pragma unroll
-            for (int s = 0; s < KS1; s++) {
-                const float a = S.W1f[(R * KS1 + s) * 64 + l];
+            for (int q = 0; q < KB1; q++) {
+                const bf16x4 a = S.W1b[(R * KB1 + q) * 64 + l];
 #pragma unroll
-                for (int T = 0; T < 4; T++) h1[R][T] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, xb[T][s], h1[R][T], 0, 0, 0);
+                for (int T = 0; T < 4; T++) h1[R][T] = mma2(a, xh[T][q], xl[T][q], h1[R][T]);
             }
 #pragma unroll
             for (int T = 0; T < 4; T++)
 #pragma unroll
                 for (int j = 0; j < 4; j++) h1[R][T][j] = gelu(h1[R][T][j]);
         }
+        bf16x4 hh[4][4], hl[4][4];
+#pragma unroll
+        for (int R = 0; R < 4; R++)
+#pragma unroll
+            for (int T = 0; T < 4; T++) split_bf16(h1[R][T], hh[R][T], hl[R][T]);
         f32x4 h2[4][4];
 #pragma unroll
         for (int R2 = 0; R2 < 4; R2++) {
@@ -347,19 +390,19 @@ __global__ __launch_bounds__(256) void k_guess_infer(int N, const float *__restr
             for (int T = 0; T < 4; T++) h2[R2][T] = a0;
 #pragma unroll
             for (int R = 0; R < 4; R++) {
-                const float4 a = S.W2f[(R2 * 4 + R) * 64 + l];
-                const float av[4] = {a.x, a.y, a.z, a.w};
+                const bf16x4 a = S.W2b[(R2 * 4 + R) * 64 + l];
 #pragma unroll
-                for (int j = 0; j < 4; j++)
-#pragma unroll
-                    for (int T = 0; T < 4; T++)
-                        h2[R2][T] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[j], h1[R][T][j], h2[R2][T], 0, 0, 0);
+                for (int T = 0; T < 4; T++) h2[R2][T] = mma2(a, hh[R][T], hl[R][T], h2[R2][T]);
             }
 #pragma unroll
             for (int T = 0; T < 4; T++)
 #pragma unroll
                 for (int j = 0; j < 4; j++) h2[R2][T][j] = gelu(h2[R2][T][j]);
         }
+#pragma unroll
+        for (int R = 0; R < 4; R++)
+#pragma unroll
+            for (int T = 0; T < 4; T++) split_bf16(h2[R][T], hh[R][T], hl[R][T]);
         f32x4 o3[4];
 #pragma unroll
         for (int T = 0; T < 4; T++)
@@ -367,12 +410,9 @@ __global__ __launch_bounds__(256) void k_guess_infer(int N, const float *__restr
             for (int j = 0; j < 4; j++) o3[T][j] = 4 * g4 + j < NO ? S.b3[4 * g4 + j] : 0.f;
 #pragma unroll
         for (int R = 0; R < 4; R++) {
-            const float4 a = S.W3f[R * 64 + l];
-            const float av[4] = {a.x, a.y, a.z, a.w};
+            const bf16x4 a = S.W3b[R * 64 + l];
 #pragma unroll
-            for (int j = 0; j < 4; j++)
-#pragma unroll
-                for (int T = 0; T < 4; T++) o3[T] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[j], h2[R][T][j], o3[T], 0, 0, 0);
+            for (int T = 0; T < 4; T++) o3[T] = mma2(a, hh[R][T], hl[R][T], o3[T]);
         }
         if (g4 == 0)
 #pragma unroll
