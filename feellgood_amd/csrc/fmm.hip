@@ -1063,13 +1063,12 @@ __global__ __launch_bounds__(256) void k_fmm_xlate(int n, const int32_t *__restr
 
 // Row tiles R.. of the M2M below: acc[R] += T_o[R rows] child_o for the wavefront's two octants,
 // over the k-steps of the input degrees that row tile's output degrees take.
-template <int P, int R, class F, int NK>
+template <int P, int R, class F, bool PAIR, int NK>
 __device__ __forceinline__ void m2m_rows(const F *__restrict__ A, const F (&bv)[2][NK],
                                          typename Mfma<F>::V (&acc)[XL<P>::NT])
 {
     if constexpr (R < XL<P>::NT) {
         using L = XL<P>;
-        constexpr bool PAIR = NK == L::NKS2 && NK != L::NKS;
         constexpr int NKS = NK, ke = PAIR ? L::ke2(R) : L::ke(0, R);
         F av[2][ke];
 #pragma unroll
@@ -1080,7 +1079,7 @@ __device__ __forceinline__ void m2m_rows(const F *__restrict__ A, const F (&bv)[
         for (int oo = 0; oo < 2; oo++)
 #pragma unroll
             for (int ks = 0; ks < ke; ks++) acc[R] = Mfma<F>::mma(av[oo][ks], bv[oo][ks], acc[R]);
-        m2m_rows<P, R + 1, F, NK>(A, bv, acc);
+        m2m_rows<P, R + 1, F, PAIR, NK>(A, bv, acc);
     }
 }
 
@@ -1136,7 +1135,7 @@ __global__ __launch_bounds__(256) void k_fmm_m2m_all(int n, const int32_t *__res
     V acc[NT];
 #pragma unroll
     for (int R = 0; R < NT; R++) acc[R] = V{F(0), F(0), F(0), F(0)};
-    m2m_rows<P, 0, F, NK>(Top + (size_t)2 * w * NT * NK * 64 + l, bv, acc);
+    m2m_rows<P, 0, F, PAIR, NK>(Top + (size_t)2 * w * NT * NK * 64 + l, bv, acc);
 #pragma unroll
     for (int R = 0; R < NT; R++) red[R][w][l] = acc[R];
     __syncthreads();

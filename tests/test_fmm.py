@@ -163,11 +163,14 @@ def test_device_fmm_matches_host_fmm_and_direct():
 
 
 @pytest.mark.gpu
-def test_device_fmm_with_many_parents_per_level():
+@pytest.mark.parametrize("order", [6, 8])
+def test_device_fmm_with_many_parents_per_level(order):
     """A 48x24x3-cell film with leaves of at most 8 sources on average: 57,350 leaves under 8,192
     level-6 parents, so the upward pass takes the all-row-tile M2M (levels of 2,048 parents or
-    more). FP64 device FMM against the host FMM within 1e-10 of the largest potential; the
-    mixed-precision pass (FP32 translations) within 1e-5 (phi) and 3e-3 (phi_v) of the FP64 one."""
+    more; in FP32 with its input reals in pair order). FP64 device FMM against the host FMM within
+    1e-10 of the largest potential; the mixed-precision pass (FP32 translations, lower far and v
+    degrees) within 1e-5 (phi) and 3e-3 (phi_v) of the FP64 one at order 8, 1e-4 and 1e-2 at
+    order 6 (far entries to degree 4; measured 1.7e-5 and 1.9e-3)."""
     from feellgood_amd.engine import Engine
     m = film(48, 24, 3)
     p = oracle.Problem(m.mesh)
@@ -183,20 +186,21 @@ def test_device_fmm_with_many_parents_per_level():
     tgt = m.arrays()["pos"]
     pots = {}
     for prec in ("fp64", "mixed"):
-        g = Engine(m.mesh, demag="fmm", fmm_order=8, fmm_leaf=8, precision=prec)
+        g = Engine(m.mesh, demag="fmm", fmm_order=order, fmm_leaf=8, precision=prec)
         info = g.fmm_info()
         assert info["leaf_level"] == 7 and info["m2m_cells"] >= 8192, info
         g.set_state(nv.FG_BOTH, u=u, v=v)
         g.calc_demag()
         pots[prec] = g.get_state(nv.FG_NEXT)[2:]
         g.close()
-    pu, pv, L = fmm_ref(xyz, qu, qv, tgt, 8, 8, far=8)
+    pu, pv, L = fmm_ref(xyz, qu, qv, tgt, order, 8, far=order)
     assert L == 7
     for k, ref in enumerate(((pu + cu) / (4 * np.pi), (pv + cv) / (4 * np.pi))):
         assert np.abs(pots["fp64"][k] - ref).max() <= 1e-10 * np.abs(ref).max(), k
     err = [np.abs(pots["mixed"][k] - pots["fp64"][k]).max() / np.abs(pots["fp64"][k]).max() for k in range(2)]
-    print(f"mixed against FP64 FMM: phi {err[0]:.2e}, phi_v {err[1]:.2e} of the largest", flush=True)
-    assert err[0] <= 1e-5 and err[1] <= 3e-3, err
+    print(f"order {order}: mixed against FP64 FMM: phi {err[0]:.2e}, phi_v {err[1]:.2e} of the largest", flush=True)
+    bound = {8: (1e-5, 3e-3), 6: (1e-4, 1e-2)}[order]
+    assert err[0] <= bound[0] and err[1] <= bound[1], err
 
 
 @pytest.mark.gpu
